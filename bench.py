@@ -1,0 +1,250 @@
+"""Benchmark: D3Q19 TRT forward+adjoint design gradient on one MI355X.
+
+Workload (one "step" = one lattice time step of the design-gradient horizon): the
+256^3 D3Q19 TRT Brinkman channel -- velocity inlet u_x = 0.03 with a parabolic duct
+profile on x = 0, zero-gradient pressure outlet rho = 1 on x = nx-1, halfway
+bounce-back walls on y and z, tau+ = 0.55, magic 3/16, a design of 64 random
+Gaussian blobs (radius U(6,20)) smoothed and sigmoided, alpha = 0.1*0.1*(1-phi)/
+(0.1+phi); objective = pressure drop between planes x = 1 and x = nx-2 averaged over
+the last 500 states.  The gradient over K steps runs forward + adjoint with binomial
+checkpointing inside a 200 GB snapshot budget, in FP64 (reference), FP32 and
+FP16-shifted storage.  Synthetic data: the design is generated from a seed.
+
+value = lattice updates of the horizon per second of the whole FP16-shifted gradient
+(cells * K / wall time / 1e6, recomputation and snapshot copies included).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per lattice update (populations read+written once, alpha, and for
+# the adjoint the state read, cotangent read+written and an FP64 alpha_bar update).
+BYTES = {
+    ("forward", "f64"): 19 * 8 * 2 + 8,
+    ("forward", "f32"): 19 * 4 * 2 + 4,
+    ("forward", "f16s"): 19 * 2 * 2 + 4,
+    ("adjoint", "f64"): 19 * 8 + 19 * 8 * 2 + 8 + 16,
+    ("adjoint", "f32"): 19 * 4 + 19 * 4 * 2 + 4 + 16,
+    ("adjoint", "f16s"): 19 * 2 + 19 * 4 * 2 + 4 + 16,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(shape, precision, budget, seed=3):
+    from implexity_amd import design
+
+    cfg = _cfg_only(shape, precision, budget)
+    phi = design.blob_design(shape, 64, seed, radius=(6.0, 20.0))
+    alpha = design.resistance(phi, 0.1, 0.1).reshape(-1)
+    return cfg, alpha
+
+
+def objective(shape, steps):
+    from implexity_amd.lbm import Objective
+
+    window = min(500, steps)
+    return Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=steps - window + 1,
+                     t_end=steps, weight=1.0 / window)
+
+
+def run_precision(shape, precision, steps, warmup, budget, alpha_cfg, timing_launches=20):
+    from implexity_amd.lbm import Lattice
+
+    cfg, alpha = alpha_cfg(precision)
+    lat = Lattice(cfg)
+    try:
+        lat.set_design(alpha)
+        ob_w = objective(shape, max(warmup, 2))
+        if warmup:
+            lat.initial_populations()
+            lat.gradient(warmup, ob_w)
+        lat.initial_populations()
+        ob = objective(shape, steps)
+        t0 = time.perf_counter()
+        J, ab, _, st = lat.gradient(steps, ob)
+        wall = time.perf_counter() - t0
+        # per-kernel device time (HIP events on the library's stream)
+        lat.initial_populations()
+        t_fwd = lat.time_kernel("forward", timing_launches)
+        lat.initial_populations()
+        t_adj = lat.time_kernel("adjoint", timing_launches)
+        return dict(J=J, alpha_bar=ab, wall=wall, stats=st.as_dict(), t_fwd_ms=t_fwd, t_adj_ms=t_adj,
+                    state_bytes=lat.state_bytes)
+    finally:
+        lat.close()
+
+
+def cpu_baseline(shape, window, alpha_cfg, threads):
+    """The oracle (C restatement of the reference CPU algorithm: AoS FP64, collide then
+    table stream, all states kept) timed on a bounded window of the same workload."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import oracle as orc
+
+    cfg, alpha = alpha_cfg("f64")
+    prob = orc.Problem(shape=shape, periodic=(False, False, False), Q=19, kind="trt", omega=cfg.omega,
+                       magic=cfg.magic, ports=[orc.Port(p.face, p.kind, p.value, p.profile) for p in cfg.ports])
+    o = orc.Oracle(prob)
+    ob = orc.Objective(kind="pressure_drop", window=(1, window), weight=1.0 / window, axis=0, plane_a=1,
+                       plane_b=shape[0] - 2)
+    t0 = time.perf_counter()
+    orc.scan_adjoint(o, alpha, window, ob)
+    dt = time.perf_counter() - t0
+    return o.n * window / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--shape", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--precisions", default="f64,f32,f16s")
+    ap.add_argument("--cpu-window", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="gradient", choices=["gradient", "kernels"])
+    ap.add_argument("--kernel-precision", default="f32")
+    ap.add_argument("--launches", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path does not shard; "
+              "see DESIGN.md for how it would)", file=sys.stderr)
+        sys.exit(2)
+    shape = tuple(args.shape)
+    budget = int(args.budget_gb * 1e9)
+    cache = {}
+
+    def alpha_cfg(precision):
+        if "alpha" not in cache:
+            cache["alpha"] = build_problem(shape, precision, budget)[1]
+        return _cfg_only(shape, precision, budget), cache["alpha"]
+
+    if args.mode == "kernels":
+        # Per-kernel timing only (used under rocprofv3): forward and adjoint launches.
+        from implexity_amd.lbm import Lattice
+
+        cfg, alpha = alpha_cfg(args.kernel_precision)
+        with Lattice(cfg) as lat:
+            lat.set_design(alpha)
+            lat.initial_populations()
+            tf = lat.time_kernel("forward", args.launches)
+            ta = lat.time_kernel("adjoint", args.launches)
+        n = int(np.prod(shape))
+        out = {"precision": args.kernel_precision, "forward_ms": tf, "adjoint_ms": ta,
+               "forward_GBs": BYTES[("forward", args.kernel_precision)] * n / (tf * 1e-3) / 1e9,
+               "adjoint_GBs": BYTES[("adjoint", args.kernel_precision)] * n / (ta * 1e-3) / 1e9}
+        print(json.dumps(out))
+        return
+
+    n = int(np.prod(shape))
+    res = {}
+    for prec in args.precisions.split(","):
+        log(f"[bench] {prec}: gradient over {args.steps} steps on {shape} (warmup {args.warmup})")
+        res[prec] = run_precision(shape, prec, args.steps, args.warmup, budget, alpha_cfg)
+        r = res[prec]
+        log(f"[bench] {prec}: wall {r['wall']:.2f} s, J {r['J']:.6e}, fwd kernel {r['t_fwd_ms']:.3f} ms, "
+            f"adj kernel {r['t_adj_ms']:.3f} ms, stats {r['stats']}")
+
+    from implexity_amd.lbm import copy_bandwidth
+
+    copy_gbs, _ = copy_bandwidth(1 << 31, 20)
+    main_prec = "f16s" if "f16s" in res else list(res)[-1]
+    r = res[main_prec]
+    value = n * args.steps / r["wall"] / 1e6
+
+    def kern(prec, which):
+        t = res[prec]["t_fwd_ms" if which == "forward" else "t_adj_ms"]
+        gbs = BYTES[(which, prec)] * n / (t * 1e-3) / 1e9
+        return t, gbs
+
+    # dominant kernel of the measured run: forward (incl. recomputation) or adjoint
+    st = r["stats"]
+    fwd_total = st["forward_steps"] * r["t_fwd_ms"]
+    adj_total = st["reverse_steps"] * r["t_adj_ms"]
+    dom = "forward" if fwd_total >= adj_total else "adjoint"
+    t_dom, gbs_dom = kern(main_prec, dom)
+    out = {
+        "metric": "D3Q19 fwd+adjoint MLUPS, % HBM roofline; FP16-store vs FP32 at grad err<=1e-3",
+        "value": value,
+        "unit": "MLUPS (fwd+adj)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16-shifted storage, f32 arithmetic, f64 sensitivities" if main_prec == "f16s" else main_prec,
+        "data": "synthetic (seeded 64-blob Brinkman design)",
+        "config": {"workload": f"D3Q19 TRT channel {shape[0]}x{shape[1]}x{shape[2]}, tau+=0.55, u_in=0.03, "
+                               f"forward+adjoint over {args.steps} steps, binomial checkpointing in "
+                               f"{args.budget_gb:g} GB", "precision": main_prec},
+        "roofline": {"bound": "hbm", "achieved": gbs_dom, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs_dom / HBM_PEAK_GBS, "traffic": None, "kernel": f"{dom} ({main_prec})",
+                     "algorithmic_bytes_per_lu": BYTES[(dom, main_prec)], "avg_kernel_ms": t_dom},
+    }
+    per = {}
+    for prec, rr in res.items():
+        tf, gf = kern(prec, "forward")
+        ta, ga = kern(prec, "adjoint")
+        per[prec] = {"gradient_wall_s": rr["wall"], "mlups_fwd_adj": n * args.steps / rr["wall"] / 1e6,
+                     "J": rr["J"], "forward_kernel_ms": tf, "forward_GBs": gf, "forward_frac": gf / HBM_PEAK_GBS,
+                     "forward_mlups": n / (tf * 1e-3) / 1e6, "adjoint_kernel_ms": ta, "adjoint_GBs": ga,
+                     "adjoint_frac": ga / HBM_PEAK_GBS, "adjoint_mlups": n / (ta * 1e-3) / 1e6,
+                     "forward_steps": rr["stats"]["forward_steps"], "snapshot_slots": rr["stats"]["snapshot_slots"],
+                     "max_mach": rr["stats"]["max_mach"]}
+    out["per_precision"] = per
+    if "f32" in res and "f16s" in res:
+        out["speedup_f16s_vs_f32"] = res["f32"]["wall"] / res["f16s"]["wall"]
+    if "f64" in res:
+        g64 = res["f64"]["alpha_bar"]
+        errs = {}
+        for prec in res:
+            if prec == "f64":
+                continue
+            g = res[prec]["alpha_bar"]
+            errs[prec] = {
+                "grad_rel_l2": float(np.linalg.norm(g - g64) / np.linalg.norm(g64)),
+                "grad_cosine": float(g @ g64 / (np.linalg.norm(g) * np.linalg.norm(g64))),
+                "J_rel": abs(res[prec]["J"] - res["f64"]["J"]) / abs(res["f64"]["J"]),
+            }
+        out["error_vs_f64"] = errs
+    out["copy_bandwidth_GBs"] = copy_gbs
+    out["roofline"]["frac_of_copy"] = gbs_dom / copy_gbs
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"[bench] cpu baseline: oracle, {args.cpu_window}-step forward+adjoint window, {threads} threads")
+        mlups, dt = cpu_baseline(shape, args.cpu_window, alpha_cfg, threads)
+        out["cpu_baseline"] = {"value": mlups, "unit": "MLUPS (fwd+adj)", "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_window}-step forward+adjoint window (all states kept) of the "
+                                         f"same {shape[0]}x{shape[1]}x{shape[2]} problem, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+def _cfg_only(shape, precision, budget):
+    from implexity_amd import design
+    from implexity_amd.lbm import LatticeConfig, Port
+
+    nx, ny, nz = shape
+    prof = design.parabolic_profile(ny, nz).reshape(-1)
+    ports = [Port("xmin", "velocity", (0.03, 0.0, 0.0), prof), Port("xmax", "pressure", (1.0,))]
+    return LatticeConfig(shape=shape, periodic=(False, False, False), lattice=19, collision="trt",
+                         precision=precision, omega=1.0 / 0.55, magic=3.0 / 16.0, ports=ports,
+                         checkpoint_budget_bytes=budget)
+
+
+if __name__ == "__main__":
+    main()

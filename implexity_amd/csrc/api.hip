@@ -1,0 +1,1126 @@
+// C ABI of the LBM library: handle lifecycle, device memory, ports, MRT basis, the
+// forward sweep and the revolve-driven discrete adjoint (include/implexity_lbm.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/implexity_lbm.h"
+#include "launch.h"
+#include "revolve.h"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Fail{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(LBM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return LBM_OK;
+    } catch (const Fail& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return LBM_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return LBM_ERR_INVALID;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return LBM_ERR_INVALID;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n, const char* what) {
+        if (p && bytes == n) return;
+        reset();
+        if (n == 0) return;
+        const hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(LBM_ERR_MEMORY, std::string("device allocation of ") + std::to_string(n) + " bytes for " + what +
+                                     " failed: " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+const Runner* runner_for(int prec) {
+    switch (prec) {
+    case LBM_F64: return runner_f64();
+    case LBM_F32: return runner_f32();
+    case LBM_F16_SHIFTED: return runner_f16s();
+    default: return runner_bf16s();
+    }
+}
+
+size_t elem_bytes(int prec) { return prec == LBM_F64 ? 8 : (prec == LBM_F32 ? 4 : 2); }
+
+// Lattice tables on the host (same order as lattice.h).
+int lat_c(int Q, int q, int d) { return Q == 19 ? cvec<19>(q, d) : cvec<27>(q, d); }
+double lat_w(int Q, int q) { return Q == 19 ? weight<19>(q) : weight<27>(q); }
+int lat_opp(int Q, int q) { return q == 0 ? 0 : Q - q; }
+
+// Weighted Gram-Schmidt moment basis (moving/collision.rs:87-153): monomials up to
+// degree 6 in the order mass, momentum, bulk, shear, then degrees 3..6; group codes
+// 0 mass, 1 momentum, 2 bulk, 3 shear, 4 odd, 5 even-high.
+void mrt_basis(int Q, std::vector<double>& rows, std::vector<double>& norms, std::vector<int>& groups) {
+    auto ipow = [](double v, int e) {
+        double r = 1.0;
+        for (int k = 0; k < e; k++) r *= v;
+        return r;
+    };
+    std::vector<std::pair<std::vector<double>, int>> cand;
+    auto poly = [&](auto fn, int group) {
+        std::vector<double> p(Q);
+        for (int i = 0; i < Q; i++) {
+            const double v[3] = {double(lat_c(Q, i, 0)), double(lat_c(Q, i, 1)), double(lat_c(Q, i, 2))};
+            p[i] = fn(v);
+        }
+        cand.emplace_back(std::move(p), group);
+    };
+    poly([](const double*) { return 1.0; }, 0);
+    poly([](const double* v) { return v[0]; }, 1);
+    poly([](const double* v) { return v[1]; }, 1);
+    poly([](const double* v) { return v[2]; }, 1);
+    poly([](const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }, 2);
+    poly([](const double* v) { return 2.0 * v[0] * v[0] - v[1] * v[1] - v[2] * v[2]; }, 3);
+    poly([](const double* v) { return v[1] * v[1] - v[2] * v[2]; }, 3);
+    poly([](const double* v) { return v[0] * v[1]; }, 3);
+    poly([](const double* v) { return v[0] * v[2]; }, 3);
+    poly([](const double* v) { return v[1] * v[2]; }, 3);
+    for (int degree = 3; degree <= 6; degree++)
+        for (int a = 0; a <= 2; a++)
+            for (int b = 0; b <= 2; b++) {
+                const int c = degree - a - b;
+                if (c < 0 || c > 2) continue;
+                poly([&](const double* v) { return ipow(v[0], a) * ipow(v[1], b) * ipow(v[2], c); },
+                     degree % 2 == 1 ? 4 : 5);
+            }
+    auto inner = [&](const std::vector<double>& x, const std::vector<double>& y) {
+        double s = 0.0;
+        for (int i = 0; i < Q; i++) s += lat_w(Q, i) * x[i] * y[i];
+        return s;
+    };
+    rows.clear();
+    norms.clear();
+    groups.clear();
+    std::vector<std::vector<double>> basis;
+    for (auto& [p, group] : cand) {
+        if (int(basis.size()) == Q) break;
+        const double scale = inner(p, p);
+        if (scale <= 0.0) continue;
+        std::vector<double> v = p;
+        for (size_t k = 0; k < basis.size(); k++) {
+            const double coef = inner(p, basis[k]) / norms[k];
+            for (int i = 0; i < Q; i++) v[i] -= coef * basis[k][i];
+        }
+        const double n = inner(v, v);
+        if (n <= 1e-10 * scale) continue;
+        norms.push_back(n);
+        groups.push_back(group);
+        basis.push_back(v);
+    }
+    if (int(basis.size()) != Q) fail(LBM_ERR_CONTRACT, "MRT moment basis does not span the lattice");
+    for (auto& r : basis) rows.insert(rows.end(), r.begin(), r.end());
+}
+
+double odd_rate(double omega, double magic) {
+    const double tp = 1.0 / omega - 0.5;
+    return 1.0 / (0.5 + magic / tp);
+}
+
+}  // namespace
+
+struct lbm_handle {
+    int Q = 19, kind = K_TRT, prec = LBM_F32, lprec = LBM_F32;
+    int nx = 0, ny = 0, nz = 0;
+    int per[3] = {0, 0, 0};
+    double omega = 1.0, om = 1.0, tau = 1.0, magic = 3.0 / 16.0, bulk = NAN, even = NAN;
+    double a[3] = {0, 0, 0};
+    bool has_mask = false;
+    Geom g{};
+    long long ncell = 0;
+    size_t state_bytes = 0, lam_bytes = 0;
+    const Runner* run = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf F, A, L, alpha, alpha_bar, mask, mrt, pcell, pnb, pkind, pval, pbuf, scal, aos, pool;
+    std::vector<uint8_t> solid;
+    std::vector<int> h_pcell, h_pnb;
+    uint64_t t = 0;
+    uint64_t budget = 0;
+    size_t pool_slots = 0;
+    int np = 0;
+    std::vector<double> h_alpha;
+
+    size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
+    LaunchCfg cfg() const {
+        const int b = std::min(256, ((nz + 63) / 64) * 64);
+        return LaunchCfg{dim3((nz + b - 1) / b, ny, nx), dim3(b), stream};
+    }
+    StepArgs step_args(void* pop, uint64_t step) const {
+        StepArgs s{};
+        s.g = g;
+        s.pop = pop;
+        s.alpha = alpha.p;
+        s.mask = mask.as<uint32_t>();
+        s.omega = omega;
+        s.om = om;
+        s.tau = tau;
+        for (int d = 0; d < 3; d++) s.a[d] = a[d];
+        s.mrt = mrt.p;
+        s.odd = int(step & 1);
+        s.diag = 0;
+        s.obj.kind = OBJ_NONE;
+        s.jacc = scal.as<double>();
+        s.macc = scal.as<double>() + 1;
+        s.dstat = reinterpret_cast<unsigned*>(scal.as<double>() + 4);
+        return s;
+    }
+    PortArgs port_args(uint64_t step) const {
+        PortArgs p{};
+        p.g = g;
+        p.np = np;
+        p.cell = pcell.as<int>();
+        p.nb = pnb.as<int>();
+        p.kind = pkind.as<int>();
+        p.value = pval.as<double>();
+        p.mask = mask.as<uint32_t>();
+        p.odd = int(step & 1);
+        return p;
+    }
+};
+
+namespace {
+
+// Objective of one state index s (1-based) inside a sweep of n steps, from a template
+// holding the plane geometry (built once per call by objective_template).
+ObjArgs stage_obj(const ObjArgs& tmpl, const lbm_objective* o, uint64_t s, uint64_t n, bool with_terminal) {
+    ObjArgs r = tmpl;
+    r.kind = OBJ_NONE;
+    if (!o || o->kind == LBM_OBJ_NONE) return r;
+    double w = 0.0;
+    if (s >= o->t_begin && s <= o->t_end && s >= 1) w += o->weight;
+    if (with_terminal && s == n) w += o->terminal_weight;
+    if (w == 0.0) return r;
+    r.kind = int(o->kind);
+    r.weight = w;
+    return r;
+}
+
+ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
+    ObjArgs r{};
+    r.kind = OBJ_NONE;
+    if (!o || o->kind == LBM_OBJ_NONE) return r;
+    r.axis = int(o->axis);
+    const int len = o->axis == 0 ? h->nx : (o->axis == 1 ? h->ny : h->nz);
+    r.plane_a = ((o->plane_a % len) + len) % len;
+    r.plane_b = ((o->plane_b % len) + len) % len;
+    if (o->kind == LBM_OBJ_PRESSURE_DROP) {
+        long long na = 0, nb = 0;
+        const long long plane = h->ncell / len;
+        if (h->solid.empty()) {
+            na = nb = plane;
+        } else {
+            for (long long x = 0; x < h->ncell; x++) {
+                if (h->solid[x]) continue;
+                const int k = int(x % h->nz), j = int((x / h->nz) % h->ny), i = int(x / ((long long)h->nz * h->ny));
+                const int c = o->axis == 0 ? i : (o->axis == 1 ? j : k);
+                na += c == r.plane_a;
+                nb += c == r.plane_b;
+            }
+        }
+        if (na == 0 || nb == 0) fail(LBM_ERR_CONTRACT, "pressure-drop plane has no fluid cells");
+        r.inv_na = 1.0 / double(na);
+        r.inv_nb = 1.0 / double(nb);
+    }
+    return r;
+}
+
+void validate_objective(const lbm_handle* h, const lbm_objective* o, uint64_t n) {
+    if (!o || o->kind == LBM_OBJ_NONE) return;
+    if (o->kind > LBM_OBJ_PRESSURE_DROP) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->axis > 2) fail(LBM_ERR_CONTRACT, "objective axis must be 0, 1 or 2");
+    if (!std::isfinite(o->weight) || !std::isfinite(o->terminal_weight))
+        fail(LBM_ERR_CONTRACT, "objective weights must be finite");
+    if (o->weight != 0.0 && (o->t_begin < 1 || o->t_end > n || o->t_begin > o->t_end))
+        fail(LBM_ERR_CONTRACT, "objective window must satisfy 1 <= t_begin <= t_end <= nsteps");
+    (void)h;
+}
+
+void ensure_aos(lbm_handle* h) { h->aos.alloc(size_t(h->ncell) * h->Q * sizeof(double), "layout conversion"); }
+
+void ensure_adjoint_buffers(lbm_handle* h) {
+    h->L.alloc(h->lam_bytes, "cotangent state");
+    h->A.alloc(h->state_bytes, "ahead state");
+    h->alpha_bar.alloc(size_t(h->ncell) * sizeof(double), "alpha_bar");
+}
+
+// One forward step of the state in `buf` (state index `step`): collide+stream, then
+// ports.  `obj` is the objective of the state being read.
+void step_once(lbm_handle* h, void* buf, uint64_t step, const ObjArgs& obj, bool diag, double* macc_slot) {
+    StepArgs s = h->step_args(buf, step);
+    s.obj = obj;
+    s.diag = diag ? 1 : 0;
+    if (macc_slot) s.macc = macc_slot;
+    hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg()), "forward step launch");
+    if (h->np) {
+        hip_check(h->run->port_apply(h->Q, h->has_mask, h->port_args(step + 1), buf,
+                                     diag ? h->scal.as<double>() + 2 : nullptr, h->stream),
+                  "port launch");
+    }
+}
+
+void download_state(lbm_handle* h, const void* buf, uint64_t step, double* out, int layout) {
+    ensure_aos(h);
+    hip_check(h->run->get_state(h->Q, h->has_mask, h->g, buf, h->mask.as<uint32_t>(), int(step & 1),
+                                h->aos.as<double>(), h->cfg()),
+              "get_state launch");
+    const size_t n = size_t(h->ncell) * h->Q;
+    if (layout == 0) {
+        hip_check(hipMemcpyAsync(out, h->aos.p, n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy out");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    } else {
+        std::vector<double> tmp(n);
+        hip_check(hipMemcpyAsync(tmp.data(), h->aos.p, n * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy out");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        for (long long x = 0; x < h->ncell; x++)
+            for (int q = 0; q < h->Q; q++) out[size_t(q) * h->ncell + x] = tmp[size_t(x) * h->Q + q];
+    }
+}
+
+void upload_state(lbm_handle* h, void* buf, uint64_t step, const double* in, int layout) {
+    ensure_aos(h);
+    const size_t n = size_t(h->ncell) * h->Q;
+    if (layout == 0) {
+        hip_check(hipMemcpyAsync(h->aos.p, in, n * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy in");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    } else {
+        std::vector<double> tmp(n);
+        for (long long x = 0; x < h->ncell; x++)
+            for (int q = 0; q < h->Q; q++) tmp[size_t(x) * h->Q + q] = in[size_t(q) * h->ncell + x];
+        hip_check(hipMemcpyAsync(h->aos.p, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                  "copy in");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    }
+    hip_check(h->run->set_state(h->Q, h->has_mask, h->g, buf, h->mask.as<uint32_t>(), int(step & 1),
+                                h->aos.as<double>(), h->cfg()),
+              "set_state launch");
+}
+
+void reset_scalars(lbm_handle* h) { hip_check(hipMemsetAsync(h->scal.p, 0, h->scal.bytes, h->stream), "memset"); }
+
+float elapsed(lbm_handle* h) {
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+int lbm_last_error(char* buf, uint64_t len) {
+    if (!buf || len == 0) return LBM_ERR_CONTRACT;
+    const size_t n = std::min<size_t>(len - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), n);
+    buf[n] = 0;
+    return LBM_OK;
+}
+
+int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_len, lbm_handle** out) {
+    return guarded([&] {
+        if (!cfg || !out) fail(LBM_ERR_CONTRACT, "null config or output handle");
+        *out = nullptr;
+        if (cfg->struct_size != sizeof(lbm_config)) fail(LBM_ERR_CONTRACT, "lbm_config struct_size mismatch");
+        auto h = std::make_unique<lbm_handle>();
+        if (cfg->lattice != 19 && cfg->lattice != 27) fail(LBM_ERR_CONTRACT, "lattice must be D3Q19 or D3Q27");
+        h->Q = int(cfg->lattice);
+        if (cfg->collision > LBM_BGK_GUO) fail(LBM_ERR_CONTRACT, "unknown collision operator");
+        h->kind = int(cfg->collision);
+        if (h->kind == K_CUMULANT && h->Q != 27)
+            fail(LBM_ERR_CONTRACT, "cumulant collision requires the D3Q27 velocity set (got D3Q19)");
+        if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
+        if (cfg->precision > LBM_BF16_SHIFTED) fail(LBM_ERR_CONTRACT, "unknown precision");
+        h->prec = int(cfg->precision);
+        if (h->prec == LBM_BF16_SHIFTED && h->kind != K_TRT && h->kind != K_BGK)
+            fail(LBM_ERR_CONTRACT, "BF16 storage supports BGK and TRT only");
+        h->lprec = h->prec == LBM_F64 ? LBM_F64 : LBM_F32;
+        for (int d = 0; d < 3; d++) {
+            if (cfg->shape[d] < 2) fail(LBM_ERR_CONTRACT, "LBM shape must contain three integers >= 2");
+            if (cfg->shape[d] > 65535) fail(LBM_ERR_CONTRACT, "lattice extent above 65535");
+            h->per[d] = cfg->periodic[d] ? 1 : 0;
+            if (!std::isfinite(cfg->accel[d])) fail(LBM_ERR_CONTRACT, "acceleration requires three finite components");
+            h->a[d] = cfg->accel[d];
+        }
+        h->nx = cfg->shape[0];
+        h->ny = cfg->shape[1];
+        h->nz = cfg->shape[2];
+        h->ncell = (long long)h->nx * h->ny * h->nz;
+        if (!std::isfinite(cfg->omega) || cfg->omega <= 0.0) fail(LBM_ERR_CONTRACT, "omega must be finite and positive");
+        h->omega = cfg->omega;
+        h->tau = 1.0 / cfg->omega;
+        if (!(h->tau >= 0.51 && h->tau <= 2.0))
+            fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
+        h->magic = cfg->magic;
+        if ((h->kind == K_TRT || h->kind == K_MRT) && !(std::isfinite(h->magic) && h->magic > 0.0))
+            fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
+        h->om = (h->kind == K_TRT || h->kind == K_MRT) ? odd_rate(h->omega, h->magic) : h->omega;
+        h->bulk = cfg->bulk_rate;
+        h->even = cfg->even_rate;
+        auto rate_ok = [](double r) { return std::isnan(r) || (std::isfinite(r) && r > 0.0 && r < 2.0); };
+        if (!rate_ok(h->bulk) || !rate_ok(h->even)) fail(LBM_ERR_CONTRACT, "relaxation rates must lie in (0, 2)");
+
+        // Solid mask and blocked links.
+        if (solid_mask) {
+            if (mask_len != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "solid mask length must equal the cell count");
+            h->solid.assign(solid_mask, solid_mask + h->ncell);
+            bool any = false, all = true;
+            for (auto s : h->solid) {
+                any |= s != 0;
+                all &= s != 0;
+            }
+            if (all) fail(LBM_ERR_CONTRACT, "Need fluid cells");
+            h->has_mask = any;
+            if (!any) h->solid.clear();
+        }
+
+        // Storage geometry: +1 ghost layer on non-periodic axes, z pitch padded to 32.
+        h->g.nx = h->nx;
+        h->g.ny = h->ny;
+        h->g.nz = h->nz;
+        h->g.px = h->per[0];
+        h->g.py = h->per[1];
+        h->g.pz = h->per[2];
+        const int SX = h->nx + (h->per[0] ? 0 : 1);
+        h->g.SY = h->ny + (h->per[1] ? 0 : 1);
+        h->g.SZ = ((h->nz + (h->per[2] ? 0 : 1) + 31) / 32) * 32;
+        h->g.sxy = (long long)h->g.SY * h->g.SZ;
+        h->g.stride = (long long)SX * h->g.sxy;
+        h->state_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
+        h->lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
+        h->run = runner_for(h->prec);
+
+        // Ports (moving/boundary.rs:318-380).
+        std::vector<int> pkind;
+        std::vector<double> pval;
+        std::vector<int> is_port(h->ncell, 0);
+        for (uint32_t p = 0; p < cfg->n_ports; p++) {
+            const lbm_port& port = cfg->ports[p];
+            if (port.face > LBM_FACE_ZMAX) fail(LBM_ERR_CONTRACT, "unknown port face");
+            if (port.kind > LBM_PORT_PRESSURE) fail(LBM_ERR_CONTRACT, "Port kind must be velocity or pressure");
+            const int axis = int(port.face) / 2;
+            const int inward = (port.face % 2 == 0) ? 1 : -1;
+            const int len = axis == 0 ? h->nx : (axis == 1 ? h->ny : h->nz);
+            if (h->per[axis]) fail(LBM_ERR_CONTRACT, "A periodic face cannot be a port");
+            if (len < 3) fail(LBM_ERR_CONTRACT, "port needs at least three cells along its axis");
+            const int layer = inward > 0 ? 0 : len - 1;
+            std::vector<long long> cells;
+            for (long long x = 0; x < h->ncell; x++) {
+                const int k = int(x % h->nz), j = int((x / h->nz) % h->ny), i = int(x / ((long long)h->nz * h->ny));
+                const int c = axis == 0 ? i : (axis == 1 ? j : k);
+                if (c == layer && (h->solid.empty() || !h->solid[x])) cells.push_back(x);
+            }
+            if (cells.empty()) fail(LBM_ERR_CONTRACT, "port has no cells");
+            if (port.profile && port.profile_len != cells.size())
+                fail(LBM_ERR_SHAPE, "port profile needs one factor per port cell");
+            for (size_t r = 0; r < cells.size(); r++) {
+                const long long x = cells[r];
+                if (is_port[x]) fail(LBM_ERR_CONTRACT, "port cells overlap");
+                const long long step = axis == 0 ? (long long)h->ny * h->nz : (axis == 1 ? h->nz : 1);
+                const long long nb = x + inward * step;
+                if (!h->solid.empty() && h->solid[nb])
+                    fail(LBM_ERR_CONTRACT, "inward neighbour of a port cell is a wall");
+                const double prof = port.profile ? port.profile[r] : 1.0;
+                if (!std::isfinite(prof)) fail(LBM_ERR_CONTRACT, "port profile must be finite");
+                is_port[x] = 1;
+                h->h_pcell.push_back(int(x));
+                h->h_pnb.push_back(int(nb));
+                pkind.push_back(int(port.kind));
+                if (port.kind == LBM_PORT_VELOCITY) {
+                    for (int d = 0; d < 3; d++) pval.push_back(port.value[d] * prof);
+                } else {
+                    if (!(port.value[0] > 0.0)) fail(LBM_ERR_CONTRACT, "pressure port density must be positive");
+                    pval.push_back(port.value[0]);
+                    pval.push_back(0.0);
+                    pval.push_back(0.0);
+                }
+            }
+        }
+        for (int nb : h->h_pnb)
+            if (is_port[nb]) fail(LBM_ERR_CONTRACT, "the inward neighbour of a port cell is itself a port cell");
+        h->np = int(h->h_pcell.size());
+
+        // Device allocations.
+        hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream create");
+        hip_check(hipEventCreate(&h->ev0), "event create");
+        hip_check(hipEventCreate(&h->ev1), "event create");
+        h->F.alloc(h->state_bytes, "population state");
+        hip_check(hipMemsetAsync(h->F.p, 0, h->F.bytes, h->stream), "memset");
+        h->alpha.alloc(size_t(h->ncell) * h->csize(), "alpha");
+        hip_check(hipMemsetAsync(h->alpha.p, 0, h->alpha.bytes, h->stream), "memset");
+        h->h_alpha.assign(h->ncell, 0.0);
+        h->scal.alloc(64, "scalars");
+        if (h->has_mask) {
+            std::vector<uint32_t> m(h->ncell, 0);
+            for (long long x = 0; x < h->ncell; x++) {
+                if (h->solid[x]) {
+                    m[x] = SOLID_BIT;
+                    continue;
+                }
+                const int k = int(x % h->nz), j = int((x / h->nz) % h->ny), i = int(x / ((long long)h->nz * h->ny));
+                uint32_t bits = 0;
+                for (int q = 1; q < h->Q; q++) {
+                    const int c[3] = {lat_c(h->Q, q, 0), lat_c(h->Q, q, 1), lat_c(h->Q, q, 2)};
+                    int y[3] = {i - c[0], j - c[1], k - c[2]};
+                    const int n[3] = {h->nx, h->ny, h->nz};
+                    bool blocked = false;
+                    for (int d = 0; d < 3; d++) {
+                        if (y[d] < 0 || y[d] >= n[d]) {
+                            if (!h->per[d]) blocked = true;
+                            y[d] = ((y[d] % n[d]) + n[d]) % n[d];
+                        }
+                    }
+                    const long long yx = ((long long)y[0] * h->ny + y[1]) * h->nz + y[2];
+                    if (blocked || h->solid[yx]) bits |= 1u << q;
+                }
+                m[x] = bits;
+            }
+            h->mask.alloc(m.size() * sizeof(uint32_t), "link mask");
+            hip_check(hipMemcpy(h->mask.p, m.data(), h->mask.bytes, hipMemcpyHostToDevice), "mask upload");
+        }
+        // Collision constants.
+        if (h->kind == K_MRT || h->kind == K_CUMULANT) {
+            std::vector<double> c;
+            if (h->kind == K_MRT) {
+                std::vector<double> rows, norms;
+                std::vector<int> groups;
+                mrt_basis(h->Q, rows, norms, groups);
+                c = rows;
+                c.insert(c.end(), norms.begin(), norms.end());
+                for (int k = 0; k < h->Q; k++) {
+                    double r = h->omega;
+                    switch (groups[k]) {
+                    case 0:
+                    case 3: r = h->omega; break;
+                    case 1:
+                    case 4: r = h->om; break;
+                    case 2: r = std::isnan(h->bulk) ? h->omega : h->bulk; break;
+                    default: r = std::isnan(h->even) ? h->omega : h->even; break;
+                    }
+                    c.push_back(r);
+                }
+            } else {
+                c.push_back(std::isnan(h->bulk) ? 1.0 : h->bulk);
+            }
+            if (h->prec == LBM_F64) {
+                h->mrt.alloc(c.size() * sizeof(double), "collision constants");
+                hip_check(hipMemcpy(h->mrt.p, c.data(), h->mrt.bytes, hipMemcpyHostToDevice), "upload");
+            } else {
+                std::vector<float> cf(c.begin(), c.end());
+                h->mrt.alloc(cf.size() * sizeof(float), "collision constants");
+                hip_check(hipMemcpy(h->mrt.p, cf.data(), h->mrt.bytes, hipMemcpyHostToDevice), "upload");
+            }
+        }
+        if (h->np) {
+            h->pcell.alloc(h->np * sizeof(int), "ports");
+            h->pnb.alloc(h->np * sizeof(int), "ports");
+            h->pkind.alloc(h->np * sizeof(int), "ports");
+            h->pval.alloc(h->np * 3 * sizeof(double), "ports");
+            h->pbuf.alloc(size_t(h->np) * h->Q * h->csize(), "port neighbour state");
+            hip_check(hipMemcpy(h->pcell.p, h->h_pcell.data(), h->pcell.bytes, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(h->pnb.p, h->h_pnb.data(), h->pnb.bytes, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(h->pkind.p, pkind.data(), h->pkind.bytes, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(h->pval.p, pval.data(), h->pval.bytes, hipMemcpyHostToDevice), "upload");
+        }
+        h->budget = cfg->checkpoint_budget_bytes;
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        *out = h.release();
+        lbm_reset_initial(*out);
+    });
+}
+
+int lbm_destroy(lbm_handle* h) {
+    return guarded([&] {
+        if (!h) return;
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        if (h->ev0) (void)hipEventDestroy(h->ev0);
+        if (h->ev1) (void)hipEventDestroy(h->ev1);
+        hipStream_t s = h->stream;
+        delete h;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int lbm_set_design(lbm_handle* h, const double* alpha, uint64_t n) {
+    return guarded([&] {
+        if (!h || !alpha) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "design needs one coefficient per cell");
+        for (uint64_t x = 0; x < n; x++)
+            if (!std::isfinite(alpha[x]) || alpha[x] < 0.0)
+                fail(LBM_ERR_CONTRACT, "Brinkman coefficients must be finite and non-negative");
+        h->h_alpha.assign(alpha, alpha + n);
+        if (h->prec == LBM_F64) {
+            hip_check(hipMemcpy(h->alpha.p, alpha, n * sizeof(double), hipMemcpyHostToDevice), "upload");
+        } else {
+            std::vector<float> f(alpha, alpha + n);
+            hip_check(hipMemcpy(h->alpha.p, f.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+        }
+    });
+}
+
+int lbm_reset_initial(lbm_handle* h) {
+    return guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        // solver.rs:592-601: fluid W - 1.5 W (c.a) so the Guo velocity is 0; solids W.
+        const int Q = h->Q;
+        std::vector<double> fluid(Q);
+        for (int q = 0; q < Q; q++) {
+            double ca = 0.0;
+            for (int d = 0; d < 3; d++) ca = ca + h->a[d] * double(lat_c(Q, q, d));
+            fluid[q] = lat_w(Q, q) - 1.5 * lat_w(Q, q) * ca;
+        }
+        std::vector<double> f(size_t(h->ncell) * Q);
+        for (long long x = 0; x < h->ncell; x++)
+            for (int q = 0; q < Q; q++)
+                f[size_t(x) * Q + q] = (!h->solid.empty() && h->solid[x]) ? lat_w(Q, q) : fluid[q];
+        h->t = 0;
+        upload_state(h, h->F.p, 0, f.data(), 0);
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_set_state(lbm_handle* h, int layout, const double* f, uint64_t len) {
+    return guarded([&] {
+        if (!h || !f) fail(LBM_ERR_CONTRACT, "null argument");
+        if (len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "state length must be cells*Q");
+        upload_state(h, h->F.p, h->t, f, layout);
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_get_state(lbm_handle* h, int layout, double* f, uint64_t len) {
+    return guarded([&] {
+        if (!h || !f) fail(LBM_ERR_CONTRACT, "null argument");
+        if (len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "state length must be cells*Q");
+        download_state(h, h->F.p, h->t, f, layout);
+    });
+}
+
+uint64_t lbm_step_count(const lbm_handle* h) { return h ? h->t : 0; }
+
+uint64_t lbm_state_bytes(const lbm_handle* h) { return h ? h->state_bytes : 0; }
+
+int lbm_step(lbm_handle* h, uint64_t nsteps) {
+    return guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        ObjArgs none{};
+        for (uint64_t s = 0; s < nsteps; s++) {
+            step_once(h, h->F.p, h->t, none, false, nullptr);
+            h->t++;
+        }
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, lbm_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        validate_objective(h, obj, nsteps);
+        const ObjArgs tmpl = objective_template(h, obj);
+        const auto t0 = std::chrono::steady_clock::now();
+        reset_scalars(h);
+        DevBuf mass;
+        mass.alloc((nsteps + 2) * sizeof(double), "mass history");
+        hip_check(hipMemsetAsync(mass.p, 0, mass.bytes, h->stream), "memset");
+        hip_check(hipEventRecord(h->ev0, h->stream), "event");
+        for (uint64_t s = 0; s < nsteps; s++) {
+            step_once(h, h->F.p, h->t, stage_obj(tmpl, obj, s, nsteps, false), true, mass.as<double>() + s);
+            h->t++;
+        }
+        hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        // Final state: objective (stage at n + terminal) and its mass.
+        StepArgs ea = h->step_args(h->F.p, h->t);
+        ea.obj = stage_obj(tmpl, obj, nsteps, nsteps, true);
+        ea.macc = mass.as<double>() + nsteps;
+        ObjArgs none{};
+        hip_check(h->run->objective(h->Q, h->has_mask, ea, none, nullptr, nullptr, 0, h->cfg()), "objective launch");
+        double sc[4];
+        unsigned ds[2];
+        hip_check(hipMemcpyAsync(sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipMemcpyAsync(ds, h->scal.as<double>() + 4, sizeof(ds), hipMemcpyDeviceToHost, h->stream), "copy");
+        std::vector<double> mh(nsteps + 1);
+        hip_check(hipMemcpyAsync(mh.data(), mass.p, mh.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        if (J) *J = sc[0];
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->forward_steps = nsteps;
+            stats->forward_ms = elapsed(h);
+            stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            float umax;
+            std::memcpy(&umax, &ds[0], 4);
+            stats->max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
+            stats->bad_cells = ds[1];
+            // relative mass drift; port exchange is accumulated into sc[2] over the sweep
+            double drift = 0.0;
+            for (uint64_t s = 0; s <= nsteps; s++) drift = std::max(drift, std::fabs(mh[s] - mh[0]));
+            stats->mass_drift = h->np ? NAN : drift / mh[0];
+        }
+    });
+}
+
+int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n) {
+    return guarded([&] {
+        if (!h || !rho || !u) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "fields need one entry per cell");
+        DevBuf d;
+        d.alloc(size_t(h->ncell) * 4 * sizeof(double), "fields");
+        double* dr = d.as<double>();
+        double* du = dr + h->ncell;
+        hip_check(h->run->fields(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1), h->alpha.p,
+                                 h->a, dr, du, h->cfg()),
+                  "fields launch");
+        hip_check(hipMemcpyAsync(rho, dr, n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipMemcpyAsync(u, du, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_port_cells(const lbm_handle* h, int32_t* cells, int32_t* nbs, uint64_t cap, uint64_t* count) {
+    return guarded([&] {
+        if (!h || !count) fail(LBM_ERR_CONTRACT, "null argument");
+        *count = uint64_t(h->np);
+        for (int r = 0; r < h->np && uint64_t(r) < cap; r++) {
+            if (cells) cells[r] = h->h_pcell[r];
+            if (nbs) nbs[r] = h->h_pnb[r];
+        }
+    });
+}
+
+int lbm_debug_link_table(lbm_handle* h, uint32_t* source, uint64_t len) {
+    return guarded([&] {
+        if (!h || !source) fail(LBM_ERR_CONTRACT, "null argument");
+        const int Q = h->Q;
+        if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "link table length must be cells*Q");
+        // Run the streaming-only operator once, in double precision, on a state whose
+        // value at (x, q) is its own reference index x*Q + q: afterwards (x, q) holds its
+        // source index (solver.rs:131-136).
+        const Runner* r64 = runner_f64();
+        DevBuf buf;
+        buf.alloc(size_t(h->g.stride) * Q * sizeof(double), "link-table state");
+        hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream), "memset");
+        std::vector<double> ids(size_t(h->ncell) * Q);
+        for (size_t i = 0; i < ids.size(); i++) ids[i] = double(i);
+        ensure_aos(h);
+        hip_check(hipMemcpyAsync(h->aos.p, ids.data(), ids.size() * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                  "copy");
+        hip_check(r64->set_state(Q, h->has_mask, h->g, buf.p, h->mask.as<uint32_t>(), 0, h->aos.as<double>(), h->cfg()),
+                  "set_state");
+        StepArgs s{};
+        s.g = h->g;
+        s.pop = buf.p;
+        s.alpha = nullptr;
+        s.mask = h->mask.as<uint32_t>();
+        s.odd = 0;
+        s.jacc = nullptr;
+        s.macc = nullptr;
+        // Streaming-only reads alpha; give it a zero array of doubles.
+        DevBuf zero;
+        zero.alloc(size_t(h->ncell) * sizeof(double), "zero alpha");
+        hip_check(hipMemsetAsync(zero.p, 0, zero.bytes, h->stream), "memset");
+        s.alpha = zero.p;
+        hip_check(r64->forward(Q, K_STREAM_ONLY, h->has_mask, s, h->cfg()), "stream-only launch");
+        hip_check(r64->get_state(Q, h->has_mask, h->g, buf.p, h->mask.as<uint32_t>(), 1, h->aos.as<double>(), h->cfg()),
+                  "get_state");
+        hip_check(hipMemcpyAsync(ids.data(), h->aos.p, ids.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        for (long long x = 0; x < h->ncell; x++)
+            for (int q = 0; q < Q; q++) {
+                const size_t i = size_t(x) * Q + q;
+                source[i] = (!h->solid.empty() && h->solid[x]) ? 0xFFFFFFFFu : uint32_t(ids[i]);
+            }
+    });
+}
+
+int lbm_revolve_schedule(uint64_t steps, uint64_t snapshots, int64_t* actions, uint64_t cap, uint64_t* count,
+                         uint64_t* forward_steps, uint64_t* reverse_start) {
+    return guarded([&] {
+        Schedule s = revolve_schedule(steps, snapshots);
+        if (count) *count = s.actions.size();
+        if (forward_steps) *forward_steps = s.forward_steps;
+        if (reverse_start) *reverse_start = s.reverse_start;
+        if (actions)
+            for (size_t i = 0; i < s.actions.size() && i < cap; i++) {
+                actions[5 * i + 0] = int64_t(s.actions[i].kind);
+                actions[5 * i + 1] = int64_t(s.actions[i].a);
+                actions[5 * i + 2] = int64_t(s.actions[i].b);
+                actions[5 * i + 3] = 0;
+                actions[5 * i + 4] = 0;
+            }
+    });
+}
+
+uint64_t lbm_revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots) {
+    return revolve_optimal_forward_steps(steps, snapshots);
+}
+
+int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, double* alpha_bar, uint64_t len) {
+    return guarded([&] {
+        if (!h || !lambda_next || !lambda_prev || !alpha_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        const int Q = h->Q;
+        if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "cotangent length must be cells*Q");
+        ensure_adjoint_buffers(h);
+        ensure_aos(h);
+        const uint64_t t = h->t;
+        // state t+1 port neighbours: step a copy of the state
+        if (h->np) {
+            hip_check(hipMemcpyAsync(h->A.p, h->F.p, h->state_bytes, hipMemcpyDeviceToDevice, h->stream), "copy");
+            ObjArgs none{};
+            step_once(h, h->A.p, t, none, false, nullptr);
+            hip_check(h->run->port_save(Q, h->has_mask, h->port_args(t + 1), h->A.p, h->pbuf.p, h->stream),
+                      "port save");
+        }
+        hip_check(hipMemcpyAsync(h->aos.p, lambda_next, len * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                  "copy");
+        hip_check(h->run->set_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int((t + 1) & 1),
+                                  h->aos.as<double>(), h->cfg()),
+                  "set_lam");
+        if (h->np)
+            hip_check(h->run->port_vjp(Q, h->has_mask, h->port_args(t + 1), h->L.p, h->pbuf.p, h->stream), "port vjp");
+        hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+        AdjArgs a{};
+        a.g = h->g;
+        a.pop = h->F.p;
+        a.lam = h->L.p;
+        a.alpha = h->alpha.p;
+        a.alpha_bar = h->alpha_bar.as<double>();
+        a.mask = h->mask.as<uint32_t>();
+        a.omega = h->omega;
+        a.om = h->om;
+        a.tau = h->tau;
+        for (int d = 0; d < 3; d++) a.a[d] = h->a[d];
+        a.mrt = h->mrt.p;
+        a.odd = int(t & 1);
+        a.obj.kind = OBJ_NONE;
+        hip_check(h->run->adjoint(Q, h->kind, h->has_mask, a, h->cfg()), "adjoint launch");
+        hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(t & 1), h->aos.as<double>(),
+                                  h->cfg()),
+                  "get_lam");
+        hip_check(hipMemcpyAsync(lambda_prev, h->aos.p, len * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        hip_check(hipMemcpyAsync(alpha_bar, h->alpha_bar.p, size_t(h->ncell) * sizeof(double), hipMemcpyDeviceToHost,
+                                 h->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
+                 uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        if (nsteps == 0) fail(LBM_ERR_INVALID, "a checkpoint schedule needs at least one step");
+        if (alpha_bar && n_alpha != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "alpha_bar needs one entry per cell");
+        if (lambda0 && lambda_len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "lambda0 length must be cells*Q");
+        validate_objective(h, obj, nsteps);
+        const ObjArgs tmpl = objective_template(h, obj);
+        const auto t0 = std::chrono::steady_clock::now();
+        const int Q = h->Q;
+        ensure_adjoint_buffers(h);
+        const uint64_t base = h->t;  // state 0 of the sweep; parities follow base + s
+        // Snapshot pool from the byte budget.
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "memory info");
+        const uint64_t budget = h->budget ? h->budget : uint64_t(free_b / 2);
+        uint64_t slots = std::min<uint64_t>(budget / h->state_bytes, nsteps);
+        if (slots == 0) fail(LBM_ERR_MEMORY, "checkpoint budget holds no snapshot of the state");
+        if (h->pool_slots < slots) {
+            h->pool.reset();
+            h->pool_slots = 0;
+            h->pool.alloc(size_t(slots) * h->state_bytes, "revolve snapshot pool");
+            h->pool_slots = slots;
+        }
+        Schedule sch = revolve_schedule(nsteps, slots);
+        auto slot_ptr = [&](uint64_t s) { return static_cast<char*>(h->pool.p) + size_t(s) * h->state_bytes; };
+
+        reset_scalars(h);
+        DevBuf mass;
+        mass.alloc((nsteps + 2) * sizeof(double), "mass history");
+        hip_check(hipMemsetAsync(mass.p, 0, mass.bytes, h->stream), "memset");
+        std::vector<hipEvent_t> evs;
+        auto event = [&]() {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "event");
+            hip_check(hipEventRecord(e, h->stream), "event record");
+            evs.push_back(e);
+            return e;
+        };
+        double fwd_ms = 0.0, adj_ms = 0.0;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd_spans, adj_spans;
+        uint64_t fsteps = 0, rsteps = 0, copies = 0;
+        uint64_t cursor = 0;  // state index held in F
+        ObjArgs none{};
+        AdjArgs aa{};
+        aa.g = h->g;
+        aa.pop = h->F.p;
+        aa.lam = h->L.p;
+        aa.alpha = h->alpha.p;
+        aa.alpha_bar = h->alpha_bar.as<double>();
+        aa.mask = h->mask.as<uint32_t>();
+        aa.omega = h->omega;
+        aa.om = h->om;
+        aa.tau = h->tau;
+        for (int d = 0; d < 3; d++) aa.a[d] = h->a[d];
+        aa.mrt = h->mrt.p;
+
+        for (size_t idx = 0; idx < sch.actions.size(); idx++) {
+            const Action& act = sch.actions[idx];
+            const bool forward_phase = idx < sch.reverse_start;
+            switch (act.kind) {
+            case ActKind::Snapshot:
+                if (cursor != act.a) fail(LBM_ERR_INVALID, "snapshot of a state the cursor does not hold");
+                hip_check(hipMemcpyAsync(slot_ptr(act.b), h->F.p, h->state_bytes, hipMemcpyDeviceToDevice, h->stream),
+                          "snapshot copy");
+                copies++;
+                break;
+            case ActKind::Restore:
+                hip_check(hipMemcpyAsync(h->F.p, slot_ptr(act.b), h->state_bytes, hipMemcpyDeviceToDevice, h->stream),
+                          "restore copy");
+                cursor = act.a;
+                copies++;
+                break;
+            case ActKind::Release: break;
+            case ActKind::Advance: {
+                if (cursor != act.a) fail(LBM_ERR_INVALID, "advance from a position the cursor does not hold");
+                const hipEvent_t e0 = event();
+                for (uint64_t t = act.a; t < act.b; t++) {
+                    const ObjArgs ob = forward_phase ? stage_obj(tmpl, obj, t, nsteps, false) : none;
+                    if (t + 1 == nsteps) {
+                        // Final step into the ahead buffer; F keeps state n-1.
+                        hip_check(hipMemcpyAsync(h->A.p, h->F.p, h->state_bytes, hipMemcpyDeviceToDevice, h->stream),
+                                  "ahead copy");
+                        step_once(h, h->A.p, base + t, ob, forward_phase, mass.as<double>() + t);
+                        fsteps++;
+                        // Objective on the final state and the cotangent seed lambda_n.
+                        StepArgs ea = h->step_args(h->A.p, base + nsteps);
+                        ea.obj = forward_phase ? stage_obj(tmpl, obj, nsteps, nsteps, true) : none;
+                        ea.macc = mass.as<double>() + nsteps;
+                        const ObjArgs seed = stage_obj(tmpl, obj, nsteps, nsteps, true);
+                        hip_check(h->run->objective(Q, h->has_mask, ea, seed, h->L.p, h->alpha_bar.as<double>(), 1,
+                                                    h->cfg()),
+                                  "seed launch");
+                        if (h->np)
+                            hip_check(h->run->port_save(Q, h->has_mask, h->port_args(base + nsteps), h->A.p,
+                                                        h->pbuf.p, h->stream),
+                                      "port save");
+                        break;
+                    }
+                    step_once(h, h->F.p, base + t, ob, forward_phase, mass.as<double>() + t);
+                    fsteps++;
+                    cursor = t + 1;
+                }
+                fwd_spans.emplace_back(e0, event());
+                break;
+            }
+            case ActKind::Reverse: {
+                const uint64_t k = act.a;  // reverse step k-1 -> k; F holds state k-1
+                if (cursor + 1 != k) fail(LBM_ERR_INVALID, "reverse of a step whose states are not held");
+                const hipEvent_t e0 = event();
+                if (h->np)
+                    hip_check(h->run->port_vjp(Q, h->has_mask, h->port_args(base + k), h->L.p, h->pbuf.p, h->stream),
+                              "port vjp");
+                aa.odd = int((base + k - 1) & 1);
+                aa.obj = stage_obj(tmpl, obj, k - 1, nsteps, false);
+                hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg()), "adjoint launch");
+                rsteps++;
+                if (h->np && k > 1)
+                    hip_check(h->run->port_save(Q, h->has_mask, h->port_args(base + k - 1), h->F.p, h->pbuf.p,
+                                                h->stream),
+                              "port save");
+                adj_spans.emplace_back(e0, event());
+                break;
+            }
+            }
+        }
+        // F holds some intermediate state; restore state 0 from slot 0 (taken first).
+        hip_check(hipMemcpyAsync(h->F.p, slot_ptr(0), h->state_bytes, hipMemcpyDeviceToDevice, h->stream), "restore");
+        double sc[4];
+        unsigned ds[2];
+        hip_check(hipMemcpyAsync(sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipMemcpyAsync(ds, h->scal.as<double>() + 4, sizeof(ds), hipMemcpyDeviceToHost, h->stream), "copy");
+        if (alpha_bar)
+            hip_check(hipMemcpyAsync(alpha_bar, h->alpha_bar.p, size_t(h->ncell) * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "copy");
+        std::vector<double> mh(nsteps + 1);
+        hip_check(hipMemcpyAsync(mh.data(), mass.p, mh.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        if (lambda0) {
+            ensure_aos(h);
+            hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(base & 1),
+                                      h->aos.as<double>(), h->cfg()),
+                      "get_lam");
+            hip_check(hipMemcpyAsync(lambda0, h->aos.p, lambda_len * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                      "copy");
+        }
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        for (auto& sp : fwd_spans) {
+            float ms;
+            hip_check(hipEventElapsedTime(&ms, sp.first, sp.second), "event time");
+            fwd_ms += ms;
+        }
+        for (auto& sp : adj_spans) {
+            float ms;
+            hip_check(hipEventElapsedTime(&ms, sp.first, sp.second), "event time");
+            adj_ms += ms;
+        }
+        for (auto e : evs) (void)hipEventDestroy(e);
+        h->t = base;
+        if (J) *J = sc[0];
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->forward_steps = fsteps;
+            stats->reverse_steps = rsteps;
+            stats->snapshot_slots = slots;
+            stats->peak_snapshots = sch.peak;
+            stats->snapshot_copies = copies;
+            stats->forward_ms = fwd_ms;
+            stats->adjoint_ms = adj_ms;
+            stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            float umax;
+            std::memcpy(&umax, &ds[0], 4);
+            stats->max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
+            stats->bad_cells = ds[1];
+            double drift = 0.0;
+            for (uint64_t s = 0; s <= nsteps; s++) drift = std::max(drift, std::fabs(mh[s] - mh[0]));
+            stats->mass_drift = h->np ? NAN : drift / mh[0];
+        }
+    });
+}
+
+int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms) {
+    return guarded([&] {
+        if (!h || !avg_ms) fail(LBM_ERR_CONTRACT, "null argument");
+        if (launches == 0) fail(LBM_ERR_CONTRACT, "launches must be positive");
+        ObjArgs none{};
+        if (which == 0) {
+            // forward steps in place on the live state (no ports: the kernel alone)
+            hip_check(hipEventRecord(h->ev0, h->stream), "event");
+            for (uint64_t s = 0; s < launches; s++) {
+                StepArgs sa = h->step_args(h->F.p, h->t);
+                hip_check(h->run->forward(h->Q, h->kind, h->has_mask, sa, h->cfg()), "forward launch");
+                h->t++;
+            }
+            hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        } else {
+            ensure_adjoint_buffers(h);
+            hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
+            hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+            AdjArgs a{};
+            a.g = h->g;
+            a.pop = h->F.p;
+            a.lam = h->L.p;
+            a.alpha = h->alpha.p;
+            a.alpha_bar = h->alpha_bar.as<double>();
+            a.mask = h->mask.as<uint32_t>();
+            a.omega = h->omega;
+            a.om = h->om;
+            a.tau = h->tau;
+            for (int d = 0; d < 3; d++) a.a[d] = h->a[d];
+            a.mrt = h->mrt.p;
+            a.obj = none;
+            hip_check(hipEventRecord(h->ev0, h->stream), "event");
+            for (uint64_t s = 0; s < launches; s++) {
+                a.odd = int((h->t + s) & 1);
+                hip_check(h->run->adjoint(h->Q, h->kind, h->has_mask, a, h->cfg()), "adjoint launch");
+            }
+            hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        }
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        *avg_ms = double(elapsed(h)) / double(launches);
+    });
+}
+
+}  // extern "C"
+
+// Plain copy kernel used as the bandwidth reference.
+__global__ void k_copy4(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms) {
+    return guarded([&] {
+        if (!gbps || iters == 0 || bytes < 16) fail(LBM_ERR_CONTRACT, "invalid copy benchmark arguments");
+        const size_t n = bytes / 16;
+        DevBuf a, b;
+        a.alloc(n * 16, "copy source");
+        b.alloc(n * 16, "copy destination");
+        hip_check(hipMemset(a.p, 0, a.bytes), "memset");
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        const int blocks = 256 * 32;
+        hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, 0, a.as<float4>(), b.as<float4>(), n);
+        hip_check(hipGetLastError(), "copy launch");
+        hip_check(hipEventRecord(e0, 0), "event");
+        for (uint64_t i = 0; i < iters; i++)
+            hipLaunchKernelGGL(k_copy4, dim3(blocks), dim3(256), 0, 0, a.as<float4>(), b.as<float4>(), n);
+        hip_check(hipGetLastError(), "copy launch");
+        hip_check(hipEventRecord(e1, 0), "event");
+        hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double avg = double(ms) / double(iters);
+        *gbps = 2.0 * double(n) * 16.0 / (avg * 1e-3) / 1e9;
+        if (avg_ms) *avg_ms = avg;
+    });
+}

@@ -1,0 +1,9 @@
+// Single-precision storage and arithmetic.
+#include "launch.h"
+
+namespace lbm {
+const Runner* runner_f32() {
+    static const RunnerImpl<P_F32, P_F32, true> r;
+    return &r;
+}
+}  // namespace lbm

@@ -1,0 +1,880 @@
+// Fused stream-collide kernels for gfx950 (forward, discrete adjoint, ports,
+// objectives, layout conversion).  Included by the per-precision translation units.
+//
+// Data layout in HBM
+//   Populations are structure-of-arrays: array q holds one value per storage cell,
+//   storage cell (i,j,k) at i*sxy + j*SZ + k (z fastest, as the reference's Grid,
+//   d3q19.rs:86-89).  Non-periodic axes carry one extra storage layer at index n
+//   (the esoteric-twist ghost layer); SZ is padded to a multiple of 32 elements.
+//
+// In-place esoteric-twist streaming
+//   With a_q = max(c_q, 0) and a_opp(q) = max(-c_q, 0) (so a_q - a_opp(q) = c_q), a
+//   step on a state of parity p (p = t mod 2) at cell x does
+//     read   f_q  from P[(p xor blocked_q) ? q : opp(q)] at x + a_opp(q)
+//     write  f*_q to   P[p ? opp(q) : q]                 at x + a_q
+//   where blocked_q means x - c_q is solid or outside a non-periodic face (halfway
+//   bounce-back, solver.rs:84-105).  Every storage location belongs to exactly one
+//   (cell, link), so one buffer is updated in place without races, each population
+//   is read and written once per step, and the state after the step is the
+//   reference's post-stream state (solver.rs:604-614) in the layout of parity 1-p.
+//   Solid cells are never updated; their slots carry the reflected populations of
+//   their fluid neighbours.
+//
+// Discrete adjoint in place
+//   The cotangent of f*_q(x) is the cotangent of whoever reads that value next step,
+//   and that reader stores its cotangent exactly where f*_q(x) was written.  So the
+//   adjoint of step t reads post_bar_q(x) at the forward write location of (x,q),
+//   applies the transposed collision Jacobian, and writes f_bar_q(x) at the forward
+//   read location of (x,q): solver.rs:642-674 (stream_vjp + collide_vjp) in one pass.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cell.h"
+#include "cumulant.h"
+
+namespace lbm {
+
+enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
+enum Kind : int { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5 };
+enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2 };
+
+constexpr uint32_t SOLID_BIT = 0x80000000u;
+
+struct Geom {
+    int nx, ny, nz;
+    int SY, SZ;
+    int px, py, pz;
+    long long sxy;     // SY*SZ
+    long long stride;  // elements per population array
+};
+
+// Objective evaluated on one state: weight * phi(state), phi per cell as
+//   dissipation:    rho * alpha * |u|^2 over fluid cells
+//   pressure drop:  (rho - 1)/3 * (1/na on plane a, -1/nb on plane b)
+struct ObjArgs {
+    int kind;
+    int axis, plane_a, plane_b;
+    double weight;
+    double inv_na, inv_nb;
+};
+
+struct StepArgs {
+    Geom g;
+    void* pop;
+    const void* alpha;
+    const uint32_t* mask;
+    double omega, om, tau;
+    double a[3];
+    const void* mrt;
+    int odd;
+    int diag;         // 1: accumulate diagnostics
+    ObjArgs obj;      // objective of the state being read (kind 0: none)
+    double* jacc;     // objective accumulator
+    double* macc;     // fluid mass of the state being read (diagnostics)
+    unsigned* dstat;  // [0] max |u|^2 (float bits), [1] bad cells (non-finite or f <= 0)
+};
+
+struct AdjArgs {
+    Geom g;
+    const void* pop;
+    void* lam;
+    const void* alpha;
+    double* alpha_bar;
+    const uint32_t* mask;
+    double omega, om, tau;
+    double a[3];
+    const void* mrt;
+    int odd;
+    ObjArgs obj;  // stage cost of the state being read (added to its cotangent)
+};
+
+// ---------------------------------------------------------------------------------
+// Storage codecs.  C is the arithmetic type; T the stored type.
+// ---------------------------------------------------------------------------------
+template <int P>
+struct Store;
+
+template <>
+struct Store<P_F64> {
+    using T = double;
+    using C = double;
+    static constexpr bool shifted = false;  // holds f itself
+    template <int Q, int q>
+    static LBM_DEV C ld(const T* p) {
+        return *p;
+    }
+    template <int Q, int q>
+    static LBM_DEV void st(T* p, C v) {
+        *p = v;
+    }
+};
+
+// FP32 holding d = f - w (same bytes as f, more significant bits of the deviation).
+template <>
+struct Store<P_F32> {
+    using T = float;
+    using C = float;
+    static constexpr bool shifted = true;
+    template <int Q, int q>
+    static LBM_DEV C ld(const T* p) {
+        return *p;
+    }
+    template <int Q, int q>
+    static LBM_DEV void st(T* p, C v) {
+        *p = v;
+    }
+};
+
+// FP16 holding d_q = f_q - w_q times 2^k_q; the arithmetic value is d (FP32).
+template <>
+struct Store<P_F16S> {
+    using T = _Float16;
+    using C = float;
+    static constexpr bool shifted = true;
+    template <int Q, int q>
+    static LBM_DEV C ld(const T* p) {
+        constexpr float inv = 1.0f / float(1u << fp16_exp<Q>(q));
+        return float(*p) * inv;
+    }
+    template <int Q, int q>
+    static LBM_DEV void st(T* p, C v) {
+        constexpr float s = float(1u << fp16_exp<Q>(q));
+        *p = _Float16(v * s);
+    }
+};
+
+LBM_DEV float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+LBM_DEV uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return uint16_t(u >> 16);
+}
+
+// BF16 holding d_q = f_q - w_q; the arithmetic value is d (FP32).
+template <>
+struct Store<P_BF16S> {
+    using T = uint16_t;
+    using C = float;
+    static constexpr bool shifted = true;
+    template <int Q, int q>
+    static LBM_DEV C ld(const T* p) {
+        return bf16_to_f32(*p);
+    }
+    template <int Q, int q>
+    static LBM_DEV void st(T* p, C v) {
+        *p = f32_to_bf16(v);
+    }
+};
+
+// Full population f from the arithmetic value of a codec (d + w when shifted).
+template <bool SH, int Q, class C>
+LBM_DEV void to_full(const C* v, C* f) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = SH ? v[q] + C(weight<Q>(q)) : v[q];
+}
+template <bool SH, int Q, class C>
+LBM_DEV void from_full(const C* f, C* v) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) v[q] = SH ? f[q] - C(weight<Q>(q)) : f[q];
+}
+
+// Cotangent (adjoint) storage: plain, no shift.
+template <int P>
+struct LamStore;
+template <>
+struct LamStore<P_F64> {
+    using T = double;
+    template <class C>
+    static LBM_DEV C ld(const T* p) {
+        return C(*p);
+    }
+    template <class C>
+    static LBM_DEV void st(T* p, C v) {
+        *p = double(v);
+    }
+};
+template <>
+struct LamStore<P_F32> {
+    using T = float;
+    template <class C>
+    static LBM_DEV C ld(const T* p) {
+        return C(*p);
+    }
+    template <class C>
+    static LBM_DEV void st(T* p, C v) {
+        *p = float(v);
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// Addressing.
+// ---------------------------------------------------------------------------------
+struct Cursor {
+    long long base;       // storage index of the cell
+    long long dx, dy, dz;  // storage offsets of the +x, +y, +z neighbours (wrapped)
+    LBM_DEV long long at(int corner) const {
+        return base + ((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0) + ((corner & 4) ? dz : 0);
+    }
+};
+
+LBM_DEV Cursor make_cursor(const Geom& g, int i, int j, int k) {
+    Cursor c;
+    c.base = (long long)i * g.sxy + (long long)j * g.SZ + k;
+    c.dx = (g.px && i + 1 == g.nx) ? -(long long)(g.nx - 1) * g.sxy : g.sxy;
+    c.dy = (g.py && j + 1 == g.ny) ? -(long long)(g.ny - 1) * g.SZ : (long long)g.SZ;
+    c.dz = (g.pz && k + 1 == g.nz) ? -(long long)(g.nz - 1) : 1;
+    return c;
+}
+
+// Blocked-link mask of a cell from the domain faces alone (no solid cells).
+template <int Q>
+LBM_DEV uint32_t face_mask(const Geom& g, int i, int j, int k) {
+    const bool lx = !g.px && i == 0, hx = !g.px && i == g.nx - 1;
+    const bool ly = !g.py && j == 0, hy = !g.py && j == g.ny - 1;
+    const bool lz = !g.pz && k == 0, hz = !g.pz && k == g.nz - 1;
+    uint32_t m = 0;
+    static_for<1, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        constexpr int cx = cvec<Q>(q, 0), cy = cvec<Q>(q, 1), cz = cvec<Q>(q, 2);
+        bool b = false;
+        if constexpr (cx > 0) b = b || lx;
+        if constexpr (cx < 0) b = b || hx;
+        if constexpr (cy > 0) b = b || ly;
+        if constexpr (cy < 0) b = b || hy;
+        if constexpr (cz > 0) b = b || lz;
+        if constexpr (cz < 0) b = b || hz;
+        m |= b ? (1u << q) : 0u;
+    });
+    return m;
+}
+
+template <int Q, bool MASK>
+LBM_DEV uint32_t cell_mask(const Geom& g, const uint32_t* mask, long long cell, int i, int j, int k) {
+    if constexpr (MASK)
+        return mask[cell];
+    else
+        return face_mask<Q>(g, i, j, k);
+}
+
+// Read location (array, storage index) of f_q at a cell for a state of parity odd.
+template <int Q, int q>
+LBM_DEV long long read_loc(const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+    const bool bl = (m >> q) & 1u;
+    const int arr = ((odd != 0) != bl) ? q : opp<Q>(q);
+    return (long long)arr * g.stride + cu.at(corner_neg<Q>(q));
+}
+// Write location of f*_q when stepping a state of parity odd.
+template <int Q, int q>
+LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
+    const int arr = odd ? opp<Q>(q) : q;
+    return (long long)arr * g.stride + cu.at(corner_pos<Q>(q));
+}
+
+template <int P, int Q>
+LBM_DEV void load_cell(const Geom& g, const typename Store<P>::T* pop, const Cursor& cu, uint32_t m, int odd,
+                       typename Store<P>::C* f) {
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        f[q] = Store<P>::template ld<Q, q>(pop + read_loc<Q, q>(g, cu, m, odd));
+    });
+}
+
+template <int P, int Q>
+LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cursor& cu, uint32_t m, int odd,
+                             const typename Store<P>::C* f) {
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        Store<P>::template st<Q, q>(pop + read_loc<Q, q>(g, cu, m, odd), f[q]);
+    });
+}
+
+// ---------------------------------------------------------------------------------
+// Reductions.
+// ---------------------------------------------------------------------------------
+template <class T>
+LBM_DEV T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+LBM_DEV float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// Sum two doubles over the block and add them to *oa, *ob with one atomic each.
+LBM_DEV void block_add2(double a, double b, double* oa, double* ob) {
+    __shared__ double red[2][16];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = a;
+        red[1][wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double sa = 0.0, sb = 0.0;
+        for (int w = 0; w < nw; w++) {
+            sa += red[0][w];
+            sb += red[1][w];
+        }
+        if (oa && sa != 0.0) unsafeAtomicAdd(oa, sa);
+        if (ob && sb != 0.0) unsafeAtomicAdd(ob, sb);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Objective per cell.
+// ---------------------------------------------------------------------------------
+template <bool SH, int Q, class C>
+LBM_DEV double objective_cell(const ObjArgs& ob, const C* f, const Macro<Q, C>& m, C alpha, int coord_axis) {
+    if (ob.kind == OBJ_DISSIPATION) {
+        const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
+        return ob.weight * double(m.rho * alpha * usq);
+    }
+    if (ob.kind == OBJ_PRESSURE_DROP) {
+        const double s = coord_axis == ob.plane_a ? ob.inv_na : (coord_axis == ob.plane_b ? -ob.inv_nb : 0.0);
+        if (s == 0.0) return 0.0;
+        // rho - 1 summed from the deviations avoids cancellation against 1.
+        C dev = SH ? f[0] : f[0] - C(weight<Q>(0));
+        static_for<1, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            dev += SH ? f[q] : f[q] - C(weight<Q>(q));
+        });
+        return ob.weight * s * (double(dev) / 3.0);
+    }
+    return 0.0;
+}
+
+// Adds weight * d phi / d f and d phi / d alpha to fb / ab.
+template <int Q, class C>
+LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, int coord_axis, C* fb, C& ab) {
+    if (ob.kind == OBJ_DISSIPATION) {
+        const C w = C(ob.weight);
+        const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
+        C rho_bar = w * alpha * usq;
+        const C usq_bar = w * m.rho * alpha;
+        C al_bar = w * m.rho * usq;
+        C jb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const C ub = usq_bar * C(2.0) * m.u[d];
+            al_bar -= ub * m.sv[d] / (m.den * m.den) * C(0.5);
+            const C sb = ub / m.den;
+            jb[d] = sb / m.rho;
+            rho_bar -= sb * m.j[d] / (m.rho * m.rho);
+        }
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            C v = rho_bar;
+            if constexpr (cvec<Q>(i, 0) != 0) v += jb[0] * C(cvec<Q>(i, 0));
+            if constexpr (cvec<Q>(i, 1) != 0) v += jb[1] * C(cvec<Q>(i, 1));
+            if constexpr (cvec<Q>(i, 2) != 0) v += jb[2] * C(cvec<Q>(i, 2));
+            fb[i] += v;
+        });
+        ab += al_bar;
+    } else if (ob.kind == OBJ_PRESSURE_DROP) {
+        const double s = coord_axis == ob.plane_a ? ob.inv_na : (coord_axis == ob.plane_b ? -ob.inv_nb : 0.0);
+        if (s != 0.0) {
+            const C gv = C(ob.weight * s / 3.0);
+#pragma unroll
+            for (int q = 0; q < Q; q++) fb[q] += gv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Collision dispatch (forward and transposed).
+// ---------------------------------------------------------------------------------
+template <int Q, class C, bool SH>
+LBM_DEV void macro_of(const C* v, C alpha, const C a[3], Macro<Q, C>& m) {
+    if constexpr (SH)
+        brinkman_macro_shifted<Q>(v, alpha, a, m);
+    else
+        brinkman_macro<Q>(v, alpha, a, m);
+}
+
+// v holds f (double precision, reference operation order) or d = f - w (single
+// precision, shifted formulation); out is returned in the same representation.
+template <int Q, int KIND, class C, bool SH>
+LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, const C* mrt, C* out,
+                     Macro<Q, C>& m) {
+    if constexpr (KIND == K_STREAM_ONLY) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) out[q] = v[q];
+        m.rho = C(1);
+    } else if constexpr (KIND == K_BGK_GUO) {
+        static_assert(Q == 19, "BGK-Guo is the D3Q19 stationary operator");
+        macro_of<Q, C, SH>(v, alpha, a, m);
+        if constexpr (SH)
+            collide_bgk_guo_shifted(v, m, tau, out);
+        else
+            collide_bgk_guo(v, alpha, a, tau, out);
+    } else {
+        macro_of<Q, C, SH>(v, alpha, a, m);
+        if constexpr (KIND == K_TRT || KIND == K_BGK) {
+            if constexpr (SH)
+                collide_pairs_shifted<Q>(v, m, omega, KIND == K_BGK ? omega : om, out);
+            else
+                collide_pairs<Q>(v, m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, out);
+        } else if constexpr (KIND == K_MRT) {
+            if constexpr (SH)
+                collide_mrt_shifted<Q>(v, m, mrt, out);
+            else
+                collide_mrt<Q>(v, m.rho, m.u, m.F, mrt, out);
+        } else {
+            C f[Q], o[Q];
+            to_full<SH, Q>(v, f);
+            collide_cumulant(f, m.rho, m.u, m.F, omega, mrt[0], o);
+            from_full<SH, Q>(o, out);
+        }
+    }
+}
+
+template <int Q, int KIND, class C, bool SH>
+LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau, const C* mrt, const C* g,
+                         Macro<Q, C>& m, C* fb, C& ab) {
+    macro_of<Q, C, SH>(v, alpha, a, m);
+    if constexpr (KIND == K_BGK_GUO) {
+        collide_bgk_guo_vjp(m, alpha, a, tau, g, fb, ab);
+    } else {
+        C cb_rho, cb_u[3], cb_F[3];
+        if constexpr (KIND == K_TRT || KIND == K_BGK) {
+            collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u, cb_F);
+        } else if constexpr (KIND == K_MRT) {
+            collide_mrt_vjp<Q>(m.rho, m.u, m.F, mrt, g, fb, cb_rho, cb_u, cb_F);
+        } else {
+            C f[Q];
+            to_full<SH, Q>(v, f);
+            collide_cumulant_vjp(f, m.rho, m.u, m.F, omega, mrt[0], g, fb, cb_rho, cb_u, cb_F);
+        }
+        brinkman_chain<Q>(m, alpha, a, cb_rho, cb_u, cb_F, fb, ab);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Forward step: one thread per cell.
+// ---------------------------------------------------------------------------------
+template <int P, int Q, int KIND, bool MASK>
+__global__ __launch_bounds__(256) void k_forward(StepArgs A) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    const bool active = k < g.nz;
+    double jsum = 0.0, msum = 0.0;
+    float umax = 0.0f;
+    unsigned bad = 0;
+    if (active) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
+        if (!(m & SOLID_BIT)) {
+            const Cursor cu = make_cursor(g, i, j, k);
+            typename S::T* pop = static_cast<typename S::T*>(A.pop);
+            C f[Q], fs[Q];
+            load_cell<P, Q>(g, pop, cu, m, A.odd, f);
+            const C alpha = static_cast<const C*>(A.alpha)[cell];
+            const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+            Macro<Q, C> mac;
+            collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                            static_cast<const C*>(A.mrt), fs, mac);
+            if (A.obj.kind != OBJ_NONE) {
+                const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+                jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
+            }
+            if (A.diag) {
+                msum = double(mac.rho);
+                umax = float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]);
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
+                    bad |= !(fq > C(0)) || !isfinite(float(fq));
+                }
+            }
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                S::template st<Q, q>(pop + write_loc<Q, q>(g, cu, A.odd), fs[q]);
+            });
+        }
+    }
+    if (A.obj.kind != OBJ_NONE || A.diag) block_add2(jsum, msum, A.jacc, A.diag ? A.macc : nullptr);
+    if (A.diag) {
+        umax = wave_max(umax);
+        const unsigned nbad = __popcll(__ballot(bad != 0));
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(A.dstat, __float_as_uint(umax));
+            if (nbad) atomicAdd(A.dstat + 1, nbad);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Adjoint step, in place on the cotangent buffer.
+// ---------------------------------------------------------------------------------
+template <int P, int LP, int Q, int KIND, bool MASK>
+__global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
+    using S = Store<P>;
+    using L = LamStore<LP>;
+    using C = typename S::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
+    if (m & SOLID_BIT) return;
+    const Cursor cu = make_cursor(g, i, j, k);
+    const typename S::T* pop = static_cast<const typename S::T*>(A.pop);
+    typename L::T* lam = static_cast<typename L::T*>(A.lam);
+    C f[Q], gb[Q], fb[Q];
+    load_cell<P, Q>(g, pop, cu, m, A.odd, f);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        gb[q] = L::template ld<C>(lam + write_loc<Q, q>(g, cu, A.odd));
+    });
+    const C alpha = static_cast<const C*>(A.alpha)[cell];
+    const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+    Macro<Q, C> mac;
+    C ab;
+    collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau), static_cast<const C*>(A.mrt), gb,
+                                        mac, fb, ab);
+    if (A.obj.kind != OBJ_NONE) {
+        const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+        objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+    }
+    A.alpha_bar[cell] += double(ab);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        L::template st<C>(lam + read_loc<Q, q>(g, cu, m, A.odd), fb[q]);
+    });
+}
+
+// ---------------------------------------------------------------------------------
+// Objective on one state: accumulate its value (eval) and/or write the cotangent seed
+// lambda = d J / d state at every fluid cell and alpha_bar = d J / d alpha (seed).
+// ---------------------------------------------------------------------------------
+template <int P, int LP, int Q, bool MASK>
+__global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, void* lam, double* alpha_bar,
+                                                   int do_seed) {
+    using S = Store<P>;
+    using L = LamStore<LP>;
+    using C = typename S::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    double jsum = 0.0, msum = 0.0;
+    if (k < g.nz) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
+        if (!(m & SOLID_BIT)) {
+            const Cursor cu = make_cursor(g, i, j, k);
+            C f[Q];
+            load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, m, A.odd, f);
+            const C alpha = static_cast<const C*>(A.alpha)[cell];
+            const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+            Macro<Q, C> mac;
+            macro_of<Q, C, S::shifted>(f, alpha, a, mac);
+            if (A.obj.kind != OBJ_NONE) {
+                const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+                jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
+            }
+            msum = double(mac.rho);
+            if (do_seed) {
+                C fb[Q];
+                C ab = C(0);
+#pragma unroll
+                for (int q = 0; q < Q; q++) fb[q] = C(0);
+                if (seed.kind != OBJ_NONE) {
+                    const int coord = seed.axis == 0 ? i : (seed.axis == 1 ? j : k);
+                    objective_grad<Q, C>(seed, mac, alpha, coord, fb, ab);
+                }
+                alpha_bar[cell] = double(ab);
+                typename L::T* lp = static_cast<typename L::T*>(lam);
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    L::template st<C>(lp + read_loc<Q, q>(g, cu, m, A.odd), fb[q]);
+                });
+            }
+        } else if (do_seed) {
+            alpha_bar[cell] = 0.0;
+        }
+    }
+    block_add2(jsum, msum, A.jacc, A.macc);
+}
+
+// ---------------------------------------------------------------------------------
+// Ports: moving/boundary.rs:445-506 (NEE port_populations and its VJP) applied in the
+// stationary order: after streaming, replace every population of a port cell.
+// ---------------------------------------------------------------------------------
+struct PortArgs {
+    Geom g;
+    int np;
+    const int* cell;      // port cell (reference index)
+    const int* nb;        // inward neighbour
+    const int* kind;      // 0 velocity, 1 pressure
+    const double* value;  // 3 per port cell
+    const uint32_t* mask;
+    int odd;              // parity of the state the ports act on
+};
+
+template <int Q, class C>
+LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out) {
+    C rho_n, j[3], u_n[3];
+    moments<Q>(nb, rho_n, j);
+#pragma unroll
+    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    C eb[Q], en[Q];
+    if (kind == 0) {
+        const C ub[3] = {C(value[0]), C(value[1]), C(value[2])};
+        equilibrium<Q>(rho_n, ub, eb);
+    } else {
+        equilibrium<Q>(C(value[0]), u_n, eb);
+    }
+    equilibrium<Q>(rho_n, u_n, en);
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = eb[q] + nb[q] - en[q];
+}
+
+template <int Q, class C>
+LBM_DEV void port_populations_vjp(const C* nb, int kind, const double* value, const C* g, C* nb_bar) {
+    C rho_n, j[3], u_n[3];
+    moments<Q>(nb, rho_n, j);
+#pragma unroll
+    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    C rb_bar = C(0), ub_bar[3] = {C(0), C(0), C(0)};
+    if (kind == 0) {
+        const C ub[3] = {C(value[0]), C(value[1]), C(value[2])};
+        equilibrium_vjp<Q>(rho_n, ub, g, rb_bar, ub_bar);
+    } else {
+        equilibrium_vjp<Q>(C(value[0]), u_n, g, rb_bar, ub_bar);
+    }
+    C neg[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) neg[q] = -g[q];
+    C rn_bar = C(0), un_bar[3] = {C(0), C(0), C(0)};
+    equilibrium_vjp<Q>(rho_n, u_n, neg, rn_bar, un_bar);
+    if (kind == 0) {
+        rn_bar += rb_bar;
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) un_bar[d] += ub_bar[d];
+    }
+    C j_bar[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        j_bar[d] = un_bar[d] / rho_n;
+        rn_bar -= un_bar[d] * j[d] / (rho_n * rho_n);
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        C v = g[i] + rn_bar;
+        if constexpr (cvec<Q>(i, 0) != 0) v += j_bar[0] * C(cvec<Q>(i, 0));
+        if constexpr (cvec<Q>(i, 1) != 0) v += j_bar[1] * C(cvec<Q>(i, 1));
+        if constexpr (cvec<Q>(i, 2) != 0) v += j_bar[2] * C(cvec<Q>(i, 2));
+        nb_bar[i] = v;
+    });
+}
+
+LBM_DEV void decode_cell(const Geom& g, long long cell, int& i, int& j, int& k) {
+    k = int(cell % g.nz);
+    const long long ij = cell / g.nz;
+    j = int(ij % g.ny);
+    i = int(ij / g.ny);
+}
+
+template <int Q, bool MASK>
+LBM_DEV uint32_t mask_of(const PortArgs& A, long long cell, Cursor& cu) {
+    int i, j, k;
+    decode_cell(A.g, cell, i, j, k);
+    cu = make_cursor(A.g, i, j, k);
+    return cell_mask<Q, MASK>(A.g, A.mask, cell, i, j, k);
+}
+
+// f(port cell) := NEE(f(neighbour)); exch[r] gets the mass change sum_q (new - old).
+template <int P, int Q, bool MASK>
+__global__ void k_port_apply(PortArgs A, void* pop_, double* exch_sum) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double delta = 0.0;
+    if (r < A.np) {
+        typename S::T* pop = static_cast<typename S::T*>(pop_);
+        Cursor cx, cn;
+        const uint32_t mx = mask_of<Q, MASK>(A, A.cell[r], cx);
+        const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
+        C vnb[Q], old[Q], fnb[Q], fout[Q], out[Q];
+        load_cell<P, Q>(A.g, pop, cn, mn, A.odd, vnb);
+        load_cell<P, Q>(A.g, pop, cx, mx, A.odd, old);
+        to_full<S::shifted, Q>(vnb, fnb);
+        port_populations<Q, C>(fnb, A.kind[r], A.value + 3 * r, fout);
+        from_full<S::shifted, Q>(fout, out);
+        store_cell_read<P, Q>(A.g, pop, cx, mx, A.odd, out);
+        // Mass exchange of the stored (encoded) values, summed in order of q.
+        C newv[Q];
+        load_cell<P, Q>(A.g, pop, cx, mx, A.odd, newv);
+#pragma unroll
+        for (int q = 0; q < Q; q++) delta += double(newv[q]) - double(old[q]);
+    }
+    if (exch_sum) {
+        delta = wave_sum(delta);
+        if ((threadIdx.x & 63) == 0 && delta != 0.0) unsafeAtomicAdd(exch_sum, delta);
+    }
+}
+
+// Save the populations of every port neighbour (state of parity A.odd) to buf[r*Q+q].
+template <int P, int Q, bool MASK>
+__global__ void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* buf) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.np) return;
+    Cursor cn;
+    const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
+    C vnb[Q], fnb[Q];
+    load_cell<P, Q>(A.g, static_cast<const typename S::T*>(pop_), cn, mn, A.odd, vnb);
+    to_full<S::shifted, Q>(vnb, fnb);
+#pragma unroll
+    for (int q = 0; q < Q; q++) buf[(long long)r * Q + q] = fnb[q];
+}
+
+// Adjoint of the port pass on the cotangent of a state of parity A.odd:
+// lam(nb) += J_NEE(f_nb)^T lam(port); lam(port) := 0   (solver.rs:642-656).
+template <int P, int LP, int Q, bool MASK>
+__global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf) {
+    using L = LamStore<LP>;
+    using C = typename Store<P>::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.np) return;
+    typename L::T* lam = static_cast<typename L::T*>(lam_);
+    Cursor cx, cn;
+    const uint32_t mx = mask_of<Q, MASK>(A, A.cell[r], cx);
+    const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
+    C gx[Q], gn[Q], fnb[Q], nbb[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        gx[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd));
+        gn[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd));
+    });
+#pragma unroll
+    for (int q = 0; q < Q; q++) fnb[q] = buf[(long long)r * Q + q];
+    port_populations_vjp<Q, C>(fnb, A.kind[r], A.value + 3 * r, gx, nbb);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        L::template st<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), gn[q] + nbb[q]);
+        L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0));
+    });
+}
+
+// ---------------------------------------------------------------------------------
+// Layout conversion between the twisted device layout and the reference AoS layout.
+// ---------------------------------------------------------------------------------
+template <int P, int Q, bool MASK>
+__global__ void k_get_state(Geom g, const void* pop_, const uint32_t* mask, int odd, double* aos) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) aos[cell * Q + q] = weight<Q>(q);
+        return;
+    }
+    const Cursor cu = make_cursor(g, i, j, k);
+    C f[Q];
+    load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
+#pragma unroll
+    for (int q = 0; q < Q; q++) aos[cell * Q + q] = S::shifted ? weight<Q>(q) + double(f[q]) : double(f[q]);
+}
+
+template <int P, int Q, bool MASK>
+__global__ void k_set_state(Geom g, void* pop_, const uint32_t* mask, int odd, const double* aos) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) return;
+    const Cursor cu = make_cursor(g, i, j, k);
+    C f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = S::shifted ? C(aos[cell * Q + q] - weight<Q>(q)) : C(aos[cell * Q + q]);
+    store_cell_read<P, Q>(g, static_cast<typename S::T*>(pop_), cu, m, odd, f);
+}
+
+template <int LP, int Q, bool MASK>
+__global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos) {
+    using L = LamStore<LP>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) aos[cell * Q + q] = 0.0;
+        return;
+    }
+    const Cursor cu = make_cursor(g, i, j, k);
+    const typename L::T* lam = static_cast<const typename L::T*>(lam_);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        aos[cell * Q + q] = L::template ld<double>(lam + read_loc<Q, q>(g, cu, m, odd));
+    });
+}
+
+template <int LP, int Q, bool MASK>
+__global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos) {
+    using L = LamStore<LP>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) return;
+    const Cursor cu = make_cursor(g, i, j, k);
+    typename L::T* lam = static_cast<typename L::T*>(lam_);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        L::template st<double>(lam + read_loc<Q, q>(g, cu, m, odd), aos[cell * Q + q]);
+    });
+}
+
+// Macroscopic fields (solver.rs:690-705 in lattice units): rho, and u (0 in solids).
+template <int P, int Q, bool MASK>
+__global__ void k_fields(Geom g, const void* pop_, const uint32_t* mask, int odd, const void* alpha_, double a0,
+                         double a1, double a2, double* rho, double* u) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= g.nz) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    C f[Q];
+    if (m & SOLID_BIT) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) f[q] = S::shifted ? C(0) : C(weight<Q>(q));
+    } else {
+        const Cursor cu = make_cursor(g, i, j, k);
+        load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
+    }
+    const C a[3] = {C(a0), C(a1), C(a2)};
+    Macro<Q, C> mac;
+    macro_of<Q, C, S::shifted>(f, static_cast<const C*>(alpha_)[cell], a, mac);
+    rho[cell] = double(mac.rho);
+    for (int d = 0; d < 3; d++) u[3 * cell + d] = (m & SOLID_BIT) ? 0.0 : double(mac.u[d]);
+}
+
+}  // namespace lbm

@@ -1,0 +1,419 @@
+"""Python binding of ``libimplexity_lbm.so`` (the C ABI in include/implexity_lbm.h).
+
+The names mirror the reference's LBM solver and adjoint interfaces
+(crates/implexity-physics-lbm/src/solver.rs:359-947 and crates/implexity-ad/src/
+scan.rs, revolve.rs) so that host code reads like the reference:
+
+    ``Lattice.initial_populations``  LbmProblem::initial_populations (solver.rs:592)
+    ``Lattice.interval``             LbmProblem::interval (solver.rs:604)
+    ``Lattice.interval_vjp``         LbmProblem::interval_vjp (solver.rs:642)
+    ``Lattice.history``              LbmProblem::history + objective (solver.rs:677)
+    ``Lattice.gradient``             history_adjoint behind scan_adjoint with
+                                     Checkpointing::Binomial (solver.rs:798, scan.rs:140)
+    ``Lattice.fields``               LbmProblem::fields in lattice units (solver.rs:690)
+    ``BinomialSchedule``             implexity-ad BinomialSchedule (revolve.rs:128)
+
+The library is the only compute path: there is no CPU fallback, and importing this
+module raises if the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libimplexity_lbm.so")
+
+PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
+COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4}
+FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
+OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2}
+MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
+
+
+class LbmError(RuntimeError):
+    """Base error; ``code`` is the lbm_status value."""
+
+    code = -1
+
+
+class ContractError(LbmError):
+    """CaeError::Contract (crates/implexity-core/src/error.rs:34)."""
+
+    code = 1
+
+
+class ConvergenceError(LbmError):
+    code = 2
+
+
+class ShapeError(LbmError):
+    code = 3
+
+
+class InvalidError(LbmError):
+    code = 4
+
+
+class DeviceError(LbmError):
+    code = 5
+
+
+class MemoryError_(LbmError):
+    code = 6
+
+
+_ERRORS = {c.code: c for c in (ContractError, ConvergenceError, ShapeError, InvalidError, DeviceError, MemoryError_)}
+
+
+class _Port(ctypes.Structure):
+    _fields_ = [
+        ("face", ctypes.c_uint32), ("kind", ctypes.c_uint32), ("value", ctypes.c_double * 3),
+        ("profile", ctypes.c_void_p), ("profile_len", ctypes.c_uint64),
+    ]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("lattice", ctypes.c_uint32), ("collision", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32), ("shape", ctypes.c_int32 * 3), ("periodic", ctypes.c_uint8 * 3),
+        ("reserved0", ctypes.c_uint8 * 5), ("omega", ctypes.c_double), ("magic", ctypes.c_double),
+        ("bulk_rate", ctypes.c_double), ("even_rate", ctypes.c_double), ("accel", ctypes.c_double * 3),
+        ("n_ports", ctypes.c_uint32), ("reserved1", ctypes.c_uint32), ("ports", ctypes.c_void_p),
+        ("checkpoint_budget_bytes", ctypes.c_uint64),
+    ]
+
+
+class _Objective(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32), ("axis", ctypes.c_uint32), ("plane_a", ctypes.c_int32),
+        ("plane_b", ctypes.c_int32), ("t_begin", ctypes.c_uint64), ("t_end", ctypes.c_uint64),
+        ("weight", ctypes.c_double), ("terminal_weight", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("forward_steps", ctypes.c_uint64), ("reverse_steps", ctypes.c_uint64),
+        ("snapshot_slots", ctypes.c_uint64), ("peak_snapshots", ctypes.c_uint64),
+        ("snapshot_copies", ctypes.c_uint64), ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double), ("max_mach", ctypes.c_double), ("mass_drift", ctypes.c_double),
+        ("bad_cells", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# Every symbol include/implexity_lbm.h declares, with its argument and result types.
+_SIGS = {
+    "lbm_abi_version": ([], ctypes.c_int),
+    "lbm_last_error": ([ctypes.c_char_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
+    "lbm_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "lbm_set_design": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_reset_initial": ([ctypes.c_void_p], ctypes.c_int),
+    "lbm_set_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_get_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_step_count": ([ctypes.c_void_p], ctypes.c_uint64),
+    "lbm_step": ([ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_forward": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                    ctypes.c_int),
+    "lbm_gradient": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
+    "lbm_step_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+                     ctypes.c_int),
+    "lbm_get_fields": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_debug_link_table": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_port_cells": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
+                       ctypes.c_int),
+    "lbm_revolve_schedule": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "lbm_revolve_optimal_forward_steps": ([ctypes.c_uint64, ctypes.c_uint64], ctypes.c_uint64),
+    "lbm_time_kernel": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
+    "lbm_copy_bandwidth": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "lbm_state_bytes": ([ctypes.c_void_p], ctypes.c_uint64),
+}
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m implexity_amd.build` "
+                "(there is no CPU fallback for the LBM hot path)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(_lib, name)
+            fn.argtypes = args
+            fn.restype = res
+    return _lib
+
+
+def _check(code: int) -> None:
+    if code == 0:
+        return
+    buf = ctypes.create_string_buffer(1024)
+    lib().lbm_last_error(buf, len(buf))
+    raise _ERRORS.get(code, LbmError)(buf.value.decode(errors="replace"))
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+@dataclass
+class Port:
+    """Inlet/outlet face (moving/boundary.rs:126-143 PortSpec, NEE rule :445-506)."""
+
+    face: str
+    kind: str  # "velocity" | "pressure"
+    value: tuple  # velocity (3,) lattice units, or (rho,)
+    profile: np.ndarray | None = None
+
+
+@dataclass
+class LatticeConfig:
+    shape: tuple
+    periodic: tuple = (False, False, False)
+    lattice: int = 19
+    collision: str = "trt"
+    precision: str = "f32"
+    omega: float = 1.0
+    magic: float = MAGIC_DEFAULT
+    bulk_rate: float = float("nan")
+    even_rate: float = float("nan")
+    accel: tuple = (0.0, 0.0, 0.0)
+    ports: list = field(default_factory=list)
+    checkpoint_budget_bytes: int = 0
+    solid: np.ndarray | None = None
+
+
+@dataclass
+class Objective:
+    """J = sum_{s in [t_begin, t_end]} weight * phi(state s) + terminal_weight * phi(state n)."""
+
+    kind: str = "none"
+    t_begin: int = 1
+    t_end: int = 1
+    weight: float = 0.0
+    terminal_weight: float = 0.0
+    axis: int = 0
+    plane_a: int = 1
+    plane_b: int = -2
+
+    def _c(self):
+        o = _Objective()
+        o.kind = OBJECTIVE[self.kind]
+        o.axis = self.axis
+        o.plane_a = self.plane_a
+        o.plane_b = self.plane_b
+        o.t_begin = self.t_begin
+        o.t_end = self.t_end
+        o.weight = self.weight
+        o.terminal_weight = self.terminal_weight
+        return o
+
+
+class Lattice:
+    """One lattice problem resident on the GPU (an lbm_handle)."""
+
+    def __init__(self, cfg: LatticeConfig):
+        self.cfg = cfg
+        self.shape = tuple(int(s) for s in cfg.shape)
+        self.n = int(np.prod(self.shape))
+        self.Q = int(cfg.lattice)
+        c = _Config()
+        c.struct_size = ctypes.sizeof(_Config)
+        c.lattice = self.Q
+        c.collision = COLLISION[cfg.collision]
+        c.precision = PRECISION[cfg.precision]
+        for d in range(3):
+            c.shape[d] = self.shape[d]
+            c.periodic[d] = 1 if cfg.periodic[d] else 0
+            c.accel[d] = float(cfg.accel[d])
+        c.omega = float(cfg.omega)
+        c.magic = float(cfg.magic)
+        c.bulk_rate = float(cfg.bulk_rate)
+        c.even_rate = float(cfg.even_rate)
+        self._profiles = []
+        ports = (_Port * max(1, len(cfg.ports)))()
+        for i, port in enumerate(cfg.ports):
+            ports[i].face = FACE[port.face]
+            ports[i].kind = 0 if port.kind == "velocity" else 1
+            vals = list(port.value) + [0.0] * (3 - len(port.value))
+            for d in range(3):
+                ports[i].value[d] = float(vals[d])
+            if port.profile is not None:
+                prof = np.ascontiguousarray(port.profile, np.float64).reshape(-1)
+                self._profiles.append(prof)
+                ports[i].profile = prof.ctypes.data
+                ports[i].profile_len = prof.size
+        c.n_ports = len(cfg.ports)
+        c.ports = ctypes.cast(ports, ctypes.c_void_p)
+        c.checkpoint_budget_bytes = int(cfg.checkpoint_budget_bytes)
+        solid = None
+        if cfg.solid is not None:
+            solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)
+        h = ctypes.c_void_p()
+        _check(lib().lbm_create(ctypes.byref(c), _p(solid), 0 if solid is None else solid.size, ctypes.byref(h)))
+        self._h = h
+        self._ports_c = ports
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- design and state ----------------------------------------------------------
+    def set_design(self, alpha):
+        a = np.ascontiguousarray(alpha, np.float64).reshape(-1)
+        _check(lib().lbm_set_design(self._h, _p(a), a.size))
+
+    def initial_populations(self):
+        _check(lib().lbm_reset_initial(self._h))
+
+    def set_state(self, f, layout="aos"):
+        f = np.ascontiguousarray(f, np.float64).reshape(-1)
+        _check(lib().lbm_set_state(self._h, 0 if layout == "aos" else 1, _p(f), f.size))
+
+    def state(self, layout="aos"):
+        out = np.empty(self.n * self.Q)
+        _check(lib().lbm_get_state(self._h, 0 if layout == "aos" else 1, _p(out), out.size))
+        return out
+
+    @property
+    def step_count(self):
+        return int(lib().lbm_step_count(self._h))
+
+    @property
+    def state_bytes(self):
+        return int(lib().lbm_state_bytes(self._h))
+
+    # --- dynamics -------------------------------------------------------------------
+    def interval(self, steps=1):
+        _check(lib().lbm_step(self._h, steps))
+
+    def history(self, steps, objective: Objective | None = None):
+        """Forward sweep; returns (J, stats)."""
+        J = ctypes.c_double(0.0)
+        st = Stats()
+        o = objective._c() if objective else None
+        _check(lib().lbm_forward(self._h, steps, ctypes.byref(o) if o else None, ctypes.byref(J), ctypes.byref(st)))
+        return J.value, st
+
+    def gradient(self, steps, objective: Objective, want_lambda0=False):
+        """Forward + discrete adjoint from the current state (binomial checkpointing).
+        Returns (J, alpha_bar, lambda0 or None, stats)."""
+        J = ctypes.c_double(0.0)
+        st = Stats()
+        ab = np.zeros(self.n)
+        lam = np.zeros(self.n * self.Q) if want_lambda0 else None
+        o = objective._c()
+        _check(lib().lbm_gradient(self._h, steps, ctypes.byref(o), ctypes.byref(J), _p(ab), ab.size, _p(lam),
+                                  0 if lam is None else lam.size, ctypes.byref(st)))
+        return J.value, ab, lam, st
+
+    def interval_vjp(self, lambda_next):
+        lam = np.ascontiguousarray(lambda_next, np.float64).reshape(-1)
+        out = np.empty_like(lam)
+        ab = np.empty(self.n)
+        _check(lib().lbm_step_vjp(self._h, _p(lam), _p(out), _p(ab), lam.size))
+        return out, ab
+
+    def fields(self):
+        rho = np.empty(self.n)
+        u = np.empty(3 * self.n)
+        _check(lib().lbm_get_fields(self._h, _p(rho), _p(u), self.n))
+        return rho, u.reshape(self.n, 3)
+
+    def link_table(self):
+        out = np.empty(self.n * self.Q, np.uint32)
+        _check(lib().lbm_debug_link_table(self._h, _p(out), out.size))
+        return out
+
+    def port_cells(self):
+        cnt = ctypes.c_uint64(0)
+        _check(lib().lbm_port_cells(self._h, None, None, 0, ctypes.byref(cnt)))
+        cells = np.empty(cnt.value, np.int32)
+        nbs = np.empty(cnt.value, np.int32)
+        _check(lib().lbm_port_cells(self._h, _p(cells), _p(nbs), cnt.value, ctypes.byref(cnt)))
+        return cells, nbs
+
+    def time_kernel(self, which="forward", launches=10):
+        ms = ctypes.c_double(0.0)
+        _check(lib().lbm_time_kernel(self._h, 0 if which == "forward" else 1, launches, ctypes.byref(ms)))
+        return ms.value
+
+
+class BinomialSchedule:
+    """Revolve schedule built by the library (revolve.rs:128-329)."""
+
+    KINDS = ("advance", "snapshot", "restore", "reverse", "release")
+
+    def __init__(self, steps: int, snapshots: int):
+        cnt = ctypes.c_uint64(0)
+        fwd = ctypes.c_uint64(0)
+        rs = ctypes.c_uint64(0)
+        _check(lib().lbm_revolve_schedule(steps, snapshots, None, 0, ctypes.byref(cnt), None, None))
+        buf = np.zeros(5 * cnt.value, np.int64)
+        _check(lib().lbm_revolve_schedule(steps, snapshots, _p(buf), cnt.value, ctypes.byref(cnt), ctypes.byref(fwd),
+                                          ctypes.byref(rs)))
+        self.steps = steps
+        self.forward_steps = fwd.value
+        self.reverse_start = rs.value
+        acts = []
+        for k, a, b, _, _ in buf.reshape(-1, 5):
+            name = self.KINDS[k]
+            if name in ("advance", "snapshot", "restore"):
+                acts.append((name, int(a), int(b)))
+            elif name == "reverse":
+                acts.append((name, int(a)))
+            else:
+                acts.append((name, int(a)))
+        self.actions = acts
+
+
+def optimal_forward_steps(steps: int, snapshots: int) -> int:
+    return int(lib().lbm_revolve_optimal_forward_steps(steps, snapshots))
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20):
+    g = ctypes.c_double(0.0)
+    ms = ctypes.c_double(0.0)
+    _check(lib().lbm_copy_bandwidth(nbytes, iters, ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value
+
+
+def odd_rate(omega: float, magic: float) -> float:
+    """TRT odd relaxation rate (moving/collision.rs:302-305)."""
+    tp = 1.0 / omega - 0.5
+    return 1.0 / (0.5 + magic / tp)
+
+
+def declared_symbols(header: str) -> list[str]:
+    """Function names declared in include/implexity_lbm.h."""
+    import re
+
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t)\s+(lbm_\w+)\s*\(", text, flags=re.M)))

@@ -1,0 +1,204 @@
+/*
+ * implexity_lbm.h -- C ABI of the MI355X differentiable lattice-Boltzmann library.
+ *
+ * Drop-in boundary for Implexity's LBM hot path.  The GPU library replaces the
+ * reference's in-process CPU loops below these interfaces (paths relative to
+ * crates/implexity-physics-lbm/src/ unless noted):
+ *
+ *   lbm_create            LbmProblem::normalize + Streaming::new      solver.rs:414-550, 83-120
+ *                         (grid, periodic axes, solid mask, halfway bounce-back table) and,
+ *                         for the collision operator, Collision::new   moving/collision.rs:244-292
+ *   lbm_set_design        the Brinkman coefficient field alpha = LbmProblem::resistance(phi)
+ *                                                                       solver.rs:580-590, design.rs:282-294
+ *   lbm_reset_initial     LbmProblem::initial_populations             solver.rs:592-601
+ *   lbm_step              LbmProblem::interval (collide->stream->ports) solver.rs:603-614
+ *   lbm_forward           LbmProblem::history + objective             solver.rs:676-687
+ *   lbm_gradient          history + history_adjoint behind scan_adjoint with binomial
+ *                         checkpointing                                solver.rs:797-815,
+ *                         crates/implexity-ad/src/scan.rs:140-390, revolve.rs:128-388
+ *   lbm_step_vjp          LbmProblem::interval_vjp                    solver.rs:641-674
+ *   lbm_get_state         the reference's AoS state vector (x*Q+q)    solver.rs:90,325
+ *   lbm_get_fields        LbmProblem::fields (lattice units)          solver.rs:689-705
+ *   lbm_debug_link_table  Streaming::source_of for every (cell, q)    solver.rs:160-168
+ *   lbm_revolve_schedule  BinomialSchedule::new / actions             revolve.rs:261-329
+ *
+ * Conventions
+ *   - Plain pointers and explicit lengths; host buffers are caller-owned (like the
+ *     reference's borrowed &[f64] inputs and owned Vec<f64> outputs).
+ *   - Every call returns an lbm_status; on failure lbm_last_error() gives the message
+ *     (thread-local).  Nothing unwinds across the ABI.
+ *   - A handle owns its device memory and one HIP stream.  A handle is not safe for
+ *     concurrent calls from several threads (wrap it in a mutex, as the reference
+ *     wraps its adjoint cache, moving/field.rs:542).
+ *   - All quantities are in lattice units.  Cell index x = (i*ny + j)*nz + k.
+ */
+#ifndef IMPLEXITY_LBM_H
+#define IMPLEXITY_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
+ * crates/implexity-ad/src/error.rs). */
+typedef enum {
+    LBM_OK = 0,
+    LBM_ERR_CONTRACT = 1,    /* CaeError::Contract: invalid problem or argument     */
+    LBM_ERR_CONVERGENCE = 2, /* CaeError::Convergence: admission refused (NaN, f<=0, Mach) */
+    LBM_ERR_SHAPE = 3,       /* AdError::Shape: buffer length mismatch              */
+    LBM_ERR_INVALID = 4,     /* AdError::Invalid: schedule or state inconsistency   */
+    LBM_ERR_DEVICE = 5,      /* HIP runtime failure                                 */
+    LBM_ERR_MEMORY = 6       /* device allocation failed or budget too small        */
+} lbm_status;
+
+typedef enum { LBM_D3Q19 = 19, LBM_D3Q27 = 27 } lbm_lattice;
+
+/* Collision operators.  BGK/TRT: moving/collision.rs:457-488; MRT: :79-197,349-375;
+ * CUMULANT (D3Q27 only): :636-752; BGK_GUO: the stationary solver's collide_cell
+ * (solver.rs:298-322, D3Q19 only).  All use the Brinkman branch of psm.rs:213-226. */
+typedef enum {
+    LBM_BGK = 0,
+    LBM_TRT = 1,
+    LBM_MRT = 2,
+    LBM_CUMULANT = 3,
+    LBM_BGK_GUO = 4
+} lbm_collision;
+
+/* Population storage.  Arithmetic is FP64 for LBM_F64 and FP32 otherwise. */
+typedef enum {
+    LBM_F64 = 0,
+    LBM_F32 = 1,
+    LBM_F16_SHIFTED = 2, /* FP16 of (f_q - w_q) times an exact power of two */
+    LBM_BF16_SHIFTED = 3 /* BF16 of (f_q - w_q); BGK and TRT only            */
+} lbm_precision;
+
+typedef enum { LBM_FACE_XMIN = 0, LBM_FACE_XMAX, LBM_FACE_YMIN, LBM_FACE_YMAX, LBM_FACE_ZMIN, LBM_FACE_ZMAX } lbm_face;
+
+typedef enum { LBM_PORT_VELOCITY = 0, LBM_PORT_PRESSURE = 1 } lbm_port_kind;
+
+/* One inlet/outlet face, non-equilibrium-extrapolation rule of moving/boundary.rs:
+ * 445-506 applied after streaming.  Port cells are the non-solid cells of the face in
+ * ascending cell order (boundary.rs:331-333); each takes NEE populations from its
+ * inward neighbour.  Velocity ports: rho_b = rho_nb, u_b = value * profile[r].
+ * Pressure ports: rho_b = value[0], u_b = u_nb (zero-gradient outlet). */
+typedef struct {
+    uint32_t face;          /* lbm_face */
+    uint32_t kind;          /* lbm_port_kind */
+    double value[3];        /* velocity (lattice) or density in value[0] */
+    const double* profile;  /* one factor per port cell, or NULL for 1 */
+    uint64_t profile_len;
+} lbm_port;
+
+typedef struct {
+    uint32_t struct_size;         /* sizeof(lbm_config) */
+    uint32_t lattice;             /* lbm_lattice */
+    uint32_t collision;           /* lbm_collision */
+    uint32_t precision;           /* lbm_precision */
+    int32_t shape[3];             /* nx, ny, nz (each >= 2) */
+    uint8_t periodic[3];          /* 0: closed halfway-bounce-back walls */
+    uint8_t reserved0[5];
+    double omega;                 /* 1/tau+ ; tau+ in [0.51, 2] */
+    double magic;                 /* TRT magic Lambda (default 3/16) / MRT odd_magic */
+    double bulk_rate;             /* MRT bulk rate or cumulant bulk rate; NaN = default */
+    double even_rate;             /* MRT even higher-moment rate; NaN = default */
+    double accel[3];              /* body acceleration (lattice) */
+    uint32_t n_ports;
+    uint32_t reserved1;
+    const lbm_port* ports;
+    uint64_t checkpoint_budget_bytes; /* device bytes for revolve snapshots (0: 1/2 of free) */
+} lbm_config;
+
+/* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
+ *             + terminal_weight * phi(final state), states indexed 1..n (scan.rs:96-107). */
+typedef enum { LBM_OBJ_NONE = 0, LBM_OBJ_DISSIPATION = 1, LBM_OBJ_PRESSURE_DROP = 2 } lbm_objective_kind;
+
+typedef struct {
+    uint32_t kind;        /* lbm_objective_kind */
+    uint32_t axis;        /* pressure drop: axis normal to the planes */
+    int32_t plane_a;      /* pressure drop: mean rho/3 on plane a minus on plane b */
+    int32_t plane_b;
+    uint64_t t_begin;     /* first state carrying the stage cost (>= 1) */
+    uint64_t t_end;       /* last state carrying the stage cost (<= n) */
+    double weight;        /* stage weight */
+    double terminal_weight;
+} lbm_objective;
+
+typedef struct {
+    uint64_t forward_steps;      /* step evaluations, recomputation included */
+    uint64_t reverse_steps;
+    uint64_t snapshot_slots;     /* slots the budget allowed */
+    uint64_t peak_snapshots;     /* BinomialSchedule::peak_snapshots */
+    uint64_t snapshot_copies;    /* snapshot + restore copies */
+    double forward_ms;           /* device time in forward-step kernels */
+    double adjoint_ms;           /* device time in adjoint-step kernels */
+    double total_ms;             /* wall time of the call */
+    double max_mach;             /* sqrt(3) max |u| over the first sweep (admit, solver.rs:742-751) */
+    double mass_drift;           /* max relative drift of fluid mass incl. port exchange */
+    uint64_t bad_cells;          /* cells with a non-finite or non-positive population */
+} lbm_stats;
+
+typedef struct lbm_handle lbm_handle;
+
+int lbm_abi_version(void);
+int lbm_last_error(char* buf, uint64_t len);
+
+int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_len, lbm_handle** out);
+int lbm_destroy(lbm_handle* h);
+
+/* alpha: one Brinkman coefficient per cell (solid cells ignored). */
+int lbm_set_design(lbm_handle* h, const double* alpha, uint64_t n);
+int lbm_reset_initial(lbm_handle* h);
+
+/* State exchange in the reference layout: layout 0 = AoS (x*Q+q), 1 = SoA (q*N+x). */
+int lbm_set_state(lbm_handle* h, int layout, const double* f, uint64_t len);
+int lbm_get_state(lbm_handle* h, int layout, double* f, uint64_t len);
+uint64_t lbm_step_count(const lbm_handle* h);
+
+int lbm_step(lbm_handle* h, uint64_t nsteps);
+int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, lbm_stats* stats);
+
+/* Forward + discrete adjoint from the current state (taken as state 0) over nsteps with
+ * binomial checkpointing inside the handle's snapshot budget.  alpha_bar (n cells,
+ * FP64) receives dJ/dalpha; lambda0 (optional, AoS N*Q) receives dJ/d state0.  The
+ * handle's state is left at state 0. */
+int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
+                 uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats);
+
+/* VJP of one step at the current state: lambda_prev = J^T lambda_next (AoS N*Q), and
+ * alpha_bar (N) = d(lambda_next . step)/d alpha. */
+int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, double* alpha_bar, uint64_t len);
+
+int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n);
+
+/* Source index of every (cell, q) exactly as Streaming::new builds it (NONE = 0xFFFFFFFF
+ * for solid cells), derived by running the device's own streaming addressing once on a
+ * state of unique identifiers. */
+int lbm_debug_link_table(lbm_handle* h, uint32_t* source, uint64_t len);
+
+/* Port cells (cell, neighbour) in application order. */
+int lbm_port_cells(const lbm_handle* h, int32_t* cells, int32_t* nbs, uint64_t cap, uint64_t* count);
+
+/* Binomial schedule as int64 quintuples (kind, a, b, 0, 0): kind 0 advance(from,to),
+ * 1 snapshot(step,slot), 2 restore(step,slot), 3 reverse(step), 4 release(slot). */
+int lbm_revolve_schedule(uint64_t steps, uint64_t snapshots, int64_t* actions, uint64_t cap, uint64_t* count,
+                         uint64_t* forward_steps, uint64_t* reverse_start);
+uint64_t lbm_revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots);
+
+/* Timing support: average device time of `launches` back-to-back launches of one kernel
+ * (0 forward step, 1 adjoint step) on the handle's stream, measured with HIP events. */
+int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
+/* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
+int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);
+/* Bytes of one population buffer on the device (storage padding included). */
+uint64_t lbm_state_bytes(const lbm_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMPLEXITY_LBM_H */

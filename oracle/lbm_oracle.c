@@ -1,0 +1,1246 @@
+/*
+ * lbm_oracle.c -- CPU FP64 restatement of the reference's differentiable LBM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (implexity_amd/, include/) links,
+ * loads or calls this file.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker / the timed CPU baseline.
+ *
+ * Every function restates one reference function operation for operation, in the
+ * reference's evaluation order, so that an FP64 build with -ffp-contract=off
+ * reproduces the reference's IEEE results (Rust never contracts a*b+c into an FMA).
+ * Reference paths are relative to crates/implexity-physics-lbm/src/ unless noted.
+ *
+ * Layout: AoS, index x*Q+q, cell x = (i*ny + j)*nz + k (z fastest), as the
+ * stationary solver (solver.rs:90,325; d3q19.rs:86-89).
+ *
+ * Parity status of each function (how it is pinned) is listed in the module
+ * docstring of oracle/oracle.py and checked by tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXQ 27
+#define NONE_U32 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------------- */
+/* Lattices: d3q19.rs:29-61 (D3Q19), moving/lattice.rs:93-129,149-156 (D3Q27), */
+/* opposite_of moving/lattice.rs:49-63.                                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int Q;
+    int c[MAXQ][3];
+    double cf[MAXQ][3];
+    double w[MAXQ];
+    int opp[MAXQ];
+} lattice_t;
+
+static lattice_t L19, L27;
+static int lattices_ready = 0;
+
+static const int C19[19][3] = {
+    {0, 0, 0},  {-1, -1, 0}, {-1, 0, -1}, {-1, 0, 0}, {-1, 0, 1}, {-1, 1, 0}, {0, -1, -1},
+    {0, -1, 0}, {0, -1, 1},  {0, 0, -1},  {0, 0, 1},  {0, 1, -1}, {0, 1, 0},  {0, 1, 1},
+    {1, -1, 0}, {1, 0, -1},  {1, 0, 0},   {1, 0, 1},  {1, 1, 0}};
+
+static void init_lattices(void) {
+    if (lattices_ready) return;
+    /* D3Q19: W0=1/3, W1=1/18 (axis), W2=1/36 (edge); OPPOSITE[i] = 19-i for i >= 1. */
+    L19.Q = 19;
+    for (int q = 0; q < 19; q++) {
+        int n = 0;
+        for (int d = 0; d < 3; d++) {
+            L19.c[q][d] = C19[q][d];
+            L19.cf[q][d] = (double)C19[q][d];
+            n += abs(C19[q][d]);
+        }
+        L19.w[q] = n == 0 ? 1.0 / 3.0 : (n == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+        L19.opp[q] = q == 0 ? 0 : 19 - q;
+    }
+    /* D3Q27: rest first, then (a,b,c) in {-1,0,1}^3 lexicographic without origin. */
+    L27.Q = 27;
+    L27.c[0][0] = L27.c[0][1] = L27.c[0][2] = 0;
+    int n = 1;
+    for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+            for (int c = -1; c <= 1; c++) {
+                if (a == 0 && b == 0 && c == 0) continue;
+                L27.c[n][0] = a;
+                L27.c[n][1] = b;
+                L27.c[n][2] = c;
+                n++;
+            }
+    for (int q = 0; q < 27; q++) {
+        int m = abs(L27.c[q][0]) + abs(L27.c[q][1]) + abs(L27.c[q][2]);
+        L27.w[q] = m == 0 ? 8.0 / 27.0 : (m == 1 ? 2.0 / 27.0 : (m == 2 ? 1.0 / 54.0 : 1.0 / 216.0));
+        for (int d = 0; d < 3; d++) L27.cf[q][d] = (double)L27.c[q][d];
+    }
+    for (int i = 0; i < 27; i++)
+        for (int j = 0; j < 27; j++)
+            if (L27.c[j][0] == -L27.c[i][0] && L27.c[j][1] == -L27.c[i][1] && L27.c[j][2] == -L27.c[i][2])
+                L27.opp[i] = j;
+    lattices_ready = 1;
+}
+
+static const lattice_t *lat(int Q) {
+    init_lattices();
+    return Q == 27 ? &L27 : &L19;
+}
+
+int orc_lattice(int Q, int *c_out, double *w_out, int *opp_out) {
+    const lattice_t *L = lat(Q);
+    if (Q != 19 && Q != 27) return -1;
+    for (int q = 0; q < Q; q++) {
+        for (int d = 0; d < 3; d++) c_out[3 * q + d] = L->c[q][d];
+        w_out[q] = L->w[q];
+        opp_out[q] = L->opp[q];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Problem description (plain data filled by oracle/oracle.py).               */
+/* ------------------------------------------------------------------------- */
+enum { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4 };
+
+typedef struct {
+    int32_t Q, kind;
+    int32_t nx, ny, nz;
+    int32_t periodic[3];
+    double omega;     /* omega+ = 1/tau+ */
+    double magic;     /* TRT magic / MRT odd_magic */
+    double bulk_rate; /* MRT bulk rate (NaN = none) / cumulant bulk rate */
+    double even_rate; /* MRT even rate (NaN = none) */
+    double accel[3];  /* lattice body acceleration */
+    int32_t n_port_cells;
+    int32_t pad_;
+    const int32_t *port_cell;
+    const int32_t *port_nb;
+    const int32_t *port_kind;   /* 0 velocity, 1 pressure */
+    const double *port_value;   /* 3 per port cell: u_b, or rho_b in [0] */
+    const uint8_t *solid;       /* N entries, may be NULL (all fluid) */
+    const double *mrt_rows;     /* Q*Q, row-major rows[k][i] (from orc_mrt_basis) */
+    const double *mrt_norms;    /* Q */
+    const int32_t *mrt_groups;  /* Q */
+} orc_problem;
+
+static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
+
+/* ------------------------------------------------------------------------- */
+/* Grid: d3q19.rs:70-128 (index, coords, wrap with rem_euclid, inside).        */
+/* ------------------------------------------------------------------------- */
+static inline long g_index(const orc_problem *p, long i, long j, long k) { return (i * p->ny + j) * p->nz + k; }
+
+static inline void g_coords(const orc_problem *p, long x, long out[3]) {
+    long k = x % p->nz;
+    long ij = x / p->nz;
+    out[0] = ij / p->ny;
+    out[1] = ij % p->ny;
+    out[2] = k;
+}
+
+static inline long rem_euclid(long v, long n) {
+    long r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+static long g_wrap(const orc_problem *p, long x, const long off[3]) {
+    long c[3], n[3] = {p->nx, p->ny, p->nz};
+    g_coords(p, x, c);
+    for (int a = 0; a < 3; a++) c[a] = rem_euclid(c[a] + off[a], n[a]);
+    return g_index(p, c[0], c[1], c[2]);
+}
+
+static int g_inside(const orc_problem *p, long x, const long off[3]) {
+    long c[3], n[3] = {p->nx, p->ny, p->nz};
+    g_coords(p, x, c);
+    for (int a = 0; a < 3; a++) {
+        long v = c[a] + off[a];
+        if (!(p->periodic[a] || (v >= 0 && v < n[a]))) return 0;
+    }
+    return 1;
+}
+
+/* Streaming::new (solver.rs:83-120): halfway bounce-back link table.
+ * blocked[x,q] = solid[wrap(x-c_q)] || !inside(x-c_q); source = x*Q+opp(q) if blocked,
+ * else wrap(x-c_q)*Q+q.  Solid cells have no source (NONE): they keep their previous
+ * state (solver.rs:131-136). */
+void orc_stream_table(const orc_problem *p, uint32_t *source) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    for (long x = 0; x < n; x++) {
+        for (int q = 0; q < Q; q++) {
+            if (is_solid(p, x)) {
+                source[x * Q + q] = NONE_U32;
+                continue;
+            }
+            long back[3] = {-L->c[q][0], -L->c[q][1], -L->c[q][2]};
+            long y = g_wrap(p, x, back);
+            int blocked = is_solid(p, y) || !g_inside(p, x, back);
+            source[x * Q + q] = blocked ? (uint32_t)(x * Q + L->opp[q]) : (uint32_t)(y * Q + q);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Moving-lattice helpers: moving/lattice.rs:160-222.                          */
+/* ------------------------------------------------------------------------- */
+static inline double dot_c(const lattice_t *L, int i, const double v[3]) {
+    /* v0*c0 + v1*c1 + v2*c2, left to right (lattice.rs:160-165). */
+    return v[0] * L->cf[i][0] + v[1] * L->cf[i][1] + v[2] * L->cf[i][2];
+}
+
+static void moments(const lattice_t *L, const double *f, double *rho, double j[3]) {
+    /* lattice.rs:168-180: rho = f0 + f1 + ...; j[d] += f_i*c_id for i >= 1. */
+    double r = f[0];
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int i = 1; i < L->Q; i++) {
+        r += f[i];
+        for (int d = 0; d < 3; d++) m[d] += f[i] * L->cf[i][d];
+    }
+    *rho = r;
+    j[0] = m[0];
+    j[1] = m[1];
+    j[2] = m[2];
+}
+
+static void equilibrium(const lattice_t *L, double rho, const double u[3], double *eq) {
+    /* lattice.rs:183-191 */
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double base = 1.0 - usq * 1.5;
+    for (int i = 0; i < L->Q; i++) {
+        double cu = dot_c(L, i, u);
+        eq[i] = rho * L->w[i] * (base + cu * 3.0 + cu * cu * 4.5);
+    }
+}
+
+static void equilibrium_vjp(const lattice_t *L, double rho, const double u[3], const double *g, double *rho_bar,
+                            double u_bar[3]) {
+    /* lattice.rs:193-222 */
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double base = 1.0 - usq * 1.5;
+    double usq_bar = 0.0;
+    for (int i = 0; i < L->Q; i++) {
+        double cu = dot_c(L, i, u);
+        double poly = base + cu * 3.0 + cu * cu * 4.5;
+        double gw = g[i] * L->w[i];
+        *rho_bar += gw * poly;
+        double e = gw * rho;
+        double cu_bar = e * (cu * 9.0 + 3.0);
+        for (int d = 0; d < 3; d++)
+            if (L->cf[i][d] != 0.0) u_bar[d] += cu_bar * L->cf[i][d];
+        usq_bar -= e * 1.5;
+    }
+    for (int d = 0; d < 3; d++) u_bar[d] += usq_bar * u[d] * 2.0;
+}
+
+static void source_term(const lattice_t *L, const double u[3], const double F[3], double *src) {
+    /* collision.rs:316-323 */
+    double uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+    for (int i = 0; i < L->Q; i++) {
+        double cu = dot_c(L, i, u);
+        double cf = dot_c(L, i, F);
+        src[i] = (cf * 3.0 + cu * cf * 9.0 - uf * 3.0) * L->w[i];
+    }
+}
+
+/* odd_rate / odd_rate_derivative: collision.rs:302-313. */
+double orc_odd_rate(double omega, double magic) {
+    double tp = 1.0 / omega - 0.5;
+    return 1.0 / (0.5 + magic / tp);
+}
+
+double orc_odd_rate_derivative(double omega, double magic) {
+    double tau = 1.0 / omega;
+    double tp = tau - 0.5;
+    double tm = 0.5 + magic / tp;
+    return -(tau * tau * magic) / (tm * tm * tp * tp);
+}
+
+/* collide_pairs: collision.rs:457-488 (TRT; BGK when om == omega). */
+static void collide_pairs(const lattice_t *L, const double *f, double rho, const double u[3], const double F[3],
+                          double omega, double om, double *out) {
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+    double base = 1.0 - usq * 1.5;
+    double kp = 1.0 - omega * 0.5;
+    double km = 1.0 - om * 0.5;
+    double w0 = L->w[0];
+    double eq0 = rho * w0 * base;
+    out[0] = f[0] - omega * (f[0] - eq0) - kp * uf * (3.0 * w0);
+    for (int i = 1; i < L->Q; i++) {
+        int o = L->opp[i];
+        if (o < i) continue;
+        double w = L->w[i];
+        double cu = dot_c(L, i, u);
+        double cf = dot_c(L, i, F);
+        double wr = rho * w;
+        double eqp = wr * (base + cu * cu * 4.5);
+        double eqm = wr * cu * 3.0;
+        double sp = (cu * cf * 9.0 - uf * 3.0) * w;
+        double sm = cf * (3.0 * w);
+        double fp = (f[i] + f[o]) * 0.5;
+        double fm = (f[i] - f[o]) * 0.5;
+        double a = kp * sp - omega * (fp - eqp);
+        double b = km * sm - om * (fm - eqm);
+        out[i] = f[i] + a + b;
+        out[o] = f[o] + a - b;
+    }
+}
+
+/* collide_pairs_vjp: collision.rs:492-570.  Returns f_bar, rho_bar, u_bar, force_bar. */
+static void collide_pairs_vjp(const lattice_t *L, const double *f, double rho, const double u[3], const double F[3],
+                              double omega, double om, const double *g, double *f_bar, double *rho_bar_out,
+                              double u_bar_out[3], double force_bar_out[3]) {
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double base = 1.0 - usq * 1.5;
+    double kp = 1.0 - omega * 0.5;
+    double km = 1.0 - om * 0.5;
+    double rho_bar = 0.0, usq_bar = 0.0, uf_bar = 0.0;
+    double u_bar[3] = {0, 0, 0}, force_bar[3] = {0, 0, 0};
+    double w0 = L->w[0];
+    double g0 = g[0];
+    for (int i = 0; i < L->Q; i++) f_bar[i] = 0.0;
+    f_bar[0] = g0 * (1.0 - omega);
+    rho_bar += g0 * omega * w0 * base;
+    usq_bar -= g0 * omega * rho * (1.5 * w0);
+    uf_bar -= g0 * kp * (3.0 * w0);
+    for (int i = 1; i < L->Q; i++) {
+        int o = L->opp[i];
+        if (o < i) continue;
+        double w = L->w[i];
+        double cu = dot_c(L, i, u);
+        double cf = dot_c(L, i, F);
+        double wr = rho * w;
+        double a_bar = g[i] + g[o];
+        double b_bar = g[i] - g[o];
+        double fp_bar = -omega * a_bar;
+        double fm_bar = -om * b_bar;
+        double eqp_bar = omega * a_bar;
+        double eqm_bar = om * b_bar;
+        double sp_bar = kp * a_bar;
+        double sm_bar = km * b_bar;
+        f_bar[i] = g[i] + (fp_bar + fm_bar) * 0.5;
+        f_bar[o] = g[o] + (fp_bar - fm_bar) * 0.5;
+        rho_bar += eqp_bar * w * (base + cu * cu * 4.5) + eqm_bar * cu * (3.0 * w);
+        usq_bar -= eqp_bar * wr * 1.5;
+        double cu_bar = eqp_bar * wr * cu * 9.0 + eqm_bar * wr * 3.0 + sp_bar * cf * (9.0 * w);
+        double cf_bar = sp_bar * cu * (9.0 * w) + sm_bar * (3.0 * w);
+        uf_bar -= sp_bar * (3.0 * w);
+        for (int d = 0; d < 3; d++) {
+            if (L->cf[i][d] != 0.0) {
+                u_bar[d] += cu_bar * L->cf[i][d];
+                force_bar[d] += cf_bar * L->cf[i][d];
+            }
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        u_bar[d] += usq_bar * u[d] * 2.0 + uf_bar * F[d];
+        force_bar[d] += uf_bar * u[d];
+    }
+    *rho_bar_out = rho_bar;
+    for (int d = 0; d < 3; d++) {
+        u_bar_out[d] = u_bar[d];
+        force_bar_out[d] = force_bar[d];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* MRT: MomentBasis::new (collision.rs:87-153), forward/inverse/transpose       */
+/* (:156-197), relax MRT arm (:349-375), relax_transpose MRT arm (:403-418).    */
+/* Groups: 0 Mass, 1 Momentum, 2 Bulk, 3 Shear, 4 Odd, 5 EvenHigh.              */
+/* ------------------------------------------------------------------------- */
+static double ipow(double v, int e) {
+    /* f64::powi with small non-negative integer exponents: repeated products. */
+    double r = 1.0;
+    for (int k = 0; k < e; k++) r *= v;
+    return r;
+}
+
+int orc_mrt_basis(int Q, double *rows_out, double *norms_out, int32_t *groups_out) {
+    const lattice_t *L = lat(Q);
+    double cand[64][MAXQ];
+    int cgroup[64];
+    int nc = 0;
+    for (int i = 0; i < Q; i++) cand[nc][i] = 1.0;
+    cgroup[nc++] = 0;
+    for (int d = 0; d < 3; d++) {
+        for (int i = 0; i < Q; i++) cand[nc][i] = L->cf[i][d];
+        cgroup[nc++] = 1;
+    }
+    for (int i = 0; i < Q; i++) {
+        const double *v = L->cf[i];
+        cand[nc][i] = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    cgroup[nc++] = 2;
+    for (int i = 0; i < Q; i++) {
+        const double *v = L->cf[i];
+        cand[nc][i] = 2.0 * v[0] * v[0] - v[1] * v[1] - v[2] * v[2];
+    }
+    cgroup[nc++] = 3;
+    for (int i = 0; i < Q; i++) {
+        const double *v = L->cf[i];
+        cand[nc][i] = v[1] * v[1] - v[2] * v[2];
+    }
+    cgroup[nc++] = 3;
+    for (int i = 0; i < Q; i++) cand[nc][i] = L->cf[i][0] * L->cf[i][1];
+    cgroup[nc++] = 3;
+    for (int i = 0; i < Q; i++) cand[nc][i] = L->cf[i][0] * L->cf[i][2];
+    cgroup[nc++] = 3;
+    for (int i = 0; i < Q; i++) cand[nc][i] = L->cf[i][1] * L->cf[i][2];
+    cgroup[nc++] = 3;
+    for (int degree = 3; degree <= 6; degree++)
+        for (int a = 0; a <= 2; a++)
+            for (int b = 0; b <= 2; b++) {
+                int cc = degree - a - b;
+                if (cc < 0 || cc > 2) continue;
+                for (int i = 0; i < Q; i++)
+                    cand[nc][i] = ipow(L->cf[i][0], a) * ipow(L->cf[i][1], b) * ipow(L->cf[i][2], cc);
+                cgroup[nc++] = (degree % 2 == 1) ? 4 : 5;
+            }
+    int nrows = 0;
+    double rows[MAXQ][MAXQ];
+    double norms[MAXQ];
+    for (int ci = 0; ci < nc; ci++) {
+        if (nrows == Q) break;
+        const double *pv = cand[ci];
+        double scale = 0.0;
+        for (int i = 0; i < Q; i++) scale += L->w[i] * pv[i] * pv[i];
+        if (scale <= 0.0) continue;
+        double v[MAXQ];
+        for (int i = 0; i < Q; i++) v[i] = pv[i];
+        for (int k = 0; k < nrows; k++) {
+            double ip = 0.0;
+            for (int i = 0; i < Q; i++) ip += L->w[i] * pv[i] * rows[k][i];
+            double coef = ip / norms[k];
+            for (int i = 0; i < Q; i++) v[i] -= coef * rows[k][i];
+        }
+        double nn = 0.0;
+        for (int i = 0; i < Q; i++) nn += L->w[i] * v[i] * v[i];
+        if (nn <= 1e-10 * scale) continue;
+        norms[nrows] = nn;
+        groups_out[nrows] = cgroup[ci];
+        for (int i = 0; i < Q; i++) rows[nrows][i] = v[i];
+        nrows++;
+    }
+    if (nrows != Q) return -1;
+    for (int k = 0; k < Q; k++) {
+        norms_out[k] = norms[k];
+        for (int i = 0; i < Q; i++) rows_out[k * Q + i] = rows[k][i];
+    }
+    return 0;
+}
+
+static void mrt_rates(const orc_problem *p, double *rate) {
+    double om = orc_odd_rate(p->omega, p->magic);
+    for (int k = 0; k < p->Q; k++) {
+        switch (p->mrt_groups[k]) {
+        case 0:
+        case 3: rate[k] = p->omega; break;
+        case 1:
+        case 4: rate[k] = om; break;
+        case 2: rate[k] = isnan(p->bulk_rate) ? p->omega : p->bulk_rate; break;
+        default: rate[k] = isnan(p->even_rate) ? p->omega : p->even_rate; break;
+        }
+    }
+}
+
+static void mrt_forward(const orc_problem *p, const double *v, double *m) {
+    int Q = p->Q;
+    for (int k = 0; k < Q; k++) {
+        double acc = 0.0;
+        for (int i = 0; i < Q; i++) {
+            double r = p->mrt_rows[k * Q + i];
+            if (r != 0.0) acc += v[i] * r;
+        }
+        m[k] = acc;
+    }
+}
+
+static void mrt_inverse(const orc_problem *p, const lattice_t *L, const double *m, double *out) {
+    int Q = p->Q;
+    double scaled[MAXQ];
+    for (int k = 0; k < Q; k++) scaled[k] = m[k] / p->mrt_norms[k];
+    for (int i = 0; i < Q; i++) {
+        double acc = 0.0;
+        for (int k = 0; k < Q; k++) {
+            double r = p->mrt_rows[k * Q + i];
+            if (r != 0.0) acc += scaled[k] * r;
+        }
+        out[i] = acc * L->w[i];
+    }
+}
+
+static void mrt_transpose(const orc_problem *p, const double *m, double *out) {
+    int Q = p->Q;
+    for (int i = 0; i < Q; i++) {
+        double acc = 0.0;
+        for (int k = 0; k < Q; k++) {
+            double r = p->mrt_rows[k * Q + i];
+            if (r != 0.0) acc += m[k] * r;
+        }
+        out[i] = acc;
+    }
+}
+
+/* Generic (MRT) collide: collision.rs:449-453 with relax MRT arm :349-375. */
+static void collide_mrt(const orc_problem *p, const lattice_t *L, const double *f, double rho, const double u[3],
+                        const double F[3], double *out) {
+    int Q = p->Q;
+    double eq[MAXQ], src[MAXQ], y[MAXQ], m[MAXQ], rm[MAXQ], r[MAXQ], rate[MAXQ];
+    equilibrium(L, rho, u, eq);
+    source_term(L, u, F, src);
+    for (int i = 0; i < Q; i++) y[i] = f[i] - eq[i] + src[i] * 0.5;
+    mrt_forward(p, y, m);
+    mrt_rates(p, rate);
+    for (int k = 0; k < Q; k++) rm[k] = m[k] * rate[k];
+    mrt_inverse(p, L, rm, r);
+    for (int i = 0; i < Q; i++) out[i] = f[i] - r[i] + src[i];
+}
+
+/* Generic collide_vjp (MRT arm): collision.rs:597-633, relax_transpose :403-418. */
+static void collide_mrt_vjp(const orc_problem *p, const lattice_t *L, const double *f, double rho, const double u[3],
+                            const double F[3], const double *g, double *f_bar, double *rho_bar_out,
+                            double u_bar_out[3], double force_bar_out[3]) {
+    (void)f;
+    int Q = p->Q;
+    double wg[MAXQ], m[MAXQ], scaled[MAXQ], rt[MAXQ], rate[MAXQ];
+    for (int i = 0; i < Q; i++) wg[i] = g[i] * L->w[i];
+    mrt_forward(p, wg, m);
+    mrt_rates(p, rate);
+    for (int k = 0; k < Q; k++) scaled[k] = m[k] * rate[k] / p->mrt_norms[k];
+    mrt_transpose(p, scaled, rt);
+    double src_bar[MAXQ];
+    for (int i = 0; i < Q; i++) {
+        f_bar[i] = g[i] - rt[i];
+        src_bar[i] = g[i] - rt[i] * 0.5;
+    }
+    double rho_bar = 0.0, u_bar[3] = {0, 0, 0};
+    equilibrium_vjp(L, rho, u, rt, &rho_bar, u_bar);
+    double force_bar[3] = {0, 0, 0}, uf_bar = 0.0;
+    for (int i = 0; i < Q; i++) {
+        double h = src_bar[i] * L->w[i];
+        double cu = dot_c(L, i, u);
+        double cf = dot_c(L, i, F);
+        double cf_bar = h * (cu * 9.0 + 3.0);
+        double cu_bar = h * cf * 9.0;
+        uf_bar -= h * 3.0;
+        for (int d = 0; d < 3; d++) {
+            if (L->cf[i][d] != 0.0) {
+                force_bar[d] += cf_bar * L->cf[i][d];
+                u_bar[d] += cu_bar * L->cf[i][d];
+            }
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        u_bar[d] += uf_bar * F[d];
+        force_bar[d] += uf_bar * u[d];
+    }
+    *rho_bar_out = rho_bar;
+    for (int d = 0; d < 3; d++) {
+        u_bar_out[d] = u_bar[d];
+        force_bar_out[d] = force_bar[d];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Cumulant (D3Q27): collision.rs:636-752, with forward-mode tangents for the  */
+/* VJP as collision.rs:755-788 does (one tangent direction per pass).          */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    double v, d;
+} dual;
+
+static inline dual dv(double v) { dual r = {v, 0.0}; return r; }
+static inline dual dadd(dual a, dual b) { dual r = {a.v + b.v, a.d + b.d}; return r; }
+static inline dual dsub(dual a, dual b) { dual r = {a.v - b.v, a.d - b.d}; return r; }
+static inline dual dmul(dual a, dual b) { dual r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static inline dual dmuls(dual a, double s) { dual r = {a.v * s, a.d * s}; return r; }
+static inline dual ddiv(dual a, dual b) {
+    /* implexity-ad dual.rs:136-146: eps = a'/b - b'*q*inv with inv = 1/b, q = a*inv. */
+    double inv = 1.0 / b.v;
+    double q = a.v * inv;
+    dual r = {a.v / b.v, a.d / b.v - b.d * q * inv};
+    return r;
+}
+static inline dual ddivs(dual a, double s) { dual r = {a.v / s, a.d / s}; return r; }
+
+/* Wick pairings per cube index (collision.rs:212-239,276-285). */
+static int wick_n[27];
+static int wick_pairs[27][16][3][2];
+static int wick_ready = 0;
+
+static int matchings_rec(const int *items, int n, int out[][3][2], int depth_base, int cap) {
+    /* Returns matchings of items[0..n) in the order of collision.rs:216-239. */
+    if (n == 0) return 1; /* one empty matching; caller handles */
+    int count = 0;
+    int first = items[0];
+    for (int j = 1; j < n; j++) {
+        int rest[8], nr = 0;
+        for (int t = 1; t < n; t++)
+            if (t != j) rest[nr++] = items[t];
+        int partner = items[j];
+        int lo = first < partner ? first : partner, hi = first < partner ? partner : first;
+        int sub[16][3][2];
+        int ns;
+        if (nr == 0) {
+            ns = 1;
+        } else {
+            ns = matchings_rec(rest, nr, sub, depth_base + 1, 16);
+        }
+        for (int s = 0; s < ns; s++) {
+            if (count >= cap) return count;
+            out[count][0][0] = lo;
+            out[count][0][1] = hi;
+            for (int pidx = 0; pidx < nr / 2; pidx++) {
+                out[count][pidx + 1][0] = sub[s][pidx][0];
+                out[count][pidx + 1][1] = sub[s][pidx][1];
+            }
+            count++;
+        }
+    }
+    return count;
+}
+
+static void init_wick(void) {
+    if (wick_ready) return;
+    for (int k = 0; k < 27; k++) {
+        int e[3] = {k / 9, (k / 3) % 3, k % 3};
+        int axes[8], na = 0;
+        for (int d = 0; d < 3; d++)
+            for (int t = 0; t < e[d]; t++) axes[na++] = d;
+        if (na % 2 == 1 || na < 4) {
+            wick_n[k] = 0;
+        } else {
+            wick_n[k] = matchings_rec(axes, na, wick_pairs[k], 0, 16);
+        }
+    }
+    wick_ready = 1;
+}
+
+int orc_wick_table(int32_t *counts, int32_t *pairs) {
+    /* counts[27]; pairs[27][16][3][2] (unused entries untouched). */
+    init_wick();
+    for (int k = 0; k < 27; k++) {
+        counts[k] = wick_n[k];
+        for (int m = 0; m < wick_n[k]; m++) {
+            int np = 0;
+            int e[3] = {k / 9, (k / 3) % 3, k % 3};
+            np = (e[0] + e[1] + e[2]) / 2;
+            for (int t = 0; t < np; t++) {
+                pairs[((k * 16 + m) * 3 + t) * 2 + 0] = wick_pairs[k][m][t][0];
+                pairs[((k * 16 + m) * 3 + t) * 2 + 1] = wick_pairs[k][m][t][1];
+            }
+        }
+    }
+    return 0;
+}
+
+static void cumulant_dual(const dual *f, dual rho, const dual u[3], const dual F[3], dual omega, double bulk,
+                          dual *out) {
+    const lattice_t *L = lat(27);
+    init_wick();
+    dual cube[27];
+    for (int i = 0; i < 27; i++) {
+        int k = (L->c[i][0] + 1) * 9 + (L->c[i][1] + 1) * 3 + (L->c[i][2] + 1);
+        cube[k] = f[i];
+    }
+    const int stride[3] = {9, 3, 1};
+    const int fwd_axes[3] = {2, 1, 0};
+    for (int ai = 0; ai < 3; ai++) {
+        int axis = fwd_axes[ai];
+        int s = stride[axis];
+        for (int base = 0; base < 27; base++) {
+            if ((base / s) % 3 != 0) continue;
+            dual fm = cube[base], f0 = cube[base + s], fp = cube[base + 2 * s];
+            dual k0 = dadd(dadd(fm, f0), fp);
+            dual diff = dsub(fp, fm);
+            dual ua = u[axis];
+            dual k1 = dsub(diff, dmul(ua, k0));
+            dual k2 = dadd(dsub(dadd(fp, fm), dmuls(dmul(ua, diff), 2.0)), dmul(dmul(ua, ua), k0));
+            cube[base] = k0;
+            cube[base + s] = k1;
+            cube[base + 2 * s] = k2;
+        }
+    }
+#define IDX(a, b, c) ((a) * 9 + (b) * 3 + (c))
+    dual kxx = cube[IDX(2, 0, 0)], kyy = cube[IDX(0, 2, 0)], kzz = cube[IDX(0, 0, 2)];
+    dual one_minus = dsub(dv(1.0), omega);
+    dual dxy = dmul(dsub(kxx, kyy), one_minus);
+    dual dxz = dmul(dsub(kxx, kzz), one_minus);
+    dual trace = dadd(dadd(kxx, kyy), kzz);
+    trace = dadd(trace, dmuls(dsub(rho, trace), bulk));
+    dual post_xx = ddivs(dadd(dadd(trace, dxy), dxz), 3.0);
+    dual post_yy = ddivs(dadd(dsub(trace, dmuls(dxy, 2.0)), dxz), 3.0);
+    dual post_zz = ddivs(dsub(dadd(trace, dxy), dmuls(dxz, 2.0)), 3.0);
+    dual post_xy = dmul(cube[IDX(1, 1, 0)], one_minus);
+    dual post_xz = dmul(cube[IDX(1, 0, 1)], one_minus);
+    dual post_yz = dmul(cube[IDX(0, 1, 1)], one_minus);
+    dual sigma[3][3] = {{ddiv(post_xx, rho), ddiv(post_xy, rho), ddiv(post_xz, rho)},
+                        {ddiv(post_xy, rho), ddiv(post_yy, rho), ddiv(post_yz, rho)},
+                        {ddiv(post_xz, rho), ddiv(post_yz, rho), ddiv(post_zz, rho)}};
+    dual post[27];
+    for (int k = 0; k < 27; k++) {
+        int e[3] = {k / 9, (k / 3) % 3, k % 3};
+        int order = e[0] + e[1] + e[2];
+        if (order == 0) {
+            post[k] = rho;
+        } else if (order == 1) {
+            int axis = e[0] == 1 ? 0 : (e[1] == 1 ? 1 : 2);
+            post[k] = dmuls(F[axis], 0.5);
+        } else if (order == 2) {
+            if (e[0] == 2) post[k] = post_xx;
+            else if (e[1] == 2) post[k] = post_yy;
+            else if (e[2] == 2) post[k] = post_zz;
+            else if (e[0] == 1 && e[1] == 1) post[k] = post_xy;
+            else if (e[0] == 1 && e[2] == 1) post[k] = post_xz;
+            else post[k] = post_yz;
+        } else {
+            dual acc = dv(0.0);
+            int np = order / 2;
+            for (int m = 0; m < wick_n[k]; m++) {
+                dual pr = dv(1.0);
+                for (int t = 0; t < np; t++) pr = dmul(pr, sigma[wick_pairs[k][m][t][0]][wick_pairs[k][m][t][1]]);
+                acc = dadd(acc, pr);
+            }
+            post[k] = dmul(acc, rho);
+        }
+    }
+#undef IDX
+    for (int axis = 0; axis < 3; axis++) {
+        int s = stride[axis];
+        for (int base = 0; base < 27; base++) {
+            if ((base / s) % 3 != 0) continue;
+            dual k0 = post[base], k1 = post[base + s], k2 = post[base + 2 * s];
+            dual ua = u[axis];
+            dual m1 = dadd(k1, dmul(ua, k0));
+            dual m2 = dadd(dadd(k2, dmuls(dmul(ua, k1), 2.0)), dmul(dmul(ua, ua), k0));
+            post[base] = dmuls(dsub(m2, m1), 0.5);
+            post[base + s] = dsub(k0, m2);
+            post[base + 2 * s] = dmuls(dadd(m2, m1), 0.5);
+        }
+    }
+    for (int i = 0; i < 27; i++) {
+        int k = (L->c[i][0] + 1) * 9 + (L->c[i][1] + 1) * 3 + (L->c[i][2] + 1);
+        out[i] = post[k];
+    }
+}
+
+static void collide_cumulant(const double *f, double rho, const double u[3], const double F[3], double omega,
+                             double bulk, double *out) {
+    dual fd[27], ud[3], Fd[3], od[27];
+    for (int i = 0; i < 27; i++) fd[i] = dv(f[i]);
+    for (int d = 0; d < 3; d++) {
+        ud[d] = dv(u[d]);
+        Fd[d] = dv(F[d]);
+    }
+    cumulant_dual(fd, dv(rho), ud, Fd, dv(omega), bulk, od);
+    for (int i = 0; i < 27; i++) out[i] = od[i].v;
+}
+
+/* cumulant_vjp (collision.rs:755-788): tangent k of the 34 inputs (27 f, rho, 3 u,
+ * 3 F) contracted with g.  The omega tangent (index 34) is not needed here. */
+static void collide_cumulant_vjp(const double *f, double rho, const double u[3], const double F[3], double omega,
+                                 double bulk, const double *g, double *f_bar, double *rho_bar, double u_bar[3],
+                                 double force_bar[3]) {
+    double bar[34];
+    for (int t = 0; t < 34; t++) {
+        dual fd[27], ud[3], Fd[3], od[27];
+        for (int i = 0; i < 27; i++) fd[i] = (dual){f[i], t == i ? 1.0 : 0.0};
+        dual rd = {rho, t == 27 ? 1.0 : 0.0};
+        for (int d = 0; d < 3; d++) {
+            ud[d] = (dual){u[d], t == 28 + d ? 1.0 : 0.0};
+            Fd[d] = (dual){F[d], t == 31 + d ? 1.0 : 0.0};
+        }
+        cumulant_dual(fd, rd, ud, Fd, dv(omega), bulk, od);
+        double acc = 0.0;
+        for (int i = 0; i < 27; i++)
+            if (g[i] != 0.0) acc += g[i] * od[i].d;
+        bar[t] = acc;
+    }
+    for (int i = 0; i < 27; i++) f_bar[i] = bar[i];
+    *rho_bar = bar[27];
+    for (int d = 0; d < 3; d++) {
+        u_bar[d] = bar[28 + d];
+        force_bar[d] = bar[31 + d];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stationary BGK-Guo (solver.rs:189-196,214-285,298-322).                      */
+/* ------------------------------------------------------------------------- */
+static void collide_bgk_guo(const double *f, double alpha, const double a[3], double tau, double *out) {
+    const lattice_t *L = lat(19);
+    double rho = f[0];
+    for (int q = 1; q < 19; q++) rho += f[q];
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    for (int q = 0; q < 19; q++) {
+        mx += f[q] * L->cf[q][0];
+        my += f[q] * L->cf[q][1];
+        mz += f[q] * L->cf[q][2];
+    }
+    double s = alpha * 0.5 + 1.0;
+    double u[3] = {(mx / rho + 0.5 * a[0]) / s, (my / rho + 0.5 * a[1]) / s, (mz / rho + 0.5 * a[2]) / s};
+    double force[3] = {rho * (a[0] - alpha * u[0]), rho * (a[1] - alpha * u[1]), rho * (a[2] - alpha * u[2])};
+    double uf = u[0] * force[0] + u[1] * force[1] + u[2] * force[2];
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double k = 1.0 - 0.5 / tau;
+    for (int q = 0; q < 19; q++) {
+        double cu = 0.0 + u[0] * L->cf[q][0] + u[1] * L->cf[q][1] + u[2] * L->cf[q][2];
+        double cf = 0.0 + force[0] * L->cf[q][0] + force[1] * L->cf[q][1] + force[2] * L->cf[q][2];
+        double eq = rho * L->w[q] * (1.0 + cu * 3.0 + cu * cu * 4.5 - usq * 1.5);
+        double source = k * L->w[q] * (cf * 3.0 - uf * 3.0 + cu * cf * 9.0);
+        out[q] = f[q] - (f[q] - eq) / tau + source;
+    }
+}
+
+/* ports.rs dot_c / moments (used by collide_cell_vjp via macroscopic). */
+static double st_dot_c(int q, const double u[3]) {
+    double acc = 0.0;
+    int first = 1;
+    for (int d = 0; d < 3; d++) {
+        double term;
+        if (C19[q][d] == 0) continue;
+        term = C19[q][d] == 1 ? u[d] : -u[d];
+        acc = first ? term : acc + term;
+        first = 0;
+    }
+    return first ? 0.0 : acc;
+}
+
+static void st_moments(const double *f, double *rho, double m[3]) {
+    double r = f[0];
+    for (int q = 1; q < 19; q++) r += f[q];
+    for (int d = 0; d < 3; d++) {
+        double acc = 0.0;
+        for (int q = 0; q < 19; q++) {
+            if (C19[q][d] == 1) acc += f[q];
+            else if (C19[q][d] == -1) acc -= f[q];
+        }
+        m[d] = acc;
+    }
+    *rho = r;
+}
+
+/* collide_cell_vjp: solver.rs:214-285 (returns f_bar, alpha_bar; tau_bar dropped). */
+static void collide_bgk_guo_vjp(const double *f, double alpha, const double a[3], double tau, const double *g,
+                                double *f_bar, double *alpha_bar_out) {
+    const lattice_t *L = lat(19);
+    double rho, m[3];
+    st_moments(f, &rho, m);
+    double s = 1.0 + 0.5 * alpha;
+    double u[3], force[3];
+    for (int d = 0; d < 3; d++) u[d] = (m[d] / rho + 0.5 * a[d]) / s;
+    for (int d = 0; d < 3; d++) force[d] = rho * (a[d] - alpha * u[d]);
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double k = 1.0 - 0.5 / tau;
+    double rho_bar = 0.0, uf_bar = 0.0, usq_bar = 0.0;
+    double u_bar[3] = {0, 0, 0}, force_bar[3] = {0, 0, 0};
+    for (int q = 0; q < 19; q++) f_bar[q] = 0.0;
+    for (int q = 0; q < 19; q++) {
+        double gq = g[q];
+        if (gq == 0.0) continue;
+        double cu = st_dot_c(q, u);
+        double cf = st_dot_c(q, force);
+        double poly = 1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq;
+        f_bar[q] += gq * (1.0 - 1.0 / tau);
+        double eq_bar = gq / tau;
+        double h = gq * L->w[q] * k;
+        double cf_bar = h * (3.0 + 9.0 * cu);
+        double cu_bar = h * 9.0 * cf;
+        uf_bar -= 3.0 * h;
+        rho_bar += eq_bar * L->w[q] * poly;
+        double e = eq_bar * rho * L->w[q];
+        cu_bar += e * (3.0 + 9.0 * cu);
+        usq_bar -= 1.5 * e;
+        for (int d = 0; d < 3; d++) {
+            double cd = L->cf[q][d];
+            u_bar[d] += cu_bar * cd;
+            force_bar[d] += cf_bar * cd;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        u_bar[d] += usq_bar * 2.0 * u[d] + uf_bar * force[d];
+        force_bar[d] += uf_bar * u[d];
+    }
+    double alpha_bar = 0.0;
+    for (int d = 0; d < 3; d++) {
+        rho_bar += force_bar[d] * (a[d] - alpha * u[d]);
+        alpha_bar -= force_bar[d] * rho * u[d];
+        u_bar[d] -= force_bar[d] * rho * alpha;
+    }
+    double m_bar[3], s_bar = 0.0;
+    for (int d = 0; d < 3; d++) {
+        m_bar[d] = u_bar[d] / (s * rho);
+        rho_bar -= u_bar[d] * m[d] / (s * rho * rho);
+        s_bar -= u_bar[d] * u[d] / s;
+    }
+    alpha_bar += 0.5 * s_bar;
+    for (int q = 0; q < 19; q++) {
+        double v = f_bar[q] + rho_bar;
+        for (int d = 0; d < 3; d++) v += m_bar[d] * L->cf[q][d];
+        f_bar[q] = v;
+    }
+    *alpha_bar_out = alpha_bar;
+}
+
+/* ------------------------------------------------------------------------- */
+/* One cell: the Brinkman branch of psm.rs::cell_update (psm.rs:213-226) with   */
+/* zero solid momentum (static design), then Collision::collide.               */
+/* ------------------------------------------------------------------------- */
+static void brinkman_macro(const lattice_t *L, const double *f, double alpha, const double a[3], double *rho,
+                           double j[3], double *den, double sv[3], double u[3], double F[3]) {
+    moments(L, f, rho, j);
+    *den = 1.0 + alpha * 0.5;
+    for (int d = 0; d < 3; d++) {
+        sv[d] = j[d] / *rho + a[d] * 0.5; /* + gamma*m*0.5 with m = 0 */
+        u[d] = sv[d] / *den;
+    }
+    for (int d = 0; d < 3; d++) {
+        double z = -(alpha * u[d]); /* gamma*m - alpha*u with m = 0 */
+        F[d] = *rho * (a[d] + z);
+    }
+}
+
+void orc_cell_collide(const orc_problem *p, const double *f, double alpha, double *out) {
+    const lattice_t *L = lat(p->Q);
+    if (p->kind == K_BGK_GUO) {
+        collide_bgk_guo(f, alpha, p->accel, 1.0 / p->omega, out);
+        return;
+    }
+    double rho, j[3], den, sv[3], u[3], F[3];
+    brinkman_macro(L, f, alpha, p->accel, &rho, j, &den, sv, u, F);
+    switch (p->kind) {
+    case K_BGK: collide_pairs(L, f, rho, u, F, p->omega, p->omega, out); break;
+    case K_TRT: collide_pairs(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), out); break;
+    case K_MRT: collide_mrt(p, L, f, rho, u, F, out); break;
+    case K_CUMULANT: collide_cumulant(f, rho, u, F, p->omega, isnan(p->bulk_rate) ? 1.0 : p->bulk_rate, out); break;
+    }
+}
+
+/* VJP of one cell w.r.t. (f, alpha): psm.rs::cell_update_vjp Brinkman branch
+ * (psm.rs:301-330 and :455-469) with m = 0, dp_bar = 0, no LES. */
+void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const double *g, double *f_bar,
+                  double *alpha_bar_out) {
+    const lattice_t *L = lat(p->Q);
+    if (p->kind == K_BGK_GUO) {
+        collide_bgk_guo_vjp(f, alpha, p->accel, 1.0 / p->omega, g, f_bar, alpha_bar_out);
+        return;
+    }
+    int Q = p->Q;
+    const double *a = p->accel;
+    double rho, j[3], den, sv[3], u[3], F[3];
+    brinkman_macro(L, f, alpha, a, &rho, j, &den, sv, u, F);
+    double cb_f[MAXQ], cb_rho, cb_u[3], cb_F[3];
+    switch (p->kind) {
+    case K_BGK: collide_pairs_vjp(L, f, rho, u, F, p->omega, p->omega, g, cb_f, &cb_rho, cb_u, cb_F); break;
+    case K_TRT:
+        collide_pairs_vjp(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), g, cb_f, &cb_rho, cb_u,
+                          cb_F);
+        break;
+    case K_MRT: collide_mrt_vjp(p, L, f, rho, u, F, g, cb_f, &cb_rho, cb_u, cb_F); break;
+    default:
+        collide_cumulant_vjp(f, rho, u, F, p->omega, isnan(p->bulk_rate) ? 1.0 : p->bulk_rate, g, cb_f, &cb_rho,
+                             cb_u, cb_F);
+        break;
+    }
+    double rho_bar = 0.0 + cb_rho;
+    double u_bar[3] = {0.0 + cb_u[0], 0.0 + cb_u[1], 0.0 + cb_u[2]};
+    double alpha_bar = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double z = -(alpha * u[d]);
+        rho_bar += cb_F[d] * (a[d] + z) + 0.0 * z;
+        double zb = (cb_F[d] + 0.0) * rho;
+        alpha_bar -= zb * u[d];
+        u_bar[d] -= zb * alpha;
+    }
+    double j_bar[3];
+    for (int d = 0; d < 3; d++) {
+        double s_bar = u_bar[d] / den;
+        alpha_bar -= u_bar[d] * sv[d] / (den * den) * 0.5;
+        j_bar[d] = s_bar / rho;
+        rho_bar -= s_bar * j[d] / (rho * rho);
+    }
+    for (int i = 0; i < Q; i++) {
+        double v = cb_f[i] + rho_bar;
+        for (int d = 0; d < 3; d++)
+            if (L->cf[i][d] != 0.0) v += j_bar[d] * L->cf[i][d];
+        f_bar[i] = v;
+    }
+    *alpha_bar_out = alpha_bar;
+}
+
+/* ------------------------------------------------------------------------- */
+/* NEE ports: moving/boundary.rs:445-506 (port_populations and its VJP).       */
+/* ------------------------------------------------------------------------- */
+void orc_port_populations(int Q, const double *nb, int kind, const double value[3], double *out) {
+    const lattice_t *L = lat(Q);
+    double rho_n, j[3], u_n[3], eb[MAXQ], en[MAXQ];
+    moments(L, nb, &rho_n, j);
+    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    if (kind == 0) {
+        equilibrium(L, rho_n, value, eb);
+    } else {
+        equilibrium(L, value[0], u_n, eb);
+    }
+    equilibrium(L, rho_n, u_n, en);
+    for (int i = 0; i < Q; i++) out[i] = eb[i] + nb[i] - en[i];
+}
+
+void orc_port_populations_vjp(int Q, const double *nb, int kind, const double value[3], const double *g,
+                              double *nb_bar) {
+    const lattice_t *L = lat(Q);
+    double rho_n, j[3], u_n[3];
+    moments(L, nb, &rho_n, j);
+    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    double rho_b = kind == 0 ? rho_n : value[0];
+    const double *u_b = kind == 0 ? value : u_n;
+    double rb_bar = 0.0, ub_bar[3] = {0, 0, 0};
+    equilibrium_vjp(L, rho_b, u_b, g, &rb_bar, ub_bar);
+    double neg[MAXQ];
+    for (int i = 0; i < Q; i++) neg[i] = -g[i];
+    double rn_bar = 0.0, un_bar[3] = {0, 0, 0};
+    equilibrium_vjp(L, rho_n, u_n, neg, &rn_bar, un_bar);
+    if (kind == 0) {
+        rn_bar += rb_bar;
+    } else {
+        for (int d = 0; d < 3; d++) un_bar[d] += ub_bar[d];
+    }
+    double j_bar[3];
+    for (int d = 0; d < 3; d++) {
+        j_bar[d] = un_bar[d] / rho_n;
+        rn_bar -= un_bar[d] * j[d] / (rho_n * rho_n);
+    }
+    for (int i = 0; i < Q; i++) {
+        double v = g[i] + rn_bar;
+        for (int d = 0; d < 3; d++)
+            if (L->cf[i][d] != 0.0) v += j_bar[d] * L->cf[i][d];
+        nb_bar[i] = v;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Problem level: initial_populations (solver.rs:593-601), interval             */
+/* (solver.rs:604-614: collide -> stream -> ports), interval_vjp (:642-674).    */
+/* ------------------------------------------------------------------------- */
+void orc_initial(const orc_problem *p, double *f0) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double fluid[MAXQ];
+    for (int q = 0; q < Q; q++) fluid[q] = L->w[q] - 1.5 * L->w[q] * dot_c(L, q, p->accel);
+    for (long x = 0; x < n; x++)
+        for (int q = 0; q < Q; q++) f0[x * Q + q] = is_solid(p, x) ? L->w[q] : fluid[q];
+}
+
+/* Stationary ports applied in place after streaming; the rule is the moving NEE
+ * port_populations from the pulled neighbour (field.rs:1251-1263), i.e. from the
+ * neighbour's streamed populations.  exchange[r] = sum_q (new - old) at port cell r
+ * (ports.rs:560-580). */
+static void apply_ports(const orc_problem *p, double *after, double *exchange) {
+    int Q = p->Q;
+    double vals[MAXQ];
+    for (int r = 0; r < p->n_port_cells; r++) {
+        long x = p->port_cell[r], nb = p->port_nb[r];
+        orc_port_populations(Q, after + nb * Q, p->port_kind[r], p->port_value + 3 * r, vals);
+        double delta = 0.0;
+        for (int q = 0; q < Q; q++) {
+            delta += vals[q] - after[x * Q + q];
+            after[x * Q + q] = vals[q];
+        }
+        if (exchange) exchange[r] = delta;
+    }
+}
+
+void orc_interval(const orc_problem *p, const uint32_t *source, const double *alpha, const double *before,
+                  double *post, double *after, double *exchange) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) {
+            for (int q = 0; q < Q; q++) post[x * Q + q] = before[x * Q + q];
+            continue;
+        }
+        orc_cell_collide(p, before + x * Q, alpha[x], post + x * Q);
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n * Q; i++) {
+        uint32_t s = source[i];
+        after[i] = s == NONE_U32 ? before[i] : post[s];
+    }
+    apply_ports(p, after, exchange);
+}
+
+/* interval_vjp (solver.rs:642-674): ports VJP, stream VJP (each fluid post value has
+ * exactly one reader), collide VJP, plus previous_bar at solid cells.
+ * `after_pre_port` holds stream(collide(before)) at the port neighbours (the
+ * reference's reflected_cells, solver.rs:616-639). */
+void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double *alpha, const double *before,
+                      const double *after_pre_port, const double *after_bar, double *work, double *f_bar,
+                      double *alpha_bar) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    /* work := reflected_bar */
+    memcpy(work, after_bar, sizeof(double) * n * Q);
+    for (int r = 0; r < p->n_port_cells; r++) {
+        long x = p->port_cell[r];
+        for (int q = 0; q < Q; q++) work[x * Q + q] = 0.0;
+    }
+    double nb_bar[MAXQ];
+    for (int r = 0; r < p->n_port_cells; r++) {
+        long x = p->port_cell[r], nb = p->port_nb[r];
+        const double *g = after_bar + x * Q;
+        int nz = 0;
+        for (int q = 0; q < Q; q++) nz |= g[q] != 0.0;
+        if (!nz) continue;
+        orc_port_populations_vjp(Q, after_pre_port + nb * Q, p->port_kind[r], p->port_value + 3 * r, g, nb_bar);
+        for (int k = 0; k < Q; k++) work[nb * Q + k] += nb_bar[k];
+    }
+    /* stream_vjp: post_bar[s] += reflected_bar[i] for every i whose source is s;
+     * previous_bar = reflected_bar at solid cells. */
+    double *post_bar = f_bar; /* reuse: filled then consumed cell by cell below */
+    memset(post_bar, 0, sizeof(double) * n * Q);
+    for (long i = 0; i < n * Q; i++) {
+        uint32_t s = source[i];
+        if (s != NONE_U32) post_bar[s] += work[i];
+    }
+    double g[MAXQ], fb[MAXQ];
+#pragma omp parallel for schedule(static) private(g, fb)
+    for (long x = 0; x < n; x++) {
+        alpha_bar[x] = 0.0;
+        if (is_solid(p, x)) {
+            for (int q = 0; q < Q; q++) f_bar[x * Q + q] = work[x * Q + q];
+            continue;
+        }
+        int nz = 0;
+        for (int q = 0; q < Q; q++) {
+            g[q] = post_bar[x * Q + q];
+            nz |= g[q] != 0.0;
+        }
+        if (!nz) {
+            for (int q = 0; q < Q; q++) f_bar[x * Q + q] = 0.0;
+            continue;
+        }
+        double ab;
+        orc_cell_vjp(p, before + x * Q, alpha[x], g, fb, &ab);
+        for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
+        alpha_bar[x] = ab;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Macroscopic fields (solver.rs:690-705 in lattice units) and objectives.      */
+/* ------------------------------------------------------------------------- */
+void orc_macroscopic(const orc_problem *p, const double *alpha, const double *state, double *rho_out,
+                     double *u_out) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    for (long x = 0; x < n; x++) {
+        double rho, j[3], den, sv[3], u[3], F[3];
+        brinkman_macro(L, state + x * Q, alpha[x], p->accel, &rho, j, &den, sv, u, F);
+        rho_out[x] = rho;
+        for (int d = 0; d < 3; d++) u_out[3 * x + d] = is_solid(p, x) ? 0.0 : u[d];
+    }
+}
+
+/* Porous dissipation sum_{fluid} rho*alpha*|u|^2: vector_cell's drag term
+ * (solver.rs:835-850) in lattice units (density, velocity, step and dV scales 1). */
+double orc_obj_dissipation(const orc_problem *p, const double *alpha, const double *state) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double acc = 0.0;
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) continue;
+        double rho, j[3], den, sv[3], u[3], F[3];
+        brinkman_macro(L, state + x * Q, alpha[x], p->accel, &rho, j, &den, sv, u, F);
+        double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        acc += rho * alpha[x] * usq;
+    }
+    return acc;
+}
+
+/* Gradient of weight * sum rho*alpha*|u|^2, accumulated into f_bar / alpha_bar. */
+void orc_obj_dissipation_grad(const orc_problem *p, const double *alpha, const double *state, double weight,
+                              double *f_bar, double *alpha_bar) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) continue;
+        double rho, j[3], den, sv[3], u[3], F[3];
+        double al = alpha[x];
+        brinkman_macro(L, state + x * Q, al, p->accel, &rho, j, &den, sv, u, F);
+        double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        /* d = rho*al*usq; u = (j/rho + a/2)/den */
+        double rho_bar = weight * al * usq;
+        double usq_bar = weight * rho * al;
+        double al_bar = weight * rho * usq;
+        double jb[3];
+        for (int d = 0; d < 3; d++) {
+            double ub = usq_bar * 2.0 * u[d];
+            al_bar -= ub * sv[d] / (den * den) * 0.5;
+            double sb = ub / den;
+            jb[d] = sb / rho;
+            rho_bar -= sb * j[d] / (rho * rho);
+        }
+        for (int i = 0; i < Q; i++) {
+            double v = rho_bar;
+            for (int d = 0; d < 3; d++) v += jb[d] * L->cf[i][d];
+            f_bar[x * Q + i] += v;
+        }
+        alpha_bar[x] += al_bar;
+    }
+}
+
+/* Pressure drop between two planes normal to `axis`: mean(rho/3) over fluid cells
+ * of plane a minus that of plane b (lattice units, cs^2 = 1/3). */
+double orc_obj_pressure_drop(const orc_problem *p, const double *state, int axis, int plane_a, int plane_b) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double sa = 0.0, sb = 0.0;
+    long na = 0, nb = 0;
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) continue;
+        long c[3];
+        g_coords(p, x, c);
+        if (c[axis] != plane_a && c[axis] != plane_b) continue;
+        double rho = state[x * Q];
+        for (int q = 1; q < Q; q++) rho += state[x * Q + q];
+        if (c[axis] == plane_a) {
+            sa += rho;
+            na++;
+        } else {
+            sb += rho;
+            nb++;
+        }
+    }
+    return (sa / (double)na - sb / (double)nb) / 3.0;
+}
+
+void orc_obj_pressure_drop_grad(const orc_problem *p, int axis, int plane_a, int plane_b, double weight,
+                                double *f_bar) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    long na = 0, nb = 0;
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) continue;
+        long c[3];
+        g_coords(p, x, c);
+        if (c[axis] == plane_a) na++;
+        else if (c[axis] == plane_b) nb++;
+    }
+    double ga = weight / (3.0 * (double)na), gb = -weight / (3.0 * (double)nb);
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) continue;
+        long c[3];
+        g_coords(p, x, c);
+        double gv = c[axis] == plane_a ? ga : (c[axis] == plane_b ? gb : 0.0);
+        if (gv == 0.0) continue;
+        for (int q = 0; q < Q; q++) f_bar[x * Q + q] += gv;
+    }
+}

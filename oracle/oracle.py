@@ -1,0 +1,438 @@
+"""CPU FP64 oracle for the differentiable LBM hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only as the checker (or the timed CPU baseline).  The product
+(``implexity_amd``) never imports it.
+
+It wraps ``oracle/lbm_oracle.c`` (a C restatement of the reference's arithmetic, built
+by ``oracle/Makefile`` with -ffp-contract=off) and adds the reference's driver logic:
+
+* port enumeration       -- moving/boundary.rs:318-380 (Topology::with_symmetry ports)
+* ``interval``           -- solver.rs:604-614 (collide -> stream -> ports)
+* ``interval_vjp``       -- solver.rs:642-674
+* ``history``            -- solver.rs:677-687 (all states kept)
+* ``scan_adjoint``       -- crates/implexity-ad/src/scan.rs:140-246 (Every(k) policy)
+* ``binomial_scan``      -- scan.rs:284-390 driven by oracle/revolve.py
+* ``fd_vs_ad_probe``     -- crates/implexity-ad/src/gradcheck.rs:50-123
+
+Parity pins (each checked in tests/test_oracle_*.py):
+
+* tables                 -- closed forms (weights sum to 1, isotropy, opposite pairs)
+* stream table           -- brute-force geometric enumeration of halfway bounce-back
+* TRT/BGK collide        -- equilibrium moments, BGK == TRT at omega- == omega+,
+                            exact mass conservation, body-force Poiseuille closed form
+* MRT collide            -- MRT with default rates == TRT (options.rs:56-59)
+* cumulant collide       -- conservation of rho and (with force) of momentum update,
+                            omega = bulk = 1 Gaussian (Wick) closure
+* every VJP              -- central finite differences and dot-product identities
+* NEE ports              -- equilibrium reproduction on uniform flow, FD of the VJP
+* objectives             -- FD of their gradients, closed form on uniform states
+* revolve                -- optimal_forward_steps identity and exhaustive replay
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblbm_oracle.so")
+
+KIND = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4}
+FACES = {"xmin": (0, 1), "xmax": (0, -1), "ymin": (1, 1), "ymax": (1, -1), "zmin": (2, 1), "zmax": (2, -1)}
+NONE_U32 = 0xFFFFFFFF
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        d = ctypes.c_double
+        pd = ctypes.POINTER(ctypes.c_double)
+        _lib.orc_odd_rate.restype = d
+        _lib.orc_odd_rate.argtypes = [d, d]
+        _lib.orc_odd_rate_derivative.restype = d
+        _lib.orc_odd_rate_derivative.argtypes = [d, d]
+        _lib.orc_obj_dissipation.restype = d
+        _lib.orc_obj_pressure_drop.restype = d
+        del pd
+    return _lib
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("Q", ctypes.c_int32), ("kind", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("periodic", ctypes.c_int32 * 3),
+        ("omega", ctypes.c_double), ("magic", ctypes.c_double),
+        ("bulk_rate", ctypes.c_double), ("even_rate", ctypes.c_double),
+        ("accel", ctypes.c_double * 3),
+        ("n_port_cells", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("port_cell", ctypes.c_void_p), ("port_nb", ctypes.c_void_p),
+        ("port_kind", ctypes.c_void_p), ("port_value", ctypes.c_void_p),
+        ("solid", ctypes.c_void_p),
+        ("mrt_rows", ctypes.c_void_p), ("mrt_norms", ctypes.c_void_p), ("mrt_groups", ctypes.c_void_p),
+    ]
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def lattice(Q: int):
+    c = np.zeros((Q, 3), np.int32)
+    w = np.zeros(Q, np.float64)
+    opp = np.zeros(Q, np.int32)
+    lib().orc_lattice(ctypes.c_int(Q), _ptr(c), _ptr(w), _ptr(opp))
+    return c, w, opp
+
+
+def mrt_basis(Q: int):
+    rows = np.zeros((Q, Q), np.float64)
+    norms = np.zeros(Q, np.float64)
+    groups = np.zeros(Q, np.int32)
+    rc = lib().orc_mrt_basis(ctypes.c_int(Q), _ptr(rows), _ptr(norms), _ptr(groups))
+    if rc != 0:
+        raise RuntimeError("MRT basis does not span the lattice")
+    return rows, norms, groups
+
+
+def odd_rate(omega: float, magic: float) -> float:
+    return lib().orc_odd_rate(omega, magic)
+
+
+@dataclass
+class Port:
+    face: str
+    kind: str  # "velocity" | "pressure"
+    value: tuple  # velocity (3,) lattice units, or (rho,)
+    profile: np.ndarray | None = None  # one factor per port cell, face order
+
+
+@dataclass
+class Problem:
+    """Plain description of one lattice problem in lattice units."""
+
+    shape: tuple
+    periodic: tuple = (False, False, False)
+    Q: int = 19
+    kind: str = "trt"
+    omega: float = 1.0
+    magic: float = 3.0 / 16.0
+    bulk_rate: float = float("nan")
+    even_rate: float = float("nan")
+    accel: tuple = (0.0, 0.0, 0.0)
+    solid: np.ndarray | None = None  # bool, shape
+    ports: list = field(default_factory=list)
+
+
+class Oracle:
+    """FP64 CPU reference implementation (AoS state, index x*Q+q)."""
+
+    def __init__(self, prob: Problem):
+        self.prob = prob
+        nx, ny, nz = prob.shape
+        self.n = nx * ny * nz
+        self.Q = prob.Q
+        self.solid = (np.zeros(self.n, np.uint8) if prob.solid is None
+                      else np.ascontiguousarray(np.asarray(prob.solid, bool).reshape(-1), np.uint8))
+        cells, nbs, kinds, values = self.port_cells()
+        self.port_cell = np.asarray(cells, np.int32)
+        self.port_nb = np.asarray(nbs, np.int32)
+        self.port_kind = np.asarray(kinds, np.int32)
+        self.port_value = np.ascontiguousarray(np.asarray(values, np.float64).reshape(-1, 3))
+        self._mrt = mrt_basis(self.Q) if prob.kind == "mrt" else None
+        p = _Problem()
+        p.Q, p.kind = self.Q, KIND[prob.kind]
+        p.nx, p.ny, p.nz = nx, ny, nz
+        for a in range(3):
+            p.periodic[a] = int(bool(prob.periodic[a]))
+            p.accel[a] = float(prob.accel[a])
+        p.omega, p.magic = float(prob.omega), float(prob.magic)
+        p.bulk_rate, p.even_rate = float(prob.bulk_rate), float(prob.even_rate)
+        p.n_port_cells = len(self.port_cell)
+        p.port_cell = _ptr(self.port_cell)
+        p.port_nb = _ptr(self.port_nb)
+        p.port_kind = _ptr(self.port_kind)
+        p.port_value = _ptr(self.port_value)
+        p.solid = _ptr(self.solid)
+        if self._mrt is not None:
+            p.mrt_rows, p.mrt_norms, p.mrt_groups = (_ptr(self._mrt[0]), _ptr(self._mrt[1]), _ptr(self._mrt[2]))
+        self._p = p
+        self._pp = ctypes.byref(p)
+        self.source = np.zeros(self.n * self.Q, np.uint32)
+        lib().orc_stream_table(self._pp, _ptr(self.source))
+
+    # --- grid helpers (d3q19.rs:70-128) -------------------------------------------
+    def coords(self, x):
+        nx, ny, nz = self.prob.shape
+        return x // (ny * nz), (x // nz) % ny, x % nz
+
+    def index(self, i, j, k):
+        nx, ny, nz = self.prob.shape
+        return (i * ny + j) * nz + k
+
+    def port_cells(self):
+        """Port cells in the order of moving/boundary.rs:318-380: ports in list order,
+        cells of a face in ascending cell index among non-wall cells; the inward
+        neighbour is one cell along the face normal; the value is the port value times
+        the cell's profile factor (field.rs:1242-1248)."""
+        nx, ny, nz = self.prob.shape
+        shape = (nx, ny, nz)
+        solid = (np.zeros(self.n, bool) if self.prob.solid is None
+                 else np.asarray(self.prob.solid, bool).reshape(-1))
+        cells, nbs, kinds, values = [], [], [], []
+        for port in self.prob.ports:
+            axis, inward = FACES[port.face]
+            if self.prob.periodic[axis]:
+                raise ValueError("port on periodic axis")
+            layer = 0 if inward > 0 else shape[axis] - 1
+            idx = np.arange(self.n)
+            co = np.stack(self.coords(idx))
+            face = idx[(co[axis] == layer) & ~solid]
+            prof = port.profile
+            if prof is not None and len(prof) != len(face):
+                raise ValueError("profile needs one factor per port cell")
+            for r, x in enumerate(face):
+                c = list(self.coords(int(x)))
+                c[axis] += inward
+                nb = self.index(*c)
+                if solid[nb]:
+                    raise ValueError("inward neighbour of a port cell is a wall")
+                factor = 1.0 if prof is None else float(prof[r])
+                cells.append(int(x))
+                nbs.append(nb)
+                if port.kind == "velocity":
+                    kinds.append(0)
+                    values.append([float(v) * factor for v in port.value])
+                else:
+                    kinds.append(1)
+                    values.append([float(port.value[0]), 0.0, 0.0])
+        return cells, nbs, kinds, values
+
+    # --- cell level ---------------------------------------------------------------------
+    def cell_collide(self, f, alpha):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros(self.Q)
+        lib().orc_cell_collide(self._pp, _ptr(f), ctypes.c_double(alpha), _ptr(out))
+        return out
+
+    def cell_vjp(self, f, alpha, g):
+        f = np.ascontiguousarray(f, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        fb = np.zeros(self.Q)
+        ab = ctypes.c_double(0.0)
+        lib().orc_cell_vjp(self._pp, _ptr(f), ctypes.c_double(alpha), _ptr(g), _ptr(fb), ctypes.byref(ab))
+        return fb, ab.value
+
+    def port_populations(self, nb, kind, value):
+        nb = np.ascontiguousarray(nb, np.float64)
+        v = np.ascontiguousarray(np.asarray(value, np.float64).reshape(3))
+        out = np.zeros(self.Q)
+        lib().orc_port_populations(ctypes.c_int(self.Q), _ptr(nb), ctypes.c_int(kind), _ptr(v), _ptr(out))
+        return out
+
+    def port_populations_vjp(self, nb, kind, value, g):
+        nb = np.ascontiguousarray(nb, np.float64)
+        v = np.ascontiguousarray(np.asarray(value, np.float64).reshape(3))
+        g = np.ascontiguousarray(g, np.float64)
+        out = np.zeros(self.Q)
+        lib().orc_port_populations_vjp(ctypes.c_int(self.Q), _ptr(nb), ctypes.c_int(kind), _ptr(v), _ptr(g),
+                                       _ptr(out))
+        return out
+
+    # --- problem level --------------------------------------------------------------------
+    def initial(self):
+        f = np.zeros(self.n * self.Q)
+        lib().orc_initial(self._pp, _ptr(f))
+        return f
+
+    def interval(self, alpha, before, want_post=False):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        before = np.ascontiguousarray(before, np.float64)
+        post = np.empty_like(before)
+        after = np.empty_like(before)
+        ex = np.zeros(max(1, len(self.port_cell)))
+        lib().orc_interval(self._pp, _ptr(self.source), _ptr(alpha), _ptr(before), _ptr(post), _ptr(after),
+                           _ptr(ex))
+        if want_post:
+            return after, ex[: len(self.port_cell)], post
+        return after
+
+    def interval_vjp(self, alpha, before, after_bar, after=None):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        before = np.ascontiguousarray(before, np.float64)
+        after_bar = np.ascontiguousarray(after_bar, np.float64)
+        if after is None:
+            after = self.interval(alpha, before)
+        work = np.empty_like(before)
+        f_bar = np.empty_like(before)
+        a_bar = np.empty(self.n)
+        lib().orc_interval_vjp(self._pp, _ptr(self.source), _ptr(alpha), _ptr(before), _ptr(after),
+                               _ptr(after_bar), _ptr(work), _ptr(f_bar), _ptr(a_bar))
+        return f_bar, a_bar
+
+    def history(self, alpha, steps, f0=None):
+        """solver.rs:677-687 -- every state kept (use small problems)."""
+        states = [self.initial() if f0 is None else np.array(f0, np.float64)]
+        for _ in range(steps):
+            states.append(self.interval(alpha, states[-1]))
+        return states
+
+    def run(self, alpha, steps, f0=None):
+        f = self.initial() if f0 is None else np.array(f0, np.float64)
+        for _ in range(steps):
+            f = self.interval(alpha, f)
+        return f
+
+    def macroscopic(self, alpha, state):
+        rho = np.zeros(self.n)
+        u = np.zeros(3 * self.n)
+        lib().orc_macroscopic(self._pp, _ptr(np.ascontiguousarray(alpha, np.float64)),
+                              _ptr(np.ascontiguousarray(state, np.float64)), _ptr(rho), _ptr(u))
+        return rho, u.reshape(self.n, 3)
+
+    # --- objectives -------------------------------------------------------------------------
+    def dissipation(self, alpha, state):
+        return lib().orc_obj_dissipation(self._pp, _ptr(np.ascontiguousarray(alpha, np.float64)),
+                                         _ptr(np.ascontiguousarray(state, np.float64)))
+
+    def dissipation_grad(self, alpha, state, weight=1.0):
+        fb = np.zeros(self.n * self.Q)
+        ab = np.zeros(self.n)
+        lib().orc_obj_dissipation_grad(self._pp, _ptr(np.ascontiguousarray(alpha, np.float64)),
+                                       _ptr(np.ascontiguousarray(state, np.float64)), ctypes.c_double(weight),
+                                       _ptr(fb), _ptr(ab))
+        return fb, ab
+
+    def pressure_drop(self, state, axis, plane_a, plane_b):
+        return lib().orc_obj_pressure_drop(self._pp, _ptr(np.ascontiguousarray(state, np.float64)),
+                                           ctypes.c_int(axis), ctypes.c_int(plane_a), ctypes.c_int(plane_b))
+
+    def pressure_drop_grad(self, axis, plane_a, plane_b, weight=1.0):
+        fb = np.zeros(self.n * self.Q)
+        lib().orc_obj_pressure_drop_grad(self._pp, ctypes.c_int(axis), ctypes.c_int(plane_a), ctypes.c_int(plane_b),
+                                         ctypes.c_double(weight), _ptr(fb))
+        return fb
+
+
+# ----------------------------------------------------------------------------------------
+# Objectives in the ScanObjective form (scan.rs:96-107): stage(t, x_{t+1}) for t in
+# [0, n), terminal(x_n).  ``window`` lists the state indices s = t+1 that carry a stage
+# cost; the cost of a state is weight * phi(state).
+# ----------------------------------------------------------------------------------------
+@dataclass
+class Objective:
+    kind: str  # "dissipation" | "pressure_drop"
+    window: tuple = (1, 1)  # first and last state index (inclusive) carrying stage cost
+    weight: float = 1.0
+    terminal_weight: float = 0.0
+    axis: int = 0
+    plane_a: int = 1
+    plane_b: int = -2
+
+    def planes(self, shape):
+        n = shape[self.axis]
+        return self.plane_a % n, self.plane_b % n
+
+    def value(self, orc: Oracle, alpha, state):
+        if self.kind == "dissipation":
+            return orc.dissipation(alpha, state)
+        a, b = self.planes(orc.prob.shape)
+        return orc.pressure_drop(state, self.axis, a, b)
+
+    def grad(self, orc: Oracle, alpha, state, weight):
+        if self.kind == "dissipation":
+            return orc.dissipation_grad(alpha, state, weight)
+        a, b = self.planes(orc.prob.shape)
+        return orc.pressure_drop_grad(self.axis, a, b, weight), np.zeros(orc.n)
+
+    def stage_weight(self, s):
+        lo, hi = self.window
+        return self.weight if lo <= s <= hi else 0.0
+
+
+def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0=None):
+    """scan_adjoint with Checkpointing::Every(k) / All (scan.rs:140-246) over the oracle
+    step.  Returns (J, alpha_bar, lambda_0, final_state)."""
+    alpha = np.asarray(alpha, np.float64)
+    seg = steps if every is None else every
+    x = orc.initial() if f0 is None else np.array(f0, np.float64)
+    J = 0.0
+    checkpoints = [(0, x.copy())]
+    for t in range(steps):
+        x = orc.interval(alpha, x)
+        w = objective.stage_weight(t + 1)
+        if w:
+            J += w * objective.value(orc, alpha, x)
+        if (t + 1) % seg == 0 and t + 1 < steps:
+            checkpoints.append((t + 1, x.copy()))
+    final = x
+    abar = np.zeros(orc.n)
+    lam = np.zeros(orc.n * orc.Q)
+    if objective.terminal_weight:
+        J += objective.terminal_weight * objective.value(orc, alpha, final)
+        g, ga = objective.grad(orc, alpha, final, objective.terminal_weight)
+        lam += g
+        abar += ga
+    after = final
+    while checkpoints:
+        start, xs = checkpoints.pop()
+        end = min(start + seg, steps)
+        states = [xs]
+        for t in range(start, end - 1):
+            states.append(orc.interval(alpha, states[-1]))
+        for t in range(end - 1, start - 1, -1):
+            x_next = states[t + 1 - start] if t + 1 < end else after
+            w = objective.stage_weight(t + 1)
+            if w:
+                g, ga = objective.grad(orc, alpha, x_next, w)
+                lam = lam + g
+                abar += ga
+            fb, ab = orc.interval_vjp(alpha, states[t - start], lam)
+            abar += ab
+            lam = fb
+        after = states[0]
+    return J, abar, lam, final
+
+
+def objective_value(orc: Oracle, alpha, steps, objective: Objective, f0=None):
+    """Forward-only objective (for finite differences)."""
+    x = orc.initial() if f0 is None else np.array(f0, np.float64)
+    J = 0.0
+    for t in range(steps):
+        x = orc.interval(alpha, x)
+        w = objective.stage_weight(t + 1)
+        if w:
+            J += w * objective.value(orc, alpha, x)
+    if objective.terminal_weight:
+        J += objective.terminal_weight * objective.value(orc, alpha, x)
+    return J
+
+
+def fd_vs_ad_probe(f, x, index, value, ad, step=None):
+    """gradcheck.rs:50-123: central difference with h = step*max(|x_i|, 1) and
+    rel_err = |ad/s - fd/s|, s = max(|ad|, |fd|)."""
+    step = np.finfo(float).eps ** (1.0 / 3.0) if step is None else step
+    x = np.asarray(x, np.float64)
+    h = step * max(abs(x[index]), 1.0)
+    xp = x.copy()
+    xp[index] += h
+    xm = x.copy()
+    xm[index] -= h
+    fd = (f(xp) - f(xm)) / (2.0 * h)
+    scale = max(abs(ad), abs(fd))
+    rel = abs(ad / scale - fd / scale) if scale > 0 else 0.0
+    return fd, rel

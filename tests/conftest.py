@@ -1,0 +1,33 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    config.addinivalue_line("markers", "slow: long-running parity case")
+
+
+def pytest_collection_modifyitems(config, items):
+    # Without a GPU driver at all, GPU tests cannot run; on a GPU box they run and fail
+    # loudly if the HIP path is broken.
+    if os.path.exists("/dev/kfd"):
+        return
+    skip = pytest.mark.skip(reason="no GPU driver (/dev/kfd) in this container")
+    for item in items:
+        if "gpu" in item.keywords:
+            item.add_marker(skip)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _oracle_built():
+    from oracle import oracle as orc
+
+    if not os.path.exists(orc.LIB_PATH):
+        orc.build()
+    yield

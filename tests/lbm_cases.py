@@ -1,0 +1,43 @@
+"""Shared problem builders: the same problem for the GPU library and for the oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+from implexity_amd import design
+from implexity_amd.lbm import LatticeConfig, Port
+from oracle import oracle as orc
+
+
+def make_pair(shape, periodic=(False, False, False), Q=19, collision="trt", precision="f64", omega=1.25,
+              magic=3.0 / 16.0, accel=(0.0, 0.0, 0.0), solid=None, ports=(), bulk_rate=float("nan"),
+              even_rate=float("nan"), budget=0):
+    """Returns (LatticeConfig, oracle.Problem) describing one problem."""
+    gp = [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
+    op = [orc.Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
+    cfg = LatticeConfig(shape=shape, periodic=periodic, lattice=Q, collision=collision, precision=precision,
+                        omega=omega, magic=magic, accel=accel, solid=solid, ports=gp, bulk_rate=bulk_rate,
+                        even_rate=even_rate, checkpoint_budget_bytes=budget)
+    prob = orc.Problem(shape=shape, periodic=periodic, Q=Q, kind=collision, omega=omega, magic=magic,
+                       bulk_rate=bulk_rate, even_rate=even_rate, accel=accel, solid=solid, ports=op)
+    return cfg, prob
+
+
+def random_alpha(shape, seed, alpha_max=0.1, solid=None):
+    phi = design.smoothed_random_design(shape, 1.5, seed)
+    a = design.resistance(phi, alpha_max).reshape(-1)
+    return a
+
+
+def random_solid(shape, seed, frac=0.08):
+    rng = np.random.default_rng(seed)
+    s = rng.random(shape) < frac
+    return s
+
+
+def channel_ports(shape, u_in=0.02, rho_out=1.0):
+    nx, ny, nz = shape
+    prof = design.parabolic_profile(ny, nz).reshape(-1)
+    return [
+        {"face": "xmin", "kind": "velocity", "value": (u_in, 0.0, 0.0), "profile": prof},
+        {"face": "xmax", "kind": "pressure", "value": (rho_out,)},
+    ]

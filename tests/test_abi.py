@@ -1,0 +1,67 @@
+"""The C ABI library, without a GPU: it loads, exports every symbol the header
+declares, builds revolve schedules on the host, and validates problems before any
+device call."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from implexity_amd import lbm
+from oracle import revolve as rev
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "implexity_lbm.h")
+
+
+def test_library_exports_every_declared_symbol():
+    names = lbm.declared_symbols(HEADER)
+    assert len(names) >= 20
+    lib = lbm.lib()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(lbm._SIGS)
+
+
+def test_abi_version():
+    assert lbm.lib().lbm_abi_version() == 1
+
+
+@pytest.mark.parametrize("n,s", [(1, 1), (3, 1), (10, 2), (100, 10), (1000, 30), (8000, 156)])
+def test_revolve_schedule_host(n, s):
+    sch = lbm.BinomialSchedule(n, s)
+    acts, rstart, fwd, _ = rev.binomial_schedule(n, s)
+    assert sch.actions == acts and sch.reverse_start == rstart and sch.forward_steps == fwd
+
+
+def test_revolve_refuses_no_slots():
+    with pytest.raises(lbm.LbmError):
+        lbm.BinomialSchedule(10, 0)
+
+
+def _create(**kw):
+    cfg = lbm.LatticeConfig(shape=kw.pop("shape", (4, 4, 4)), **kw)
+    return lbm.Lattice(cfg)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(shape=(1, 4, 4)), "shape"),
+    (dict(omega=2.5), "relaxation time"),
+    (dict(collision="cumulant"), "D3Q27"),
+    (dict(lattice=27, collision="bgk_guo"), "D3Q19"),
+    (dict(magic=-1.0), "magic"),
+    (dict(precision="bf16s", collision="mrt"), "BF16"),
+    (dict(periodic=(True, False, False), ports=[lbm.Port("xmin", "pressure", (1.0,))]), "periodic"),
+])
+def test_contract_errors_before_device(kw, msg):
+    """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are
+    refused with status 1 before anything touches the GPU."""
+    with pytest.raises(lbm.ContractError, match=msg):
+        _create(**kw)
+
+
+def test_solid_mask_length_is_checked():
+    with pytest.raises(lbm.ShapeError):
+        _create(solid=np.zeros(5, bool))
