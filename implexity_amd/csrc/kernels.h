@@ -5,7 +5,7 @@
 //   Populations are structure-of-arrays: array q holds one value per storage cell,
 //   storage cell (i,j,k) at i*sxy + j*SZ + k (z fastest, as the reference's Grid,
 //   d3q19.rs:86-89).  Non-periodic axes carry one extra storage layer at index n
-//   (the esoteric-twist ghost layer); SZ is padded to a multiple of 32 elements.
+//   (the esoteric-twist ghost layer); the z pitch SZ is nz + 1.
 //
 // In-place esoteric-twist streaming
 //   With a_q = max(c_q, 0) and a_opp(q) = max(-c_q, 0) (so a_q - a_opp(q) = c_q), a
@@ -18,7 +18,10 @@
 //   is read and written once per step, and the state after the step is the
 //   reference's post-stream state (solver.rs:604-614) in the layout of parity 1-p.
 //   Solid cells are never updated; their slots carry the reflected populations of
-//   their fluid neighbours.
+//   their fluid neighbours.  The same step may also run out of place (read buffer A,
+//   write buffer B): the locations it writes are exactly those the next step reads,
+//   so B needs no other content -- this lets the checkpoint driver start a step from
+//   a snapshot slot, or finish one into a slot, without copying whole states.
 //
 // Discrete adjoint in place
 //   The cotangent of f*_q(x) is the cotangent of whoever reads that value next step,
@@ -63,7 +66,8 @@ struct ObjArgs {
 
 struct StepArgs {
     Geom g;
-    void* pop;
+    void* pop;        // state read (parity odd)
+    void* pop_out;    // buffer written (== pop for an in-place step)
     const void* alpha;
     const uint32_t* mask;
     double omega, om, tau;
@@ -72,8 +76,10 @@ struct StepArgs {
     int odd;
     int diag;         // 1: accumulate diagnostics
     ObjArgs obj;      // objective of the state being read (kind 0: none)
-    double* jacc;     // objective accumulator
-    double* macc;     // fluid mass of the state being read (diagnostics)
+    double* part;
+    long long npart;  // capacity of part, in blocks     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
+    double* jacc;     // objective accumulator      (written by k_reduce_partials)
+    double* macc;     // fluid mass of the state    (written by k_reduce_partials)
     unsigned* dstat;  // [0] max |u|^2 (float bits), [1] bad cells (non-finite or f <= 0)
 };
 
@@ -103,6 +109,14 @@ struct Store<P_F64> {
     using C = double;
     static constexpr bool shifted = false;  // holds f itself
     template <int Q, int q>
+    static LBM_DEV C dec(T v) {
+        return v;
+    }
+    template <int Q, int q>
+    static LBM_DEV T enc(C v) {
+        return v;
+    }
+    template <int Q, int q>
     static LBM_DEV C ld(const T* p) {
         return *p;
     }
@@ -118,6 +132,14 @@ struct Store<P_F32> {
     using T = float;
     using C = float;
     static constexpr bool shifted = true;
+    template <int Q, int q>
+    static LBM_DEV C dec(T v) {
+        return v;
+    }
+    template <int Q, int q>
+    static LBM_DEV T enc(C v) {
+        return v;
+    }
     template <int Q, int q>
     static LBM_DEV C ld(const T* p) {
         return *p;
@@ -135,14 +157,22 @@ struct Store<P_F16S> {
     using C = float;
     static constexpr bool shifted = true;
     template <int Q, int q>
-    static LBM_DEV C ld(const T* p) {
+    static LBM_DEV C dec(T v) {
         constexpr float inv = 1.0f / float(1u << fp16_exp<Q>(q));
-        return float(*p) * inv;
+        return float(v) * inv;
+    }
+    template <int Q, int q>
+    static LBM_DEV T enc(C v) {
+        constexpr float s = float(1u << fp16_exp<Q>(q));
+        return _Float16(v * s);
+    }
+    template <int Q, int q>
+    static LBM_DEV C ld(const T* p) {
+        return dec<Q, q>(*p);
     }
     template <int Q, int q>
     static LBM_DEV void st(T* p, C v) {
-        constexpr float s = float(1u << fp16_exp<Q>(q));
-        *p = _Float16(v * s);
+        *p = enc<Q, q>(v);
     }
 };
 
@@ -159,6 +189,14 @@ struct Store<P_BF16S> {
     using T = uint16_t;
     using C = float;
     static constexpr bool shifted = true;
+    template <int Q, int q>
+    static LBM_DEV C dec(T v) {
+        return bf16_to_f32(v);
+    }
+    template <int Q, int q>
+    static LBM_DEV T enc(C v) {
+        return f32_to_bf16(v);
+    }
     template <int Q, int q>
     static LBM_DEV C ld(const T* p) {
         return bf16_to_f32(*p);
@@ -306,26 +344,85 @@ LBM_DEV float wave_max(float v) {
     return v;
 }
 
-// Sum two doubles over the block and add them to *oa, *ob with one atomic each.
-LBM_DEV void block_add2(double a, double b, double* oa, double* ob) {
-    __shared__ double red[2][16];
-    a = wave_sum(a);
-    b = wave_sum(b);
+LBM_DEV long long block_linear() {
+    return (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * blockIdx.z);
+}
+
+// Per-block partials of the objective, the fluid mass, max |u|^2 and the count of bad
+// cells, written without atomics (a single address hit by every block serialises in
+// L2); k_reduce_partials folds them, in a fixed order, after the step.
+LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* part) {
+    __shared__ double red[4][16];
+    jv = wave_sum(jv);
+    mv = wave_sum(mv);
+    umax = wave_max(umax);
+    const double nbad = double(__popcll(__ballot(bad)));
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
-        red[0][wid] = a;
-        red[1][wid] = b;
+        red[0][wid] = jv;
+        red[1][wid] = mv;
+        red[2][wid] = double(umax);
+        red[3][wid] = nbad;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         const int nw = (blockDim.x + 63) >> 6;
-        double sa = 0.0, sb = 0.0;
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
         for (int w = 0; w < nw; w++) {
+            a += red[0][w];
+            b += red[1][w];
+            c = fmax(c, red[2][w]);
+            d += red[3][w];
+        }
+        double* o = part + 4 * block_linear();
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+        o[3] = d;
+    }
+}
+
+// One block of 1024 threads folds nb partials: jacc += sum J, *macc = sum mass (if
+// set), dstat[0] = max(dstat[0], max |u|^2), dstat[1] += bad cells.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double* part, long long nb, double* jacc,
+                                                          double* macc, unsigned* dstat) {
+    __shared__ double red[4][16];
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    for (long long i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += part[4 * i];
+        b += part[4 * i + 1];
+        c = fmax(c, part[4 * i + 2]);
+        d += part[4 * i + 3];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    d = wave_sum(d);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c = fmax(c, __shfl_down(c, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wid] = a;
+        red[1][wid] = b;
+        red[2][wid] = c;
+        red[3][wid] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+        for (int w = 0; w < int(blockDim.x >> 6); w++) {
             sa += red[0][w];
             sb += red[1][w];
+            sc = fmax(sc, red[2][w]);
+            sd += red[3][w];
         }
-        if (oa && sa != 0.0) unsafeAtomicAdd(oa, sa);
-        if (ob && sb != 0.0) unsafeAtomicAdd(ob, sb);
+        if (jacc) *jacc += sa;
+        if (macc) *macc = sb;
+        if (dstat) {
+            const unsigned ub = __float_as_uint(float(sc));
+            if (ub > dstat[0]) dstat[0] = ub;
+            dstat[1] += unsigned(sd);
+        }
     }
 }
 
@@ -498,21 +595,14 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                     bad |= !(fq > C(0)) || !isfinite(float(fq));
                 }
             }
+            typename S::T* out = static_cast<typename S::T*>(A.pop_out);
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
-                S::template st<Q, q>(pop + write_loc<Q, q>(g, cu, A.odd), fs[q]);
+                S::template st<Q, q>(out + write_loc<Q, q>(g, cu, A.odd), fs[q]);
             });
         }
     }
-    if (A.obj.kind != OBJ_NONE || A.diag) block_add2(jsum, msum, A.jacc, A.diag ? A.macc : nullptr);
-    if (A.diag) {
-        umax = wave_max(umax);
-        const unsigned nbad = __popcll(__ballot(bad != 0));
-        if ((threadIdx.x & 63) == 0) {
-            atomicMax(A.dstat, __float_as_uint(umax));
-            if (nbad) atomicAdd(A.dstat + 1, nbad);
-        }
-    }
+    if (A.part) block_partials(jsum, msum, umax, bad != 0, A.part);
 }
 
 // ---------------------------------------------------------------------------------
@@ -554,6 +644,264 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
         constexpr int q = decltype(I)::value;
         L::template st<C>(lam + read_loc<Q, q>(g, cu, m, A.odd), fb[q]);
     });
+}
+
+
+// ---------------------------------------------------------------------------------
+// Vectorised steps: each thread updates V consecutive cells along z with V-wide loads
+// and stores per population array (the esoteric-twist +z offsets make some of them
+// one element off alignment; gfx950 serves unaligned dword vector accesses).  Links
+// blocked by a wall are patched per cell with scalar loads (reads) or stores
+// (adjoint writes).  Chunks that contain a solid cell take the per-cell path, since
+// a vector store would overwrite the bounce-back slots that solid cells lend their
+// neighbours.  Requires nz % V == 0 and a non-periodic z axis.
+// ---------------------------------------------------------------------------------
+template <class T, int V>
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(V), aligned(sizeof(T))));
+};
+
+struct Chunk {
+    int i, j, k0;
+    long long cell0;
+};
+
+template <int V>
+LBM_DEV bool chunk_of(const Geom& g, Chunk& c) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per_row = g.nz / V;
+    const long long row = t / per_row;
+    if (row >= (long long)g.nx * g.ny) return false;
+    c.k0 = int(t - row * per_row) * V;
+    c.j = int(row % g.ny);
+    c.i = int(row / g.ny);
+    c.cell0 = row * g.nz + c.k0;
+    return true;
+}
+
+template <int Q, bool MASK, int V>
+LBM_DEV bool chunk_masks(const Geom& g, const uint32_t* mask, const Chunk& c, uint32_t* m) {
+    bool solid = false;
+    if constexpr (MASK) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            m[v] = mask[c.cell0 + v];
+            solid |= (m[v] & SOLID_BIT) != 0;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) m[v] = face_mask<Q>(g, c.i, c.j, c.k0 + v);
+    }
+    return solid;
+}
+
+// Vector read of f at the read locations of V cells (parity odd), patched per cell for
+// blocked links.
+template <int P, int Q, int V>
+LBM_DEV void load_chunk(const Geom& g, const typename Store<P>::T* pop, const Cursor& cu, const uint32_t* m,
+                        int odd, typename VecOf<typename Store<P>::T, V>::type* vin) {
+    using VT = typename VecOf<typename Store<P>::T, V>::type;
+    uint32_t any = 0;
+#pragma unroll
+    for (int v = 0; v < V; v++) any |= m[v];
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        const long long pos = cu.at(corner_neg<Q>(q));
+        const int arr_n = odd ? q : opp<Q>(q);
+        vin[q] = *reinterpret_cast<const VT*>(pop + (long long)arr_n * g.stride + pos);
+        if ((any >> q) & 1u) {
+            const int arr_b = odd ? opp<Q>(q) : q;
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                if ((m[v] >> q) & 1u) vin[q][v] = pop[(long long)arr_b * g.stride + pos + v];
+        }
+    });
+}
+
+template <int P, int Q, int KIND, bool MASK, int V>
+__global__ __launch_bounds__(128) void k_forward_v(StepArgs A) {
+    using S = Store<P>;
+    using C = typename S::C;
+    using T = typename S::T;
+    using VT = typename VecOf<T, V>::type;
+    const Geom& g = A.g;
+    Chunk ch;
+    double jsum = 0.0, msum = 0.0;
+    float umax = 0.0f;
+    unsigned bad = 0;
+    if (chunk_of<V>(g, ch)) {
+        uint32_t m[V];
+        const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m);
+        T* pop = static_cast<T*>(A.pop);
+        const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+        const C* alpha_p = static_cast<const C*>(A.alpha);
+        if (!has_solid) {
+            const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0);
+            VT vin[Q];
+            load_chunk<P, Q, V>(g, pop, cu, m, A.odd, vin);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                C f[Q], fs[Q];
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    f[q] = S::template dec<Q, q>(vin[q][v]);
+                });
+                const C alpha = alpha_p[ch.cell0 + v];
+                Macro<Q, C> mac;
+                collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                static_cast<const C*>(A.mrt), fs, mac);
+                if (A.obj.kind != OBJ_NONE) {
+                    const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
+                    jsum += objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
+                }
+                if (A.diag) {
+                    msum += double(mac.rho);
+                    umax = fmaxf(umax, float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]));
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
+                        bad |= !(fq > C(0)) || !isfinite(float(fq));
+                    }
+                }
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    vin[q][v] = S::template enc<Q, q>(fs[q]);
+                });
+            }
+            T* out = static_cast<T*>(A.pop_out);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                *reinterpret_cast<VT*>(out + write_loc<Q, q>(g, cu, A.odd)) = vin[q];
+            });
+        } else {
+            for (int v = 0; v < V; v++) {
+                if (m[v] & SOLID_BIT) continue;
+                const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0 + v);
+                C f[Q], fs[Q];
+                load_cell<P, Q>(g, pop, cu, m[v], A.odd, f);
+                const C alpha = alpha_p[ch.cell0 + v];
+                Macro<Q, C> mac;
+                collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                static_cast<const C*>(A.mrt), fs, mac);
+                if (A.obj.kind != OBJ_NONE) {
+                    const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
+                    jsum += objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
+                }
+                if (A.diag) {
+                    msum += double(mac.rho);
+                    umax = fmaxf(umax, float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]));
+                }
+                T* out = static_cast<T*>(A.pop_out);
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    S::template st<Q, q>(out + write_loc<Q, q>(g, cu, A.odd), fs[q]);
+                });
+            }
+        }
+    }
+    if (A.part) block_partials(jsum, msum, umax, bad != 0, A.part);
+}
+
+template <int P, int LP, int Q, int KIND, bool MASK, int V>
+__global__ __launch_bounds__(128) void k_adjoint_v(AdjArgs A) {
+    using S = Store<P>;
+    using L = LamStore<LP>;
+    using C = typename S::C;
+    using T = typename S::T;
+    using LT = typename L::T;
+    using VT = typename VecOf<T, V>::type;
+    using VL = typename VecOf<LT, V>::type;
+    const Geom& g = A.g;
+    Chunk ch;
+    if (!chunk_of<V>(g, ch)) return;
+    uint32_t m[V];
+    const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m);
+    const T* pop = static_cast<const T*>(A.pop);
+    LT* lam = static_cast<LT*>(A.lam);
+    const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+    const C* alpha_p = static_cast<const C*>(A.alpha);
+    if (!has_solid) {
+        const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0);
+        uint32_t any = 0;
+#pragma unroll
+        for (int v = 0; v < V; v++) any |= m[v];
+        VT vin[Q];
+        VL vl[Q];
+        load_chunk<P, Q, V>(g, pop, cu, m, A.odd, vin);
+        static_for<0, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            vl[q] = *reinterpret_cast<const VL*>(lam + write_loc<Q, q>(g, cu, A.odd));
+        });
+        double abv[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            C f[Q], gb[Q], fb[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                f[q] = S::template dec<Q, q>(vin[q][v]);
+                gb[q] = C(vl[q][v]);
+            });
+            const C alpha = alpha_p[ch.cell0 + v];
+            Macro<Q, C> mac;
+            C ab;
+            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
+            if (A.obj.kind != OBJ_NONE) {
+                const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
+                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+            }
+            abv[v] = double(ab);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                vl[q][v] = LT(fb[q]);
+            });
+        }
+        typedef double VD __attribute__((ext_vector_type(V), aligned(8)));
+        VD* abp = reinterpret_cast<VD*>(A.alpha_bar + ch.cell0);
+        VD acc = *abp;
+#pragma unroll
+        for (int v = 0; v < V; v++) acc[v] += abv[v];
+        *abp = acc;
+        // Cotangents go to the read locations: vector store at the unblocked ones,
+        // then the blocked links of each cell to their bounce-back slots.
+        static_for<0, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            const long long pos = cu.at(corner_neg<Q>(q));
+            const int arr_n = A.odd ? q : opp<Q>(q);
+            *reinterpret_cast<VL*>(lam + (long long)arr_n * g.stride + pos) = vl[q];
+            if ((any >> q) & 1u) {
+                const int arr_b = A.odd ? opp<Q>(q) : q;
+#pragma unroll
+                for (int v = 0; v < V; v++)
+                    if ((m[v] >> q) & 1u) lam[(long long)arr_b * g.stride + pos + v] = vl[q][v];
+            }
+        });
+    } else {
+        for (int v = 0; v < V; v++) {
+            if (m[v] & SOLID_BIT) continue;
+            const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0 + v);
+            C f[Q], gb[Q], fb[Q];
+            load_cell<P, Q>(g, pop, cu, m[v], A.odd, f);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                gb[q] = L::template ld<C>(lam + write_loc<Q, q>(g, cu, A.odd));
+            });
+            const C alpha = alpha_p[ch.cell0 + v];
+            Macro<Q, C> mac;
+            C ab;
+            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
+            if (A.obj.kind != OBJ_NONE) {
+                const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
+                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+            }
+            A.alpha_bar[ch.cell0 + v] += double(ab);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                L::template st<C>(lam + read_loc<Q, q>(g, cu, m[v], A.odd), fb[q]);
+            });
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -606,7 +954,7 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
             alpha_bar[cell] = 0.0;
         }
     }
-    block_add2(jsum, msum, A.jacc, A.macc);
+    if (A.part) block_partials(jsum, msum, 0.0f, false, A.part);
 }
 
 // ---------------------------------------------------------------------------------

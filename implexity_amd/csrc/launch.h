@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace lbm {
@@ -53,27 +55,99 @@ const Runner* runner_bf16s();
 // and the streaming-only debug operator; D3Q27 with BGK, TRT, MRT and cumulant.
 template <int P, int LP, bool ALL_KINDS>
 struct RunnerImpl final : Runner {
+    // Cells per thread, chosen from 256^3 measurements: one for forward steps; two for
+    // FP32/FP64 adjoint steps (8- and 16-byte accesses) and one for 16-bit storage.
+    // LBM_FWD_CELLS / LBM_ADJ_CELLS = 1 | 2 override them.
+    static int width(const char* env, int dflt) {
+        const char* e = std::getenv(env);
+        const int v = e ? std::atoi(e) : dflt;
+        return v == 2 ? 2 : 1;
+    }
+    static int fwd_width() {
+        static const int w = width("LBM_FWD_CELLS", 1);
+        return w;
+    }
+    static int adj_width() {
+        static const int w = width("LBM_ADJ_CELLS", (P == P_F32 || P == P_F64) ? 2 : 1);
+        return w;
+    }
+    static LaunchCfg vec_cfg(const Geom& g, int V, hipStream_t s) {
+        const long long chunks = (long long)g.nx * g.ny * (g.nz / V);
+        return LaunchCfg{dim3(unsigned((chunks + 127) / 128)), dim3(128), s};
+    }
+    template <int Q, bool M, int V>
+    static hipError_t fwd_vec(int kind, const StepArgs& a, const LaunchCfg& c) {
+        switch (kind) {
+        case K_TRT: LBM_LAUNCH((k_forward_v<P, Q, K_TRT, M, V>), c, a);
+        case K_BGK: LBM_LAUNCH((k_forward_v<P, Q, K_BGK, M, V>), c, a);
+        case K_STREAM_ONLY: LBM_LAUNCH((k_forward_v<P, Q, K_STREAM_ONLY, M, V>), c, a);
+        default: break;
+        }
+        if constexpr (ALL_KINDS) {
+            if (kind == K_MRT) LBM_LAUNCH((k_forward_v<P, Q, K_MRT, M, V>), c, a);
+            if constexpr (Q == 19) {
+                if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward_v<P, 19, K_BGK_GUO, M, V>), c, a);
+            } else {
+                if (kind == K_CUMULANT) LBM_LAUNCH((k_forward_v<P, 27, K_CUMULANT, M, V>), c, a);
+            }
+        }
+        return hipErrorInvalidValue;
+    }
+    template <int Q, bool M, int V>
+    static hipError_t adj_vec(int kind, const AdjArgs& a, const LaunchCfg& c) {
+        switch (kind) {
+        case K_TRT: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_TRT, M, V>), c, a);
+        case K_BGK: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_BGK, M, V>), c, a);
+        default: break;
+        }
+        if constexpr (ALL_KINDS) {
+            if (kind == K_MRT) LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_MRT, M, V>), c, a);
+            if constexpr (Q == 19) {
+                if (kind == K_BGK_GUO) LBM_LAUNCH((k_adjoint_v<P, LP, 19, K_BGK_GUO, M, V>), c, a);
+            } else {
+                if (kind == K_CUMULANT) LBM_LAUNCH((k_adjoint_v<P, LP, 27, K_CUMULANT, M, V>), c, a);
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     template <int Q, bool M>
-    static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c) {
+    static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c0, LaunchCfg& used) {
+        const int V = fwd_width();
+        if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
+            const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
+            if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
+            used = c;
+            const hipError_t e = fwd_vec<Q, M, 2>(kind, a, c);
+            if (e != hipErrorInvalidValue) return e;
+        }
+        const LaunchCfg& c = c0;
+        if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
+        used = c0;
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_forward<P, Q, K_TRT, M>), c, a);
         case K_BGK: LBM_LAUNCH((k_forward<P, Q, K_BGK, M>), c, a);
+        case K_STREAM_ONLY: LBM_LAUNCH((k_forward<P, Q, K_STREAM_ONLY, M>), c, a);
         default: break;
         }
         if constexpr (ALL_KINDS) {
             if (kind == K_MRT) LBM_LAUNCH((k_forward<P, Q, K_MRT, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward<P, 19, K_BGK_GUO, M>), c, a);
-                if (kind == K_STREAM_ONLY) LBM_LAUNCH((k_forward<P, 19, K_STREAM_ONLY, M>), c, a);
             } else {
                 if (kind == K_CUMULANT) LBM_LAUNCH((k_forward<P, 27, K_CUMULANT, M>), c, a);
-                if (kind == K_STREAM_ONLY) LBM_LAUNCH((k_forward<P, 27, K_STREAM_ONLY, M>), c, a);
             }
         }
         return hipErrorInvalidValue;
     }
     template <int Q, bool M>
-    static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c) {
+    static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c0) {
+        const int V = adj_width();
+        if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
+            const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
+            const hipError_t e = adj_vec<Q, M, 2>(kind, a, c);
+            if (e != hipErrorInvalidValue) return e;
+        }
+        const LaunchCfg& c = c0;
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT, M>), c, a);
         case K_BGK: LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK, M>), c, a);
@@ -90,27 +164,81 @@ struct RunnerImpl final : Runner {
         return hipErrorInvalidValue;
     }
 
+    // Host-side checks before any launch: every pointer a kernel dereferences for this
+    // configuration must be set.
+    static bool ok_step(int kind, bool mask, const StepArgs& a) {
+        if (!a.pop || !a.pop_out || !a.alpha) return false;
+        if (mask && !a.mask) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
+        if ((a.obj.kind != OBJ_NONE || a.diag) && (!a.part || !a.jacc)) return false;
+        if (a.diag && (!a.dstat || !a.macc)) return false;
+        return true;
+    }
+    static bool ok_adj(int kind, bool mask, const AdjArgs& a) {
+        if (!a.pop || !a.lam || !a.alpha || !a.alpha_bar) return false;
+        if (mask && !a.mask) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
+        return true;
+    }
+    static bool ok_ports(bool mask, const PortArgs& a) {
+        if (a.np <= 0) return true;
+        return a.cell && a.nb && a.kind && a.value && (!mask || a.mask);
+    }
+
+    static long long blocks(const LaunchCfg& c) { return (long long)c.grid.x * c.grid.y * c.grid.z; }
+
+    // Folds the per-block partials of a launch that wrote them.
+    static hipError_t reduce(const StepArgs& a, const LaunchCfg& used, bool mass, bool stat) {
+        hipLaunchKernelGGL(k_reduce_partials<0>, dim3(1), dim3(1024), 0, used.stream, a.part, blocks(used), a.jacc,
+                           mass ? a.macc : nullptr, stat ? a.dstat : nullptr);
+        return hipGetLastError();
+    }
+
     hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c) const override {
-        if (Q == 19) return mask ? fwd_q<19, true>(kind, a, c) : fwd_q<19, false>(kind, a, c);
-        return mask ? fwd_q<27, true>(kind, a, c) : fwd_q<27, false>(kind, a, c);
+        if (!ok_step(kind, mask, a)) return hipErrorInvalidValue;
+        LaunchCfg used = c;
+        hipError_t e;
+        if (Q == 19)
+            e = mask ? fwd_q<19, true>(kind, a, c, used) : fwd_q<19, false>(kind, a, c, used);
+        else
+            e = mask ? fwd_q<27, true>(kind, a, c, used) : fwd_q<27, false>(kind, a, c, used);
+        if (e != hipSuccess || !a.part) return e;
+        return reduce(a, used, a.diag != 0, a.diag != 0);
     }
     hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c) const override {
+        if (!ok_adj(kind, mask, a)) return hipErrorInvalidValue;
         if (Q == 19) return mask ? adj_q<19, true>(kind, a, c) : adj_q<19, false>(kind, a, c);
         return mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
     }
     hipError_t objective(int Q, bool mask, const StepArgs& a, const ObjArgs& seed, void* lam, double* abar,
                          int do_seed, const LaunchCfg& c) const override {
+        if (!a.pop || !a.alpha || !a.part || blocks(c) > a.npart || (mask && !a.mask) ||
+            (do_seed && (!lam || !abar)))
+            return hipErrorInvalidValue;
         if (Q == 19) {
-            if (mask) LBM_LAUNCH((k_objective<P, LP, 19, true>), c, a, seed, lam, abar, do_seed);
-            LBM_LAUNCH((k_objective<P, LP, 19, false>), c, a, seed, lam, abar, do_seed);
+            if (mask)
+                hipLaunchKernelGGL((k_objective<P, LP, 19, true>), c.grid, c.block, 0, c.stream, a, seed, lam, abar,
+                                   do_seed);
+            else
+                hipLaunchKernelGGL((k_objective<P, LP, 19, false>), c.grid, c.block, 0, c.stream, a, seed, lam, abar,
+                                   do_seed);
+        } else {
+            if (mask)
+                hipLaunchKernelGGL((k_objective<P, LP, 27, true>), c.grid, c.block, 0, c.stream, a, seed, lam, abar,
+                                   do_seed);
+            else
+                hipLaunchKernelGGL((k_objective<P, LP, 27, false>), c.grid, c.block, 0, c.stream, a, seed, lam, abar,
+                                   do_seed);
         }
-        if (mask) LBM_LAUNCH((k_objective<P, LP, 27, true>), c, a, seed, lam, abar, do_seed);
-        LBM_LAUNCH((k_objective<P, LP, 27, false>), c, a, seed, lam, abar, do_seed);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return reduce(a, c, true, false);
     }
     static LaunchCfg port_cfg(const PortArgs& a, hipStream_t s) {
         return LaunchCfg{dim3((a.np + 63) / 64), dim3(64), s};
     }
     hipError_t port_apply(int Q, bool mask, const PortArgs& a, void* pop, double* exch, hipStream_t s) const override {
+        if (!ok_ports(mask, a) || !pop) return hipErrorInvalidValue;
         const LaunchCfg c = port_cfg(a, s);
         if (Q == 19) {
             if (mask) LBM_LAUNCH((k_port_apply<P, 19, true>), c, a, pop, exch);
@@ -122,6 +250,7 @@ struct RunnerImpl final : Runner {
     hipError_t port_save(int Q, bool mask, const PortArgs& a, const void* pop, void* buf,
                          hipStream_t s) const override {
         using C = typename Store<P>::C;
+        if (!ok_ports(mask, a) || !pop || !buf) return hipErrorInvalidValue;
         const LaunchCfg c = port_cfg(a, s);
         C* b = static_cast<C*>(buf);
         if (Q == 19) {
@@ -134,6 +263,7 @@ struct RunnerImpl final : Runner {
     hipError_t port_vjp(int Q, bool mask, const PortArgs& a, void* lam, const void* buf,
                         hipStream_t s) const override {
         using C = typename Store<P>::C;
+        if (!ok_ports(mask, a) || !lam || !buf) return hipErrorInvalidValue;
         const LaunchCfg c = port_cfg(a, s);
         const C* b = static_cast<const C*>(buf);
         if (Q == 19) {

@@ -129,6 +129,7 @@ _SIGS = {
                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_step_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                      ctypes.c_int),
+    "lbm_reserve": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_get_fields": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_debug_link_table": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_port_cells": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
@@ -322,6 +323,12 @@ class Lattice:
         _check(lib().lbm_forward(self._h, steps, ctypes.byref(o) if o else None, ctypes.byref(J), ctypes.byref(st)))
         return J.value, st
 
+    def reserve(self, steps):
+        """Allocate the snapshot pool for a horizon ahead of time; returns the slot count."""
+        slots = ctypes.c_uint64(0)
+        _check(lib().lbm_reserve(self._h, steps, ctypes.byref(slots)))
+        return slots.value
+
     def gradient(self, steps, objective: Objective, want_lambda0=False):
         """Forward + discrete adjoint from the current state (binomial checkpointing).
         Returns (J, alpha_bar, lambda0 or None, stats)."""
@@ -362,7 +369,8 @@ class Lattice:
 
     def time_kernel(self, which="forward", launches=10):
         ms = ctypes.c_double(0.0)
-        _check(lib().lbm_time_kernel(self._h, 0 if which == "forward" else 1, launches, ctypes.byref(ms)))
+        code = {"forward": 0, "adjoint": 1, "stream": 2}[which]
+        _check(lib().lbm_time_kernel(self._h, code, launches, ctypes.byref(ms)))
         return ms.value
 
 

@@ -169,6 +169,11 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
 int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
                  uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats);
 
+/* Allocate the adjoint buffers and the revolve snapshot pool for a horizon of nsteps
+ * (min(budget / state bytes, nsteps) slots) ahead of lbm_gradient, so that the
+ * allocation is not part of a timed gradient.  lbm_gradient does this itself when needed. */
+int lbm_reserve(lbm_handle* h, uint64_t nsteps, uint64_t* slots);
+
 /* VJP of one step at the current state: lambda_prev = J^T lambda_next (AoS N*Q), and
  * alpha_bar (N) = d(lambda_next . step)/d alpha. */
 int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, double* alpha_bar, uint64_t len);
@@ -190,7 +195,8 @@ int lbm_revolve_schedule(uint64_t steps, uint64_t snapshots, int64_t* actions, u
 uint64_t lbm_revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots);
 
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
- * (0 forward step, 1 adjoint step) on the handle's stream, measured with HIP events. */
+ * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
+ * a copy) on the handle's stream, measured with HIP events. */
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
 /* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
 int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);

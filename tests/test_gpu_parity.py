@@ -38,17 +38,26 @@ def test_link_table_bit_exact(shape, periodic, Q, with_solid):
     assert np.array_equal(got, want)
 
 
+# nz divisible by 4 with a non-periodic z axis takes the vectorised 4-cells-per-thread
+# kernels; other shapes take the one-cell-per-thread kernels.  Both are covered.
 FWD_CASES = [
     # shape, periodic, Q, collision, with_solid, with_ports, accel
     ((8, 6, 5), (True, False, False), 19, "trt", False, False, (1e-4, 0, 0)),
+    ((8, 6, 8), (True, False, False), 19, "trt", False, False, (1e-4, 0, 0)),
     ((8, 6, 5), (True, False, False), 19, "bgk", True, False, (1e-4, 2e-5, 0)),
+    ((8, 6, 8), (True, False, False), 19, "bgk", True, False, (1e-4, 2e-5, 0)),
     ((8, 6, 5), (True, False, False), 19, "bgk_guo", True, False, (1e-4, 0, -3e-5)),
+    ((8, 6, 12), (True, False, False), 19, "bgk_guo", False, False, (1e-4, 0, -3e-5)),
     ((7, 6, 5), (False, False, False), 19, "trt", True, True, (0, 0, 0)),
+    ((7, 6, 8), (False, False, False), 19, "trt", True, True, (0, 0, 0)),
     ((7, 5, 6), (True, True, False), 19, "mrt", False, False, (1e-4, 0, 0)),
+    ((7, 5, 8), (True, True, False), 19, "mrt", False, False, (1e-4, 0, 0)),
     ((6, 5, 4), (True, False, False), 27, "trt", True, False, (1e-4, 0, 0)),
     ((6, 5, 4), (True, False, False), 27, "mrt", False, False, (1e-4, 0, 0)),
-    ((6, 5, 4), (True, False, False), 27, "cumulant", True, False, (1e-4, 0, 0)),
+    ((6, 5, 5), (True, False, False), 27, "cumulant", True, False, (1e-4, 0, 0)),
+    ((6, 5, 8), (True, False, False), 27, "cumulant", True, False, (1e-4, 0, 0)),
     ((7, 5, 4), (False, False, False), 27, "cumulant", False, True, (0, 0, 0)),
+    ((6, 5, 7), (True, True, True), 19, "trt", True, False, (1e-4, 1e-5, 2e-5)),
 ]
 
 
@@ -84,7 +93,13 @@ def test_forward_f64_bit_exact(shape, periodic, Q, coll, with_solid, with_ports,
 def test_forward_reduced_precision(precision, tol, Q, coll):
     if precision == "bf16s" and coll not in ("trt", "bgk"):
         pytest.skip("BF16 storage supports BGK and TRT only")
-    shape = (8, 6, 5)
+    if precision == "f32" and coll == "cumulant":
+        # The cumulant collision works on full populations f = w + d in FP32 (the
+        # chimera transform and the Wick closure are not linear in d), so each step
+        # rounds at ulp(w) ~ 3e-8 against deviations of ~1e-3: 20 steps leave a few
+        # 1e-4 of the deviation scale. TRT/MRT stay in shifted form and meet 2e-5.
+        tol = 1e-3
+    shape = (8, 6, 8)
     got, want, o = _run_both(shape, (True, False, False), Q, coll, False, False, (1e-4, 0, 0), precision, 20)
     _, w, _ = orc.lattice(Q)
     dev_w = want.reshape(-1, Q) - w
@@ -116,12 +131,16 @@ def test_revolve_schedule_matches_reference(n, s):
 
 VJP_CASES = [
     ((7, 6, 5), (True, False, False), 19, "trt", True, False),
+    ((7, 6, 8), (True, False, False), 19, "trt", True, False),
     ((7, 6, 5), (False, False, False), 19, "trt", False, True),
+    ((7, 6, 8), (False, False, False), 19, "trt", False, True),
     ((7, 6, 5), (True, False, False), 19, "bgk_guo", True, False),
     ((6, 5, 6), (True, True, False), 19, "mrt", False, False),
+    ((6, 5, 8), (True, True, False), 19, "mrt", False, False),
     ((6, 5, 4), (True, False, False), 27, "trt", True, False),
     ((6, 5, 4), (True, False, False), 27, "cumulant", False, False),
     ((7, 5, 4), (False, False, False), 27, "cumulant", False, True),
+    ((7, 5, 5), (False, False, False), 27, "cumulant", False, True),
 ]
 
 
@@ -160,6 +179,8 @@ def _objective(kind, steps, shape):
 
 GRAD_CASES = [
     ((8, 6, 5), (True, False, False), 19, "trt", "dissipation", False, 0),
+    ((8, 6, 8), (True, False, False), 19, "trt", "dissipation", False, 6),
+    ((8, 6, 8), (False, False, False), 19, "trt", "pressure_drop", True, 4),
     ((8, 6, 5), (False, False, False), 19, "trt", "pressure_drop", True, 0),
     ((8, 6, 5), (False, False, False), 19, "trt", "pressure_drop", True, 3),
     ((8, 6, 5), (True, False, False), 19, "mrt", "dissipation", False, 4),
