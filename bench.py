@@ -27,16 +27,24 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per lattice update (populations read+written once, alpha, and for
-# the adjoint the state read, cotangent read+written and an FP64 alpha_bar update).
-BYTES = {
-    ("forward", "f64"): 19 * 8 * 2 + 8,
-    ("forward", "f32"): 19 * 4 * 2 + 4,
-    ("forward", "f16s"): 19 * 2 * 2 + 4,
-    ("adjoint", "f64"): 19 * 8 + 19 * 8 * 2 + 8 + 16,
-    ("adjoint", "f32"): 19 * 4 + 19 * 4 * 2 + 4 + 16,
-    ("adjoint", "f16s"): 19 * 2 + 19 * 4 * 2 + 4 + 16,
+# Variants: name -> (state storage, cotangent storage, state bytes/value, cotangent
+# bytes/value, alpha bytes).  f16s keeps FP16 cotangents (scaled), f16s_l32 FP32 ones.
+VARIANTS = {
+    "f64": ("f64", "auto", 8, 8, 8),
+    "f32": ("f32", "auto", 4, 4, 4),
+    "f16s": ("f16s", "f16", 2, 2, 4),
+    "f16s_l32": ("f16s", "f32", 2, 4, 4),
 }
+
+
+def algorithmic_bytes(which, variant, Q=19):
+    """Bytes per lattice update: the forward step reads and writes every population once
+    and reads alpha; the adjoint step reads the state, reads and writes every cotangent,
+    reads alpha and updates the FP64 alpha_bar (16 B)."""
+    _, _, s, l, a = VARIANTS[variant]
+    if which == "forward":
+        return Q * s * 2 + a
+    return Q * s + Q * l * 2 + a + 16
 
 
 def log(msg):
@@ -121,8 +129,9 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this path does not shard; "
-              "see DESIGN.md for how it would)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the lattice step runs on one "
+              "GPU; slab decomposition along x with a per-step halo exchange is designed, not built)",
+              file=sys.stderr)
         sys.exit(2)
     shape = tuple(args.shape)
     budget = int(args.budget_gb * 1e9)
@@ -146,9 +155,9 @@ def main():
             ts = lat.time_kernel("stream", args.launches)
         n = int(np.prod(shape))
         out = {"precision": args.kernel_precision, "forward_ms": tf, "adjoint_ms": ta, "stream_only_ms": ts,
-               "forward_GBs": BYTES[("forward", args.kernel_precision)] * n / (tf * 1e-3) / 1e9,
-               "adjoint_GBs": BYTES[("adjoint", args.kernel_precision)] * n / (ta * 1e-3) / 1e9,
-               "stream_only_GBs": BYTES[("forward", args.kernel_precision)] * n / (ts * 1e-3) / 1e9,
+               "forward_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (tf * 1e-3) / 1e9,
+               "adjoint_GBs": algorithmic_bytes("adjoint", args.kernel_precision) * n / (ta * 1e-3) / 1e9,
+               "stream_only_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (ts * 1e-3) / 1e9,
                "cells_per_thread": os.environ.get("LBM_CELLS_PER_THREAD", "default")}
         print(json.dumps(out))
         return
@@ -171,7 +180,7 @@ def main():
 
     def kern(prec, which):
         t = res[prec]["t_fwd_ms" if which == "forward" else "t_adj_ms"]
-        gbs = BYTES[(which, prec)] * n / (t * 1e-3) / 1e9
+        gbs = algorithmic_bytes(which, prec) * n / (t * 1e-3) / 1e9
         return t, gbs
 
     # dominant kernel of the measured run: forward (incl. recomputation) or adjoint
@@ -198,7 +207,7 @@ def main():
                                f"{args.budget_gb:g} GB", "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": gbs_dom, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs_dom / HBM_PEAK_GBS, "traffic": None, "kernel": f"{dom} ({main_prec})",
-                     "algorithmic_bytes_per_lu": BYTES[(dom, main_prec)], "avg_kernel_ms": t_dom},
+                     "algorithmic_bytes_per_lu": algorithmic_bytes(dom, main_prec), "avg_kernel_ms": t_dom},
     }
     per = {}
     for prec, rr in res.items():
@@ -213,6 +222,8 @@ def main():
     out["per_precision"] = per
     if "f32" in res and "f16s" in res:
         out["speedup_f16s_vs_f32"] = res["f32"]["wall"] / res["f16s"]["wall"]
+    if "f32" in res and "f16s_l32" in res:
+        out["speedup_f16s_l32_vs_f32"] = res["f32"]["wall"] / res["f16s_l32"]["wall"]
     if "f64" in res:
         g64 = res["f64"]["alpha_bar"]
         errs = {}
@@ -238,16 +249,17 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def _cfg_only(shape, precision, budget):
+def _cfg_only(shape, variant, budget):
     from implexity_amd import design
     from implexity_amd.lbm import LatticeConfig, Port
 
+    precision, adjoint_storage = VARIANTS[variant][:2]
     nx, ny, nz = shape
     prof = design.parabolic_profile(ny, nz).reshape(-1)
     ports = [Port("xmin", "velocity", (0.03, 0.0, 0.0), prof), Port("xmax", "pressure", (1.0,))]
     return LatticeConfig(shape=shape, periodic=(False, False, False), lattice=19, collision="trt",
                          precision=precision, omega=1.0 / 0.55, magic=3.0 / 16.0, ports=ports,
-                         checkpoint_budget_bytes=budget)
+                         checkpoint_budget_bytes=budget, adjoint_storage=adjoint_storage)
 
 
 if __name__ == "__main__":

@@ -27,6 +27,7 @@ UNITS = {
     "kern_f64.hip": ["-ffp-contract=off"],
     "kern_f32.hip": [],
     "kern_f16s.hip": [],
+    "kern_f16s_l16.hip": [],
     "kern_bf16s.hip": [],
     "api.hip": ["-ffp-contract=off"],
     "revolve.cpp": ["-x", "hip"],

@@ -82,6 +82,18 @@ LBM_DEV void moments(const C* f, C& rho, C j[3]) {
 
 // Brinkman macroscopic quantities (psm.rs:213-222 with zero solid momentum, equal in
 // value to solver.rs:190-196): u = (j/rho + a/2)/(1 + alpha/2), F = rho (a - alpha u).
+// a / b: an IEEE division in FP64, keeping the reference's operations bit for bit; in
+// FP32 a multiply by the hardware reciprocal (1 ulp), shared by the compiler between
+// divisions by the same value (an FP32 IEEE division is about ten instructions).
+template <class C>
+LBM_DEV C qdiv(C a, C b) {
+    return a / b;
+}
+template <>
+LBM_DEV float qdiv<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 template <int Q, class C>
 struct Macro {
     C rho, drho, j[3], den, sv[3], u[3], F[3];  // drho = rho - 1
@@ -94,8 +106,8 @@ LBM_DEV void brinkman_macro(const C* f, C alpha, const C a[3], Macro<Q, C>& m) {
     m.den = C(1) + alpha * C(0.5);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        m.sv[d] = m.j[d] / m.rho + a[d] * C(0.5);
-        m.u[d] = m.sv[d] / m.den;
+        m.sv[d] = qdiv(m.j[d], m.rho) + a[d] * C(0.5);
+        m.u[d] = qdiv(m.sv[d], m.den);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) m.F[d] = m.rho * (a[d] + (-(alpha * m.u[d])));
@@ -216,10 +228,10 @@ LBM_DEV void brinkman_chain(const Macro<Q, C>& m, C alpha, const C a[3], C cb_rh
     C j_bar[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const C s_bar = u_bar[d] / m.den;
-        ab -= u_bar[d] * m.sv[d] / (m.den * m.den) * C(0.5);
-        j_bar[d] = s_bar / m.rho;
-        rho_bar -= s_bar * m.j[d] / (m.rho * m.rho);
+        const C s_bar = qdiv(u_bar[d], m.den);
+        ab -= qdiv(u_bar[d] * m.sv[d], m.den * m.den) * C(0.5);
+        j_bar[d] = qdiv(s_bar, m.rho);
+        rho_bar -= qdiv(s_bar * m.j[d], m.rho * m.rho);
     }
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -490,8 +502,8 @@ LBM_DEV void brinkman_macro_shifted(const C* d, C alpha, const C a[3], Macro<Q, 
     m.den = C(1) + alpha * C(0.5);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        m.sv[k] = m.j[k] / m.rho + a[k] * C(0.5);
-        m.u[k] = m.sv[k] / m.den;
+        m.sv[k] = qdiv(m.j[k], m.rho) + a[k] * C(0.5);
+        m.u[k] = qdiv(m.sv[k], m.den);
     }
 #pragma unroll
     for (int k = 0; k < 3; k++) m.F[k] = m.rho * (a[k] - alpha * m.u[k]);

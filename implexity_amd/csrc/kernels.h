@@ -30,6 +30,7 @@
 //   applies the transposed collision Jacobian, and writes f_bar_q(x) at the forward
 //   read location of (x,q): solver.rs:642-674 (stream_vjp + collide_vjp) in one pass.
 #pragma once
+#include <type_traits>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -76,10 +77,13 @@ struct StepArgs {
     int odd;
     int diag;         // 1: accumulate diagnostics
     ObjArgs obj;      // objective of the state being read (kind 0: none)
-    double* part;
-    long long npart;  // capacity of part, in blocks     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
-    double* jacc;     // objective accumulator      (written by k_reduce_partials)
-    double* macc;     // fluid mass of the state    (written by k_reduce_partials)
+    double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
+    long long npart;  // capacity of part, in blocks
+    double* stage;    // FOLD_BLOCKS x 4 fold stage and its ticket counter (k_fold)
+    unsigned* ticket;
+    int* lsc;         // scale record of FP16 cotangents (seed writes), else null
+    double* jacc;     // objective accumulator      (written by k_fold)
+    double* macc;     // fluid mass of the state    (written by k_fold)
     unsigned* dstat;  // [0] max |u|^2 (float bits), [1] bad cells (non-finite or f <= 0)
 };
 
@@ -95,6 +99,14 @@ struct AdjArgs {
     const void* mrt;
     int odd;
     ObjArgs obj;  // stage cost of the state being read (added to its cotangent)
+    // Scaled FP16 cotangents only: per-block max |lambda| partials, the scale record and
+    // the bound on the stage-cost seed of the following reverse step.
+    double* part;
+    long long npart;
+    double* stage;
+    unsigned* ticket;
+    int* lsc;
+    double s_next;
 };
 
 // ---------------------------------------------------------------------------------
@@ -222,49 +234,123 @@ LBM_DEV void from_full(const C* f, C* v) {
 // Cotangent (adjoint) storage: plain, no shift.
 template <int P>
 struct LamStore;
+// Scale of FP16 cotangents: lambda is stored as lambda * 2^e with one exponent for the
+// whole field.  The record lsc[4] on the device holds
+//   lsc[0]  e of the field as stored now,
+//   lsc[1]  e for the next whole-field write (an adjoint step),
+//   lsc[2]  max |lambda| of the last adjoint step (float bits),
+//   lsc[3]  count of adjoint steps whose scaled values left the FP16 range.
+// Readers decode with lsc[0]; an adjoint step writes with lsc[1], after which
+// k_fold moves lsc[1] into lsc[0] and picks the next exponent from the measured
+// max.  Writers that do not replace the whole field (ports, seeds) keep lsc[0].
+struct LScale {
+    float dec, enc;
+};
+template <int LP>
+LBM_DEV LScale lam_scale(const int* lsc, bool whole_write) {
+    if constexpr (LP != P_F16S) {
+        return LScale{1.0f, 1.0f};
+    } else {
+        const int ec = lsc[0];
+        return LScale{ldexpf(1.0f, -ec), ldexpf(1.0f, whole_write ? lsc[1] : ec)};
+    }
+}
+
+// Growth allowed between two measured maxima (one port pass plus one transposed step),
+// and the exponent that maps a bound b on |lambda| to at most 2^14 (FP16 overflows at
+// 65504 ~ 2^16; values down to 2^-14 stay normal, 28 binades below the bound).
+constexpr double LAM_GROWTH = 256.0;
+LBM_HD inline int lam_exponent(double b) {
+    if (!(b > 0.0) || !(b < 1e300)) return 0;
+    int ex = 0;
+    (void)frexp(b, &ex);  // b < 2^ex
+    const int e = 14 - ex;
+    return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
 template <>
 struct LamStore<P_F64> {
     using T = double;
     template <class C>
-    static LBM_DEV C ld(const T* p) {
+    static LBM_DEV C ld(const T* p, LScale = LScale{1.0f, 1.0f}) {
         return C(*p);
     }
     template <class C>
-    static LBM_DEV void st(T* p, C v) {
+    static LBM_DEV void st(T* p, C v, LScale = LScale{1.0f, 1.0f}) {
         *p = double(v);
+    }
+    template <class C>
+    static LBM_DEV T enc(C v, LScale = LScale{1.0f, 1.0f}) {
+        return double(v);
     }
 };
 template <>
 struct LamStore<P_F32> {
     using T = float;
     template <class C>
-    static LBM_DEV C ld(const T* p) {
+    static LBM_DEV C ld(const T* p, LScale = LScale{1.0f, 1.0f}) {
         return C(*p);
     }
     template <class C>
-    static LBM_DEV void st(T* p, C v) {
+    static LBM_DEV void st(T* p, C v, LScale = LScale{1.0f, 1.0f}) {
         *p = float(v);
+    }
+    template <class C>
+    static LBM_DEV T enc(C v, LScale = LScale{1.0f, 1.0f}) {
+        return float(v);
+    }
+};
+// FP16 cotangent times 2^e (LScale carries 2^-e for loads and the write exponent).
+template <>
+struct LamStore<P_F16S> {
+    using T = _Float16;
+    template <class C>
+    static LBM_DEV C ld(const T* p, LScale s) {
+        return C(float(*p) * s.dec);
+    }
+    template <class C>
+    static LBM_DEV void st(T* p, C v, LScale s) {
+        *p = _Float16(float(v) * s.enc);
+    }
+    template <class C>
+    static LBM_DEV T enc(C v, LScale s) {
+        return _Float16(float(v) * s.enc);
     }
 };
 
 // ---------------------------------------------------------------------------------
 // Addressing.
 // ---------------------------------------------------------------------------------
+// Storage indices are 32-bit: the host refuses lattices whose population arrays exceed
+// 4 GiB, so (index * element size) is a 32-bit byte offset from a wave-uniform array
+// base, which the compiler turns into SGPR-base + VGPR-offset global accesses with no
+// per-access 64-bit address arithmetic.
 struct Cursor {
-    long long base;       // storage index of the cell
-    long long dx, dy, dz;  // storage offsets of the +x, +y, +z neighbours (wrapped)
-    LBM_DEV long long at(int corner) const {
-        return base + ((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0) + ((corner & 4) ? dz : 0);
+    uint32_t base;          // storage index of the cell
+    int32_t dx, dy, dz;     // storage offsets of the +x, +y, +z neighbours (wrapped)
+    LBM_DEV uint32_t at(int corner) const {
+        return base + uint32_t(((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0) + ((corner & 4) ? dz : 0));
     }
 };
 
 LBM_DEV Cursor make_cursor(const Geom& g, int i, int j, int k) {
     Cursor c;
-    c.base = (long long)i * g.sxy + (long long)j * g.SZ + k;
-    c.dx = (g.px && i + 1 == g.nx) ? -(long long)(g.nx - 1) * g.sxy : g.sxy;
-    c.dy = (g.py && j + 1 == g.ny) ? -(long long)(g.ny - 1) * g.SZ : (long long)g.SZ;
-    c.dz = (g.pz && k + 1 == g.nz) ? -(long long)(g.nz - 1) : 1;
+    c.base = uint32_t(i) * uint32_t(g.sxy) + uint32_t(j) * uint32_t(g.SZ) + uint32_t(k);
+    c.dx = (g.px && i + 1 == g.nx) ? -int32_t(g.nx - 1) * int32_t(g.sxy) : int32_t(g.sxy);
+    c.dy = (g.py && j + 1 == g.ny) ? -int32_t(g.ny - 1) * int32_t(g.SZ) : int32_t(g.SZ);
+    c.dz = (g.pz && k + 1 == g.nz) ? -int32_t(g.nz - 1) : 1;
     return c;
+}
+
+// Population array `arr` of a field (uniform) and one element of it by storage index.
+template <class T>
+LBM_DEV T* arr_base(T* pop, const Geom& g, int arr) {
+    return pop + size_t(arr) * size_t(g.stride);
+}
+template <class T>
+LBM_DEV T& elem(T* base, uint32_t idx) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return *reinterpret_cast<T*>(reinterpret_cast<B*>(base) + size_t(idx * uint32_t(sizeof(T))));
 }
 
 // Blocked-link mask of a cell from the domain faces alone (no solid cells).
@@ -311,12 +397,40 @@ LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
     return (long long)arr * g.stride + cu.at(corner_pos<Q>(q));
 }
 
+// Element at the read location of f_q (see read_loc): the slot at x + a_opp(q) in the
+// wave-uniform array of an open link, or in the opposite array for a blocked link
+// (halfway bounce-back).  The array choice is a select between two uniform bases.
+template <int Q, int q, class T>
+LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+    const int arr_open = odd ? q : opp<Q>(q);
+    if constexpr (q == 0) {
+        return elem(arr_base(pop, g, arr_open), cu.at(corner_neg<Q>(q)));
+    } else {
+        const int arr_blocked = odd ? opp<Q>(q) : q;
+        T* b = ((m >> q) & 1u) ? arr_base(pop, g, arr_blocked) : arr_base(pop, g, arr_open);
+        return elem(b, cu.at(corner_neg<Q>(q)));
+    }
+}
+template <int Q, int q, class T>
+LBM_DEV T load_read(const T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+    return at_read<Q, q>(pop, g, cu, m, odd);
+}
+template <int Q, int q, class T>
+LBM_DEV void store_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd, T v) {
+    at_read<Q, q>(pop, g, cu, m, odd) = v;
+}
+// Element at the write location of f*_q (see write_loc).
+template <int Q, int q, class T>
+LBM_DEV T& at_write(T* pop, const Geom& g, const Cursor& cu, int odd) {
+    return elem(arr_base(pop, g, odd ? opp<Q>(q) : q), cu.at(corner_pos<Q>(q)));
+}
+
 template <int P, int Q>
 LBM_DEV void load_cell(const Geom& g, const typename Store<P>::T* pop, const Cursor& cu, uint32_t m, int odd,
                        typename Store<P>::C* f) {
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        f[q] = Store<P>::template ld<Q, q>(pop + read_loc<Q, q>(g, cu, m, odd));
+        f[q] = Store<P>::template dec<Q, q>(load_read<Q, q>(pop, g, cu, m, odd));
     });
 }
 
@@ -325,7 +439,7 @@ LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cur
                              const typename Store<P>::C* f) {
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        Store<P>::template st<Q, q>(pop + read_loc<Q, q>(g, cu, m, odd), f[q]);
+        store_read<Q, q>(pop, g, cu, m, odd, Store<P>::template enc<Q, q>(f[q]));
     });
 }
 
@@ -350,7 +464,7 @@ LBM_DEV long long block_linear() {
 
 // Per-block partials of the objective, the fluid mass, max |u|^2 and the count of bad
 // cells, written without atomics (a single address hit by every block serialises in
-// L2); k_reduce_partials folds them, in a fixed order, after the step.
+// L2); k_fold combines them, in a fixed order, after the step.
 LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* part) {
     __shared__ double red[4][16];
     jv = wave_sum(jv);
@@ -382,18 +496,51 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
     }
 }
 
-// One block of 1024 threads folds nb partials: jacc += sum J, *macc = sum mass (if
-// set), dstat[0] = max(dstat[0], max |u|^2), dstat[1] += bad cells.
-template <int UNUSED = 0>
-__global__ __launch_bounds__(1024) void k_reduce_partials(const double* part, long long nb, double* jacc,
-                                                          double* macc, unsigned* dstat) {
-    __shared__ double red[4][16];
+
+// Block max of |lambda| into part[4 * block + 2] (FP16 cotangent steps).
+LBM_DEV void block_max_partial(float v, double* part) {
+    __shared__ float red[16];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        float m = 0.0f;
+        for (int w = 0; w < nw; w++) m = fmaxf(m, red[w]);
+        part[4 * block_linear() + 2] = double(m);
+    }
+}
+
+// Folds the per-block partials of one launch (part[4 b + 0..3] = objective, mass,
+// max, bad-cell count) in two fixed-order stages: fold block b sums its contiguous chunk
+// into stage[4 b ..]; the last fold block to finish (a ticket counter, released with
+// device-scope fences) combines the stage values in block order, writes the results
+// and resets the ticket.  Deterministic for a given launch shape.
+//   FOLD_STEP:   jacc += objective; *macc = mass (if set); dstat[0] = max(dstat[0],
+//                max |u|^2) and dstat[1] += bad (if set).
+//   FOLD_LAMBDA: max |lambda| of an FP16 cotangent step: the field now has exponent
+//                lsc[1]; record the max, count an overflow, and choose the next write
+//                exponent from LAM_GROWTH * max + s_next.
+constexpr int FOLD_BLOCKS = 64;
+enum FoldMode { FOLD_STEP = 0, FOLD_LAMBDA = 1 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, double* stage, unsigned* ticket,
+                                             double* jacc, double* macc, unsigned* dstat, int* lsc, double s_next) {
+    __shared__ double red[4][4];
+    __shared__ bool last;
+    const long long chunk = (nb + gridDim.x - 1) / gridDim.x;
+    const long long lo = (long long)blockIdx.x * chunk;
+    const long long hi = lo + chunk < nb ? lo + chunk : nb;
     double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-    for (long long i = threadIdx.x; i < nb; i += blockDim.x) {
-        a += part[4 * i];
-        b += part[4 * i + 1];
+    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        if (MODE == FOLD_STEP) {
+            a += part[4 * i];
+            b += part[4 * i + 1];
+            d += part[4 * i + 3];
+        }
         c = fmax(c, part[4 * i + 2]);
-        d += part[4 * i + 3];
     }
     a = wave_sum(a);
     b = wave_sum(b);
@@ -416,14 +563,40 @@ __global__ __launch_bounds__(1024) void k_reduce_partials(const double* part, lo
             sc = fmax(sc, red[2][w]);
             sd += red[3][w];
         }
-        if (jacc) *jacc += sa;
-        if (macc) *macc = sb;
-        if (dstat) {
-            const unsigned ub = __float_as_uint(float(sc));
-            if (ub > dstat[0]) dstat[0] = ub;
-            dstat[1] += unsigned(sd);
-        }
+        stage[4 * blockIdx.x] = sa;
+        stage[4 * blockIdx.x + 1] = sb;
+        stage[4 * blockIdx.x + 2] = sc;
+        stage[4 * blockIdx.x + 3] = sd;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    double ta = 0.0, tb = 0.0, tc = 0.0, td = 0.0;
+    for (unsigned k = 0; k < gridDim.x; k++) {
+        ta += __hip_atomic_load(&stage[4 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tb += __hip_atomic_load(&stage[4 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tc = fmax(tc, __hip_atomic_load(&stage[4 * k + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        td += __hip_atomic_load(&stage[4 * k + 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (MODE == FOLD_STEP) {
+        if (jacc) *jacc += ta;
+        if (macc) *macc = tb;
+        if (dstat) {
+            const unsigned ub = __float_as_uint(float(tc));
+            if (ub > dstat[0]) dstat[0] = ub;
+            dstat[1] += unsigned(td);
+        }
+    } else {
+        const float mx = float(tc);
+        const int ew = lsc[1];
+        if (!(double(mx) * ldexp(1.0, ew) <= 65504.0)) lsc[3] = lsc[3] + 1;
+        lsc[0] = ew;
+        lsc[2] = __float_as_int(mx);
+        lsc[1] = lam_exponent(LAM_GROWTH * double(mx) + s_next);
+    }
+    *ticket = 0u;
 }
 
 // ---------------------------------------------------------------------------------
@@ -462,10 +635,10 @@ LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, in
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const C ub = usq_bar * C(2.0) * m.u[d];
-            al_bar -= ub * m.sv[d] / (m.den * m.den) * C(0.5);
-            const C sb = ub / m.den;
-            jb[d] = sb / m.rho;
-            rho_bar -= sb * m.j[d] / (m.rho * m.rho);
+            al_bar -= qdiv(ub * m.sv[d], m.den * m.den) * C(0.5);
+            const C sb = qdiv(ub, m.den);
+            jb[d] = qdiv(sb, m.rho);
+            rho_bar -= qdiv(sb * m.j[d], m.rho * m.rho);
         }
         static_for<0, Q>([&](auto I) {
             constexpr int i = decltype(I)::value;
@@ -598,7 +771,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             typename S::T* out = static_cast<typename S::T*>(A.pop_out);
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
-                S::template st<Q, q>(out + write_loc<Q, q>(g, cu, A.odd), fs[q]);
+                at_write<Q, q>(out, g, cu, A.odd) = S::template enc<Q, q>(fs[q]);
             });
         }
     }
@@ -616,34 +789,45 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
-    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
-    if (m & SOLID_BIT) return;
-    const Cursor cu = make_cursor(g, i, j, k);
-    const typename S::T* pop = static_cast<const typename S::T*>(A.pop);
-    typename L::T* lam = static_cast<typename L::T*>(A.lam);
-    C f[Q], gb[Q], fb[Q];
-    load_cell<P, Q>(g, pop, cu, m, A.odd, f);
-    static_for<0, Q>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        gb[q] = L::template ld<C>(lam + write_loc<Q, q>(g, cu, A.odd));
-    });
-    const C alpha = static_cast<const C*>(A.alpha)[cell];
-    const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
-    Macro<Q, C> mac;
-    C ab;
-    collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau), static_cast<const C*>(A.mrt), gb,
-                                        mac, fb, ab);
-    if (A.obj.kind != OBJ_NONE) {
-        const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
-        objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+    float lmax = 0.0f;  // max |lambda| written (FP16 cotangents; a NaN makes it inf)
+    if (k < g.nz) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
+        if (!(m & SOLID_BIT)) {
+            const LScale sc = lam_scale<LP>(A.lsc, true);
+            const Cursor cu = make_cursor(g, i, j, k);
+            const typename S::T* pop = static_cast<const typename S::T*>(A.pop);
+            typename L::T* lam = static_cast<typename L::T*>(A.lam);
+            C f[Q], gb[Q], fb[Q];
+            load_cell<P, Q>(g, pop, cu, m, A.odd, f);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                gb[q] = L::template ld<C>(&at_write<Q, q>(lam, g, cu, A.odd), sc);
+            });
+            const C alpha = static_cast<const C*>(A.alpha)[cell];
+            const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+            Macro<Q, C> mac;
+            C ab;
+            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
+            if (A.obj.kind != OBJ_NONE) {
+                const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+            }
+            A.alpha_bar[cell] += double(ab);
+            float nan_probe = 0.0f;
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                store_read<Q, q>(lam, g, cu, m, A.odd, L::template enc<C>(fb[q], sc));
+                if constexpr (LP == P_F16S) {
+                    lmax = fmaxf(lmax, fabsf(float(fb[q])));
+                    nan_probe += float(fb[q]);
+                }
+            });
+            if (nan_probe != nan_probe) lmax = INFINITY;
+        }
     }
-    A.alpha_bar[cell] += double(ab);
-    static_for<0, Q>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        L::template st<C>(lam + read_loc<Q, q>(g, cu, m, A.odd), fb[q]);
-    });
+    if constexpr (LP == P_F16S) block_max_partial(lmax, A.part);
 }
 
 
@@ -804,6 +988,7 @@ __global__ __launch_bounds__(128) void k_forward_v(StepArgs A) {
 
 template <int P, int LP, int Q, int KIND, bool MASK, int V>
 __global__ __launch_bounds__(128) void k_adjoint_v(AdjArgs A) {
+    static_assert(LP != P_F16S, "scaled FP16 cotangents take the per-cell adjoint kernel");
     using S = Store<P>;
     using L = LamStore<LP>;
     using C = typename S::C;
@@ -945,9 +1130,10 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
                 }
                 alpha_bar[cell] = double(ab);
                 typename L::T* lp = static_cast<typename L::T*>(lam);
+                const LScale sc = lam_scale<LP>(A.lsc, false);
                 static_for<0, Q>([&](auto I) {
                     constexpr int q = decltype(I)::value;
-                    L::template st<C>(lp + read_loc<Q, q>(g, cu, m, A.odd), fb[q]);
+                    L::template st<C>(lp + read_loc<Q, q>(g, cu, m, A.odd), fb[q], sc);
                 });
             }
         } else if (do_seed) {
@@ -977,7 +1163,7 @@ LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out
     C rho_n, j[3], u_n[3];
     moments<Q>(nb, rho_n, j);
 #pragma unroll
-    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    for (int d = 0; d < 3; d++) u_n[d] = qdiv(j[d], rho_n);
     C eb[Q], en[Q];
     if (kind == 0) {
         const C ub[3] = {C(value[0]), C(value[1]), C(value[2])};
@@ -995,7 +1181,7 @@ LBM_DEV void port_populations_vjp(const C* nb, int kind, const double* value, co
     C rho_n, j[3], u_n[3];
     moments<Q>(nb, rho_n, j);
 #pragma unroll
-    for (int d = 0; d < 3; d++) u_n[d] = j[d] / rho_n;
+    for (int d = 0; d < 3; d++) u_n[d] = qdiv(j[d], rho_n);
     C rb_bar = C(0), ub_bar[3] = {C(0), C(0), C(0)};
     if (kind == 0) {
         const C ub[3] = {C(value[0]), C(value[1]), C(value[2])};
@@ -1017,8 +1203,8 @@ LBM_DEV void port_populations_vjp(const C* nb, int kind, const double* value, co
     C j_bar[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        j_bar[d] = un_bar[d] / rho_n;
-        rn_bar -= un_bar[d] * j[d] / (rho_n * rho_n);
+        j_bar[d] = qdiv(un_bar[d], rho_n);
+        rn_bar -= qdiv(un_bar[d] * j[d], rho_n * rho_n);
     }
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -1031,10 +1217,11 @@ LBM_DEV void port_populations_vjp(const C* nb, int kind, const double* value, co
 }
 
 LBM_DEV void decode_cell(const Geom& g, long long cell, int& i, int& j, int& k) {
-    k = int(cell % g.nz);
-    const long long ij = cell / g.nz;
-    j = int(ij % g.ny);
-    i = int(ij / g.ny);
+    const uint32_t c = uint32_t(cell);  // cell < 2^31
+    const uint32_t ij = c / uint32_t(g.nz);
+    k = int(c - ij * uint32_t(g.nz));
+    i = int(ij / uint32_t(g.ny));
+    j = int(ij - uint32_t(i) * uint32_t(g.ny));
 }
 
 template <int Q, bool MASK>
@@ -1065,10 +1252,10 @@ __global__ void k_port_apply(PortArgs A, void* pop_, double* exch_sum) {
         from_full<S::shifted, Q>(fout, out);
         store_cell_read<P, Q>(A.g, pop, cx, mx, A.odd, out);
         // Mass exchange of the stored (encoded) values, summed in order of q.
-        C newv[Q];
-        load_cell<P, Q>(A.g, pop, cx, mx, A.odd, newv);
-#pragma unroll
-        for (int q = 0; q < Q; q++) delta += double(newv[q]) - double(old[q]);
+        static_for<0, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            delta += double(S::template dec<Q, q>(S::template enc<Q, q>(out[q]))) - double(old[q]);
+        });
     }
     if (exch_sum) {
         delta = wave_sum(delta);
@@ -1076,7 +1263,7 @@ __global__ void k_port_apply(PortArgs A, void* pop_, double* exch_sum) {
     }
 }
 
-// Save the populations of every port neighbour (state of parity A.odd) to buf[r*Q+q].
+// Save the populations of every port neighbour (state of parity A.odd) to buf[q*np+r].
 template <int P, int Q, bool MASK>
 __global__ void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* buf) {
     using S = Store<P>;
@@ -1089,17 +1276,18 @@ __global__ void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* 
     load_cell<P, Q>(A.g, static_cast<const typename S::T*>(pop_), cn, mn, A.odd, vnb);
     to_full<S::shifted, Q>(vnb, fnb);
 #pragma unroll
-    for (int q = 0; q < Q; q++) buf[(long long)r * Q + q] = fnb[q];
+    for (int q = 0; q < Q; q++) buf[(long long)q * A.np + r] = fnb[q];
 }
 
 // Adjoint of the port pass on the cotangent of a state of parity A.odd:
 // lam(nb) += J_NEE(f_nb)^T lam(port); lam(port) := 0   (solver.rs:642-656).
 template <int P, int LP, int Q, bool MASK>
-__global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf) {
+__global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf, const int* lsc) {
     using L = LamStore<LP>;
     using C = typename Store<P>::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= A.np) return;
+    const LScale sc = lam_scale<LP>(lsc, false);
     typename L::T* lam = static_cast<typename L::T*>(lam_);
     Cursor cx, cn;
     const uint32_t mx = mask_of<Q, MASK>(A, A.cell[r], cx);
@@ -1107,16 +1295,16 @@ __global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* b
     C gx[Q], gn[Q], fnb[Q], nbb[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        gx[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd));
-        gn[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd));
+        gx[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), sc);
+        gn[q] = L::template ld<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), sc);
     });
 #pragma unroll
-    for (int q = 0; q < Q; q++) fnb[q] = buf[(long long)r * Q + q];
+    for (int q = 0; q < Q; q++) fnb[q] = buf[(long long)q * A.np + r];
     port_populations_vjp<Q, C>(fnb, A.kind[r], A.value + 3 * r, gx, nbb);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        L::template st<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), gn[q] + nbb[q]);
-        L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0));
+        L::template st<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), gn[q] + nbb[q], sc);
+        L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0), sc);
     });
 }
 
@@ -1162,7 +1350,7 @@ __global__ void k_set_state(Geom g, void* pop_, const uint32_t* mask, int odd, c
 }
 
 template <int LP, int Q, bool MASK>
-__global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos) {
+__global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
@@ -1176,14 +1364,15 @@ __global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int od
     }
     const Cursor cu = make_cursor(g, i, j, k);
     const typename L::T* lam = static_cast<const typename L::T*>(lam_);
+    const LScale sc = lam_scale<LP>(lsc, false);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        aos[cell * Q + q] = L::template ld<double>(lam + read_loc<Q, q>(g, cu, m, odd));
+        aos[cell * Q + q] = L::template ld<double>(lam + read_loc<Q, q>(g, cu, m, odd), sc);
     });
 }
 
 template <int LP, int Q, bool MASK>
-__global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos) {
+__global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
@@ -1193,9 +1382,10 @@ __global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, con
     if (m & SOLID_BIT) return;
     const Cursor cu = make_cursor(g, i, j, k);
     typename L::T* lam = static_cast<typename L::T*>(lam_);
+    const LScale sc = lam_scale<LP>(lsc, false);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        L::template st<double>(lam + read_loc<Q, q>(g, cu, m, odd), aos[cell * Q + q]);
+        L::template st<double>(lam + read_loc<Q, q>(g, cu, m, odd), aos[cell * Q + q], sc);
     });
 }
 

@@ -25,16 +25,16 @@ struct Runner {
     virtual hipError_t port_apply(int Q, bool mask, const PortArgs& a, void* pop, double* exch, hipStream_t s) const = 0;
     virtual hipError_t port_save(int Q, bool mask, const PortArgs& a, const void* pop, void* buf,
                                  hipStream_t s) const = 0;
-    virtual hipError_t port_vjp(int Q, bool mask, const PortArgs& a, void* lam, const void* buf,
+    virtual hipError_t port_vjp(int Q, bool mask, const PortArgs& a, void* lam, const void* buf, const int* lsc,
                                 hipStream_t s) const = 0;
     virtual hipError_t get_state(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
                                  double* aos, const LaunchCfg& c) const = 0;
     virtual hipError_t set_state(int Q, bool mask, const Geom& g, void* pop, const uint32_t* m, int odd,
                                  const double* aos, const LaunchCfg& c) const = 0;
     virtual hipError_t get_lam(int Q, bool mask, const Geom& g, const void* lam, const uint32_t* m, int odd,
-                               double* aos, const LaunchCfg& c) const = 0;
+                               double* aos, const int* lsc, const LaunchCfg& c) const = 0;
     virtual hipError_t set_lam(int Q, bool mask, const Geom& g, void* lam, const uint32_t* m, int odd,
-                               const double* aos, const LaunchCfg& c) const = 0;
+                               const double* aos, const int* lsc, const LaunchCfg& c) const = 0;
     virtual hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
                               const void* alpha, const double a[3], double* rho, double* u,
                               const LaunchCfg& c) const = 0;
@@ -42,7 +42,8 @@ struct Runner {
 
 const Runner* runner_f64();
 const Runner* runner_f32();
-const Runner* runner_f16s();
+const Runner* runner_f16s();      // FP16 state, FP32 cotangents
+const Runner* runner_f16s_l16();  // FP16 state, scaled FP16 cotangents
 const Runner* runner_bf16s();
 
 #define LBM_LAUNCH(kern, cfg, ...)                                                     \
@@ -55,8 +56,8 @@ const Runner* runner_bf16s();
 // and the streaming-only debug operator; D3Q27 with BGK, TRT, MRT and cumulant.
 template <int P, int LP, bool ALL_KINDS>
 struct RunnerImpl final : Runner {
-    // Cells per thread, chosen from 256^3 measurements: one for forward steps; two for
-    // FP32/FP64 adjoint steps (8- and 16-byte accesses) and one for 16-bit storage.
+    // Cells per thread, chosen from 256^3 D3Q19 TRT measurements: one for forward steps;
+    // two for FP64 adjoint steps (16-byte accesses), one for the others.
     // LBM_FWD_CELLS / LBM_ADJ_CELLS = 1 | 2 override them.
     static int width(const char* env, int dflt) {
         const char* e = std::getenv(env);
@@ -68,7 +69,7 @@ struct RunnerImpl final : Runner {
         return w;
     }
     static int adj_width() {
-        static const int w = width("LBM_ADJ_CELLS", (P == P_F32 || P == P_F64) ? 2 : 1);
+        static const int w = width("LBM_ADJ_CELLS", P == P_F64 ? 2 : 1);
         return w;
     }
     static LaunchCfg vec_cfg(const Geom& g, int V, hipStream_t s) {
@@ -141,11 +142,13 @@ struct RunnerImpl final : Runner {
     }
     template <int Q, bool M>
     static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c0) {
-        const int V = adj_width();
-        if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
-            const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
-            const hipError_t e = adj_vec<Q, M, 2>(kind, a, c);
-            if (e != hipErrorInvalidValue) return e;
+        if constexpr (LP != P_F16S) {  // scaled FP16 cotangents: per-cell kernel only
+            const int V = adj_width();
+            if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
+                const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
+                const hipError_t e = adj_vec<Q, M, 2>(kind, a, c);
+                if (e != hipErrorInvalidValue) return e;
+            }
         }
         const LaunchCfg& c = c0;
         switch (kind) {
@@ -176,6 +179,7 @@ struct RunnerImpl final : Runner {
     }
     static bool ok_adj(int kind, bool mask, const AdjArgs& a) {
         if (!a.pop || !a.lam || !a.alpha || !a.alpha_bar) return false;
+        if (LP == P_F16S && (!a.part || !a.lsc || !a.stage || !a.ticket)) return false;
         if (mask && !a.mask) return false;
         if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
         return true;
@@ -189,8 +193,9 @@ struct RunnerImpl final : Runner {
 
     // Folds the per-block partials of a launch that wrote them.
     static hipError_t reduce(const StepArgs& a, const LaunchCfg& used, bool mass, bool stat) {
-        hipLaunchKernelGGL(k_reduce_partials<0>, dim3(1), dim3(1024), 0, used.stream, a.part, blocks(used), a.jacc,
-                           mass ? a.macc : nullptr, stat ? a.dstat : nullptr);
+        if (!a.stage || !a.ticket) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_fold<FOLD_STEP>, dim3(FOLD_BLOCKS), dim3(256), 0, used.stream, a.part, blocks(used),
+                           a.stage, a.ticket, a.jacc, mass ? a.macc : nullptr, stat ? a.dstat : nullptr, nullptr, 0.0);
         return hipGetLastError();
     }
 
@@ -207,13 +212,22 @@ struct RunnerImpl final : Runner {
     }
     hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c) const override {
         if (!ok_adj(kind, mask, a)) return hipErrorInvalidValue;
-        if (Q == 19) return mask ? adj_q<19, true>(kind, a, c) : adj_q<19, false>(kind, a, c);
-        return mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
+        if (LP == P_F16S && blocks(c) > a.npart) return hipErrorInvalidValue;
+        hipError_t e;
+        if (Q == 19)
+            e = mask ? adj_q<19, true>(kind, a, c) : adj_q<19, false>(kind, a, c);
+        else
+            e = mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
+        if (LP != P_F16S || e != hipSuccess) return e;
+        // The per-cell kernel ran on c (FP16 cotangents never take the vector path).
+        hipLaunchKernelGGL(k_fold<FOLD_LAMBDA>, dim3(FOLD_BLOCKS), dim3(256), 0, c.stream, a.part, blocks(c),
+                           a.stage, a.ticket, nullptr, nullptr, nullptr, a.lsc, a.s_next);
+        return hipGetLastError();
     }
     hipError_t objective(int Q, bool mask, const StepArgs& a, const ObjArgs& seed, void* lam, double* abar,
                          int do_seed, const LaunchCfg& c) const override {
         if (!a.pop || !a.alpha || !a.part || blocks(c) > a.npart || (mask && !a.mask) ||
-            (do_seed && (!lam || !abar)))
+            (do_seed && (!lam || !abar || (LP == P_F16S && !a.lsc))))
             return hipErrorInvalidValue;
         if (Q == 19) {
             if (mask)
@@ -235,7 +249,7 @@ struct RunnerImpl final : Runner {
         return reduce(a, c, true, false);
     }
     static LaunchCfg port_cfg(const PortArgs& a, hipStream_t s) {
-        return LaunchCfg{dim3((a.np + 63) / 64), dim3(64), s};
+        return LaunchCfg{dim3((a.np + 255) / 256), dim3(256), s};
     }
     hipError_t port_apply(int Q, bool mask, const PortArgs& a, void* pop, double* exch, hipStream_t s) const override {
         if (!ok_ports(mask, a) || !pop) return hipErrorInvalidValue;
@@ -260,18 +274,18 @@ struct RunnerImpl final : Runner {
         if (mask) LBM_LAUNCH((k_port_save<P, 27, true>), c, a, pop, b);
         LBM_LAUNCH((k_port_save<P, 27, false>), c, a, pop, b);
     }
-    hipError_t port_vjp(int Q, bool mask, const PortArgs& a, void* lam, const void* buf,
+    hipError_t port_vjp(int Q, bool mask, const PortArgs& a, void* lam, const void* buf, const int* lsc,
                         hipStream_t s) const override {
         using C = typename Store<P>::C;
-        if (!ok_ports(mask, a) || !lam || !buf) return hipErrorInvalidValue;
+        if (!ok_ports(mask, a) || !lam || !buf || (LP == P_F16S && !lsc)) return hipErrorInvalidValue;
         const LaunchCfg c = port_cfg(a, s);
         const C* b = static_cast<const C*>(buf);
         if (Q == 19) {
-            if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 19, true>), c, a, lam, b);
-            LBM_LAUNCH((k_port_vjp<P, LP, 19, false>), c, a, lam, b);
+            if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 19, true>), c, a, lam, b, lsc);
+            LBM_LAUNCH((k_port_vjp<P, LP, 19, false>), c, a, lam, b, lsc);
         }
-        if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 27, true>), c, a, lam, b);
-        LBM_LAUNCH((k_port_vjp<P, LP, 27, false>), c, a, lam, b);
+        if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 27, true>), c, a, lam, b, lsc);
+        LBM_LAUNCH((k_port_vjp<P, LP, 27, false>), c, a, lam, b, lsc);
     }
     hipError_t get_state(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, double* aos,
                          const LaunchCfg& c) const override {
@@ -292,22 +306,24 @@ struct RunnerImpl final : Runner {
         LBM_LAUNCH((k_set_state<P, 27, false>), c, g, pop, m, odd, aos);
     }
     hipError_t get_lam(int Q, bool mask, const Geom& g, const void* lam, const uint32_t* m, int odd, double* aos,
-                       const LaunchCfg& c) const override {
+                       const int* lsc, const LaunchCfg& c) const override {
+        if (!lam || !aos || (mask && !m) || (LP == P_F16S && !lsc)) return hipErrorInvalidValue;
         if (Q == 19) {
-            if (mask) LBM_LAUNCH((k_get_lam<LP, 19, true>), c, g, lam, m, odd, aos);
-            LBM_LAUNCH((k_get_lam<LP, 19, false>), c, g, lam, m, odd, aos);
+            if (mask) LBM_LAUNCH((k_get_lam<LP, 19, true>), c, g, lam, m, odd, aos, lsc);
+            LBM_LAUNCH((k_get_lam<LP, 19, false>), c, g, lam, m, odd, aos, lsc);
         }
-        if (mask) LBM_LAUNCH((k_get_lam<LP, 27, true>), c, g, lam, m, odd, aos);
-        LBM_LAUNCH((k_get_lam<LP, 27, false>), c, g, lam, m, odd, aos);
+        if (mask) LBM_LAUNCH((k_get_lam<LP, 27, true>), c, g, lam, m, odd, aos, lsc);
+        LBM_LAUNCH((k_get_lam<LP, 27, false>), c, g, lam, m, odd, aos, lsc);
     }
     hipError_t set_lam(int Q, bool mask, const Geom& g, void* lam, const uint32_t* m, int odd, const double* aos,
-                       const LaunchCfg& c) const override {
+                       const int* lsc, const LaunchCfg& c) const override {
+        if (!lam || !aos || (mask && !m) || (LP == P_F16S && !lsc)) return hipErrorInvalidValue;
         if (Q == 19) {
-            if (mask) LBM_LAUNCH((k_set_lam<LP, 19, true>), c, g, lam, m, odd, aos);
-            LBM_LAUNCH((k_set_lam<LP, 19, false>), c, g, lam, m, odd, aos);
+            if (mask) LBM_LAUNCH((k_set_lam<LP, 19, true>), c, g, lam, m, odd, aos, lsc);
+            LBM_LAUNCH((k_set_lam<LP, 19, false>), c, g, lam, m, odd, aos, lsc);
         }
-        if (mask) LBM_LAUNCH((k_set_lam<LP, 27, true>), c, g, lam, m, odd, aos);
-        LBM_LAUNCH((k_set_lam<LP, 27, false>), c, g, lam, m, odd, aos);
+        if (mask) LBM_LAUNCH((k_set_lam<LP, 27, true>), c, g, lam, m, odd, aos, lsc);
+        LBM_LAUNCH((k_set_lam<LP, 27, false>), c, g, lam, m, odd, aos, lsc);
     }
     hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, const void* alpha,
                       const double a[3], double* rho, double* u, const LaunchCfg& c) const override {

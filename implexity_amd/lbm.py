@@ -29,6 +29,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libimplexity_lbm.so")
 
 PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
+ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
 COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
 OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2}
@@ -84,7 +85,8 @@ class _Config(ctypes.Structure):
         ("reserved0", ctypes.c_uint8 * 5), ("omega", ctypes.c_double), ("magic", ctypes.c_double),
         ("bulk_rate", ctypes.c_double), ("even_rate", ctypes.c_double), ("accel", ctypes.c_double * 3),
         ("n_ports", ctypes.c_uint32), ("reserved1", ctypes.c_uint32), ("ports", ctypes.c_void_p),
-        ("checkpoint_budget_bytes", ctypes.c_uint64),
+        ("checkpoint_budget_bytes", ctypes.c_uint64), ("adjoint_storage", ctypes.c_uint32),
+        ("reserved2", ctypes.c_uint32),
     ]
 
 
@@ -196,6 +198,9 @@ class LatticeConfig:
     ports: list = field(default_factory=list)
     checkpoint_budget_bytes: int = 0
     solid: np.ndarray | None = None
+    # cotangent storage: "auto" (FP64 / FP32 / scaled FP16 for f64 / f32, bf16s / f16s),
+    # "f64", "f32" or "f16" (scaled FP16, f16s storage only)
+    adjoint_storage: str = "auto"
 
 
 @dataclass
@@ -261,6 +266,9 @@ class Lattice:
         c.n_ports = len(cfg.ports)
         c.ports = ctypes.cast(ports, ctypes.c_void_p)
         c.checkpoint_budget_bytes = int(cfg.checkpoint_budget_bytes)
+        if cfg.adjoint_storage not in ADJOINT_STORAGE:
+            raise ValueError(f"adjoint_storage must be one of {sorted(ADJOINT_STORAGE)}")
+        c.adjoint_storage = ADJOINT_STORAGE[cfg.adjoint_storage]
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)

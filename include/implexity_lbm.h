@@ -94,6 +94,19 @@ typedef struct {
     uint64_t profile_len;
 } lbm_port;
 
+/* Storage of the adjoint cotangent field (lambda).  AUTO: FP64 for FP64 storage, FP16
+ * for FP16-shifted storage, FP32 otherwise.  F16_SCALED stores lambda * 2^e with one
+ * exponent for the whole field, chosen every adjoint step from the measured max |lambda|
+ * (no counterpart in the reference, whose cotangents are FP64 Vec<f64>); a step that
+ * would leave the FP16 range makes lbm_gradient / lbm_step_vjp fail with
+ * LBM_ERR_CONVERGENCE. */
+typedef enum {
+    LBM_ADJ_AUTO = 0,
+    LBM_ADJ_F64 = 1,
+    LBM_ADJ_F32 = 2,
+    LBM_ADJ_F16_SCALED = 3
+} lbm_adjoint_storage;
+
 typedef struct {
     uint32_t struct_size;         /* sizeof(lbm_config) */
     uint32_t lattice;             /* lbm_lattice */
@@ -111,6 +124,8 @@ typedef struct {
     uint32_t reserved1;
     const lbm_port* ports;
     uint64_t checkpoint_budget_bytes; /* device bytes for revolve snapshots (0: 1/2 of free) */
+    uint32_t adjoint_storage;     /* lbm_adjoint_storage */
+    uint32_t reserved2;
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

@@ -242,3 +242,78 @@ def test_gradient_central_differences_f64():
             fd = (vals[0] - vals[1]) / (2 * h)
             scale = max(abs(fd), abs(ab[x]))
             assert abs(fd / scale - ab[x] / scale) <= 1e-6, (x, fd, ab[x])
+
+
+# Storage, cotangent storage, tolerance on max |error| / max |value|: FP32 rounds at
+# 6e-8; FP16-shifted states carry 11 significant bits of f - w; scaled FP16 cotangents
+# carry 11 significant bits of lambda (2^-12 relative rounding per value).
+VJP_REDUCED = [("f32", "auto", 2e-6), ("f16s", "f32", 1e-3), ("f16s", "f16", 3e-3)]
+
+
+@pytest.mark.parametrize("precision,adj,tol", VJP_REDUCED)
+@pytest.mark.parametrize("Q,coll,with_ports", [(19, "trt", True), (19, "mrt", False), (27, "cumulant", False)])
+def test_step_vjp_reduced_precision(precision, adj, tol, Q, coll, with_ports):
+    shape = (7, 6, 5)
+    ports = channel_ports(shape, 0.03) if with_ports else ()
+    periodic = (False, False, False) if with_ports else (True, False, False)
+    cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision=precision,
+                          accel=(0, 0, 0) if ports else (1e-4, 0, 0), ports=ports,
+                          bulk_rate=1.0 if coll == "cumulant" else float("nan"))
+    cfg.adjoint_storage = adj
+    alpha = random_alpha(shape, 17)
+    o = orc.Oracle(prob)
+    f = o.run(alpha, 5)
+    lam = np.random.default_rng(1).standard_normal(o.n * Q) * 1e-3
+    want_f, want_a = o.interval_vjp(alpha, f, lam)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        got_f, got_a = lat.interval_vjp(lam)
+    ef = np.abs(got_f - want_f).max() / np.abs(want_f).max()
+    ea = np.abs(got_a - want_a).max() / np.abs(want_a).max()
+    assert ef <= tol and ea <= tol, (ef, ea)
+
+
+@pytest.mark.parametrize("precision,adj,tol", [("f32", "auto", 2e-5), ("f16s", "f32", 2e-2), ("f16s", "f16", 2e-2)])
+def test_gradient_reduced_precision_channel(precision, adj, tol):
+    """Inlet/outlet channel, pressure drop over the last half of 60 steps: relative L2
+    error of dJ/dalpha against the FP64 oracle's scan_adjoint, and the scaled FP16
+    cotangents against FP32 cotangents on the same FP16 states."""
+    shape = (12, 8, 8)
+    steps = 60
+    ports = channel_ports(shape, 0.03)
+    cfg, prob = make_pair(shape, (False, False, False), precision=precision, ports=ports)
+    cfg.adjoint_storage = adj
+    alpha = random_alpha(shape, 31)
+    ob = Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=30, t_end=steps,
+                   weight=1.0 / 31)
+    oob = orc.Objective(kind="pressure_drop", window=(30, steps), weight=1.0 / 31, axis=0, plane_a=1,
+                        plane_b=shape[0] - 2)
+    J_want, ab_want, _, _ = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, st = lat.gradient(steps, ob)
+    err = np.linalg.norm(ab - ab_want) / np.linalg.norm(ab_want)
+    assert err <= tol, err
+    assert abs(J - J_want) <= tol * abs(J_want)
+    assert np.isfinite(st.mass_drift)
+    if adj == "f16":
+        cfg.adjoint_storage = "f32"
+        with Lattice(cfg) as lat:
+            lat.set_design(alpha)
+            lat.initial_populations()
+            _, ab32, _, _ = lat.gradient(steps, ob)
+        assert np.linalg.norm(ab - ab32) / np.linalg.norm(ab32) <= 2e-3
+
+
+def test_mass_drift_includes_port_exchange_f64():
+    """admit's drift (solver.rs:707-793) counts the mass the ports add or remove: with
+    that exchange booked, an FP64 channel conserves mass to rounding."""
+    shape = (12, 8, 8)
+    cfg, _ = make_pair(shape, (False, False, False), precision="f64", ports=channel_ports(shape, 0.03))
+    with Lattice(cfg) as lat:
+        lat.set_design(random_alpha(shape, 3))
+        lat.initial_populations()
+        _, st = lat.history(80)
+    assert st.mass_drift <= 1e-13, st.mass_drift
