@@ -153,12 +153,17 @@ def main():
             tf = lat.time_kernel("forward", args.launches)
             ta = lat.time_kernel("adjoint", args.launches)
             ts = lat.time_kernel("stream", args.launches)
+        from implexity_amd.lbm import copy_bandwidth
+
+        copy_gbs, copy_ms = copy_bandwidth(1 << 30, args.launches)  # 1 GiB read + 1 GiB written
         n = int(np.prod(shape))
         out = {"precision": args.kernel_precision, "forward_ms": tf, "adjoint_ms": ta, "stream_only_ms": ts,
                "forward_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (tf * 1e-3) / 1e9,
                "adjoint_GBs": algorithmic_bytes("adjoint", args.kernel_precision) * n / (ta * 1e-3) / 1e9,
                "stream_only_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (ts * 1e-3) / 1e9,
-               "cells_per_thread": os.environ.get("LBM_CELLS_PER_THREAD", "default")}
+               "copy_GBs": copy_gbs, "copy_ms": copy_ms,
+               "cells_per_thread": {"forward": os.environ.get("LBM_FWD_CELLS", "default"),
+                                    "adjoint": os.environ.get("LBM_ADJ_CELLS", "default")}}
         print(json.dumps(out))
         return
 

@@ -229,7 +229,7 @@ struct lbm_handle {
         s.part = nullptr;
         s.npart = npart;
         s.stage = fold.as<double>();
-        s.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_BLOCKS);
+        s.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
         s.jacc = scal.as<double>();
         s.macc = scal.as<double>() + 1;
         s.dstat = reinterpret_cast<unsigned*>(scal.as<double>() + 4);
@@ -256,7 +256,7 @@ struct lbm_handle {
         a.part = part.as<double>();
         a.npart = npart;
         a.stage = fold.as<double>();
-        a.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_BLOCKS);
+        a.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
         a.lsc = lsc_ptr();
         a.s_next = 0.0;
         return a;
@@ -392,31 +392,31 @@ uint64_t reserve_pool(lbm_handle* h, uint64_t nsteps) {
     return slots;
 }
 
-// One forward step of state `step` read from `src`, written to `dst` (in place when
-// they are the same buffer): collide+stream, then ports on dst.  `obj` is the
-// objective of the state being read.
-// Fluid mass of every state of a sweep and the mass the ports exchange in every step,
-// for the admission drift |M_s - M_0 - sum_{r<s} X_r| / M_0 (solver.rs:707-793).
+// Fluid mass of every state of a sweep and the mass the ports exchange in every step
+// (one partial per port block), for the admission drift
+// |M_s - M_0 - sum_{r<s} X_r| / M_0 (solver.rs:707-793).
 struct MassLedger {
     DevBuf buf;
     uint64_t n = 0;
+    long long nbp = 0;  // port blocks
     void init(lbm_handle* h, uint64_t nsteps) {
         n = nsteps;
-        buf.alloc(2 * (n + 2) * sizeof(double), "mass ledger");
+        nbp = (h->np + PORT_BLOCK - 1) / PORT_BLOCK;
+        buf.alloc(((n + 2) + (n + 2) * size_t(nbp)) * sizeof(double), "mass ledger");
         hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream), "memset");
     }
     double* mass(uint64_t s) const { return buf.as<double>() + s; }
-    double* exch(uint64_t s) const { return buf.as<double>() + (n + 2) + s; }
+    double* exch(uint64_t s) const { return buf.as<double>() + (n + 2) + s * size_t(nbp); }
     // Call after the stream has been synchronised.
     double drift() const {
-        std::vector<double> v(2 * (n + 2));
+        std::vector<double> v(buf.bytes / sizeof(double));
         hip_check(hipMemcpy(v.data(), buf.p, buf.bytes, hipMemcpyDeviceToHost), "copy");
         const double* M = v.data();
-        const double* X = v.data() + (n + 2);
         double ex = 0.0, worst = 0.0;
         for (uint64_t s = 0; s <= n; s++) {
             worst = std::max(worst, std::fabs(M[s] - M[0] - ex));
-            ex += X[s];
+            const double* X = v.data() + (n + 2) + s * size_t(nbp);
+            for (long long b = 0; b < nbp; b++) ex += X[b];
         }
         return worst / M[0];
     }
@@ -665,7 +665,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             const long long blocks_v = ((long long)h->nx * h->ny * (h->nz / 2) + 127) / 128;
             h->npart = std::max(blocks_3d, blocks_v);
             h->part.alloc(size_t(4 * h->npart) * sizeof(double), "block partials");
-            h->fold.alloc((4 * FOLD_BLOCKS + 2) * sizeof(double), "fold stage");
+            h->fold.alloc((4 * FOLD_MAX_BLOCKS + 2) * sizeof(double), "fold stage");
             hip_check(hipMemsetAsync(h->fold.p, 0, h->fold.bytes, h->stream), "memset");
         }
         if (h->has_mask) {

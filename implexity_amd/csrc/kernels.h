@@ -513,35 +513,27 @@ LBM_DEV void block_max_partial(float v, double* part) {
 }
 
 // Folds the per-block partials of one launch (part[4 b + 0..3] = objective, mass,
-// max, bad-cell count) in two fixed-order stages: fold block b sums its contiguous chunk
-// into stage[4 b ..]; the last fold block to finish (a ticket counter, released with
-// device-scope fences) combines the stage values in block order, writes the results
-// and resets the ticket.  Deterministic for a given launch shape.
+// max, bad-cell count) in two fixed-order stages: each fold block reduces 256 partials
+// per pass (one 32-byte load per thread) into stage[4 block ..]; the last fold block to
+// finish (a ticket counter, released with device-scope fences) loads the stage values
+// one per thread, reduces them in the same fixed tree order, writes the results and
+// resets the ticket.  Deterministic for a given launch shape.
 //   FOLD_STEP:   jacc += objective; *macc = mass (if set); dstat[0] = max(dstat[0],
 //                max |u|^2) and dstat[1] += bad (if set).
 //   FOLD_LAMBDA: max |lambda| of an FP16 cotangent step: the field now has exponent
 //                lsc[1]; record the max, count an overflow, and choose the next write
 //                exponent from LAM_GROWTH * max + s_next.
-constexpr int FOLD_BLOCKS = 64;
+constexpr int FOLD_MAX_BLOCKS = 256;
 enum FoldMode { FOLD_STEP = 0, FOLD_LAMBDA = 1 };
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, double* stage, unsigned* ticket,
-                                             double* jacc, double* macc, unsigned* dstat, int* lsc, double s_next) {
+LBM_HD inline int fold_blocks(long long nb) {
+    const long long g = (nb + 255) / 256;
+    return int(g < 1 ? 1 : (g > FOLD_MAX_BLOCKS ? FOLD_MAX_BLOCKS : g));
+}
+
+// Block-wide fixed-order reduction of (sum a, sum b, max c, sum d); result in thread 0.
+LBM_DEV void block_fold4(double& a, double& b, double& c, double& d) {
     __shared__ double red[4][4];
-    __shared__ bool last;
-    const long long chunk = (nb + gridDim.x - 1) / gridDim.x;
-    const long long lo = (long long)blockIdx.x * chunk;
-    const long long hi = lo + chunk < nb ? lo + chunk : nb;
-    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        if (MODE == FOLD_STEP) {
-            a += part[4 * i];
-            b += part[4 * i + 1];
-            d += part[4 * i + 3];
-        }
-        c = fmax(c, part[4 * i + 2]);
-    }
     a = wave_sum(a);
     b = wave_sum(b);
     d = wave_sum(d);
@@ -556,40 +548,58 @@ __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+        a = b = c = d = 0.0;
         for (int w = 0; w < int(blockDim.x >> 6); w++) {
-            sa += red[0][w];
-            sb += red[1][w];
-            sc = fmax(sc, red[2][w]);
-            sd += red[3][w];
+            a += red[0][w];
+            b += red[1][w];
+            c = fmax(c, red[2][w]);
+            d += red[3][w];
         }
-        stage[4 * blockIdx.x] = sa;
-        stage[4 * blockIdx.x + 1] = sb;
-        stage[4 * blockIdx.x + 2] = sc;
-        stage[4 * blockIdx.x + 3] = sd;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, double* stage, unsigned* ticket,
+                                             double* jacc, double* macc, unsigned* dstat, int* lsc, double s_next) {
+    __shared__ bool last;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += step) {
+        const double4 v = *reinterpret_cast<const double4*>(part + 4 * i);
+        a += v.x;
+        b += v.y;
+        c = fmax(c, v.z);
+        d += v.w;
+    }
+    block_fold4(a, b, c, d);
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<double4*>(stage + 4 * blockIdx.x) = double4{a, b, c, d};
         __threadfence();
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last) return;
     __threadfence();
-    double ta = 0.0, tb = 0.0, tc = 0.0, td = 0.0;
-    for (unsigned k = 0; k < gridDim.x; k++) {
-        ta += __hip_atomic_load(&stage[4 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tb += __hip_atomic_load(&stage[4 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tc = fmax(tc, __hip_atomic_load(&stage[4 * k + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        td += __hip_atomic_load(&stage[4 * k + 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a = b = c = d = 0.0;
+    if (threadIdx.x < gridDim.x) {
+        const double* sp = stage + 4 * threadIdx.x;
+        a = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = __hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c = __hip_atomic_load(sp + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        d = __hip_atomic_load(sp + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    block_fold4(a, b, c, d);
+    if (threadIdx.x != 0) return;
     if (MODE == FOLD_STEP) {
-        if (jacc) *jacc += ta;
-        if (macc) *macc = tb;
+        if (jacc) *jacc += a;
+        if (macc) *macc = b;
         if (dstat) {
-            const unsigned ub = __float_as_uint(float(tc));
+            const unsigned ub = __float_as_uint(float(c));
             if (ub > dstat[0]) dstat[0] = ub;
-            dstat[1] += unsigned(td);
+            dstat[1] += unsigned(d);
         }
     } else {
-        const float mx = float(tc);
+        const float mx = float(c);
         const int ew = lsc[1];
         if (!(double(mx) * ldexp(1.0, ew) <= 65504.0)) lsc[3] = lsc[3] + 1;
         lsc[0] = ew;
@@ -1147,6 +1157,8 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
 // Ports: moving/boundary.rs:445-506 (NEE port_populations and its VJP) applied in the
 // stationary order: after streaming, replace every population of a port cell.
 // ---------------------------------------------------------------------------------
+constexpr int PORT_BLOCK = 128;  // threads per block of the port kernels
+
 struct PortArgs {
     Geom g;
     int np;
@@ -1232,9 +1244,10 @@ LBM_DEV uint32_t mask_of(const PortArgs& A, long long cell, Cursor& cu) {
     return cell_mask<Q, MASK>(A.g, A.mask, cell, i, j, k);
 }
 
-// f(port cell) := NEE(f(neighbour)); exch[r] gets the mass change sum_q (new - old).
+// f(port cell) := NEE(f(neighbour)); exch_part[block] gets the block's mass change
+// sum_q (new - old).
 template <int P, int Q, bool MASK>
-__global__ void k_port_apply(PortArgs A, void* pop_, double* exch_sum) {
+__global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, double* exch_part) {
     using S = Store<P>;
     using C = typename S::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1257,15 +1270,18 @@ __global__ void k_port_apply(PortArgs A, void* pop_, double* exch_sum) {
             delta += double(S::template dec<Q, q>(S::template enc<Q, q>(out[q]))) - double(old[q]);
         });
     }
-    if (exch_sum) {
+    if (exch_part) {  // per-block mass exchange, summed on the host in block order
+        __shared__ double red[2];
         delta = wave_sum(delta);
-        if ((threadIdx.x & 63) == 0 && delta != 0.0) unsafeAtomicAdd(exch_sum, delta);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = delta;
+        __syncthreads();
+        if (threadIdx.x == 0) exch_part[blockIdx.x] = blockDim.x > 64 ? red[0] + red[1] : red[0];
     }
 }
 
 // Save the populations of every port neighbour (state of parity A.odd) to buf[q*np+r].
 template <int P, int Q, bool MASK>
-__global__ void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* buf) {
+__global__ __launch_bounds__(128) void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* buf) {
     using S = Store<P>;
     using C = typename S::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1282,7 +1298,7 @@ __global__ void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* 
 // Adjoint of the port pass on the cotangent of a state of parity A.odd:
 // lam(nb) += J_NEE(f_nb)^T lam(port); lam(port) := 0   (solver.rs:642-656).
 template <int P, int LP, int Q, bool MASK>
-__global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf, const int* lsc) {
+__global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf, const int* lsc) {
     using L = LamStore<LP>;
     using C = typename Store<P>::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1312,7 +1328,7 @@ __global__ void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* b
 // Layout conversion between the twisted device layout and the reference AoS layout.
 // ---------------------------------------------------------------------------------
 template <int P, int Q, bool MASK>
-__global__ void k_get_state(Geom g, const void* pop_, const uint32_t* mask, int odd, double* aos) {
+__global__ __launch_bounds__(256) void k_get_state(Geom g, const void* pop_, const uint32_t* mask, int odd, double* aos) {
     using S = Store<P>;
     using C = typename S::C;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1333,7 +1349,7 @@ __global__ void k_get_state(Geom g, const void* pop_, const uint32_t* mask, int 
 }
 
 template <int P, int Q, bool MASK>
-__global__ void k_set_state(Geom g, void* pop_, const uint32_t* mask, int odd, const double* aos) {
+__global__ __launch_bounds__(256) void k_set_state(Geom g, void* pop_, const uint32_t* mask, int odd, const double* aos) {
     using S = Store<P>;
     using C = typename S::C;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1350,7 +1366,7 @@ __global__ void k_set_state(Geom g, void* pop_, const uint32_t* mask, int odd, c
 }
 
 template <int LP, int Q, bool MASK>
-__global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos, const int* lsc) {
+__global__ __launch_bounds__(256) void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
@@ -1372,7 +1388,7 @@ __global__ void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int od
 }
 
 template <int LP, int Q, bool MASK>
-__global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos, const int* lsc) {
+__global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
@@ -1391,7 +1407,7 @@ __global__ void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, con
 
 // Macroscopic fields (solver.rs:690-705 in lattice units): rho, and u (0 in solids).
 template <int P, int Q, bool MASK>
-__global__ void k_fields(Geom g, const void* pop_, const uint32_t* mask, int odd, const void* alpha_, double a0,
+__global__ __launch_bounds__(256) void k_fields(Geom g, const void* pop_, const uint32_t* mask, int odd, const void* alpha_, double a0,
                          double a1, double a2, double* rho, double* u) {
     using S = Store<P>;
     using C = typename S::C;

@@ -76,40 +76,25 @@ struct RunnerImpl final : Runner {
         const long long chunks = (long long)g.nx * g.ny * (g.nz / V);
         return LaunchCfg{dim3(unsigned((chunks + 127) / 128)), dim3(128), s};
     }
+    // Vector paths (V cells per thread along z) exist for the two-relaxation kinds and the
+    // streaming-only operator; MRT, cumulant and BGK-Guo take the per-cell kernels (their
+    // register footprint at V = 2 spills).
     template <int Q, bool M, int V>
     static hipError_t fwd_vec(int kind, const StepArgs& a, const LaunchCfg& c) {
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_forward_v<P, Q, K_TRT, M, V>), c, a);
         case K_BGK: LBM_LAUNCH((k_forward_v<P, Q, K_BGK, M, V>), c, a);
         case K_STREAM_ONLY: LBM_LAUNCH((k_forward_v<P, Q, K_STREAM_ONLY, M, V>), c, a);
-        default: break;
+        default: return hipErrorInvalidValue;
         }
-        if constexpr (ALL_KINDS) {
-            if (kind == K_MRT) LBM_LAUNCH((k_forward_v<P, Q, K_MRT, M, V>), c, a);
-            if constexpr (Q == 19) {
-                if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward_v<P, 19, K_BGK_GUO, M, V>), c, a);
-            } else {
-                if (kind == K_CUMULANT) LBM_LAUNCH((k_forward_v<P, 27, K_CUMULANT, M, V>), c, a);
-            }
-        }
-        return hipErrorInvalidValue;
     }
     template <int Q, bool M, int V>
     static hipError_t adj_vec(int kind, const AdjArgs& a, const LaunchCfg& c) {
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_TRT, M, V>), c, a);
         case K_BGK: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_BGK, M, V>), c, a);
-        default: break;
+        default: return hipErrorInvalidValue;
         }
-        if constexpr (ALL_KINDS) {
-            if (kind == K_MRT) LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_MRT, M, V>), c, a);
-            if constexpr (Q == 19) {
-                if (kind == K_BGK_GUO) LBM_LAUNCH((k_adjoint_v<P, LP, 19, K_BGK_GUO, M, V>), c, a);
-            } else {
-                if (kind == K_CUMULANT) LBM_LAUNCH((k_adjoint_v<P, LP, 27, K_CUMULANT, M, V>), c, a);
-            }
-        }
-        return hipErrorInvalidValue;
     }
     template <int Q, bool M>
     static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c0, LaunchCfg& used) {
@@ -194,7 +179,7 @@ struct RunnerImpl final : Runner {
     // Folds the per-block partials of a launch that wrote them.
     static hipError_t reduce(const StepArgs& a, const LaunchCfg& used, bool mass, bool stat) {
         if (!a.stage || !a.ticket) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_fold<FOLD_STEP>, dim3(FOLD_BLOCKS), dim3(256), 0, used.stream, a.part, blocks(used),
+        hipLaunchKernelGGL(k_fold<FOLD_STEP>, dim3(fold_blocks(blocks(used))), dim3(256), 0, used.stream, a.part, blocks(used),
                            a.stage, a.ticket, a.jacc, mass ? a.macc : nullptr, stat ? a.dstat : nullptr, nullptr, 0.0);
         return hipGetLastError();
     }
@@ -220,7 +205,7 @@ struct RunnerImpl final : Runner {
             e = mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
         if (LP != P_F16S || e != hipSuccess) return e;
         // The per-cell kernel ran on c (FP16 cotangents never take the vector path).
-        hipLaunchKernelGGL(k_fold<FOLD_LAMBDA>, dim3(FOLD_BLOCKS), dim3(256), 0, c.stream, a.part, blocks(c),
+        hipLaunchKernelGGL(k_fold<FOLD_LAMBDA>, dim3(fold_blocks(blocks(c))), dim3(256), 0, c.stream, a.part, blocks(c),
                            a.stage, a.ticket, nullptr, nullptr, nullptr, a.lsc, a.s_next);
         return hipGetLastError();
     }
@@ -249,7 +234,7 @@ struct RunnerImpl final : Runner {
         return reduce(a, c, true, false);
     }
     static LaunchCfg port_cfg(const PortArgs& a, hipStream_t s) {
-        return LaunchCfg{dim3((a.np + 255) / 256), dim3(256), s};
+        return LaunchCfg{dim3((a.np + PORT_BLOCK - 1) / PORT_BLOCK), dim3(PORT_BLOCK), s};
     }
     hipError_t port_apply(int Q, bool mask, const PortArgs& a, void* pop, double* exch, hipStream_t s) const override {
         if (!ok_ports(mask, a) || !pop) return hipErrorInvalidValue;
