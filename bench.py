@@ -37,6 +37,19 @@ VARIANTS = {
 }
 
 
+# HBM traffic per lattice update of the step kernels at 256^3 (D3Q19 TRT, no solids),
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py --mode kernels`,
+# each counter scaled by its reading for the 1 GiB copy kernel of the same run
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the passes and the scaling are recorded in
+# profiles/pmc_traffic_d3q19_trt_256cubed.txt.
+PMC_TRAFFIC_BYTES_PER_LU = {
+    ("forward", "f16s"): 92.6,
+    ("adjoint", "f16s"): 156.6,
+    ("forward", "f32"): 168.9,
+    ("adjoint", "f32"): 271.0,
+}
+
+
 def algorithmic_bytes(which, variant, Q=19):
     """Bytes per lattice update: the forward step reads and writes every population once
     and reads alpha; the adjoint step reads the state, reads and writes every cotangent,
@@ -91,7 +104,8 @@ def run_precision(shape, precision, steps, warmup, budget, alpha_cfg, timing_lau
         lat.initial_populations()
         t_adj = lat.time_kernel("adjoint", timing_launches)
         return dict(J=J, alpha_bar=ab, wall=wall, stats=st.as_dict(), t_fwd_ms=t_fwd, t_adj_ms=t_adj,
-                    state_bytes=lat.state_bytes)
+                    state_bytes=lat.state_bytes, run_fwd_ms=st.forward_kernel_avg_ms,
+                    run_adj_ms=st.adjoint_kernel_avg_ms)
     finally:
         lat.close()
 
@@ -122,7 +136,7 @@ def main():
     ap.add_argument("--shape", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--precisions", default="f64,f32,f16s")
-    ap.add_argument("--cpu-window", type=int, default=2)
+    ap.add_argument("--cpu-window", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="gradient", choices=["gradient", "kernels"])
     ap.add_argument("--kernel-precision", default="f32")
@@ -183,17 +197,27 @@ def main():
     r = res[main_prec]
     value = n * args.steps / r["wall"] / 1e6
 
-    def kern(prec, which):
-        t = res[prec]["t_fwd_ms" if which == "forward" else "t_adj_ms"]
+    def kern(prec, which, live=True):
+        """Average kernel time (ms) and algorithmic GB/s: live = HIP events around every
+        64th step kernel inside the timed gradient; otherwise back-to-back launches."""
+        if live:
+            t = res[prec]["run_fwd_ms" if which == "forward" else "run_adj_ms"]
+        else:
+            t = res[prec]["t_fwd_ms" if which == "forward" else "t_adj_ms"]
         gbs = algorithmic_bytes(which, prec) * n / (t * 1e-3) / 1e9
         return t, gbs
 
-    # dominant kernel of the measured run: forward (incl. recomputation) or adjoint
+    # dominant kernel of the timed run: forward (incl. recomputation) or adjoint
     st = r["stats"]
-    fwd_total = st["forward_steps"] * r["t_fwd_ms"]
-    adj_total = st["reverse_steps"] * r["t_adj_ms"]
+    fwd_total = st["forward_steps"] * r["run_fwd_ms"]
+    adj_total = st["reverse_steps"] * r["run_adj_ms"]
     dom = "forward" if fwd_total >= adj_total else "adjoint"
     t_dom, gbs_dom = kern(main_prec, dom)
+    traffic = PMC_TRAFFIC_BYTES_PER_LU.get((dom, main_prec))
+    if traffic is not None and shape == (256, 256, 256):
+        traffic = traffic * n
+    else:
+        traffic = None
     out = {
         "metric": "D3Q19 fwd+adjoint MLUPS, % HBM roofline; FP16-store vs FP32 at grad err<=1e-3",
         "value": value,
@@ -211,19 +235,25 @@ def main():
                                f"forward+adjoint over {args.steps} steps, binomial checkpointing in "
                                f"{args.budget_gb:g} GB", "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": gbs_dom, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs_dom / HBM_PEAK_GBS, "traffic": None, "kernel": f"{dom} ({main_prec})",
-                     "algorithmic_bytes_per_lu": algorithmic_bytes(dom, main_prec), "avg_kernel_ms": t_dom},
+                     "frac": gbs_dom / HBM_PEAK_GBS, "traffic": traffic, "kernel": f"{dom} ({main_prec})",
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(dom, main_prec) * n,
+                     "algorithmic_bytes_per_lu": algorithmic_bytes(dom, main_prec), "avg_kernel_ms": t_dom,
+                     "avg_kernel_ms_source": "HIP events around every 64th step kernel of the timed gradient",
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-kernel scaled"},
     }
     per = {}
     for prec, rr in res.items():
         tf, gf = kern(prec, "forward")
         ta, ga = kern(prec, "adjoint")
+        tf_iso, _ = kern(prec, "forward", live=False)
+        ta_iso, _ = kern(prec, "adjoint", live=False)
         per[prec] = {"gradient_wall_s": rr["wall"], "mlups_fwd_adj": n * args.steps / rr["wall"] / 1e6,
                      "J": rr["J"], "forward_kernel_ms": tf, "forward_GBs": gf, "forward_frac": gf / HBM_PEAK_GBS,
                      "forward_mlups": n / (tf * 1e-3) / 1e6, "adjoint_kernel_ms": ta, "adjoint_GBs": ga,
                      "adjoint_frac": ga / HBM_PEAK_GBS, "adjoint_mlups": n / (ta * 1e-3) / 1e6,
                      "forward_steps": rr["stats"]["forward_steps"], "snapshot_slots": rr["stats"]["snapshot_slots"],
-                     "max_mach": rr["stats"]["max_mach"]}
+                     "max_mach": rr["stats"]["max_mach"], "mass_drift": rr["stats"]["mass_drift"],
+                     "forward_kernel_ms_back_to_back": tf_iso, "adjoint_kernel_ms_back_to_back": ta_iso}
     out["per_precision"] = per
     if "f32" in res and "f16s" in res:
         out["speedup_f16s_vs_f32"] = res["f32"]["wall"] / res["f16s"]["wall"]

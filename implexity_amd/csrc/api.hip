@@ -207,6 +207,8 @@ struct lbm_handle {
     std::vector<double> h_alpha;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
+    // Step grid: one block of b cells (a multiple of 64) along a z row, one row per grid
+    // y, one x plane per grid z.  (Blocks of two y rows measured no faster at 256^3.)
     LaunchCfg cfg() const {
         const int b = std::min(256, ((nz + 63) / 64) * 64);
         return LaunchCfg{dim3((nz + b - 1) / b, ny, nx), dim3(b), stream};
@@ -425,7 +427,7 @@ struct MassLedger {
 // One forward step of state `step` read from `src`, written to `dst`; with a ledger,
 // the step records the mass of the state it reads and the port exchange it makes.
 void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs& obj, const MassLedger* led,
-               uint64_t s_index) {
+               uint64_t s_index, const std::pair<hipEvent_t, hipEvent_t>* kspan = nullptr) {
     const bool diag = led != nullptr;
     StepArgs s = h->step_args(src, step);
     s.pop_out = dst;
@@ -433,7 +435,9 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
     s.diag = diag ? 1 : 0;
     if (obj.kind != OBJ_NONE || diag) s.part = h->part.as<double>();
     if (diag) s.macc = led->mass(s_index);
-    hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg()), "forward step launch");
+    if (kspan) hip_check(hipEventRecord(kspan->first, h->stream), "event record");
+    hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg(), kspan ? kspan->second : nullptr),
+              "forward step launch");
     if (h->np) {
         hip_check(h->run->port_apply(h->Q, h->has_mask, h->port_args(step + 1), dst,
                                      diag ? led->exch(s_index) : nullptr, h->stream),
@@ -1052,7 +1056,20 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
             return e;
         };
         double fwd_ms = 0.0, adj_ms = 0.0;
-        std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd_spans, adj_spans;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd_spans, adj_spans, fwd_k, adj_k;
+        // Every KSAMPLE-th step kernel is bracketed by events to time the kernel alone.
+        constexpr uint64_t KSAMPLE = 64;
+        auto kspan = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, uint64_t count)
+            -> const std::pair<hipEvent_t, hipEvent_t>* {
+            if (count % KSAMPLE != KSAMPLE / 2) return nullptr;
+            hipEvent_t a, b;
+            hip_check(hipEventCreate(&a), "event");
+            hip_check(hipEventCreate(&b), "event");
+            evs.push_back(a);
+            evs.push_back(b);
+            v.emplace_back(a, b);
+            return &v.back();
+        };
         uint64_t fsteps = 0, rsteps = 0, copies = 0;
         uint64_t cursor = 0;  // state index held in F
         ObjArgs none{};
@@ -1093,7 +1110,8 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
                     const ObjArgs ob = forward_phase ? stage_obj(tmpl, obj, t, nsteps, false) : none;
                     if (t + 1 == nsteps) {
                         // Final step into the ahead buffer; the cursor keeps state n-1.
-                        step_into(h, cur, h->A.p, base + t, ob, forward_phase ? &led : nullptr, t);
+                        step_into(h, cur, h->A.p, base + t, ob, forward_phase ? &led : nullptr, t,
+                                  kspan(fwd_k, fsteps));
                         fsteps++;
                         // Objective on the final state and the cotangent seed lambda_n.
                         StepArgs ea = h->step_args(h->A.p, base + nsteps);
@@ -1113,7 +1131,7 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
                         break;
                     }
                     void* dst = (t + 1 == act.b) ? last_dst : h->F.p;
-                    step_into(h, cur, dst, base + t, ob, forward_phase ? &led : nullptr, t);
+                    step_into(h, cur, dst, base + t, ob, forward_phase ? &led : nullptr, t, kspan(fwd_k, fsteps));
                     cur = dst;
                     fsteps++;
                     cursor = t + 1;
@@ -1133,7 +1151,12 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
                 aa.odd = int((base + k - 1) & 1);
                 aa.obj = stage_obj(tmpl, obj, k - 1, nsteps, false);
                 aa.s_next = k >= 2 ? seed_bound(h, stage_obj(tmpl, obj, k - 2, nsteps, false)) : 0.0;
-                hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg()), "adjoint launch");
+                {
+                    const auto* ks = kspan(adj_k, rsteps);
+                    if (ks) hip_check(hipEventRecord(ks->first, h->stream), "event record");
+                    hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg(), ks ? ks->second : nullptr),
+                              "adjoint launch");
+                }
                 rsteps++;
                 if (h->np && k > 1)
                     hip_check(h->run->port_save(Q, h->has_mask, h->port_args(base + k - 1), cur, h->pbuf.p,
@@ -1173,6 +1196,16 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
             hip_check(hipEventElapsedTime(&ms, sp.first, sp.second), "event time");
             adj_ms += ms;
         }
+        auto kavg = [&](const std::vector<std::pair<hipEvent_t, hipEvent_t>>& v) {
+            double acc = 0.0;
+            for (auto& sp : v) {
+                float ms;
+                hip_check(hipEventElapsedTime(&ms, sp.first, sp.second), "event time");
+                acc += ms;
+            }
+            return v.empty() ? NAN : acc / double(v.size());
+        };
+        const double fwd_kavg = kavg(fwd_k), adj_kavg = kavg(adj_k);
         for (auto e : evs) (void)hipEventDestroy(e);
         h->t = base;
         if (J) *J = sc[0];
@@ -1191,6 +1224,9 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
             stats->max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
             stats->bad_cells = ds[1];
             stats->mass_drift = led.drift();
+            stats->forward_kernel_avg_ms = fwd_kavg;
+            stats->adjoint_kernel_avg_ms = adj_kavg;
+            stats->kernel_samples = fwd_k.size() + adj_k.size();
         }
         check_lam_range(h);
     });

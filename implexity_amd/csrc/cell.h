@@ -209,6 +209,75 @@ LBM_DEV void collide_pairs_vjp(const C rho, const C u[3], const C F[3], C omega,
     rho_bar = rb;
 }
 
+// FP32 form of collide_pairs_vjp (same cotangents, regrouped per pair with the
+// per-weight coefficients formed once per cell); the FP64 path keeps the reference's
+// operation order above.
+template <int Q>
+LBM_DEV void collide_pairs_vjp_fast(const float rho, const float u[3], const float F[3], float omega, float om,
+                                    const float* g, float* fb, float& rho_bar, float u_bar[3], float F_bar[3]) {
+    const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const float base = 1.0f - usq * 1.5f;
+    const float kp = 1.0f - omega * 0.5f;
+    const float km = 1.0f - om * 0.5f;
+    const float hp = omega * 0.5f, hm = om * 0.5f;
+    const float r1 = omega * base;        // times w: d rho_bar / d (g_i + g_o), cu-free part
+    const float r2 = 4.5f * omega;        // times w cu^2
+    const float r3 = 3.0f * om;           // times w cu (g_i - g_o)
+    const float c1 = 9.0f * omega * rho;  // cu_bar: times w cu (g_i + g_o)
+    const float c2 = 9.0f * kp;           // cu_bar: times w cf (g_i + g_o); cf_bar: times w cu (g_i + g_o)
+    const float c3 = 3.0f * om * rho;     // cu_bar: times w (g_i - g_o)
+    const float c4 = 3.0f * km;           // cf_bar: times w (g_i - g_o)
+    float rb = 0.0f, wa = 0.0f;           // wa = sum_pairs w (g_i + g_o) (+ w0 g0)
+    float ub0 = 0.0f, ub1 = 0.0f, ub2 = 0.0f, fb0 = 0.0f, fb1 = 0.0f, fb2 = 0.0f;
+    constexpr float w0 = float(weight<Q>(0));
+    const float g0 = g[0];
+    fb[0] = g0 - omega * g0;
+    rb += w0 * g0 * r1;
+    wa += w0 * g0;
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int o = opp<Q>(i);
+        if constexpr (o > i) {
+            constexpr float w = float(weight<Q>(i));
+            const float cu = dotc<Q, i>(u);
+            const float cf = dotc<Q, i>(F);
+            const float a = g[i] + g[o];
+            const float b = g[i] - g[o];
+            const float hb = hm * b;
+            fb[i] = g[i] - (hp * a + hb);
+            fb[o] = g[o] - (hp * a - hb);
+            const float wa_i = w * a, wb_i = w * b;
+            rb += wa_i * (r1 + r2 * cu * cu) + wb_i * (r3 * cu);
+            wa += wa_i;
+            const float cu_bar = wa_i * (c1 * cu + c2 * cf) + wb_i * c3;
+            const float cf_bar = wa_i * (c2 * cu) + wb_i * c4;
+            constexpr int e0 = cvec<Q>(i, 0), e1 = cvec<Q>(i, 1), e2 = cvec<Q>(i, 2);
+            if constexpr (e0 != 0) {
+                ub0 += e0 > 0 ? cu_bar : -cu_bar;
+                fb0 += e0 > 0 ? cf_bar : -cf_bar;
+            }
+            if constexpr (e1 != 0) {
+                ub1 += e1 > 0 ? cu_bar : -cu_bar;
+                fb1 += e1 > 0 ? cf_bar : -cf_bar;
+            }
+            if constexpr (e2 != 0) {
+                ub2 += e2 > 0 ? cu_bar : -cu_bar;
+                fb2 += e2 > 0 ? cf_bar : -cf_bar;
+            }
+        }
+    });
+    // usq_bar = -1.5 omega rho sum w a; uf_bar = -3 kp sum w a
+    const float usq_bar = -1.5f * omega * rho * wa;
+    const float uf_bar = -3.0f * kp * wa;
+    u_bar[0] = ub0 + (usq_bar * u[0] * 2.0f + uf_bar * F[0]);
+    u_bar[1] = ub1 + (usq_bar * u[1] * 2.0f + uf_bar * F[1]);
+    u_bar[2] = ub2 + (usq_bar * u[2] * 2.0f + uf_bar * F[2]);
+    F_bar[0] = fb0 + uf_bar * u[0];
+    F_bar[1] = fb1 + uf_bar * u[1];
+    F_bar[2] = fb2 + uf_bar * u[2];
+    rho_bar = rb;
+}
+
 // Chain the collision cotangents through the Brinkman macroscopic map back to f and
 // alpha: psm.rs:317-330 and :455-469 with m = 0, dp_bar = 0.
 template <int Q, class C>
@@ -511,30 +580,37 @@ LBM_DEV void brinkman_macro_shifted(const C* d, C alpha, const C a[3], Macro<Q, 
 
 template <int Q, class C>
 LBM_DEV void collide_pairs_shifted(const C* d, const Macro<Q, C>& m, C omega, C om, C* out) {
+    // Pair (i, o = opp(i)) with s = d_i + d_o, t = d_i - d_o, cu = c_i.u, cf = c_i.F:
+    //   aa = kp s+ - omega (s/2 - e+) = w (omega A - 3 kp uf) + cu (4.5 omega w rho cu + 9 kp w cf) - omega/2 s
+    //   bb = km s- - om (t/2 - e-)    = 3 w om rho cu + 3 w km cf - om/2 t
+    // with A = drho - 1.5 rho |u|^2 (so e+ - w = w (A + 4.5 rho cu^2)); out_i = d_i + aa + bb,
+    // out_o = d_o + aa - bb.  The per-weight coefficients are formed once per cell.
     const C* u = m.u;
     const C* F = m.F;
     const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const C uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
     const C kp = C(1) - omega * C(0.5);
     const C km = C(1) - om * C(0.5);
+    const C A = m.drho - C(1.5) * m.rho * usq;
+    const C hp = omega * C(0.5), hm = om * C(0.5);
+    const C p0 = omega * A - C(3) * kp * uf;  // times w
+    const C pc = C(4.5) * omega * m.rho;      // times w
+    const C pf = C(9) * kp;                   // times w
+    const C mc = C(3) * om * m.rho;           // times w
+    const C mf = C(3) * km;                   // times w
     constexpr double w0 = weight<Q>(0);
-    const C e0 = C(w0) * (m.drho - C(1.5) * m.rho * usq);  // eq0 - w0
-    out[0] = d[0] - omega * (d[0] - e0) - kp * uf * C(3.0 * w0);
+    out[0] = d[0] - omega * d[0] + C(w0) * p0;
     static_for<1, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int o = opp<Q>(i);
         if constexpr (o > i) {
-            constexpr double w = weight<Q>(i);
+            const C w = C(weight<Q>(i));
             const C cu = dotc<Q, i>(u);
             const C cf = dotc<Q, i>(F);
-            const C ep = C(w) * (m.drho + m.rho * (C(4.5) * cu * cu - C(1.5) * usq));  // eq+ - w
-            const C em = C(3.0 * w) * m.rho * cu;                                       // eq-
-            const C sp = (cu * cf * C(9.0) - uf * C(3.0)) * C(w);
-            const C sm = cf * C(3.0 * w);
-            const C dp = (d[i] + d[o]) * C(0.5);
-            const C dm = (d[i] - d[o]) * C(0.5);
-            const C aa = kp * sp - omega * (dp - ep);
-            const C bb = km * sm - om * (dm - em);
+            const C sp = d[i] + d[o];
+            const C tm = d[i] - d[o];
+            const C aa = w * (p0 + cu * (pc * cu + pf * cf)) - hp * sp;
+            const C bb = w * (mc * cu + mf * cf) - hm * tm;
             out[i] = d[i] + aa + bb;
             out[o] = d[o] + aa - bb;
         }

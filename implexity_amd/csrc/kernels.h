@@ -471,7 +471,8 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
     mv = wave_sum(mv);
     umax = wave_max(umax);
     const double nbad = double(__popcll(__ballot(bad)));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;  // blockDim.x is a multiple of 64
+    const int lane = tid & 63, wid = tid >> 6;
     if (lane == 0) {
         red[0][wid] = jv;
         red[1][wid] = mv;
@@ -479,8 +480,8 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
         red[3][wid] = nbad;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
+    if (tid == 0) {
+        const int nw = int(blockDim.x * blockDim.y + 63) >> 6;
         double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
         for (int w = 0; w < nw; w++) {
             a += red[0][w];
@@ -501,11 +502,12 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
 LBM_DEV void block_max_partial(float v, double* part) {
     __shared__ float red[16];
     v = wave_max(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int lane = tid & 63, wid = tid >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
+    if (tid == 0) {
+        const int nw = int(blockDim.x * blockDim.y + 63) >> 6;
         float m = 0.0f;
         for (int w = 0; w < nw; w++) m = fmaxf(m, red[w]);
         part[4 * block_linear() + 2] = double(m);
@@ -726,7 +728,11 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
     } else {
         C cb_rho, cb_u[3], cb_F[3];
         if constexpr (KIND == K_TRT || KIND == K_BGK) {
-            collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u, cb_F);
+            if constexpr (std::is_same<C, float>::value)
+                collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,
+                                          cb_F);
+            else
+                collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u, cb_F);
         } else if constexpr (KIND == K_MRT) {
             collide_mrt_vjp<Q>(m.rho, m.u, m.F, mrt, g, fb, cb_rho, cb_u, cb_F);
         } else {
@@ -747,8 +753,8 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
     using C = typename S::C;
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    const bool active = k < g.nz;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    const bool active = k < g.nz && j < g.ny;
     double jsum = 0.0, msum = 0.0;
     float umax = 0.0f;
     unsigned bad = 0;
@@ -798,9 +804,9 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     using C = typename S::C;
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     float lmax = 0.0f;  // max |lambda| written (FP16 cotangents; a NaN makes it inf)
-    if (k < g.nz) {
+    if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
@@ -1111,9 +1117,9 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
     using C = typename S::C;
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     double jsum = 0.0, msum = 0.0;
-    if (k < g.nz) {
+    if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
@@ -1332,8 +1338,8 @@ __global__ __launch_bounds__(256) void k_get_state(Geom g, const void* pop_, con
     using S = Store<P>;
     using C = typename S::C;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) {
@@ -1353,8 +1359,8 @@ __global__ __launch_bounds__(256) void k_set_state(Geom g, void* pop_, const uin
     using S = Store<P>;
     using C = typename S::C;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
@@ -1369,8 +1375,8 @@ template <int LP, int Q, bool MASK>
 __global__ __launch_bounds__(256) void k_get_lam(Geom g, const void* lam_, const uint32_t* mask, int odd, double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) {
@@ -1391,8 +1397,8 @@ template <int LP, int Q, bool MASK>
 __global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos, const int* lsc) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
@@ -1412,8 +1418,8 @@ __global__ __launch_bounds__(256) void k_fields(Geom g, const void* pop_, const 
     using S = Store<P>;
     using C = typename S::C;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= g.nz) return;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     C f[Q];

@@ -18,8 +18,12 @@ struct LaunchCfg {
 
 struct Runner {
     virtual ~Runner() = default;
-    virtual hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c) const = 0;
-    virtual hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c) const = 0;
+    // `after`, when set, is recorded right after the step kernel (before any fold of its
+    // partials), so that the kernel alone can be timed from an event recorded before it.
+    virtual hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c,
+                               hipEvent_t after = nullptr) const = 0;
+    virtual hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c,
+                               hipEvent_t after = nullptr) const = 0;
     virtual hipError_t objective(int Q, bool mask, const StepArgs& a, const ObjArgs& seed, void* lam, double* abar,
                                  int do_seed, const LaunchCfg& c) const = 0;
     virtual hipError_t port_apply(int Q, bool mask, const PortArgs& a, void* pop, double* exch, hipStream_t s) const = 0;
@@ -184,7 +188,8 @@ struct RunnerImpl final : Runner {
         return hipGetLastError();
     }
 
-    hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c) const override {
+    hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c,
+                       hipEvent_t after) const override {
         if (!ok_step(kind, mask, a)) return hipErrorInvalidValue;
         LaunchCfg used = c;
         hipError_t e;
@@ -192,10 +197,12 @@ struct RunnerImpl final : Runner {
             e = mask ? fwd_q<19, true>(kind, a, c, used) : fwd_q<19, false>(kind, a, c, used);
         else
             e = mask ? fwd_q<27, true>(kind, a, c, used) : fwd_q<27, false>(kind, a, c, used);
+        if (e == hipSuccess && after) e = hipEventRecord(after, c.stream);
         if (e != hipSuccess || !a.part) return e;
         return reduce(a, used, a.diag != 0, a.diag != 0);
     }
-    hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c) const override {
+    hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c,
+                       hipEvent_t after) const override {
         if (!ok_adj(kind, mask, a)) return hipErrorInvalidValue;
         if (LP == P_F16S && blocks(c) > a.npart) return hipErrorInvalidValue;
         hipError_t e;
@@ -203,6 +210,7 @@ struct RunnerImpl final : Runner {
             e = mask ? adj_q<19, true>(kind, a, c) : adj_q<19, false>(kind, a, c);
         else
             e = mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
+        if (e == hipSuccess && after) e = hipEventRecord(after, c.stream);
         if (LP != P_F16S || e != hipSuccess) return e;
         // The per-cell kernel ran on c (FP16 cotangents never take the vector path).
         hipLaunchKernelGGL(k_fold<FOLD_LAMBDA>, dim3(fold_blocks(blocks(c))), dim3(256), 0, c.stream, a.part, blocks(c),

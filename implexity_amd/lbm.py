@@ -104,7 +104,8 @@ class Stats(ctypes.Structure):
         ("snapshot_slots", ctypes.c_uint64), ("peak_snapshots", ctypes.c_uint64),
         ("snapshot_copies", ctypes.c_uint64), ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double), ("max_mach", ctypes.c_double), ("mass_drift", ctypes.c_double),
-        ("bad_cells", ctypes.c_uint64),
+        ("bad_cells", ctypes.c_uint64), ("forward_kernel_avg_ms", ctypes.c_double),
+        ("adjoint_kernel_avg_ms", ctypes.c_double), ("kernel_samples", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -155,6 +155,9 @@ typedef struct {
     double max_mach;             /* sqrt(3) max |u| over the first sweep (admit, solver.rs:742-751) */
     double mass_drift;           /* max relative drift of fluid mass incl. port exchange */
     uint64_t bad_cells;          /* cells with a non-finite or non-positive population */
+    double forward_kernel_avg_ms; /* forward step kernel alone, HIP events on every 64th step */
+    double adjoint_kernel_avg_ms; /* adjoint step kernel alone, HIP events on every 64th step */
+    uint64_t kernel_samples;      /* steps timed for the two averages (gradient only) */
 } lbm_stats;
 
 typedef struct lbm_handle lbm_handle;
