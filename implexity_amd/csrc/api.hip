@@ -184,6 +184,7 @@ double odd_rate(double omega, double magic) {
 
 struct lbm_handle {
     int Q = 19, kind = K_TRT, prec = LBM_F32, lprec = LBM_F32;
+    bool regularized = false;
     int nx = 0, ny = 0, nz = 0;
     int per[3] = {0, 0, 0};
     double omega = 1.0, om = 1.0, tau = 1.0, magic = 3.0 / 16.0, bulk = NAN, even = NAN;
@@ -294,10 +295,32 @@ ObjArgs stage_obj(const ObjArgs& tmpl, const lbm_objective* o, uint64_t s, uint6
     return r;
 }
 
+long long fluid_cells(const lbm_handle* h) {
+    long long n = h->ncell;
+    if (!h->solid.empty())
+        for (long long x = 0; x < h->ncell; x++) n -= h->solid[x] ? 1 : 0;
+    return n;
+}
+
+// Per-cell weights of w . vector (vector_vjp's weights, solver.rs:905-912) without the
+// design-volume term.
+void vector_weights(const lbm_handle* h, const double w[6], const lbm_vector_scales& sc, ObjArgs& r) {
+    const double fc = double(fluid_cells(h));
+    r.vw[0] = w[0] / fc;
+    r.vw[1] = w[1] / fc;
+    r.vw[2] = w[2] / fc;
+    r.vw[3] = 0.5 * w[3] * sc.cell_volume;
+    r.vw[4] = w[4] * sc.cell_volume;
+    r.vel = sc.velocity;
+    r.dens = sc.density;
+    r.inv_dt = 1.0 / sc.step_s;
+}
+
 ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
     ObjArgs r{};
     r.kind = OBJ_NONE;
     if (!o || o->kind == LBM_OBJ_NONE) return r;
+    if (o->kind == LBM_OBJ_VECTOR) vector_weights(h, o->vector_weights, o->scales, r);
     r.axis = int(o->axis);
     const int len = o->axis == 0 ? h->nx : (o->axis == 1 ? h->ny : h->nz);
     r.plane_a = ((o->plane_a % len) + len) % len;
@@ -325,7 +348,15 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
 
 void validate_objective(const lbm_handle* h, const lbm_objective* o, uint64_t n) {
     if (!o || o->kind == LBM_OBJ_NONE) return;
-    if (o->kind > LBM_OBJ_PRESSURE_DROP) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->kind > LBM_OBJ_VECTOR) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->kind == LBM_OBJ_VECTOR) {
+        for (int k = 0; k < 6; k++)
+            if (!std::isfinite(o->vector_weights[k])) fail(LBM_ERR_CONTRACT, "vector weights must be finite");
+        const lbm_vector_scales& sc = o->scales;
+        if (!(sc.velocity > 0.0) || !(sc.density > 0.0) || !(sc.step_s > 0.0) || !(sc.cell_volume > 0.0) ||
+            !std::isfinite(sc.velocity + sc.density + sc.step_s + sc.cell_volume))
+            fail(LBM_ERR_CONTRACT, "vector scales must be positive and finite");
+    }
     if (o->axis > 2) fail(LBM_ERR_CONTRACT, "objective axis must be 0, 1 or 2");
     if (!std::isfinite(o->weight) || !std::isfinite(o->terminal_weight))
         fail(LBM_ERR_CONTRACT, "objective weights must be finite");
@@ -343,6 +374,13 @@ void ensure_aos(lbm_handle* h) { h->aos.alloc(size_t(h->ncell) * h->Q * sizeof(d
 double seed_bound(const lbm_handle* h, const ObjArgs& o) {
     if (o.kind == OBJ_PRESSURE_DROP) return std::fabs(o.weight) * std::max(o.inv_na, o.inv_nb) / 3.0;
     if (o.kind == OBJ_DISSIPATION) return 16.0 * std::fabs(o.weight) * h->alpha_max;
+    if (o.kind == OBJ_VECTOR) {
+        // phi = sum_d A_d u_d + B rho |u|^2: |d phi/d f_q| <= 80 |B| + 8 sum |A_d| for rho >= 1/2, |u| <= 1
+        const double v2 = o.vel * o.vel * o.dens;
+        const double B = v2 * (std::fabs(o.vw[3]) + std::fabs(o.vw[4]) * h->alpha_max * o.inv_dt);
+        const double A = o.vel * (std::fabs(o.vw[0]) + std::fabs(o.vw[1]) + std::fabs(o.vw[2]));
+        return std::fabs(o.weight) * (80.0 * B + 8.0 * A);
+    }
     return 0.0;
 }
 
@@ -518,8 +556,10 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         auto h = std::make_unique<lbm_handle>();
         if (cfg->lattice != 19 && cfg->lattice != 27) fail(LBM_ERR_CONTRACT, "lattice must be D3Q19 or D3Q27");
         h->Q = int(cfg->lattice);
-        if (cfg->collision > LBM_BGK_GUO) fail(LBM_ERR_CONTRACT, "unknown collision operator");
-        h->kind = int(cfg->collision);
+        if (cfg->collision > LBM_REGULARIZED) fail(LBM_ERR_CONTRACT, "unknown collision operator");
+        // The regularized operator is the moment-space operator with its own rates.
+        h->regularized = cfg->collision == LBM_REGULARIZED;
+        h->kind = h->regularized ? int(K_MRT) : int(cfg->collision);
         if (h->kind == K_CUMULANT && h->Q != 27)
             fail(LBM_ERR_CONTRACT, "cumulant collision requires the D3Q27 velocity set (got D3Q19)");
         if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
@@ -711,6 +751,10 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
                 c.insert(c.end(), norms.begin(), norms.end());
                 for (int k = 0; k < h->Q; k++) {
                     double r = h->omega;
+                    if (h->regularized) {  // bulk and shear relax with omega, the rest is removed
+                        c.push_back((groups[k] == 2 || groups[k] == 3) ? h->omega : 1.0);
+                        continue;
+                    }
                     switch (groups[k]) {
                     case 0:
                     case 3: r = h->omega; break;
@@ -879,6 +923,74 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
     });
 }
 
+namespace {
+void check_scales(const lbm_vector_scales* sc) {
+    if (!sc) fail(LBM_ERR_CONTRACT, "null scales");
+    if (!(sc->velocity > 0.0) || !(sc->density > 0.0) || !(sc->step_s > 0.0) || !(sc->cell_volume > 0.0) ||
+        !std::isfinite(sc->velocity + sc->density + sc->step_s + sc->cell_volume))
+        fail(LBM_ERR_CONTRACT, "vector scales must be positive and finite");
+}
+}  // namespace
+
+int lbm_vector(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, double* out) {
+    return guarded([&] {
+        if (!h || !out) fail(LBM_ERR_CONTRACT, "null argument");
+        check_scales(scales);
+        if (phi && n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi needs one entry per cell");
+        const LaunchCfg c = h->cfg();
+        const size_t nb = size_t(c.grid.x) * c.grid.y * c.grid.z;
+        DevBuf sums;
+        sums.alloc(nb * 5 * sizeof(double), "vector sums");
+        hip_check(h->run->vector(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1), h->alpha.p,
+                                 h->a, scales->velocity, scales->density, 1.0 / scales->step_s, sums.as<double>(), c),
+                  "vector launch");
+        std::vector<double> v(nb * 5);
+        hip_check(hipMemcpyAsync(v.data(), sums.p, sums.bytes, hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (size_t b = 0; b < nb; b++)
+            for (int k = 0; k < 5; k++) acc[k] += v[5 * b + k];
+        double vol = 0.0;
+        if (phi)
+            for (uint64_t x = 0; x < n; x++) vol += phi[x];
+        const double fc = double(fluid_cells(h));
+        out[0] = acc[0] / fc;
+        out[1] = acc[1] / fc;
+        out[2] = acc[2] / fc;
+        out[3] = 0.5 * acc[3] * scales->cell_volume;
+        out[4] = acc[4] * scales->cell_volume;
+        out[5] = vol * scales->cell_volume;
+    });
+}
+
+int lbm_vector_vjp(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, const double* w,
+                   double* f_bar, uint64_t f_len, double* alpha_bar, double* phi_bar) {
+    return guarded([&] {
+        if (!h || !w || !f_bar || !alpha_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        check_scales(scales);
+        if (f_len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "f_bar length must be cells*Q");
+        if (phi && n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi needs one entry per cell");
+        for (int k = 0; k < 6; k++)
+            if (!std::isfinite(w[k])) fail(LBM_ERR_CONTRACT, "vector weights must be finite");
+        ObjArgs ob{};
+        ob.kind = OBJ_VECTOR;
+        ob.weight = 1.0;
+        vector_weights(h, w, *scales, ob);
+        ensure_aos(h);
+        DevBuf ab;
+        ab.alloc(size_t(h->ncell) * sizeof(double), "vector alpha_bar");
+        hip_check(h->run->vector_vjp(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1),
+                                     h->alpha.p, h->a, ob, h->aos.as<double>(), ab.as<double>(), h->cfg()),
+                  "vector vjp launch");
+        hip_check(hipMemcpyAsync(f_bar, h->aos.p, f_len * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipMemcpyAsync(alpha_bar, ab.p, ab.bytes, hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        if (phi_bar)
+            for (long long x = 0; x < h->ncell; x++) phi_bar[x] = w[5] * scales->cell_volume;
+        (void)phi;
+    });
+}
+
 int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n) {
     return guarded([&] {
         if (!h || !rho || !u) fail(LBM_ERR_CONTRACT, "null argument");
@@ -1005,7 +1117,7 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
         hip_check(hipMemcpyAsync(h->aos.p, lambda_next, len * sizeof(double), hipMemcpyHostToDevice, h->stream),
                   "copy");
         hip_check(h->run->set_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int((t + 1) & 1),
-                                  h->aos.as<double>(), h->lsc_ptr(), h->cfg()),
+                                  h->aos.as<double>(), h->lsc_ptr(), 0, h->cfg()),
                   "set_lam");
         if (h->np)
             hip_check(h->run->port_vjp(Q, h->has_mask, h->port_args(t + 1), h->L.p, h->pbuf.p, h->lsc_ptr(),
@@ -1029,9 +1141,28 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
 
 int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
                  uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats) {
+    return lbm_gradient_seeded(h, nsteps, obj, nullptr, 0, J, alpha_bar, n_alpha, lambda0, lambda_len, stats);
+}
+
+int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, const double* lambda_final,
+                        uint64_t lambda_final_len, double* J, double* alpha_bar, uint64_t n_alpha, double* lambda0,
+                        uint64_t lambda_len, lbm_stats* stats) {
     return guarded([&] {
         if (!h) fail(LBM_ERR_CONTRACT, "null handle");
         if (nsteps == 0) fail(LBM_ERR_INVALID, "a checkpoint schedule needs at least one step");
+        double lf_max = 0.0;  // max |lambda_final|
+        if (lambda_final) {
+            if (lambda_final_len != uint64_t(h->ncell) * h->Q)
+                fail(LBM_ERR_SHAPE, "lambda_final length must be cells*Q");
+            for (uint64_t x = 0; x < lambda_final_len; x++) {
+                if (!std::isfinite(lambda_final[x])) fail(LBM_ERR_CONTRACT, "lambda_final must be finite");
+                lf_max = std::max(lf_max, std::fabs(lambda_final[x]));
+            }
+            ensure_aos(h);
+            hip_check(hipMemcpyAsync(h->aos.p, lambda_final, lambda_final_len * sizeof(double),
+                                     hipMemcpyHostToDevice, h->stream),
+                      "copy");
+        }
         if (alpha_bar && n_alpha != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "alpha_bar needs one entry per cell");
         if (lambda0 && lambda_len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "lambda0 length must be cells*Q");
         validate_objective(h, obj, nsteps);
@@ -1119,11 +1250,17 @@ int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, doubl
                         ea.obj = forward_phase ? stage_obj(tmpl, obj, nsteps, nsteps, true) : none;
                         ea.macc = led.mass(nsteps);
                         const ObjArgs seed = stage_obj(tmpl, obj, nsteps, nsteps, true);
-                        lam_init(h, seed_bound(h, seed), seed_bound(h, stage_obj(tmpl, obj, nsteps - 1, nsteps, false)));
+                        lam_init(h, seed_bound(h, seed) + lf_max,
+                                 seed_bound(h, stage_obj(tmpl, obj, nsteps - 1, nsteps, false)));
                         ea.lsc = h->lsc_ptr();
                         hip_check(h->run->objective(Q, h->has_mask, ea, seed, h->L.p, h->alpha_bar.as<double>(), 1,
                                                     h->cfg()),
                                   "seed launch");
+                        if (lambda_final)  // lambda_n += lambda_final (the caller's terminal cotangent)
+                            hip_check(h->run->set_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(),
+                                                      int((base + nsteps) & 1), h->aos.as<double>(), h->lsc_ptr(),
+                                                      1, h->cfg()),
+                                      "lambda_final upload");
                         if (h->np)
                             hip_check(h->run->port_save(Q, h->has_mask, h->port_args(base + nsteps), h->A.p,
                                                         h->pbuf.p, h->stream),

@@ -43,7 +43,7 @@ namespace lbm {
 
 enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
 enum Kind : int { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5 };
-enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2 };
+enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
 
@@ -63,6 +63,11 @@ struct ObjArgs {
     int axis, plane_a, plane_b;
     double weight;
     double inv_na, inv_nb;
+    // OBJ_VECTOR: per-cell weights of the terminal responses (solver.rs:896-915):
+    // [w0/fluid, w1/fluid, w2/fluid, 0.5 w3 dV, w4 dV] on [u_x, u_y, u_z, rho|u|^2,
+    // rho alpha/dt |u|^2] in physical units (u * vel, rho * dens).
+    double vw[5];
+    double vel, dens, inv_dt;
 };
 
 struct StepArgs {
@@ -620,6 +625,14 @@ LBM_DEV double objective_cell(const ObjArgs& ob, const C* f, const Macro<Q, C>& 
         const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
         return ob.weight * double(m.rho * alpha * usq);
     }
+    if (ob.kind == OBJ_VECTOR) {
+        const double u0 = double(m.u[0]) * ob.vel, u1 = double(m.u[1]) * ob.vel, u2 = double(m.u[2]) * ob.vel;
+        const double r = double(m.rho) * ob.dens;
+        const double usq = u0 * u0 + u1 * u1 + u2 * u2;
+        const double v = ob.vw[0] * u0 + ob.vw[1] * u1 + ob.vw[2] * u2 + ob.vw[3] * (r * usq) +
+                         ob.vw[4] * (r * double(alpha) * ob.inv_dt * usq);
+        return ob.weight * v;
+    }
     if (ob.kind == OBJ_PRESSURE_DROP) {
         const double s = coord_axis == ob.plane_a ? ob.inv_na : (coord_axis == ob.plane_b ? -ob.inv_nb : 0.0);
         if (s == 0.0) return 0.0;
@@ -647,6 +660,35 @@ LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, in
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const C ub = usq_bar * C(2.0) * m.u[d];
+            al_bar -= qdiv(ub * m.sv[d], m.den * m.den) * C(0.5);
+            const C sb = qdiv(ub, m.den);
+            jb[d] = qdiv(sb, m.rho);
+            rho_bar -= qdiv(sb * m.j[d], m.rho * m.rho);
+        }
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            C v = rho_bar;
+            if constexpr (cvec<Q>(i, 0) != 0) v += jb[0] * C(cvec<Q>(i, 0));
+            if constexpr (cvec<Q>(i, 1) != 0) v += jb[1] * C(cvec<Q>(i, 1));
+            if constexpr (cvec<Q>(i, 2) != 0) v += jb[2] * C(cvec<Q>(i, 2));
+            fb[i] += v;
+        });
+        ab += al_bar;
+    } else if (ob.kind == OBJ_VECTOR) {
+        // phi = sum_d A_d u_d + B rho |u|^2 with A_d = w vw_d vel and
+        // B = w dens vel^2 (vw3 + vw4 alpha / dt); then the Brinkman chain as above.
+        const C w = C(ob.weight);
+        const C V = C(ob.vel);
+        const C bk = w * C(ob.dens) * V * V * C(ob.vw[3]);
+        const C bd = w * C(ob.dens) * V * V * C(ob.vw[4]) * C(ob.inv_dt);
+        const C B = bk + bd * alpha;
+        const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
+        C rho_bar = B * usq;
+        C al_bar = bd * m.rho * usq;
+        C jb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const C ub = w * C(ob.vw[d]) * V + C(2) * B * m.rho * m.u[d];
             al_bar -= qdiv(ub * m.sv[d], m.den * m.den) * C(0.5);
             const C sb = qdiv(ub, m.den);
             jb[d] = qdiv(sb, m.rho);
@@ -1394,7 +1436,8 @@ __global__ __launch_bounds__(256) void k_get_lam(Geom g, const void* lam_, const
 }
 
 template <int LP, int Q, bool MASK>
-__global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos, const int* lsc) {
+__global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint32_t* mask, int odd, const double* aos,
+                                                const int* lsc, int accumulate) {
     using L = LamStore<LP>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
@@ -1407,11 +1450,100 @@ __global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint3
     const LScale sc = lam_scale<LP>(lsc, false);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        L::template st<double>(lam + read_loc<Q, q>(g, cu, m, odd), aos[cell * Q + q], sc);
+        typename L::T* p = lam + read_loc<Q, q>(g, cu, m, odd);
+        const double prev = accumulate ? L::template ld<double>(p, sc) : 0.0;
+        L::template st<double>(p, prev + aos[cell * Q + q], sc);
     });
 }
 
 // Macroscopic fields (solver.rs:690-705 in lattice units): rho, and u (0 in solids).
+// Terminal responses of a state (solver.rs:874-894): per block the sums of
+// [u_x, u_y, u_z, rho|u|^2, rho alpha/dt |u|^2] (physical units, fluid cells) into
+// sums[5 * block + 0..4], folded on the host in block order.
+template <int P, int Q, bool MASK>
+__global__ __launch_bounds__(256) void k_vector(Geom g, const void* pop_, const uint32_t* mask, int odd,
+                                               const void* alpha_, double a0, double a1, double a2, double vel,
+                                               double dens, double inv_dt, double* sums) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (k < g.nz && j < g.ny) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+        if (!(m & SOLID_BIT)) {
+            const Cursor cu = make_cursor(g, i, j, k);
+            C f[Q];
+            load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
+            const C alpha = static_cast<const C*>(alpha_)[cell];
+            const C a[3] = {C(a0), C(a1), C(a2)};
+            Macro<Q, C> mac;
+            macro_of<Q, C, S::shifted>(f, alpha, a, mac);
+            const double u0 = double(mac.u[0]) * vel, u1 = double(mac.u[1]) * vel, u2 = double(mac.u[2]) * vel;
+            const double r = double(mac.rho) * dens;
+            const double usq = u0 * u0 + u1 * u1 + u2 * u2;
+            v[0] = u0;
+            v[1] = u1;
+            v[2] = u2;
+            v[3] = r * usq;
+            v[4] = r * double(alpha) * inv_dt * usq;
+        }
+    }
+    __shared__ double red[5][4];
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const double t = wave_sum(v[c]);
+        if ((tid & 63) == 0) red[c][tid >> 6] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = int(blockDim.x * blockDim.y + 63) >> 6;
+        for (int c = 0; c < 5; c++) {
+            double acc = 0.0;
+            for (int w = 0; w < nw; w++) acc += red[c][w];
+            sums[5 * block_linear() + c] = acc;
+        }
+    }
+}
+
+// vector_vjp (solver.rs:896-947) without the design-volume term: f_bar (reference AoS,
+// w.r.t. f) and alpha_bar of ob (an OBJ_VECTOR objective of weight 1) at every cell;
+// zero at solid cells.
+template <int P, int Q, bool MASK>
+__global__ __launch_bounds__(256) void k_vector_vjp(Geom g, const void* pop_, const uint32_t* mask, int odd,
+                                                   const void* alpha_, double a0, double a1, double a2, ObjArgs ob,
+                                                   double* f_bar, double* alpha_bar) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) f_bar[cell * Q + q] = 0.0;
+        alpha_bar[cell] = 0.0;
+        return;
+    }
+    const Cursor cu = make_cursor(g, i, j, k);
+    C f[Q], fb[Q];
+    load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
+    const C alpha = static_cast<const C*>(alpha_)[cell];
+    const C a[3] = {C(a0), C(a1), C(a2)};
+    Macro<Q, C> mac;
+    macro_of<Q, C, S::shifted>(f, alpha, a, mac);
+#pragma unroll
+    for (int q = 0; q < Q; q++) fb[q] = C(0);
+    C ab = C(0);
+    objective_grad<Q, C>(ob, mac, alpha, 0, fb, ab);
+#pragma unroll
+    for (int q = 0; q < Q; q++) f_bar[cell * Q + q] = double(fb[q]);
+    alpha_bar[cell] = double(ab);
+}
+
 template <int P, int Q, bool MASK>
 __global__ __launch_bounds__(256) void k_fields(Geom g, const void* pop_, const uint32_t* mask, int odd, const void* alpha_, double a0,
                          double a1, double a2, double* rho, double* u) {

@@ -38,7 +38,13 @@ struct Runner {
     virtual hipError_t get_lam(int Q, bool mask, const Geom& g, const void* lam, const uint32_t* m, int odd,
                                double* aos, const int* lsc, const LaunchCfg& c) const = 0;
     virtual hipError_t set_lam(int Q, bool mask, const Geom& g, void* lam, const uint32_t* m, int odd,
-                               const double* aos, const int* lsc, const LaunchCfg& c) const = 0;
+                               const double* aos, const int* lsc, int accumulate, const LaunchCfg& c) const = 0;
+    virtual hipError_t vector(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
+                              const void* alpha, const double a[3], double vel, double dens, double inv_dt,
+                              double* sums, const LaunchCfg& c) const = 0;
+    virtual hipError_t vector_vjp(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
+                                  const void* alpha, const double a[3], const ObjArgs& ob, double* f_bar,
+                                  double* alpha_bar, const LaunchCfg& c) const = 0;
     virtual hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
                               const void* alpha, const double a[3], double* rho, double* u,
                               const LaunchCfg& c) const = 0;
@@ -309,14 +315,36 @@ struct RunnerImpl final : Runner {
         LBM_LAUNCH((k_get_lam<LP, 27, false>), c, g, lam, m, odd, aos, lsc);
     }
     hipError_t set_lam(int Q, bool mask, const Geom& g, void* lam, const uint32_t* m, int odd, const double* aos,
-                       const int* lsc, const LaunchCfg& c) const override {
+                       const int* lsc, int acc, const LaunchCfg& c) const override {
         if (!lam || !aos || (mask && !m) || (LP == P_F16S && !lsc)) return hipErrorInvalidValue;
         if (Q == 19) {
-            if (mask) LBM_LAUNCH((k_set_lam<LP, 19, true>), c, g, lam, m, odd, aos, lsc);
-            LBM_LAUNCH((k_set_lam<LP, 19, false>), c, g, lam, m, odd, aos, lsc);
+            if (mask) LBM_LAUNCH((k_set_lam<LP, 19, true>), c, g, lam, m, odd, aos, lsc, acc);
+            LBM_LAUNCH((k_set_lam<LP, 19, false>), c, g, lam, m, odd, aos, lsc, acc);
         }
-        if (mask) LBM_LAUNCH((k_set_lam<LP, 27, true>), c, g, lam, m, odd, aos, lsc);
-        LBM_LAUNCH((k_set_lam<LP, 27, false>), c, g, lam, m, odd, aos, lsc);
+        if (mask) LBM_LAUNCH((k_set_lam<LP, 27, true>), c, g, lam, m, odd, aos, lsc, acc);
+        LBM_LAUNCH((k_set_lam<LP, 27, false>), c, g, lam, m, odd, aos, lsc, acc);
+    }
+    hipError_t vector(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, const void* alpha,
+                      const double a[3], double vel, double dens, double inv_dt, double* sums,
+                      const LaunchCfg& c) const override {
+        if (!pop || !alpha || !sums || (mask && !m)) return hipErrorInvalidValue;
+        if (Q == 19) {
+            if (mask) LBM_LAUNCH((k_vector<P, 19, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], vel, dens, inv_dt, sums);
+            LBM_LAUNCH((k_vector<P, 19, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], vel, dens, inv_dt, sums);
+        }
+        if (mask) LBM_LAUNCH((k_vector<P, 27, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], vel, dens, inv_dt, sums);
+        LBM_LAUNCH((k_vector<P, 27, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], vel, dens, inv_dt, sums);
+    }
+    hipError_t vector_vjp(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
+                          const void* alpha, const double a[3], const ObjArgs& ob, double* f_bar, double* alpha_bar,
+                          const LaunchCfg& c) const override {
+        if (!pop || !alpha || !f_bar || !alpha_bar || (mask && !m)) return hipErrorInvalidValue;
+        if (Q == 19) {
+            if (mask) LBM_LAUNCH((k_vector_vjp<P, 19, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
+            LBM_LAUNCH((k_vector_vjp<P, 19, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
+        }
+        if (mask) LBM_LAUNCH((k_vector_vjp<P, 27, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
+        LBM_LAUNCH((k_vector_vjp<P, 27, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
     }
     hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, const void* alpha,
                       const double a[3], double* rho, double* u, const LaunchCfg& c) const override {

@@ -30,9 +30,9 @@ LIB_PATH = os.path.join(_HERE, "libimplexity_lbm.so")
 
 PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
 ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
-COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4}
+COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
-OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2}
+OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3}
 MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
 
 
@@ -90,11 +90,23 @@ class _Config(ctypes.Structure):
     ]
 
 
+class _Scales(ctypes.Structure):
+    _fields_ = [("velocity", ctypes.c_double), ("density", ctypes.c_double), ("step_s", ctypes.c_double),
+                ("cell_volume", ctypes.c_double)]
+
+
+def _scales(sc):
+    v = _Scales()
+    v.velocity, v.density, v.step_s, v.cell_volume = (float(x) for x in sc)
+    return v
+
+
 class _Objective(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_uint32), ("axis", ctypes.c_uint32), ("plane_a", ctypes.c_int32),
         ("plane_b", ctypes.c_int32), ("t_begin", ctypes.c_uint64), ("t_end", ctypes.c_uint64),
         ("weight", ctypes.c_double), ("terminal_weight", ctypes.c_double),
+        ("vector_weights", ctypes.c_double * 6), ("scales", _Scales),
     ]
 
 
@@ -130,6 +142,12 @@ _SIGS = {
                     ctypes.c_int),
     "lbm_gradient": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
+    "lbm_gradient_seeded": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                             ctypes.c_void_p], ctypes.c_int),
+    "lbm_vector": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "lbm_vector_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "lbm_step_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                      ctypes.c_int),
     "lbm_reserve": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
@@ -216,9 +234,15 @@ class Objective:
     axis: int = 0
     plane_a: int = 1
     plane_b: int = -2
+    # "vector": phi = w . vector(state) (LbmProblem::vector, solver.rs:874-894) with
+    # scales = (velocity, density, step_s, cell_volume); w[5] (design volume) is not in J
+    vector_weights: tuple = (0.0,) * 6
+    scales: tuple = (1.0, 1.0, 1.0, 1.0)
 
     def _c(self):
         o = _Objective()
+        o.vector_weights = (ctypes.c_double * 6)(*[float(x) for x in self.vector_weights])
+        o.scales = _scales(self.scales)
         o.kind = OBJECTIVE[self.kind]
         o.axis = self.axis
         o.plane_a = self.plane_a
@@ -338,17 +362,41 @@ class Lattice:
         _check(lib().lbm_reserve(self._h, steps, ctypes.byref(slots)))
         return slots.value
 
-    def gradient(self, steps, objective: Objective, want_lambda0=False):
+    def gradient(self, steps, objective: Objective | None, want_lambda0=False, lambda_final=None):
         """Forward + discrete adjoint from the current state (binomial checkpointing).
+        lambda_final (reference AoS, optional) is added to the final state's cotangent, as
+        history_adjoint's lambda_final (solver.rs:798-815).
         Returns (J, alpha_bar, lambda0 or None, stats)."""
         J = ctypes.c_double(0.0)
         st = Stats()
         ab = np.zeros(self.n)
         lam = np.zeros(self.n * self.Q) if want_lambda0 else None
-        o = objective._c()
-        _check(lib().lbm_gradient(self._h, steps, ctypes.byref(o), ctypes.byref(J), _p(ab), ab.size, _p(lam),
-                                  0 if lam is None else lam.size, ctypes.byref(st)))
+        o = objective._c() if objective is not None else None
+        lf = None if lambda_final is None else np.ascontiguousarray(lambda_final, np.float64).reshape(-1)
+        _check(lib().lbm_gradient_seeded(self._h, steps, ctypes.byref(o) if o is not None else None, _p(lf),
+                                         0 if lf is None else lf.size, ctypes.byref(J), _p(ab), ab.size, _p(lam),
+                                         0 if lam is None else lam.size, ctypes.byref(st)))
         return J.value, ab, lam, st
+
+    def vector(self, scales, phi=None):
+        """Terminal responses of the current state (solver.rs:874-894)."""
+        out = np.zeros(6)
+        ph = None if phi is None else np.ascontiguousarray(phi, np.float64).reshape(-1)
+        sc = _scales(scales)
+        _check(lib().lbm_vector(self._h, _p(ph), 0 if ph is None else ph.size, ctypes.byref(sc), _p(out)))
+        return out
+
+    def vector_vjp(self, scales, w, phi=None):
+        """(f_bar, alpha_bar, phi_bar) of w . vector at the current state (solver.rs:896-947)."""
+        ph = None if phi is None else np.ascontiguousarray(phi, np.float64).reshape(-1)
+        ww = np.ascontiguousarray(w, np.float64)
+        fb = np.zeros(self.n * self.Q)
+        ab = np.zeros(self.n)
+        pb = np.zeros(self.n)
+        sc = _scales(scales)
+        _check(lib().lbm_vector_vjp(self._h, _p(ph), 0 if ph is None else ph.size, ctypes.byref(sc), _p(ww), _p(fb),
+                                    fb.size, _p(ab), _p(pb)))
+        return fb, ab, pb
 
     def interval_vjp(self, lambda_next):
         lam = np.ascontiguousarray(lambda_next, np.float64).reshape(-1)

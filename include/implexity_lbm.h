@@ -66,7 +66,8 @@ typedef enum {
     LBM_TRT = 1,
     LBM_MRT = 2,
     LBM_CUMULANT = 3,
-    LBM_BGK_GUO = 4
+    LBM_BGK_GUO = 4,
+    LBM_REGULARIZED = 5  /* CollisionKind::Regularized (collision.rs:376-394,419-431) */
 } lbm_collision;
 
 /* Population storage.  Arithmetic is FP64 for LBM_F64 and FP32 otherwise. */
@@ -130,7 +131,22 @@ typedef struct {
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
  *             + terminal_weight * phi(final state), states indexed 1..n (scan.rs:96-107). */
-typedef enum { LBM_OBJ_NONE = 0, LBM_OBJ_DISSIPATION = 1, LBM_OBJ_PRESSURE_DROP = 2 } lbm_objective_kind;
+typedef enum {
+    LBM_OBJ_NONE = 0,
+    LBM_OBJ_DISSIPATION = 1,   /* sum over fluid cells of rho alpha |u|^2 (lattice units) */
+    LBM_OBJ_PRESSURE_DROP = 2, /* mean rho/3 on plane a minus mean rho/3 on plane b */
+    LBM_OBJ_VECTOR = 3         /* w . vector(state), the terminal responses of solver.rs:874-894
+                                  (the design-volume term w[5] is not state-dependent and is
+                                  left out of J; lbm_vector reports it) */
+} lbm_objective_kind;
+
+/* Physical scales of the terminal responses (LbmProblem::vector_scales, solver.rs:862-871). */
+typedef struct {
+    double velocity;    /* spacing / step (m/s per lattice unit) */
+    double density;     /* kg/m^3 per lattice unit */
+    double step_s;      /* time step (s) */
+    double cell_volume; /* spacing^3 (m^3) */
+} lbm_vector_scales;
 
 typedef struct {
     uint32_t kind;        /* lbm_objective_kind */
@@ -141,6 +157,8 @@ typedef struct {
     uint64_t t_end;       /* last state carrying the stage cost (<= n) */
     double weight;        /* stage weight */
     double terminal_weight;
+    double vector_weights[6];     /* LBM_OBJ_VECTOR: w */
+    lbm_vector_scales scales;     /* LBM_OBJ_VECTOR */
 } lbm_objective;
 
 typedef struct {
@@ -186,6 +204,25 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
  * handle's state is left at state 0. */
 int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
                  uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats);
+
+/* lbm_gradient with an additional cotangent lambda_final (AoS N*Q, in the reference
+ * layout) added to the seed of the final state: the drop-in for
+ * LbmProblem::history_adjoint(history, alpha, tau, lambda_final) (solver.rs:798-815),
+ * where the caller computed lambda_final (e.g. with lbm_vector_vjp).  obj may be NULL. */
+int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, const double* lambda_final,
+                        uint64_t lambda_final_len, double* J, double* alpha_bar, uint64_t n_alpha, double* lambda0,
+                        uint64_t lambda_len, lbm_stats* stats);
+
+/* Terminal responses of the current state (LbmProblem::vector, solver.rs:874-894):
+ * out = [mean u_x, mean u_y, mean u_z over fluid cells, 0.5 sum rho |u|^2 dV,
+ * sum rho alpha/dt |u|^2 dV, sum phi dV], u and rho scaled to physical units.  phi
+ * (n = cells) may be NULL (volume 0).  Sums are FP64. */
+int lbm_vector(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, double* out);
+
+/* VJP of w . vector at the current state (LbmProblem::vector_vjp, solver.rs:896-947):
+ * f_bar (AoS N*Q), alpha_bar (N), phi_bar (N, may be NULL). */
+int lbm_vector_vjp(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, const double* w,
+                   double* f_bar, uint64_t f_len, double* alpha_bar, double* phi_bar);
 
 /* Allocate the adjoint buffers and the revolve snapshot pool for a horizon of nsteps
  * (min(budget / state bytes, nsteps) slots) ahead of lbm_gradient, so that the

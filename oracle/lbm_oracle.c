@@ -102,7 +102,7 @@ int orc_lattice(int Q, int *c_out, double *w_out, int *opp_out) {
 /* ------------------------------------------------------------------------- */
 /* Problem description (plain data filled by oracle/oracle.py).               */
 /* ------------------------------------------------------------------------- */
-enum { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4 };
+enum { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_REGULARIZED = 5 };
 
 typedef struct {
     int32_t Q, kind;
@@ -433,8 +433,14 @@ int orc_mrt_basis(int Q, double *rows_out, double *norms_out, int32_t *groups_ou
     return 0;
 }
 
+/* MRT arm of relax (collision.rs:349-375) and the Regularized arm (:376-394): bulk and
+ * shear moments relax with omega, every other moment is removed (rate 1). */
 static void mrt_rates(const orc_problem *p, double *rate) {
     double om = orc_odd_rate(p->omega, p->magic);
+    if (p->kind == K_REGULARIZED) {
+        for (int k = 0; k < p->Q; k++) rate[k] = (p->mrt_groups[k] == 2 || p->mrt_groups[k] == 3) ? p->omega : 1.0;
+        return;
+    }
     for (int k = 0; k < p->Q; k++) {
         switch (p->mrt_groups[k]) {
         case 0:
@@ -915,7 +921,8 @@ void orc_cell_collide(const orc_problem *p, const double *f, double alpha, doubl
     switch (p->kind) {
     case K_BGK: collide_pairs(L, f, rho, u, F, p->omega, p->omega, out); break;
     case K_TRT: collide_pairs(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), out); break;
-    case K_MRT: collide_mrt(p, L, f, rho, u, F, out); break;
+    case K_MRT:
+    case K_REGULARIZED: collide_mrt(p, L, f, rho, u, F, out); break;
     case K_CUMULANT: collide_cumulant(f, rho, u, F, p->omega, isnan(p->bulk_rate) ? 1.0 : p->bulk_rate, out); break;
     }
 }
@@ -940,7 +947,8 @@ void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const dou
         collide_pairs_vjp(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), g, cb_f, &cb_rho, cb_u,
                           cb_F);
         break;
-    case K_MRT: collide_mrt_vjp(p, L, f, rho, u, F, g, cb_f, &cb_rho, cb_u, cb_F); break;
+    case K_MRT:
+    case K_REGULARIZED: collide_mrt_vjp(p, L, f, rho, u, F, g, cb_f, &cb_rho, cb_u, cb_F); break;
     default:
         collide_cumulant_vjp(f, rho, u, F, p->omega, isnan(p->bulk_rate) ? 1.0 : p->bulk_rate, g, cb_f, &cb_rho,
                              cb_u, cb_F);
@@ -1242,5 +1250,116 @@ void orc_obj_pressure_drop_grad(const orc_problem *p, int axis, int plane_a, int
         double gv = c[axis] == plane_a ? ga : (c[axis] == plane_b ? gb : 0.0);
         if (gv == 0.0) continue;
         for (int q = 0; q < Q; q++) f_bar[x * Q + q] += gv;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Terminal responses: vector_cell / vector / vector_vjp (solver.rs:835-947).  */
+/* scales = {velocity, density, step_s, dv}; out = [mean u_x, u_y, u_z over   */
+/* fluid cells, 0.5 sum rho|u|^2 dV, sum rho alpha/dt |u|^2 dV, sum phi dV].   */
+/* The VJP is the forward-mode gradient the reference takes (one tangent per   */
+/* pass over the Q + 2 inputs f, alpha, phi).                                  */
+/* ------------------------------------------------------------------------- */
+static void vec_moments(const orc_problem *p, const double *f, double *rho, double m[3]) {
+    if (p->Q == 19) {
+        st_moments(f, rho, m);
+    } else {
+        moments(lat(p->Q), f, rho, m);
+    }
+}
+
+static void vector_cell(const orc_problem *p, const double *f, double alpha, double phi, int solid,
+                        const double sc[4], double out[6]) {
+    double rho, m[3];
+    vec_moments(p, f, &rho, m);
+    double s = alpha * 0.5 + 1.0;
+    double u[3];
+    for (int d = 0; d < 3; d++) u[d] = (solid ? 0.0 : (m[d] / rho + 0.5 * p->accel[d]) / s) * sc[0];
+    double r = rho * sc[1];
+    double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    out[0] = u[0];
+    out[1] = u[1];
+    out[2] = u[2];
+    out[3] = solid ? 0.0 : r * usq;
+    out[4] = solid ? 0.0 : r * alpha / sc[2] * usq;
+    out[5] = phi;
+}
+
+void orc_vector(const orc_problem *p, const double *phi, const double *alpha, const double *state, const double sc[4],
+                double out[6]) {
+    long n = (long)p->nx * p->ny * p->nz;
+    double sums[6] = {0, 0, 0, 0, 0, 0};
+    double fluid = 0.0;
+    for (long x = 0; x < n; x++) fluid += is_solid(p, x) ? 0.0 : 1.0;
+    for (long x = 0; x < n; x++) {
+        double c[6];
+        vector_cell(p, state + x * p->Q, alpha[x], phi ? phi[x] : 0.0, is_solid(p, x), sc, c);
+        for (int k = 0; k < 6; k++) sums[k] += c[k];
+    }
+    out[0] = sums[0] / fluid;
+    out[1] = sums[1] / fluid;
+    out[2] = sums[2] / fluid;
+    out[3] = 0.5 * sums[3] * sc[3];
+    out[4] = sums[4] * sc[3];
+    out[5] = sums[5] * sc[3];
+}
+
+static dual dual_sum_c(const orc_problem *p, const dual *f, int d) {
+    /* stationary ports.rs moments order: acc starts at 0, q ascending */
+    const lattice_t *L = lat(p->Q);
+    dual acc = dv(0.0);
+    for (int q = 0; q < p->Q; q++) {
+        if (L->c[q][d] == 1) acc = dadd(acc, f[q]);
+        else if (L->c[q][d] == -1) acc = dsub(acc, f[q]);
+    }
+    return acc;
+}
+
+static dual vector_cell_weighted_dual(const orc_problem *p, const dual *f, dual alpha, dual phi, int solid,
+                                      const double sc[4], const double wts[6]) {
+    dual rho = f[0];
+    for (int q = 1; q < p->Q; q++) rho = dadd(rho, f[q]);
+    dual m[3];
+    for (int d = 0; d < 3; d++) m[d] = dual_sum_c(p, f, d);
+    dual s = dadd(dmuls(alpha, 0.5), dv(1.0));
+    dual u[3];
+    for (int d = 0; d < 3; d++) {
+        dual ud = ddiv(dadd(ddiv(m[d], rho), dv(0.5 * p->accel[d])), s);
+        u[d] = dmuls(solid ? dv(0.0) : ud, sc[0]);
+    }
+    dual r = dmuls(rho, sc[1]);
+    dual usq = dadd(dadd(dmul(u[0], u[0]), dmul(u[1], u[1])), dmul(u[2], u[2]));
+    dual c[6];
+    c[0] = u[0];
+    c[1] = u[1];
+    c[2] = u[2];
+    c[3] = solid ? dv(0.0) : dmul(r, usq);
+    c[4] = solid ? dv(0.0) : dmul(ddivs(dmul(r, alpha), sc[2]), usq);
+    c[5] = phi;
+    dual acc = dmuls(c[0], wts[0]);
+    for (int k = 1; k < 6; k++) acc = dadd(acc, dmuls(c[k], wts[k]));
+    return acc;
+}
+
+void orc_vector_vjp(const orc_problem *p, const double *phi, const double *alpha, const double *state,
+                    const double sc[4], const double w[6], double *f_bar, double *alpha_bar, double *phi_bar) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double fluid = 0.0;
+    for (long x = 0; x < n; x++) fluid += is_solid(p, x) ? 0.0 : 1.0;
+    double wts[6] = {w[0] / fluid, w[1] / fluid, w[2] / fluid, 0.5 * w[3] * sc[3], w[4] * sc[3], w[5] * sc[3]};
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        dual in[MAXQ + 2];
+        for (int t = 0; t < Q + 2; t++) {
+            for (int q = 0; q < Q; q++) in[q] = dv(state[x * Q + q]);
+            in[Q] = dv(alpha[x]);
+            in[Q + 1] = dv(phi ? phi[x] : 0.0);
+            in[t].d = 1.0;
+            dual acc = vector_cell_weighted_dual(p, in, in[Q], in[Q + 1], is_solid(p, x), sc, wts);
+            if (t < Q) f_bar[x * Q + t] = acc.d;
+            else if (t == Q) alpha_bar[x] = acc.d;
+            else phi_bar[x] = acc.d;
+        }
     }
 }

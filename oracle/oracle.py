@@ -42,7 +42,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liblbm_oracle.so")
 
-KIND = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4}
+KIND = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACES = {"xmin": (0, 1), "xmax": (0, -1), "ymin": (1, 1), "ymax": (1, -1), "zmin": (2, 1), "zmax": (2, -1)}
 NONE_U32 = 0xFFFFFFFF
 
@@ -153,7 +153,7 @@ class Oracle:
         self.port_nb = np.asarray(nbs, np.int32)
         self.port_kind = np.asarray(kinds, np.int32)
         self.port_value = np.ascontiguousarray(np.asarray(values, np.float64).reshape(-1, 3))
-        self._mrt = mrt_basis(self.Q) if prob.kind == "mrt" else None
+        self._mrt = mrt_basis(self.Q) if prob.kind in ("mrt", "regularized") else None
         p = _Problem()
         p.Q, p.kind = self.Q, KIND[prob.kind]
         p.nx, p.ny, p.nz = nx, ny, nz
@@ -321,6 +321,28 @@ class Oracle:
         return lib().orc_obj_pressure_drop(self._pp, _ptr(np.ascontiguousarray(state, np.float64)),
                                            ctypes.c_int(axis), ctypes.c_int(plane_a), ctypes.c_int(plane_b))
 
+    def vector(self, phi, alpha, state, scales):
+        """solver.rs:874-894: [mean u (3), kinetic, porous dissipation, volume]."""
+        out = np.zeros(6)
+        sc = np.asarray(scales, np.float64)
+        lib().orc_vector(self._pp, _ptr(None if phi is None else np.ascontiguousarray(phi, np.float64)),
+                         _ptr(np.ascontiguousarray(alpha, np.float64)), _ptr(np.ascontiguousarray(state, np.float64)),
+                         _ptr(sc), _ptr(out))
+        return out
+
+    def vector_vjp(self, phi, alpha, state, scales, w):
+        """solver.rs:896-947 (forward-mode gradient of w . vector): (f_bar, alpha_bar, phi_bar)."""
+        fb = np.zeros(self.n * self.Q)
+        ab = np.zeros(self.n)
+        pb = np.zeros(self.n)
+        sc = np.asarray(scales, np.float64)
+        ww = np.asarray(w, np.float64)
+        lib().orc_vector_vjp(self._pp, _ptr(None if phi is None else np.ascontiguousarray(phi, np.float64)),
+                             _ptr(np.ascontiguousarray(alpha, np.float64)),
+                             _ptr(np.ascontiguousarray(state, np.float64)), _ptr(sc), _ptr(ww), _ptr(fb), _ptr(ab),
+                             _ptr(pb))
+        return fb, ab, pb
+
     def pressure_drop_grad(self, axis, plane_a, plane_b, weight=1.0):
         fb = np.zeros(self.n * self.Q)
         lib().orc_obj_pressure_drop_grad(self._pp, ctypes.c_int(axis), ctypes.c_int(plane_a), ctypes.c_int(plane_b),
@@ -335,13 +357,15 @@ class Oracle:
 # ----------------------------------------------------------------------------------------
 @dataclass
 class Objective:
-    kind: str  # "dissipation" | "pressure_drop"
+    kind: str  # "dissipation" | "pressure_drop" | "vector"
     window: tuple = (1, 1)  # first and last state index (inclusive) carrying stage cost
     weight: float = 1.0
     terminal_weight: float = 0.0
     axis: int = 0
     plane_a: int = 1
     plane_b: int = -2
+    vector_weights: tuple = (0.0,) * 6  # "vector": phi = w . vector(state) (phi input not used)
+    scales: tuple = (1.0, 1.0, 1.0, 1.0)  # velocity, density, step_s, cell volume
 
     def planes(self, shape):
         n = shape[self.axis]
@@ -350,12 +374,17 @@ class Objective:
     def value(self, orc: Oracle, alpha, state):
         if self.kind == "dissipation":
             return orc.dissipation(alpha, state)
+        if self.kind == "vector":
+            return float(np.dot(self.vector_weights, orc.vector(None, alpha, state, self.scales)))
         a, b = self.planes(orc.prob.shape)
         return orc.pressure_drop(state, self.axis, a, b)
 
     def grad(self, orc: Oracle, alpha, state, weight):
         if self.kind == "dissipation":
             return orc.dissipation_grad(alpha, state, weight)
+        if self.kind == "vector":
+            fb, ab, _ = orc.vector_vjp(None, alpha, state, self.scales, np.asarray(self.vector_weights) * weight)
+            return fb, ab
         a, b = self.planes(orc.prob.shape)
         return orc.pressure_drop_grad(self.axis, a, b, weight), np.zeros(orc.n)
 

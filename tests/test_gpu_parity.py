@@ -58,6 +58,8 @@ FWD_CASES = [
     ((6, 5, 8), (True, False, False), 27, "cumulant", True, False, (1e-4, 0, 0)),
     ((7, 5, 4), (False, False, False), 27, "cumulant", False, True, (0, 0, 0)),
     ((6, 5, 7), (True, True, True), 19, "trt", True, False, (1e-4, 1e-5, 2e-5)),
+    ((7, 5, 6), (True, True, False), 19, "regularized", True, False, (1e-4, 0, 0)),
+    ((6, 5, 4), (True, False, False), 27, "regularized", False, False, (1e-4, 0, 0)),
 ]
 
 
@@ -89,7 +91,7 @@ def test_forward_f64_bit_exact(shape, periodic, Q, coll, with_solid, with_ports,
 
 
 @pytest.mark.parametrize("precision,tol", [("f32", 2e-5), ("f16s", 2e-3), ("bf16s", 1.5e-2)])
-@pytest.mark.parametrize("Q,coll", [(19, "trt"), (27, "trt"), (19, "mrt"), (27, "cumulant")])
+@pytest.mark.parametrize("Q,coll", [(19, "trt"), (27, "trt"), (19, "mrt"), (19, "regularized"), (27, "cumulant")])
 def test_forward_reduced_precision(precision, tol, Q, coll):
     if precision == "bf16s" and coll not in ("trt", "bgk"):
         pytest.skip("BF16 storage supports BGK and TRT only")
@@ -141,6 +143,7 @@ VJP_CASES = [
     ((6, 5, 4), (True, False, False), 27, "cumulant", False, False),
     ((7, 5, 4), (False, False, False), 27, "cumulant", False, True),
     ((7, 5, 5), (False, False, False), 27, "cumulant", False, True),
+    ((6, 5, 6), (True, True, False), 19, "regularized", True, False),
 ]
 
 
@@ -317,3 +320,81 @@ def test_mass_drift_includes_port_exchange_f64():
         lat.initial_populations()
         _, st = lat.history(80)
     assert st.mass_drift <= 1e-13, st.mass_drift
+
+
+SCALES = (0.7, 998.0, 2.5e-3, 1.3e-6)  # velocity, density, step_s, cell volume
+
+
+@pytest.mark.parametrize("Q,coll,with_solid,with_ports", [(19, "trt", True, False), (19, "trt", False, True),
+                                                          (27, "cumulant", False, False)])
+def test_vector_and_vjp_f64_match_oracle(Q, coll, with_solid, with_ports):
+    """lbm_vector / lbm_vector_vjp against the oracle's restatement of vector and its
+    forward-mode gradient (solver.rs:874-947) on a developed FP64 state."""
+    shape = (7, 6, 5)
+    solid = random_solid(shape, 19) if with_solid else None
+    ports = channel_ports(shape, 0.03) if with_ports else ()
+    cfg, prob = make_pair(shape, (False, False, False) if ports else (True, False, False), Q=Q, collision=coll,
+                          precision="f64", accel=(0, 0, 0) if ports else (1e-4, 0, 0), solid=solid, ports=ports,
+                          bulk_rate=1.0 if coll == "cumulant" else float("nan"))
+    alpha = random_alpha(shape, 5)
+    phi = np.random.default_rng(8).random(int(np.prod(shape)))
+    w = np.array([0.3, -1.2, 0.5, 2.0, 0.7, 0.25])
+    o = orc.Oracle(prob)
+    f = o.run(alpha, 6)
+    want = o.vector(phi, alpha, f, SCALES)
+    wf, wa, wp = o.vector_vjp(phi, alpha, f, SCALES, w)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        got = lat.vector(SCALES, phi)
+        gf, ga, gp = lat.vector_vjp(SCALES, w, phi)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-300)
+    assert np.abs(gf - wf).max() <= 1e-12 * np.abs(wf).max()
+    assert np.abs(ga - wa).max() <= 1e-12 * np.abs(wa).max()
+    np.testing.assert_allclose(gp, wp, rtol=1e-15)
+
+
+def test_gradient_vector_objective_and_seeded_form_f64():
+    """J = w . vector(final state) through lbm_gradient against the oracle's scan_adjoint,
+    and the history_adjoint form (solver.rs:798-815): a gradient seeded with
+    lambda_final = vector_vjp(final state) plus vector_vjp's own alpha_bar gives the same
+    dJ/dalpha."""
+    shape = (9, 6, 6)
+    steps = 20
+    cfg, prob = make_pair(shape, (False, False, False), precision="f64", ports=channel_ports(shape, 0.03))
+    alpha = random_alpha(shape, 21)
+    w = (0.5, 0.1, -0.2, 3.0, 1.5, 0.0)
+    ob = Objective(kind="vector", terminal_weight=1.0, vector_weights=w, scales=SCALES)
+    oob = orc.Objective(kind="vector", window=(1, 1), weight=0.0, terminal_weight=1.0, vector_weights=w,
+                        scales=SCALES)
+    J_want, ab_want, _, f_final = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+        lat.set_state(f_final)
+        fb, ab_direct, _ = lat.vector_vjp(SCALES, w)
+        lat.initial_populations()
+        _, ab_seeded, _, _ = lat.gradient(steps, None, lambda_final=fb)
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+    assert np.abs(ab_seeded + ab_direct - ab).max() <= 1e-12 * np.abs(ab).max()
+
+
+def test_gradient_vector_objective_f16_cotangents():
+    """Scaled FP16 cotangents with a vector objective (its seed bound sets the first
+    exponent) against FP32 cotangents on the same FP16 states."""
+    shape = (12, 8, 8)
+    steps = 40
+    cfg, _ = make_pair(shape, (False, False, False), precision="f16s", ports=channel_ports(shape, 0.03))
+    alpha = random_alpha(shape, 23)
+    ob = Objective(kind="vector", t_begin=20, t_end=steps, weight=1.0 / 21, terminal_weight=1.0,
+                   vector_weights=(1.0, 0.0, 0.0, 2.0, 5.0, 0.0), scales=SCALES)
+    res = {}
+    for adj in ("f16", "f32"):
+        cfg.adjoint_storage = adj
+        with Lattice(cfg) as lat:
+            lat.set_design(alpha)
+            lat.initial_populations()
+            res[adj] = lat.gradient(steps, ob)[1]
+    assert np.linalg.norm(res["f16"] - res["f32"]) / np.linalg.norm(res["f32"]) <= 2e-3

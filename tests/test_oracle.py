@@ -80,7 +80,8 @@ def _cell_problem(Q, kind, **kw):
     return orc.Oracle(orc.Problem(shape=(3, 3, 3), periodic=(True, True, True), Q=Q, kind=kind, **kw))
 
 
-KINDS = [(19, "bgk"), (19, "trt"), (19, "mrt"), (19, "bgk_guo"), (27, "trt"), (27, "mrt"), (27, "cumulant")]
+KINDS = [(19, "bgk"), (19, "trt"), (19, "mrt"), (19, "bgk_guo"), (19, "regularized"), (27, "trt"), (27, "mrt"),
+         (27, "regularized"), (27, "cumulant")]
 
 
 @pytest.mark.parametrize("Q,kind", KINDS)
@@ -134,6 +135,32 @@ def test_bgk_is_trt_with_equal_rates():
     bgk, trt = _cell_problem(19, "bgk", omega=omega), _cell_problem(19, "trt", omega=omega, magic=magic)
     f = w * (1 + 0.05 * np.random.default_rng(4).standard_normal(19))
     np.testing.assert_allclose(trt.cell_collide(f, 0.01), bgk.cell_collide(f, 0.01), atol=1e-16)
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_regularized_at_unit_rate_is_bgk(Q):
+    """Regularized relax (collision.rs:376-394) keeps bulk and shear moments at rate omega
+    and removes every other moment (rate 1): at omega = 1 every moment has rate 1, which
+    is BGK with omega = 1 (collide_pairs, a different code path)."""
+    c, w, _ = orc.lattice(Q)
+    a = (2e-4, -1e-4, 5e-5)
+    reg, bgk = _cell_problem(Q, "regularized", omega=1.0, accel=a), _cell_problem(Q, "bgk", omega=1.0, accel=a)
+    f = w * (1 + 0.05 * np.random.default_rng(6).standard_normal(Q))
+    np.testing.assert_allclose(reg.cell_collide(f, 0.02), bgk.cell_collide(f, 0.02), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("Q", [19, 27])
+def test_regularized_differs_from_mrt_only_off_bulk_and_shear(Q):
+    """Away from omega = 1 the regularized operator differs from MRT with default rates
+    (which is TRT) -- the non-hydrodynamic moments are removed instead of relaxed."""
+    c, w, _ = orc.lattice(Q)
+    reg, mrt = _cell_problem(Q, "regularized", omega=1.6), _cell_problem(Q, "mrt", omega=1.6)
+    f = w * (1 + 0.05 * np.random.default_rng(7).standard_normal(Q))
+    out_r, out_m = reg.cell_collide(f, 0.0), mrt.cell_collide(f, 0.0)
+    assert np.abs(out_r - out_m).max() > 1e-6
+    # mass and momentum are conserved by both
+    np.testing.assert_allclose(out_r.sum(), f.sum(), rtol=1e-14)
+    np.testing.assert_allclose(c.T @ out_r, c.T @ f, atol=1e-15)
 
 
 def test_stationary_bgk_guo_equals_moving_bgk_without_brinkman():
@@ -322,6 +349,56 @@ def test_dissipation_gradient_central_differences():
         a3[x] -= h
         fd = (o.dissipation(a2, f) - o.dissipation(a3, f)) / (2 * h)
         assert abs(fd - ab[x]) <= 1e-6 * abs(fd) + 1e-12
+
+
+SCALES = (0.7, 998.0, 2.5e-3, 1.3e-6)  # velocity, density, step_s, cell volume
+
+
+def test_vector_closed_form_on_equilibrium_states():
+    """vector (solver.rs:874-894) of f = f_eq(rho0, U) with uniform alpha0: the Brinkman
+    velocity is U / (1 + alpha0/2) (solver.rs:189-196), so mean u, kinetic energy and
+    porous dissipation follow in closed form; the volume is sum(phi) dV."""
+    o = _interval_pair()
+    c, w, _ = orc.lattice(o.Q)
+    rho0, U, alpha0 = 1.02, np.array([0.03, -0.01, 0.005]), 0.04
+    cu = c @ U
+    eq = w * rho0 * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * U @ U)
+    state = np.tile(eq, o.n)
+    alpha = np.full(o.n, alpha0)
+    phi = np.random.default_rng(3).random(o.n)
+    v = o.vector(phi, alpha, state, SCALES)
+    vel, dens, dt, dv = SCALES
+    u = U / (1 + alpha0 / 2) * vel
+    usq = u @ u
+    np.testing.assert_allclose(v[:3], u, rtol=1e-13)
+    np.testing.assert_allclose(v[3], 0.5 * rho0 * dens * usq * o.n * dv, rtol=1e-12)
+    np.testing.assert_allclose(v[4], rho0 * dens * alpha0 / dt * usq * o.n * dv, rtol=1e-12)
+    np.testing.assert_allclose(v[5], phi.sum() * dv, rtol=1e-13)
+
+
+def test_vector_vjp_central_differences():
+    """vector_vjp (solver.rs:896-947) against central differences of w . vector."""
+    o = _interval_pair()
+    rng = np.random.default_rng(17)
+    alpha = rng.random(o.n) * 0.05
+    phi = rng.random(o.n)
+    f = o.run(alpha, 4)
+    wv = rng.standard_normal(6)
+    fb, ab, pb = o.vector_vjp(phi, alpha, f, SCALES, wv)
+    val = lambda ff, aa, pp: float(wv @ o.vector(pp, aa, ff, SCALES))
+    h = 1e-7
+    for idx in rng.choice(f.size, 6, replace=False):
+        e = np.zeros(f.size)
+        e[idx] = h
+        fd = (val(f + e, alpha, phi) - val(f - e, alpha, phi)) / (2 * h)
+        assert abs(fd - fb[idx]) <= 1e-6 * abs(fd) + 1e-10 * np.abs(fb).max()
+    for x in rng.choice(o.n, 4, replace=False):
+        e = np.zeros(o.n)
+        e[x] = h
+        fd = (val(f, alpha + e, phi) - val(f, alpha - e, phi)) / (2 * h)
+        assert abs(fd - ab[x]) <= 1e-6 * abs(fd) + 1e-10 * np.abs(ab).max()
+        fd = (val(f, alpha, phi + e) - val(f, alpha, phi - e)) / (2 * h)
+        assert abs(fd - pb[x]) <= 1e-6 * abs(fd) + 1e-12
 
 
 def test_pressure_drop_uniform_state():
