@@ -185,6 +185,7 @@ double odd_rate(double omega, double magic) {
 struct lbm_handle {
     int Q = 19, kind = K_TRT, prec = LBM_F32, lprec = LBM_F32;
     bool regularized = false;
+    uint64_t every = 0;  // Checkpointing::Every(k); 0 = binomial
     int nx = 0, ny = 0, nz = 0;
     int per[3] = {0, 0, 0};
     double omega = 1.0, om = 1.0, tau = 1.0, magic = 3.0 / 16.0, bulk = NAN, even = NAN;
@@ -789,6 +790,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             hip_check(hipMemcpy(h->pval.p, pval.data(), h->pval.bytes, hipMemcpyHostToDevice), "upload");
         }
         h->budget = cfg->checkpoint_budget_bytes;
+        h->every = cfg->checkpoint_every;
         hip_check(hipStreamSynchronize(h->stream), "sync");
         *out = h.release();
         lbm_reset_initial(*out);
@@ -1060,6 +1062,30 @@ int lbm_debug_link_table(lbm_handle* h, uint32_t* source, uint64_t len) {
     });
 }
 
+namespace {
+void export_schedule(const Schedule& s, int64_t* actions, uint64_t cap, uint64_t* count, uint64_t* forward_steps,
+                     uint64_t* reverse_start) {
+    if (count) *count = s.actions.size();
+    if (forward_steps) *forward_steps = s.forward_steps;
+    if (reverse_start) *reverse_start = s.reverse_start;
+    if (actions)
+        for (size_t i = 0; i < s.actions.size() && i < cap; i++) {
+            actions[5 * i + 0] = int64_t(s.actions[i].kind);
+            actions[5 * i + 1] = int64_t(s.actions[i].a);
+            actions[5 * i + 2] = int64_t(s.actions[i].b);
+            actions[5 * i + 3] = 0;
+            actions[5 * i + 4] = 0;
+        }
+}
+}  // namespace
+
+int lbm_every_schedule(uint64_t steps, uint64_t every, int64_t* actions, uint64_t cap, uint64_t* count,
+                       uint64_t* forward_steps, uint64_t* reverse_start) {
+    return guarded([&] {
+        export_schedule(every_schedule(steps, every), actions, cap, count, forward_steps, reverse_start);
+    });
+}
+
 int lbm_revolve_schedule(uint64_t steps, uint64_t snapshots, int64_t* actions, uint64_t cap, uint64_t* count,
                          uint64_t* forward_steps, uint64_t* reverse_start) {
     return guarded([&] {
@@ -1172,7 +1198,10 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         ensure_adjoint_buffers(h);
         const uint64_t base = h->t;  // state 0 of the sweep; parities follow base + s
         const uint64_t slots = reserve_pool(h, nsteps);
-        Schedule sch = revolve_schedule(nsteps, slots);
+        Schedule sch = h->every ? every_schedule(nsteps, h->every) : revolve_schedule(nsteps, slots);
+        if (sch.peak > slots)
+            fail(LBM_ERR_MEMORY, "checkpoint budget holds " + std::to_string(slots) + " snapshots; checkpoint_every " +
+                                     std::to_string(h->every) + " needs " + std::to_string(sch.peak));
         auto slot_ptr = [&](uint64_t s) { return static_cast<char*>(h->pool.p) + size_t(s) * h->state_bytes; };
 
         reset_scalars(h);

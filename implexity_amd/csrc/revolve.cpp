@@ -166,4 +166,46 @@ Schedule revolve_schedule(uint64_t steps, uint64_t snapshots) {
     return s;
 }
 
+Schedule every_schedule(uint64_t steps, uint64_t every) {
+    if (steps == 0) throw std::invalid_argument("a checkpoint schedule needs at least one step");
+    if (every == 0) throw std::invalid_argument("checkpoint interval must be at least one step");
+    Schedule s;
+    s.steps = steps;
+    const uint64_t nseg = (steps + every - 1) / every;
+    // slots 0..nseg-1 hold the segment starts, nseg.. the states inside a segment
+    auto& A = s.actions;
+    A.push_back({ActKind::Snapshot, 0, 0});
+    uint64_t fwd = 0;
+    for (uint64_t g = 0; g < nseg; g++) {
+        const uint64_t a = g * every, e = std::min(a + every, steps);
+        A.push_back({ActKind::Advance, a, e});
+        fwd += e - a;
+        if (e < steps) A.push_back({ActKind::Snapshot, e, g + 1});
+    }
+    s.reverse_start = A.size();
+    for (uint64_t g = nseg; g-- > 0;) {
+        const uint64_t a = g * every, e = std::min(a + every, steps);
+        // recompute a+1 .. e-1 into the segment slots (the state e-1 is the cursor)
+        A.push_back({ActKind::Restore, a, g});
+        for (uint64_t t = a; t + 1 < e; t++) {
+            A.push_back({ActKind::Advance, t, t + 1});
+            fwd++;
+            if (t + 2 < e) A.push_back({ActKind::Snapshot, t + 1, nseg + (t - a)});
+        }
+        for (uint64_t k = e; k > a; k--) {
+            if (k < e) {
+                // cursor must hold state k-1: a segment slot (k-1 > a) or the segment start
+                const uint64_t slot = (k - 1 == a) ? g : nseg + (k - 2 - a);
+                A.push_back({ActKind::Restore, k - 1, slot});
+            }
+            A.push_back({ActKind::Reverse, k, 0});
+        }
+        for (uint64_t t = a + 1; t + 1 < e; t++) A.push_back({ActKind::Release, nseg + (t - a - 1), 0});
+        A.push_back({ActKind::Release, g, 0});
+    }
+    s.forward_steps = fwd;
+    s.peak = nseg + (every > 2 ? every - 2 : 0);
+    return s;
+}
+
 }  // namespace lbm

@@ -34,4 +34,11 @@ uint64_t revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots);
 // Throws std::invalid_argument on an impossible request (no steps or no slots).
 Schedule revolve_schedule(uint64_t steps, uint64_t snapshots);
 
+// Checkpointing::Every(k) of scan_adjoint (crates/implexity-ad/src/scan.rs:140-246)
+// as an action list for the same driver: a snapshot every `every` steps in the first
+// sweep; in reverse, each segment [s, e) is recomputed from its snapshot with all of its
+// states kept (e - s - 1 steps) and reversed.  forward_steps = n + sum (len - 1);
+// slots = ceil(n / every) + max(every - 2, 0).
+Schedule every_schedule(uint64_t steps, uint64_t every);
+
 }  // namespace lbm

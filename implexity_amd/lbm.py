@@ -86,7 +86,7 @@ class _Config(ctypes.Structure):
         ("bulk_rate", ctypes.c_double), ("even_rate", ctypes.c_double), ("accel", ctypes.c_double * 3),
         ("n_ports", ctypes.c_uint32), ("reserved1", ctypes.c_uint32), ("ports", ctypes.c_void_p),
         ("checkpoint_budget_bytes", ctypes.c_uint64), ("adjoint_storage", ctypes.c_uint32),
-        ("reserved2", ctypes.c_uint32),
+        ("checkpoint_every", ctypes.c_uint32),
     ]
 
 
@@ -157,6 +157,8 @@ _SIGS = {
                        ctypes.c_int),
     "lbm_revolve_schedule": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                               ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "lbm_every_schedule": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                            ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "lbm_revolve_optimal_forward_steps": ([ctypes.c_uint64, ctypes.c_uint64], ctypes.c_uint64),
     "lbm_time_kernel": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_copy_bandwidth": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
@@ -220,6 +222,8 @@ class LatticeConfig:
     # cotangent storage: "auto" (FP64 / FP32 / scaled FP16 for f64 / f32, bf16s / f16s),
     # "f64", "f32" or "f16" (scaled FP16, f16s storage only)
     adjoint_storage: str = "auto"
+    # 0: binomial checkpointing inside the budget; k: Checkpointing::Every(k) (scan.rs:109-246)
+    checkpoint_every: int = 0
 
 
 @dataclass
@@ -294,6 +298,7 @@ class Lattice:
         if cfg.adjoint_storage not in ADJOINT_STORAGE:
             raise ValueError(f"adjoint_storage must be one of {sorted(ADJOINT_STORAGE)}")
         c.adjoint_storage = ADJOINT_STORAGE[cfg.adjoint_storage]
+        c.checkpoint_every = int(cfg.checkpoint_every)
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)
@@ -436,14 +441,16 @@ class BinomialSchedule:
 
     KINDS = ("advance", "snapshot", "restore", "reverse", "release")
 
+    _EXPORT = "lbm_revolve_schedule"
+
     def __init__(self, steps: int, snapshots: int):
         cnt = ctypes.c_uint64(0)
         fwd = ctypes.c_uint64(0)
         rs = ctypes.c_uint64(0)
-        _check(lib().lbm_revolve_schedule(steps, snapshots, None, 0, ctypes.byref(cnt), None, None))
+        export = getattr(lib(), self._EXPORT)
+        _check(export(steps, snapshots, None, 0, ctypes.byref(cnt), None, None))
         buf = np.zeros(5 * cnt.value, np.int64)
-        _check(lib().lbm_revolve_schedule(steps, snapshots, _p(buf), cnt.value, ctypes.byref(cnt), ctypes.byref(fwd),
-                                          ctypes.byref(rs)))
+        _check(export(steps, snapshots, _p(buf), cnt.value, ctypes.byref(cnt), ctypes.byref(fwd), ctypes.byref(rs)))
         self.steps = steps
         self.forward_steps = fwd.value
         self.reverse_start = rs.value
@@ -457,6 +464,12 @@ class BinomialSchedule:
             else:
                 acts.append((name, int(a)))
         self.actions = acts
+
+
+class EverySchedule(BinomialSchedule):
+    """Checkpointing::Every(k) action list (scan.rs:109-246) as the gradient driver runs it."""
+
+    _EXPORT = "lbm_every_schedule"
 
 
 def optimal_forward_steps(steps: int, snapshots: int) -> int:

@@ -126,7 +126,8 @@ typedef struct {
     const lbm_port* ports;
     uint64_t checkpoint_budget_bytes; /* device bytes for revolve snapshots (0: 1/2 of free) */
     uint32_t adjoint_storage;     /* lbm_adjoint_storage */
-    uint32_t reserved2;
+    uint32_t checkpoint_every;    /* 0: binomial (revolve) in the budget; k > 0: Checkpointing::Every(k)
+                                     (scan.rs:109-246), needing ceil(n/k) + max(k-2, 0) snapshots */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
@@ -248,6 +249,10 @@ int lbm_port_cells(const lbm_handle* h, int32_t* cells, int32_t* nbs, uint64_t c
 int lbm_revolve_schedule(uint64_t steps, uint64_t snapshots, int64_t* actions, uint64_t cap, uint64_t* count,
                          uint64_t* forward_steps, uint64_t* reverse_start);
 uint64_t lbm_revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots);
+/* The Checkpointing::Every(k) action list the gradient driver runs when
+ * checkpoint_every = k (same encoding as lbm_revolve_schedule). */
+int lbm_every_schedule(uint64_t steps, uint64_t every, int64_t* actions, uint64_t cap, uint64_t* count,
+                       uint64_t* forward_steps, uint64_t* reverse_start);
 
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by

@@ -162,6 +162,11 @@ def replay_check(steps: int, snapshots: int) -> int:
     """Exhaustive replay: every Reverse(k) must hold state k-1 at the cursor and state k
     ahead (scan.rs:328-388 semantics).  Returns the number of forward step evaluations."""
     actions, _, _, _ = binomial_schedule(steps, snapshots)
+    return replay_actions(steps, actions)
+
+
+def replay_actions(steps: int, actions) -> int:
+    """replay_check on a given action list (tuples as binomial_schedule builds them)."""
     slots = {}
     cursor, ahead = 0, None
     evaluations = 0

@@ -36,6 +36,20 @@ def test_revolve_schedule_host(n, s):
     assert sch.actions == acts and sch.reverse_start == rstart and sch.forward_steps == fwd
 
 
+@pytest.mark.parametrize("n,k", [(1, 1), (10, 1), (10, 2), (7, 3), (24, 5), (100, 10), (3000, 100)])
+def test_every_schedule_replays(n, k):
+    """Checkpointing::Every(k) (scan.rs:109-246) as an action list: every reverse holds
+    state k-1 at the cursor and k ahead, the recomputation is n + sum(len - 1) steps, and
+    at most ceil(n/k) + max(k - 2, 0) slots are live."""
+    sch = lbm.EverySchedule(n, k)
+    evals = rev.replay_actions(n, sch.actions)
+    nseg = -(-n // k)
+    lens = [min(k, n - g * k) for g in range(nseg)]
+    assert evals == sch.forward_steps == n + sum(length - 1 for length in lens)
+    used = {a[2] for a in sch.actions if a[0] == "snapshot"}
+    assert max(used) + 1 <= nseg + max(k - 2, 0)
+
+
 def test_revolve_refuses_no_slots():
     with pytest.raises(lbm.LbmError):
         lbm.BinomialSchedule(10, 0)

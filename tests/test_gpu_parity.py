@@ -219,6 +219,28 @@ def test_gradient_f64_matches_oracle(shape, periodic, Q, coll, obj, with_ports, 
         assert st.forward_steps == optimal_forward_steps(steps, slots)
 
 
+def test_gradient_every_k_checkpointing_f64():
+    """checkpoint_every = 5 runs Checkpointing::Every(5) (scan.rs:109-246): same J and
+    dJ/dalpha as the oracle's scan_adjoint with every = 5, recomputing n + sum(len - 1)
+    steps."""
+    shape = (8, 6, 5)
+    steps = 23
+    cfg, prob = make_pair(shape, (False, False, False), precision="f64", ports=channel_ports(shape, 0.03))
+    cfg.checkpoint_every = 5
+    alpha = random_alpha(shape, 27)
+    ob = _objective("pressure_drop", steps, shape)
+    oob = orc.Objective(kind=ob.kind, window=(ob.t_begin, ob.t_end), weight=ob.weight, axis=ob.axis,
+                        plane_a=ob.plane_a, plane_b=ob.plane_b)
+    J_want, ab_want, _, _ = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob, every=5)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, st = lat.gradient(steps, ob)
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+    assert st.forward_steps == steps + 4 * 4 + 2
+
+
 def test_gradient_central_differences_f64():
     """dJ/dalpha against central differences (h = 1e-6) of the GPU forward objective on
     the eight voxels with the largest sensitivity, rel_err as gradcheck.rs:107-112."""
