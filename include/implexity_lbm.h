@@ -16,11 +16,15 @@
  *   lbm_gradient          history + history_adjoint behind scan_adjoint with binomial
  *                         checkpointing                                solver.rs:797-815,
  *                         crates/implexity-ad/src/scan.rs:140-390, revolve.rs:128-388
+ *   lbm_gradient_seeded   history_adjoint(history, alpha, tau, lambda_final) solver.rs:798-815
  *   lbm_step_vjp          LbmProblem::interval_vjp                    solver.rs:641-674
+ *   lbm_vector            LbmProblem::vector (terminal responses)      solver.rs:874-894
+ *   lbm_vector_vjp        LbmProblem::vector_vjp                      solver.rs:896-947
  *   lbm_get_state         the reference's AoS state vector (x*Q+q)    solver.rs:90,325
  *   lbm_get_fields        LbmProblem::fields (lattice units)          solver.rs:689-705
  *   lbm_debug_link_table  Streaming::source_of for every (cell, q)    solver.rs:160-168
  *   lbm_revolve_schedule  BinomialSchedule::new / actions             revolve.rs:261-329
+ *   lbm_every_schedule    Checkpointing::Every(k) of scan_adjoint     scan.rs:109-246
  *
  * Conventions
  *   - Plain pointers and explicit lengths; host buffers are caller-owned (like the
