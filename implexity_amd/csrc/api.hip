@@ -623,9 +623,12 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             if (!any) h->solid.clear();
         }
 
-        // Storage geometry: +1 ghost layer on non-periodic axes.  The z pitch is nz + 1
-        // for every axis kind: a pitch that is a multiple of 128 bytes aliases rows onto
-        // the same HBM channels and measured 16 % slower on a 256^3 step.
+        // Storage geometry: one extra storage layer (the esoteric-twist ghost) on each
+        // non-periodic x and y axis; rows of SZ = nz rounded up to 64 elements, so that a
+        // row of any population array is a whole number of 128-byte lines; the +z ghost
+        // slots of a non-periodic z axis in their own region after the cells.  With the
+        // ghost inside the row (pitch nz + 1) neighbouring rows shared a line that two
+        // blocks fetched: 15-22 % more HBM reads than the algorithmic bytes at 256^3.
         h->g.nx = h->nx;
         h->g.ny = h->ny;
         h->g.nz = h->nz;
@@ -634,12 +637,14 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->g.pz = h->per[2];
         const int SX = h->nx + (h->per[0] ? 0 : 1);
         h->g.SY = h->ny + (h->per[1] ? 0 : 1);
-        h->g.SZ = h->nz + 1;
+        h->g.SZ = ((h->nz + 63) / 64) * 64;
         h->g.sxy = (long long)h->g.SY * h->g.SZ;
-        h->g.stride = (long long)SX * h->g.sxy;
-        // Kernels address each population array with 32-bit byte offsets.
-        if (uint64_t(h->g.stride) * std::max(elem_bytes(h->prec), elem_bytes(h->lprec)) >= (uint64_t(1) << 32))
-            fail(LBM_ERR_CONTRACT, "lattice too large: one population array would exceed 4 GiB");
+        h->g.zg = (long long)SX * h->g.sxy;
+        const long long cells_and_ghosts = h->g.zg + (h->per[2] ? 0 : (long long)SX * h->g.SY);
+        h->g.stride = ((cells_and_ghosts + 63) / 64) * 64;
+        // Kernels address an opposite pair of population arrays with 32-bit byte offsets.
+        if (uint64_t(h->g.stride) * std::max(elem_bytes(h->prec), elem_bytes(h->lprec)) >= (uint64_t(1) << 31))
+            fail(LBM_ERR_CONTRACT, "lattice too large: one population array would exceed 2 GiB");
         h->state_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
         h->lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
         h->run = runner_for(h->prec, h->lprec);

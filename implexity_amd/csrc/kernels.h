@@ -53,6 +53,7 @@ struct Geom {
     int px, py, pz;
     long long sxy;     // SY*SZ
     long long stride;  // elements per population array
+    long long zg;      // start of the +z ghost slots (non-periodic z), one per (i, j) storage row
 };
 
 // Objective evaluated on one state: weight * phi(state), phi per cell as
@@ -330,27 +331,48 @@ struct LamStore<P_F16S> {
 // 4 GiB, so (index * element size) is a 32-bit byte offset from a wave-uniform array
 // base, which the compiler turns into SGPR-base + VGPR-offset global accesses with no
 // per-access 64-bit address arithmetic.
+// Rows are whole: a storage row holds the nz cells of one (i, j) and starts on a
+// 128-byte line (SZ is a multiple of 64 elements).  The +z ghost slot of a row (the
+// esoteric-twist slot beyond the last cell of a non-periodic z axis) lives in a
+// separate region at zg + i*SY + j, so that a row of one array is fetched as whole lines
+// with no line shared with the next row.
 struct Cursor {
     uint32_t base;          // storage index of the cell
     int32_t dx, dy, dz;     // storage offsets of the +x, +y, +z neighbours (wrapped)
+    uint32_t gbase;         // ghost slot of this row (used when zghost)
+    int32_t gdx, gdy;       // ghost-region offsets of the +x, +y rows (wrapped)
+    bool zghost;            // the +z neighbour is the ghost slot (k = nz-1, z not periodic)
     LBM_DEV uint32_t at(int corner) const {
-        return base + uint32_t(((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0) + ((corner & 4) ? dz : 0));
+        const int32_t xy = ((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0);
+        if (!(corner & 4)) return base + uint32_t(xy);
+        if (!zghost) return base + uint32_t(xy + dz);
+        return gbase + uint32_t(((corner & 1) ? gdx : 0) + ((corner & 2) ? gdy : 0));
     }
 };
 
 LBM_DEV Cursor make_cursor(const Geom& g, int i, int j, int k) {
     Cursor c;
     c.base = uint32_t(i) * uint32_t(g.sxy) + uint32_t(j) * uint32_t(g.SZ) + uint32_t(k);
-    c.dx = (g.px && i + 1 == g.nx) ? -int32_t(g.nx - 1) * int32_t(g.sxy) : int32_t(g.sxy);
-    c.dy = (g.py && j + 1 == g.ny) ? -int32_t(g.ny - 1) * int32_t(g.SZ) : int32_t(g.SZ);
+    const bool wx = g.px && i + 1 == g.nx, wy = g.py && j + 1 == g.ny;
+    c.dx = wx ? -int32_t(g.nx - 1) * int32_t(g.sxy) : int32_t(g.sxy);
+    c.dy = wy ? -int32_t(g.ny - 1) * int32_t(g.SZ) : int32_t(g.SZ);
     c.dz = (g.pz && k + 1 == g.nz) ? -int32_t(g.nz - 1) : 1;
+    c.zghost = !g.pz && k + 1 == g.nz;
+    c.gbase = uint32_t(g.zg) + uint32_t(i) * uint32_t(g.SY) + uint32_t(j);
+    c.gdx = wx ? -int32_t(g.nx - 1) * int32_t(g.SY) : int32_t(g.SY);
+    c.gdy = wy ? -int32_t(g.ny - 1) : 1;
     return c;
 }
 
 // Population array `arr` of a field (uniform) and one element of it by storage index.
-template <class T>
+template <int Q, class T>
 LBM_DEV T* arr_base(T* pop, const Geom& g, int arr) {
-    return pop + size_t(arr) * size_t(g.stride);
+    return pop + size_t(apos<Q>(arr)) * size_t(g.stride);
+}
+// Element index of (array arr, storage index idx) from the field base.
+template <int Q>
+LBM_DEV long long field_index(const Geom& g, int arr, long long idx) {
+    return (long long)apos<Q>(arr) * g.stride + idx;
 }
 template <class T>
 LBM_DEV T& elem(T* base, uint32_t idx) {
@@ -393,13 +415,13 @@ template <int Q, int q>
 LBM_DEV long long read_loc(const Geom& g, const Cursor& cu, uint32_t m, int odd) {
     const bool bl = (m >> q) & 1u;
     const int arr = ((odd != 0) != bl) ? q : opp<Q>(q);
-    return (long long)arr * g.stride + cu.at(corner_neg<Q>(q));
+    return field_index<Q>(g, arr, cu.at(corner_neg<Q>(q)));
 }
 // Write location of f*_q when stepping a state of parity odd.
 template <int Q, int q>
 LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
     const int arr = odd ? opp<Q>(q) : q;
-    return (long long)arr * g.stride + cu.at(corner_pos<Q>(q));
+    return field_index<Q>(g, arr, cu.at(corner_pos<Q>(q)));
 }
 
 // Element at the read location of f_q (see read_loc): the slot at x + a_opp(q) in the
@@ -407,13 +429,20 @@ LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
 // (halfway bounce-back).  The array choice is a select between two uniform bases.
 template <int Q, int q, class T>
 LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
-    const int arr_open = odd ? q : opp<Q>(q);
     if constexpr (q == 0) {
-        return elem(arr_base(pop, g, arr_open), cu.at(corner_neg<Q>(q)));
+        return elem(arr_base<Q>(pop, g, 0), cu.at(corner_neg<Q>(q)));
     } else {
-        const int arr_blocked = odd ? opp<Q>(q) : q;
-        T* b = ((m >> q) & 1u) ? arr_base(pop, g, arr_blocked) : arr_base(pop, g, arr_open);
-        return elem(b, cu.at(corner_neg<Q>(q)));
+        // Array read: (odd xor blocked) ? q : opp(q); the pair's higher index sits one
+        // array after the lower one.
+        constexpr int lo = q < opp<Q>(q) ? q : opp<Q>(q);
+        constexpr bool q_is_hi = q != lo;
+        const bool pick_q = (odd != 0) != (((m >> q) & 1u) != 0);
+        const bool hi = pick_q == q_is_hi;
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        B* base = reinterpret_cast<B*>(arr_base<Q>(pop, g, lo));
+        const uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T)) +
+                             (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
+        return *reinterpret_cast<T*>(base + size_t(off));
     }
 }
 template <int Q, int q, class T>
@@ -427,7 +456,20 @@ LBM_DEV void store_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int
 // Element at the write location of f*_q (see write_loc).
 template <int Q, int q, class T>
 LBM_DEV T& at_write(T* pop, const Geom& g, const Cursor& cu, int odd) {
-    return elem(arr_base(pop, g, odd ? opp<Q>(q) : q), cu.at(corner_pos<Q>(q)));
+    if constexpr (q == 0) {
+        return elem(arr_base<Q>(pop, g, 0), cu.at(corner_pos<Q>(q)));
+    } else {
+        // Array written: odd ? opp(q) : q, addressed from the pair's base like at_read so
+        // that reads and writes of a field share one base per pair.
+        constexpr int lo = q < opp<Q>(q) ? q : opp<Q>(q);
+        constexpr bool q_is_hi = q != lo;
+        const bool hi = (odd == 0) == q_is_hi;
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        B* base = reinterpret_cast<B*>(arr_base<Q>(pop, g, lo));
+        const uint32_t off = cu.at(corner_pos<Q>(q)) * uint32_t(sizeof(T)) +
+                             (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
+        return *reinterpret_cast<T*>(base + size_t(off));
+    }
 }
 
 template <int P, int Q>
@@ -896,7 +938,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
 // blocked by a wall are patched per cell with scalar loads (reads) or stores
 // (adjoint writes).  Chunks that contain a solid cell take the per-cell path, since
 // a vector store would overwrite the bounce-back slots that solid cells lend their
-// neighbours.  Requires nz % V == 0 and a non-periodic z axis.
+// neighbours, and so does the last chunk of a row (its +z slots are in the ghost
+// region).  Requires nz % V == 0 and a non-periodic z axis.
 // ---------------------------------------------------------------------------------
 template <class T, int V>
 struct VecOf {
@@ -950,12 +993,12 @@ LBM_DEV void load_chunk(const Geom& g, const typename Store<P>::T* pop, const Cu
         constexpr int q = decltype(I)::value;
         const long long pos = cu.at(corner_neg<Q>(q));
         const int arr_n = odd ? q : opp<Q>(q);
-        vin[q] = *reinterpret_cast<const VT*>(pop + (long long)arr_n * g.stride + pos);
+        vin[q] = *reinterpret_cast<const VT*>(pop + field_index<Q>(g, arr_n, pos));
         if ((any >> q) & 1u) {
             const int arr_b = odd ? opp<Q>(q) : q;
 #pragma unroll
             for (int v = 0; v < V; v++)
-                if ((m[v] >> q) & 1u) vin[q][v] = pop[(long long)arr_b * g.stride + pos + v];
+                if ((m[v] >> q) & 1u) vin[q][v] = pop[field_index<Q>(g, arr_b, pos + v)];
         }
     });
 }
@@ -973,7 +1016,8 @@ __global__ __launch_bounds__(128) void k_forward_v(StepArgs A) {
     unsigned bad = 0;
     if (chunk_of<V>(g, ch)) {
         uint32_t m[V];
-        const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m);
+        // the last chunk of a row on a non-periodic z axis reaches the ghost region
+        const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m) || (!g.pz && ch.k0 + V == g.nz);
         T* pop = static_cast<T*>(A.pop);
         const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
         const C* alpha_p = static_cast<const C*>(A.alpha);
@@ -1058,7 +1102,7 @@ __global__ __launch_bounds__(128) void k_adjoint_v(AdjArgs A) {
     Chunk ch;
     if (!chunk_of<V>(g, ch)) return;
     uint32_t m[V];
-    const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m);
+    const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m) || (!g.pz && ch.k0 + V == g.nz);
     const T* pop = static_cast<const T*>(A.pop);
     LT* lam = static_cast<LT*>(A.lam);
     const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
@@ -1111,12 +1155,12 @@ __global__ __launch_bounds__(128) void k_adjoint_v(AdjArgs A) {
             constexpr int q = decltype(I)::value;
             const long long pos = cu.at(corner_neg<Q>(q));
             const int arr_n = A.odd ? q : opp<Q>(q);
-            *reinterpret_cast<VL*>(lam + (long long)arr_n * g.stride + pos) = vl[q];
+            *reinterpret_cast<VL*>(lam + field_index<Q>(g, arr_n, pos)) = vl[q];
             if ((any >> q) & 1u) {
                 const int arr_b = A.odd ? opp<Q>(q) : q;
 #pragma unroll
                 for (int v = 0; v < V; v++)
-                    if ((m[v] >> q) & 1u) lam[(long long)arr_b * g.stride + pos + v] = vl[q][v];
+                    if ((m[v] >> q) & 1u) lam[field_index<Q>(g, arr_b, pos + v)] = vl[q][v];
             }
         });
     } else {

@@ -39,6 +39,15 @@ LBM_HD constexpr int opp(int q) {
     return q == 0 ? 0 : Q - q;
 }
 
+// Storage position of population array q: the rest population first, then every
+// opposite pair (q, Q - q), q < Q - q, in two adjacent arrays.  A bounce-back read
+// chooses between the two arrays of one pair, so it is a select of a byte offset from
+// one wave-uniform base instead of a select of 64-bit pointers.
+template <int Q>
+LBM_HD constexpr int apos(int q) {
+    return q == 0 ? 0 : (q < Q - q ? 2 * q - 1 : 2 * (Q - q));
+}
+
 template <int Q>
 LBM_HD constexpr int vorder(int q) {
     return (cvec<Q>(q, 0) != 0) + (cvec<Q>(q, 1) != 0) + (cvec<Q>(q, 2) != 0);

@@ -66,8 +66,9 @@ const Runner* runner_bf16s();
 // and the streaming-only debug operator; D3Q27 with BGK, TRT, MRT and cumulant.
 template <int P, int LP, bool ALL_KINDS>
 struct RunnerImpl final : Runner {
-    // Cells per thread, chosen from 256^3 D3Q19 TRT measurements: one for forward steps;
-    // two for FP64 adjoint steps (16-byte accesses), one for the others.
+    // Cells per thread, chosen from 256^3 D3Q19 TRT measurements: one (the two-cell
+    // kernels must take the per-cell path for the last chunk of every row, whose +z slots
+    // are in the ghost region, which costs more than the wider accesses gain).
     // LBM_FWD_CELLS / LBM_ADJ_CELLS = 1 | 2 override them.
     static int width(const char* env, int dflt) {
         const char* e = std::getenv(env);
@@ -79,7 +80,7 @@ struct RunnerImpl final : Runner {
         return w;
     }
     static int adj_width() {
-        static const int w = width("LBM_ADJ_CELLS", P == P_F64 ? 2 : 1);
+        static const int w = width("LBM_ADJ_CELLS", 1);
         return w;
     }
     static LaunchCfg vec_cfg(const Geom& g, int V, hipStream_t s) {
