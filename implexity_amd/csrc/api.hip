@@ -1398,6 +1398,16 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
             stats->forward_kernel_avg_ms = fwd_kavg;
             stats->adjoint_kernel_avg_ms = adj_kavg;
             stats->kernel_samples = fwd_k.size() + adj_k.size();
+            stats->cotangent_max = NAN;
+            stats->cotangent_exponent = 0;
+            if (h->lam16()) {
+                int rec[4];
+                hip_check(hipMemcpy(rec, h->lsc.p, sizeof(rec), hipMemcpyDeviceToHost), "copy");
+                float mx;
+                std::memcpy(&mx, &rec[2], sizeof(mx));
+                stats->cotangent_max = mx;
+                stats->cotangent_exponent = rec[0];
+            }
         }
         check_lam_range(h);
     });

@@ -505,6 +505,20 @@ LBM_DEV float wave_max(float v) {
     return v;
 }
 
+// Max over the 64 lanes of non-negative floats (compared as their bit patterns) with
+// DPP row shifts and row broadcasts, six VALU operations; lanes shifted in from outside
+// a row read 0.  The result is read from lane 63.
+LBM_DEV float wave_max_nonneg(float v) {
+    unsigned x = __float_as_uint(v);
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, false)));  // row_shr:1
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, false)));  // row_shr:2
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xf, 0xf, false)));  // row_shr:4
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xf, 0xf, false)));  // row_shr:8
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false)));  // row_bcast:15
+    x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xc, 0xf, false)));  // row_bcast:31
+    return __uint_as_float(unsigned(__builtin_amdgcn_readlane(int(x), 63)));
+}
+
 LBM_DEV long long block_linear() {
     return (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * blockIdx.z);
 }
@@ -548,7 +562,7 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
 // Block max of |lambda| into part[4 * block + 2] (FP16 cotangent steps).
 LBM_DEV void block_max_partial(float v, double* part) {
     __shared__ float red[16];
-    v = wave_max(v);
+    v = wave_max_nonneg(v);
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
     const int lane = tid & 63, wid = tid >> 6;
     if (lane == 0) red[wid] = v;
@@ -889,7 +903,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
-    float lmax = 0.0f;  // max |lambda| written (FP16 cotangents; a NaN makes it inf)
+    float lmax = 0.0f;  // max |lambda| written (FP16 cotangents)
     if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
@@ -915,16 +929,18 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                 objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
             }
             A.alpha_bar[cell] += double(ab);
-            float nan_probe = 0.0f;
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 store_read<Q, q>(lam, g, cu, m, A.odd, L::template enc<C>(fb[q], sc));
-                if constexpr (LP == P_F16S) {
-                    lmax = fmaxf(lmax, fabsf(float(fb[q])));
-                    nan_probe += float(fb[q]);
-                }
             });
-            if (nan_probe != nan_probe) lmax = INFINITY;
+            if constexpr (LP == P_F16S) {
+                // The first value to leave the FP16 range shows as a large or infinite max
+                // (a NaN can only follow an earlier overflow or a non-finite state).
+                float m0 = fabsf(float(fb[0]));
+#pragma unroll
+                for (int q = 1; q + 1 < Q; q += 2) m0 = fmaxf(m0, fmaxf(fabsf(float(fb[q])), fabsf(float(fb[q + 1]))));
+                lmax = m0;
+            }
         }
     }
     if constexpr (LP == P_F16S) block_max_partial(lmax, A.part);

@@ -118,6 +118,7 @@ class Stats(ctypes.Structure):
         ("total_ms", ctypes.c_double), ("max_mach", ctypes.c_double), ("mass_drift", ctypes.c_double),
         ("bad_cells", ctypes.c_uint64), ("forward_kernel_avg_ms", ctypes.c_double),
         ("adjoint_kernel_avg_ms", ctypes.c_double), ("kernel_samples", ctypes.c_uint64),
+        ("cotangent_max", ctypes.c_double), ("cotangent_exponent", ctypes.c_int64),
     ]
 
     def as_dict(self):

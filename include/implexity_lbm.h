@@ -181,6 +181,8 @@ typedef struct {
     double forward_kernel_avg_ms; /* forward step kernel alone, HIP events on every 64th step */
     double adjoint_kernel_avg_ms; /* adjoint step kernel alone, HIP events on every 64th step */
     uint64_t kernel_samples;      /* steps timed for the two averages (gradient only) */
+    double cotangent_max;         /* FP16 cotangents: max |lambda| of the last adjoint step (else NaN) */
+    int64_t cotangent_exponent;   /* FP16 cotangents: exponent e of the stored lambda * 2^e (else 0) */
 } lbm_stats;
 
 typedef struct lbm_handle lbm_handle;

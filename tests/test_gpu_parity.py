@@ -324,6 +324,14 @@ def test_gradient_reduced_precision_channel(precision, adj, tol):
     assert abs(J - J_want) <= tol * abs(J_want)
     assert np.isfinite(st.mass_drift)
     if adj == "f16":
+        # the recorded max of the last adjoint step (a wave/block/grid reduction) is the
+        # max |lambda_0| before FP16 rounding
+        with Lattice(cfg) as lat:
+            lat.set_design(alpha)
+            lat.initial_populations()
+            _, _, lam0, st16 = lat.gradient(steps, ob, want_lambda0=True)
+        assert abs(st16.cotangent_max - np.abs(lam0).max()) <= 2e-3 * np.abs(lam0).max()
+        assert st16.cotangent_max * 2.0 ** st16.cotangent_exponent <= 65504
         cfg.adjoint_storage = "f32"
         with Lattice(cfg) as lat:
             lat.set_design(alpha)
