@@ -102,6 +102,7 @@ __global__ void k_lam_init(int* lsc, int e_cur, int e_next) {
         lsc[2] = 0;
         lsc[3] = 0;
     }
+    if (threadIdx.x < LAM_SLOTS) lsc[LAM_SLOT0 + threadIdx.x] = 0;
 }
 
 size_t elem_bytes(int prec) { return prec == LBM_F64 ? 8 : (prec == LBM_F32 ? 4 : 2); }
@@ -408,7 +409,7 @@ void check_lam_range(lbm_handle* h) {
 
 void ensure_adjoint_buffers(lbm_handle* h) {
     if (h->lam16() && !h->lsc.p) {
-        h->lsc.alloc(4 * sizeof(int), "cotangent scale");
+        h->lsc.alloc(LAM_REC_INTS * sizeof(int), "cotangent scale");
         hip_check(hipMemsetAsync(h->lsc.p, 0, h->lsc.bytes, h->stream), "memset");
     }
     h->L.alloc(h->lam_bytes, "cotangent state");

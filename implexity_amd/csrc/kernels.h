@@ -241,14 +241,17 @@ LBM_DEV void from_full(const C* f, C* v) {
 template <int P>
 struct LamStore;
 // Scale of FP16 cotangents: lambda is stored as lambda * 2^e with one exponent for the
-// whole field.  The record lsc[4] on the device holds
+// whole field.  The record lsc[LAM_REC_INTS] on the device holds
 //   lsc[0]  e of the field as stored now,
 //   lsc[1]  e for the next whole-field write (an adjoint step),
 //   lsc[2]  max |lambda| of the last adjoint step (float bits),
-//   lsc[3]  count of adjoint steps whose scaled values left the FP16 range.
+//   lsc[3]  count of adjoint steps whose scaled values left the FP16 range,
+//   lsc[LAM_SLOT0 ..]  LAM_SLOTS max slots the blocks of an adjoint step raise.
 // Readers decode with lsc[0]; an adjoint step writes with lsc[1], after which
-// k_fold moves lsc[1] into lsc[0] and picks the next exponent from the measured
-// max.  Writers that do not replace the whole field (ports, seeds) keep lsc[0].
+// k_lam_update folds the slots, moves lsc[1] into lsc[0] and picks the next exponent
+// from the measured max.  Writers that do not replace the whole field (ports, seeds)
+// keep lsc[0].
+constexpr int LAM_SLOT0 = 8, LAM_SLOTS = 64, LAM_REC_INTS = LAM_SLOT0 + LAM_SLOTS;
 struct LScale {
     float dec, enc;
 };
@@ -559,8 +562,9 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
 }
 
 
-// Block max of |lambda| into part[4 * block + 2] (FP16 cotangent steps).
-LBM_DEV void block_max_partial(float v, double* part) {
+// Block max of |lambda| (>= 0) raised into one of the LAM_SLOTS slots of the scale
+// record (float bits order like the values; one atomic per block).
+LBM_DEV void block_raise_max(float v, int* lsc) {
     __shared__ float red[16];
     v = wave_max_nonneg(v);
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
@@ -571,7 +575,25 @@ LBM_DEV void block_max_partial(float v, double* part) {
         const int nw = int(blockDim.x * blockDim.y + 63) >> 6;
         float m = 0.0f;
         for (int w = 0; w < nw; w++) m = fmaxf(m, red[w]);
-        part[4 * block_linear() + 2] = double(m);
+        atomicMax(reinterpret_cast<unsigned*>(lsc + LAM_SLOT0 + int(block_linear() % LAM_SLOTS)), __float_as_uint(m));
+    }
+}
+
+// After an FP16 cotangent step (one wave of LAM_SLOTS lanes): the field now has exponent
+// lsc[1]; record the max of the slots, count an overflow, choose the next exponent from
+// LAM_GROWTH * max + s_next (s_next bounds the stage seed of the following step) and
+// clear the slots.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(64) void k_lam_update(int* lsc, double s_next) {
+    const int t = threadIdx.x;
+    const float m = wave_max_nonneg(__uint_as_float(unsigned(lsc[LAM_SLOT0 + t])));
+    lsc[LAM_SLOT0 + t] = 0;
+    if (t == 0) {
+        const int ew = lsc[1];
+        if (!(double(m) * ldexp(1.0, ew) <= 65504.0)) lsc[3] = lsc[3] + 1;
+        lsc[0] = ew;
+        lsc[2] = __float_as_int(m);
+        lsc[1] = lam_exponent(LAM_GROWTH * double(m) + s_next);
     }
 }
 
@@ -583,11 +605,8 @@ LBM_DEV void block_max_partial(float v, double* part) {
 // resets the ticket.  Deterministic for a given launch shape.
 //   FOLD_STEP:   jacc += objective; *macc = mass (if set); dstat[0] = max(dstat[0],
 //                max |u|^2) and dstat[1] += bad (if set).
-//   FOLD_LAMBDA: max |lambda| of an FP16 cotangent step: the field now has exponent
-//                lsc[1]; record the max, count an overflow, and choose the next write
-//                exponent from LAM_GROWTH * max + s_next.
 constexpr int FOLD_MAX_BLOCKS = 256;
-enum FoldMode { FOLD_STEP = 0, FOLD_LAMBDA = 1 };
+enum FoldMode { FOLD_STEP = 0 };
 
 LBM_HD inline int fold_blocks(long long nb) {
     const long long g = (nb + 255) / 256;
@@ -623,7 +642,7 @@ LBM_DEV void block_fold4(double& a, double& b, double& c, double& d) {
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, double* stage, unsigned* ticket,
-                                             double* jacc, double* macc, unsigned* dstat, int* lsc, double s_next) {
+                                             double* jacc, double* macc, unsigned* dstat) {
     __shared__ bool last;
     double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
     const long long step = (long long)gridDim.x * blockDim.x;
@@ -661,13 +680,6 @@ __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, 
             if (ub > dstat[0]) dstat[0] = ub;
             dstat[1] += unsigned(d);
         }
-    } else {
-        const float mx = float(c);
-        const int ew = lsc[1];
-        if (!(double(mx) * ldexp(1.0, ew) <= 65504.0)) lsc[3] = lsc[3] + 1;
-        lsc[0] = ew;
-        lsc[2] = __float_as_int(mx);
-        lsc[1] = lam_exponent(LAM_GROWTH * double(mx) + s_next);
     }
     *ticket = 0u;
 }
@@ -943,7 +955,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
             }
         }
     }
-    if constexpr (LP == P_F16S) block_max_partial(lmax, A.part);
+    if constexpr (LP == P_F16S) block_raise_max(lmax, A.lsc);
 }
 
 

@@ -175,7 +175,7 @@ struct RunnerImpl final : Runner {
     }
     static bool ok_adj(int kind, bool mask, const AdjArgs& a) {
         if (!a.pop || !a.lam || !a.alpha || !a.alpha_bar) return false;
-        if (LP == P_F16S && (!a.part || !a.lsc || !a.stage || !a.ticket)) return false;
+        if (LP == P_F16S && !a.lsc) return false;
         if (mask && !a.mask) return false;
         if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
         return true;
@@ -191,7 +191,7 @@ struct RunnerImpl final : Runner {
     static hipError_t reduce(const StepArgs& a, const LaunchCfg& used, bool mass, bool stat) {
         if (!a.stage || !a.ticket) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_fold<FOLD_STEP>, dim3(fold_blocks(blocks(used))), dim3(256), 0, used.stream, a.part, blocks(used),
-                           a.stage, a.ticket, a.jacc, mass ? a.macc : nullptr, stat ? a.dstat : nullptr, nullptr, 0.0);
+                           a.stage, a.ticket, a.jacc, mass ? a.macc : nullptr, stat ? a.dstat : nullptr);
         return hipGetLastError();
     }
 
@@ -211,7 +211,6 @@ struct RunnerImpl final : Runner {
     hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c,
                        hipEvent_t after) const override {
         if (!ok_adj(kind, mask, a)) return hipErrorInvalidValue;
-        if (LP == P_F16S && blocks(c) > a.npart) return hipErrorInvalidValue;
         hipError_t e;
         if (Q == 19)
             e = mask ? adj_q<19, true>(kind, a, c) : adj_q<19, false>(kind, a, c);
@@ -219,9 +218,7 @@ struct RunnerImpl final : Runner {
             e = mask ? adj_q<27, true>(kind, a, c) : adj_q<27, false>(kind, a, c);
         if (e == hipSuccess && after) e = hipEventRecord(after, c.stream);
         if (LP != P_F16S || e != hipSuccess) return e;
-        // The per-cell kernel ran on c (FP16 cotangents never take the vector path).
-        hipLaunchKernelGGL(k_fold<FOLD_LAMBDA>, dim3(fold_blocks(blocks(c))), dim3(256), 0, c.stream, a.part, blocks(c),
-                           a.stage, a.ticket, nullptr, nullptr, nullptr, a.lsc, a.s_next);
+        hipLaunchKernelGGL(k_lam_update<0>, dim3(1), dim3(LAM_SLOTS), 0, c.stream, a.lsc, a.s_next);
         return hipGetLastError();
     }
     hipError_t objective(int Q, bool mask, const StepArgs& a, const ObjArgs& seed, void* lam, double* abar,
