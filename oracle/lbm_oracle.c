@@ -1162,6 +1162,7 @@ double orc_obj_dissipation(const orc_problem *p, const double *alpha, const doub
     long n = (long)p->nx * p->ny * p->nz;
     int Q = p->Q;
     double acc = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
     for (long x = 0; x < n; x++) {
         if (is_solid(p, x)) continue;
         double rho, j[3], den, sv[3], u[3], F[3];

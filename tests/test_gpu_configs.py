@@ -1,0 +1,221 @@
+"""The benchmark problems at their full sizes, through the C ABI on one GPU.
+
+* 64^3 D3Q19 TRT, periodic x, halfway bounce-back walls in y and z, body force
+  g_x = 1e-6, tau+ = 0.8, Lambda = 3/16, smoothed random design (sigma 3), 2000 steps:
+  the FP64 state after all 2000 steps is bit-identical to the oracle's, the summed
+  porous dissipation and the fields agree to 1e-12; FP32 and FP16-shifted velocity
+  fields are compared with the FP64 run.
+* 128^3 D3Q19 TRT inlet/outlet channel (parabolic u_x = 0.02 inlet, density-1
+  outlet), sigma-4 design, pressure drop averaged over steps 2501-3000, checkpoints
+  every 100 steps: dJ/drho against central differences (h = 1e-6, FP64) on 32 seeded
+  voxels, the FP32 gradient against the FP64 one, and the FP64 gradient of a 20-step
+  window at full size against the oracle's scan_adjoint.
+* 256^3 D3Q27 cumulant, omega = 1.6, g_x = 5e-7, sigma-6 design: two FP64 steps at full
+  size bit-identical to the oracle; 5000 steps in FP64, FP32 and FP16-shifted storage,
+  with the relative L2 velocity error and the error of the summed dissipation of the
+  reduced precisions against FP64.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from implexity_amd import design
+from implexity_amd.lbm import Lattice, LatticeConfig, Objective
+from oracle import oracle as orc
+from tests.lbm_cases import channel_ports, make_pair
+
+pytestmark = pytest.mark.gpu
+
+TAU0 = 0.8
+
+
+def _periodic_x_problem(shape, Q, coll, omega, accel, sigma, seed, precision="f64"):
+    phi = design.smoothed_random_design(shape, sigma, seed, periodic=(True, False, False))
+    alpha = design.resistance(phi, 0.1, 0.1).reshape(-1)
+    cfg, prob = make_pair(shape, (True, False, False), Q=Q, collision=coll, precision=precision, omega=omega,
+                          accel=accel, bulk_rate=1.0 if coll == "cumulant" else float("nan"))
+    return cfg, prob, alpha
+
+
+def _run(cfg, alpha, steps, objective=None):
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, st = lat.history(steps, objective)
+        rho, u = lat.fields()
+        f = lat.state() if cfg.precision == "f64" else None
+    return J, rho, u, f, st
+
+
+def _l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# --- 64^3 periodic channel, 2000 steps -----------------------------------------------------
+SHAPE0, STEPS0 = (64, 64, 64), 2000
+
+
+def _config0(precision="f64"):
+    return _periodic_x_problem(SHAPE0, 19, "trt", 1.0 / TAU0, (1e-6, 0.0, 0.0), 3.0, 2024, precision)
+
+
+def test_periodic_channel_64_full_run_f64_bit_exact():
+    cfg, prob, alpha = _config0()
+    ob = Objective(kind="dissipation", t_begin=1, t_end=STEPS0, weight=1.0)
+    J, rho, u, f, st = _run(cfg, alpha, STEPS0, ob)
+    o = orc.Oracle(prob)
+    oob = orc.Objective(kind="dissipation", window=(1, STEPS0), weight=1.0)
+    x = o.initial()
+    J_want = 0.0
+    for t in range(STEPS0):
+        x = o.interval(alpha, x)
+        J_want += oob.weight * o.dissipation(alpha, x)
+    assert np.array_equal(f, x), np.abs(f - x).max()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want), (J, J_want)
+    rho_w, u_w = o.macroscopic(alpha, x)
+    assert np.abs(rho - rho_w).max() <= 1e-12 * np.abs(rho_w).max()
+    assert np.abs(u - u_w).max() <= 1e-12 * np.abs(u_w).max()
+    # closed, periodic-x domain: FP64 conserves mass to rounding over 2000 steps
+    assert abs(st.mass_drift) < 1e-12
+    assert st.bad_cells == 0
+
+
+@pytest.mark.parametrize("precision,tol_u,tol_j", [("f32", 1e-4, 1e-4), ("f16s", 1e-2, 1e-2)])
+def test_periodic_channel_64_reduced_precision(precision, tol_u, tol_j):
+    ob = Objective(kind="dissipation", t_begin=1, t_end=STEPS0, weight=1.0)
+    cfg, _, alpha = _config0("f64")
+    J64, _, u64, _, _ = _run(cfg, alpha, STEPS0, ob)
+    cfg, _, _ = _config0(precision)
+    J, _, u, _, _ = _run(cfg, alpha, STEPS0, ob)
+    eu, ej = _l2(u, u64), abs(J - J64) / abs(J64)
+    print(f"\n64^3 periodic channel {precision}: velocity L2 {eu:.3e}, dissipation {ej:.3e}")
+    assert eu <= tol_u and ej <= tol_j, (eu, ej)
+
+
+# --- 128^3 inlet/outlet channel, 3000 steps ----------------------------------------------------
+SHAPE1, STEPS1 = (128, 128, 128), 3000
+
+
+def _config1(precision="f64", every=100):
+    phi = design.smoothed_random_design(SHAPE1, 4.0, 2025, periodic=(False, False, False)).reshape(-1)
+    cfg = LatticeConfig(shape=SHAPE1, periodic=(False, False, False), lattice=19, collision="trt",
+                        precision=precision, omega=1.0 / TAU0, ports=_ports(SHAPE1, 0.02), checkpoint_every=every)
+    return cfg, phi
+
+
+def _ports(shape, u_in):
+    from implexity_amd.lbm import Port
+    return [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in channel_ports(shape, u_in)]
+
+
+def _dp_objective(t_begin, t_end, n_x):
+    return Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=n_x - 2, t_begin=t_begin, t_end=t_end,
+                     weight=1.0 / (t_end - t_begin + 1))
+
+
+def _design_gradient(cfg, phi, steps, ob):
+    with Lattice(cfg) as lat:
+        lat.set_design(design.resistance(phi))
+        lat.initial_populations()
+        J, ab, _, st = lat.gradient(steps, ob)
+    return J, ab * design.resistance_derivative(phi), st
+
+
+def test_inlet_outlet_128_gradient_central_differences_f64():
+    cfg, phi = _config1()
+    ob = _dp_objective(2501, STEPS1, SHAPE1[0])
+    J, g, st = _design_gradient(cfg, phi, STEPS1, ob)
+    assert st.forward_steps == STEPS1 + 29 * 99 + 99
+    # 32 voxels drawn with a fixed seed among those whose sensitivity is at least 1 % of
+    # the largest.  With h = 1e-6 the two objectives differ by ~5e-14 while each carries a
+    # rounding error of a few ulps of J ~ 1e-2 (1.7e-18 each), so central differences
+    # resolve the derivative to ~1e-4: per voxel rel_err (gradcheck.rs:107-112) <= 2e-3,
+    # and the relative L2 error over the 32 voxels <= 5e-4.
+    cand = np.flatnonzero(np.abs(g) >= 1e-2 * np.abs(g).max())
+    voxels = np.random.default_rng(32).choice(cand, 32, replace=False)
+    h = 1e-6
+    fds, rels = [], []
+    with Lattice(cfg) as lat:
+        for v in voxels:
+            vals = []
+            for sgn in (1.0, -1.0):
+                p2 = phi.copy()
+                p2[v] += sgn * h
+                lat.set_design(design.resistance(p2))
+                lat.initial_populations()
+                vals.append(lat.history(STEPS1, ob)[0])
+            fd = (vals[0] - vals[1]) / (2.0 * h)
+            scale = max(abs(fd), abs(g[v]))
+            fds.append(fd)
+            rels.append(abs(fd / scale - g[v] / scale))
+    fds = np.array(fds)
+    l2 = _l2(fds, g[voxels])
+    print(f"\n128^3 channel: J {J:.6e}, central differences on 32 voxels: worst rel_err {max(rels):.2e}, "
+          f"median {np.median(rels):.2e}, relative L2 {l2:.2e}")
+    assert max(rels) <= 2e-3 and l2 <= 5e-4, (max(rels), l2)
+
+
+def test_inlet_outlet_128_gradient_f32_vs_f64():
+    ob = _dp_objective(2501, STEPS1, SHAPE1[0])
+    cfg, phi = _config1("f64")
+    J64, g64, _ = _design_gradient(cfg, phi, STEPS1, ob)
+    cfg, _ = _config1("f32")
+    J32, g32, _ = _design_gradient(cfg, phi, STEPS1, ob)
+    err = _l2(g32, g64)
+    cos = float(np.dot(g32, g64) / np.linalg.norm(g32) / np.linalg.norm(g64))
+    print(f"\n128^3 channel FP32: gradient rel L2 {err:.3e}, cosine {cos:.9f}, J rel {abs(J32 - J64) / abs(J64):.3e}")
+    assert err <= 1e-3 and abs(J32 - J64) <= 1e-4 * abs(J64)
+
+
+def test_inlet_outlet_128_window_gradient_matches_oracle_f64():
+    steps = 20
+    cfg, phi = _config1("f64", every=10)
+    alpha = design.resistance(phi)
+    ob = _dp_objective(11, steps, SHAPE1[0])
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+    _, prob = make_pair(SHAPE1, (False, False, False), Q=19, collision="trt", omega=1.0 / TAU0,
+                        ports=channel_ports(SHAPE1, 0.02))
+    oob = orc.Objective(kind="pressure_drop", window=(11, steps), weight=ob.weight, axis=0, plane_a=1,
+                        plane_b=SHAPE1[0] - 2)
+    J_want, ab_want, _, _ = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob, every=10)
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+
+
+# --- 256^3 D3Q27 cumulant, 5000 steps ---------------------------------------------------------
+SHAPE2, STEPS2 = (256, 256, 256), 5000
+
+
+def _config2(precision="f64"):
+    return _periodic_x_problem(SHAPE2, 27, "cumulant", 1.6, (5e-7, 0.0, 0.0), 6.0, 2026, precision)
+
+
+def test_cumulant_256_two_steps_f64_bit_exact():
+    cfg, prob, alpha = _config2()
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        lat.interval(2)
+        got = lat.state()
+    want = orc.Oracle(prob).run(alpha, 2)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+def test_cumulant_256_storage_precisions():
+    ob = Objective(kind="dissipation", t_begin=1, t_end=STEPS2, weight=1.0)
+    cfg, _, alpha = _config2("f64")
+    J64, _, u64, _, st = _run(cfg, alpha, STEPS2, ob)
+    assert st.bad_cells == 0 and abs(st.mass_drift) < 1e-11
+    out = {}
+    for precision in ("f32", "f16s"):
+        cfg, _, _ = _config2(precision)
+        J, _, u, _, st = _run(cfg, alpha, STEPS2, ob)
+        out[precision] = (_l2(u, u64), abs(J - J64) / abs(J64))
+    print("\n256^3 D3Q27 cumulant, 5000 steps: " +
+          ", ".join(f"{p}: velocity L2 {e[0]:.3e}, dissipation {e[1]:.3e}" for p, e in out.items()))
+    assert out["f32"][0] <= 1e-3 and out["f32"][1] <= 1e-3, out
+    assert out["f16s"][0] <= 5e-2 and out["f16s"][1] <= 5e-2, out
