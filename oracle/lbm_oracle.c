@@ -36,7 +36,7 @@ typedef struct {
     int opp[MAXQ];
 } lattice_t;
 
-static lattice_t L19, L27;
+static lattice_t L19, L27, L7;
 static int lattices_ready = 0;
 
 static const int C19[19][3] = {
@@ -80,17 +80,38 @@ static void init_lattices(void) {
         for (int j = 0; j < 27; j++)
             if (L27.c[j][0] == -L27.c[i][0] && L27.c[j][1] == -L27.c[i][1] && L27.c[j][2] == -L27.c[i][2])
                 L27.opp[i] = j;
+    /* D3Q7 (the thermal lattice; no counterpart in the reference): rest first, then the
+     * six axis vectors in lexicographic order, so OPPOSITE[i] = 7-i; weights 1/4, 1/8,
+     * sound speed c_s^2 = 1/4. */
+    L7.Q = 7;
+    n = 0;
+    for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+            for (int c = -1; c <= 1; c++) {
+                if (abs(a) + abs(b) + abs(c) > 1) continue;
+                if (a == 0 && b == 0 && c == 0) continue;
+                n++;
+                L7.c[n][0] = a;
+                L7.c[n][1] = b;
+                L7.c[n][2] = c;
+            }
+    L7.c[0][0] = L7.c[0][1] = L7.c[0][2] = 0;
+    for (int q = 0; q < 7; q++) {
+        L7.w[q] = q == 0 ? 0.25 : 0.125;
+        L7.opp[q] = q == 0 ? 0 : 7 - q;
+        for (int d = 0; d < 3; d++) L7.cf[q][d] = (double)L7.c[q][d];
+    }
     lattices_ready = 1;
 }
 
 static const lattice_t *lat(int Q) {
     init_lattices();
-    return Q == 27 ? &L27 : &L19;
+    return Q == 27 ? &L27 : (Q == 7 ? &L7 : &L19);
 }
 
 int orc_lattice(int Q, int *c_out, double *w_out, int *opp_out) {
     const lattice_t *L = lat(Q);
-    if (Q != 19 && Q != 27) return -1;
+    if (Q != 19 && Q != 27 && Q != 7) return -1;
     for (int q = 0; q < Q; q++) {
         for (int d = 0; d < 3; d++) c_out[3 * q + d] = L->c[q][d];
         w_out[q] = L->w[q];
@@ -1362,5 +1383,212 @@ void orc_vector_vjp(const orc_problem *p, const double *phi, const double *alpha
             else if (t == Q) alpha_bar[x] = acc.d;
             else phi_bar[x] = acc.d;
         }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Coupled D3Q7 advection-diffusion of a temperature T (double distribution).  */
+/* Not in the reference (its thermal coupling is finite-volume transport on    */
+/* D3Q19 links, lbm/thermal.rs:91-134); the model is defined here:             */
+/*   T = sum_i g_i,  g_i^eq = w_i T (1 + 4 c_i.u),  u = the Brinkman velocity   */
+/*   of the flow state the same step collides (brinkman_macro),                */
+/*   kappa(phi) = kappa_f (1 + (ratio - 1) phi),  tau_T = 1/2 + 4 kappa,       */
+/*   S(phi) = source * phi,                                                    */
+/*   g_i* = g_i - omega_T (g_i - g_i^eq) + w_i S,                              */
+/* streamed with halfway bounce-back (adiabatic walls) and, at the flow's port */
+/* cells after streaming: velocity ports hold T = T_in by non-equilibrium      */
+/* extrapolation g_b = w T_in + (g_nb - w T_nb); pressure ports are            */
+/* zero-gradient, g_b = g_nb.  Hard solid cells are not part of this model.    */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    double kappa;  /* fluid diffusivity (lattice units) */
+    double ratio;  /* solid:fluid diffusivity ratio */
+    double source; /* heat source per step at phi = 1 */
+    double inlet;  /* T held at velocity ports */
+} orc_thermal;
+
+static inline double th_omega(const orc_thermal *th, double phi) {
+    return 1.0 / (0.5 + 4.0 * (th->kappa * (1.0 + (th->ratio - 1.0) * phi)));
+}
+
+/* One thermal cell: g (7) and the flow velocity u -> post-collision g*. */
+static void thermal_cell(const double *g, const double u[3], double om, double S, double *out) {
+    const lattice_t *L = lat(7);
+    double T = 0.0;
+    for (int q = 0; q < 7; q++) T += g[q];
+    for (int q = 0; q < 7; q++) {
+        double cu = dot_c(L, q, u);
+        double eq = L->w[q] * T * (1.0 + 4.0 * cu);
+        out[q] = g[q] - om * (g[q] - eq) + L->w[q] * S;
+    }
+}
+
+/* VJP of thermal_cell: gb = cotangent of g*; returns g_bar (7), u_bar (3), and the
+ * cotangents of omega and S. */
+static void thermal_cell_vjp(const double *g, const double u[3], double om, const double *gb, double *g_bar,
+                             double u_bar[3], double *om_bar, double *S_bar) {
+    const lattice_t *L = lat(7);
+    double T = 0.0;
+    for (int q = 0; q < 7; q++) T += g[q];
+    double t_bar = 0.0, ob = 0.0, sb = 0.0;
+    u_bar[0] = u_bar[1] = u_bar[2] = 0.0;
+    for (int q = 0; q < 7; q++) {
+        double cu = dot_c(L, q, u);
+        double eq = L->w[q] * T * (1.0 + 4.0 * cu);
+        double eq_bar = om * gb[q];
+        g_bar[q] = (1.0 - om) * gb[q];
+        ob += gb[q] * (eq - g[q]);
+        sb += gb[q] * L->w[q];
+        t_bar += eq_bar * L->w[q] * (1.0 + 4.0 * cu);
+        double cu_bar = eq_bar * L->w[q] * T * 4.0;
+        for (int d = 0; d < 3; d++) u_bar[d] += cu_bar * L->cf[q][d];
+    }
+    for (int q = 0; q < 7; q++) g_bar[q] += t_bar;
+    *om_bar = ob;
+    *S_bar = sb;
+}
+
+void orc_thermal_cell(const double *g, const double u[3], double om, double S, double *out) {
+    thermal_cell(g, u, om, S, out);
+}
+
+void orc_thermal_cell_vjp(const double *g, const double u[3], double om, const double *gb, double *g_bar,
+                          double *u_bar, double *om_bar, double *S_bar) {
+    thermal_cell_vjp(g, u, om, gb, g_bar, u_bar, om_bar, S_bar);
+}
+
+void orc_thermal_stream_table(const orc_problem *p, uint32_t *source) {
+    orc_problem q7 = *p;
+    q7.Q = 7;
+    q7.solid = NULL;
+    orc_stream_table(&q7, source);
+}
+
+void orc_thermal_initial(const orc_problem *p, double *g0) {
+    long n = (long)p->nx * p->ny * p->nz;
+    for (long i = 0; i < n * 7; i++) g0[i] = 0.0;
+}
+
+static void thermal_ports(const orc_problem *p, const orc_thermal *th, double *after) {
+    const lattice_t *L = lat(7);
+    for (int r = 0; r < p->n_port_cells; r++) {
+        long x = p->port_cell[r], nb = p->port_nb[r];
+        const double *gn = after + nb * 7;
+        if (p->port_kind[r] == 0) {
+            double Tn = 0.0;
+            for (int q = 0; q < 7; q++) Tn += gn[q];
+            for (int q = 0; q < 7; q++) after[x * 7 + q] = L->w[q] * th->inlet + (gn[q] - L->w[q] * Tn);
+        } else {
+            for (int q = 0; q < 7; q++) after[x * 7 + q] = gn[q];
+        }
+    }
+}
+
+/* One thermal step on the flow state f_before (pre-collision, the state the flow step of
+ * the same interval reads): collide, stream with source7 (orc_thermal_stream_table),
+ * ports.  post is scratch (n*7). */
+void orc_thermal_interval(const orc_problem *p, const orc_thermal *th, const uint32_t *source7, const double *alpha,
+                          const double *phi, const double *f_before, const double *g_before, double *post,
+                          double *g_after) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        double rho, j[3], den, sv[3], u[3], F[3];
+        brinkman_macro(L, f_before + x * Q, alpha[x], p->accel, &rho, j, &den, sv, u, F);
+        thermal_cell(g_before + x * 7, u, th_omega(th, phi[x]), th->source * phi[x], post + x * 7);
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n * 7; i++) g_after[i] = post[source7[i]];
+    thermal_ports(p, th, g_after);
+}
+
+/* VJP of orc_thermal_interval.  g_after_bar (n*7) is consumed (the ports' transpose
+ * works on it in place).  Outputs: g_bar (n*7, overwritten); f_bar, alpha_bar, phi_bar
+ * accumulated (+=). */
+void orc_thermal_interval_vjp(const orc_problem *p, const orc_thermal *th, const uint32_t *source7,
+                              const double *alpha, const double *phi, const double *f_before,
+                              const double *g_before, double *g_after_bar, double *post_bar, double *g_bar,
+                              double *f_bar, double *alpha_bar, double *phi_bar) {
+    const lattice_t *L = lat(p->Q);
+    const lattice_t *L7p = lat(7);
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    /* ports, in reverse order of application */
+    for (int r = p->n_port_cells - 1; r >= 0; r--) {
+        long x = p->port_cell[r], nb = p->port_nb[r];
+        double *bx = g_after_bar + x * 7, *bn = g_after_bar + nb * 7;
+        if (p->port_kind[r] == 0) {
+            double s = 0.0;
+            for (int q = 0; q < 7; q++) s += L7p->w[q] * bx[q];
+            for (int q = 0; q < 7; q++) bn[q] += bx[q] - s;
+        } else {
+            for (int q = 0; q < 7; q++) bn[q] += bx[q];
+        }
+        for (int q = 0; q < 7; q++) bx[q] = 0.0;
+    }
+    /* stream: every post slot has exactly one reader */
+    for (long i = 0; i < n * 7; i++) post_bar[source7[i]] = g_after_bar[i];
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        double rho, j[3], den, sv[3], u[3], F[3];
+        const double *f = f_before + x * Q;
+        brinkman_macro(L, f, alpha[x], p->accel, &rho, j, &den, sv, u, F);
+        double om = th_omega(th, phi[x]);
+        double ub[3], omb, sb;
+        thermal_cell_vjp(g_before + x * 7, u, om, post_bar + x * 7, g_bar + x * 7, ub, &omb, &sb);
+        /* u = (j/rho + a/2) / (1 + alpha/2) */
+        double v[3], s[3];
+        for (int d = 0; d < 3; d++) {
+            v[d] = j[d] / rho;
+            s[d] = ub[d] / den / rho;
+        }
+        for (int q = 0; q < Q; q++) {
+            double acc = 0.0;
+            for (int d = 0; d < 3; d++) acc += s[d] * (L->cf[q][d] - v[d]);
+            f_bar[x * Q + q] += acc;
+        }
+        double ab = 0.0;
+        for (int d = 0; d < 3; d++) ab += ub[d] * u[d];
+        alpha_bar[x] += -0.5 * ab / den;
+        /* omega = 1 / (1/2 + 4 kappa_f (1 + (ratio - 1) phi)) */
+        double dom = -om * om * 4.0 * th->kappa * (th->ratio - 1.0);
+        phi_bar[x] += omb * dom + sb * th->source;
+    }
+}
+
+/* Mean temperature over the cells with phi > 1/2 ("solid" cells of the design). */
+double orc_obj_solid_temperature(const orc_problem *p, const double *phi, const double *g) {
+    long n = (long)p->nx * p->ny * p->nz;
+    double acc = 0.0;
+    long cnt = 0;
+    for (long x = 0; x < n; x++) {
+        if (!(phi[x] > 0.5)) continue;
+        double T = 0.0;
+        for (int q = 0; q < 7; q++) T += g[x * 7 + q];
+        acc += T;
+        cnt++;
+    }
+    return cnt ? acc / (double)cnt : 0.0;
+}
+
+void orc_obj_solid_temperature_grad(const orc_problem *p, const double *phi, double weight, double *g_bar) {
+    long n = (long)p->nx * p->ny * p->nz;
+    long cnt = 0;
+    for (long x = 0; x < n; x++) cnt += phi[x] > 0.5;
+    if (!cnt) return;
+    double c = weight / (double)cnt;
+    for (long x = 0; x < n; x++)
+        if (phi[x] > 0.5)
+            for (int q = 0; q < 7; q++) g_bar[x * 7 + q] += c;
+}
+
+void orc_temperature(const orc_problem *p, const double *g, double *T) {
+    long n = (long)p->nx * p->ny * p->nz;
+    for (long x = 0; x < n; x++) {
+        double t = 0.0;
+        for (int q = 0; q < 7; q++) t += g[x * 7 + q];
+        T[x] = t;
     }
 }
