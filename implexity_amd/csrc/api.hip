@@ -208,6 +208,15 @@ struct lbm_handle {
     size_t pool_slots = 0;
     int np = 0;
     std::vector<double> h_alpha;
+    // Coupled D3Q7 temperature field: its populations follow the Q flow arrays in every
+    // state buffer (flow_bytes in), its cotangents the flow cotangents (flow_lam_bytes in).
+    bool thermal = false;
+    double tkappa = 0.0, tratio = 1.0, tsource = 0.0, tinlet = 0.0;
+    float tenc = 1.0f, tdec = 1.0f;  // FP16 storage scale of the temperature populations
+    size_t flow_bytes = 0, flow_lam_bytes = 0;
+    long long n_tsolid = 0;  // cells with phi > 1/2
+    DevBuf phi, phi_bar;
+    const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
     // Step grid: one block of b cells (a multiple of 64) along a z row, one row per grid
@@ -239,6 +248,37 @@ struct lbm_handle {
         s.macc = scal.as<double>() + 1;
         s.dstat = reinterpret_cast<unsigned*>(scal.as<double>() + 4);
         return s;
+    }
+    void* tpop(void* buf) const { return static_cast<char*>(buf) + flow_bytes; }
+    const void* tpop(const void* buf) const { return static_cast<const char*>(buf) + flow_bytes; }
+    void* tlam() const { return static_cast<char*>(L.p) + flow_lam_bytes; }
+    // Thermal step arguments: flow and temperature read from src (parity of `step`),
+    // temperature written to dst; no stage cost.
+    ThermalArgs thermal_args(const void* src, void* dst, uint64_t step) const {
+        ThermalArgs t{};
+        t.g = g;
+        t.pop = src;
+        t.tpop = tpop(src);
+        t.tpop_out = dst ? tpop(dst) : nullptr;
+        t.tlam = L.p ? tlam() : nullptr;
+        t.lam = L.p;
+        t.alpha = alpha.p;
+        t.phi = phi.p;
+        t.alpha_bar = alpha_bar.as<double>();
+        t.phi_bar = phi_bar.as<double>();
+        for (int d = 0; d < 3; d++) t.a[d] = a[d];
+        t.kappa = tkappa;
+        t.ratio_m1 = tratio - 1.0;
+        t.source = tsource;
+        t.tenc = tenc;
+        t.tdec = tdec;
+        t.odd = int(step & 1);
+        t.obj_w = 0.0;
+        t.part = nullptr;
+        t.stage = fold.as<double>();
+        t.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
+        t.jacc = scal.as<double>();
+        return t;
     }
     bool lam16() const { return lprec == LBM_F16_SHIFTED; }
     int* lsc_ptr() const { return lam16() ? lsc.as<int>() : nullptr; }
@@ -292,6 +332,13 @@ ObjArgs stage_obj(const ObjArgs& tmpl, const lbm_objective* o, uint64_t s, uint6
     if (s >= o->t_begin && s <= o->t_end && s >= 1) w += o->weight;
     if (with_terminal && s == n) w += o->terminal_weight;
     if (w == 0.0) return r;
+    if (o->kind == LBM_OBJ_THERMAL) {
+        // temperature part on the thermal kernels, pressure-drop part on the flow kernels
+        r.tw = w * o->vector_weights[0] * tmpl.inv_ns;
+        r.weight = w * o->vector_weights[1];
+        r.kind = r.weight != 0.0 ? int(OBJ_PRESSURE_DROP) : int(OBJ_NONE);
+        return r;
+    }
     r.kind = int(o->kind);
     r.weight = w;
     return r;
@@ -327,7 +374,11 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
     const int len = o->axis == 0 ? h->nx : (o->axis == 1 ? h->ny : h->nz);
     r.plane_a = ((o->plane_a % len) + len) % len;
     r.plane_b = ((o->plane_b % len) + len) % len;
-    if (o->kind == LBM_OBJ_PRESSURE_DROP) {
+    if (o->kind == LBM_OBJ_THERMAL && o->vector_weights[0] != 0.0) {
+        if (h->n_tsolid == 0) fail(LBM_ERR_CONTRACT, "no cell has thermal design phi > 1/2 (mean solid temperature)");
+        r.inv_ns = 1.0 / double(h->n_tsolid);
+    }
+    if (o->kind == LBM_OBJ_PRESSURE_DROP || o->kind == LBM_OBJ_THERMAL) {
         long long na = 0, nb = 0;
         const long long plane = h->ncell / len;
         if (h->solid.empty()) {
@@ -350,7 +401,12 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
 
 void validate_objective(const lbm_handle* h, const lbm_objective* o, uint64_t n) {
     if (!o || o->kind == LBM_OBJ_NONE) return;
-    if (o->kind > LBM_OBJ_VECTOR) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->kind > LBM_OBJ_THERMAL) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->kind == LBM_OBJ_THERMAL) {
+        if (!h->thermal) fail(LBM_ERR_CONTRACT, "the thermal objective needs the coupled temperature field");
+        if (!std::isfinite(o->vector_weights[0]) || !std::isfinite(o->vector_weights[1]))
+            fail(LBM_ERR_CONTRACT, "thermal objective weights must be finite");
+    }
     if (o->kind == LBM_OBJ_VECTOR) {
         for (int k = 0; k < 6; k++)
             if (!std::isfinite(o->vector_weights[k])) fail(LBM_ERR_CONTRACT, "vector weights must be finite");
@@ -415,6 +471,7 @@ void ensure_adjoint_buffers(lbm_handle* h) {
     h->L.alloc(h->lam_bytes, "cotangent state");
     h->A.alloc(h->state_bytes, "ahead state");
     h->alpha_bar.alloc(size_t(h->ncell) * sizeof(double), "alpha_bar");
+    if (h->thermal) h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
 }
 
 // Snapshot slots for a horizon: min(budget / state bytes, nsteps); the pool only grows.
@@ -475,6 +532,12 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
     s.diag = diag ? 1 : 0;
     if (obj.kind != OBJ_NONE || diag) s.part = h->part.as<double>();
     if (diag) s.macc = led->mass(s_index);
+    if (h->thermal) {  // before the flow kernel, which overwrites the flow state in place
+        ThermalArgs ta = h->thermal_args(src, dst, step);
+        ta.obj_w = obj.tw;
+        if (obj.tw != 0.0) ta.part = h->part.as<double>();
+        hip_check(h->trun->forward(h->Q, ta, h->cfg()), "thermal step launch");
+    }
     if (kspan) hip_check(hipEventRecord(kspan->first, h->stream), "event record");
     hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg(), kspan ? kspan->second : nullptr),
               "forward step launch");
@@ -482,7 +545,20 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
         hip_check(h->run->port_apply(h->Q, h->has_mask, h->port_args(step + 1), dst,
                                      diag ? led->exch(s_index) : nullptr, h->stream),
                   "port launch");
+        if (h->thermal)
+            hip_check(h->trun->ports(h->port_args(step + 1), h->tpop(dst), h->tinlet, h->tenc, h->tdec, h->stream),
+                      "thermal port launch");
     }
+}
+
+// Stage cost of the temperature field on a final state (into J) and, with do_seed, its
+// cotangent seed lambda_g.
+void thermal_final(lbm_handle* h, void* buf, uint64_t step, double obj_w, double seed_w, int do_seed) {
+    if (!h->thermal || (obj_w == 0.0 && !do_seed)) return;
+    ThermalArgs ta = h->thermal_args(buf, nullptr, step);
+    ta.obj_w = obj_w;
+    if (obj_w != 0.0) ta.part = h->part.as<double>();
+    hip_check(h->trun->seed(ta, seed_w, do_seed, h->cfg()), "thermal objective launch");
 }
 
 void step_once(lbm_handle* h, void* buf, uint64_t step, const ObjArgs& obj, const MassLedger* led,
@@ -585,6 +661,34 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             break;
         default: fail(LBM_ERR_CONTRACT, "unknown adjoint storage");
         }
+        h->thermal = cfg->thermal != 0;
+        if (cfg->thermal > 1) fail(LBM_ERR_CONTRACT, "thermal must be 0 or 1");
+        if (h->thermal) {
+            if (h->kind == K_BGK_GUO) fail(LBM_ERR_CONTRACT, "the temperature field couples to the Brinkman operators, not BGK-Guo");
+            if (h->prec == LBM_BF16_SHIFTED) fail(LBM_ERR_CONTRACT, "the temperature field supports FP64, FP32 and FP16 storage");
+            if (cfg->adjoint_storage == LBM_ADJ_F16_SCALED)
+                fail(LBM_ERR_CONTRACT, "the coupled temperature field keeps FP32 cotangents");
+            if (h->lprec == LBM_F16_SHIFTED) h->lprec = LBM_F32;
+            if (!(cfg->thermal_kappa > 0.0) || !std::isfinite(cfg->thermal_kappa))
+                fail(LBM_ERR_CONTRACT, "thermal_kappa must be finite and positive");
+            if (!(cfg->thermal_ratio > 0.0) || !std::isfinite(cfg->thermal_ratio))
+                fail(LBM_ERR_CONTRACT, "thermal_ratio must be finite and positive");
+            if (!std::isfinite(cfg->thermal_source) || !std::isfinite(cfg->thermal_inlet))
+                fail(LBM_ERR_CONTRACT, "thermal source and inlet temperature must be finite");
+            if (!(cfg->thermal_range >= 0.0) || !std::isfinite(cfg->thermal_range))
+                fail(LBM_ERR_CONTRACT, "thermal_range must be finite and non-negative");
+            h->tkappa = cfg->thermal_kappa;
+            h->tratio = cfg->thermal_ratio;
+            h->tsource = cfg->thermal_source;
+            h->tinlet = cfg->thermal_inlet;
+            if (h->prec == LBM_F16_SHIFTED) {
+                const int e = lam_exponent(cfg->thermal_range > 0.0 ? cfg->thermal_range : 1.0);
+                h->tenc = std::ldexp(1.0f, e);
+                h->tdec = std::ldexp(1.0f, -e);
+            }
+            h->trun = h->prec == LBM_F64 ? thermal_runner_f64()
+                                         : (h->prec == LBM_F32 ? thermal_runner_f32() : thermal_runner_f16s());
+        }
         for (int d = 0; d < 3; d++) {
             if (cfg->shape[d] < 2) fail(LBM_ERR_CONTRACT, "LBM shape must contain three integers >= 2");
             if (cfg->shape[d] > 65535) fail(LBM_ERR_CONTRACT, "lattice extent above 65535");
@@ -621,6 +725,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             }
             if (all) fail(LBM_ERR_CONTRACT, "Need fluid cells");
             h->has_mask = any;
+            if (any && h->thermal) fail(LBM_ERR_CONTRACT, "the coupled temperature field has no solid cells");
             if (!any) h->solid.clear();
         }
 
@@ -646,8 +751,11 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         // Kernels address an opposite pair of population arrays with 32-bit byte offsets.
         if (uint64_t(h->g.stride) * std::max(elem_bytes(h->prec), elem_bytes(h->lprec)) >= (uint64_t(1) << 31))
             fail(LBM_ERR_CONTRACT, "lattice too large: one population array would exceed 2 GiB");
-        h->state_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
-        h->lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
+        h->flow_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
+        h->flow_lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
+        const int TQn = h->thermal ? TQ : 0;
+        h->state_bytes = size_t(h->g.stride) * (h->Q + TQn) * elem_bytes(h->prec);
+        h->lam_bytes = size_t(h->g.stride) * (h->Q + TQn) * elem_bytes(h->lprec);
         h->run = runner_for(h->prec, h->lprec);
 
         // Ports (moving/boundary.rs:318-380).
@@ -709,6 +817,10 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->alpha.alloc(size_t(h->ncell) * h->csize(), "alpha");
         hip_check(hipMemsetAsync(h->alpha.p, 0, h->alpha.bytes, h->stream), "memset");
         h->h_alpha.assign(h->ncell, 0.0);
+        if (h->thermal) {
+            h->phi.alloc(size_t(h->ncell) * h->csize(), "thermal design");
+            hip_check(hipMemsetAsync(h->phi.p, 0, h->phi.bytes, h->stream), "memset");
+        }
         h->scal.alloc(64, "scalars");
         {
             const long long b = std::min(256, ((h->nz + 63) / 64) * 64);
@@ -852,6 +964,8 @@ int lbm_reset_initial(lbm_handle* h) {
                 f[size_t(x) * Q + q] = (!h->solid.empty() && h->solid[x]) ? lat_w(Q, q) : fluid[q];
         h->t = 0;
         upload_state(h, h->F.p, 0, f.data(), 0);
+        if (h->thermal)  // T = 0 everywhere
+            hip_check(hipMemsetAsync(h->tpop(h->F.p), 0, h->state_bytes - h->flow_bytes, h->stream), "memset");
         hip_check(hipStreamSynchronize(h->stream), "sync");
     });
 }
@@ -911,6 +1025,7 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
         ea.macc = led.mass(nsteps);
         ObjArgs none{};
         hip_check(h->run->objective(h->Q, h->has_mask, ea, none, nullptr, nullptr, 0, h->cfg()), "objective launch");
+        thermal_final(h, h->F.p, h->t, ea.obj.tw, 0.0, 0);
         double sc[4];
         unsigned ds[2];
         hip_check(hipMemcpyAsync(sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream), "copy");
@@ -996,6 +1111,107 @@ int lbm_vector_vjp(lbm_handle* h, const double* phi, uint64_t n, const lbm_vecto
         if (phi_bar)
             for (long long x = 0; x < h->ncell; x++) phi_bar[x] = w[5] * scales->cell_volume;
         (void)phi;
+    });
+}
+
+namespace {
+void need_thermal(const lbm_handle* h) {
+    if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+    if (!h->thermal) fail(LBM_ERR_CONTRACT, "no temperature field on this handle (lbm_config.thermal = 0)");
+}
+void ensure_thermal_aos(lbm_handle* h) {
+    ensure_aos(h);  // ncell * Q doubles >= ncell * 7
+}
+}  // namespace
+
+int lbm_set_thermal_design(lbm_handle* h, const double* phi, uint64_t n) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!phi) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "thermal design needs one value per cell");
+        long long ns = 0;
+        for (uint64_t x = 0; x < n; x++) {
+            if (!(phi[x] >= 0.0 && phi[x] <= 1.0)) fail(LBM_ERR_CONTRACT, "thermal design values must lie in [0, 1]");
+            ns += phi[x] > 0.5;
+        }
+        h->n_tsolid = ns;
+        if (h->prec == LBM_F64) {
+            hip_check(hipMemcpy(h->phi.p, phi, n * sizeof(double), hipMemcpyHostToDevice), "upload");
+        } else {
+            std::vector<float> f(phi, phi + n);
+            hip_check(hipMemcpy(h->phi.p, f.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
+        }
+    });
+}
+
+int lbm_get_thermal_state(lbm_handle* h, double* g, uint64_t len) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!g) fail(LBM_ERR_CONTRACT, "null argument");
+        if (len != uint64_t(h->ncell) * TQ) fail(LBM_ERR_SHAPE, "temperature state length must be cells*7");
+        ensure_thermal_aos(h);
+        hip_check(h->trun->get_state(h->g, h->tpop(h->F.p), int(h->t & 1), h->aos.as<double>(), h->tdec, h->cfg()),
+                  "get_state launch");
+        hip_check(hipMemcpyAsync(g, h->aos.p, len * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_set_thermal_state(lbm_handle* h, const double* g, uint64_t len) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!g) fail(LBM_ERR_CONTRACT, "null argument");
+        if (len != uint64_t(h->ncell) * TQ) fail(LBM_ERR_SHAPE, "temperature state length must be cells*7");
+        for (uint64_t x = 0; x < len; x++)
+            if (!std::isfinite(g[x])) fail(LBM_ERR_CONTRACT, "temperature populations must be finite");
+        ensure_thermal_aos(h);
+        hip_check(hipMemcpyAsync(h->aos.p, g, len * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy");
+        hip_check(h->trun->set_state(h->g, h->tpop(h->F.p), int(h->t & 1), h->aos.as<double>(), h->tenc, h->cfg()),
+                  "set_state launch");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+    });
+}
+
+int lbm_get_temperature(lbm_handle* h, double* T, uint64_t n) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!T) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "temperature needs one entry per cell");
+        std::vector<double> g(size_t(h->ncell) * TQ);
+        ensure_thermal_aos(h);
+        hip_check(h->trun->get_state(h->g, h->tpop(h->F.p), int(h->t & 1), h->aos.as<double>(), h->tdec, h->cfg()),
+                  "get_state launch");
+        hip_check(hipMemcpyAsync(g.data(), h->aos.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        for (long long x = 0; x < h->ncell; x++) {
+            double t = 0.0;
+            for (int q = 0; q < TQ; q++) t += g[size_t(x) * TQ + q];
+            T[x] = t;
+        }
+    });
+}
+
+int lbm_thermal_design_bar(lbm_handle* h, double* phi_bar, uint64_t n) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!phi_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi_bar needs one entry per cell");
+        if (!h->phi_bar.p) fail(LBM_ERR_INVALID, "no gradient has been computed on this handle");
+        hip_check(hipMemcpy(phi_bar, h->phi_bar.p, n * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    });
+}
+
+int lbm_get_thermal_lambda(lbm_handle* h, double* lambda, uint64_t len) {
+    return guarded([&] {
+        need_thermal(h);
+        if (!lambda) fail(LBM_ERR_CONTRACT, "null argument");
+        if (len != uint64_t(h->ncell) * TQ) fail(LBM_ERR_SHAPE, "temperature cotangent length must be cells*7");
+        if (!h->L.p) fail(LBM_ERR_INVALID, "no gradient has been computed on this handle");
+        ensure_thermal_aos(h);
+        hip_check(h->trun->get_lam(h->g, h->tlam(), int(h->t & 1), h->aos.as<double>(), h->cfg()), "get_lam launch");
+        hip_check(hipMemcpyAsync(lambda, h->aos.p, len * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
     });
 }
 
@@ -1126,6 +1342,8 @@ int lbm_reserve(lbm_handle* h, uint64_t nsteps, uint64_t* slots) {
 int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, double* alpha_bar, uint64_t len) {
     return guarded([&] {
         if (!h || !lambda_next || !lambda_prev || !alpha_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (h->thermal)
+            fail(LBM_ERR_CONTRACT, "lbm_step_vjp covers flow-only handles; use lbm_gradient with the temperature field");
         const int Q = h->Q;
         if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "cotangent length must be cells*Q");
         ensure_adjoint_buffers(h);
@@ -1211,6 +1429,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         auto slot_ptr = [&](uint64_t s) { return static_cast<char*>(h->pool.p) + size_t(s) * h->state_bytes; };
 
         reset_scalars(h);
+        if (h->thermal) hip_check(hipMemsetAsync(h->phi_bar.p, 0, h->phi_bar.bytes, h->stream), "memset");
         MassLedger led;
         led.init(h, nsteps);
         std::vector<hipEvent_t> evs;
@@ -1291,6 +1510,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                         hip_check(h->run->objective(Q, h->has_mask, ea, seed, h->L.p, h->alpha_bar.as<double>(), 1,
                                                     h->cfg()),
                                   "seed launch");
+                        thermal_final(h, h->A.p, base + nsteps, ea.obj.tw, seed.tw, 1);
                         if (lambda_final)  // lambda_n += lambda_final (the caller's terminal cotangent)
                             hip_check(h->run->set_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(),
                                                       int((base + nsteps) & 1), h->aos.as<double>(), h->lsc_ptr(),
@@ -1319,6 +1539,8 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     hip_check(h->run->port_vjp(Q, h->has_mask, h->port_args(base + k), h->L.p, h->pbuf.p,
                                                h->lsc_ptr(), h->stream),
                               "port vjp");
+                if (h->np && h->thermal)
+                    hip_check(h->trun->port_vjp(h->port_args(base + k), h->tlam(), h->stream), "thermal port vjp");
                 aa.pop = cur;
                 aa.odd = int((base + k - 1) & 1);
                 aa.obj = stage_obj(tmpl, obj, k - 1, nsteps, false);
@@ -1328,6 +1550,11 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     if (ks) hip_check(hipEventRecord(ks->first, h->stream), "event record");
                     hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg(), ks ? ks->second : nullptr),
                               "adjoint launch");
+                }
+                if (h->thermal) {  // after the flow kernel: adds the coupling to its cotangent
+                    ThermalArgs ta = h->thermal_args(cur, nullptr, base + k - 1);
+                    ta.obj_w = aa.obj.tw;
+                    hip_check(h->trun->adjoint(Q, ta, h->cfg()), "thermal adjoint launch");
                 }
                 rsteps++;
                 if (h->np && k > 1)
@@ -1419,7 +1646,25 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
         if (!h || !avg_ms) fail(LBM_ERR_CONTRACT, "null argument");
         if (launches == 0) fail(LBM_ERR_CONTRACT, "launches must be positive");
         ObjArgs none{};
-        if (which == 0 || which == 2) {
+        if (which == 3 || which == 4) {
+            // thermal kernels alone, in place (the state is not meaningful afterwards)
+            if (!h->thermal) fail(LBM_ERR_CONTRACT, "no temperature field on this handle");
+            if (which == 4) {
+                ensure_adjoint_buffers(h);
+                hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
+                hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+                hip_check(hipMemsetAsync(h->phi_bar.p, 0, h->phi_bar.bytes, h->stream), "memset");
+            }
+            auto launch = [&](uint64_t s) {
+                ThermalArgs ta = h->thermal_args(h->F.p, h->F.p, h->t + s);
+                hip_check(which == 3 ? h->trun->forward(h->Q, ta, h->cfg()) : h->trun->adjoint(h->Q, ta, h->cfg()),
+                          "thermal launch");
+            };
+            launch(0);  // untimed
+            hip_check(hipEventRecord(h->ev0, h->stream), "event");
+            for (uint64_t s = 0; s < launches; s++) launch(s + 1);
+            hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        } else if (which == 0 || which == 2) {
             // forward steps in place on the live state (no ports: the kernel alone); which
             // 2 runs the same addressing with the collision replaced by a copy (the
             // streaming-only memory ceiling of the layout)

@@ -6,4 +6,8 @@ const Runner* runner_f16s() {
     static const RunnerImpl<P_F16S, P_F32, true> r;
     return &r;
 }
+const ThermalRunner* thermal_runner_f16s() {
+    static const ThermalRunnerImpl<P_F16S, P_F32> r;
+    return &r;
+}
 }  // namespace lbm

@@ -7,4 +7,8 @@ const Runner* runner_f64() {
     static const RunnerImpl<P_F64, P_F64, true> r;
     return &r;
 }
+const ThermalRunner* thermal_runner_f64() {
+    static const ThermalRunnerImpl<P_F64, P_F64> r;
+    return &r;
+}
 }  // namespace lbm

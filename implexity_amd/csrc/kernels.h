@@ -69,6 +69,9 @@ struct ObjArgs {
     // rho alpha/dt |u|^2] in physical units (u * vel, rho * dens).
     double vw[5];
     double vel, dens, inv_dt;
+    // Coupled temperature field (host side only): 1 / (cells with phi > 1/2) in the
+    // template, and the stage weight per such cell on T of the state.
+    double inv_ns, tw;
 };
 
 struct StepArgs {

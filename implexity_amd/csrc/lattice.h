@@ -3,7 +3,9 @@
 // D3Q19 order and weights follow crates/implexity-physics-lbm/src/d3q19.rs:29-61
 // (rest first, then the lexicographic velocity list; OPPOSITE[i] = 19 - i).
 // D3Q27 follows moving/lattice.rs:93-129 (rest first, then (a,b,c) in {-1,0,1}^3
-// lexicographic without the origin; the opposite of i >= 1 is 27 - i).
+// lexicographic without the origin; the opposite of i >= 1 is 27 - i).  D3Q7 is the
+// lattice of the coupled temperature field (no counterpart in the reference), ordered
+// the same way.
 // Both orders must match bit for bit: the oracle and the reference index populations
 // this way.  Everything here is a constexpr function of the direction index (no
 // tables), so unrolled device loops fold to immediates.
@@ -22,7 +24,13 @@ namespace lbm {
 // Velocity component d of direction q.
 template <int Q>
 LBM_HD constexpr int cvec(int q, int d) {
-    if constexpr (Q == 19) {
+    if constexpr (Q == 7) {
+        // The thermal D3Q7 lattice: rest, then the six axis vectors in lexicographic
+        // order (opposite of i >= 1 is 7 - i), packed like D3Q19 below.
+        constexpr int code[7] = {13, 4, 10, 12, 14, 16, 22};
+        const int k = code[q];
+        return d == 0 ? k / 9 - 1 : (d == 1 ? (k / 3) % 3 - 1 : k % 3 - 1);
+    } else if constexpr (Q == 19) {
         // d3q19.rs:29-49, packed as (cx+1)*9 + (cy+1)*3 + (cz+1).
         constexpr int code[19] = {13, 1, 3, 4, 5, 7, 9, 10, 11, 12, 14, 15, 16, 17, 19, 21, 22, 23, 25};
         const int k = code[q];
@@ -55,7 +63,9 @@ LBM_HD constexpr int vorder(int q) {
 
 template <int Q>
 LBM_HD constexpr double weight(int q) {
-    if constexpr (Q == 19)
+    if constexpr (Q == 7)
+        return vorder<Q>(q) == 0 ? 0.25 : 0.125;  // c_s^2 = 1/4
+    else if constexpr (Q == 19)
         return vorder<Q>(q) == 0 ? 1.0 / 3.0 : (vorder<Q>(q) == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
     else
         return vorder<Q>(q) == 0 ? 8.0 / 27.0

@@ -8,6 +8,7 @@
 #include <cstdlib>
 
 #include "kernels.h"
+#include "thermal.h"
 
 namespace lbm {
 
@@ -352,6 +353,86 @@ struct RunnerImpl final : Runner {
         }
         if (mask) LBM_LAUNCH((k_fields<P, 27, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], rho, u);
         LBM_LAUNCH((k_fields<P, 27, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], rho, u);
+    }
+};
+
+// Launchers of the coupled temperature kernels (thermal.h) for D3Q19 and D3Q27 flows.
+struct ThermalRunner {
+    virtual ~ThermalRunner() = default;
+    // One thermal interval; with a.part, the stage cost is folded into a.jacc.
+    virtual hipError_t forward(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
+    virtual hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
+    virtual hipError_t seed(const ThermalArgs& a, double seed_w, int do_seed, const LaunchCfg& c) const = 0;
+    virtual hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec,
+                             hipStream_t s) const = 0;
+    virtual hipError_t port_vjp(const PortArgs& p, void* tlam, hipStream_t s) const = 0;
+    virtual hipError_t get_state(const Geom& g, const void* tpop, int odd, double* aos, float tdec,
+                                 const LaunchCfg& c) const = 0;
+    virtual hipError_t set_state(const Geom& g, void* tpop, int odd, const double* aos, float tenc,
+                                 const LaunchCfg& c) const = 0;
+    virtual hipError_t get_lam(const Geom& g, const void* tlam, int odd, double* aos, const LaunchCfg& c) const = 0;
+    virtual hipError_t set_lam(const Geom& g, void* tlam, int odd, const double* aos, const LaunchCfg& c) const = 0;
+};
+
+const ThermalRunner* thermal_runner_f64();
+const ThermalRunner* thermal_runner_f32();
+const ThermalRunner* thermal_runner_f16s();
+
+template <int P, int LP>
+struct ThermalRunnerImpl final : ThermalRunner {
+    static long long blocks(const LaunchCfg& c) { return (long long)c.grid.x * c.grid.y * c.grid.z; }
+    static hipError_t fold(const ThermalArgs& a, const LaunchCfg& c) {
+        hipLaunchKernelGGL(k_fold<FOLD_STEP>, dim3(fold_blocks(blocks(c))), dim3(256), 0, c.stream, a.part,
+                           blocks(c), a.stage, a.ticket, a.jacc, nullptr, nullptr);
+        return hipGetLastError();
+    }
+    static hipError_t done(const ThermalArgs& a, const LaunchCfg& c) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !a.part) return e;
+        return fold(a, c);
+    }
+    hipError_t forward(int Q, const ThermalArgs& a, const LaunchCfg& c) const override {
+        if (Q == 19)
+            hipLaunchKernelGGL((k_thermal_fwd<P, 19>), c.grid, c.block, 0, c.stream, a);
+        else if (Q == 27)
+            hipLaunchKernelGGL((k_thermal_fwd<P, 27>), c.grid, c.block, 0, c.stream, a);
+        else
+            return hipErrorInvalidValue;
+        return done(a, c);
+    }
+    hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const override {
+        if (!a.tlam || !a.lam || !a.alpha_bar || !a.phi_bar) return hipErrorInvalidValue;
+        if (Q == 19) LBM_LAUNCH((k_thermal_adj<P, LP, 19>), c, a);
+        if (Q == 27) LBM_LAUNCH((k_thermal_adj<P, LP, 27>), c, a);
+        return hipErrorInvalidValue;
+    }
+    hipError_t seed(const ThermalArgs& a, double seed_w, int do_seed, const LaunchCfg& c) const override {
+        if (do_seed && !a.tlam) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_thermal_seed<P, LP>), c.grid, c.block, 0, c.stream, a, seed_w, do_seed);
+        return done(a, c);
+    }
+    hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec,
+                     hipStream_t s) const override {
+        LBM_LAUNCH((k_thermal_port<P>), (LaunchCfg{dim3((p.np + PORT_BLOCK - 1) / PORT_BLOCK), dim3(PORT_BLOCK), s}),
+                   p, tpop, t_in, tenc, tdec);
+    }
+    hipError_t port_vjp(const PortArgs& p, void* tlam, hipStream_t s) const override {
+        LBM_LAUNCH((k_thermal_port_vjp<LP, typename Store<P>::C>),
+                   (LaunchCfg{dim3((p.np + PORT_BLOCK - 1) / PORT_BLOCK), dim3(PORT_BLOCK), s}), p, tlam);
+    }
+    hipError_t get_state(const Geom& g, const void* tpop, int odd, double* aos, float tdec,
+                         const LaunchCfg& c) const override {
+        LBM_LAUNCH((k_get_tstate<P>), c, g, tpop, odd, aos, tdec);
+    }
+    hipError_t set_state(const Geom& g, void* tpop, int odd, const double* aos, float tenc,
+                         const LaunchCfg& c) const override {
+        LBM_LAUNCH((k_set_tstate<P>), c, g, tpop, odd, aos, tenc);
+    }
+    hipError_t get_lam(const Geom& g, const void* tlam, int odd, double* aos, const LaunchCfg& c) const override {
+        LBM_LAUNCH((k_get_tlam<LP>), c, g, tlam, odd, aos);
+    }
+    hipError_t set_lam(const Geom& g, void* tlam, int odd, const double* aos, const LaunchCfg& c) const override {
+        LBM_LAUNCH((k_set_tlam<LP>), c, g, tlam, odd, aos);
     }
 };
 

@@ -1,0 +1,397 @@
+// Coupled D3Q7 temperature field (a double-distribution advection-diffusion lattice)
+// on the storage geometry of the flow.  No counterpart in the reference, whose thermal
+// coupling is finite-volume energy transport on D3Q19 links
+// (crates/implexity-physics-lbm/src/lbm/thermal.rs:91-134); the model:
+//   T = sum_i g_i,  g_i^eq = w_i T (1 + 4 c_i.u)   (w = 1/4, 1/8; c_s^2 = 1/4),
+//   u = the Brinkman velocity of the flow state the same interval collides,
+//   kappa(phi) = kappa_f (1 + (ratio - 1) phi),  tau_T = 1/2 + 4 kappa,  S = source phi,
+//   g_i* = g_i - omega_T (g_i - g_i^eq) + w_i S,
+// streamed in place with the esoteric twist and halfway bounce-back (adiabatic walls),
+// then at the flow's port cells: velocity ports hold T = T_in (g_b = w T_in + g_nb -
+// w T_nb), pressure ports are zero-gradient (g_b = g_nb).
+//
+// The temperature populations sit in the same state buffer as the flow, after its Q
+// arrays, so snapshots and restores move both fields together.  One interval runs the
+// thermal kernel first (it reads the flow state before the flow kernel overwrites it in
+// place), then the flow kernel, the flow ports and the thermal ports.  Hard solid cells
+// are not part of the model (the host refuses them), so every link mask is the face
+// mask.
+#pragma once
+#include "kernels.h"
+
+namespace lbm {
+
+// Temperature population storage: FP64 / FP32 as is; FP16 times a power of two chosen
+// on the host from the expected temperature range (tenc = 2^e, tdec = 2^-e).
+template <int P>
+struct TStore;
+template <>
+struct TStore<P_F64> {
+    using T = double;
+    using C = double;
+    static LBM_DEV C dec(T v, float) { return v; }
+    static LBM_DEV T enc(C v, float) { return v; }
+};
+template <>
+struct TStore<P_F32> {
+    using T = float;
+    using C = float;
+    static LBM_DEV C dec(T v, float) { return v; }
+    static LBM_DEV T enc(C v, float) { return v; }
+};
+template <>
+struct TStore<P_F16S> {
+    using T = _Float16;
+    using C = float;
+    static LBM_DEV C dec(T v, float s) { return float(v) * s; }
+    static LBM_DEV T enc(C v, float s) { return _Float16(v * s); }
+};
+
+struct ThermalArgs {
+    Geom g;
+    const void* pop;    // flow state read (parity odd)
+    const void* tpop;   // temperature populations read
+    void* tpop_out;     // temperature populations written (== tpop in place)
+    void* tlam;         // cotangent of the temperature populations (adjoint)
+    void* lam;          // cotangent of the flow populations (adjoint adds the coupling)
+    const void* alpha;  // flow design (arithmetic precision)
+    const void* phi;    // thermal design (arithmetic precision)
+    double* alpha_bar;
+    double* phi_bar;
+    double a[3];
+    double kappa, ratio_m1, source;  // ratio_m1 = ratio - 1
+    float tenc, tdec;
+    int odd;
+    double obj_w;  // stage cost of the state read: obj_w * T on every cell with phi > 1/2
+    double* part;  // per-block partials (J in slot 0), or null
+    double* stage;
+    unsigned* ticket;
+    double* jacc;
+};
+
+constexpr int TQ = 7;
+
+template <class C>
+LBM_DEV C thermal_omega(const ThermalArgs& A, C phi) {
+    return C(1) / (C(0.5) + C(4) * (C(A.kappa) * (C(1) + C(A.ratio_m1) * phi)));
+}
+
+template <class C>
+LBM_DEV C thermal_cu(const C u[3], int q) {
+    return u[0] * C(cvec<TQ>(q, 0)) + u[1] * C(cvec<TQ>(q, 1)) + u[2] * C(cvec<TQ>(q, 2));
+}
+
+// One cell: populations g, flow velocity u -> post-collision populations; T = sum g.
+template <class C>
+LBM_DEV void thermal_collide(const C* gv, const C u[3], C om, C S, C* out, C& T) {
+    T = C(0);
+#pragma unroll
+    for (int q = 0; q < TQ; q++) T += gv[q];
+#pragma unroll
+    for (int q = 0; q < TQ; q++) {
+        const C w = C(weight<TQ>(q));
+        const C eq = w * T * (C(1) + C(4) * thermal_cu(u, q));
+        out[q] = gv[q] - om * (gv[q] - eq) + w * S;
+    }
+}
+
+// Transpose of thermal_collide: gb = cotangent of the output.
+template <class C>
+LBM_DEV void thermal_collide_vjp(const C* gv, const C u[3], C om, const C* gb, C* g_bar, C u_bar[3], C& om_bar,
+                                 C& S_bar) {
+    C T = C(0);
+#pragma unroll
+    for (int q = 0; q < TQ; q++) T += gv[q];
+    C t_bar = C(0), ob = C(0), sb = C(0);
+    u_bar[0] = u_bar[1] = u_bar[2] = C(0);
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        const C w = C(weight<TQ>(q));
+        const C cu = thermal_cu(u, q);
+        const C eq = w * T * (C(1) + C(4) * cu);
+        const C eq_bar = om * gb[q];
+        g_bar[q] = (C(1) - om) * gb[q];
+        ob += gb[q] * (eq - gv[q]);
+        sb += gb[q] * w;
+        t_bar += eq_bar * w * (C(1) + C(4) * cu);
+        const C cu_bar = eq_bar * w * T * C(4);
+        if constexpr (cvec<TQ>(q, 0) != 0) u_bar[0] += cu_bar * C(cvec<TQ>(q, 0));
+        if constexpr (cvec<TQ>(q, 1) != 0) u_bar[1] += cu_bar * C(cvec<TQ>(q, 1));
+        if constexpr (cvec<TQ>(q, 2) != 0) u_bar[2] += cu_bar * C(cvec<TQ>(q, 2));
+    });
+#pragma unroll
+    for (int q = 0; q < TQ; q++) g_bar[q] += t_bar;
+    om_bar = ob;
+    S_bar = sb;
+}
+
+template <int P>
+LBM_DEV void load_tcell(const Geom& g, const typename TStore<P>::T* tp, const Cursor& cu, uint32_t mt, int odd,
+                        float tdec, typename TStore<P>::C* gv) {
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        gv[q] = TStore<P>::dec(load_read<TQ, q>(tp, g, cu, mt, odd), tdec);
+    });
+}
+
+// Forward: one thread per cell, before the flow kernel of the same interval.
+template <int P, int Q>
+__global__ __launch_bounds__(256) void k_thermal_fwd(ThermalArgs A) {
+    using S = Store<P>;
+    using TS = TStore<P>;
+    using C = typename S::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    double jsum = 0.0;
+    if (k < g.nz && j < g.ny) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const Cursor cu = make_cursor(g, i, j, k);
+        C f[Q];
+        load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, face_mask<Q>(g, i, j, k), A.odd, f);
+        const C alpha = static_cast<const C*>(A.alpha)[cell];
+        const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+        Macro<Q, C> mac;
+        macro_of<Q, C, S::shifted>(f, alpha, a, mac);
+        const uint32_t mt = face_mask<TQ>(g, i, j, k);
+        C gv[TQ], out[TQ], T;
+        load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
+        const C phi = static_cast<const C*>(A.phi)[cell];
+        thermal_collide<C>(gv, mac.u, thermal_omega<C>(A, phi), C(A.source) * phi, out, T);
+        if (A.obj_w != 0.0 && phi > C(0.5)) jsum = A.obj_w * double(T);
+        typename TS::T* to = static_cast<typename TS::T*>(A.tpop_out);
+        static_for<0, TQ>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            at_write<TQ, q>(to, g, cu, A.odd) = TS::enc(out[q], A.tenc);
+        });
+    }
+    if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
+}
+
+// Adjoint, after the flow adjoint kernel of the same reverse step: lambda_g in place
+// (write locations -> read locations), the velocity coupling added to the flow
+// cotangent the flow kernel has just written at the cell's read locations, and the
+// design cotangents.
+template <int P, int LP, int Q>
+__global__ __launch_bounds__(256) void k_thermal_adj(ThermalArgs A) {
+    using S = Store<P>;
+    using L = LamStore<LP>;
+    using C = typename S::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const Cursor cu = make_cursor(g, i, j, k);
+    const uint32_t m = face_mask<Q>(g, i, j, k);
+    C f[Q];
+    load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, m, A.odd, f);
+    const C alpha = static_cast<const C*>(A.alpha)[cell];
+    const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
+    Macro<Q, C> mac;
+    macro_of<Q, C, S::shifted>(f, alpha, a, mac);
+    const uint32_t mt = face_mask<TQ>(g, i, j, k);
+    C gv[TQ], gb[TQ], g_bar[TQ], ub[3], ob, sb;
+    load_tcell<P>(g, static_cast<const typename TStore<P>::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
+    typename L::T* tl = static_cast<typename L::T*>(A.tlam);
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        gb[q] = L::template ld<C>(&at_write<TQ, q>(tl, g, cu, A.odd));
+    });
+    const C phi = static_cast<const C*>(A.phi)[cell];
+    const C om = thermal_omega<C>(A, phi);
+    thermal_collide_vjp<C>(gv, mac.u, om, gb, g_bar, ub, ob, sb);
+    if (A.obj_w != 0.0 && phi > C(0.5)) {
+#pragma unroll
+        for (int q = 0; q < TQ; q++) g_bar[q] += C(A.obj_w);
+    }
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        store_read<TQ, q>(tl, g, cu, mt, A.odd, L::template enc<C>(g_bar[q]));
+    });
+    // u = (j / rho + a / 2) / (1 + alpha / 2):  d u_d / d f_q = (c_qd - j_d / rho) / (rho den)
+    C v[3], s[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v[d] = mac.j[d] / mac.rho;
+        s[d] = ub[d] / mac.den / mac.rho;
+    }
+    typename L::T* lf = static_cast<typename L::T*>(A.lam);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        C acc = C(0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc += s[d] * (C(cvec<Q>(q, d)) - v[d]);
+        typename L::T& slot = at_read<Q, q>(lf, g, cu, m, A.odd);
+        slot = L::template enc<C>(L::template ld<C>(&slot) + acc);
+    });
+    C au = C(0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) au += ub[d] * mac.u[d];
+    A.alpha_bar[cell] += double(C(-0.5) * au / mac.den);
+    const C dom = -om * om * C(4) * C(A.kappa) * C(A.ratio_m1);
+    A.phi_bar[cell] += double(ob * dom + sb * C(A.source));
+}
+
+// Final state of a sweep: its stage/terminal cost into the partials and, with do_seed,
+// lambda_g of the state := seed_w on the cells with phi > 1/2 (0 elsewhere).
+template <int P, int LP>
+__global__ __launch_bounds__(256) void k_thermal_seed(ThermalArgs A, double seed_w, int do_seed) {
+    using TS = TStore<P>;
+    using L = LamStore<LP>;
+    using C = typename TS::C;
+    const Geom& g = A.g;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    double jsum = 0.0;
+    if (k < g.nz && j < g.ny) {
+        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+        const Cursor cu = make_cursor(g, i, j, k);
+        const uint32_t mt = face_mask<TQ>(g, i, j, k);
+        const C phi = static_cast<const C*>(A.phi)[cell];
+        const bool solid = phi > C(0.5);
+        if (A.obj_w != 0.0 && solid) {
+            C gv[TQ];
+            load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
+            C T = C(0);
+#pragma unroll
+            for (int q = 0; q < TQ; q++) T += gv[q];
+            jsum = A.obj_w * double(T);
+        }
+        if (do_seed) {
+            typename L::T* tl = static_cast<typename L::T*>(A.tlam);
+            const C s = solid ? C(seed_w) : C(0);
+            static_for<0, TQ>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                store_read<TQ, q>(tl, g, cu, mt, A.odd, L::template enc<C>(s));
+            });
+        }
+    }
+    if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
+}
+
+// Port cells after streaming (state of parity A.odd).
+template <int P>
+__global__ __launch_bounds__(128) void k_thermal_port(PortArgs A, void* tpop_, double t_in, float tenc, float tdec) {
+    using TS = TStore<P>;
+    using C = typename TS::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.np) return;
+    typename TS::T* tp = static_cast<typename TS::T*>(tpop_);
+    Cursor cx, cn;
+    const uint32_t mx = mask_of<TQ, false>(A, A.cell[r], cx);
+    const uint32_t mn = mask_of<TQ, false>(A, A.nb[r], cn);
+    C gn[TQ], out[TQ];
+    load_tcell<P>(A.g, tp, cn, mn, A.odd, tdec, gn);
+    if (A.kind[r] == 0) {
+        C Tn = C(0);
+#pragma unroll
+        for (int q = 0; q < TQ; q++) Tn += gn[q];
+#pragma unroll
+        for (int q = 0; q < TQ; q++) {
+            const C w = C(weight<TQ>(q));
+            out[q] = w * C(t_in) + (gn[q] - w * Tn);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TQ; q++) out[q] = gn[q];
+    }
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        store_read<TQ, q>(tp, A.g, cx, mx, A.odd, TS::enc(out[q], tenc));
+    });
+}
+
+// Transpose of the thermal port pass on lambda_g of a state of parity A.odd.
+template <int LP, class C>
+__global__ __launch_bounds__(128) void k_thermal_port_vjp(PortArgs A, void* tlam_) {
+    using L = LamStore<LP>;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.np) return;
+    typename L::T* tl = static_cast<typename L::T*>(tlam_);
+    Cursor cx, cn;
+    const uint32_t mx = mask_of<TQ, false>(A, A.cell[r], cx);
+    const uint32_t mn = mask_of<TQ, false>(A, A.nb[r], cn);
+    C gx[TQ], gn[TQ];
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        gx[q] = L::template ld<C>(&at_read<TQ, q>(tl, A.g, cx, mx, A.odd));
+        gn[q] = L::template ld<C>(&at_read<TQ, q>(tl, A.g, cn, mn, A.odd));
+    });
+    C s = C(0);
+    if (A.kind[r] == 0) {
+#pragma unroll
+        for (int q = 0; q < TQ; q++) s += C(weight<TQ>(q)) * gx[q];
+    }
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        store_read<TQ, q>(tl, A.g, cn, mn, A.odd, L::template enc<C>(gn[q] + (gx[q] - s)));
+        store_read<TQ, q>(tl, A.g, cx, mx, A.odd, L::template enc<C>(C(0)));
+    });
+}
+
+// Layout conversion of the temperature populations / their cotangent (AoS x*7+q).
+template <int P>
+__global__ __launch_bounds__(256) void k_get_tstate(Geom g, const void* tpop, int odd, double* aos, float tdec) {
+    using TS = TStore<P>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const Cursor cu = make_cursor(g, i, j, k);
+    typename TS::C gv[TQ];
+    load_tcell<P>(g, static_cast<const typename TS::T*>(tpop), cu, face_mask<TQ>(g, i, j, k), odd, tdec, gv);
+#pragma unroll
+    for (int q = 0; q < TQ; q++) aos[cell * TQ + q] = double(gv[q]);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_set_tstate(Geom g, void* tpop, int odd, const double* aos, float tenc) {
+    using TS = TStore<P>;
+    using C = typename TS::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const Cursor cu = make_cursor(g, i, j, k);
+    const uint32_t mt = face_mask<TQ>(g, i, j, k);
+    typename TS::T* tp = static_cast<typename TS::T*>(tpop);
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        store_read<TQ, q>(tp, g, cu, mt, odd, TS::enc(C(aos[cell * TQ + q]), tenc));
+    });
+}
+
+template <int LP>
+__global__ __launch_bounds__(256) void k_get_tlam(Geom g, const void* tlam, int odd, double* aos) {
+    using L = LamStore<LP>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const Cursor cu = make_cursor(g, i, j, k);
+    const uint32_t mt = face_mask<TQ>(g, i, j, k);
+    const typename L::T* tl = static_cast<const typename L::T*>(tlam);
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        aos[cell * TQ + q] = L::template ld<double>(&at_read<TQ, q>(tl, g, cu, mt, odd));
+    });
+}
+
+template <int LP>
+__global__ __launch_bounds__(256) void k_set_tlam(Geom g, void* tlam, int odd, const double* aos) {
+    using L = LamStore<LP>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    const Cursor cu = make_cursor(g, i, j, k);
+    const uint32_t mt = face_mask<TQ>(g, i, j, k);
+    typename L::T* tl = static_cast<typename L::T*>(tlam);
+    static_for<0, TQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        store_read<TQ, q>(tl, g, cu, mt, odd, L::template enc<double>(aos[cell * TQ + q]));
+    });
+}
+
+}  // namespace lbm

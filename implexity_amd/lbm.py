@@ -32,7 +32,7 @@ PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
 ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
 COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
-OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3}
+OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3, "thermal": 4}
 MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
 
 
@@ -87,6 +87,9 @@ class _Config(ctypes.Structure):
         ("n_ports", ctypes.c_uint32), ("reserved1", ctypes.c_uint32), ("ports", ctypes.c_void_p),
         ("checkpoint_budget_bytes", ctypes.c_uint64), ("adjoint_storage", ctypes.c_uint32),
         ("checkpoint_every", ctypes.c_uint32),
+        ("thermal", ctypes.c_uint32), ("reserved2", ctypes.c_uint32), ("thermal_kappa", ctypes.c_double),
+        ("thermal_ratio", ctypes.c_double), ("thermal_source", ctypes.c_double),
+        ("thermal_inlet", ctypes.c_double), ("thermal_range", ctypes.c_double),
     ]
 
 
@@ -164,6 +167,12 @@ _SIGS = {
     "lbm_time_kernel": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_copy_bandwidth": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "lbm_state_bytes": ([ctypes.c_void_p], ctypes.c_uint64),
+    "lbm_set_thermal_design": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_get_thermal_state": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_set_thermal_state": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_get_temperature": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_thermal_design_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_get_thermal_lambda": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
 }
 
 
@@ -225,6 +234,15 @@ class LatticeConfig:
     adjoint_storage: str = "auto"
     # 0: binomial checkpointing inside the budget; k: Checkpointing::Every(k) (scan.rs:109-246)
     checkpoint_every: int = 0
+    # Coupled D3Q7 temperature field (no counterpart in the reference): diffusivity
+    # kappa (1 + (ratio - 1) phi), source * phi per step, T = inlet at velocity ports,
+    # zero-gradient at pressure ports; `thermal_range` bounds |T| for FP16 storage.
+    thermal: bool = False
+    thermal_kappa: float = 0.0
+    thermal_ratio: float = 1.0
+    thermal_source: float = 0.0
+    thermal_inlet: float = 0.0
+    thermal_range: float = 0.0
 
 
 @dataclass
@@ -300,6 +318,12 @@ class Lattice:
             raise ValueError(f"adjoint_storage must be one of {sorted(ADJOINT_STORAGE)}")
         c.adjoint_storage = ADJOINT_STORAGE[cfg.adjoint_storage]
         c.checkpoint_every = int(cfg.checkpoint_every)
+        c.thermal = 1 if cfg.thermal else 0
+        c.thermal_kappa = float(cfg.thermal_kappa)
+        c.thermal_ratio = float(cfg.thermal_ratio)
+        c.thermal_source = float(cfg.thermal_source)
+        c.thermal_inlet = float(cfg.thermal_inlet)
+        c.thermal_range = float(cfg.thermal_range)
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)
@@ -332,6 +356,37 @@ class Lattice:
 
     def initial_populations(self):
         _check(lib().lbm_reset_initial(self._h))
+
+    # --- coupled temperature field ------------------------------------------------------
+    def set_thermal_design(self, phi):
+        a = np.ascontiguousarray(phi, np.float64).reshape(-1)
+        _check(lib().lbm_set_thermal_design(self._h, _p(a), a.size))
+
+    def thermal_state(self):
+        out = np.empty(self.n * 7)
+        _check(lib().lbm_get_thermal_state(self._h, _p(out), out.size))
+        return out
+
+    def set_thermal_state(self, g):
+        g = np.ascontiguousarray(g, np.float64).reshape(-1)
+        _check(lib().lbm_set_thermal_state(self._h, _p(g), g.size))
+
+    def temperature(self):
+        out = np.empty(self.n)
+        _check(lib().lbm_get_temperature(self._h, _p(out), out.size))
+        return out
+
+    def thermal_design_bar(self):
+        """dJ/dphi through the temperature field of the last gradient call."""
+        out = np.empty(self.n)
+        _check(lib().lbm_thermal_design_bar(self._h, _p(out), out.size))
+        return out
+
+    def thermal_lambda(self):
+        """Cotangent of the temperature populations of state 0 of the last gradient call."""
+        out = np.empty(self.n * 7)
+        _check(lib().lbm_get_thermal_lambda(self._h, _p(out), out.size))
+        return out
 
     def set_state(self, f, layout="aos"):
         f = np.ascontiguousarray(f, np.float64).reshape(-1)
@@ -432,7 +487,7 @@ class Lattice:
 
     def time_kernel(self, which="forward", launches=10):
         ms = ctypes.c_double(0.0)
-        code = {"forward": 0, "adjoint": 1, "stream": 2}[which]
+        code = {"forward": 0, "adjoint": 1, "stream": 2, "thermal_forward": 3, "thermal_adjoint": 4}[which]
         _check(lib().lbm_time_kernel(self._h, code, launches, ctypes.byref(ms)))
         return ms.value
 

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 /* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
  * crates/implexity-ad/src/error.rs). */
@@ -132,6 +132,22 @@ typedef struct {
     uint32_t adjoint_storage;     /* lbm_adjoint_storage */
     uint32_t checkpoint_every;    /* 0: binomial (revolve) in the budget; k > 0: Checkpointing::Every(k)
                                      (scan.rs:109-246), needing ceil(n/k) + max(k-2, 0) snapshots */
+    /* Coupled D3Q7 temperature field (double distribution).  No counterpart in the
+     * reference, whose thermal coupling is finite-volume transport on D3Q19 links
+     * (lbm/thermal.rs:91-134).  T = sum g_i, g_i^eq = w_i T (1 + 4 c_i.u) with the flow's
+     * Brinkman velocity u, diffusivity kappa(phi) = thermal_kappa (1 + (thermal_ratio - 1)
+     * phi), tau_T = 1/2 + 4 kappa, source thermal_source * phi per step, adiabatic
+     * (bounce-back) walls; at the flow's ports: velocity ports hold T = thermal_inlet,
+     * pressure ports are zero-gradient.  phi is the thermal design (lbm_set_thermal_design,
+     * default 0).  Needs a fluid-only lattice (no solid mask), a non-BGK-Guo operator and
+     * FP64/FP32/FP16 storage; cotangents stay FP64 (FP64 storage) or FP32. */
+    uint32_t thermal;             /* 0: flow only; 1: flow + temperature */
+    uint32_t reserved2;
+    double thermal_kappa;         /* fluid diffusivity (lattice units), > 0 */
+    double thermal_ratio;         /* solid:fluid diffusivity ratio, > 0 */
+    double thermal_source;        /* heat source per step at phi = 1 */
+    double thermal_inlet;         /* temperature held at velocity ports */
+    double thermal_range;         /* FP16 storage: bound on |T| that sets the storage scale (0: 1) */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
@@ -140,9 +156,13 @@ typedef enum {
     LBM_OBJ_NONE = 0,
     LBM_OBJ_DISSIPATION = 1,   /* sum over fluid cells of rho alpha |u|^2 (lattice units) */
     LBM_OBJ_PRESSURE_DROP = 2, /* mean rho/3 on plane a minus mean rho/3 on plane b */
-    LBM_OBJ_VECTOR = 3         /* w . vector(state), the terminal responses of solver.rs:874-894
+    LBM_OBJ_VECTOR = 3,        /* w . vector(state), the terminal responses of solver.rs:874-894
                                   (the design-volume term w[5] is not state-dependent and is
                                   left out of J; lbm_vector reports it) */
+    LBM_OBJ_THERMAL = 4        /* coupled temperature field: w[0] * (mean T over the cells with
+                                  thermal design phi > 1/2) + w[1] * (pressure drop between
+                                  plane_a and plane_b); w = vector_weights (no counterpart in
+                                  the reference) */
 } lbm_objective_kind;
 
 /* Physical scales of the terminal responses (LbmProblem::vector_scales, solver.rs:862-871). */
@@ -162,7 +182,7 @@ typedef struct {
     uint64_t t_end;       /* last state carrying the stage cost (<= n) */
     double weight;        /* stage weight */
     double terminal_weight;
-    double vector_weights[6];     /* LBM_OBJ_VECTOR: w */
+    double vector_weights[6];     /* LBM_OBJ_VECTOR and LBM_OBJ_THERMAL: w */
     lbm_vector_scales scales;     /* LBM_OBJ_VECTOR */
 } lbm_objective;
 
@@ -260,9 +280,26 @@ uint64_t lbm_revolve_optimal_forward_steps(uint64_t steps, uint64_t snapshots);
 int lbm_every_schedule(uint64_t steps, uint64_t every, int64_t* actions, uint64_t cap, uint64_t* count,
                        uint64_t* forward_steps, uint64_t* reverse_start);
 
+/* Coupled temperature field (lbm_config.thermal = 1); no counterpart in the reference.
+ * phi: thermal design per cell in [0, 1] (diffusivity and source interpolate with it;
+ * the cells with phi > 1/2 are the "solid" cells of LBM_OBJ_THERMAL).  States are AoS
+ * x*7+q over the D3Q7 lattice (rest, then the axis vectors in lexicographic order).
+ * lbm_thermal_design_bar returns dJ/dphi through the temperature field of the last
+ * lbm_gradient / lbm_gradient_seeded call (dJ/dalpha of the flow, coupling included,
+ * is that call's alpha_bar); lbm_get_thermal_lambda returns the cotangent of the
+ * temperature populations of state 0 of that call.  lbm_step_vjp covers flow-only
+ * handles. */
+int lbm_set_thermal_design(lbm_handle* h, const double* phi, uint64_t n);
+int lbm_get_thermal_state(lbm_handle* h, double* g, uint64_t len);
+int lbm_set_thermal_state(lbm_handle* h, const double* g, uint64_t len);
+int lbm_get_temperature(lbm_handle* h, double* T, uint64_t n);
+int lbm_thermal_design_bar(lbm_handle* h, double* phi_bar, uint64_t n);
+int lbm_get_thermal_lambda(lbm_handle* h, double* lambda, uint64_t len);
+
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
- * a copy) on the handle's stream, measured with HIP events. */
+ * a copy, 3 thermal forward step, 4 thermal adjoint step) on the handle's stream,
+ * measured with HIP events. */
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
 /* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
 int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);

@@ -1,0 +1,156 @@
+"""GPU parity of the coupled D3Q7 temperature field against the thermal oracle.
+
+Bars: FP64 flow and temperature states bit-identical to the oracle after several
+coupled intervals (ports, walls, periodic axes, D3Q19 TRT and D3Q27 cumulant flows); the
+FP64 gradient (J, dJ/dalpha with the coupling, dJ/dphi, both state-0 cotangents) within
+1e-11 of the oracle's coupled scan with binomial and Every(k) checkpointing; FP32 and
+FP16 temperature states within 2e-6 and 2e-3 of max |T| after 20 intervals; reduced
+precision gradients within 1e-4 (FP32) and 2e-2 (FP16 storage) relative L2.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from implexity_amd.lbm import Lattice, Objective
+from oracle import oracle as orc
+from oracle.thermal import CoupledObjective, Thermal, ThermalOracle
+from tests.lbm_cases import channel_ports, make_pair, random_alpha
+
+pytestmark = pytest.mark.gpu
+
+TH = Thermal(kappa=0.02, ratio=10.0, source=1e-3, inlet=0.3)
+
+
+def _pair(shape, periodic, ports, accel, precision="f64", Q=19, coll="trt", budget=0):
+    cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision=precision, omega=1.25, accel=accel,
+                          ports=ports, bulk_rate=1.0 if coll == "cumulant" else float("nan"), budget=budget)
+    cfg.thermal = True
+    cfg.thermal_kappa = TH.kappa
+    cfg.thermal_ratio = TH.ratio
+    cfg.thermal_source = TH.source
+    cfg.thermal_inlet = TH.inlet
+    return cfg, ThermalOracle(orc.Oracle(prob), TH)
+
+
+CASES = [
+    # shape, periodic, with ports, accel, Q, collision
+    ((7, 6, 5), (False, False, False), True, (0.0, 0.0, 0.0), 19, "trt"),
+    ((8, 6, 8), (True, False, False), False, (1e-4, 2e-5, 0.0), 19, "trt"),
+    ((6, 5, 7), (True, True, True), False, (1e-4, 0.0, 3e-5), 19, "trt"),
+    ((6, 5, 4), (True, False, False), False, (1e-4, 0.0, 0.0), 27, "cumulant"),
+    ((7, 5, 6), (False, False, False), True, (0.0, 0.0, 0.0), 19, "mrt"),
+]
+
+
+def _design(t, seed):
+    shape = t.orc.prob.shape
+    return random_alpha(shape, seed), np.random.default_rng(seed + 1).random(t.n)
+
+
+@pytest.mark.parametrize("shape,periodic,with_ports,accel,Q,coll", CASES)
+def test_coupled_forward_f64_bit_exact(shape, periodic, with_ports, accel, Q, coll):
+    ports = channel_ports(shape, 0.03) if with_ports else ()
+    cfg, t = _pair(shape, periodic, ports, accel, Q=Q, coll=coll)
+    alpha, phi = _design(t, 3)
+    g0 = np.random.default_rng(5).random(t.n * 7) * 0.05
+    steps = 11
+    f_want, g_want = t.run(alpha, phi, steps, g0=g0)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        lat.set_thermal_state(g0)
+        lat.interval(steps)
+        f, g, T = lat.state(), lat.thermal_state(), lat.temperature()
+    assert np.array_equal(f, f_want), np.abs(f - f_want).max()
+    assert np.array_equal(g, g_want), np.abs(g - g_want).max()
+    assert np.array_equal(T, t.temperature(g_want))
+
+
+def _objective(steps, nx):
+    ob = Objective(kind="thermal", t_begin=steps // 2, t_end=steps, weight=1.0 / (steps - steps // 2 + 1),
+                   terminal_weight=0.5, axis=0, plane_a=1, plane_b=nx - 2, vector_weights=(1.0, 0.1, 0, 0, 0, 0))
+    oob = CoupledObjective(window=(ob.t_begin, ob.t_end), weight=ob.weight, terminal_weight=0.5, w_T=1.0, w_dp=0.1,
+                           axis=0, plane_a=1, plane_b=nx - 2)
+    return ob, oob
+
+
+@pytest.mark.parametrize("every", [0, 4])
+def test_coupled_gradient_f64_matches_oracle(every):
+    shape = (8, 6, 5)
+    steps = 13
+    cfg, t = _pair(shape, (False, False, False), channel_ports(shape, 0.03), (0.0, 0.0, 0.0))
+    alpha, phi = _design(t, 7)
+    ob, oob = _objective(steps, shape[0])
+    J_want, ab_want, pb_want, lf_want, lg_want = t.scan_adjoint(oob, alpha, phi, steps)
+    cfg.checkpoint_every = every
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        J, ab, lf, st = lat.gradient(steps, ob, want_lambda0=True)
+        pb, lg = lat.thermal_design_bar(), lat.thermal_lambda()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want), (J, J_want)
+    for got, want in ((ab, ab_want), (pb, pb_want), (lf, lf_want), (lg, lg_want)):
+        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max(), np.abs(got - want).max() / np.abs(want).max()
+    assert st.reverse_steps == steps
+
+
+def test_coupled_gradient_binomial_recompute_f64():
+    """A snapshot budget of three coupled states: the revolve sweep recomputes both
+    fields from the snapshots and reproduces the all-states gradient."""
+    shape = (8, 6, 5)
+    steps = 17
+    cfg, t = _pair(shape, (False, False, False), channel_ports(shape, 0.03), (0.0, 0.0, 0.0))
+    alpha, phi = _design(t, 9)
+    ob, oob = _objective(steps, shape[0])
+    _, ab_want, pb_want, _, _ = t.scan_adjoint(oob, alpha, phi, steps)
+    with Lattice(cfg) as lat:
+        cfg.checkpoint_budget_bytes = 3 * lat.state_bytes
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        _, ab, _, st = lat.gradient(steps, ob)
+        pb = lat.thermal_design_bar()
+    assert st.snapshot_slots == 3 and st.forward_steps > steps
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+    assert np.abs(pb - pb_want).max() <= 1e-11 * np.abs(pb_want).max()
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 2e-6), ("f16s", 2e-3)])
+def test_coupled_forward_reduced_precision(precision, tol):
+    shape = (8, 6, 8)
+    cfg, t = _pair(shape, (False, False, False), channel_ports(shape, 0.03), (0.0, 0.0, 0.0), precision=precision)
+    alpha, phi = _design(t, 11)
+    steps = 20
+    _, g_want = t.run(alpha, phi, steps)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        lat.interval(steps)
+        T = lat.temperature()
+    T_want = t.temperature(g_want)
+    err = np.abs(T - T_want).max() / np.abs(T_want).max()
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 1e-4), ("f16s", 2e-2)])
+def test_coupled_gradient_reduced_precision(precision, tol):
+    shape = (12, 8, 8)
+    steps = 40
+    cfg, t = _pair(shape, (False, False, False), channel_ports(shape, 0.03), (0.0, 0.0, 0.0), precision=precision)
+    alpha, phi = _design(t, 13)
+    ob, oob = _objective(steps, shape[0])
+    J_want, ab_want, pb_want, _, _ = t.scan_adjoint(oob, alpha, phi, steps)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+        pb = lat.thermal_design_bar()
+    ea = np.linalg.norm(ab - ab_want) / np.linalg.norm(ab_want)
+    ep = np.linalg.norm(pb - pb_want) / np.linalg.norm(pb_want)
+    assert ea <= tol and ep <= tol and abs(J - J_want) <= tol * abs(J_want), (ea, ep, J, J_want)
