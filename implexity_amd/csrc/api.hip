@@ -273,6 +273,7 @@ struct lbm_handle {
         t.tenc = tenc;
         t.tdec = tdec;
         t.odd = int(step & 1);
+        t.step = uint32_t(step);
         t.obj_w = 0.0;
         t.part = nullptr;
         t.stage = fold.as<double>();
@@ -546,7 +547,8 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
                                      diag ? led->exch(s_index) : nullptr, h->stream),
                   "port launch");
         if (h->thermal)
-            hip_check(h->trun->ports(h->port_args(step + 1), h->tpop(dst), h->tinlet, h->tenc, h->tdec, h->stream),
+            hip_check(h->trun->ports(h->port_args(step + 1), h->tpop(dst), h->tinlet, h->tenc, h->tdec,
+                                     uint32_t(step), h->stream),
                       "thermal port launch");
     }
 }

@@ -363,7 +363,7 @@ struct ThermalRunner {
     virtual hipError_t forward(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
     virtual hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
     virtual hipError_t seed(const ThermalArgs& a, double seed_w, int do_seed, const LaunchCfg& c) const = 0;
-    virtual hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec,
+    virtual hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec, uint32_t step,
                              hipStream_t s) const = 0;
     virtual hipError_t port_vjp(const PortArgs& p, void* tlam, hipStream_t s) const = 0;
     virtual hipError_t get_state(const Geom& g, const void* tpop, int odd, double* aos, float tdec,
@@ -411,10 +411,10 @@ struct ThermalRunnerImpl final : ThermalRunner {
         hipLaunchKernelGGL((k_thermal_seed<P, LP>), c.grid, c.block, 0, c.stream, a, seed_w, do_seed);
         return done(a, c);
     }
-    hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec,
+    hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec, uint32_t step,
                      hipStream_t s) const override {
         LBM_LAUNCH((k_thermal_port<P>), (LaunchCfg{dim3((p.np + PORT_BLOCK - 1) / PORT_BLOCK), dim3(PORT_BLOCK), s}),
-                   p, tpop, t_in, tenc, tdec);
+                   p, tpop, t_in, tenc, tdec, step);
     }
     hipError_t port_vjp(const PortArgs& p, void* tlam, hipStream_t s) const override {
         LBM_LAUNCH((k_thermal_port_vjp<LP, typename Store<P>::C>),

@@ -47,6 +47,38 @@ struct TStore<P_F16S> {
     static LBM_DEV T enc(C v, float s) { return _Float16(v * s); }
 };
 
+// 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
+LBM_DEV uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// FP16 with stochastic rounding: the 13 float significand bits below the FP16 ones are
+// replaced by a carry from r's low 13 bits, so a value rounds up with probability equal
+// to its distance from the lower FP16 neighbour (E[result] = x).  The temperature moves
+// by small increments per step (sources, fluxes) next to an 11-bit stored value; nearest
+// rounding turns that into a bias that accumulates over thousands of steps, stochastic
+// rounding into zero-mean noise.  Magnitudes below the FP16 normal range round to nearest.
+LBM_DEV _Float16 f16_stochastic(float x, uint32_t r) {
+    const uint32_t b = __float_as_uint(x);
+    if ((b & 0x7fffffffu) < 0x38800000u) return _Float16(x);
+    return _Float16(__uint_as_float((b + (r & 0x1fffu)) & ~0x1fffu));
+}
+
+// Store of a temperature population written by a step: stochastic rounding for FP16
+// (bits from the cell, the direction and the step, so recomputed steps reproduce it).
+template <int P>
+LBM_DEV typename TStore<P>::T tstore_step(typename TStore<P>::C v, float s, uint32_t key) {
+    if constexpr (P == P_F16S)
+        return f16_stochastic(v * s, mix32(key));
+    else
+        return TStore<P>::enc(v, s);
+}
+
 struct ThermalArgs {
     Geom g;
     const void* pop;    // flow state read (parity odd)
@@ -62,6 +94,7 @@ struct ThermalArgs {
     double kappa, ratio_m1, source;  // ratio_m1 = ratio - 1
     float tenc, tdec;
     int odd;
+    uint32_t step;  // index of the interval (rounding key of FP16 stores)
     double obj_w;  // stage cost of the state read: obj_w * T on every cell with phi > 1/2
     double* part;  // per-block partials (J in slot 0), or null
     double* stage;
@@ -160,9 +193,10 @@ __global__ __launch_bounds__(256) void k_thermal_fwd(ThermalArgs A) {
         thermal_collide<C>(gv, mac.u, thermal_omega<C>(A, phi), C(A.source) * phi, out, T);
         if (A.obj_w != 0.0 && phi > C(0.5)) jsum = A.obj_w * double(T);
         typename TS::T* to = static_cast<typename TS::T*>(A.tpop_out);
+        const uint32_t key = (uint32_t(cell) * uint32_t(TQ)) ^ (A.step * 0x9e3779b9u);
         static_for<0, TQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
-            at_write<TQ, q>(to, g, cu, A.odd) = TS::enc(out[q], A.tenc);
+            at_write<TQ, q>(to, g, cu, A.odd) = tstore_step<P>(out[q], A.tenc, key + uint32_t(q));
         });
     }
     if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
@@ -272,7 +306,8 @@ __global__ __launch_bounds__(256) void k_thermal_seed(ThermalArgs A, double seed
 
 // Port cells after streaming (state of parity A.odd).
 template <int P>
-__global__ __launch_bounds__(128) void k_thermal_port(PortArgs A, void* tpop_, double t_in, float tenc, float tdec) {
+__global__ __launch_bounds__(128) void k_thermal_port(PortArgs A, void* tpop_, double t_in, float tenc, float tdec,
+                                                       uint32_t step) {
     using TS = TStore<P>;
     using C = typename TS::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,9 +331,10 @@ __global__ __launch_bounds__(128) void k_thermal_port(PortArgs A, void* tpop_, d
 #pragma unroll
         for (int q = 0; q < TQ; q++) out[q] = gn[q];
     }
+    const uint32_t key = (uint32_t(A.cell[r]) * uint32_t(TQ)) ^ (step * 0x85ebca6bu);
     static_for<0, TQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        store_read<TQ, q>(tp, A.g, cx, mx, A.odd, TS::enc(out[q], tenc));
+        store_read<TQ, q>(tp, A.g, cx, mx, A.odd, tstore_step<P>(out[q], tenc, key + uint32_t(q)));
     });
 }
 

@@ -14,6 +14,9 @@
   size bit-identical to the oracle; 5000 steps in FP64, FP32 and FP16-shifted storage,
   with the relative L2 velocity error and the error of the summed dissipation of the
   reduced precisions against FP64.
+* 384 x 192 x 192 D3Q19 flow coupled to the D3Q7 temperature field, 6000 steps: the
+  FP16-storage design gradient against the FP64 one, and FP64 central differences on 16
+  seeded voxels over a 1000-step horizon.
 """
 from __future__ import annotations
 
@@ -219,3 +222,101 @@ def test_cumulant_256_storage_precisions():
           ", ".join(f"{p}: velocity L2 {e[0]:.3e}, dissipation {e[1]:.3e}" for p, e in out.items()))
     assert out["f32"][0] <= 1e-3 and out["f32"][1] <= 1e-3, out
     assert out["f16s"][0] <= 5e-2 and out["f16s"][1] <= 5e-2, out
+
+
+# --- 384 x 192 x 192 flow + temperature, 6000 steps -----------------------------------------
+# D3Q19 TRT flow (tau+ = 0.55, parabolic inlet u_x = 0.02, density-1 outlet) coupled to the
+# D3Q7 temperature field: T = 0 at the inlet, zero gradient at the outlet, source 1e-5 per
+# step and a 10x diffusivity in the solid, design of 64 random blobs.  One design density
+# rho drives both fields: the Brinkman drag alpha(rho) = 0.01 (1 - rho) / (0.1 + rho),
+# which makes rho = 1 fluid and rho = 0 solid, and the thermal solid fraction
+# phi = 1 - rho (the source, the diffusivity and the objective's solid cells phi > 1/2
+# follow the solid the drag makes).  J = mean solid T + 0.1 * pressure drop, averaged
+# over the last 500 steps; dJ/drho = alpha_bar alpha'(rho) - phi_bar.
+SHAPE4, STEPS4 = (384, 192, 192), 6000
+
+
+def _config4(precision="f64"):
+    phi = design.blob_design(SHAPE4, 64, 4, radius=(6.0, 20.0)).reshape(-1)
+    cfg = LatticeConfig(shape=SHAPE4, periodic=(False, False, False), lattice=19, collision="trt",
+                        precision=precision, omega=1.0 / 0.55, ports=_ports(SHAPE4, 0.02),
+                        checkpoint_budget_bytes=200_000_000_000, thermal=True, thermal_kappa=0.02,
+                        thermal_ratio=10.0, thermal_source=1e-5, thermal_inlet=0.0, thermal_range=0.25)
+    return cfg, phi
+
+
+def _thermal_objective(steps, window=500):
+    return Objective(kind="thermal", axis=0, plane_a=1, plane_b=SHAPE4[0] - 2, t_begin=steps - window + 1,
+                     t_end=steps, weight=1.0 / window, vector_weights=(1.0, 0.1, 0.0, 0.0, 0.0, 0.0))
+
+
+def _set_thermofluid_design(lat, rho):
+    lat.set_design(design.resistance(rho))
+    lat.set_thermal_design(1.0 - rho)
+
+
+def _thermal_gradient(cfg, rho, steps, ob, lat=None):
+    """J and dJ/drho = alpha_bar * alpha'(rho) - phi_bar (phi = 1 - rho)."""
+    own = lat is None
+    lat = Lattice(cfg) if own else lat
+    try:
+        _set_thermofluid_design(lat, rho)
+        lat.initial_populations()
+        J, ab, _, st = lat.gradient(steps, ob)
+        g = ab * design.resistance_derivative(rho) - lat.thermal_design_bar()
+    finally:
+        if own:
+            lat.close()
+    return J, g, st
+
+
+def test_thermofluid_384_gradient_f16_vs_f64():
+    ob = _thermal_objective(STEPS4)
+    cfg, phi = _config4("f64")
+    J64, g64, st = _thermal_gradient(cfg, phi, STEPS4, ob)
+    assert st.bad_cells == 0 and st.reverse_steps == STEPS4
+    cfg, _ = _config4("f16s")
+    J16, g16, st16 = _thermal_gradient(cfg, phi, STEPS4, ob)
+    err = _l2(g16, g64)
+    cos = float(np.dot(g16, g64) / np.linalg.norm(g16) / np.linalg.norm(g64))
+    top = np.argsort(-np.abs(g64))[:1000]
+    top_rel = float(np.abs(g16[top] - g64[top]).max() / np.abs(g64[top]).max())
+    print(f"\n384x192x192 flow+temperature, 6000 steps: J64 {J64:.6e}, FP16 J rel {abs(J16 - J64) / abs(J64):.3e}, "
+          f"gradient rel L2 {err:.3e}, cosine {cos:.9f}, max rel error on the top 1000 voxels {top_rel:.3e}; "
+          f"wall FP64 {st.total_ms / 1e3:.1f} s, FP16 {st16.total_ms / 1e3:.1f} s")
+    # FP16 temperature stores round stochastically; with nearest rounding the source and
+    # flux increments (~1e-3 of T per step, at FP16's 5e-4 resolution) biased J by 1.3 %
+    assert err <= 1e-3 and cos >= 0.99999 and abs(J16 - J64) <= 1e-4 * abs(J64)
+
+
+def test_thermofluid_384_central_differences_f64():
+    """16 seeded voxels, central differences of the FP64 coupled objective over a
+    1000-step horizon (the last 200 averaged) at the full lattice size.  A one-voxel change
+    of rho by h moves J ~ 4e-3 by ~1e-14 h / 1e-6, a few hundred rounding units of the
+    7-million-cell solid mean, so h = 1e-4 (truncation error ~1e-6 relative through the
+    curvature of alpha(rho)) resolves the derivative where h = 1e-6 leaves ~3e-3 noise."""
+    steps = 1000
+    ob = _thermal_objective(steps, window=200)
+    cfg, phi = _config4("f64")
+    with Lattice(cfg) as lat:
+        J, g, _ = _thermal_gradient(cfg, phi, steps, ob, lat)
+        cand = np.flatnonzero(np.abs(g) >= 1e-2 * np.abs(g).max())
+        voxels = np.random.default_rng(16).choice(cand, 16, replace=False)
+        h = 1e-4
+        fds, rels = [], []
+        for v in voxels:
+            vals = []
+            for sgn in (1.0, -1.0):
+                p2 = phi.copy()
+                p2[v] += sgn * h
+                _set_thermofluid_design(lat, p2)
+                lat.initial_populations()
+                vals.append(lat.history(steps, ob)[0])
+            fd = (vals[0] - vals[1]) / (2.0 * h)
+            scale = max(abs(fd), abs(g[v]))
+            fds.append(fd)
+            rels.append(abs(fd / scale - g[v] / scale))
+    l2 = _l2(np.array(fds), g[voxels])
+    print(f"\n384x192x192 flow+temperature, 1000 steps: J {J:.6e}, central differences on 16 voxels: worst rel_err "
+          f"{max(rels):.2e}, median {np.median(rels):.2e}, relative L2 {l2:.2e}")
+    assert max(rels) <= 1e-3 and l2 <= 2e-4, (max(rels), l2)
