@@ -128,6 +128,80 @@ def cpu_baseline(shape, window, alpha_cfg, threads):
     return o.n * window / dt / 1e6, dt
 
 
+# Algorithmic bytes per lattice update of the thermal kernels (D3Q19 flow, D3Q7
+# temperature): forward reads the 19 flow populations (for u), alpha and phi, and reads
+# and writes the 7 temperature populations; adjoint reads the flow and temperature
+# state, alpha and phi, reads and writes the temperature cotangent, writes the velocity
+# cotangent (3 values, read back by the flow adjoint) and updates the FP64 phi_bar.
+def thermal_bytes(which, precision):
+    s, a = {"f64": (8, 8), "f32": (4, 4), "f16s": (2, 4)}[precision]
+    l = 8 if precision == "f64" else 4
+    if which == "forward":
+        return 19 * s + 2 * a + 14 * s
+    return 19 * s + 7 * s + 2 * a + 14 * l + 3 * a + 16
+
+
+def thermofluid(args):
+    """The 384 x 192 x 192 flow + temperature design gradient (6000 steps by default):
+    D3Q19 TRT flow (tau+ = 0.55, inlet u_x = 0.02, density-1 outlet) coupled to the D3Q7
+    temperature (inlet T = 0, zero-gradient outlet, source 1e-5 per step and 10x
+    diffusivity in the solid phi = 1 - rho), J = mean solid T + 0.1 pressure drop over the
+    last 500 steps.  Prints one JSON line (not the headline metric)."""
+    from implexity_amd import design
+    from implexity_amd.lbm import Lattice, LatticeConfig, Objective, Port
+
+    shape = (384, 192, 192) if tuple(args.shape) == (256, 256, 256) else tuple(args.shape)
+    steps = args.steps if args.steps != 8000 else 6000
+    n = int(np.prod(shape))
+    rho = design.blob_design(shape, 64, 4, radius=(6.0, 20.0)).reshape(-1)
+    prof = design.parabolic_profile(shape[1], shape[2]).reshape(-1)
+    ports = [Port("xmin", "velocity", (0.02, 0.0, 0.0), prof), Port("xmax", "pressure", (1.0,))]
+    window = min(500, steps)
+    ob = Objective(kind="thermal", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=steps - window + 1,
+                   t_end=steps, weight=1.0 / window, vector_weights=(1.0, 0.1, 0.0, 0.0, 0.0, 0.0))
+    res = {}
+    for prec in args.precisions.split(","):
+        cfg = LatticeConfig(shape=shape, periodic=(False, False, False), lattice=19, collision="trt",
+                            precision=prec, omega=1.0 / 0.55, ports=ports,
+                            checkpoint_budget_bytes=int(args.budget_gb * 1e9), thermal=True, thermal_kappa=0.02,
+                            thermal_ratio=10.0, thermal_source=1e-5, thermal_inlet=0.0, thermal_range=0.25)
+        with Lattice(cfg) as lat:
+            lat.set_design(design.resistance(rho))
+            lat.set_thermal_design(1.0 - rho)
+            lat.reserve(steps)
+            lat.initial_populations()
+            lat.gradient(min(steps, 20), Objective(kind="thermal", axis=0, plane_a=1, plane_b=shape[0] - 2,
+                                                   t_begin=1, t_end=min(steps, 20), weight=1.0,
+                                                   vector_weights=(1.0, 0.1, 0, 0, 0, 0)))
+            lat.initial_populations()
+            t0 = time.perf_counter()
+            J, ab, _, st = lat.gradient(steps, ob)
+            wall = time.perf_counter() - t0
+            g = ab * design.resistance_derivative(rho) - lat.thermal_design_bar()
+            lat.initial_populations()
+            tf = lat.time_kernel("thermal_forward", args.launches)
+            ta = lat.time_kernel("thermal_adjoint", args.launches)
+        res[prec] = {"J": J, "wall_s": wall, "mlups_fwd_adj": n * steps / wall / 1e6, "grad": g,
+                     "flow_forward_kernel_ms": st.forward_kernel_avg_ms,
+                     "flow_adjoint_kernel_ms": st.adjoint_kernel_avg_ms, "forward_steps": st.forward_steps,
+                     "snapshot_slots": st.snapshot_slots, "thermal_forward_ms": tf, "thermal_adjoint_ms": ta,
+                     "thermal_forward_GBs": thermal_bytes("forward", prec) * n / (tf * 1e-3) / 1e9,
+                     "thermal_adjoint_GBs": thermal_bytes("adjoint", prec) * n / (ta * 1e-3) / 1e9}
+        log(f"[thermofluid] {prec}: wall {wall:.2f} s, J {J:.6e}, thermal kernels {tf:.3f} / {ta:.3f} ms")
+    out = {"workload": f"D3Q19 TRT + D3Q7 temperature {shape[0]}x{shape[1]}x{shape[2]}, {steps} steps",
+           "per_precision": {p: {k: v for k, v in r.items() if k != "grad"} for p, r in res.items()}}
+    if "f64" in res:
+        g64 = res["f64"]["grad"]
+        out["error_vs_f64"] = {
+            p: {"grad_rel_l2": float(np.linalg.norm(r["grad"] - g64) / np.linalg.norm(g64)),
+                "grad_cosine": float(np.dot(r["grad"], g64) / np.linalg.norm(r["grad"]) / np.linalg.norm(g64)),
+                "J_rel": abs(r["J"] - res["f64"]["J"]) / abs(res["f64"]["J"])}
+            for p, r in res.items() if p != "f64"}
+    if "f32" in res and "f16s" in res:
+        out["speedup_f16s_vs_f32"] = res["f32"]["wall_s"] / res["f16s"]["wall_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,7 +212,7 @@ def main():
     ap.add_argument("--precisions", default="f64,f32,f16s")
     ap.add_argument("--cpu-window", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", default="gradient", choices=["gradient", "kernels"])
+    ap.add_argument("--mode", default="gradient", choices=["gradient", "kernels", "thermofluid"])
     ap.add_argument("--kernel-precision", default="f32")
     ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
@@ -155,6 +229,10 @@ def main():
         if "alpha" not in cache:
             cache["alpha"] = build_problem(shape, precision, budget)[1]
         return _cfg_only(shape, precision, budget), cache["alpha"]
+
+    if args.mode == "thermofluid":
+        thermofluid(args)
+        return
 
     if args.mode == "kernels":
         # Per-kernel timing only (used under rocprofv3): forward and adjoint launches.

@@ -215,7 +215,7 @@ struct lbm_handle {
     float tenc = 1.0f, tdec = 1.0f;  // FP16 storage scale of the temperature populations
     size_t flow_bytes = 0, flow_lam_bytes = 0;
     long long n_tsolid = 0;  // cells with phi > 1/2
-    DevBuf phi, phi_bar;
+    DevBuf phi, phi_bar, ubar;  // ubar: velocity cotangent handed from the thermal to the flow adjoint
     const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
@@ -261,7 +261,7 @@ struct lbm_handle {
         t.tpop = tpop(src);
         t.tpop_out = dst ? tpop(dst) : nullptr;
         t.tlam = L.p ? tlam() : nullptr;
-        t.lam = L.p;
+        t.ubar = ubar.p;
         t.alpha = alpha.p;
         t.phi = phi.p;
         t.alpha_bar = alpha_bar.as<double>();
@@ -472,7 +472,10 @@ void ensure_adjoint_buffers(lbm_handle* h) {
     h->L.alloc(h->lam_bytes, "cotangent state");
     h->A.alloc(h->state_bytes, "ahead state");
     h->alpha_bar.alloc(size_t(h->ncell) * sizeof(double), "alpha_bar");
-    if (h->thermal) h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
+    if (h->thermal) {
+        h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
+        h->ubar.alloc(size_t(h->ncell) * 3 * h->csize(), "velocity cotangent");
+    }
 }
 
 // Snapshot slots for a horizon: min(budget / state bytes, nsteps); the pool only grows.
@@ -1546,17 +1549,18 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                 aa.pop = cur;
                 aa.odd = int((base + k - 1) & 1);
                 aa.obj = stage_obj(tmpl, obj, k - 1, nsteps, false);
+                if (h->thermal) {  // lambda_g and the velocity cotangent the flow kernel adds
+                    ThermalArgs ta = h->thermal_args(cur, nullptr, base + k - 1);
+                    ta.obj_w = aa.obj.tw;
+                    hip_check(h->trun->adjoint(Q, ta, h->cfg()), "thermal adjoint launch");
+                    aa.ubar = h->ubar.p;
+                }
                 aa.s_next = k >= 2 ? seed_bound(h, stage_obj(tmpl, obj, k - 2, nsteps, false)) : 0.0;
                 {
                     const auto* ks = kspan(adj_k, rsteps);
                     if (ks) hip_check(hipEventRecord(ks->first, h->stream), "event record");
                     hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg(), ks ? ks->second : nullptr),
                               "adjoint launch");
-                }
-                if (h->thermal) {  // after the flow kernel: adds the coupling to its cotangent
-                    ThermalArgs ta = h->thermal_args(cur, nullptr, base + k - 1);
-                    ta.obj_w = aa.obj.tw;
-                    hip_check(h->trun->adjoint(Q, ta, h->cfg()), "thermal adjoint launch");
                 }
                 rsteps++;
                 if (h->np && k > 1)

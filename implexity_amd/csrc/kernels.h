@@ -116,6 +116,9 @@ struct AdjArgs {
     unsigned* ticket;
     int* lsc;
     double s_next;
+    // Coupled temperature field: cotangent of the Brinkman velocity from the thermal
+    // step (SoA, 3 x cells, arithmetic precision), added through u(f, alpha); else null.
+    const void* ubar;
 };
 
 // ---------------------------------------------------------------------------------
@@ -910,7 +913,31 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
 // ---------------------------------------------------------------------------------
 // Adjoint step, in place on the cotangent buffer.
 // ---------------------------------------------------------------------------------
-template <int P, int LP, int Q, int KIND, bool MASK>
+// u = (j / rho + a / 2) / (1 + alpha / 2): adds (du/df)^T ub to fb and (du/dalpha) . ub
+// to ab, with d u_d / d f_q = (c_qd - j_d / rho) / (rho den) and d u / d alpha = -u / (2 den).
+template <int Q, class C>
+LBM_DEV void brinkman_u_vjp(const Macro<Q, C>& m, const C ub[3], C* fb, C& ab) {
+    C v[3], s[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v[d] = m.j[d] / m.rho;
+        s[d] = ub[d] / m.den / m.rho;
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        C acc = C(0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc += s[d] * (C(cvec<Q>(q, d)) - v[d]);
+        fb[q] += acc;
+    });
+    C au = C(0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) au += ub[d] * m.u[d];
+    ab += C(-0.5) * au / m.den;
+}
+
+// UB: add the velocity cotangent A.ubar of the coupled temperature field.
+template <int P, int LP, int Q, int KIND, bool MASK, bool UB = false>
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     using S = Store<P>;
     using L = LamStore<LP>;
@@ -942,6 +969,12 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
             if (A.obj.kind != OBJ_NONE) {
                 const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
                 objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+            }
+            if constexpr (UB) {
+                const C* U = static_cast<const C*>(A.ubar);
+                const long long n = (long long)g.nx * g.ny * g.nz;
+                const C ub[3] = {U[cell], U[n + cell], U[2 * n + cell]};
+                brinkman_u_vjp<Q, C>(mac, ub, fb, ab);
             }
             A.alpha_bar[cell] += double(ab);
             static_for<0, Q>([&](auto I) {

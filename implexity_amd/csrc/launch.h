@@ -137,8 +137,28 @@ struct RunnerImpl final : Runner {
         }
         return hipErrorInvalidValue;
     }
+    // Flow adjoint with the velocity cotangent of the coupled temperature field (no solid
+    // mask, FP32/FP64 cotangents, per-cell kernels).
+    template <int Q>
+    static hipError_t adj_ub(int kind, const AdjArgs& a, const LaunchCfg& c) {
+        if constexpr (LP != P_F16S) {
+            switch (kind) {
+            case K_TRT: LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT, false, true>), c, a);
+            case K_BGK: LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK, false, true>), c, a);
+            default: break;
+            }
+            if constexpr (ALL_KINDS) {
+                if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, false, true>), c, a);
+                if constexpr (Q == 27) {
+                    if (kind == K_CUMULANT) LBM_LAUNCH((k_adjoint<P, LP, 27, K_CUMULANT, false, true>), c, a);
+                }
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     template <int Q, bool M>
     static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c0) {
+        if (a.ubar) return M ? hipErrorInvalidValue : adj_ub<Q>(kind, a, c0);
         if constexpr (LP != P_F16S) {  // scaled FP16 cotangents: per-cell kernel only
             const int V = adj_width();
             if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
@@ -401,7 +421,7 @@ struct ThermalRunnerImpl final : ThermalRunner {
         return done(a, c);
     }
     hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const override {
-        if (!a.tlam || !a.lam || !a.alpha_bar || !a.phi_bar) return hipErrorInvalidValue;
+        if (!a.tlam || !a.ubar || !a.phi_bar) return hipErrorInvalidValue;
         if (Q == 19) LBM_LAUNCH((k_thermal_adj<P, LP, 19>), c, a);
         if (Q == 27) LBM_LAUNCH((k_thermal_adj<P, LP, 27>), c, a);
         return hipErrorInvalidValue;

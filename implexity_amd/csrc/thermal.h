@@ -85,7 +85,7 @@ struct ThermalArgs {
     const void* tpop;   // temperature populations read
     void* tpop_out;     // temperature populations written (== tpop in place)
     void* tlam;         // cotangent of the temperature populations (adjoint)
-    void* lam;          // cotangent of the flow populations (adjoint adds the coupling)
+    void* ubar;         // adjoint: cotangent of the flow's Brinkman velocity (SoA 3 x cells)
     const void* alpha;  // flow design (arithmetic precision)
     const void* phi;    // thermal design (arithmetic precision)
     double* alpha_bar;
@@ -202,10 +202,10 @@ __global__ __launch_bounds__(256) void k_thermal_fwd(ThermalArgs A) {
     if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
 }
 
-// Adjoint, after the flow adjoint kernel of the same reverse step: lambda_g in place
-// (write locations -> read locations), the velocity coupling added to the flow
-// cotangent the flow kernel has just written at the cell's read locations, and the
-// design cotangents.
+// Adjoint, before the flow adjoint kernel of the same reverse step: lambda_g in place
+// (write locations -> read locations), the design cotangent phi_bar, and the cotangent
+// of the flow velocity the step advected with, which the flow adjoint kernel adds
+// through u(f, alpha) (brinkman_u_vjp).
 template <int P, int LP, int Q>
 __global__ __launch_bounds__(256) void k_thermal_adj(ThermalArgs A) {
     using S = Store<P>;
@@ -243,26 +243,12 @@ __global__ __launch_bounds__(256) void k_thermal_adj(ThermalArgs A) {
         constexpr int q = decltype(I)::value;
         store_read<TQ, q>(tl, g, cu, mt, A.odd, L::template enc<C>(g_bar[q]));
     });
-    // u = (j / rho + a / 2) / (1 + alpha / 2):  d u_d / d f_q = (c_qd - j_d / rho) / (rho den)
-    C v[3], s[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        v[d] = mac.j[d] / mac.rho;
-        s[d] = ub[d] / mac.den / mac.rho;
-    }
-    typename L::T* lf = static_cast<typename L::T*>(A.lam);
-    static_for<0, Q>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        C acc = C(0);
-#pragma unroll
-        for (int d = 0; d < 3; d++) acc += s[d] * (C(cvec<Q>(q, d)) - v[d]);
-        typename L::T& slot = at_read<Q, q>(lf, g, cu, m, A.odd);
-        slot = L::template enc<C>(L::template ld<C>(&slot) + acc);
-    });
-    C au = C(0);
-#pragma unroll
-    for (int d = 0; d < 3; d++) au += ub[d] * mac.u[d];
-    A.alpha_bar[cell] += double(C(-0.5) * au / mac.den);
+    // the velocity cotangent goes to the flow adjoint kernel of the same reverse step
+    C* U = static_cast<C*>(A.ubar);
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    U[cell] = ub[0];
+    U[n + cell] = ub[1];
+    U[2 * n + cell] = ub[2];
     const C dom = -om * om * C(4) * C(A.kappa) * C(A.ratio_m1);
     A.phi_bar[cell] += double(ob * dom + sb * C(A.source));
 }
