@@ -238,6 +238,7 @@ struct lbm_handle {
         for (int d = 0; d < 3; d++) s.a[d] = a[d];
         s.mrt = mrt.p;
         s.odd = int(step & 1);
+        s.step = uint32_t(step);
         s.diag = 0;
         s.obj.kind = OBJ_NONE;
         s.part = nullptr;

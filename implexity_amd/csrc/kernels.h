@@ -39,6 +39,10 @@
 #include "cell.h"
 #include "cumulant.h"
 
+#ifndef LBM_FLOW_SR
+#define LBM_FLOW_SR 0
+#endif
+
 namespace lbm {
 
 enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
@@ -85,6 +89,7 @@ struct StepArgs {
     const void* mrt;
     int odd;
     int diag;         // 1: accumulate diagnostics
+    uint32_t step;    // index of the interval (rounding key of stochastically rounded stores)
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
     long long npart;  // capacity of part, in blocks
@@ -120,6 +125,28 @@ struct AdjArgs {
     // step (SoA, 3 x cells, arithmetic precision), added through u(f, alpha); else null.
     const void* ubar;
 };
+
+// 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
+LBM_DEV uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// FP16 with stochastic rounding: the 13 float significand bits below the FP16 ones are
+// replaced by a carry from r's low 13 bits, so a value rounds up with probability equal
+// to its distance from the lower FP16 neighbour (E[result] = x).  A field that moves by
+// small increments per step next to an 11-bit stored value turns nearest rounding into a
+// bias that accumulates over thousands of steps; stochastic rounding makes it zero-mean
+// noise.  Magnitudes below the FP16 normal range round to nearest.
+LBM_DEV _Float16 f16_stochastic(float x, uint32_t r) {
+    const uint32_t b = __float_as_uint(x);
+    if ((b & 0x7fffffffu) < 0x38800000u) return _Float16(x);
+    return _Float16(__uint_as_float((b + (r & 0x1fffu)) & ~0x1fffu));
+}
 
 // ---------------------------------------------------------------------------------
 // Storage codecs.  C is the arithmetic type; T the stored type.
@@ -901,6 +928,17 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                 }
             }
             typename S::T* out = static_cast<typename S::T*>(A.pop_out);
+#if LBM_FLOW_SR
+            if constexpr (P == P_F16S) {
+                const uint32_t hsh = mix32(uint32_t(cell) ^ (A.step * 0x9e3779b9u));
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    constexpr float sc = float(1u << fp16_exp<Q>(q));
+                    const uint32_t r = (hsh * (0x9e3779b1u + 0x3c6ef372u * uint32_t(q))) >> 19;
+                    at_write<Q, q>(out, g, cu, A.odd) = f16_stochastic(fs[q] * sc, r);
+                });
+            } else
+#endif
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 at_write<Q, q>(out, g, cu, A.odd) = S::template enc<Q, q>(fs[q]);

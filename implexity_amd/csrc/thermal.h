@@ -47,28 +47,6 @@ struct TStore<P_F16S> {
     static LBM_DEV T enc(C v, float s) { return _Float16(v * s); }
 };
 
-// 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
-LBM_DEV uint32_t mix32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x7feb352du;
-    x ^= x >> 15;
-    x *= 0x846ca68bu;
-    x ^= x >> 16;
-    return x;
-}
-
-// FP16 with stochastic rounding: the 13 float significand bits below the FP16 ones are
-// replaced by a carry from r's low 13 bits, so a value rounds up with probability equal
-// to its distance from the lower FP16 neighbour (E[result] = x).  The temperature moves
-// by small increments per step (sources, fluxes) next to an 11-bit stored value; nearest
-// rounding turns that into a bias that accumulates over thousands of steps, stochastic
-// rounding into zero-mean noise.  Magnitudes below the FP16 normal range round to nearest.
-LBM_DEV _Float16 f16_stochastic(float x, uint32_t r) {
-    const uint32_t b = __float_as_uint(x);
-    if ((b & 0x7fffffffu) < 0x38800000u) return _Float16(x);
-    return _Float16(__uint_as_float((b + (r & 0x1fffu)) & ~0x1fffu));
-}
-
 // Store of a temperature population written by a step: stochastic rounding for FP16
 // (bits from the cell, the direction and the step, so recomputed steps reproduce it).
 template <int P>
