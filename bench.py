@@ -128,16 +128,17 @@ def cpu_baseline(shape, window, alpha_cfg, threads):
     return o.n * window / dt / 1e6, dt
 
 
-# Algorithmic bytes per lattice update of the thermal kernels (D3Q19 flow, D3Q7
-# temperature): forward reads the 19 flow populations (for u), alpha and phi, and reads
-# and writes the 7 temperature populations; adjoint reads the flow and temperature
-# state, alpha and phi, reads and writes the temperature cotangent, writes the velocity
-# cotangent (3 values, read back by the flow adjoint) and updates the FP64 phi_bar.
+# Algorithmic bytes per lattice update of the coupled kernels (D3Q19 flow, D3Q7
+# temperature): the forward pass reads and writes the 19 flow and 7 temperature
+# populations and reads alpha and phi; the thermal adjoint reads the flow and
+# temperature state, alpha and phi, reads and writes the temperature cotangent, writes
+# the velocity cotangent (3 values, read back by the flow adjoint) and updates the FP64
+# phi_bar.
 def thermal_bytes(which, precision):
     s, a = {"f64": (8, 8), "f32": (4, 4), "f16s": (2, 4)}[precision]
     l = 8 if precision == "f64" else 4
     if which == "forward":
-        return 19 * s + 2 * a + 14 * s
+        return 38 * s + 14 * s + 2 * a
     return 19 * s + 7 * s + 2 * a + 14 * l + 3 * a + 16
 
 
@@ -179,15 +180,16 @@ def thermofluid(args):
             wall = time.perf_counter() - t0
             g = ab * design.resistance_derivative(rho) - lat.thermal_design_bar()
             lat.initial_populations()
-            tf = lat.time_kernel("thermal_forward", args.launches)
+            tf = lat.time_kernel("coupled_forward", args.launches)
             ta = lat.time_kernel("thermal_adjoint", args.launches)
         res[prec] = {"J": J, "wall_s": wall, "mlups_fwd_adj": n * steps / wall / 1e6, "grad": g,
                      "flow_forward_kernel_ms": st.forward_kernel_avg_ms,
                      "flow_adjoint_kernel_ms": st.adjoint_kernel_avg_ms, "forward_steps": st.forward_steps,
-                     "snapshot_slots": st.snapshot_slots, "thermal_forward_ms": tf, "thermal_adjoint_ms": ta,
-                     "thermal_forward_GBs": thermal_bytes("forward", prec) * n / (tf * 1e-3) / 1e9,
+                     "snapshot_slots": st.snapshot_slots, "coupled_forward_ms": tf, "thermal_adjoint_ms": ta,
+                     "coupled_forward_GBs": thermal_bytes("forward", prec) * n / (tf * 1e-3) / 1e9,
                      "thermal_adjoint_GBs": thermal_bytes("adjoint", prec) * n / (ta * 1e-3) / 1e9}
-        log(f"[thermofluid] {prec}: wall {wall:.2f} s, J {J:.6e}, thermal kernels {tf:.3f} / {ta:.3f} ms")
+        log(f"[thermofluid] {prec}: wall {wall:.2f} s, J {J:.6e}, coupled forward {tf:.3f} ms, thermal adjoint "
+            f"{ta:.3f} ms")
     out = {"workload": f"D3Q19 TRT + D3Q7 temperature {shape[0]}x{shape[1]}x{shape[2]}, {steps} steps",
            "per_precision": {p: {k: v for k, v in r.items() if k != "grad"} for p, r in res.items()}}
     if "f64" in res:

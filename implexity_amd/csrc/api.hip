@@ -239,6 +239,16 @@ struct lbm_handle {
         s.mrt = mrt.p;
         s.odd = int(step & 1);
         s.step = uint32_t(step);
+        if (thermal) {  // the flow kernel advances the temperature field in the same pass
+            s.tpop = tpop(static_cast<const void*>(pop));
+            s.tpop_out = tpop(pop);
+            s.phi = phi.p;
+            s.kappa = tkappa;
+            s.ratio_m1 = tratio - 1.0;
+            s.source = tsource;
+            s.tenc = tenc;
+            s.tdec = tdec;
+        }
         s.diag = 0;
         s.obj.kind = OBJ_NONE;
         s.part = nullptr;
@@ -535,14 +545,12 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
     s.pop_out = dst;
     s.obj = obj;
     s.diag = diag ? 1 : 0;
-    if (obj.kind != OBJ_NONE || diag) s.part = h->part.as<double>();
-    if (diag) s.macc = led->mass(s_index);
-    if (h->thermal) {  // before the flow kernel, which overwrites the flow state in place
-        ThermalArgs ta = h->thermal_args(src, dst, step);
-        ta.obj_w = obj.tw;
-        if (obj.tw != 0.0) ta.part = h->part.as<double>();
-        hip_check(h->trun->forward(h->Q, ta, h->cfg()), "thermal step launch");
+    if (h->thermal) {
+        s.tpop_out = h->tpop(dst);
+        s.tw = obj.tw;
     }
+    if (obj.kind != OBJ_NONE || obj.tw != 0.0 || diag) s.part = h->part.as<double>();
+    if (diag) s.macc = led->mass(s_index);
     if (kspan) hip_check(hipEventRecord(kspan->first, h->stream), "event record");
     hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg(), kspan ? kspan->second : nullptr),
               "forward step launch");
@@ -1276,6 +1284,7 @@ int lbm_debug_link_table(lbm_handle* h, uint32_t* source, uint64_t len) {
         hip_check(hipMemsetAsync(zero.p, 0, zero.bytes, h->stream), "memset");
         s.alpha = zero.p;
         s.mrt = nullptr;
+        s.tpop = nullptr;  // the flow populations only
         hip_check(r64->forward(Q, K_STREAM_ONLY, h->has_mask, s, h->cfg()), "stream-only launch");
         hip_check(r64->get_state(Q, h->has_mask, h->g, buf.p, h->mask.as<uint32_t>(), 1, h->aos.as<double>(), h->cfg()),
                   "get_state");
@@ -1653,10 +1662,10 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
         if (!h || !avg_ms) fail(LBM_ERR_CONTRACT, "null argument");
         if (launches == 0) fail(LBM_ERR_CONTRACT, "launches must be positive");
         ObjArgs none{};
-        if (which == 3 || which == 4) {
-            // thermal kernels alone, in place (the state is not meaningful afterwards)
+        if (which == 4) {
+            // thermal adjoint kernel alone, in place (the state is not meaningful afterwards)
             if (!h->thermal) fail(LBM_ERR_CONTRACT, "no temperature field on this handle");
-            if (which == 4) {
+            {
                 ensure_adjoint_buffers(h);
                 hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
                 hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
@@ -1664,26 +1673,32 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             }
             auto launch = [&](uint64_t s) {
                 ThermalArgs ta = h->thermal_args(h->F.p, h->F.p, h->t + s);
-                hip_check(which == 3 ? h->trun->forward(h->Q, ta, h->cfg()) : h->trun->adjoint(h->Q, ta, h->cfg()),
-                          "thermal launch");
+                hip_check(h->trun->adjoint(h->Q, ta, h->cfg()), "thermal launch");
             };
             launch(0);  // untimed
             hip_check(hipEventRecord(h->ev0, h->stream), "event");
             for (uint64_t s = 0; s < launches; s++) launch(s + 1);
             hip_check(hipEventRecord(h->ev1, h->stream), "event");
-        } else if (which == 0 || which == 2) {
+        } else if (which == 0 || which == 2 || which == 3) {
             // forward steps in place on the live state (no ports: the kernel alone); which
             // 2 runs the same addressing with the collision replaced by a copy (the
-            // streaming-only memory ceiling of the layout)
-            const int kind = which == 0 ? h->kind : int(K_STREAM_ONLY);
+            // streaming-only memory ceiling of the layout); which 3 the flow kernel that
+            // also advances the temperature field (0 and 2 leave it out)
+            if (which == 3 && !h->thermal) fail(LBM_ERR_CONTRACT, "no temperature field on this handle");
+            const int kind = which == 2 ? int(K_STREAM_ONLY) : h->kind;
+            auto args = [&](uint64_t t) {
+                StepArgs sa = h->step_args(h->F.p, t);
+                if (which != 3) sa.tpop = nullptr;
+                return sa;
+            };
             {  // untimed first launch (code object load)
-                StepArgs sa = h->step_args(h->F.p, h->t);
+                StepArgs sa = args(h->t);
                 hip_check(h->run->forward(h->Q, kind, h->has_mask, sa, h->cfg()), "forward launch");
                 h->t++;
             }
             hip_check(hipEventRecord(h->ev0, h->stream), "event");
             for (uint64_t s = 0; s < launches; s++) {
-                StepArgs sa = h->step_args(h->F.p, h->t);
+                StepArgs sa = args(h->t);
                 hip_check(h->run->forward(h->Q, kind, h->has_mask, sa, h->cfg()), "forward launch");
                 h->t++;
             }

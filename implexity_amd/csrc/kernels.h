@@ -90,6 +90,14 @@ struct StepArgs {
     int odd;
     int diag;         // 1: accumulate diagnostics
     uint32_t step;    // index of the interval (rounding key of stochastically rounded stores)
+    // Coupled temperature field (k_forward with TH): temperature populations read and
+    // written in the same pass, the thermal design, its constants, the FP16 storage scale
+    // and the stage weight on T per cell with phi > 1/2.
+    const void* tpop;
+    void* tpop_out;
+    const void* phi;
+    double kappa, ratio_m1, source, tw;
+    float tenc, tdec;
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
     long long npart;  // capacity of part, in blocks
@@ -526,6 +534,8 @@ LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cur
     });
 }
 
+#include "thermal_cell.h"
+
 // ---------------------------------------------------------------------------------
 // Reductions.
 // ---------------------------------------------------------------------------------
@@ -890,7 +900,9 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
 // ---------------------------------------------------------------------------------
 // Forward step: one thread per cell.
 // ---------------------------------------------------------------------------------
-template <int P, int Q, int KIND, bool MASK>
+// TH: also advance the coupled temperature field of the cell (thermal_cell.h) with the
+// Brinkman velocity the flow collision computes.
+template <int P, int Q, int KIND, bool MASK, bool TH = false>
 __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
     using S = Store<P>;
     using C = typename S::C;
@@ -926,6 +938,21 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                     const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
                     bad |= !(fq > C(0)) || !isfinite(float(fq));
                 }
+            }
+            if constexpr (TH) {
+                using TS = TStore<P>;
+                const uint32_t mt = face_mask<TQ>(g, i, j, k);
+                C gv[TQ], go[TQ], T;
+                load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
+                const C phi = static_cast<const C*>(A.phi)[cell];
+                thermal_collide<C>(gv, mac.u, thermal_omega<C>(A.kappa, A.ratio_m1, phi), C(A.source) * phi, go, T);
+                if (A.tw != 0.0 && phi > C(0.5)) jsum += A.tw * double(T);
+                typename TS::T* to = static_cast<typename TS::T*>(A.tpop_out);
+                const uint32_t key = (uint32_t(cell) * uint32_t(TQ)) ^ (A.step * 0x9e3779b9u);
+                static_for<0, TQ>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    at_write<TQ, q>(to, g, cu, A.odd) = tstore_step<P>(go[q], A.tenc, key + uint32_t(q));
+                });
             }
             typename S::T* out = static_cast<typename S::T*>(A.pop_out);
 #if LBM_FLOW_SR

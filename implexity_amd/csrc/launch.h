@@ -108,8 +108,32 @@ struct RunnerImpl final : Runner {
         default: return hipErrorInvalidValue;
         }
     }
+    // Flow + coupled temperature field in one pass (no solid mask, per-cell kernels,
+    // storages the temperature field supports).
+    template <int Q>
+    static hipError_t fwd_th(int kind, const StepArgs& a, const LaunchCfg& c) {
+        if constexpr (LP != P_F16S && P != P_BF16S) {
+            if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
+            switch (kind) {
+            case K_TRT: LBM_LAUNCH((k_forward<P, Q, K_TRT, false, true>), c, a);
+            case K_BGK: LBM_LAUNCH((k_forward<P, Q, K_BGK, false, true>), c, a);
+            default: break;
+            }
+            if constexpr (ALL_KINDS) {
+                if (kind == K_MRT) LBM_LAUNCH((k_forward<P, Q, K_MRT, false, true>), c, a);
+                if constexpr (Q == 27) {
+                    if (kind == K_CUMULANT) LBM_LAUNCH((k_forward<P, 27, K_CUMULANT, false, true>), c, a);
+                }
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     template <int Q, bool M>
     static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c0, LaunchCfg& used) {
+        if (a.tpop) {
+            used = c0;
+            return M ? hipErrorInvalidValue : fwd_th<Q>(kind, a, c0);
+        }
         const int V = fwd_width();
         if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
             const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
@@ -188,6 +212,8 @@ struct RunnerImpl final : Runner {
     // configuration must be set.
     static bool ok_step(int kind, bool mask, const StepArgs& a) {
         if (!a.pop || !a.pop_out || !a.alpha) return false;
+        if (a.tpop && (!a.tpop_out || !a.phi)) return false;
+        if (a.tw != 0.0 && (!a.part || !a.jacc)) return false;
         if (mask && !a.mask) return false;
         if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
         if ((a.obj.kind != OBJ_NONE || a.diag) && (!a.part || !a.jacc)) return false;
@@ -379,8 +405,6 @@ struct RunnerImpl final : Runner {
 // Launchers of the coupled temperature kernels (thermal.h) for D3Q19 and D3Q27 flows.
 struct ThermalRunner {
     virtual ~ThermalRunner() = default;
-    // One thermal interval; with a.part, the stage cost is folded into a.jacc.
-    virtual hipError_t forward(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
     virtual hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const = 0;
     virtual hipError_t seed(const ThermalArgs& a, double seed_w, int do_seed, const LaunchCfg& c) const = 0;
     virtual hipError_t ports(const PortArgs& p, void* tpop, double t_in, float tenc, float tdec, uint32_t step,
@@ -410,15 +434,6 @@ struct ThermalRunnerImpl final : ThermalRunner {
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess || !a.part) return e;
         return fold(a, c);
-    }
-    hipError_t forward(int Q, const ThermalArgs& a, const LaunchCfg& c) const override {
-        if (Q == 19)
-            hipLaunchKernelGGL((k_thermal_fwd<P, 19>), c.grid, c.block, 0, c.stream, a);
-        else if (Q == 27)
-            hipLaunchKernelGGL((k_thermal_fwd<P, 27>), c.grid, c.block, 0, c.stream, a);
-        else
-            return hipErrorInvalidValue;
-        return done(a, c);
     }
     hipError_t adjoint(int Q, const ThermalArgs& a, const LaunchCfg& c) const override {
         if (!a.tlam || !a.ubar || !a.phi_bar) return hipErrorInvalidValue;

@@ -11,51 +11,16 @@
 // w T_nb), pressure ports are zero-gradient (g_b = g_nb).
 //
 // The temperature populations sit in the same state buffer as the flow, after its Q
-// arrays, so snapshots and restores move both fields together.  One interval runs the
-// thermal kernel first (it reads the flow state before the flow kernel overwrites it in
-// place), then the flow kernel, the flow ports and the thermal ports.  Hard solid cells
-// are not part of the model (the host refuses them), so every link mask is the face
-// mask.
+// arrays, so snapshots and restores move both fields together.  The flow's forward
+// kernel advances both fields of a cell in one pass (k_forward with TH, using the
+// velocity its collision computes; thermal_cell.h); the ports of both fields follow.
+// A reverse step runs k_thermal_adj, then the flow adjoint kernel, which adds the
+// velocity cotangent k_thermal_adj hands over.  Hard solid cells are not part of the
+// model (the host refuses them), so every link mask is the face mask.
 #pragma once
 #include "kernels.h"
 
 namespace lbm {
-
-// Temperature population storage: FP64 / FP32 as is; FP16 times a power of two chosen
-// on the host from the expected temperature range (tenc = 2^e, tdec = 2^-e).
-template <int P>
-struct TStore;
-template <>
-struct TStore<P_F64> {
-    using T = double;
-    using C = double;
-    static LBM_DEV C dec(T v, float) { return v; }
-    static LBM_DEV T enc(C v, float) { return v; }
-};
-template <>
-struct TStore<P_F32> {
-    using T = float;
-    using C = float;
-    static LBM_DEV C dec(T v, float) { return v; }
-    static LBM_DEV T enc(C v, float) { return v; }
-};
-template <>
-struct TStore<P_F16S> {
-    using T = _Float16;
-    using C = float;
-    static LBM_DEV C dec(T v, float s) { return float(v) * s; }
-    static LBM_DEV T enc(C v, float s) { return _Float16(v * s); }
-};
-
-// Store of a temperature population written by a step: stochastic rounding for FP16
-// (bits from the cell, the direction and the step, so recomputed steps reproduce it).
-template <int P>
-LBM_DEV typename TStore<P>::T tstore_step(typename TStore<P>::C v, float s, uint32_t key) {
-    if constexpr (P == P_F16S)
-        return f16_stochastic(v * s, mix32(key));
-    else
-        return TStore<P>::enc(v, s);
-}
 
 struct ThermalArgs {
     Geom g;
@@ -80,105 +45,6 @@ struct ThermalArgs {
     double* jacc;
 };
 
-constexpr int TQ = 7;
-
-template <class C>
-LBM_DEV C thermal_omega(const ThermalArgs& A, C phi) {
-    return C(1) / (C(0.5) + C(4) * (C(A.kappa) * (C(1) + C(A.ratio_m1) * phi)));
-}
-
-template <class C>
-LBM_DEV C thermal_cu(const C u[3], int q) {
-    return u[0] * C(cvec<TQ>(q, 0)) + u[1] * C(cvec<TQ>(q, 1)) + u[2] * C(cvec<TQ>(q, 2));
-}
-
-// One cell: populations g, flow velocity u -> post-collision populations; T = sum g.
-template <class C>
-LBM_DEV void thermal_collide(const C* gv, const C u[3], C om, C S, C* out, C& T) {
-    T = C(0);
-#pragma unroll
-    for (int q = 0; q < TQ; q++) T += gv[q];
-#pragma unroll
-    for (int q = 0; q < TQ; q++) {
-        const C w = C(weight<TQ>(q));
-        const C eq = w * T * (C(1) + C(4) * thermal_cu(u, q));
-        out[q] = gv[q] - om * (gv[q] - eq) + w * S;
-    }
-}
-
-// Transpose of thermal_collide: gb = cotangent of the output.
-template <class C>
-LBM_DEV void thermal_collide_vjp(const C* gv, const C u[3], C om, const C* gb, C* g_bar, C u_bar[3], C& om_bar,
-                                 C& S_bar) {
-    C T = C(0);
-#pragma unroll
-    for (int q = 0; q < TQ; q++) T += gv[q];
-    C t_bar = C(0), ob = C(0), sb = C(0);
-    u_bar[0] = u_bar[1] = u_bar[2] = C(0);
-    static_for<0, TQ>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        const C w = C(weight<TQ>(q));
-        const C cu = thermal_cu(u, q);
-        const C eq = w * T * (C(1) + C(4) * cu);
-        const C eq_bar = om * gb[q];
-        g_bar[q] = (C(1) - om) * gb[q];
-        ob += gb[q] * (eq - gv[q]);
-        sb += gb[q] * w;
-        t_bar += eq_bar * w * (C(1) + C(4) * cu);
-        const C cu_bar = eq_bar * w * T * C(4);
-        if constexpr (cvec<TQ>(q, 0) != 0) u_bar[0] += cu_bar * C(cvec<TQ>(q, 0));
-        if constexpr (cvec<TQ>(q, 1) != 0) u_bar[1] += cu_bar * C(cvec<TQ>(q, 1));
-        if constexpr (cvec<TQ>(q, 2) != 0) u_bar[2] += cu_bar * C(cvec<TQ>(q, 2));
-    });
-#pragma unroll
-    for (int q = 0; q < TQ; q++) g_bar[q] += t_bar;
-    om_bar = ob;
-    S_bar = sb;
-}
-
-template <int P>
-LBM_DEV void load_tcell(const Geom& g, const typename TStore<P>::T* tp, const Cursor& cu, uint32_t mt, int odd,
-                        float tdec, typename TStore<P>::C* gv) {
-    static_for<0, TQ>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        gv[q] = TStore<P>::dec(load_read<TQ, q>(tp, g, cu, mt, odd), tdec);
-    });
-}
-
-// Forward: one thread per cell, before the flow kernel of the same interval.
-template <int P, int Q>
-__global__ __launch_bounds__(256) void k_thermal_fwd(ThermalArgs A) {
-    using S = Store<P>;
-    using TS = TStore<P>;
-    using C = typename S::C;
-    const Geom& g = A.g;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
-    double jsum = 0.0;
-    if (k < g.nz && j < g.ny) {
-        const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-        const Cursor cu = make_cursor(g, i, j, k);
-        C f[Q];
-        load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, face_mask<Q>(g, i, j, k), A.odd, f);
-        const C alpha = static_cast<const C*>(A.alpha)[cell];
-        const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
-        Macro<Q, C> mac;
-        macro_of<Q, C, S::shifted>(f, alpha, a, mac);
-        const uint32_t mt = face_mask<TQ>(g, i, j, k);
-        C gv[TQ], out[TQ], T;
-        load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
-        const C phi = static_cast<const C*>(A.phi)[cell];
-        thermal_collide<C>(gv, mac.u, thermal_omega<C>(A, phi), C(A.source) * phi, out, T);
-        if (A.obj_w != 0.0 && phi > C(0.5)) jsum = A.obj_w * double(T);
-        typename TS::T* to = static_cast<typename TS::T*>(A.tpop_out);
-        const uint32_t key = (uint32_t(cell) * uint32_t(TQ)) ^ (A.step * 0x9e3779b9u);
-        static_for<0, TQ>([&](auto I) {
-            constexpr int q = decltype(I)::value;
-            at_write<TQ, q>(to, g, cu, A.odd) = tstore_step<P>(out[q], A.tenc, key + uint32_t(q));
-        });
-    }
-    if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
-}
 
 // Adjoint, before the flow adjoint kernel of the same reverse step: lambda_g in place
 // (write locations -> read locations), the design cotangent phi_bar, and the cotangent
@@ -211,7 +77,7 @@ __global__ __launch_bounds__(256) void k_thermal_adj(ThermalArgs A) {
         gb[q] = L::template ld<C>(&at_write<TQ, q>(tl, g, cu, A.odd));
     });
     const C phi = static_cast<const C*>(A.phi)[cell];
-    const C om = thermal_omega<C>(A, phi);
+    const C om = thermal_omega<C>(A.kappa, A.ratio_m1, phi);
     thermal_collide_vjp<C>(gv, mac.u, om, gb, g_bar, ub, ob, sb);
     if (A.obj_w != 0.0 && phi > C(0.5)) {
 #pragma unroll
