@@ -186,6 +186,7 @@ double odd_rate(double omega, double magic) {
 struct lbm_handle {
     int Q = 19, kind = K_TRT, prec = LBM_F32, lprec = LBM_F32;
     bool regularized = false;
+    double les_c = 0.0, les_floor = 0.0;  // Smagorinsky closure (K_TRT_LES / K_BGK_LES)
     uint64_t every = 0;  // Checkpointing::Every(k); 0 = binomial
     int nx = 0, ny = 0, nz = 0;
     int per[3] = {0, 0, 0};
@@ -652,6 +653,16 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         // The regularized operator is the moment-space operator with its own rates.
         h->regularized = cfg->collision == LBM_REGULARIZED;
         h->kind = h->regularized ? int(K_MRT) : int(cfg->collision);
+        if (!(cfg->les_constant >= 0.0) || !std::isfinite(cfg->les_constant) || !(cfg->les_norm_floor >= 0.0) ||
+            !std::isfinite(cfg->les_norm_floor))
+            fail(LBM_ERR_CONTRACT, "Smagorinsky constant and norm floor must be finite and non-negative");
+        if (cfg->les_constant > 0.0) {
+            if (h->kind != K_TRT && h->kind != K_BGK)
+                fail(LBM_ERR_CONTRACT, "the Smagorinsky closure applies to the BGK and TRT operators");
+            h->kind = h->kind == K_TRT ? int(K_TRT_LES) : int(K_BGK_LES);
+            h->les_c = cfg->les_constant;
+            h->les_floor = cfg->les_norm_floor;
+        }
         if (h->kind == K_CUMULANT && h->Q != 27)
             fail(LBM_ERR_CONTRACT, "cumulant collision requires the D3Q27 velocity set (got D3Q19)");
         if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
@@ -720,9 +731,10 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (!(h->tau >= 0.51 && h->tau <= 2.0))
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
-        if ((h->kind == K_TRT || h->kind == K_MRT) && !(std::isfinite(h->magic) && h->magic > 0.0))
+        const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES;
+        if (trt_like && !(std::isfinite(h->magic) && h->magic > 0.0))
             fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
-        h->om = (h->kind == K_TRT || h->kind == K_MRT) ? odd_rate(h->omega, h->magic) : h->omega;
+        h->om = trt_like ? odd_rate(h->omega, h->magic) : h->omega;
         h->bulk = cfg->bulk_rate;
         h->even = cfg->even_rate;
         auto rate_ok = [](double r) { return std::isnan(r) || (std::isfinite(r) && r > 0.0 && r < 2.0); };
@@ -874,9 +886,12 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             hip_check(hipMemcpy(h->mask.p, m.data(), h->mask.bytes, hipMemcpyHostToDevice), "mask upload");
         }
         // Collision constants.
-        if (h->kind == K_MRT || h->kind == K_CUMULANT) {
+        if (h->kind == K_MRT || h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
             std::vector<double> c;
-            if (h->kind == K_MRT) {
+            if (h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
+                // [magic, k = 18 sqrt(2) C^2 (psm.rs:193 evaluation order), floor]
+                c = {h->magic, 18.0 * 1.4142135623730951 * h->les_c * h->les_c, h->les_floor};
+            } else if (h->kind == K_MRT) {
                 std::vector<double> rows, norms;
                 std::vector<int> groups;
                 mrt_basis(h->Q, rows, norms, groups);

@@ -38,6 +38,7 @@
 
 #include "cell.h"
 #include "cumulant.h"
+#include "les.h"
 
 #ifndef LBM_FLOW_SR
 #define LBM_FLOW_SR 0
@@ -46,7 +47,11 @@
 namespace lbm {
 
 enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
-enum Kind : int { K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5 };
+// K_TRT_LES / K_BGK_LES: TRT / BGK with the Smagorinsky closure (les.h); their constants
+// [magic, k, floor] travel in the collision-constant array (`mrt`).
+enum Kind : int {
+    K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7
+};
 enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
@@ -853,7 +858,14 @@ LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, co
             collide_bgk_guo(v, alpha, a, tau, out);
     } else {
         macro_of<Q, C, SH>(v, alpha, a, m);
-        if constexpr (KIND == K_TRT || KIND == K_BGK) {
+        if constexpr (KIND == K_TRT_LES || KIND == K_BGK_LES) {
+            const C oe = smagorinsky<Q, C, SH>(v, m, omega, mrt);
+            const C oo = KIND == K_TRT_LES ? odd_rate_of(oe, mrt[0]) : oe;
+            if constexpr (SH)
+                collide_pairs_shifted<Q>(v, m, oe, oo, out);
+            else
+                collide_pairs<Q>(v, m.rho, m.u, m.F, oe, oo, out);
+        } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (SH)
                 collide_pairs_shifted<Q>(v, m, omega, KIND == K_BGK ? omega : om, out);
             else
@@ -880,7 +892,18 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
         collide_bgk_guo_vjp(m, alpha, a, tau, g, fb, ab);
     } else {
         C cb_rho, cb_u[3], cb_F[3];
-        if constexpr (KIND == K_TRT || KIND == K_BGK) {
+        if constexpr (KIND == K_TRT_LES || KIND == K_BGK_LES) {
+            const C oe = smagorinsky<Q, C, SH>(v, m, omega, mrt);
+            const C oo = KIND == K_TRT_LES ? odd_rate_of(oe, mrt[0]) : oe;
+            const C dom = KIND == K_TRT_LES ? odd_rate_derivative_of(oe, mrt[0]) : C(1);
+            if constexpr (std::is_same<C, float>::value)
+                collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, oe, oo, g, fb, cb_rho, cb_u, cb_F);
+            else
+                collide_pairs_vjp<Q>(m.rho, m.u, m.F, oe, oo, g, fb, cb_rho, cb_u, cb_F);
+            C ob, omb;
+            pairs_omega_bar<Q, C, SH>(v, m, g, ob, omb);
+            smagorinsky_vjp<Q, C, SH>(v, m, omega, mrt, ob + omb * dom, fb, cb_rho, cb_u);
+        } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,
                                           cb_F);

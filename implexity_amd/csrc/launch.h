@@ -121,6 +121,8 @@ struct RunnerImpl final : Runner {
             }
             if constexpr (ALL_KINDS) {
                 if (kind == K_MRT) LBM_LAUNCH((k_forward<P, Q, K_MRT, false, true>), c, a);
+                if (kind == K_TRT_LES) LBM_LAUNCH((k_forward<P, Q, K_TRT_LES, false, true>), c, a);
+                if (kind == K_BGK_LES) LBM_LAUNCH((k_forward<P, Q, K_BGK_LES, false, true>), c, a);
                 if constexpr (Q == 27) {
                     if (kind == K_CUMULANT) LBM_LAUNCH((k_forward<P, 27, K_CUMULANT, false, true>), c, a);
                 }
@@ -153,6 +155,8 @@ struct RunnerImpl final : Runner {
         }
         if constexpr (ALL_KINDS) {
             if (kind == K_MRT) LBM_LAUNCH((k_forward<P, Q, K_MRT, M>), c, a);
+            if (kind == K_TRT_LES) LBM_LAUNCH((k_forward<P, Q, K_TRT_LES, M>), c, a);
+            if (kind == K_BGK_LES) LBM_LAUNCH((k_forward<P, Q, K_BGK_LES, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward<P, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -173,6 +177,8 @@ struct RunnerImpl final : Runner {
             }
             if constexpr (ALL_KINDS) {
                 if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, false, true>), c, a);
+                if (kind == K_TRT_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_LES, false, true>), c, a);
+                if (kind == K_BGK_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_LES, false, true>), c, a);
                 if constexpr (Q == 27) {
                     if (kind == K_CUMULANT) LBM_LAUNCH((k_adjoint<P, LP, 27, K_CUMULANT, false, true>), c, a);
                 }
@@ -199,6 +205,8 @@ struct RunnerImpl final : Runner {
         }
         if constexpr (ALL_KINDS) {
             if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, M>), c, a);
+            if (kind == K_TRT_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_LES, M>), c, a);
+            if (kind == K_BGK_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_LES, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_adjoint<P, LP, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -215,7 +223,7 @@ struct RunnerImpl final : Runner {
         if (a.tpop && (!a.tpop_out || !a.phi)) return false;
         if (a.tw != 0.0 && (!a.part || !a.jacc)) return false;
         if (mask && !a.mask) return false;
-        if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt) return false;
         if ((a.obj.kind != OBJ_NONE || a.diag) && (!a.part || !a.jacc)) return false;
         if (a.diag && (!a.dstat || !a.macc)) return false;
         return true;
@@ -224,7 +232,7 @@ struct RunnerImpl final : Runner {
         if (!a.pop || !a.lam || !a.alpha || !a.alpha_bar) return false;
         if (LP == P_F16S && !a.lsc) return false;
         if (mask && !a.mask) return false;
-        if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt) return false;
         return true;
     }
     static bool ok_ports(bool mask, const PortArgs& a) {

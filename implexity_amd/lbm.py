@@ -90,6 +90,7 @@ class _Config(ctypes.Structure):
         ("thermal", ctypes.c_uint32), ("reserved2", ctypes.c_uint32), ("thermal_kappa", ctypes.c_double),
         ("thermal_ratio", ctypes.c_double), ("thermal_source", ctypes.c_double),
         ("thermal_inlet", ctypes.c_double), ("thermal_range", ctypes.c_double),
+        ("les_constant", ctypes.c_double), ("les_norm_floor", ctypes.c_double),
     ]
 
 
@@ -243,6 +244,9 @@ class LatticeConfig:
     thermal_source: float = 0.0
     thermal_inlet: float = 0.0
     thermal_range: float = 0.0
+    # Smagorinsky closure on BGK/TRT (moving/psm.rs:45-48,161-197); 0 = off
+    les_constant: float = 0.0
+    les_norm_floor: float = 0.0
 
 
 @dataclass
@@ -324,6 +328,8 @@ class Lattice:
         c.thermal_source = float(cfg.thermal_source)
         c.thermal_inlet = float(cfg.thermal_inlet)
         c.thermal_range = float(cfg.thermal_range)
+        c.les_constant = float(cfg.les_constant)
+        c.les_norm_floor = float(cfg.les_norm_floor)
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 /* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
  * crates/implexity-ad/src/error.rs). */
@@ -148,6 +148,12 @@ typedef struct {
     double thermal_source;        /* heat source per step at phi = 1 */
     double thermal_inlet;         /* temperature held at velocity ports */
     double thermal_range;         /* FP16 storage: bound on |T| that sets the storage scale (0: 1) */
+    /* Smagorinsky closure on BGK/TRT (moving/psm.rs:45-48,161-197): the effective
+     * omega+ of a cell is 1/tau_e, tau_e = (tau0 + sqrt(tau0^2 + 18 sqrt(2) C^2 P / rho)) / 2,
+     * P = sqrt(|Pi_neq|^2 + floor^2) - floor; TRT's omega- follows through the magic
+     * parameter.  les_constant = 0: no closure. */
+    double les_constant;          /* C >= 0 */
+    double les_norm_floor;        /* floor >= 0 */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

@@ -144,6 +144,8 @@ typedef struct {
     const double *mrt_rows;     /* Q*Q, row-major rows[k][i] (from orc_mrt_basis) */
     const double *mrt_norms;    /* Q */
     const int32_t *mrt_groups;  /* Q */
+    double les_constant;        /* Smagorinsky constant (psm.rs:45-48); <= 0 or NaN: no LES */
+    double les_floor;           /* Smagorinsky norm_floor */
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -314,7 +316,8 @@ static void collide_pairs(const lattice_t *L, const double *f, double rho, const
 /* collide_pairs_vjp: collision.rs:492-570.  Returns f_bar, rho_bar, u_bar, force_bar. */
 static void collide_pairs_vjp(const lattice_t *L, const double *f, double rho, const double u[3], const double F[3],
                               double omega, double om, const double *g, double *f_bar, double *rho_bar_out,
-                              double u_bar_out[3], double force_bar_out[3]) {
+                              double u_bar_out[3], double force_bar_out[3], double *omega_bar_out,
+                              double *om_bar_out) {
     double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     double base = 1.0 - usq * 1.5;
     double kp = 1.0 - omega * 0.5;
@@ -323,11 +326,16 @@ static void collide_pairs_vjp(const lattice_t *L, const double *f, double rho, c
     double u_bar[3] = {0, 0, 0}, force_bar[3] = {0, 0, 0};
     double w0 = L->w[0];
     double g0 = g[0];
+    double uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+    double eq0 = rho * w0 * base;
+    double s0 = -uf * (3.0 * w0);
+    double omega_bar = 0.0, om_bar = 0.0;
     for (int i = 0; i < L->Q; i++) f_bar[i] = 0.0;
     f_bar[0] = g0 * (1.0 - omega);
     rho_bar += g0 * omega * w0 * base;
     usq_bar -= g0 * omega * rho * (1.5 * w0);
     uf_bar -= g0 * kp * (3.0 * w0);
+    omega_bar -= g0 * ((f[0] - eq0) + s0 * 0.5);
     for (int i = 1; i < L->Q; i++) {
         int o = L->opp[i];
         if (o < i) continue;
@@ -337,6 +345,16 @@ static void collide_pairs_vjp(const lattice_t *L, const double *f, double rho, c
         double wr = rho * w;
         double a_bar = g[i] + g[o];
         double b_bar = g[i] - g[o];
+        {
+            double eqp = wr * (base + cu * cu * 4.5);
+            double eqm = wr * cu * 3.0;
+            double sp = (cu * cf * 9.0 - uf * 3.0) * w;
+            double sm = cf * (3.0 * w);
+            double fp = (f[i] + f[o]) * 0.5;
+            double fm = (f[i] - f[o]) * 0.5;
+            omega_bar -= a_bar * ((fp - eqp) + sp * 0.5);
+            om_bar -= b_bar * ((fm - eqm) + sm * 0.5);
+        }
         double fp_bar = -omega * a_bar;
         double fm_bar = -om * b_bar;
         double eqp_bar = omega * a_bar;
@@ -365,7 +383,83 @@ static void collide_pairs_vjp(const lattice_t *L, const double *f, double rho, c
     for (int d = 0; d < 3; d++) {
         u_bar_out[d] = u_bar[d];
         force_bar_out[d] = force_bar[d];
+    }    if (omega_bar_out) *omega_bar_out = omega_bar;
+    if (om_bar_out) *om_bar_out = om_bar;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Smagorinsky LES (moving/psm.rs:161-197): effective omega from the norm of    */
+/* the non-equilibrium momentum flux, applied to BGK/TRT (omega- follows from  */
+/* the effective omega+ through the magic parameter).                          */
+/* ------------------------------------------------------------------------- */
+static int les_on(const orc_problem *p) {
+    return p->les_constant > 0.0 && (p->kind == K_BGK || p->kind == K_TRT);
+}
+
+/* pi[x][y] = sum_i (f_i - eq_i) c_ix c_iy and its squared norm, psm.rs:169-189. */
+static double les_flux(const lattice_t *L, const double *f, double rho, const double u[3], double pi[3][3]) {
+    double eq[MAXQ];
+    equilibrium(L, rho, u, eq);
+    for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) pi[x][y] = 0.0;
+    for (int i = 0; i < L->Q; i++) {
+        double n = f[i] - eq[i];
+        for (int x = 0; x < 3; x++) {
+            if (L->cf[i][x] == 0.0) continue;
+            for (int y = 0; y < 3; y++)
+                if (L->cf[i][y] != 0.0) pi[x][y] += n * (L->cf[i][x] * L->cf[i][y]);
+        }
     }
+    double qn = 0.0;
+    for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) qn += pi[x][y] * pi[x][y];
+    return qn;
+}
+
+static double smagorinsky(const orc_problem *p, const lattice_t *L, const double *f, double rho, const double u[3],
+                          double omega) {
+    double pi[3][3];
+    double qn = les_flux(L, f, rho, u, pi);
+    double delta = p->les_floor;
+    double pn = sqrt(qn + delta * delta) - delta;
+    double k = 18.0 * 1.4142135623730951 * p->les_constant * p->les_constant;
+    double tau0 = 1.0 / omega;
+    double r = sqrt(tau0 * tau0 + pn * k / rho);
+    return 1.0 / ((tau0 + r) * 0.5);
+}
+
+/* Transpose of smagorinsky (psm.rs:415-473) given the cotangent of the effective
+ * omega: adds to f_bar, rho_bar, u_bar (the equilibrium's inputs). */
+static void smagorinsky_vjp(const orc_problem *p, const lattice_t *L, const double *f, double rho, const double u[3],
+                            double omega, double omega_eff_bar, double *f_bar, double *rho_bar, double u_bar[3]) {
+    double pi[3][3];
+    double qn = les_flux(L, f, rho, u, pi);
+    double delta = p->les_floor;
+    double root = sqrt(qn + delta * delta);
+    double pn = root - delta;
+    double k = 18.0 * 1.4142135623730951 * p->les_constant * p->les_constant;
+    double tau0 = 1.0 / omega;
+    double r = sqrt(tau0 * tau0 + pn * k / rho);
+    double tau_e = (tau0 + r) * 0.5;
+    double tau_e_bar = -omega_eff_bar / (tau_e * tau_e);
+    double r_bar = tau_e_bar * 0.5;
+    double inner_bar = r_bar / (r * 2.0);
+    double p_bar = inner_bar * k / rho;
+    *rho_bar -= inner_bar * pn * k / (rho * rho);
+    double qn_bar = p_bar / (root * 2.0);
+    double n_bar[MAXQ], neg[MAXQ];
+    for (int i = 0; i < L->Q; i++) {
+        double acc = 0.0;
+        for (int x = 0; x < 3; x++) {
+            if (L->cf[i][x] == 0.0) continue;
+            for (int y = 0; y < 3; y++)
+                if (L->cf[i][y] != 0.0) acc += pi[x][y] * (L->cf[i][x] * L->cf[i][y]);
+        }
+        n_bar[i] = acc * qn_bar * 2.0;
+        f_bar[i] += n_bar[i];
+        neg[i] = -n_bar[i];
+    }
+    equilibrium_vjp(L, rho, u, neg, rho_bar, u_bar);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -939,9 +1033,10 @@ void orc_cell_collide(const orc_problem *p, const double *f, double alpha, doubl
     }
     double rho, j[3], den, sv[3], u[3], F[3];
     brinkman_macro(L, f, alpha, p->accel, &rho, j, &den, sv, u, F);
+    const double om_eff = les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega;
     switch (p->kind) {
-    case K_BGK: collide_pairs(L, f, rho, u, F, p->omega, p->omega, out); break;
-    case K_TRT: collide_pairs(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), out); break;
+    case K_BGK: collide_pairs(L, f, rho, u, F, om_eff, om_eff, out); break;
+    case K_TRT: collide_pairs(L, f, rho, u, F, om_eff, orc_odd_rate(om_eff, p->magic), out); break;
     case K_MRT:
     case K_REGULARIZED: collide_mrt(p, L, f, rho, u, F, out); break;
     case K_CUMULANT: collide_cumulant(f, rho, u, F, p->omega, isnan(p->bulk_rate) ? 1.0 : p->bulk_rate, out); break;
@@ -962,11 +1057,14 @@ void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const dou
     double rho, j[3], den, sv[3], u[3], F[3];
     brinkman_macro(L, f, alpha, a, &rho, j, &den, sv, u, F);
     double cb_f[MAXQ], cb_rho, cb_u[3], cb_F[3];
+    const double om_eff = les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega;
+    double ob = 0.0, omb = 0.0, dom = 1.0;
     switch (p->kind) {
-    case K_BGK: collide_pairs_vjp(L, f, rho, u, F, p->omega, p->omega, g, cb_f, &cb_rho, cb_u, cb_F); break;
+    case K_BGK: collide_pairs_vjp(L, f, rho, u, F, om_eff, om_eff, g, cb_f, &cb_rho, cb_u, cb_F, &ob, &omb); break;
     case K_TRT:
-        collide_pairs_vjp(L, f, rho, u, F, p->omega, orc_odd_rate(p->omega, p->magic), g, cb_f, &cb_rho, cb_u,
-                          cb_F);
+        collide_pairs_vjp(L, f, rho, u, F, om_eff, orc_odd_rate(om_eff, p->magic), g, cb_f, &cb_rho, cb_u, cb_F,
+                          &ob, &omb);
+        dom = orc_odd_rate_derivative(om_eff, p->magic);
         break;
     case K_MRT:
     case K_REGULARIZED: collide_mrt_vjp(p, L, f, rho, u, F, g, cb_f, &cb_rho, cb_u, cb_F); break;
@@ -985,6 +1083,8 @@ void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const dou
         alpha_bar -= zb * u[d];
         u_bar[d] -= zb * alpha;
     }
+    if (les_on(p))  /* psm.rs:415-473: the effective omega's cotangent into f, rho, u */
+        smagorinsky_vjp(p, L, f, rho, u, p->omega, ob + omb * dom, cb_f, &rho_bar, u_bar);
     double j_bar[3];
     for (int d = 0; d < 3; d++) {
         double s_bar = u_bar[d] / den;

@@ -84,6 +84,7 @@ class _Problem(ctypes.Structure):
         ("port_kind", ctypes.c_void_p), ("port_value", ctypes.c_void_p),
         ("solid", ctypes.c_void_p),
         ("mrt_rows", ctypes.c_void_p), ("mrt_norms", ctypes.c_void_p), ("mrt_groups", ctypes.c_void_p),
+        ("les_constant", ctypes.c_double), ("les_floor", ctypes.c_double),
     ]
 
 
@@ -136,6 +137,9 @@ class Problem:
     accel: tuple = (0.0, 0.0, 0.0)
     solid: np.ndarray | None = None  # bool, shape
     ports: list = field(default_factory=list)
+    # Smagorinsky LES on BGK/TRT (psm.rs:45-48,161-197): constant (0 = off), norm floor
+    les_constant: float = 0.0
+    les_floor: float = 0.0
 
 
 class Oracle:
@@ -168,6 +172,7 @@ class Oracle:
         p.port_kind = _ptr(self.port_kind)
         p.port_value = _ptr(self.port_value)
         p.solid = _ptr(self.solid)
+        p.les_constant, p.les_floor = float(prob.les_constant), float(prob.les_floor)
         if self._mrt is not None:
             p.mrt_rows, p.mrt_norms, p.mrt_groups = (_ptr(self._mrt[0]), _ptr(self._mrt[1]), _ptr(self._mrt[2]))
         self._p = p

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lbm.lib().lbm_abi_version() == 2
+    assert lbm.lib().lbm_abi_version() == 3
 
 
 @pytest.mark.parametrize("n,s", [(1, 1), (3, 1), (10, 2), (100, 10), (1000, 30), (8000, 156)])
