@@ -213,8 +213,10 @@ struct lbm_handle {
     // state buffer (flow_bytes in), its cotangents the flow cotangents (flow_lam_bytes in).
     bool thermal = false;
     double tkappa = 0.0, tratio = 1.0, tsource = 0.0, tinlet = 0.0;
+    double trange = 1.0;  // bound on |T| (storage scale, cotangent bounds)
     float tenc = 1.0f, tdec = 1.0f;  // FP16 storage scale of the temperature populations
     size_t flow_bytes = 0, flow_lam_bytes = 0;
+    size_t tlam_elem = 4;  // bytes of one temperature cotangent
     long long n_tsolid = 0;  // cells with phi > 1/2
     DevBuf phi, phi_bar, ubar;  // ubar: velocity cotangent handed from the thermal to the flow adjoint
     const ThermalRunner* trun = nullptr;
@@ -442,7 +444,7 @@ void ensure_aos(lbm_handle* h) { h->aos.alloc(size_t(h->ncell) * h->Q * sizeof(d
 // Pressure drop: weight / (3 * plane cells).  Porous dissipation rho alpha |u|^2:
 // |rho_bar| + sum_d |j_bar_d| <= weight * alpha * (3 / rho + 2 sqrt(3) |u|) <= 16 weight
 // alpha_max for rho >= 1/2 and |u| <= 1; the FP16 exponent keeps 4x headroom above it.
-double seed_bound(const lbm_handle* h, const ObjArgs& o) {
+double seed_bound_flow(const lbm_handle* h, const ObjArgs& o) {
     if (o.kind == OBJ_PRESSURE_DROP) return std::fabs(o.weight) * std::max(o.inv_na, o.inv_nb) / 3.0;
     if (o.kind == OBJ_DISSIPATION) return 16.0 * std::fabs(o.weight) * h->alpha_max;
     if (o.kind == OBJ_VECTOR) {
@@ -453,6 +455,17 @@ double seed_bound(const lbm_handle* h, const ObjArgs& o) {
         return std::fabs(o.weight) * (80.0 * B + 8.0 * A);
     }
     return 0.0;
+}
+
+// With the temperature field, a stage cost tw * T reaches the flow cotangent in the
+// next transposed step through the advecting velocity: |u_bar| <= omega_T |T| sum_i
+// 4 w_i |c_i| |lambda_g| <= 2 |T| tw and |d u / d f_q| <= 3 / (rho den), so
+// 16 * (bound on |T|) * tw bounds it for rho >= 1/2 (the FP16 exponent keeps 28
+// binades below the bound, so the estimate may be generous).
+double seed_bound(const lbm_handle* h, const ObjArgs& o) {
+    double b = seed_bound_flow(h, o);
+    if (h->thermal && o.tw != 0.0) b += 16.0 * h->trange * std::fabs(o.tw);
+    return b;
 }
 
 // Cotangent scale record before a field of |lambda| <= bound is written (seed or
@@ -691,9 +704,9 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (h->thermal) {
             if (h->kind == K_BGK_GUO) fail(LBM_ERR_CONTRACT, "the temperature field couples to the Brinkman operators, not BGK-Guo");
             if (h->prec == LBM_BF16_SHIFTED) fail(LBM_ERR_CONTRACT, "the temperature field supports FP64, FP32 and FP16 storage");
-            if (cfg->adjoint_storage == LBM_ADJ_F16_SCALED)
-                fail(LBM_ERR_CONTRACT, "the coupled temperature field keeps FP32 cotangents");
-            if (h->lprec == LBM_F16_SHIFTED) h->lprec = LBM_F32;
+            // the temperature cotangents stay FP32 (FP64 with FP64 storage) next to the
+            // flow's, which may be scaled FP16
+            h->tlam_elem = h->prec == LBM_F64 ? 8 : 4;
             if (!(cfg->thermal_kappa > 0.0) || !std::isfinite(cfg->thermal_kappa))
                 fail(LBM_ERR_CONTRACT, "thermal_kappa must be finite and positive");
             if (!(cfg->thermal_ratio > 0.0) || !std::isfinite(cfg->thermal_ratio))
@@ -706,6 +719,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             h->tratio = cfg->thermal_ratio;
             h->tsource = cfg->thermal_source;
             h->tinlet = cfg->thermal_inlet;
+            h->trange = cfg->thermal_range > 0.0 ? cfg->thermal_range : 1.0;
             if (h->prec == LBM_F16_SHIFTED) {
                 const int e = lam_exponent(cfg->thermal_range > 0.0 ? cfg->thermal_range : 1.0);
                 h->tenc = std::ldexp(1.0f, e);
@@ -775,13 +789,15 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         const long long cells_and_ghosts = h->g.zg + (h->per[2] ? 0 : (long long)SX * h->g.SY);
         h->g.stride = ((cells_and_ghosts + 63) / 64) * 64;
         // Kernels address an opposite pair of population arrays with 32-bit byte offsets.
-        if (uint64_t(h->g.stride) * std::max(elem_bytes(h->prec), elem_bytes(h->lprec)) >= (uint64_t(1) << 31))
+        if (uint64_t(h->g.stride) * std::max({elem_bytes(h->prec), elem_bytes(h->lprec),
+                                              h->thermal ? h->tlam_elem : size_t(0)}) >=
+            (uint64_t(1) << 31))
             fail(LBM_ERR_CONTRACT, "lattice too large: one population array would exceed 2 GiB");
         h->flow_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
         h->flow_lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
         const int TQn = h->thermal ? TQ : 0;
         h->state_bytes = size_t(h->g.stride) * (h->Q + TQn) * elem_bytes(h->prec);
-        h->lam_bytes = size_t(h->g.stride) * (h->Q + TQn) * elem_bytes(h->lprec);
+        h->lam_bytes = h->flow_lam_bytes + size_t(h->g.stride) * TQn * h->tlam_elem;
         h->run = runner_for(h->prec, h->lprec);
 
         // Ports (moving/boundary.rs:318-380).

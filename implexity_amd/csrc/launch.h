@@ -112,7 +112,7 @@ struct RunnerImpl final : Runner {
     // storages the temperature field supports).
     template <int Q>
     static hipError_t fwd_th(int kind, const StepArgs& a, const LaunchCfg& c) {
-        if constexpr (LP != P_F16S && P != P_BF16S) {
+        if constexpr (P != P_BF16S) {
             if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
             switch (kind) {
             case K_TRT: LBM_LAUNCH((k_forward<P, Q, K_TRT, false, true>), c, a);
@@ -166,10 +166,10 @@ struct RunnerImpl final : Runner {
         return hipErrorInvalidValue;
     }
     // Flow adjoint with the velocity cotangent of the coupled temperature field (no solid
-    // mask, FP32/FP64 cotangents, per-cell kernels).
+    // mask, per-cell kernels).
     template <int Q>
     static hipError_t adj_ub(int kind, const AdjArgs& a, const LaunchCfg& c) {
-        if constexpr (LP != P_F16S) {
+        if constexpr (P != P_BF16S) {
             switch (kind) {
             case K_TRT: LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT, false, true>), c, a);
             case K_BGK: LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK, false, true>), c, a);
