@@ -859,7 +859,9 @@ LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, co
     } else {
         macro_of<Q, C, SH>(v, alpha, a, m);
         if constexpr (KIND == K_TRT_LES || KIND == K_BGK_LES) {
-            const C oe = smagorinsky<Q, C, SH>(v, m, omega, mrt);
+            LesEval<Q, C> e;
+            les_eval<Q, C, SH>(v, m, omega, mrt, e);
+            const C oe = e.oe;
             const C oo = KIND == K_TRT_LES ? odd_rate_of(oe, mrt[0]) : oe;
             if constexpr (SH)
                 collide_pairs_shifted<Q>(v, m, oe, oo, out);
@@ -893,7 +895,9 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
     } else {
         C cb_rho, cb_u[3], cb_F[3];
         if constexpr (KIND == K_TRT_LES || KIND == K_BGK_LES) {
-            const C oe = smagorinsky<Q, C, SH>(v, m, omega, mrt);
+            LesEval<Q, C> e;
+            les_eval<Q, C, SH>(v, m, omega, mrt, e);
+            const C oe = e.oe;
             const C oo = KIND == K_TRT_LES ? odd_rate_of(oe, mrt[0]) : oe;
             const C dom = KIND == K_TRT_LES ? odd_rate_derivative_of(oe, mrt[0]) : C(1);
             if constexpr (std::is_same<C, float>::value)
@@ -901,8 +905,8 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
             else
                 collide_pairs_vjp<Q>(m.rho, m.u, m.F, oe, oo, g, fb, cb_rho, cb_u, cb_F);
             C ob, omb;
-            pairs_omega_bar<Q, C, SH>(v, m, g, ob, omb);
-            smagorinsky_vjp<Q, C, SH>(v, m, omega, mrt, ob + omb * dom, fb, cb_rho, cb_u);
+            pairs_omega_bar<Q, C>(e.n, m, g, ob, omb);
+            smagorinsky_vjp<Q, C>(e, m, mrt, ob + omb * dom, fb, cb_rho, cb_u);
         } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,

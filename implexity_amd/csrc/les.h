@@ -59,16 +59,25 @@ LBM_DEV C les_flux(const C* n, C pi[6]) {
     return q;
 }
 
+// The closure of one cell, evaluated once and shared by the forward rate, the omega
+// cotangents and the transpose.
+template <int Q, class C>
+struct LesEval {
+    C n[Q];    // non-equilibrium populations
+    C pi[6];   // Pi_xx, Pi_xy, Pi_xz, Pi_yy, Pi_yz, Pi_zz
+    C root, pn, tau0, r, oe;  // sqrt(|Pi|^2 + floor^2), P, 1/omega, sqrt(tau0^2 + P k / rho), omega_eff
+};
+
 template <int Q, class C, bool SH>
-LBM_DEV C smagorinsky(const C* v, const Macro<Q, C>& m, C omega, const C* les) {
-    C n[Q], pi[6];
-    nonequilibrium<Q, C, SH>(v, m, n);
-    const C qn = les_flux<Q, C>(n, pi);
+LBM_DEV void les_eval(const C* v, const Macro<Q, C>& m, C omega, const C* les, LesEval<Q, C>& e) {
+    nonequilibrium<Q, C, SH>(v, m, e.n);
+    const C qn = les_flux<Q, C>(e.n, e.pi);
     const C delta = les[2];
-    const C pn = sqrt(qn + delta * delta) - delta;
-    const C tau0 = C(1) / omega;
-    const C r = sqrt(tau0 * tau0 + pn * les[1] / m.rho);
-    return C(1) / ((tau0 + r) * C(0.5));
+    e.root = sqrt(qn + delta * delta);
+    e.pn = e.root - delta;
+    e.tau0 = C(1) / omega;
+    e.r = sqrt(e.tau0 * e.tau0 + e.pn * les[1] / m.rho);
+    e.oe = C(1) / ((e.tau0 + e.r) * C(0.5));
 }
 
 // omega- = 1 / (1/2 + magic / (1 / omega - 1/2)) and its derivative (collision.rs:302-313).
@@ -88,13 +97,11 @@ LBM_DEV C odd_rate_derivative_of(C omega, C magic) {
 // Cotangents of omega+ and omega- of collide_pairs (collision.rs:513-569):
 //   ob  = -g_0 ((f_0 - eq_0) + s_0 / 2) - sum_pairs (g_i + g_o) ((fp - eqp) + sp / 2),
 //   omb = -sum_pairs (g_i - g_o) ((fm - eqm) + sm / 2).
-template <int Q, class C, bool SH>
-LBM_DEV void pairs_omega_bar(const C* v, const Macro<Q, C>& m, const C* g, C& ob, C& omb) {
+template <int Q, class C>
+LBM_DEV void pairs_omega_bar(const C* n, const Macro<Q, C>& m, const C* g, C& ob, C& omb) {
     const C* u = m.u;
     const C* F = m.F;
     const C uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
-    C n[Q];
-    nonequilibrium<Q, C, SH>(v, m, n);
     constexpr double w0 = weight<Q>(0);
     C o = -g[0] * (n[0] + (-uf * C(3.0 * w0)) * C(0.5));
     C om = C(0);
@@ -116,37 +123,30 @@ LBM_DEV void pairs_omega_bar(const C* v, const Macro<Q, C>& m, const C* g, C& ob
     omb = om;
 }
 
-// Transpose of smagorinsky given the cotangent of omega_eff: adds to fb (through the
+// Transpose of the closure given the cotangent of omega_eff: adds to fb (through the
 // non-equilibrium part) and to the equilibrium's inputs rho_bar, u_bar.
-template <int Q, class C, bool SH>
-LBM_DEV void smagorinsky_vjp(const C* v, const Macro<Q, C>& m, C omega, const C* les, C oe_bar, C* fb, C& rho_bar,
-                             C u_bar[3]) {
-    C n[Q], pi[6];
-    nonequilibrium<Q, C, SH>(v, m, n);
-    const C qn = les_flux<Q, C>(n, pi);
-    const C delta = les[2], k = les[1];
-    const C root = sqrt(qn + delta * delta);
-    const C pn = root - delta;
-    const C tau0 = C(1) / omega;
-    const C r = sqrt(tau0 * tau0 + pn * k / m.rho);
-    const C tau_e = (tau0 + r) * C(0.5);
+template <int Q, class C>
+LBM_DEV void smagorinsky_vjp(const LesEval<Q, C>& e, const Macro<Q, C>& m, const C* les, C oe_bar, C* fb,
+                             C& rho_bar, C u_bar[3]) {
+    const C k = les[1];
+    const C tau_e = (e.tau0 + e.r) * C(0.5);
     const C tau_e_bar = -oe_bar / (tau_e * tau_e);
     const C r_bar = tau_e_bar * C(0.5);
-    const C inner_bar = r_bar / (r * C(2.0));
+    const C inner_bar = r_bar / (e.r * C(2.0));
     const C p_bar = inner_bar * k / m.rho;
-    rho_bar -= inner_bar * pn * k / (m.rho * m.rho);
-    const C qn_bar2 = p_bar / (root * C(2.0)) * C(2.0);
+    rho_bar -= inner_bar * e.pn * k / (m.rho * m.rho);
+    const C qn_bar2 = p_bar / (e.root * C(2.0)) * C(2.0);
     C nb[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int cx = cvec<Q>(i, 0), cy = cvec<Q>(i, 1), cz = cvec<Q>(i, 2);
         C acc = C(0);
-        if constexpr (cx != 0) acc += pi[0] * C(cx * cx);
-        if constexpr (cx != 0 && cy != 0) acc += C(2) * pi[1] * C(cx * cy);
-        if constexpr (cx != 0 && cz != 0) acc += C(2) * pi[2] * C(cx * cz);
-        if constexpr (cy != 0) acc += pi[3] * C(cy * cy);
-        if constexpr (cy != 0 && cz != 0) acc += C(2) * pi[4] * C(cy * cz);
-        if constexpr (cz != 0) acc += pi[5] * C(cz * cz);
+        if constexpr (cx != 0) acc += e.pi[0] * C(cx * cx);
+        if constexpr (cx != 0 && cy != 0) acc += C(2) * e.pi[1] * C(cx * cy);
+        if constexpr (cx != 0 && cz != 0) acc += C(2) * e.pi[2] * C(cx * cz);
+        if constexpr (cy != 0) acc += e.pi[3] * C(cy * cy);
+        if constexpr (cy != 0 && cz != 0) acc += C(2) * e.pi[4] * C(cy * cz);
+        if constexpr (cz != 0) acc += e.pi[5] * C(cz * cz);
         nb[i] = acc * qn_bar2;
         fb[i] += nb[i];
         nb[i] = -nb[i];
