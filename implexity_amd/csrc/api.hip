@@ -848,6 +848,13 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         }
         for (int nb : h->h_pnb)
             if (is_port[nb]) fail(LBM_ERR_CONTRACT, "the inward neighbour of a port cell is itself a port cell");
+        {  // the port transposes add into the neighbours' cotangents, one port cell each
+            std::vector<int> nbs(h->h_pnb);
+            std::sort(nbs.begin(), nbs.end());
+            if (std::adjacent_find(nbs.begin(), nbs.end()) != nbs.end())
+                fail(LBM_ERR_CONTRACT, "two port cells share an inward neighbour (an axis with ports on both faces "
+                                       "needs at least four cells)");
+        }
         h->np = int(h->h_pcell.size());
 
         // Device allocations.

@@ -70,6 +70,15 @@ def _create(**kw):
     (dict(precision="f32", adjoint_storage="f16"), "FP16 cotangents"),
     (dict(precision="f64", adjoint_storage="f32"), "FP64 storage"),
     (dict(periodic=(True, False, False), ports=[lbm.Port("xmin", "pressure", (1.0,))]), "periodic"),
+    (dict(shape=(3, 4, 4), ports=[lbm.Port("xmin", "velocity", (0.01, 0.0, 0.0)), lbm.Port("xmax", "pressure", (1.0,))]),
+     "share an inward neighbour"),
+    (dict(thermal=True, thermal_kappa=0.0), "thermal_kappa"),
+    (dict(thermal=True, thermal_kappa=0.02, thermal_ratio=-1.0), "thermal_ratio"),
+    (dict(thermal=True, thermal_kappa=0.02, collision="bgk_guo"), "BGK-Guo"),
+    (dict(thermal=True, thermal_kappa=0.02, precision="bf16s"), "FP64, FP32 and FP16"),
+    (dict(thermal=True, thermal_kappa=0.02, solid=np.eye(4, dtype=bool)[None].repeat(4, 0)), "no solid cells"),
+    (dict(les_constant=0.1, collision="mrt"), "Smagorinsky closure applies"),
+    (dict(les_constant=-0.1), "non-negative"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are
