@@ -669,6 +669,16 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (!(cfg->les_constant >= 0.0) || !std::isfinite(cfg->les_constant) || !(cfg->les_norm_floor >= 0.0) ||
             !std::isfinite(cfg->les_norm_floor))
             fail(LBM_ERR_CONTRACT, "Smagorinsky constant and norm floor must be finite and non-negative");
+        if (!(cfg->wale_constant >= 0.0) || !std::isfinite(cfg->wale_constant))
+            fail(LBM_ERR_CONTRACT, "WALE constant must be finite and non-negative");
+        if (cfg->wale_constant > 0.0) {
+            if (!(cfg->wale_floor > 0.0) || !std::isfinite(cfg->wale_floor))
+                fail(LBM_ERR_CONTRACT, "WALE constant and denominator_floor must be finite and positive");
+            if (cfg->les_constant > 0.0) fail(LBM_ERR_CONTRACT, "choose one closure: Smagorinsky or WALE");
+            if (h->kind != K_TRT && h->kind != K_BGK)
+                fail(LBM_ERR_CONTRACT, "the WALE closure applies to the BGK and TRT operators");
+            fail(LBM_ERR_CONTRACT, "the WALE closure is not available on the device yet");
+        }
         if (cfg->les_constant > 0.0) {
             if (h->kind != K_TRT && h->kind != K_BGK)
                 fail(LBM_ERR_CONTRACT, "the Smagorinsky closure applies to the BGK and TRT operators");

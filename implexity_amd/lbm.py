@@ -91,6 +91,7 @@ class _Config(ctypes.Structure):
         ("thermal_ratio", ctypes.c_double), ("thermal_source", ctypes.c_double),
         ("thermal_inlet", ctypes.c_double), ("thermal_range", ctypes.c_double),
         ("les_constant", ctypes.c_double), ("les_norm_floor", ctypes.c_double),
+        ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
     ]
 
 
@@ -247,6 +248,9 @@ class LatticeConfig:
     # Smagorinsky closure on BGK/TRT (moving/psm.rs:45-48,161-197); 0 = off
     les_constant: float = 0.0
     les_norm_floor: float = 0.0
+    # WALE closure on BGK/TRT (moving/wale.rs); 0 = off
+    wale_constant: float = 0.0
+    wale_floor: float = 1e-12
 
 
 @dataclass
@@ -330,6 +334,8 @@ class Lattice:
         c.thermal_range = float(cfg.thermal_range)
         c.les_constant = float(cfg.les_constant)
         c.les_norm_floor = float(cfg.les_norm_floor)
+        c.wale_constant = float(cfg.wale_constant)
+        c.wale_floor = float(cfg.wale_floor)
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)

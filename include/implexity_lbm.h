@@ -154,6 +154,12 @@ typedef struct {
      * parameter.  les_constant = 0: no closure. */
     double les_constant;          /* C >= 0 */
     double les_norm_floor;        /* floor >= 0 */
+    /* WALE closure on BGK/TRT (moving/wale.rs), exclusive with Smagorinsky: the rate of a
+     * fluid cell is 1 / (tau0 + 3 nu_t), nu_t = C^2 A^(3/2) / (B^(5/2) + A^(5/4) + floor)
+     * from the invariants A, B of the velocity gradient of u = j / rho of the state the
+     * step collides (central differences, one-sided next to walls and closed faces). */
+    double wale_constant;         /* C >= 0; 0: no closure */
+    double wale_floor;            /* > 0 when wale_constant > 0 */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

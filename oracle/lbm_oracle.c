@@ -146,6 +146,8 @@ typedef struct {
     const int32_t *mrt_groups;  /* Q */
     double les_constant;        /* Smagorinsky constant (psm.rs:45-48); <= 0 or NaN: no LES */
     double les_floor;           /* Smagorinsky norm_floor */
+    double wale_constant;       /* WALE constant (moving/wale.rs); <= 0 or NaN: none */
+    double wale_floor;          /* WALE denominator_floor */
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -1025,7 +1027,9 @@ static void brinkman_macro(const lattice_t *L, const double *f, double alpha, co
     }
 }
 
-void orc_cell_collide(const orc_problem *p, const double *f, double alpha, double *out) {
+/* One cell with omega+ given (rate < 0: the handle's omega, through the Smagorinsky
+ * closure when it is on). */
+static void cell_collide_rate(const orc_problem *p, const double *f, double alpha, double rate, double *out) {
     const lattice_t *L = lat(p->Q);
     if (p->kind == K_BGK_GUO) {
         collide_bgk_guo(f, alpha, p->accel, 1.0 / p->omega, out);
@@ -1033,7 +1037,7 @@ void orc_cell_collide(const orc_problem *p, const double *f, double alpha, doubl
     }
     double rho, j[3], den, sv[3], u[3], F[3];
     brinkman_macro(L, f, alpha, p->accel, &rho, j, &den, sv, u, F);
-    const double om_eff = les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega;
+    const double om_eff = rate >= 0.0 ? rate : (les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega);
     switch (p->kind) {
     case K_BGK: collide_pairs(L, f, rho, u, F, om_eff, om_eff, out); break;
     case K_TRT: collide_pairs(L, f, rho, u, F, om_eff, orc_odd_rate(om_eff, p->magic), out); break;
@@ -1043,10 +1047,23 @@ void orc_cell_collide(const orc_problem *p, const double *f, double alpha, doubl
     }
 }
 
+void orc_cell_collide(const orc_problem *p, const double *f, double alpha, double *out) {
+    cell_collide_rate(p, f, alpha, -1.0, out);
+}
+
 /* VJP of one cell w.r.t. (f, alpha): psm.rs::cell_update_vjp Brinkman branch
- * (psm.rs:301-330 and :455-469) with m = 0, dp_bar = 0, no LES. */
+ * (psm.rs:301-330 and :455-469) with m = 0, dp_bar = 0; rate as in cell_collide_rate,
+ * and with rate >= 0 also the cotangent of omega+ in *rate_bar (BGK/TRT). */
+static void cell_vjp_rate(const orc_problem *p, const double *f, double alpha, double rate, const double *g,
+                          double *f_bar, double *alpha_bar_out, double *rate_bar);
+
 void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const double *g, double *f_bar,
                   double *alpha_bar_out) {
+    cell_vjp_rate(p, f, alpha, -1.0, g, f_bar, alpha_bar_out, NULL);
+}
+
+static void cell_vjp_rate(const orc_problem *p, const double *f, double alpha, double rate, const double *g,
+                          double *f_bar, double *alpha_bar_out, double *rate_bar) {
     const lattice_t *L = lat(p->Q);
     if (p->kind == K_BGK_GUO) {
         collide_bgk_guo_vjp(f, alpha, p->accel, 1.0 / p->omega, g, f_bar, alpha_bar_out);
@@ -1057,7 +1074,7 @@ void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const dou
     double rho, j[3], den, sv[3], u[3], F[3];
     brinkman_macro(L, f, alpha, a, &rho, j, &den, sv, u, F);
     double cb_f[MAXQ], cb_rho, cb_u[3], cb_F[3];
-    const double om_eff = les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega;
+    const double om_eff = rate >= 0.0 ? rate : (les_on(p) ? smagorinsky(p, L, f, rho, u, p->omega) : p->omega);
     double ob = 0.0, omb = 0.0, dom = 1.0;
     switch (p->kind) {
     case K_BGK: collide_pairs_vjp(L, f, rho, u, F, om_eff, om_eff, g, cb_f, &cb_rho, cb_u, cb_F, &ob, &omb); break;
@@ -1083,7 +1100,8 @@ void orc_cell_vjp(const orc_problem *p, const double *f, double alpha, const dou
         alpha_bar -= zb * u[d];
         u_bar[d] -= zb * alpha;
     }
-    if (les_on(p))  /* psm.rs:415-473: the effective omega's cotangent into f, rho, u */
+    if (rate_bar) *rate_bar = ob + omb * dom;
+    if (rate < 0.0 && les_on(p))  /* psm.rs:415-473: the effective omega's cotangent into f, rho, u */
         smagorinsky_vjp(p, L, f, rho, u, p->omega, ob + omb * dom, cb_f, &rho_bar, u_bar);
     double j_bar[3];
     for (int d = 0; d < 3; d++) {
@@ -1154,6 +1172,163 @@ void orc_port_populations_vjp(int Q, const double *nb, int kind, const double va
 /* Problem level: initial_populations (solver.rs:593-601), interval             */
 /* (solver.rs:604-614: collide -> stream -> ports), interval_vjp (:642-674).    */
 /* ------------------------------------------------------------------------- */
+
+/* ------------------------------------------------------------------------- */
+/* WALE closure (moving/wale.rs) on BGK/TRT, restated for the stationary step:  */
+/* the rate of a fluid cell is 1 / (tau0 + 3 nu_t) with nu_t from the velocity  */
+/* gradient of u = j / rho of the state the step collides (field.rs:1294-1320), */
+/* by central differences, or one-sided ones next to walls and closed faces     */
+/* (Stencil::new / gradient).                                                   */
+/* ------------------------------------------------------------------------- */
+static int wale_on(const orc_problem *p) {
+    return p->wale_constant > 0.0 && (p->kind == K_BGK || p->kind == K_TRT);
+}
+
+/* u = j / rho per fluid cell (0 at solid cells) and rho. */
+static void wale_velocity(const orc_problem *p, const double *state, double *u, double *rho_out) {
+    const lattice_t *L = lat(p->Q);
+    long n = (long)p->nx * p->ny * p->nz;
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) {
+            u[3 * x] = u[3 * x + 1] = u[3 * x + 2] = 0.0;
+            rho_out[x] = 1.0;
+            continue;
+        }
+        double rho, j[3];
+        moments(L, state + x * p->Q, &rho, j);
+        for (int d = 0; d < 3; d++) u[3 * x + d] = j[d] / rho;
+        rho_out[x] = rho;
+    }
+}
+
+/* Neighbour of x one cell along +-axis that the stencil may use (inside or across a
+ * periodic axis, not solid); -1 if none. */
+static long wale_nb(const orc_problem *p, long x, int axis, int dir) {
+    long off[3] = {0, 0, 0};
+    off[axis] = dir;
+    if (!g_inside(p, x, off)) return -1;
+    long y = g_wrap(p, x, off);
+    return is_solid(p, y) ? -1 : y;
+}
+
+/* Difference kind along axis b at x: 0 zero, 1 central (pp, mm), 2 forward (pp), 3 backward (mm). */
+static int wale_diff(const orc_problem *p, long x, int b, long *pp, long *mm) {
+    const int n[3] = {p->nx, p->ny, p->nz};
+    *pp = *mm = -1;
+    if (is_solid(p, x) || n[b] == 1) return 0;
+    *pp = wale_nb(p, x, b, 1);
+    *mm = wale_nb(p, x, b, -1);
+    if (*pp >= 0 && *mm >= 0) return 1;
+    if (*pp >= 0) return 2;
+    if (*mm >= 0) return 3;
+    return 0;
+}
+
+/* g[a][b] = d u_a / d x_b (Stencil::gradient). */
+static void wale_gradient(const orc_problem *p, const double *u, long x, double g[3][3]) {
+    for (int b = 0; b < 3; b++) {
+        long pp, mm;
+        int kind = wale_diff(p, x, b, &pp, &mm);
+        for (int a = 0; a < 3; a++) {
+            switch (kind) {
+            case 1: g[a][b] = (u[3 * pp + a] - u[3 * mm + a]) * 0.5; break;
+            case 2: g[a][b] = u[3 * pp + a] - u[3 * x + a]; break;
+            case 3: g[a][b] = u[3 * x + a] - u[3 * mm + a]; break;
+            default: g[a][b] = 0.0; break;
+            }
+        }
+    }
+}
+
+static void wale_invariants(const double g[3][3], double sd[3][3], double s[3][3], double *a_inv, double *b_inv) {
+    double g2[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            g2[a][b] = 0.0;
+            for (int c = 0; c < 3; c++) g2[a][b] += g[a][c] * g[c][b];
+        }
+    double tr = (g2[0][0] + g2[1][1] + g2[2][2]) / 3.0;
+    *a_inv = 0.0;
+    *b_inv = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            sd[a][b] = (g2[a][b] + g2[b][a]) * 0.5 - (a == b ? tr : 0.0);
+            s[a][b] = (g[a][b] + g[b][a]) * 0.5;
+            *a_inv += sd[a][b] * sd[a][b];
+            *b_inv += s[a][b] * s[a][b];
+        }
+}
+
+/* omega = 1 / (tau0 + 3 C^2 A^(3/2) / (B^(5/2) + A^(5/4) + floor)) (wale.rs omega). */
+static double wale_omega(const orc_problem *p, const double g[3][3], double tau0) {
+    double sd[3][3], s[3][3], a, b;
+    wale_invariants(g, sd, s, &a, &b);
+    double num = pow(a, 1.5);
+    double den = pow(b, 2.5) + pow(a, 1.25) + p->wale_floor;
+    double nu = num / den * (p->wale_constant * p->wale_constant);
+    return 1.0 / (tau0 + nu * 3.0);
+}
+
+/* wale.rs omega_vjp: cotangent of the gradient given the cotangent w of omega. */
+static void wale_omega_vjp(const orc_problem *p, const double g[3][3], double tau0, double w, double g_bar[3][3]) {
+    double sd[3][3], s[3][3], a, b;
+    wale_invariants(g, sd, s, &a, &b);
+    double c2 = p->wale_constant * p->wale_constant;
+    double num = pow(a, 1.5);
+    double den = pow(b, 2.5) + pow(a, 1.25) + p->wale_floor;
+    double nu = num / den * c2;
+    double tau = tau0 + nu * 3.0;
+    double tau_bar = -w / (tau * tau);
+    double nu_bar = tau_bar * 3.0;
+    double a_bar = nu_bar * c2 * (pow(a, 0.5) * 1.5 / den - num * pow(a, 0.25) * 1.25 / (den * den));
+    double b_bar = -nu_bar * c2 * num * pow(b, 1.5) * 2.5 / (den * den);
+    for (int x = 0; x < 3; x++)
+        for (int y = 0; y < 3; y++) {
+            double da = 0.0;
+            for (int k = 0; k < 3; k++) da += sd[x][k] * g[y][k] + g[k][x] * sd[k][y];
+            g_bar[x][y] = a_bar * da * 2.0 + b_bar * s[x][y] * 2.0;
+        }
+}
+
+/* Rate of every cell of a state (p->omega at solid cells). */
+static void wale_rates(const orc_problem *p, const double *state, double *rate, double *u, double *rho) {
+    long n = (long)p->nx * p->ny * p->nz;
+    wale_velocity(p, state, u, rho);
+    const double tau0 = 1.0 / p->omega;
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x)) {
+            rate[x] = p->omega;
+            continue;
+        }
+        double g[3][3];
+        wale_gradient(p, u, x, g);
+        rate[x] = wale_omega(p, g, tau0);
+    }
+}
+
+void orc_wale_rates(const orc_problem *p, const double *state, double *rate) {
+    long n = (long)p->nx * p->ny * p->nz;
+    double *u = (double *)malloc(sizeof(double) * 3 * n), *rho = (double *)malloc(sizeof(double) * n);
+    wale_rates(p, state, rate, u, rho);
+    free(u);
+    free(rho);
+}
+
+void orc_wale_omega(const orc_problem *p, const double *g9, double tau0, double *omega_out) {
+    double g[3][3];
+    for (int a = 0; a < 9; a++) g[a / 3][a % 3] = g9[a];
+    *omega_out = wale_omega(p, g, tau0);
+}
+
+void orc_wale_omega_vjp(const orc_problem *p, const double *g9, double tau0, double w, double *g_bar9) {
+    double g[3][3], gb[3][3];
+    for (int a = 0; a < 9; a++) g[a / 3][a % 3] = g9[a];
+    wale_omega_vjp(p, g, tau0, w, gb);
+    for (int a = 0; a < 9; a++) g_bar9[a] = gb[a / 3][a % 3];
+}
+
 void orc_initial(const orc_problem *p, double *f0) {
     const lattice_t *L = lat(p->Q);
     long n = (long)p->nx * p->ny * p->nz;
@@ -1187,14 +1362,24 @@ void orc_interval(const orc_problem *p, const uint32_t *source, const double *al
                   double *post, double *after, double *exchange) {
     long n = (long)p->nx * p->ny * p->nz;
     int Q = p->Q;
+    double *rate = NULL, *wu = NULL, *wr = NULL;
+    if (wale_on(p)) {
+        rate = (double *)malloc(sizeof(double) * n);
+        wu = (double *)malloc(sizeof(double) * 3 * n);
+        wr = (double *)malloc(sizeof(double) * n);
+        wale_rates(p, before, rate, wu, wr);
+    }
 #pragma omp parallel for schedule(static)
     for (long x = 0; x < n; x++) {
         if (is_solid(p, x)) {
             for (int q = 0; q < Q; q++) post[x * Q + q] = before[x * Q + q];
             continue;
         }
-        orc_cell_collide(p, before + x * Q, alpha[x], post + x * Q);
+        cell_collide_rate(p, before + x * Q, alpha[x], rate ? rate[x] : -1.0, post + x * Q);
     }
+    free(rate);
+    free(wu);
+    free(wr);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n * Q; i++) {
         uint32_t s = source[i];
@@ -1236,6 +1421,16 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
         uint32_t s = source[i];
         if (s != NONE_U32) post_bar[s] += work[i];
     }
+    const int wale = wale_on(p);
+    double *rate = NULL, *wu = NULL, *wr = NULL, *gbar = NULL;
+    if (wale) {
+        rate = (double *)malloc(sizeof(double) * n);
+        wu = (double *)malloc(sizeof(double) * 3 * n);
+        wr = (double *)malloc(sizeof(double) * n);
+        gbar = (double *)calloc((size_t)9 * n, sizeof(double));
+        wale_rates(p, before, rate, wu, wr);
+    }
+    const double tau0 = 1.0 / p->omega;
     double g[MAXQ], fb[MAXQ];
 #pragma omp parallel for schedule(static) private(g, fb)
     for (long x = 0; x < n; x++) {
@@ -1253,11 +1448,52 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
             for (int q = 0; q < Q; q++) f_bar[x * Q + q] = 0.0;
             continue;
         }
-        double ab;
-        orc_cell_vjp(p, before + x * Q, alpha[x], g, fb, &ab);
+        double ab, rb = 0.0;
+        cell_vjp_rate(p, before + x * Q, alpha[x], wale ? rate[x] : -1.0, g, fb, &ab, wale ? &rb : NULL);
         for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
         alpha_bar[x] = ab;
+        if (wale) {  /* cotangent of the cell's velocity gradient */
+            double gr[3][3], gb[3][3];
+            wale_gradient(p, wu, x, gr);
+            wale_omega_vjp(p, gr, tau0, rb, gb);
+            for (int a = 0; a < 9; a++) gbar[9 * x + a] = gb[a / 3][a % 3];
+        }
     }
+    if (wale) {
+        /* Stencil::gradient_transpose by scatter: u_bar of every cell a gradient used */
+        double *ub = (double *)calloc((size_t)3 * n, sizeof(double));
+        for (long x = 0; x < n; x++) {
+            if (is_solid(p, x)) continue;
+            for (int b = 0; b < 3; b++) {
+                long pp, mm;
+                int kind = wale_diff(p, x, b, &pp, &mm);
+                for (int a = 0; a < 3; a++) {
+                    double v = gbar[9 * x + 3 * a + b];
+                    switch (kind) {
+                    case 1: ub[3 * pp + a] += v * 0.5; ub[3 * mm + a] -= v * 0.5; break;
+                    case 2: ub[3 * pp + a] += v; ub[3 * x + a] -= v; break;
+                    case 3: ub[3 * x + a] += v; ub[3 * mm + a] -= v; break;
+                    default: break;
+                    }
+                }
+            }
+        }
+        /* u = j / rho:  d u_a / d f_q = (c_qa - u_a) / rho */
+        const lattice_t *L = lat(Q);
+        for (long y = 0; y < n; y++) {
+            if (is_solid(p, y)) continue;
+            for (int q = 0; q < Q; q++) {
+                double acc = 0.0;
+                for (int a = 0; a < 3; a++) acc += ub[3 * y + a] * (L->cf[q][a] - wu[3 * y + a]);
+                f_bar[y * Q + q] += acc / wr[y];
+            }
+        }
+        free(ub);
+    }
+    free(rate);
+    free(wu);
+    free(wr);
+    free(gbar);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -28,6 +28,9 @@ Parity pins (each checked in tests/test_oracle_*.py):
 * NEE ports              -- equilibrium reproduction on uniform flow, FD of the VJP
 * objectives             -- FD of their gradients, closed form on uniform states
 * revolve                -- optimal_forward_steps identity and exhaustive replay
+* Smagorinsky / WALE     -- Hou et al.'s and Nicoud & Ducros' closed forms, pure-shear
+                            invariance, linear-field stencil, central differences, the
+                            dot-product identity of the step (tests/test_oracle_les.py)
 """
 from __future__ import annotations
 
@@ -85,6 +88,7 @@ class _Problem(ctypes.Structure):
         ("solid", ctypes.c_void_p),
         ("mrt_rows", ctypes.c_void_p), ("mrt_norms", ctypes.c_void_p), ("mrt_groups", ctypes.c_void_p),
         ("les_constant", ctypes.c_double), ("les_floor", ctypes.c_double),
+        ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
     ]
 
 
@@ -140,6 +144,9 @@ class Problem:
     # Smagorinsky LES on BGK/TRT (psm.rs:45-48,161-197): constant (0 = off), norm floor
     les_constant: float = 0.0
     les_floor: float = 0.0
+    # WALE closure on BGK/TRT (moving/wale.rs): constant (0 = off), denominator floor
+    wale_constant: float = 0.0
+    wale_floor: float = 1e-12
 
 
 class Oracle:
@@ -173,6 +180,7 @@ class Oracle:
         p.port_value = _ptr(self.port_value)
         p.solid = _ptr(self.solid)
         p.les_constant, p.les_floor = float(prob.les_constant), float(prob.les_floor)
+        p.wale_constant, p.wale_floor = float(prob.wale_constant), float(prob.wale_floor)
         if self._mrt is not None:
             p.mrt_rows, p.mrt_norms, p.mrt_groups = (_ptr(self._mrt[0]), _ptr(self._mrt[1]), _ptr(self._mrt[2]))
         self._p = p
@@ -257,6 +265,24 @@ class Oracle:
         lib().orc_port_populations_vjp(ctypes.c_int(self.Q), _ptr(nb), ctypes.c_int(kind), _ptr(v), _ptr(g),
                                        _ptr(out))
         return out
+
+    # --- WALE closure (moving/wale.rs) -------------------------------------------------------
+    def wale_rates(self, state):
+        rate = np.zeros(self.n)
+        lib().orc_wale_rates(self._pp, _ptr(np.ascontiguousarray(state, np.float64)), _ptr(rate))
+        return rate
+
+    def wale_omega(self, g, tau0):
+        out = ctypes.c_double(0.0)
+        lib().orc_wale_omega(self._pp, _ptr(np.ascontiguousarray(g, np.float64).reshape(9)), ctypes.c_double(tau0),
+                             ctypes.byref(out))
+        return out.value
+
+    def wale_omega_vjp(self, g, tau0, w):
+        gb = np.zeros(9)
+        lib().orc_wale_omega_vjp(self._pp, _ptr(np.ascontiguousarray(g, np.float64).reshape(9)),
+                                 ctypes.c_double(tau0), ctypes.c_double(w), _ptr(gb))
+        return gb.reshape(3, 3)
 
     # --- problem level --------------------------------------------------------------------
     def initial(self):

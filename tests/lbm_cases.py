@@ -10,18 +10,18 @@ from oracle import oracle as orc
 
 def make_pair(shape, periodic=(False, False, False), Q=19, collision="trt", precision="f64", omega=1.25,
               magic=3.0 / 16.0, accel=(0.0, 0.0, 0.0), solid=None, ports=(), bulk_rate=float("nan"),
-              even_rate=float("nan"), budget=0, les=(0.0, 0.0)):
+              even_rate=float("nan"), budget=0, les=(0.0, 0.0), wale=(0.0, 1e-12)):
     """Returns (LatticeConfig, oracle.Problem) describing one problem; les = (Smagorinsky
-    constant, norm floor)."""
+    constant, norm floor), wale = (WALE constant, denominator floor)."""
     gp = [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
     op = [orc.Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
     cfg = LatticeConfig(shape=shape, periodic=periodic, lattice=Q, collision=collision, precision=precision,
                         omega=omega, magic=magic, accel=accel, solid=solid, ports=gp, bulk_rate=bulk_rate,
                         even_rate=even_rate, checkpoint_budget_bytes=budget, les_constant=les[0],
-                        les_norm_floor=les[1])
+                        les_norm_floor=les[1], wale_constant=wale[0], wale_floor=wale[1])
     prob = orc.Problem(shape=shape, periodic=periodic, Q=Q, kind=collision, omega=omega, magic=magic,
                        bulk_rate=bulk_rate, even_rate=even_rate, accel=accel, solid=solid, ports=op,
-                       les_constant=les[0], les_floor=les[1])
+                       les_constant=les[0], les_floor=les[1], wale_constant=wale[0], wale_floor=wale[1])
     return cfg, prob
 
 
