@@ -187,6 +187,7 @@ struct lbm_handle {
     int Q = 19, kind = K_TRT, prec = LBM_F32, lprec = LBM_F32;
     bool regularized = false;
     double les_c = 0.0, les_floor = 0.0;  // Smagorinsky closure (K_TRT_LES / K_BGK_LES)
+    double wale_c2 = 0.0, wale_floor = 0.0;  // WALE closure (K_TRT_WALE / K_BGK_WALE)
     uint64_t every = 0;  // Checkpointing::Every(k); 0 = binomial
     int nx = 0, ny = 0, nz = 0;
     int per[3] = {0, 0, 0};
@@ -219,6 +220,7 @@ struct lbm_handle {
     size_t tlam_elem = 4;  // bytes of one temperature cotangent
     long long n_tsolid = 0;  // cells with phi > 1/2
     DevBuf phi, phi_bar, ubar;  // ubar: velocity cotangent handed from the thermal to the flow adjoint
+    DevBuf wu, wgb;             // WALE: u and rho of the state a step reads; gradient cotangents
     const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
@@ -242,6 +244,7 @@ struct lbm_handle {
         s.mrt = mrt.p;
         s.odd = int(step & 1);
         s.step = uint32_t(step);
+        s.wu = wu.p;
         if (thermal) {  // the flow kernel advances the temperature field in the same pass
             s.tpop = tpop(static_cast<const void*>(pop));
             s.tpop_out = tpop(pop);
@@ -501,6 +504,7 @@ void ensure_adjoint_buffers(lbm_handle* h) {
         h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
         h->ubar.alloc(size_t(h->ncell) * 3 * h->csize(), "velocity cotangent");
     }
+    if (is_wale(h->kind)) h->wgb.alloc(size_t(h->ncell) * 9 * h->csize(), "WALE gradient cotangents");
 }
 
 // Snapshot slots for a horizon: min(budget / state bytes, nsteps); the pool only grows.
@@ -550,6 +554,22 @@ struct MassLedger {
     }
 };
 
+// The flow's transposed step on the cotangent buffer (a.pop and a.odd name the state);
+// with the WALE closure, the velocity pre-pass before it and the gradient transpose after.
+void flow_adjoint(lbm_handle* h, AdjArgs& a, hipEvent_t after = nullptr) {
+    if (is_wale(h->kind)) {
+        hip_check(h->run->wale_u(h->Q, h->has_mask, h->g, a.pop, h->mask.as<uint32_t>(), a.odd, h->wu.p, h->cfg()),
+                  "WALE velocity launch");
+        a.wu = h->wu.p;
+        a.wgb = h->wgb.p;
+    }
+    hip_check(h->run->adjoint(h->Q, h->kind, h->has_mask, a, h->cfg(), after), "adjoint launch");
+    if (is_wale(h->kind))
+        hip_check(h->run->wale_transpose(h->Q, h->has_mask, h->g, h->mask.as<uint32_t>(), h->wu.p, h->wgb.p, a.lam,
+                                         a.odd, h->cfg()),
+                  "WALE transpose launch");
+}
+
 // One forward step of state `step` read from `src`, written to `dst`; with a ledger,
 // the step records the mass of the state it reads and the port exchange it makes.
 void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs& obj, const MassLedger* led,
@@ -565,6 +585,10 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
     }
     if (obj.kind != OBJ_NONE || obj.tw != 0.0 || diag) s.part = h->part.as<double>();
     if (diag) s.macc = led->mass(s_index);
+    if (is_wale(h->kind))  // velocities of the state read, before the in-place step overwrites them
+        hip_check(h->run->wale_u(h->Q, h->has_mask, h->g, src, h->mask.as<uint32_t>(), int(step & 1), h->wu.p,
+                                 h->cfg()),
+                  "WALE velocity launch");
     if (kspan) hip_check(hipEventRecord(kspan->first, h->stream), "event record");
     hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg(), kspan ? kspan->second : nullptr),
               "forward step launch");
@@ -677,7 +701,9 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             if (cfg->les_constant > 0.0) fail(LBM_ERR_CONTRACT, "choose one closure: Smagorinsky or WALE");
             if (h->kind != K_TRT && h->kind != K_BGK)
                 fail(LBM_ERR_CONTRACT, "the WALE closure applies to the BGK and TRT operators");
-            fail(LBM_ERR_CONTRACT, "the WALE closure is not available on the device yet");
+            h->kind = h->kind == K_TRT ? int(K_TRT_WALE) : int(K_BGK_WALE);
+            h->wale_c2 = cfg->wale_constant * cfg->wale_constant;
+            h->wale_floor = cfg->wale_floor;
         }
         if (cfg->les_constant > 0.0) {
             if (h->kind != K_TRT && h->kind != K_BGK)
@@ -708,6 +734,12 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             h->lprec = LBM_F16_SHIFTED;
             break;
         default: fail(LBM_ERR_CONTRACT, "unknown adjoint storage");
+        }
+        if (is_wale(h->kind)) {  // the gradient transpose adds into the cotangent: FP32/FP64 only
+            if (cfg->adjoint_storage == LBM_ADJ_F16_SCALED)
+                fail(LBM_ERR_CONTRACT, "the WALE closure keeps FP32 cotangents");
+            if (h->lprec == LBM_F16_SHIFTED) h->lprec = LBM_F32;
+            if (cfg->thermal) fail(LBM_ERR_CONTRACT, "the WALE closure and the temperature field are not combined");
         }
         h->thermal = cfg->thermal != 0;
         if (cfg->thermal > 1) fail(LBM_ERR_CONTRACT, "thermal must be 0 or 1");
@@ -755,7 +787,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (!(h->tau >= 0.51 && h->tau <= 2.0))
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
-        const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES;
+        const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES || h->kind == K_TRT_WALE;
         if (trt_like && !(std::isfinite(h->magic) && h->magic > 0.0))
             fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
         h->om = trt_like ? odd_rate(h->omega, h->magic) : h->omega;
@@ -876,6 +908,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->alpha.alloc(size_t(h->ncell) * h->csize(), "alpha");
         hip_check(hipMemsetAsync(h->alpha.p, 0, h->alpha.bytes, h->stream), "memset");
         h->h_alpha.assign(h->ncell, 0.0);
+        if (is_wale(h->kind)) h->wu.alloc(size_t(h->ncell) * 4 * h->csize(), "WALE velocities");
         if (h->thermal) {
             h->phi.alloc(size_t(h->ncell) * h->csize(), "thermal design");
             hip_check(hipMemsetAsync(h->phi.p, 0, h->phi.bytes, h->stream), "memset");
@@ -919,9 +952,12 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             hip_check(hipMemcpy(h->mask.p, m.data(), h->mask.bytes, hipMemcpyHostToDevice), "mask upload");
         }
         // Collision constants.
-        if (h->kind == K_MRT || h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
+        if (h->kind == K_MRT || h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES ||
+            is_wale(h->kind)) {
             std::vector<double> c;
-            if (h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
+            if (is_wale(h->kind)) {
+                c = {h->magic, h->wale_c2, h->wale_floor};
+            } else if (h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
                 // [magic, k = 18 sqrt(2) C^2 (psm.rs:193 evaluation order), floor]
                 c = {h->magic, 18.0 * 1.4142135623730951 * h->les_c * h->les_c, h->les_floor};
             } else if (h->kind == K_MRT) {
@@ -1437,8 +1473,8 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
                                        h->stream),
                       "port vjp");
         hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
-        const AdjArgs a = h->adj_args(h->F.p, int(t & 1));
-        hip_check(h->run->adjoint(Q, h->kind, h->has_mask, a, h->cfg()), "adjoint launch");
+        AdjArgs a = h->adj_args(h->F.p, int(t & 1));
+        flow_adjoint(h, a);
         hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(t & 1), h->aos.as<double>(),
                                   h->lsc_ptr(), h->cfg()),
                   "get_lam");
@@ -1617,8 +1653,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                 {
                     const auto* ks = kspan(adj_k, rsteps);
                     if (ks) hip_check(hipEventRecord(ks->first, h->stream), "event record");
-                    hip_check(h->run->adjoint(Q, h->kind, h->has_mask, aa, h->cfg(), ks ? ks->second : nullptr),
-                              "adjoint launch");
+                    flow_adjoint(h, aa, ks ? ks->second : nullptr);
                 }
                 rsteps++;
                 if (h->np && k > 1)
@@ -1739,6 +1774,10 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
                 if (which != 3) sa.tpop = nullptr;
                 return sa;
             };
+            if (is_wale(h->kind))  // velocities the WALE kernel reads (the pre-pass itself is not timed)
+                hip_check(h->run->wale_u(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1),
+                                         h->wu.p, h->cfg()),
+                          "WALE velocity launch");
             {  // untimed first launch (code object load)
                 StepArgs sa = args(h->t);
                 hip_check(h->run->forward(h->Q, kind, h->has_mask, sa, h->cfg()), "forward launch");
@@ -1758,11 +1797,11 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             lam_init(h, 0.0, 0.0);
             AdjArgs a = h->adj_args(h->F.p, 0);
             a.odd = int(h->t & 1);
-            hip_check(h->run->adjoint(h->Q, h->kind, h->has_mask, a, h->cfg()), "adjoint launch");  // untimed
+            flow_adjoint(h, a);  // untimed
             hip_check(hipEventRecord(h->ev0, h->stream), "event");
             for (uint64_t s = 0; s < launches; s++) {
                 a.odd = int((h->t + s) & 1);
-                hip_check(h->run->adjoint(h->Q, h->kind, h->has_mask, a, h->cfg()), "adjoint launch");
+                flow_adjoint(h, a);
             }
             hip_check(hipEventRecord(h->ev1, h->stream), "event");
         }

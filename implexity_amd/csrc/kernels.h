@@ -48,10 +48,13 @@ namespace lbm {
 
 enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
 // K_TRT_LES / K_BGK_LES: TRT / BGK with the Smagorinsky closure (les.h); their constants
-// [magic, k, floor] travel in the collision-constant array (`mrt`).
+// [magic, k, floor] travel in the collision-constant array (`mrt`).  K_TRT_WALE /
+// K_BGK_WALE: TRT / BGK with the WALE closure (wale.h), constants [magic, C^2, floor].
 enum Kind : int {
-    K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7
+    K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7,
+    K_TRT_WALE = 8, K_BGK_WALE = 9
 };
+constexpr bool is_wale(int kind) { return kind == K_TRT_WALE || kind == K_BGK_WALE; }
 enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
@@ -103,6 +106,7 @@ struct StepArgs {
     const void* phi;
     double kappa, ratio_m1, source, tw;
     float tenc, tdec;
+    const void* wu;   // WALE: u (3 x cells) and rho (cells) of the state read, arithmetic precision
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
     long long npart;  // capacity of part, in blocks
@@ -137,6 +141,10 @@ struct AdjArgs {
     // Coupled temperature field: cotangent of the Brinkman velocity from the thermal
     // step (SoA, 3 x cells, arithmetic precision), added through u(f, alpha); else null.
     const void* ubar;
+    // WALE: u and rho of the state read (as StepArgs::wu) and the cotangent of every
+    // cell's velocity gradient written for k_wale_transpose (9 x cells, G[a][b] at 3a+b).
+    const void* wu;
+    void* wgb;
 };
 
 // 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
@@ -540,6 +548,7 @@ LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cur
 }
 
 #include "thermal_cell.h"
+#include "wale.h"
 
 // ---------------------------------------------------------------------------------
 // Reductions.
@@ -867,6 +876,12 @@ LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, co
                 collide_pairs_shifted<Q>(v, m, oe, oo, out);
             else
                 collide_pairs<Q>(v, m.rho, m.u, m.F, oe, oo, out);
+        } else if constexpr (is_wale(KIND)) {  // omega is the cell's WALE rate
+            const C oo = KIND == K_TRT_WALE ? odd_rate_of(omega, mrt[0]) : omega;
+            if constexpr (SH)
+                collide_pairs_shifted<Q>(v, m, omega, oo, out);
+            else
+                collide_pairs<Q>(v, m.rho, m.u, m.F, omega, oo, out);
         } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (SH)
                 collide_pairs_shifted<Q>(v, m, omega, KIND == K_BGK ? omega : om, out);
@@ -886,9 +901,10 @@ LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, co
     }
 }
 
+// WALE kinds: omega is the cell's rate and *rate_bar receives its cotangent.
 template <int Q, int KIND, class C, bool SH>
 LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau, const C* mrt, const C* g,
-                         Macro<Q, C>& m, C* fb, C& ab) {
+                         Macro<Q, C>& m, C* fb, C& ab, C* rate_bar = nullptr) {
     macro_of<Q, C, SH>(v, alpha, a, m);
     if constexpr (KIND == K_BGK_GUO) {
         collide_bgk_guo_vjp(m, alpha, a, tau, g, fb, ab);
@@ -907,6 +923,17 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
             C ob, omb;
             pairs_omega_bar<Q, C>(e.n, m, g, ob, omb);
             smagorinsky_vjp<Q, C>(e, m, mrt, ob + omb * dom, fb, cb_rho, cb_u);
+        } else if constexpr (is_wale(KIND)) {
+            const C oo = KIND == K_TRT_WALE ? odd_rate_of(omega, mrt[0]) : omega;
+            const C dom = KIND == K_TRT_WALE ? odd_rate_derivative_of(omega, mrt[0]) : C(1);
+            if constexpr (std::is_same<C, float>::value)
+                collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
+            else
+                collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
+            C n[Q], ob, omb;
+            nonequilibrium<Q, C, SH>(v, m, n);
+            pairs_omega_bar<Q, C>(n, m, g, ob, omb);
+            *rate_bar = ob + omb * dom;
         } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,
@@ -951,8 +978,15 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             const C alpha = static_cast<const C*>(A.alpha)[cell];
             const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
             Macro<Q, C> mac;
-            collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                            static_cast<const C*>(A.mrt), fs, mac);
+            C rate = C(A.omega);
+            if constexpr (is_wale(KIND)) {
+                C G[3][3];
+                wale_gradient<C, MASK>(g, A.mask, static_cast<const C*>(A.wu), (long long)g.nx * g.ny * g.nz, i, j,
+                                       k, cell, G);
+                rate = wale_omega<C>(G, C(1) / C(A.omega), static_cast<const C*>(A.mrt));
+            }
+            collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt), fs,
+                                            mac);
             if (A.obj.kind != OBJ_NONE) {
                 const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
                 jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
@@ -1056,8 +1090,22 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
             const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
             Macro<Q, C> mac;
             C ab;
-            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
+            if constexpr (is_wale(KIND)) {
+                const long long n = (long long)g.nx * g.ny * g.nz;
+                const C* wk = static_cast<const C*>(A.mrt);
+                const C tau0 = C(1) / C(A.omega);
+                C G[3][3], Gb[3][3], rb;
+                wale_gradient<C, MASK>(g, A.mask, static_cast<const C*>(A.wu), n, i, j, k, cell, G);
+                const C rate = wale_omega<C>(G, tau0, wk);
+                collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), wk, gb, mac, fb, ab, &rb);
+                wale_omega_vjp<C>(G, tau0, wk, rb, Gb);
+                C* W = static_cast<C*>(A.wgb);
+#pragma unroll
+                for (int t = 0; t < 9; t++) W[t * n + cell] = Gb[t / 3][t % 3];
+            } else {
+                collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
+                                                    static_cast<const C*>(A.mrt), gb, mac, fb, ab);
+            }
             if (A.obj.kind != OBJ_NONE) {
                 const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
                 objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
@@ -1086,6 +1134,110 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     if constexpr (LP == P_F16S) block_raise_max(lmax, A.lsc);
 }
 
+
+// ---------------------------------------------------------------------------------
+// WALE pre-pass and transpose (wale.h).
+// ---------------------------------------------------------------------------------
+// U[a * n + cell] = u_a = j_a / rho and U[3n + cell] = rho of the state of parity odd
+// (u = 0, rho = 1 at solid cells); field.rs:1298-1308.
+template <int P, int Q, bool MASK>
+__global__ __launch_bounds__(256) void k_wale_u(Geom g, const void* pop_, const uint32_t* mask, int odd, void* U_) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    const long long cell = ((long long)i * g.ny + j) * g.nz + k;
+    C* U = static_cast<C*>(U_);
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
+    if (m & SOLID_BIT) {
+        U[cell] = U[n + cell] = U[2 * n + cell] = C(0);
+        U[3 * n + cell] = C(1);
+        return;
+    }
+    const Cursor cu = make_cursor(g, i, j, k);
+    C f[Q], rho, jv[3];
+    load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
+    moments<Q>(f, rho, jv);
+    if constexpr (S::shifted) rho = C(1) + rho;
+#pragma unroll
+    for (int d = 0; d < 3; d++) U[d * n + cell] = jv[d] / rho;
+    U[3 * n + cell] = rho;
+}
+
+// Transpose of the gradients (Stencil::gradient_transpose): every fluid cell y gathers
+// the gradient cotangents GB of the cells whose stencil used u_y (itself and its
+// neighbours along each axis) into u_bar, then adds (du/df)^T u_bar,
+// d u_a / d f_q = (c_qa - u_a) / rho, to its cotangent at its read locations (state of
+// parity odd, after the adjoint kernel wrote it).
+template <int P, int LP, int Q, bool MASK>
+__global__ __launch_bounds__(256) void k_wale_transpose(Geom g, const uint32_t* mask, const void* U_, const void* GB_,
+                                                        void* lam_, int odd) {
+    using C = typename Store<P>::C;
+    using L = LamStore<LP>;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    const long long y = ((long long)i * g.ny + j) * g.nz + k;
+    const uint32_t m = cell_mask<Q, MASK>(g, mask, y, i, j, k);
+    if (m & SOLID_BIT) return;
+    const C* U = static_cast<const C*>(U_);
+    const C* GB = static_cast<const C*>(GB_);
+    C ub[3] = {C(0), C(0), C(0)};
+    auto add = [&](long long x, int b, C coef) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) ub[a] += GB[(3 * a + b) * n + x] * coef;
+    };
+    const int ijk[3] = {i, j, k};
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        // candidates: y itself and its neighbours along b (each cell once)
+        long long cand[3];
+        int nc = 0;
+        cand[nc++] = y;
+        const long long xp = wale_nb<MASK>(g, mask, i, j, k, b, 1);
+        const long long xm = wale_nb<MASK>(g, mask, i, j, k, b, -1);
+        if (xp >= 0 && xp != y) cand[nc++] = xp;
+        if (xm >= 0 && xm != y && xm != xp) cand[nc++] = xm;
+        for (int t = 0; t < nc; t++) {
+            const long long x = cand[t];
+            int c3[3] = {ijk[0], ijk[1], ijk[2]};
+            if (x != y) {  // coordinates of the neighbour
+                const int nb3[3] = {g.nx, g.ny, g.nz};
+                c3[b] += (x == xp) ? 1 : -1;
+                if (c3[b] < 0) c3[b] += nb3[b];
+                if (c3[b] >= nb3[b]) c3[b] -= nb3[b];
+            }
+            long long pp, mm;
+            const int kind = wale_diff<MASK>(g, mask, c3[0], c3[1], c3[2], b, pp, mm);
+            if (kind == 1) {
+                if (pp == y) add(x, b, C(0.5));
+                if (mm == y) add(x, b, C(-0.5));
+            } else if (kind == 2) {
+                if (pp == y) add(x, b, C(1));
+                if (x == y) add(x, b, C(-1));
+            } else if (kind == 3) {
+                if (x == y) add(x, b, C(1));
+                if (mm == y) add(x, b, C(-1));
+            }
+        }
+    }
+    if (ub[0] == C(0) && ub[1] == C(0) && ub[2] == C(0)) return;
+    const C rho = U[3 * n + y];
+    const C u[3] = {U[y], U[n + y], U[2 * n + y]};
+    const Cursor cu = make_cursor(g, i, j, k);
+    typename L::T* lam = static_cast<typename L::T*>(lam_);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        C acc = C(0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) acc += ub[a] * (C(cvec<Q>(q, a)) - u[a]);
+        typename L::T& slot = at_read<Q, q>(lam, g, cu, m, odd);
+        slot = L::template enc<C>(L::template ld<C>(&slot) + acc / rho);
+    });
+}
 
 // ---------------------------------------------------------------------------------
 // Vectorised steps: each thread updates V consecutive cells along z with V-wide loads

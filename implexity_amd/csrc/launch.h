@@ -49,6 +49,12 @@ struct Runner {
     virtual hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd,
                               const void* alpha, const double a[3], double* rho, double* u,
                               const LaunchCfg& c) const = 0;
+    // WALE closure: velocity pre-pass of a state, and the transpose of the gradients into
+    // the flow cotangent of that state.
+    virtual hipError_t wale_u(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, void* U,
+                              const LaunchCfg& c) const = 0;
+    virtual hipError_t wale_transpose(int Q, bool mask, const Geom& g, const uint32_t* m, const void* U,
+                                      const void* GB, void* lam, int odd, const LaunchCfg& c) const = 0;
 };
 
 const Runner* runner_f64();
@@ -157,6 +163,8 @@ struct RunnerImpl final : Runner {
             if (kind == K_MRT) LBM_LAUNCH((k_forward<P, Q, K_MRT, M>), c, a);
             if (kind == K_TRT_LES) LBM_LAUNCH((k_forward<P, Q, K_TRT_LES, M>), c, a);
             if (kind == K_BGK_LES) LBM_LAUNCH((k_forward<P, Q, K_BGK_LES, M>), c, a);
+            if (kind == K_TRT_WALE && a.wu) LBM_LAUNCH((k_forward<P, Q, K_TRT_WALE, M>), c, a);
+            if (kind == K_BGK_WALE && a.wu) LBM_LAUNCH((k_forward<P, Q, K_BGK_WALE, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward<P, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -207,6 +215,10 @@ struct RunnerImpl final : Runner {
             if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, M>), c, a);
             if (kind == K_TRT_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_LES, M>), c, a);
             if (kind == K_BGK_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_LES, M>), c, a);
+            if constexpr (LP != P_F16S) {
+                if (kind == K_TRT_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_WALE, M>), c, a);
+                if (kind == K_BGK_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_WALE, M>), c, a);
+            }
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_adjoint<P, LP, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -223,7 +235,9 @@ struct RunnerImpl final : Runner {
         if (a.tpop && (!a.tpop_out || !a.phi)) return false;
         if (a.tw != 0.0 && (!a.part || !a.jacc)) return false;
         if (mask && !a.mask) return false;
-        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
+            return false;
+        if (is_wale(kind) && !a.wu) return false;
         if ((a.obj.kind != OBJ_NONE || a.diag) && (!a.part || !a.jacc)) return false;
         if (a.diag && (!a.dstat || !a.macc)) return false;
         return true;
@@ -232,7 +246,9 @@ struct RunnerImpl final : Runner {
         if (!a.pop || !a.lam || !a.alpha || !a.alpha_bar) return false;
         if (LP == P_F16S && !a.lsc) return false;
         if (mask && !a.mask) return false;
-        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt) return false;
+        if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
+            return false;
+        if (is_wale(kind) && (!a.wu || !a.wgb)) return false;
         return true;
     }
     static bool ok_ports(bool mask, const PortArgs& a) {
@@ -398,6 +414,30 @@ struct RunnerImpl final : Runner {
         }
         if (mask) LBM_LAUNCH((k_vector_vjp<P, 27, true>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
         LBM_LAUNCH((k_vector_vjp<P, 27, false>), c, g, pop, m, odd, alpha, a[0], a[1], a[2], ob, f_bar, alpha_bar);
+    }
+    hipError_t wale_u(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, void* U,
+                      const LaunchCfg& c) const override {
+        if (!pop || !U || (mask && !m)) return hipErrorInvalidValue;
+        if (Q == 19) {
+            if (mask) LBM_LAUNCH((k_wale_u<P, 19, true>), c, g, pop, m, odd, U);
+            LBM_LAUNCH((k_wale_u<P, 19, false>), c, g, pop, m, odd, U);
+        }
+        if (mask) LBM_LAUNCH((k_wale_u<P, 27, true>), c, g, pop, m, odd, U);
+        LBM_LAUNCH((k_wale_u<P, 27, false>), c, g, pop, m, odd, U);
+    }
+    hipError_t wale_transpose(int Q, bool mask, const Geom& g, const uint32_t* m, const void* U, const void* GB,
+                              void* lam, int odd, const LaunchCfg& c) const override {
+        if (!U || !GB || !lam || (mask && !m)) return hipErrorInvalidValue;
+        if constexpr (LP == P_F16S) {
+            return hipErrorInvalidValue;  // WALE keeps FP32/FP64 cotangents
+        } else {
+            if (Q == 19) {
+                if (mask) LBM_LAUNCH((k_wale_transpose<P, LP, 19, true>), c, g, m, U, GB, lam, odd);
+                LBM_LAUNCH((k_wale_transpose<P, LP, 19, false>), c, g, m, U, GB, lam, odd);
+            }
+            if (mask) LBM_LAUNCH((k_wale_transpose<P, LP, 27, true>), c, g, m, U, GB, lam, odd);
+            LBM_LAUNCH((k_wale_transpose<P, LP, 27, false>), c, g, m, U, GB, lam, odd);
+        }
     }
     hipError_t fields(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, const void* alpha,
                       const double a[3], double* rho, double* u, const LaunchCfg& c) const override {

@@ -79,6 +79,10 @@ def _create(**kw):
     (dict(thermal=True, thermal_kappa=0.02, solid=np.eye(4, dtype=bool)[None].repeat(4, 0)), "no solid cells"),
     (dict(les_constant=0.1, collision="mrt"), "Smagorinsky closure applies"),
     (dict(les_constant=-0.1), "non-negative"),
+    (dict(wale_constant=0.5, collision="mrt"), "WALE closure applies"),
+    (dict(wale_constant=0.5, les_constant=0.1), "choose one closure"),
+    (dict(wale_constant=0.5, wale_floor=0.0), "denominator_floor"),
+    (dict(wale_constant=0.5, precision="f16s", adjoint_storage="f16"), "FP32 cotangents"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are

@@ -1,9 +1,10 @@
-"""GPU parity of the Smagorinsky closure (moving/psm.rs:161-197, 415-473) on BGK/TRT.
+"""GPU parity of the Smagorinsky (moving/psm.rs:161-197, 415-473) and WALE
+(moving/wale.rs) closures on BGK/TRT.
 
 Bars: FP64 forward states bit-identical to the oracle (D3Q19 and D3Q27, solids, ports,
 periodic axes, with the coupled temperature field); the FP64 step VJP within 1e-12 and
 the FP64 gradient within 1e-11 of the oracle; FP32 / FP16 states within 2e-5 / 3e-3 of
-the deviation scale after 20 steps.
+the deviation scale after 20 steps.  WALE's FP64 states within 1e-14 (its powers).
 """
 from __future__ import annotations
 
@@ -118,3 +119,95 @@ def test_les_with_temperature_field_f64_bit_exact():
         lat.interval(9)
         f, g = lat.state(), lat.thermal_state()
     assert np.array_equal(f, f_want) and np.array_equal(g, g_want)
+
+
+# --- WALE (moving/wale.rs) -------------------------------------------------------------------
+WALE = (0.5, 1e-12)
+
+WFWD = [
+    # shape, periodic, Q, collision, with solid, with ports, accel
+    ((7, 6, 5), (True, False, False), 19, "trt", True, False, (2e-4, 0.0, 0.0)),
+    ((7, 6, 8), (False, False, False), 19, "trt", False, True, (0.0, 0.0, 0.0)),
+    ((6, 5, 7), (True, True, True), 19, "bgk", True, False, (2e-4, 1e-5, 0.0)),
+    ((6, 2, 4), (True, True, False), 27, "trt", False, False, (2e-4, 0.0, 0.0)),
+]
+
+
+def _wcase(shape, periodic, Q, coll, with_solid, with_ports, accel, precision="f64"):
+    solid = random_solid(shape, 5) if with_solid else None
+    ports = channel_ports(shape, 0.06) if with_ports else ()
+    return make_pair(shape, periodic, Q=Q, collision=coll, precision=precision, omega=1.8, accel=accel,
+                     solid=solid, ports=ports, wale=WALE), solid
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,accel", WFWD)
+def test_wale_forward_f64(shape, periodic, Q, coll, with_solid, with_ports, accel):
+    """FP64 states against the oracle: the closure's powers are the device's and the host
+    C library's pow, each within an ulp, so the bar is 1e-14 of the deviation scale
+    (every other operation is in the oracle's order)."""
+    (cfg, prob), _ = _wcase(shape, periodic, Q, coll, with_solid, with_ports, accel)
+    alpha = random_alpha(shape, 7)
+    o = orc.Oracle(prob)
+    want = o.run(alpha, 14)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        lat.interval(14)
+        got = lat.state()
+    _, w, _ = orc.lattice(Q)
+    dev = np.abs(want.reshape(-1, Q) - w).max()
+    assert np.abs(got - want).max() <= 1e-14 * dev, np.abs(got - want).max() / dev
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,accel", WFWD)
+def test_wale_step_vjp_f64(shape, periodic, Q, coll, with_solid, with_ports, accel):
+    (cfg, prob), solid = _wcase(shape, periodic, Q, coll, with_solid, with_ports, accel)
+    alpha = random_alpha(shape, 9)
+    o = orc.Oracle(prob)
+    f = o.run(alpha, 6)
+    lam = np.random.default_rng(1).standard_normal(o.n * Q)
+    if solid is not None:
+        lam.reshape(-1, Q)[solid.reshape(-1)] = 0.0
+    want_f, want_a = o.interval_vjp(alpha, f, lam)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        got_f, got_a = lat.interval_vjp(lam)
+    fluid = np.ones(o.n, bool) if solid is None else ~solid.reshape(-1)
+    gf, wf = got_f.reshape(-1, Q)[fluid], want_f.reshape(-1, Q)[fluid]
+    assert np.abs(gf - wf).max() <= 1e-12 * np.abs(wf).max(), np.abs(gf - wf).max() / np.abs(wf).max()
+    assert np.abs(got_a - want_a).max() <= 1e-12 * np.abs(want_a).max()
+
+
+def test_wale_gradient_f64_matches_oracle():
+    shape, steps = (8, 6, 5), 20
+    (cfg, prob), _ = _wcase(shape, (False, False, False), 19, "trt", False, True, (0.0, 0.0, 0.0))
+    alpha = random_alpha(shape, 11)
+    ob = Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=10, t_end=steps,
+                   weight=1.0 / 11)
+    oob = orc.Objective(kind="pressure_drop", window=(10, steps), weight=1.0 / 11, axis=0, plane_a=1,
+                        plane_b=shape[0] - 2)
+    J_want, ab_want, _, _ = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 2e-5), ("f16s", 3e-3)])
+def test_wale_forward_reduced_precision(precision, tol):
+    shape = (8, 6, 8)
+    (cfg, prob), _ = _wcase(shape, (True, False, False), 19, "trt", False, False, (2e-4, 0.0, 0.0), precision)
+    alpha = random_alpha(shape, 13)
+    want = orc.Oracle(prob).run(alpha, 20)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        lat.interval(20)
+        got = lat.state()
+    _, w, _ = orc.lattice(19)
+    dev = np.abs(want.reshape(-1, 19) - w).max()
+    err = np.abs(got - want).max() / dev
+    assert err <= tol, err
