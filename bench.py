@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--les", type=float, default=0.0,
                     help="kernels mode: Smagorinsky constant of the timed operator (0: plain TRT)")
+    ap.add_argument("--wale", type=float, default=0.0,
+                    help="kernels mode: WALE constant of the timed operator (0: plain TRT)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the lattice step runs on one "
@@ -244,6 +246,9 @@ def main():
 
         cfg, alpha = alpha_cfg(args.kernel_precision)
         cfg.les_constant = args.les
+        cfg.wale_constant = args.wale
+        if args.wale and cfg.adjoint_storage == "f16":
+            cfg.adjoint_storage = "auto"  # WALE keeps FP32 cotangents
         with Lattice(cfg) as lat:
             lat.set_design(alpha)
             lat.initial_populations()
@@ -254,7 +259,8 @@ def main():
 
         copy_gbs, copy_ms = copy_bandwidth(1 << 30, args.launches)  # 1 GiB read + 1 GiB written
         n = int(np.prod(shape))
-        out = {"precision": args.kernel_precision, "les_constant": args.les, "forward_ms": tf, "adjoint_ms": ta,
+        out = {"precision": args.kernel_precision, "les_constant": args.les, "wale_constant": args.wale,
+               "forward_ms": tf, "adjoint_ms": ta,
                "stream_only_ms": ts,
                "forward_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (tf * 1e-3) / 1e9,
                "adjoint_GBs": algorithmic_bytes("adjoint", args.kernel_precision) * n / (ta * 1e-3) / 1e9,
