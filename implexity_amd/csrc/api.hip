@@ -221,6 +221,11 @@ struct lbm_handle {
     long long n_tsolid = 0;  // cells with phi > 1/2
     DevBuf phi, phi_bar, ubar;  // ubar: velocity cotangent handed from the thermal to the flow adjoint
     DevBuf wu, wgb;             // WALE: u and rho of the state a step reads; gradient cotangents
+    // Sponge layers: the specifications (far-field targets are formed from them for every
+    // step) and, on the device, the covered fluid cells with their summed strength and parts.
+    std::vector<lbm_sponge> sponges;
+    int nsc = 0;
+    DevBuf sp_cell, sp_sigma, sp_off, sp_part, sp_weight, sp_phase;
     const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
@@ -264,6 +269,44 @@ struct lbm_handle {
         s.jacc = scal.as<double>();
         s.macc = scal.as<double>() + 1;
         s.dstat = reinterpret_cast<unsigned*>(scal.as<double>() + 4);
+        return s;
+    }
+    // Sponge arguments of the step that reads state `step`: the slots it writes (parity
+    // step & 1) and the far-field targets at time t = step (field.rs:1015-1036 with unit
+    // spacing and step; the signal's ramp as boundary.rs:110-118).
+    SpongeArgs sponge_args(uint64_t step) const {
+        SpongeArgs s{};
+        s.g = g;
+        s.n = nsc;
+        s.ns = int(sponges.size());
+        s.odd = int(step & 1);
+        s.cell = sp_cell.as<int>();
+        s.sigma = sp_sigma.as<double>();
+        s.off = sp_off.as<int>();
+        s.part = sp_part.as<int>();
+        s.weight = sp_weight.as<double>();
+        s.phase = sp_phase.as<double>();
+        const double t = double(step);
+        for (size_t l = 0; l < sponges.size(); l++) {
+            const lbm_sponge& sp = sponges[l];
+            double* r = s.target[l];
+            r[0] = sp.density;
+            for (int d = 0; d < 3; d++) r[1 + d] = sp.velocity[d];
+            if (sp.wave) {
+                double ramp = 1.0;
+                if (sp.wave_ramp > 0.0 && t < sp.wave_ramp) {
+                    double x = t / sp.wave_ramp;
+                    if (x < 0.0) x = 0.0;
+                    ramp = x * x * x * (10.0 - x * 15.0 + x * x * 6.0);
+                }
+                const double arg = t * (2.0 * M_PI * sp.wave_frequency) + sp.wave_phase;
+                const double sw = ramp * std::sin(arg), cw = ramp * std::cos(arg);
+                for (int d = 0; d < 3; d++) {
+                    r[4 + d] = sw * sp.wave_amplitude[d];
+                    r[7 + d] = cw * sp.wave_amplitude[d];
+                }
+            }
+        }
         return s;
     }
     void* tpop(void* buf) const { return static_cast<char*>(buf) + flow_bytes; }
@@ -524,16 +567,18 @@ uint64_t reserve_pool(lbm_handle* h, uint64_t nsteps) {
     return slots;
 }
 
-// Fluid mass of every state of a sweep and the mass the ports exchange in every step
-// (one partial per port block), for the admission drift
+// Fluid mass of every state of a sweep and the mass the ports and sponges exchange in
+// every step (one partial per port or sponge block), for the admission drift
 // |M_s - M_0 - sum_{r<s} X_r| / M_0 (solver.rs:707-793).
 struct MassLedger {
     DevBuf buf;
     uint64_t n = 0;
-    long long nbp = 0;  // port blocks
+    long long nbp = 0;  // port blocks, then sponge blocks
+    long long npb = 0;  // port blocks
     void init(lbm_handle* h, uint64_t nsteps) {
         n = nsteps;
-        nbp = (h->np + PORT_BLOCK - 1) / PORT_BLOCK;
+        npb = (h->np + PORT_BLOCK - 1) / PORT_BLOCK;
+        nbp = npb + (h->nsc + SPONGE_BLOCK - 1) / SPONGE_BLOCK;
         buf.alloc(((n + 2) + (n + 2) * size_t(nbp)) * sizeof(double), "mass ledger");
         hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream), "memset");
     }
@@ -554,9 +599,15 @@ struct MassLedger {
     }
 };
 
-// The flow's transposed step on the cotangent buffer (a.pop and a.odd name the state);
-// with the WALE closure, the velocity pre-pass before it and the gradient transpose after.
+// The flow's transposed step on the cotangent buffer (a.pop and a.odd name the state):
+// the sponge transpose first; with the WALE closure, the velocity pre-pass before the
+// step and the gradient transpose after.
 void flow_adjoint(lbm_handle* h, AdjArgs& a, hipEvent_t after = nullptr) {
+    if (h->nsc) {  // transpose of the sponge blend on the post-collision cotangents
+        SpongeArgs sa = h->sponge_args(0);
+        sa.odd = a.odd;
+        hip_check(h->run->sponge_vjp(h->Q, sa, a.lam, h->lsc_ptr(), h->stream), "sponge transpose launch");
+    }
     if (is_wale(h->kind)) {
         hip_check(h->run->wale_u(h->Q, h->has_mask, h->g, a.pop, h->mask.as<uint32_t>(), a.odd, h->wu.p, h->cfg()),
                   "WALE velocity launch");
@@ -592,6 +643,10 @@ void step_into(lbm_handle* h, void* src, void* dst, uint64_t step, const ObjArgs
     if (kspan) hip_check(hipEventRecord(kspan->first, h->stream), "event record");
     hip_check(h->run->forward(h->Q, h->kind, h->has_mask, s, h->cfg(), kspan ? kspan->second : nullptr),
               "forward step launch");
+    if (h->nsc)  // blend of the sponge cells' post-collision values, before the ports
+        hip_check(h->run->sponge(h->Q, h->sponge_args(step), dst, diag ? led->exch(s_index) + led->npb : nullptr,
+                                 h->stream),
+                  "sponge launch");
     if (h->np) {
         hip_check(h->run->port_apply(h->Q, h->has_mask, h->port_args(step + 1), dst,
                                      diag ? led->exch(s_index) : nullptr, h->stream),
@@ -654,6 +709,112 @@ void upload_state(lbm_handle* h, void* buf, uint64_t step, const double* in, int
     hip_check(h->run->set_state(h->Q, h->has_mask, h->g, buf, h->mask.as<uint32_t>(), int(step & 1),
                                 h->aos.as<double>(), h->cfg()),
               "set_state launch");
+}
+
+// Sponge layers on the host: the covered fluid cells in ascending order, their summed
+// strength and their parts (sponge index, s_l, cos theta, sin theta) in sponge order.
+struct SpongeField {
+    std::vector<int> cell, off{0}, part;
+    std::vector<double> sigma, weight, phase;
+};
+
+// ramp of moving/sponge.rs:72-81, in its operation order.
+double sponge_ramp(double s) {
+    if (s <= 0.0) return 0.0;
+    if (s >= 1.0) return 1.0;
+    return s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
+}
+
+// SpongeField::new (moving/sponge.rs:112-181) on the lattice with unit spacing and the
+// origin at 0 (cell centres at ijk + 1/2), with the reference's contract checks
+// (TravellingWave::validate, sponge.rs:33-60); solid cells are not listed.
+SpongeField build_sponges(const lbm_handle* h, const lbm_config* cfg) {
+    SpongeField f;
+    if (cfg->n_sponges == 0) return f;
+    static_assert(LBM_MAX_SPONGES == MAX_SPONGES, "sponge capacity");
+    if (!cfg->sponges) fail(LBM_ERR_CONTRACT, "n_sponges > 0 needs a sponge array");
+    if (cfg->n_sponges > LBM_MAX_SPONGES) fail(LBM_ERR_CONTRACT, "at most 8 sponge layers");
+    static const char* names[6] = {"xmin", "xmax", "ymin", "ymax", "zmin", "zmax"};
+    const long long n = h->ncell;
+    const int N[3] = {h->nx, h->ny, h->nz};
+    std::vector<double> sigma(n, 0.0);
+    struct Part {
+        int l;
+        double s, c, sn;
+    };
+    std::vector<std::vector<Part>> parts;
+    std::vector<int> slot(n, -1);
+    auto finite3 = [](const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); };
+    for (uint32_t l = 0; l < cfg->n_sponges; l++) {
+        const lbm_sponge& sp = cfg->sponges[l];
+        if (sp.face > LBM_FACE_ZMAX) fail(LBM_ERR_CONTRACT, "unknown sponge face");
+        if (sp.wave > 1) fail(LBM_ERR_CONTRACT, "sponge wave must be 0 or 1");
+        if (sp.wave) {
+            const double* d = sp.wave_direction;
+            const double norm = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (!(finite3(sp.wave_amplitude) && finite3(d) && finite3(sp.wave_origin) && std::isfinite(sp.wave_speed) &&
+                  sp.wave_speed > 0.0 && std::isfinite(sp.wave_frequency) && sp.wave_frequency > 0.0 &&
+                  std::isfinite(sp.wave_phase) && std::isfinite(sp.wave_ramp) && sp.wave_ramp >= 0.0 &&
+                  std::fabs(norm - 1.0) <= 1e-9))
+                fail(LBM_ERR_CONTRACT, "sponge wave needs finite amplitude and origin, a unit direction, a positive "
+                                       "speed and a positive frequency");
+        }
+        const int axis = int(sp.face) / 2;
+        const bool lower = sp.face % 2 == 0;
+        if (h->per[axis])
+            fail(LBM_ERR_CONTRACT, std::string("sponge on face ") + names[sp.face] + " lies on a periodic axis");
+        if (sp.thickness == 0 || 2 * uint64_t(sp.thickness) > uint64_t(N[axis]))
+            fail(LBM_ERR_CONTRACT, std::string("sponge on face ") + names[sp.face] +
+                                       " must be between 1 cell and half the lattice thick");
+        if (!(std::isfinite(sp.strength) && sp.strength > 0.0 && sp.strength < 1.0))
+            fail(LBM_ERR_CONTRACT, "sponge strength must lie in (0, 1)");
+        if (!std::isfinite(sp.density) || !finite3(sp.velocity))
+            fail(LBM_ERR_CONTRACT, "sponge far-field state must be finite");
+        const double len = double(sp.thickness);
+        for (long long x = 0; x < n; x++) {
+            const int co[3] = {int(x / ((long long)h->nz * h->ny)), int((x / h->nz) % h->ny), int(x % h->nz)};
+            const int c = co[axis];
+            const double depth =
+                lower ? double(sp.thickness) - double(c) : double(c) - double(N[axis] - int(sp.thickness)) + 1.0;
+            if (depth <= 0.0) continue;
+            const double sv = sp.strength * sponge_ramp((depth - 0.5) / len);
+            if (sv <= 0.0) continue;
+            sigma[x] += sv;
+            double pc = 1.0, ps = 0.0;
+            if (sp.wave) {
+                double dist = 0.0;
+                for (int a = 0; a < 3; a++) dist += (double(co[a]) + 0.5 - sp.wave_origin[a]) * sp.wave_direction[a];
+                const double theta = 2.0 * M_PI * sp.wave_frequency * dist / sp.wave_speed;
+                pc = std::cos(theta);
+                ps = std::sin(theta);
+            }
+            if (slot[x] < 0) {
+                slot[x] = int(parts.size());
+                parts.emplace_back();
+            }
+            parts[slot[x]].push_back(Part{int(l), sv, pc, ps});
+        }
+    }
+    for (long long x = 0; x < n; x++)
+        if (sigma[x] >= 1.0) fail(LBM_ERR_CONTRACT, "summed sponge strength must stay below one");
+    for (long long x = 0; x < n; x++) {
+        if (slot[x] < 0 || (!h->solid.empty() && h->solid[x])) continue;
+        f.cell.push_back(int(x));
+        f.sigma.push_back(sigma[x]);
+        for (const Part& p : parts[slot[x]]) {
+            f.part.push_back(p.l);
+            f.weight.push_back(p.s);
+            f.phase.push_back(p.c);
+            f.phase.push_back(p.sn);
+        }
+        f.off.push_back(int(f.part.size()));
+    }
+    return f;
+}
+
+void upload(DevBuf& b, const void* src, size_t bytes, const char* what) {
+    b.alloc(bytes, what);
+    hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), what);
 }
 
 void reset_scalars(lbm_handle* h) { hip_check(hipMemsetAsync(h->scal.p, 0, h->scal.bytes, h->stream), "memset"); }
@@ -899,6 +1060,11 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         }
         h->np = int(h->h_pcell.size());
 
+        // Sponge layers (moving/sponge.rs:112-181).
+        const SpongeField sf = build_sponges(h.get(), cfg);
+        h->sponges.assign(cfg->sponges, cfg->sponges + (sf.cell.empty() ? 0 : cfg->n_sponges));
+        h->nsc = int(sf.cell.size());
+
         // Device allocations.
         hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream create");
         hip_check(hipEventCreate(&h->ev0), "event create");
@@ -950,6 +1116,14 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             }
             h->mask.alloc(m.size() * sizeof(uint32_t), "link mask");
             hip_check(hipMemcpy(h->mask.p, m.data(), h->mask.bytes, hipMemcpyHostToDevice), "mask upload");
+        }
+        if (h->nsc) {
+            upload(h->sp_cell, sf.cell.data(), sf.cell.size() * sizeof(int), "sponge cells");
+            upload(h->sp_sigma, sf.sigma.data(), sf.sigma.size() * sizeof(double), "sponge strength");
+            upload(h->sp_off, sf.off.data(), sf.off.size() * sizeof(int), "sponge parts");
+            upload(h->sp_part, sf.part.data(), sf.part.size() * sizeof(int), "sponge parts");
+            upload(h->sp_weight, sf.weight.data(), sf.weight.size() * sizeof(double), "sponge parts");
+            upload(h->sp_phase, sf.phase.data(), sf.phase.size() * sizeof(double), "sponge phases");
         }
         // Collision constants.
         if (h->kind == K_MRT || h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES ||

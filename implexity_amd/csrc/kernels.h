@@ -1725,6 +1725,122 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
 }
 
 // ---------------------------------------------------------------------------------
+// Sponge layers (moving/sponge.rs; the blend of field.rs:1461-1475):
+//   post := post (1 - sigma) + sum_l s_l f^eq(rho_l, u_l),
+//   u_l = mean_l + sin_l cos(theta) - cos_l sin(theta)   (Target::velocity, field.rs:373-376).
+// The step kernel leaves the post-collision value of (cell, q) in the cell's write slot,
+// which no other cell touches in that step, so the blend is a pass over the sponge cells'
+// write slots after the step and before the ports, and its transpose scales the
+// cotangents in the same slots before the transposed step reads them.  The pass touches
+// only the listed cells (the layers), so the step kernels keep their register budget.
+// ---------------------------------------------------------------------------------
+constexpr int SPONGE_BLOCK = 128;
+constexpr int MAX_SPONGES = 8;
+
+struct SpongeArgs {
+    Geom g;
+    int n;                // listed cells
+    int ns;               // sponges
+    int odd;              // parity of the step that wrote the slots
+    const int* cell;      // ascending fluid cells covered by some sponge
+    const double* sigma;  // summed strength of each listed cell
+    const int* off;       // parts of listed cell r: [off[r], off[r + 1]), in sponge order
+    const int* part;      // sponge index of each part
+    const double* weight; // s_l of each part
+    const double* phase;  // [cos theta, sin theta] of each part
+    double target[MAX_SPONGES][10];  // at the step's time: rho, mean[3], sin[3], cos[3]
+};
+
+// The far-field target sum_l s_l f^eq(rho_l, u_l) of listed cell r (f^eq - w in the
+// shifted storages), accumulated in sponge order.  The sponge index of a part selects
+// its target row through a loop over compile-time rows, so the rows stay kernel-argument
+// (scalar) loads.
+template <int Q, class C, bool SH>
+LBM_DEV void sponge_target(const SpongeArgs& A, int r, C* tgt) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) tgt[q] = C(0);
+    int p = A.off[r];
+    const int end = A.off[r + 1];
+#pragma unroll
+    for (int l = 0; l < MAX_SPONGES; l++) {
+        if (l < A.ns && p < end && A.part[p] == l) {
+            const double* t = A.target[l];
+            const C c0 = C(A.phase[2 * p]), c1 = C(A.phase[2 * p + 1]);
+            Macro<Q, C> m;
+            m.rho = C(t[0]);
+            m.drho = C(t[0] - 1.0);
+#pragma unroll
+            for (int d = 0; d < 3; d++) m.u[d] = C(t[1 + d]) + C(t[4 + d]) * c0 - C(t[7 + d]) * c1;
+            C e[Q];
+            if constexpr (SH)
+                equilibrium_shifted<Q>(m, e);
+            else
+                equilibrium<Q>(m.rho, m.u, e);
+            const C s = C(A.weight[p]);
+#pragma unroll
+            for (int q = 0; q < Q; q++) tgt[q] += e[q] * s;
+            p++;
+        }
+    }
+}
+
+// Blend of the slots written by the step of parity A.odd; exch_part[block] gets the
+// block's mass change sum_q (new - old) of the stored values.
+template <int P, int Q>
+__global__ __launch_bounds__(SPONGE_BLOCK) void k_sponge(SpongeArgs A, void* pop_, double* exch_part) {
+    using S = Store<P>;
+    using C = typename S::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double delta = 0.0;
+    if (r < A.n) {
+        typename S::T* pop = static_cast<typename S::T*>(pop_);
+        int i, j, k;
+        decode_cell(A.g, A.cell[r], i, j, k);
+        const Cursor cu = make_cursor(A.g, i, j, k);
+        C tgt[Q];
+        sponge_target<Q, C, S::shifted>(A, r, tgt);
+        // in the shifted storages d' = d (1 - sigma) + sum_l s_l (f^eq_l - w), as sigma = sum_l s_l
+        const C keep = C(1.0 - A.sigma[r]);
+        static_for<0, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            typename S::T& slot = at_write<Q, q>(pop, A.g, cu, A.odd);
+            const C old = S::template dec<Q, q>(slot);
+            const typename S::T nv = S::template enc<Q, q>(old * keep + tgt[q]);
+            slot = nv;
+            delta += double(S::template dec<Q, q>(nv)) - double(old);
+        });
+    }
+    if (exch_part) {  // per-block mass exchange, summed on the host in block order
+        __shared__ double red[SPONGE_BLOCK / 64];
+        delta = wave_sum(delta);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = delta;
+        __syncthreads();
+        if (threadIdx.x == 0) exch_part[blockIdx.x] = red[0] + red[1];
+    }
+}
+
+// Transpose of the blend on the cotangent of the post-collision values (the write slots
+// of parity A.odd): lambda *= 1 - sigma.  The far field does not depend on the state.
+template <int P, int LP, int Q>
+__global__ __launch_bounds__(SPONGE_BLOCK) void k_sponge_vjp(SpongeArgs A, void* lam_, const int* lsc) {
+    using L = LamStore<LP>;
+    using C = typename Store<P>::C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    const LScale sc = lam_scale<LP>(lsc, false);
+    typename L::T* lam = static_cast<typename L::T*>(lam_);
+    int i, j, k;
+    decode_cell(A.g, A.cell[r], i, j, k);
+    const Cursor cu = make_cursor(A.g, i, j, k);
+    const C keep = C(1.0 - A.sigma[r]);
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        typename L::T* slot = &at_write<Q, q>(lam, A.g, cu, A.odd);
+        L::template st<C>(slot, L::template ld<C>(slot, sc) * keep, sc);
+    });
+}
+
+// ---------------------------------------------------------------------------------
 // Layout conversion between the twisted device layout and the reference AoS layout.
 // ---------------------------------------------------------------------------------
 template <int P, int Q, bool MASK>

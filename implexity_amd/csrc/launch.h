@@ -55,6 +55,10 @@ struct Runner {
                               const LaunchCfg& c) const = 0;
     virtual hipError_t wale_transpose(int Q, bool mask, const Geom& g, const uint32_t* m, const void* U,
                                       const void* GB, void* lam, int odd, const LaunchCfg& c) const = 0;
+    // Sponge layers: the blend of the slots a step wrote (with its mass exchange per
+    // block when exch is set) and its transpose on the cotangent.
+    virtual hipError_t sponge(int Q, const SpongeArgs& a, void* pop, double* exch, hipStream_t s) const = 0;
+    virtual hipError_t sponge_vjp(int Q, const SpongeArgs& a, void* lam, const int* lsc, hipStream_t s) const = 0;
 };
 
 const Runner* runner_f64();
@@ -315,6 +319,22 @@ struct RunnerImpl final : Runner {
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         return reduce(a, c, true, false);
+    }
+    static bool ok_sponge(const SpongeArgs& a) {
+        return a.n > 0 && a.ns > 0 && a.ns <= MAX_SPONGES && a.cell && a.sigma && a.off && a.part && a.weight &&
+               a.phase;
+    }
+    hipError_t sponge(int Q, const SpongeArgs& a, void* pop, double* exch, hipStream_t s) const override {
+        if (!ok_sponge(a) || !pop) return hipErrorInvalidValue;
+        const LaunchCfg c{dim3((a.n + SPONGE_BLOCK - 1) / SPONGE_BLOCK), dim3(SPONGE_BLOCK), s};
+        if (Q == 19) LBM_LAUNCH((k_sponge<P, 19>), c, a, pop, exch);
+        LBM_LAUNCH((k_sponge<P, 27>), c, a, pop, exch);
+    }
+    hipError_t sponge_vjp(int Q, const SpongeArgs& a, void* lam, const int* lsc, hipStream_t s) const override {
+        if (!ok_sponge(a) || !lam || (LP == P_F16S && !lsc)) return hipErrorInvalidValue;
+        const LaunchCfg c{dim3((a.n + SPONGE_BLOCK - 1) / SPONGE_BLOCK), dim3(SPONGE_BLOCK), s};
+        if (Q == 19) LBM_LAUNCH((k_sponge_vjp<P, LP, 19>), c, a, lam, lsc);
+        LBM_LAUNCH((k_sponge_vjp<P, LP, 27>), c, a, lam, lsc);
     }
     static LaunchCfg port_cfg(const PortArgs& a, hipStream_t s) {
         return LaunchCfg{dim3((a.np + PORT_BLOCK - 1) / PORT_BLOCK), dim3(PORT_BLOCK), s};

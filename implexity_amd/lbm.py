@@ -78,6 +78,17 @@ class _Port(ctypes.Structure):
     ]
 
 
+class _Sponge(ctypes.Structure):
+    _fields_ = [
+        ("face", ctypes.c_uint32), ("thickness", ctypes.c_uint32), ("strength", ctypes.c_double),
+        ("density", ctypes.c_double), ("velocity", ctypes.c_double * 3), ("wave", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32), ("wave_amplitude", ctypes.c_double * 3),
+        ("wave_direction", ctypes.c_double * 3), ("wave_speed", ctypes.c_double),
+        ("wave_frequency", ctypes.c_double), ("wave_phase", ctypes.c_double), ("wave_ramp", ctypes.c_double),
+        ("wave_origin", ctypes.c_double * 3),
+    ]
+
+
 class _Config(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32), ("lattice", ctypes.c_uint32), ("collision", ctypes.c_uint32),
@@ -92,6 +103,7 @@ class _Config(ctypes.Structure):
         ("thermal_inlet", ctypes.c_double), ("thermal_range", ctypes.c_double),
         ("les_constant", ctypes.c_double), ("les_norm_floor", ctypes.c_double),
         ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
+        ("n_sponges", ctypes.c_uint32), ("reserved3", ctypes.c_uint32), ("sponges", ctypes.c_void_p),
     ]
 
 
@@ -217,6 +229,34 @@ class Port:
 
 
 @dataclass
+class Wave:
+    """Travelling wave on a sponge's far-field velocity (moving/sponge.rs:22-29), lattice
+    units: amplitude (velocity), unit direction, speed (cells per step), frequency
+    (cycles per step), phase (rad), ramp (steps), origin (cells)."""
+
+    amplitude: tuple
+    direction: tuple
+    speed: float
+    frequency: float
+    phase: float = 0.0
+    ramp: float = 0.0
+    origin: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class Sponge:
+    """Sponge layer (moving/sponge.rs:12-20 SpongeSpec) in lattice units: the far field
+    is a density and a velocity."""
+
+    face: str
+    thickness: int
+    strength: float
+    density: float = 1.0
+    velocity: tuple = (0.0, 0.0, 0.0)
+    wave: Wave | None = None
+
+
+@dataclass
 class LatticeConfig:
     shape: tuple
     periodic: tuple = (False, False, False)
@@ -251,6 +291,8 @@ class LatticeConfig:
     # WALE closure on BGK/TRT (moving/wale.rs); 0 = off
     wale_constant: float = 0.0
     wale_floor: float = 1e-12
+    # Sponge layers blended after collision (moving/sponge.rs, field.rs:1461-1475)
+    sponges: list = field(default_factory=list)
 
 
 @dataclass
@@ -336,6 +378,27 @@ class Lattice:
         c.les_norm_floor = float(cfg.les_norm_floor)
         c.wale_constant = float(cfg.wale_constant)
         c.wale_floor = float(cfg.wale_floor)
+        sponges = (_Sponge * max(1, len(cfg.sponges)))()
+        for i, sp in enumerate(cfg.sponges):
+            sponges[i].face = FACE[sp.face]
+            sponges[i].thickness = int(sp.thickness)
+            sponges[i].strength = float(sp.strength)
+            sponges[i].density = float(sp.density)
+            for d in range(3):
+                sponges[i].velocity[d] = float(sp.velocity[d])
+            if sp.wave is not None:
+                w = sp.wave
+                sponges[i].wave = 1
+                for d in range(3):
+                    sponges[i].wave_amplitude[d] = float(w.amplitude[d])
+                    sponges[i].wave_direction[d] = float(w.direction[d])
+                    sponges[i].wave_origin[d] = float(w.origin[d])
+                sponges[i].wave_speed = float(w.speed)
+                sponges[i].wave_frequency = float(w.frequency)
+                sponges[i].wave_phase = float(w.phase)
+                sponges[i].wave_ramp = float(w.ramp)
+        c.n_sponges = len(cfg.sponges)
+        c.sponges = ctypes.cast(sponges, ctypes.c_void_p)
         solid = None
         if cfg.solid is not None:
             solid = np.ascontiguousarray(np.asarray(cfg.solid, bool).reshape(-1), np.uint8)
@@ -343,6 +406,7 @@ class Lattice:
         _check(lib().lbm_create(ctypes.byref(c), _p(solid), 0 if solid is None else solid.size, ctypes.byref(h)))
         self._h = h
         self._ports_c = ports
+        self._sponges_c = sponges
 
     def close(self):
         if getattr(self, "_h", None):

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 3
+#define LBM_ABI_VERSION 4
 
 /* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
  * crates/implexity-ad/src/error.rs). */
@@ -105,6 +105,34 @@ typedef struct {
  * (no counterpart in the reference, whose cotangents are FP64 Vec<f64>); a step that
  * would leave the FP16 range makes lbm_gradient / lbm_step_vjp fail with
  * LBM_ERR_CONVERGENCE. */
+/* One sponge layer (moving/sponge.rs:12-29, SpongeField::new :112-181), in lattice units:
+ * the fluid cells within `thickness` layers of `face` are blended after collision,
+ *   post := post (1 - sigma) + sum_l s_l f^eq(rho_l, u_l)       (field.rs:1461-1475),
+ * s_l = strength * ramp((depth - 1/2) / thickness) with the quintic smoothstep ramp and
+ * depth 1 .. thickness counted inward from the face's innermost layer (thickness at the
+ * face), sigma = sum_l s_l < 1.  The far-field target of the step that reads state t is
+ *   rho_l = density,  u_l = velocity + A r(t) sin(2 pi f t + phase - theta(x)),
+ *   theta(x) = 2 pi f ((x - origin) . direction) / speed,  x = cell centre (ijk + 1/2),
+ * r(t) the smoothstep over wave_ramp steps (field.rs:1015-1036, boundary.rs:110-125). */
+typedef struct {
+    uint32_t face;              /* lbm_face; the axis must not be periodic */
+    uint32_t thickness;         /* 1 .. half the lattice along the face's axis */
+    double strength;            /* in (0, 1) */
+    double density;             /* far-field density */
+    double velocity[3];         /* far-field mean velocity */
+    uint32_t wave;              /* 0: steady far field; 1: travelling wave below */
+    uint32_t reserved;
+    double wave_amplitude[3];   /* A (velocity) */
+    double wave_direction[3];   /* unit vector */
+    double wave_speed;          /* cells per step, > 0 */
+    double wave_frequency;      /* cycles per step, > 0 */
+    double wave_phase;          /* rad */
+    double wave_ramp;           /* steps over which the wave ramps in, >= 0 */
+    double wave_origin[3];      /* cells */
+} lbm_sponge;
+
+#define LBM_MAX_SPONGES 8
+
 typedef enum {
     LBM_ADJ_AUTO = 0,
     LBM_ADJ_F64 = 1,
@@ -160,6 +188,10 @@ typedef struct {
      * step collides (central differences, one-sided next to walls and closed faces). */
     double wale_constant;         /* C >= 0; 0: no closure */
     double wale_floor;            /* > 0 when wale_constant > 0 */
+    /* Sponge layers, at most LBM_MAX_SPONGES (lbm_sponge). */
+    uint32_t n_sponges;
+    uint32_t reserved3;
+    const lbm_sponge* sponges;
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

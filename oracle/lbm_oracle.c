@@ -148,6 +148,19 @@ typedef struct {
     double les_floor;           /* Smagorinsky norm_floor */
     double wale_constant;       /* WALE constant (moving/wale.rs); <= 0 or NaN: none */
     double wale_floor;          /* WALE denominator_floor */
+    /* Sponge layers (moving/sponge.rs, blended after collision as field.rs:1461-1475):
+     * the listed fluid cells take post (1 - sigma) + sum_parts s_l f^eq(rho_l, u_l) with
+     * u_l = mean_l + sin_l cos(theta) - cos_l sin(theta).  Built by oracle/oracle.py
+     * (SpongeField::new, moving/sponge.rs:112-181; targets field.rs:1015-1036). */
+    int32_t n_sponge_cells;
+    int32_t n_sponges;
+    const int32_t *sponge_cell;    /* ascending cell indices */
+    const double *sponge_sigma;    /* summed strength of each listed cell */
+    const int32_t *sponge_off;     /* parts of listed cell r: [off[r], off[r+1]) */
+    const int32_t *sponge_part;    /* sponge index l of each part */
+    const double *sponge_weight;   /* s_l of each part */
+    const double *sponge_phase;    /* [cos theta, sin theta] of each part */
+    const double *sponge_target;   /* per sponge at the step's time: rho, mean[3], sin[3], cos[3] */
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -1358,6 +1371,29 @@ static void apply_ports(const orc_problem *p, double *after, double *exchange) {
     }
 }
 
+/* The sponge blend of field.rs:1461-1475 on the post-collision populations:
+ * target = sum over parts (in sponge order) of f^eq(rho_l, velocity(phase)) * s_l
+ * (Target::velocity, field.rs:373-376), then post = post * (1 - sigma) + target. */
+static void apply_sponges(const orc_problem *p, double *post) {
+    const lattice_t *L = lat(p->Q);
+    int Q = p->Q;
+    for (int r = 0; r < p->n_sponge_cells; r++) {
+        long x = p->sponge_cell[r];
+        double target[MAXQ], e[MAXQ];
+        for (int q = 0; q < Q; q++) target[q] = 0.0;
+        for (int k = p->sponge_off[r]; k < p->sponge_off[r + 1]; k++) {
+            const double *t = p->sponge_target + 10 * p->sponge_part[k];
+            const double *ph = p->sponge_phase + 2 * k;
+            double u[3];
+            for (int d = 0; d < 3; d++) u[d] = t[1 + d] + t[4 + d] * ph[0] - t[7 + d] * ph[1];
+            equilibrium(L, t[0], u, e);
+            for (int q = 0; q < Q; q++) target[q] += e[q] * p->sponge_weight[k];
+        }
+        double keep = 1.0 - p->sponge_sigma[r];
+        for (int q = 0; q < Q; q++) post[x * Q + q] = post[x * Q + q] * keep + target[q];
+    }
+}
+
 void orc_interval(const orc_problem *p, const uint32_t *source, const double *alpha, const double *before,
                   double *post, double *after, double *exchange) {
     long n = (long)p->nx * p->ny * p->nz;
@@ -1380,6 +1416,7 @@ void orc_interval(const orc_problem *p, const uint32_t *source, const double *al
     free(rate);
     free(wu);
     free(wr);
+    apply_sponges(p, post);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n * Q; i++) {
         uint32_t s = source[i];
@@ -1420,6 +1457,12 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
     for (long i = 0; i < n * Q; i++) {
         uint32_t s = source[i];
         if (s != NONE_U32) post_bar[s] += work[i];
+    }
+    /* the sponge blend is affine in post: post_bar *= 1 - sigma at the listed cells */
+    for (int r = 0; r < p->n_sponge_cells; r++) {
+        long x = p->sponge_cell[r];
+        double keep = 1.0 - p->sponge_sigma[r];
+        for (int q = 0; q < Q; q++) post_bar[x * Q + q] *= keep;
     }
     const int wale = wale_on(p);
     double *rate = NULL, *wu = NULL, *wr = NULL, *gbar = NULL;

@@ -31,6 +31,11 @@ Parity pins (each checked in tests/test_oracle_*.py):
 * Smagorinsky / WALE     -- Hou et al.'s and Nicoud & Ducros' closed forms, pure-shear
                             invariance, linear-field stencil, central differences, the
                             dot-product identity of the step (tests/test_oracle_les.py)
+* sponge layers          -- the ramp's Hermite conditions, mirror symmetry of opposite
+                            faces, the far-field equilibrium as a fixed point, the blend's
+                            mass and momentum, travelling-wave invariance, the step's
+                            dot-product identity and finite differences
+                            (tests/test_oracle_sponge.py)
 """
 from __future__ import annotations
 
@@ -89,6 +94,10 @@ class _Problem(ctypes.Structure):
         ("mrt_rows", ctypes.c_void_p), ("mrt_norms", ctypes.c_void_p), ("mrt_groups", ctypes.c_void_p),
         ("les_constant", ctypes.c_double), ("les_floor", ctypes.c_double),
         ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
+        ("n_sponge_cells", ctypes.c_int32), ("n_sponges", ctypes.c_int32),
+        ("sponge_cell", ctypes.c_void_p), ("sponge_sigma", ctypes.c_void_p), ("sponge_off", ctypes.c_void_p),
+        ("sponge_part", ctypes.c_void_p), ("sponge_weight", ctypes.c_void_p), ("sponge_phase", ctypes.c_void_p),
+        ("sponge_target", ctypes.c_void_p),
     ]
 
 
@@ -127,6 +136,144 @@ class Port:
 
 
 @dataclass
+class Wave:
+    """Travelling wave on a sponge's far-field velocity (moving/sponge.rs:22-29), in
+    lattice units: amplitude (velocity), unit direction, speed (cells per step),
+    frequency (cycles per step), phase (rad), ramp (steps), origin (cells)."""
+    amplitude: tuple
+    direction: tuple
+    speed: float
+    frequency: float
+    phase: float = 0.0
+    ramp: float = 0.0
+    origin: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class Sponge:
+    """SpongeSpec (moving/sponge.rs:12-20) in lattice units: the far-field state is a
+    density and a velocity instead of a pressure in Pa and a velocity in m/s."""
+    face: str
+    thickness: int
+    strength: float
+    density: float = 1.0
+    velocity: tuple = (0.0, 0.0, 0.0)
+    wave: Wave | None = None
+
+
+def sponge_ramp(s: float) -> float:
+    """ramp of moving/sponge.rs:72-81 (the quintic smoothstep, in its operation order)."""
+    if s <= 0.0:
+        return 0.0
+    if s >= 1.0:
+        return 1.0
+    return s * s * s * (10.0 - 15.0 * s + 6.0 * s * s)
+
+
+def signal_factors(ramp_steps: float, frequency: float, phase: float, t: float):
+    """Signal::factors (moving/boundary.rs:110-125) in steps: (ramp, wave)."""
+    if ramp_steps > 0.0 and t < ramp_steps:
+        s = t / ramp_steps
+        s = 0.0 if s < 0.0 else s
+        ramp = s * s * s * (10.0 - s * 15.0 + s * s * 6.0)
+    else:
+        ramp = 1.0
+    wave = 0.0 if frequency == 0.0 else math.sin(t * (2.0 * math.pi * frequency) + phase)
+    return ramp, wave
+
+
+def _wave_ok(w: Wave) -> bool:
+    """TravellingWave::validate (moving/sponge.rs:33-60); the signal's frequency and phase
+    are the wave's own."""
+    vals = list(w.amplitude) + list(w.direction) + list(w.origin)
+    norm = math.sqrt(sum(v * v for v in w.direction))
+    return (all(math.isfinite(v) for v in vals) and math.isfinite(w.speed) and w.speed > 0.0
+            and math.isfinite(w.frequency) and w.frequency > 0.0 and math.isfinite(w.phase)
+            and math.isfinite(w.ramp) and w.ramp >= 0.0 and abs(norm - 1.0) <= 1e-9)
+
+
+def sponge_field(shape, periodic, sponges, solid=None):
+    """SpongeField::new (moving/sponge.rs:112-181) on the stationary grid with unit
+    spacing and the lattice origin at 0 (cell centres at ijk + 1/2).  Returns
+    (cells, sigma, off, part, weight, phase) of the fluid cells some sponge covers, in
+    ascending cell order; raises ValueError with the reference's contract messages."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    empty = (np.zeros(0, np.int32), np.zeros(0), np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0),
+             np.zeros(0))
+    if not sponges:
+        return empty
+    solid = np.zeros(n, bool) if solid is None else np.asarray(solid, bool).reshape(-1)
+    sigma = np.zeros(n)
+    parts = {}
+    idx = np.arange(n)
+    co = np.stack([idx // (ny * nz), (idx // nz) % ny, idx % nz])
+    for l, sp in enumerate(sponges):
+        if sp.wave is not None and not _wave_ok(sp.wave):
+            raise ValueError("sponge wave needs finite amplitude and origin, a unit direction, a positive speed "
+                             "and a positive frequency")
+        axis, inward = FACES[sp.face]
+        if periodic[axis]:
+            raise ValueError(f"sponge on face {sp.face} lies on a periodic axis")
+        if sp.thickness == 0 or 2 * sp.thickness > shape[axis]:
+            raise ValueError(f"sponge on face {sp.face} must be between 1 cell and half the lattice thick")
+        if not (math.isfinite(sp.strength) and 0.0 < sp.strength < 1.0):
+            raise ValueError("sponge strength must lie in (0, 1)")
+        if not math.isfinite(sp.density) or not all(math.isfinite(v) for v in sp.velocity):
+            raise ValueError("sponge far-field state must be finite")
+        length = float(sp.thickness)
+        c = co[axis].astype(np.float64)
+        depth = (float(sp.thickness) - c) if inward > 0 else (c - float(shape[axis] - sp.thickness) + 1.0)
+        for x in np.nonzero(depth > 0.0)[0]:
+            sv = sp.strength * sponge_ramp((float(depth[x]) - 0.5) / length)
+            if sv <= 0.0:
+                continue
+            sigma[x] += sv
+            if sp.wave is None:
+                ph = (1.0, 0.0)
+            else:
+                w = sp.wave
+                pos = [float(co[a, x]) + 0.5 for a in range(3)]
+                s = 0.0
+                for a in range(3):
+                    s += (pos[a] - w.origin[a]) * w.direction[a]
+                theta = 2.0 * math.pi * w.frequency * s / w.speed
+                ph = (math.cos(theta), math.sin(theta))
+            parts.setdefault(int(x), []).append((l, sv, ph))
+    if np.any(sigma >= 1.0):
+        raise ValueError("summed sponge strength must stay below one")
+    cells = [x for x in sorted(parts) if not solid[x]]
+    off = [0]
+    part, weight, phase = [], [], []
+    for x in cells:
+        for l, sv, ph in parts[x]:
+            part.append(l)
+            weight.append(sv)
+            phase.extend(ph)
+        off.append(len(part))
+    return (np.asarray(cells, np.int32), np.asarray([sigma[x] for x in cells], np.float64),
+            np.asarray(off, np.int32), np.asarray(part, np.int32), np.asarray(weight, np.float64),
+            np.asarray(phase, np.float64))
+
+
+def sponge_targets(sponges, t: float):
+    """Far-field targets of every sponge at step time t (field.rs:1015-1036 with c = 1):
+    rows [rho, mean[3], sin[3], cos[3]]."""
+    out = np.zeros((max(1, len(sponges)), 10))
+    for l, sp in enumerate(sponges):
+        out[l, 0] = sp.density
+        out[l, 1:4] = sp.velocity
+        if sp.wave is not None:
+            w = sp.wave
+            ramp, _ = signal_factors(w.ramp, w.frequency, w.phase, t)
+            arg = t * (2.0 * math.pi * w.frequency) + w.phase
+            sw, cw = ramp * math.sin(arg), ramp * math.cos(arg)
+            out[l, 4:7] = [sw * a for a in w.amplitude]
+            out[l, 7:10] = [cw * a for a in w.amplitude]
+    return out
+
+
+@dataclass
 class Problem:
     """Plain description of one lattice problem in lattice units."""
 
@@ -147,6 +294,8 @@ class Problem:
     # WALE closure on BGK/TRT (moving/wale.rs): constant (0 = off), denominator floor
     wale_constant: float = 0.0
     wale_floor: float = 1e-12
+    # Sponge layers (moving/sponge.rs) blended after collision
+    sponges: list = field(default_factory=list)
 
 
 class Oracle:
@@ -183,6 +332,13 @@ class Oracle:
         p.wale_constant, p.wale_floor = float(prob.wale_constant), float(prob.wale_floor)
         if self._mrt is not None:
             p.mrt_rows, p.mrt_norms, p.mrt_groups = (_ptr(self._mrt[0]), _ptr(self._mrt[1]), _ptr(self._mrt[2]))
+        self._sp = sponge_field(prob.shape, prob.periodic, prob.sponges, self.solid.astype(bool))
+        cells, sigma, off, part, weight, phase = self._sp
+        p.n_sponge_cells, p.n_sponges = len(cells), len(prob.sponges)
+        p.sponge_cell, p.sponge_sigma, p.sponge_off = _ptr(cells), _ptr(sigma), _ptr(off)
+        p.sponge_part, p.sponge_weight, p.sponge_phase = _ptr(part), _ptr(weight), _ptr(phase)
+        self._target = sponge_targets(prob.sponges, 0.0)
+        p.sponge_target = _ptr(self._target)
         self._p = p
         self._pp = ctypes.byref(p)
         self.source = np.zeros(self.n * self.Q, np.uint32)
@@ -290,7 +446,12 @@ class Oracle:
         lib().orc_initial(self._pp, _ptr(f))
         return f
 
-    def interval(self, alpha, before, want_post=False):
+    def set_time(self, t):
+        """Sponge targets of the step that reads state t."""
+        self._target[:] = sponge_targets(self.prob.sponges, float(t))
+
+    def interval(self, alpha, before, want_post=False, t=0):
+        self.set_time(t)
         alpha = np.ascontiguousarray(alpha, np.float64)
         before = np.ascontiguousarray(before, np.float64)
         post = np.empty_like(before)
@@ -302,12 +463,13 @@ class Oracle:
             return after, ex[: len(self.port_cell)], post
         return after
 
-    def interval_vjp(self, alpha, before, after_bar, after=None):
+    def interval_vjp(self, alpha, before, after_bar, after=None, t=0):
         alpha = np.ascontiguousarray(alpha, np.float64)
         before = np.ascontiguousarray(before, np.float64)
         after_bar = np.ascontiguousarray(after_bar, np.float64)
         if after is None:
-            after = self.interval(alpha, before)
+            after = self.interval(alpha, before, t=t)
+        self.set_time(t)
         work = np.empty_like(before)
         f_bar = np.empty_like(before)
         a_bar = np.empty(self.n)
@@ -318,14 +480,14 @@ class Oracle:
     def history(self, alpha, steps, f0=None):
         """solver.rs:677-687 -- every state kept (use small problems)."""
         states = [self.initial() if f0 is None else np.array(f0, np.float64)]
-        for _ in range(steps):
-            states.append(self.interval(alpha, states[-1]))
+        for t in range(steps):
+            states.append(self.interval(alpha, states[-1], t=t))
         return states
 
-    def run(self, alpha, steps, f0=None):
+    def run(self, alpha, steps, f0=None, t0=0):
         f = self.initial() if f0 is None else np.array(f0, np.float64)
-        for _ in range(steps):
-            f = self.interval(alpha, f)
+        for t in range(steps):
+            f = self.interval(alpha, f, t=t0 + t)
         return f
 
     def macroscopic(self, alpha, state):
@@ -433,7 +595,7 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
     J = 0.0
     checkpoints = [(0, x.copy())]
     for t in range(steps):
-        x = orc.interval(alpha, x)
+        x = orc.interval(alpha, x, t=t)
         w = objective.stage_weight(t + 1)
         if w:
             J += w * objective.value(orc, alpha, x)
@@ -453,7 +615,7 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
         end = min(start + seg, steps)
         states = [xs]
         for t in range(start, end - 1):
-            states.append(orc.interval(alpha, states[-1]))
+            states.append(orc.interval(alpha, states[-1], t=t))
         for t in range(end - 1, start - 1, -1):
             x_next = states[t + 1 - start] if t + 1 < end else after
             w = objective.stage_weight(t + 1)
@@ -461,7 +623,7 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
                 g, ga = objective.grad(orc, alpha, x_next, w)
                 lam = lam + g
                 abar += ga
-            fb, ab = orc.interval_vjp(alpha, states[t - start], lam)
+            fb, ab = orc.interval_vjp(alpha, states[t - start], lam, t=t)
             abar += ab
             lam = fb
         after = states[0]
@@ -473,7 +635,7 @@ def objective_value(orc: Oracle, alpha, steps, objective: Objective, f0=None):
     x = orc.initial() if f0 is None else np.array(f0, np.float64)
     J = 0.0
     for t in range(steps):
-        x = orc.interval(alpha, x)
+        x = orc.interval(alpha, x, t=t)
         w = objective.stage_weight(t + 1)
         if w:
             J += w * objective.value(orc, alpha, x)

@@ -147,14 +147,14 @@ class ThermalOracle:
         return gb
 
     # --- coupled problem ----------------------------------------------------------------
-    def step(self, alpha, phi, f, g):
-        return self.orc.interval(alpha, f), self.interval(alpha, phi, f, g)
+    def step(self, alpha, phi, f, g, t=0):
+        return self.orc.interval(alpha, f, t=t), self.interval(alpha, phi, f, g)
 
     def run(self, alpha, phi, steps, f0=None, g0=None):
         f = self.orc.initial() if f0 is None else np.array(f0, np.float64)
         g = self.initial() if g0 is None else np.array(g0, np.float64)
-        for _ in range(steps):
-            f, g = self.step(alpha, phi, f, g)
+        for t in range(steps):
+            f, g = self.step(alpha, phi, f, g, t)
         return f, g
 
     def objective(self, ob: CoupledObjective, alpha, phi, f, g):
@@ -180,7 +180,7 @@ class ThermalOracle:
         f, g = self.orc.initial(), self.initial()
         J = 0.0
         for t in range(steps):
-            f, g = self.step(alpha, phi, f, g)
+            f, g = self.step(alpha, phi, f, g, t)
             w = ob.stage_weight(t + 1)
             if w:
                 J += w * self.objective(ob, alpha, phi, f, g)
@@ -188,10 +188,11 @@ class ThermalOracle:
             J += ob.terminal_weight * self.objective(ob, alpha, phi, f, g)
         return J
 
-    def step_vjp(self, alpha, phi, f, g, lam_f, lam_g):
-        """Transpose of one coupled interval at (f, g): (f_bar, g_bar, alpha_bar, phi_bar)."""
+    def step_vjp(self, alpha, phi, f, g, lam_f, lam_g, t=0):
+        """Transpose of one coupled interval at (f, g) read at step t: (f_bar, g_bar,
+        alpha_bar, phi_bar)."""
         gb, fb_t, ab_t, pb = self.interval_vjp(alpha, phi, f, g, lam_g)
-        fb, ab = self.orc.interval_vjp(alpha, f, lam_f)
+        fb, ab = self.orc.interval_vjp(alpha, f, lam_f, t=t)
         return fb + fb_t, gb, ab + ab_t, pb
 
     def scan_adjoint(self, ob: CoupledObjective, alpha, phi, steps):
@@ -202,7 +203,7 @@ class ThermalOracle:
         fs, gs = [self.orc.initial()], [self.initial()]
         J = 0.0
         for t in range(steps):
-            f, g = self.step(alpha, phi, fs[-1], gs[-1])
+            f, g = self.step(alpha, phi, fs[-1], gs[-1], t)
             fs.append(f)
             gs.append(g)
             w = ob.stage_weight(t + 1)
@@ -223,7 +224,7 @@ class ThermalOracle:
                 fb, gb = self.objective_grad(ob, phi, w)
                 lam_f = lam_f + fb
                 lam_g = lam_g + gb
-            lam_f, lam_g, ab, pb = self.step_vjp(alpha, phi, fs[t], gs[t], lam_f, lam_g)
+            lam_f, lam_g, ab, pb = self.step_vjp(alpha, phi, fs[t], gs[t], lam_f, lam_g, t)
             abar += ab
             pbar += pb
         return J, abar, pbar, lam_f, lam_g

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lbm.lib().lbm_abi_version() == 3
+    assert lbm.lib().lbm_abi_version() == 4
 
 
 @pytest.mark.parametrize("n,s", [(1, 1), (3, 1), (10, 2), (100, 10), (1000, 30), (8000, 156)])
@@ -83,6 +83,17 @@ def _create(**kw):
     (dict(wale_constant=0.5, les_constant=0.1), "choose one closure"),
     (dict(wale_constant=0.5, wale_floor=0.0), "denominator_floor"),
     (dict(wale_constant=0.5, precision="f16s", adjoint_storage="f16"), "FP32 cotangents"),
+    (dict(sponges=[lbm.Sponge("xmin", 2, 0.5)], periodic=(True, False, False)), "periodic axis"),
+    (dict(sponges=[lbm.Sponge("ymax", 3, 0.5)]), "half the lattice"),
+    (dict(sponges=[lbm.Sponge("ymax", 0, 0.5)]), "half the lattice"),
+    (dict(sponges=[lbm.Sponge("zmin", 1, 1.0)]), r"\(0, 1\)"),
+    (dict(sponges=[lbm.Sponge("zmin", 1, 0.5, density=float("inf"))]), "far-field state must be finite"),
+    (dict(sponges=[lbm.Sponge("zmin", 1, 0.5, wave=lbm.Wave((0.01, 0, 0), (1.0, 1.0, 0.0), 0.1, 0.01))]),
+     "unit direction"),
+    (dict(sponges=[lbm.Sponge("zmin", 1, 0.5, wave=lbm.Wave((0.01, 0, 0), (1.0, 0.0, 0.0), 0.1, -0.01))]),
+     "positive frequency"),
+    (dict(sponges=[lbm.Sponge("xmin", 2, 0.6), lbm.Sponge("xmin", 2, 0.6)]), "below one"),
+    (dict(sponges=[lbm.Sponge("xmin", 1, 0.1)] * 9), "at most 8"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are
