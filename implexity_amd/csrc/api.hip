@@ -226,6 +226,12 @@ struct lbm_handle {
     std::vector<lbm_sponge> sponges;
     int nsc = 0;
     DevBuf sp_cell, sp_sigma, sp_off, sp_part, sp_weight, sp_phase;
+    // Occupancy coupling (K_TRT_PSM / K_BGK_PSM): the design array is the occupancy d;
+    // solid momentum m (SoA, arithmetic precision), its FP64 cotangent, and a zero design
+    // for the Brinkman-velocity outputs (fields).
+    int psm_law = -1;
+    double psm_kappa = 0.0;
+    DevBuf sm, sm_bar, alpha0;
     const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
@@ -250,6 +256,7 @@ struct lbm_handle {
         s.odd = int(step & 1);
         s.step = uint32_t(step);
         s.wu = wu.p;
+        s.sm = sm.p;
         if (thermal) {  // the flow kernel advances the temperature field in the same pass
             s.tpop = tpop(static_cast<const void*>(pop));
             s.tpop_out = tpop(pop);
@@ -358,6 +365,8 @@ struct lbm_handle {
         for (int d = 0; d < 3; d++) a.a[d] = this->a[d];
         a.mrt = mrt.p;
         a.odd = odd;
+        a.sm = sm.p;
+        a.sm_bar = sm_bar.as<double>();
         a.obj.kind = OBJ_NONE;
         a.part = part.as<double>();
         a.npart = npart;
@@ -430,6 +439,9 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
     ObjArgs r{};
     r.kind = OBJ_NONE;
     if (!o || o->kind == LBM_OBJ_NONE) return r;
+    if (is_psm(h->kind) && o->kind != LBM_OBJ_PRESSURE_DROP)
+        fail(LBM_ERR_CONTRACT, "with the occupancy coupling the objective is the pressure drop (the other "
+                               "objectives read a Brinkman coefficient)");
     if (o->kind == LBM_OBJ_VECTOR) vector_weights(h, o->vector_weights, o->scales, r);
     r.axis = int(o->axis);
     const int len = o->axis == 0 ? h->nx : (o->axis == 1 ? h->ny : h->nz);
@@ -548,6 +560,7 @@ void ensure_adjoint_buffers(lbm_handle* h) {
         h->ubar.alloc(size_t(h->ncell) * 3 * h->csize(), "velocity cotangent");
     }
     if (is_wale(h->kind)) h->wgb.alloc(size_t(h->ncell) * 9 * h->csize(), "WALE gradient cotangents");
+    if (is_psm(h->kind)) h->sm_bar.alloc(size_t(h->ncell) * 3 * sizeof(double), "solid momentum cotangent");
 }
 
 // Snapshot slots for a horizon: min(budget / state bytes, nsteps); the pool only grows.
@@ -873,6 +886,19 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             h->les_c = cfg->les_constant;
             h->les_floor = cfg->les_norm_floor;
         }
+        if (cfg->coupling > LBM_COUPLING_PSM_SUPERPOSITION) fail(LBM_ERR_CONTRACT, "unknown coupling law");
+        if (cfg->coupling != LBM_COUPLING_BRINKMAN) {
+            if (h->kind != K_TRT && h->kind != K_BGK)
+                fail(LBM_ERR_CONTRACT, "the occupancy coupling applies to the plain BGK and TRT operators");
+            if (cfg->thermal) fail(LBM_ERR_CONTRACT, "the occupancy coupling and the temperature field are not combined");
+            if (cfg->precision == LBM_BF16_SHIFTED)
+                fail(LBM_ERR_CONTRACT, "the occupancy coupling supports FP64, FP32 and FP16 storage");
+            if (!(cfg->psm_kappa >= 0.0 && cfg->psm_kappa < 1.0))
+                fail(LBM_ERR_CONTRACT, "saturation width kappa must lie in [0, 1)");
+            h->kind = h->kind == K_TRT ? int(K_TRT_PSM) : int(K_BGK_PSM);
+            h->psm_law = cfg->coupling == LBM_COUPLING_PSM ? 0 : 1;
+            h->psm_kappa = cfg->psm_kappa;
+        }
         if (h->kind == K_CUMULANT && h->Q != 27)
             fail(LBM_ERR_CONTRACT, "cumulant collision requires the D3Q27 velocity set (got D3Q19)");
         if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
@@ -948,7 +974,8 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (!(h->tau >= 0.51 && h->tau <= 2.0))
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
-        const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES || h->kind == K_TRT_WALE;
+        const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES || h->kind == K_TRT_WALE ||
+                              h->kind == K_TRT_PSM;
         if (trt_like && !(std::isfinite(h->magic) && h->magic > 0.0))
             fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
         h->om = trt_like ? odd_rate(h->omega, h->magic) : h->omega;
@@ -1075,6 +1102,12 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         hip_check(hipMemsetAsync(h->alpha.p, 0, h->alpha.bytes, h->stream), "memset");
         h->h_alpha.assign(h->ncell, 0.0);
         if (is_wale(h->kind)) h->wu.alloc(size_t(h->ncell) * 4 * h->csize(), "WALE velocities");
+        if (is_psm(h->kind)) {
+            h->sm.alloc(size_t(h->ncell) * 3 * h->csize(), "solid momentum");
+            hip_check(hipMemsetAsync(h->sm.p, 0, h->sm.bytes, h->stream), "memset");
+            h->alpha0.alloc(size_t(h->ncell) * h->csize(), "zero design");
+            hip_check(hipMemsetAsync(h->alpha0.p, 0, h->alpha0.bytes, h->stream), "memset");
+        }
         if (h->thermal) {
             h->phi.alloc(size_t(h->ncell) * h->csize(), "thermal design");
             hip_check(hipMemsetAsync(h->phi.p, 0, h->phi.bytes, h->stream), "memset");
@@ -1127,9 +1160,11 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         }
         // Collision constants.
         if (h->kind == K_MRT || h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES ||
-            is_wale(h->kind)) {
+            is_wale(h->kind) || is_psm(h->kind)) {
             std::vector<double> c;
-            if (is_wale(h->kind)) {
+            if (is_psm(h->kind)) {
+                c = {h->magic, h->psm_kappa, double(h->psm_law)};
+            } else if (is_wale(h->kind)) {
                 c = {h->magic, h->wale_c2, h->wale_floor};
             } else if (h->kind == K_TRT_LES || h->kind == K_BGK_LES) {
                 // [magic, k = 18 sqrt(2) C^2 (psm.rs:193 evaluation order), floor]
@@ -1216,6 +1251,44 @@ int lbm_set_design(lbm_handle* h, const double* alpha, uint64_t n) {
             std::vector<float> f(alpha, alpha + n);
             hip_check(hipMemcpy(h->alpha.p, f.data(), n * sizeof(float), hipMemcpyHostToDevice), "upload");
         }
+    });
+}
+
+int lbm_set_solid_momentum(lbm_handle* h, const double* m, uint64_t len) {
+    return guarded([&] {
+        if (!h || !m) fail(LBM_ERR_CONTRACT, "null argument");
+        if (!is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "solid momentum needs the occupancy coupling");
+        if (len != uint64_t(h->ncell) * 3) fail(LBM_ERR_SHAPE, "solid momentum needs three components per cell");
+        const long long n = h->ncell;
+        std::vector<double> soa(size_t(3 * n));
+        for (long long x = 0; x < n; x++)
+            for (int a = 0; a < 3; a++) {
+                const double v = m[3 * x + a];
+                if (!std::isfinite(v)) fail(LBM_ERR_CONTRACT, "solid momentum must be finite");
+                soa[size_t(a * n + x)] = v;
+            }
+        if (h->prec == LBM_F64) {
+            hip_check(hipMemcpy(h->sm.p, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+        } else {
+            std::vector<float> f(soa.begin(), soa.end());
+            hip_check(hipMemcpy(h->sm.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+        }
+    });
+}
+
+int lbm_solid_momentum_bar(lbm_handle* h, double* out, uint64_t len) {
+    return guarded([&] {
+        if (!h || !out) fail(LBM_ERR_CONTRACT, "null argument");
+        if (!is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "solid momentum needs the occupancy coupling");
+        if (len != uint64_t(h->ncell) * 3) fail(LBM_ERR_SHAPE, "solid momentum needs three components per cell");
+        if (!h->sm_bar.p) fail(LBM_ERR_INVALID, "no adjoint sweep has run on this handle");
+        const long long n = h->ncell;
+        std::vector<double> soa(size_t(3 * n));
+        hip_check(hipMemcpyAsync(soa.data(), h->sm_bar.p, soa.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                  "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        for (long long x = 0; x < n; x++)
+            for (int a = 0; a < 3; a++) out[3 * x + a] = soa[size_t(a * n + x)];
     });
 }
 
@@ -1330,6 +1403,7 @@ void check_scales(const lbm_vector_scales* sc) {
 int lbm_vector(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, double* out) {
     return guarded([&] {
         if (!h || !out) fail(LBM_ERR_CONTRACT, "null argument");
+        if (is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "terminal responses read a Brinkman coefficient; not with the occupancy coupling");
         check_scales(scales);
         if (phi && n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi needs one entry per cell");
         const LaunchCfg c = h->cfg();
@@ -1362,6 +1436,7 @@ int lbm_vector_vjp(lbm_handle* h, const double* phi, uint64_t n, const lbm_vecto
                    double* f_bar, uint64_t f_len, double* alpha_bar, double* phi_bar) {
     return guarded([&] {
         if (!h || !w || !f_bar || !alpha_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "terminal responses read a Brinkman coefficient; not with the occupancy coupling");
         check_scales(scales);
         if (f_len != uint64_t(h->ncell) * h->Q) fail(LBM_ERR_SHAPE, "f_bar length must be cells*Q");
         if (phi && n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi needs one entry per cell");
@@ -1495,7 +1570,8 @@ int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n) {
         d.alloc(size_t(h->ncell) * 4 * sizeof(double), "fields");
         double* dr = d.as<double>();
         double* du = dr + h->ncell;
-        hip_check(h->run->fields(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1), h->alpha.p,
+        hip_check(h->run->fields(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1),
+                                 is_psm(h->kind) ? h->alpha0.p : h->alpha.p,
                                  h->a, dr, du, h->cfg()),
                   "fields launch");
         hip_check(hipMemcpyAsync(rho, dr, n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
@@ -1647,6 +1723,7 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
                                        h->stream),
                       "port vjp");
         hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+        if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
         AdjArgs a = h->adj_args(h->F.p, int(t & 1));
         flow_adjoint(h, a);
         hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(t & 1), h->aos.as<double>(),
@@ -1780,6 +1857,9 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                         lam_init(h, seed_bound(h, seed) + lf_max,
                                  seed_bound(h, stage_obj(tmpl, obj, nsteps - 1, nsteps, false)));
                         ea.lsc = h->lsc_ptr();
+                        // the seed kernel writes alpha_bar; the momentum cotangent starts at zero
+                        if (h->sm_bar.p)
+                            hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
                         hip_check(h->run->objective(Q, h->has_mask, ea, seed, h->L.p, h->alpha_bar.as<double>(), 1,
                                                     h->cfg()),
                                   "seed launch");
@@ -1926,6 +2006,7 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
                 ensure_adjoint_buffers(h);
                 hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
                 hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+                if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
                 hip_check(hipMemsetAsync(h->phi_bar.p, 0, h->phi_bar.bytes, h->stream), "memset");
             }
             auto launch = [&](uint64_t s) {
@@ -1968,6 +2049,7 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             ensure_adjoint_buffers(h);
             hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
             hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
+            if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
             lam_init(h, 0.0, 0.0);
             AdjArgs a = h->adj_args(h->F.p, 0);
             a.odd = int(h->t & 1);

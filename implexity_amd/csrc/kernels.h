@@ -52,9 +52,13 @@ enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
 // K_BGK_WALE: TRT / BGK with the WALE closure (wale.h), constants [magic, C^2, floor].
 enum Kind : int {
     K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7,
-    K_TRT_WALE = 8, K_BGK_WALE = 9
+    K_TRT_WALE = 8, K_BGK_WALE = 9, K_TRT_PSM = 10, K_BGK_PSM = 11
 };
 constexpr bool is_wale(int kind) { return kind == K_TRT_WALE || kind == K_BGK_WALE; }
+// K_TRT_PSM / K_BGK_PSM: TRT / BGK with the occupancy coupling (psm.h); the design array
+// holds the occupancy d, constants [magic, kappa, law].
+constexpr bool is_psm(int kind) { return kind == K_TRT_PSM || kind == K_BGK_PSM; }
+constexpr int psm_base(int kind) { return kind == K_TRT_PSM ? K_TRT : K_BGK; }
 enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
@@ -107,6 +111,7 @@ struct StepArgs {
     double kappa, ratio_m1, source, tw;
     float tenc, tdec;
     const void* wu;   // WALE: u (3 x cells) and rho (cells) of the state read, arithmetic precision
+    const void* sm;   // occupancy coupling: solid momentum m (3 x cells), arithmetic precision
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
     long long npart;  // capacity of part, in blocks
@@ -145,6 +150,10 @@ struct AdjArgs {
     // cell's velocity gradient written for k_wale_transpose (9 x cells, G[a][b] at 3a+b).
     const void* wu;
     void* wgb;
+    // Occupancy coupling: solid momentum (as StepArgs::sm) and its cotangent, FP64
+    // (3 x cells), accumulated over the reverse sweep.
+    const void* sm;
+    double* sm_bar;
 };
 
 // 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
@@ -549,6 +558,7 @@ LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cur
 
 #include "thermal_cell.h"
 #include "wale.h"
+#include "psm.h"
 
 // ---------------------------------------------------------------------------------
 // Reductions.
@@ -985,8 +995,19 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                                        k, cell, G);
                 rate = wale_omega<C>(G, C(1) / C(A.omega), static_cast<const C*>(A.mrt));
             }
-            collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt), fs,
-                                            mac);
+            if constexpr (is_psm(KIND)) {  // the design is the occupancy d
+                const C* pk = static_cast<const C*>(A.mrt);
+                collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
+                if (alpha > C(0)) {
+                    const C* M = static_cast<const C*>(A.sm);
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                    psm_blend<Q, C, S::shifted>(f, mac, alpha, ms, pk, C(A.omega), fs);
+                }
+            } else {
+                collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt),
+                                                fs, mac);
+            }
             if (A.obj.kind != OBJ_NONE) {
                 const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
                 jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
@@ -1102,6 +1123,28 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                 C* W = static_cast<C*>(A.wgb);
 #pragma unroll
                 for (int t = 0; t < 9; t++) W[t * n + cell] = Gb[t / 3][t % 3];
+            } else if constexpr (is_psm(KIND)) {  // the design is the occupancy d; ab := d_bar
+                constexpr int BK = psm_base(KIND);
+                const C* pk = static_cast<const C*>(A.mrt);
+                if (alpha > C(0)) {
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C* M = static_cast<const C*>(A.sm);
+                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                    C fl[Q], fx[Q], rb, ub[3], db, mb[3];
+                    collide<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, fl, mac);
+                    psm_blend_vjp<Q, C, S::shifted>(f, fl, mac, alpha, ms, pk, C(A.omega), gb, fl, fx, rb, ub, db, mb);
+                    collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, fl, mac, fb, ab);
+                    psm_macro_chain<Q, C>(mac, rb, ub, fx);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) fb[q] += fx[q];
+                    ab = db;
+                    A.sm_bar[cell] += double(mb[0]);
+                    A.sm_bar[n + cell] += double(mb[1]);
+                    A.sm_bar[2 * n + cell] += double(mb[2]);
+                } else {
+                    collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
+                    ab = C(0);
+                }
             } else {
                 collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
                                                     static_cast<const C*>(A.mrt), gb, mac, fb, ab);

@@ -169,6 +169,8 @@ struct RunnerImpl final : Runner {
             if (kind == K_BGK_LES) LBM_LAUNCH((k_forward<P, Q, K_BGK_LES, M>), c, a);
             if (kind == K_TRT_WALE && a.wu) LBM_LAUNCH((k_forward<P, Q, K_TRT_WALE, M>), c, a);
             if (kind == K_BGK_WALE && a.wu) LBM_LAUNCH((k_forward<P, Q, K_BGK_WALE, M>), c, a);
+            if (kind == K_TRT_PSM && a.sm) LBM_LAUNCH((k_forward<P, Q, K_TRT_PSM, M>), c, a);
+            if (kind == K_BGK_PSM && a.sm) LBM_LAUNCH((k_forward<P, Q, K_BGK_PSM, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward<P, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -219,6 +221,8 @@ struct RunnerImpl final : Runner {
             if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, M>), c, a);
             if (kind == K_TRT_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_LES, M>), c, a);
             if (kind == K_BGK_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_LES, M>), c, a);
+            if (kind == K_TRT_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_PSM, M>), c, a);
+            if (kind == K_BGK_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSM, M>), c, a);
             if constexpr (LP != P_F16S) {
                 if (kind == K_TRT_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_WALE, M>), c, a);
                 if (kind == K_BGK_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_WALE, M>), c, a);
@@ -242,6 +246,7 @@ struct RunnerImpl final : Runner {
         if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
             return false;
         if (is_wale(kind) && !a.wu) return false;
+        if (is_psm(kind) && (!a.sm || !a.mrt)) return false;
         if ((a.obj.kind != OBJ_NONE || a.diag) && (!a.part || !a.jacc)) return false;
         if (a.diag && (!a.dstat || !a.macc)) return false;
         return true;
@@ -253,6 +258,7 @@ struct RunnerImpl final : Runner {
         if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
             return false;
         if (is_wale(kind) && (!a.wu || !a.wgb)) return false;
+        if (is_psm(kind) && (!a.sm || !a.sm_bar || !a.mrt)) return false;
         return true;
     }
     static bool ok_ports(bool mask, const PortArgs& a) {

@@ -32,6 +32,7 @@ PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
 ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
 COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
+COUPLING = {"brinkman": 0, "psm": 1, "psm_superposition": 2}
 OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3, "thermal": 4}
 MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
 
@@ -104,6 +105,7 @@ class _Config(ctypes.Structure):
         ("les_constant", ctypes.c_double), ("les_norm_floor", ctypes.c_double),
         ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
         ("n_sponges", ctypes.c_uint32), ("reserved3", ctypes.c_uint32), ("sponges", ctypes.c_void_p),
+        ("coupling", ctypes.c_uint32), ("reserved4", ctypes.c_uint32), ("psm_kappa", ctypes.c_double),
     ]
 
 
@@ -151,6 +153,8 @@ _SIGS = {
     "lbm_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_design": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_set_solid_momentum": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_solid_momentum_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_reset_initial": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_get_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
@@ -293,6 +297,11 @@ class LatticeConfig:
     wale_floor: float = 1e-12
     # Sponge layers blended after collision (moving/sponge.rs, field.rs:1461-1475)
     sponges: list = field(default_factory=list)
+    # Fluid-solid coupling (moving/psm.rs:198-267): "brinkman" (the design is alpha), or
+    # "psm" / "psm_superposition" (the design is the occupancy d, with a solid momentum
+    # per cell and the saturation width psm_kappa)
+    coupling: str = "brinkman"
+    psm_kappa: float = 0.0
 
 
 @dataclass
@@ -397,6 +406,10 @@ class Lattice:
                 sponges[i].wave_frequency = float(w.frequency)
                 sponges[i].wave_phase = float(w.phase)
                 sponges[i].wave_ramp = float(w.ramp)
+        if cfg.coupling not in COUPLING:
+            raise ValueError(f"coupling must be one of {sorted(COUPLING)}")
+        c.coupling = COUPLING[cfg.coupling]
+        c.psm_kappa = float(cfg.psm_kappa)
         c.n_sponges = len(cfg.sponges)
         c.sponges = ctypes.cast(sponges, ctypes.c_void_p)
         solid = None
@@ -432,6 +445,17 @@ class Lattice:
 
     def initial_populations(self):
         _check(lib().lbm_reset_initial(self._h))
+
+    # --- occupancy coupling ---------------------------------------------------------------
+    def set_solid_momentum(self, m):
+        """Solid momentum per cell, (cells, 3) (PSM couplings; the design is the occupancy)."""
+        a = np.ascontiguousarray(m, np.float64).reshape(-1)
+        _check(lib().lbm_set_solid_momentum(self._h, _p(a), a.size))
+
+    def solid_momentum_bar(self):
+        out = np.zeros(self.n * 3)
+        _check(lib().lbm_solid_momentum_bar(self._h, _p(out), out.size))
+        return out.reshape(self.n, 3)
 
     # --- coupled temperature field ------------------------------------------------------
     def set_thermal_design(self, phi):

@@ -133,6 +133,25 @@ typedef struct {
 
 #define LBM_MAX_SPONGES 8
 
+/* Fluid-solid coupling of a cell.  BRINKMAN: the design array is the Brinkman
+ * coefficient alpha (psm.rs:213-226 with m = 0).  PSM / PSM_SUPERPOSITION: the design
+ * array is the occupancy d (solid fraction; d = 0 is fluid), with the solid momentum m
+ * of lbm_set_solid_momentum (default 0), the Psm and PsmSuperposition branches of
+ * cell_update (moving/psm.rs:228-266): a cell with d > 0 leaves the fluid collision
+ * (u = j/rho + a/2, force rho a) as
+ *   PSM:                f + (1 - B)(fluid - f) + B (f_opp - eq_opp - f) + s_eq
+ *   PSM_SUPERPOSITION:  fluid + s_eq - B eq
+ * with s_eq = rho w (B + (B/d)(3 c.m + 4.5 (c.m)^2/d - 1.5 |m|^2/d)) and Noble &
+ * Torczynski's B = d t / (1 - d + t), t = tau+ - 1/2, saturated smoothly above
+ * 1 - psm_kappa (psm.rs:96-152).  The gradient's alpha_bar is then d_bar (0 at d = 0),
+ * and lbm_solid_momentum_bar returns m_bar.  BGK and TRT only; the objective is the
+ * pressure drop. */
+typedef enum {
+    LBM_COUPLING_BRINKMAN = 0,
+    LBM_COUPLING_PSM = 1,
+    LBM_COUPLING_PSM_SUPERPOSITION = 2
+} lbm_coupling;
+
 typedef enum {
     LBM_ADJ_AUTO = 0,
     LBM_ADJ_F64 = 1,
@@ -192,6 +211,9 @@ typedef struct {
     uint32_t n_sponges;
     uint32_t reserved3;
     const lbm_sponge* sponges;
+    uint32_t coupling;            /* lbm_coupling */
+    uint32_t reserved4;
+    double psm_kappa;             /* saturation width in [0, 1) (PSM couplings) */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
@@ -257,9 +279,16 @@ int lbm_last_error(char* buf, uint64_t len);
 int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_len, lbm_handle** out);
 int lbm_destroy(lbm_handle* h);
 
-/* alpha: one Brinkman coefficient per cell (solid cells ignored). */
+/* alpha: one Brinkman coefficient per cell (solid cells ignored); with a PSM coupling,
+ * the occupancy d of every cell. */
 int lbm_set_design(lbm_handle* h, const double* alpha, uint64_t n);
 int lbm_reset_initial(lbm_handle* h);
+
+/* PSM couplings: the solid momentum m of every cell (SolidInput::m, psm.rs:56-59; the
+ * solid velocity times d), AoS [cell][3], len = 3 * cells, and its cotangent summed over
+ * the last lbm_gradient / lbm_step_vjp (CellBar::m). */
+int lbm_set_solid_momentum(lbm_handle* h, const double* m, uint64_t len);
+int lbm_solid_momentum_bar(lbm_handle* h, double* out, uint64_t len);
 
 /* State exchange in the reference layout: layout 0 = AoS (x*Q+q), 1 = SoA (q*N+x). */
 int lbm_set_state(lbm_handle* h, int layout, const double* f, uint64_t len);

@@ -161,6 +161,15 @@ typedef struct {
     const double *sponge_weight;   /* s_l of each part */
     const double *sponge_phase;    /* [cos theta, sin theta] of each part */
     const double *sponge_target;   /* per sponge at the step's time: rho, mean[3], sin[3], cos[3] */
+    /* Occupancy coupling (moving/psm.rs:198-267, cell_update without Brinkman): with
+     * psm_law >= 0 the per-cell design array is the occupancy d, cells with d > 0 couple
+     * to a solid of momentum occ_m (3 per cell, NULL: at rest); orc_interval_vjp writes
+     * the momentum cotangent into occ_m_bar (3 per cell) when it is set. */
+    int32_t psm_law;               /* -1 none, 0 Psm, 1 PsmSuperposition */
+    int32_t pad2_;
+    double psm_kappa;              /* saturation width */
+    const double *occ_m;
+    double *occ_m_bar;
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -1135,6 +1144,168 @@ static void cell_vjp_rate(const orc_problem *p, const double *f, double alpha, d
 /* ------------------------------------------------------------------------- */
 /* NEE ports: moving/boundary.rs:445-506 (port_populations and its VJP).       */
 /* ------------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------- */
+/* Occupancy coupling: saturate / saturate_derivative / beta (psm.rs:96-152) and the   */
+/* Psm / PsmSuperposition branches of cell_update (psm.rs:228-266) and of          */
+/* cell_update_vjp (psm.rs:331-415, 460-510).                                       */
+/* ------------------------------------------------------------------------- */
+static double psm_saturate(double d, double kappa) {
+    double lo = 1.0 - kappa;
+    if (d <= lo) return d;
+    if (d >= 1.0 + kappa) return 1.0;
+    double t = (d - lo) / (2.0 * kappa);
+    double t2 = t * t;
+    double h = t - t2 * t + t2 * t2 * 0.5;
+    return lo + h * (2.0 * kappa);
+}
+
+static double psm_saturate_derivative(double d, double kappa) {
+    double lo = 1.0 - kappa;
+    if (d <= lo) return 1.0;
+    if (d >= 1.0 + kappa) return 0.0;
+    double t = (d - lo) / (2.0 * kappa);
+    double s = 1.0 - t;
+    return s * s * (t * 2.0 + 1.0);
+}
+
+/* (B / d, dB/dd / ..., dB/dt / d) as beta() returns them: B / d, d(B/d)/dd, d(B/d)/dt. */
+static void psm_beta(double d, double t, double kappa, double *b_over_d, double *beta_d, double *beta_t) {
+    if (d <= 1.0 - kappa) {
+        double den = 1.0 - d + t;
+        double den2 = den * den;
+        *b_over_d = t / den;
+        *beta_d = t / den2;
+        *beta_t = (1.0 - d) / den2;
+    } else {
+        double e = psm_saturate(d, kappa);
+        double de = psm_saturate_derivative(d, kappa);
+        double den = 1.0 - e + t;
+        double bf = e * t / den;
+        double den2 = den * den;
+        double bf_e = t * (t + 1.0) / den2;
+        double bf_t = e * (1.0 - e) / den2;
+        *b_over_d = bf / d;
+        *beta_d = bf_e * de / d - bf / (d * d);
+        *beta_t = bf_t / d;
+    }
+}
+
+void orc_psm_beta(double d, double t, double kappa, double *out3) { psm_beta(d, t, kappa, out3, out3 + 1, out3 + 2); }
+double orc_psm_saturate(double d, double kappa) { return psm_saturate(d, kappa); }
+
+/* One coupled cell (d > 0): post-collision populations and the momentum exchange dp. */
+static void psm_cell(const orc_problem *p, const double *f, double d, const double m[3], double *post, double dp[3]) {
+    const lattice_t *L = lat(p->Q);
+    int Q = p->Q;
+    double rho, j[3], den, sv[3], u[3], F[3];
+    brinkman_macro(L, f, 0.0, p->accel, &rho, j, &den, sv, u, F); /* u = j/rho + a/2, F = rho a */
+    double fluid[MAXQ], eq[MAXQ];
+    cell_collide_rate(p, f, 0.0, -1.0, fluid);
+    double t = 1.0 / p->omega - 0.5;
+    double b_over_d, beta_d, beta_t;
+    psm_beta(d, t, p->psm_kappa, &b_over_d, &beta_d, &beta_t);
+    double bb = b_over_d * d;
+    equilibrium(L, rho, u, eq);
+    double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    dp[0] = dp[1] = dp[2] = 0.0;
+    for (int i = 0; i < Q; i++) {
+        int o = L->opp[i];
+        double cm = dot_c(L, i, m);
+        double solid_eq =
+            rho * L->w[i] * (bb + b_over_d * cm * 3.0 + b_over_d * cm * cm * 4.5 / d - b_over_d * m2 * 1.5 / d);
+        double omega_s = p->psm_law == 1 ? solid_eq - bb * eq[i] : bb * (f[o] - eq[o] - f[i]) + solid_eq;
+        for (int a = 0; a < 3; a++)
+            if (L->cf[i][a] != 0.0) dp[a] += omega_s * L->cf[i][a];
+        post[i] = p->psm_law == 1 ? fluid[i] + omega_s : f[i] + (1.0 - bb) * (fluid[i] - f[i]) + omega_s;
+    }
+}
+
+/* Cotangents (f_bar, d_bar, m_bar) of a coupled cell given the cotangent g of post and
+ * dp_bar of dp.  The fluid collision's transpose is the Brinkman one at alpha = 0 (the
+ * same u and force); the equilibrium and solid-equilibrium terms are chained through
+ * rho and u = j/rho + a/2 separately (the reference sums both chains first). */
+static void psm_cell_vjp(const orc_problem *p, const double *f, double d, const double m[3], const double *g,
+                         const double dp_bar[3], double *f_bar, double *d_bar_out, double m_bar[3]) {
+    const lattice_t *L = lat(p->Q);
+    int Q = p->Q;
+    double rho, j[3], den, sv[3], u[3], F[3];
+    brinkman_macro(L, f, 0.0, p->accel, &rho, j, &den, sv, u, F);
+    double fluid[MAXQ], eq[MAXQ], eq_bar[MAXQ], fluid_bar[MAXQ];
+    cell_collide_rate(p, f, 0.0, -1.0, fluid);
+    double t = 1.0 / p->omega - 0.5;
+    double b_over_d, beta_d, beta_t;
+    psm_beta(d, t, p->psm_kappa, &b_over_d, &beta_d, &beta_t);
+    double bb = b_over_d * d;
+    equilibrium(L, rho, u, eq);
+    double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    double b_bar = 0.0, beta_bar = 0.0, wsum = 0.0, rho_bar = 0.0, u_bar[3] = {0.0, 0.0, 0.0}, d_bar = 0.0;
+    for (int i = 0; i < Q; i++) {
+        f_bar[i] = 0.0;
+        eq_bar[i] = 0.0;
+    }
+    m_bar[0] = m_bar[1] = m_bar[2] = 0.0;
+    const int sup = p->psm_law == 1;
+    for (int i = 0; i < Q; i++) {
+        int o = L->opp[i];
+        double h = g[i];
+        for (int a = 0; a < 3; a++)
+            if (L->cf[i][a] != 0.0) h += dp_bar[a] * L->cf[i][a];
+        if (sup) {
+            fluid_bar[i] = g[i];
+            b_bar += h * (rho * L->w[i] - eq[i]);
+            eq_bar[i] -= h * bb;
+        } else {
+            fluid_bar[i] = g[i] * (1.0 - bb);
+            f_bar[i] += g[i] * bb;
+            b_bar += g[i] * (f[i] - fluid[i]);
+            b_bar += h * (f[o] - eq[o] - f[i] + rho * L->w[i]);
+            f_bar[o] += h * bb;
+            f_bar[i] -= h * bb;
+            eq_bar[o] -= h * bb;
+        }
+        double cm = dot_c(L, i, m);
+        double w = h * L->w[i];
+        rho_bar += w * (bb + b_over_d * cm * 3.0 + b_over_d * cm * cm * 4.5 / d - b_over_d * m2 * 1.5 / d);
+        double wr = w * rho;
+        beta_bar += wr * (cm * 3.0 + cm * cm * 4.5 / d - m2 * 1.5 / d);
+        double cm_bar = wr * b_over_d * (cm * 9.0 / d + 3.0);
+        for (int a = 0; a < 3; a++)
+            if (L->cf[i][a] != 0.0) m_bar[a] += cm_bar * L->cf[i][a];
+        wsum += wr;
+        d_bar += wr * b_over_d * (m2 * 1.5 - cm * cm * 4.5) / (d * d);
+    }
+    for (int a = 0; a < 3; a++) m_bar[a] -= wsum * b_over_d * m[a] * 3.0 / d;
+    equilibrium_vjp(L, rho, u, eq_bar, &rho_bar, u_bar);
+    beta_bar += b_bar * d;
+    d_bar += b_bar * b_over_d + beta_bar * beta_d;
+    double fb[MAXQ], ab;
+    cell_vjp_rate(p, f, 0.0, -1.0, fluid_bar, fb, &ab, NULL);
+    double j_bar[3];
+    for (int a = 0; a < 3; a++) {
+        j_bar[a] = u_bar[a] / rho;
+        rho_bar -= u_bar[a] * j[a] / (rho * rho);
+    }
+    for (int i = 0; i < Q; i++) {
+        double v = f_bar[i] + fb[i] + rho_bar;
+        for (int a = 0; a < 3; a++)
+            if (L->cf[i][a] != 0.0) v += j_bar[a] * L->cf[i][a];
+        f_bar[i] = v;
+    }
+    *d_bar_out = d_bar;
+}
+
+void orc_psm_cell(const orc_problem *p, const double *f, double d, const double *m, double *post, double *dp) {
+    psm_cell(p, f, d, m, post, dp);
+}
+
+void orc_psm_cell_vjp(const orc_problem *p, const double *f, double d, const double *m, const double *g,
+                      const double *dp_bar, double *f_bar, double *d_bar, double *m_bar) {
+    psm_cell_vjp(p, f, d, m, g, dp_bar, f_bar, d_bar, m_bar);
+}
+
+static const double PSM_REST[3] = {0.0, 0.0, 0.0};
+static inline const double *psm_m(const orc_problem *p, long x) { return p->occ_m ? p->occ_m + 3 * x : PSM_REST; }
+
 void orc_port_populations(int Q, const double *nb, int kind, const double value[3], double *out) {
     const lattice_t *L = lat(Q);
     double rho_n, j[3], u_n[3], eb[MAXQ], en[MAXQ];
@@ -1411,6 +1582,14 @@ void orc_interval(const orc_problem *p, const uint32_t *source, const double *al
             for (int q = 0; q < Q; q++) post[x * Q + q] = before[x * Q + q];
             continue;
         }
+        if (p->psm_law >= 0) {
+            double dp[3];
+            if (alpha[x] > 0.0)
+                psm_cell(p, before + x * Q, alpha[x], psm_m(p, x), post + x * Q, dp);
+            else
+                cell_collide_rate(p, before + x * Q, 0.0, -1.0, post + x * Q);
+            continue;
+        }
         cell_collide_rate(p, before + x * Q, alpha[x], rate ? rate[x] : -1.0, post + x * Q);
     }
     free(rate);
@@ -1474,6 +1653,7 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
         wale_rates(p, before, rate, wu, wr);
     }
     const double tau0 = 1.0 / p->omega;
+    if (p->psm_law >= 0 && p->occ_m_bar) memset(p->occ_m_bar, 0, sizeof(double) * 3 * n);
     double g[MAXQ], fb[MAXQ];
 #pragma omp parallel for schedule(static) private(g, fb)
     for (long x = 0; x < n; x++) {
@@ -1492,6 +1672,21 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
             continue;
         }
         double ab, rb = 0.0;
+        if (p->psm_law >= 0) {
+            static const double no_dp[3] = {0.0, 0.0, 0.0};
+            double mb[3] = {0.0, 0.0, 0.0};
+            if (alpha[x] > 0.0) {
+                psm_cell_vjp(p, before + x * Q, alpha[x], psm_m(p, x), g, no_dp, fb, &ab, mb);
+            } else {
+                cell_vjp_rate(p, before + x * Q, 0.0, -1.0, g, fb, &ab, NULL);
+                ab = 0.0;
+            }
+            for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
+            alpha_bar[x] = ab;
+            if (p->occ_m_bar)
+                for (int a = 0; a < 3; a++) p->occ_m_bar[3 * x + a] = mb[a];
+            continue;
+        }
         cell_vjp_rate(p, before + x * Q, alpha[x], wale ? rate[x] : -1.0, g, fb, &ab, wale ? &rb : NULL);
         for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
         alpha_bar[x] = ab;

@@ -31,6 +31,12 @@ Parity pins (each checked in tests/test_oracle_*.py):
 * Smagorinsky / WALE     -- Hou et al.'s and Nicoud & Ducros' closed forms, pure-shear
                             invariance, linear-field stencil, central differences, the
                             dot-product identity of the step (tests/test_oracle_les.py)
+* occupancy coupling     -- Noble & Torczynski's weighting B = d t / (1 - d + t), the
+                            saturation's C1 joins, full-solid cells moving with the solid
+                            (mass kept, momentum rho (u_s + a/2)), the momentum exchange
+                            as the change of momentum, d -> 0 continuity, finite
+                            differences of the cell and step transposes
+                            (tests/test_oracle_psm.py)
 * sponge layers          -- the ramp's Hermite conditions, mirror symmetry of opposite
                             faces, the far-field equilibrium as a fixed point, the blend's
                             mass and momentum, travelling-wave invariance, the step's
@@ -75,6 +81,8 @@ def lib():
         _lib.orc_odd_rate_derivative.argtypes = [d, d]
         _lib.orc_obj_dissipation.restype = d
         _lib.orc_obj_pressure_drop.restype = d
+        _lib.orc_psm_saturate.restype = d
+        _lib.orc_psm_saturate.argtypes = [d, d]
         del pd
     return _lib
 
@@ -98,7 +106,11 @@ class _Problem(ctypes.Structure):
         ("sponge_cell", ctypes.c_void_p), ("sponge_sigma", ctypes.c_void_p), ("sponge_off", ctypes.c_void_p),
         ("sponge_part", ctypes.c_void_p), ("sponge_weight", ctypes.c_void_p), ("sponge_phase", ctypes.c_void_p),
         ("sponge_target", ctypes.c_void_p),
+        ("psm_law", ctypes.c_int32), ("pad2_", ctypes.c_int32), ("psm_kappa", ctypes.c_double),
+        ("occ_m", ctypes.c_void_p), ("occ_m_bar", ctypes.c_void_p),
     ]
+
+PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1}
 
 
 def _ptr(a):
@@ -125,6 +137,17 @@ def mrt_basis(Q: int):
 
 def odd_rate(omega: float, magic: float) -> float:
     return lib().orc_odd_rate(omega, magic)
+
+
+def psm_beta(d: float, t: float, kappa: float):
+    """beta of psm.rs:123-140: (B/d, d(B/d)/dd, d(B/d)/dt)."""
+    out = np.zeros(3)
+    lib().orc_psm_beta(ctypes.c_double(d), ctypes.c_double(t), ctypes.c_double(kappa), _ptr(out))
+    return tuple(out)
+
+
+def psm_saturate(d: float, kappa: float) -> float:
+    return lib().orc_psm_saturate(ctypes.c_double(d), ctypes.c_double(kappa))
 
 
 @dataclass
@@ -296,6 +319,11 @@ class Problem:
     wale_floor: float = 1e-12
     # Sponge layers (moving/sponge.rs) blended after collision
     sponges: list = field(default_factory=list)
+    # Occupancy coupling (moving/psm.rs:198-267): "brinkman" (the design is alpha), or
+    # "psm" / "psm_superposition" (the design is the occupancy d; solid momentum m per cell)
+    coupling: str = "brinkman"
+    psm_kappa: float = 0.0
+    solid_momentum: np.ndarray | None = None  # (cells, 3)
 
 
 class Oracle:
@@ -339,6 +367,14 @@ class Oracle:
         p.sponge_part, p.sponge_weight, p.sponge_phase = _ptr(part), _ptr(weight), _ptr(phase)
         self._target = sponge_targets(prob.sponges, 0.0)
         p.sponge_target = _ptr(self._target)
+        p.psm_law, p.psm_kappa = PSM_LAW[prob.coupling], float(prob.psm_kappa)
+        self.psm = prob.coupling != "brinkman"
+        self._m = None
+        self.m_bar = np.zeros((self.n, 3)) if self.psm else None
+        if self.psm:
+            self._m = (np.zeros((self.n, 3)) if prob.solid_momentum is None
+                       else np.ascontiguousarray(np.asarray(prob.solid_momentum, np.float64).reshape(self.n, 3)))
+            p.occ_m, p.occ_m_bar = _ptr(self._m), _ptr(self.m_bar)
         self._p = p
         self._pp = ctypes.byref(p)
         self.source = np.zeros(self.n * self.Q, np.uint32)
@@ -421,6 +457,21 @@ class Oracle:
         lib().orc_port_populations_vjp(ctypes.c_int(self.Q), _ptr(nb), ctypes.c_int(kind), _ptr(v), _ptr(g),
                                        _ptr(out))
         return out
+
+    # --- occupancy coupling (moving/psm.rs:96-152, 228-266, 331-510) -------------------------
+    def psm_cell(self, f, d, m):
+        post, dp = np.zeros(self.Q), np.zeros(3)
+        lib().orc_psm_cell(self._pp, _ptr(np.ascontiguousarray(f, np.float64)), ctypes.c_double(d),
+                           _ptr(np.ascontiguousarray(m, np.float64)), _ptr(post), _ptr(dp))
+        return post, dp
+
+    def psm_cell_vjp(self, f, d, m, g, dp_bar=(0.0, 0.0, 0.0)):
+        fb, mb = np.zeros(self.Q), np.zeros(3)
+        db = ctypes.c_double(0.0)
+        lib().orc_psm_cell_vjp(self._pp, _ptr(np.ascontiguousarray(f, np.float64)), ctypes.c_double(d),
+                               _ptr(np.ascontiguousarray(m, np.float64)), _ptr(np.ascontiguousarray(g, np.float64)),
+                               _ptr(np.ascontiguousarray(dp_bar, np.float64)), _ptr(fb), ctypes.byref(db), _ptr(mb))
+        return fb, db.value, mb
 
     # --- WALE closure (moving/wale.rs) -------------------------------------------------------
     def wale_rates(self, state):
@@ -586,9 +637,10 @@ class Objective:
         return self.weight if lo <= s <= hi else 0.0
 
 
-def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0=None):
+def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0=None, m_bar=None):
     """scan_adjoint with Checkpointing::Every(k) / All (scan.rs:140-246) over the oracle
-    step.  Returns (J, alpha_bar, lambda_0, final_state)."""
+    step.  Returns (J, alpha_bar, lambda_0, final_state); with occupancy coupling the
+    solid-momentum cotangent is summed into m_bar (cells x 3) when given."""
     alpha = np.asarray(alpha, np.float64)
     seg = steps if every is None else every
     x = orc.initial() if f0 is None else np.array(f0, np.float64)
@@ -625,6 +677,8 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
                 abar += ga
             fb, ab = orc.interval_vjp(alpha, states[t - start], lam, t=t)
             abar += ab
+            if m_bar is not None:
+                m_bar += orc.m_bar
             lam = fb
         after = states[0]
     return J, abar, lam, final

@@ -94,6 +94,11 @@ def _create(**kw):
      "positive frequency"),
     (dict(sponges=[lbm.Sponge("xmin", 2, 0.6), lbm.Sponge("xmin", 2, 0.6)]), "below one"),
     (dict(sponges=[lbm.Sponge("xmin", 1, 0.1)] * 9), "at most 8"),
+    (dict(coupling="psm", collision="mrt"), "plain BGK and TRT"),
+    (dict(coupling="psm_superposition", les_constant=0.1), "plain BGK and TRT"),
+    (dict(coupling="psm", psm_kappa=1.0), "kappa"),
+    (dict(coupling="psm", precision="bf16s"), "FP64, FP32 and FP16"),
+    (dict(coupling="psm", thermal=True, thermal_kappa=0.02), "temperature field"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are
