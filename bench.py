@@ -221,6 +221,12 @@ def main():
                     help="kernels mode: Smagorinsky constant of the timed operator (0: plain TRT)")
     ap.add_argument("--wale", type=float, default=0.0,
                     help="kernels mode: WALE constant of the timed operator (0: plain TRT)")
+    ap.add_argument("--coupling", default="brinkman", choices=["brinkman", "psm", "psm_superposition"],
+                    help="kernels mode: fluid-solid coupling (PSM laws take the blob design's solid fraction as "
+                         "occupancy)")
+    ap.add_argument("--sponge", type=int, default=0,
+                    help="kernels mode: thickness of sponge layers on the outlet and on one side wall (0: none); "
+                         "also times the sponge pass and its transpose")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the lattice step runs on one "
@@ -244,23 +250,49 @@ def main():
         # Per-kernel timing only (used under rocprofv3): forward and adjoint launches.
         from implexity_amd.lbm import Lattice
 
+        from implexity_amd.lbm import Sponge, Wave
+
         cfg, alpha = alpha_cfg(args.kernel_precision)
         cfg.les_constant = args.les
         cfg.wale_constant = args.wale
         if args.wale and cfg.adjoint_storage == "f16":
             cfg.adjoint_storage = "auto"  # WALE keeps FP32 cotangents
+        design_in = alpha
+        if args.coupling != "brinkman":
+            from implexity_amd import design
+
+            cfg.coupling, cfg.psm_kappa = args.coupling, 0.1
+            design_in = 1.0 - design.blob_design(shape, 64, 3, radius=(6.0, 20.0)).reshape(-1)  # solid fraction
+        if args.sponge:
+            cfg.sponges = [Sponge("xmax", args.sponge, 0.5, density=1.0, velocity=(0.03, 0.0, 0.0)),
+                           Sponge("ymin", args.sponge, 0.3, wave=Wave((0.003, 0.0, 0.0), (0.0, 1.0, 0.0), 0.1, 0.002))]
+        sp = {}
         with Lattice(cfg) as lat:
-            lat.set_design(alpha)
+            lat.set_design(design_in)
             lat.initial_populations()
             tf = lat.time_kernel("forward", args.launches)
             ta = lat.time_kernel("adjoint", args.launches)
             ts = lat.time_kernel("stream", args.launches)
+            if args.sponge:
+                t_sp = lat.time_kernel("sponge", args.launches)
+                t_spv = lat.time_kernel("sponge_transpose", args.launches)
+                T = args.sponge
+                nx, ny, nz = shape
+                cells = 2 * T * ny * nz - T * T * nz  # the two layers share a T x T x nz corner
+                parts = cells + T * T * nz
+                store = {"f64": 8, "f32": 4, "f16s": 2, "bf16s": 2}[args.kernel_precision]
+                lam = {"f64": 8, "f32": 4, "f16s": 2 if cfg.adjoint_storage in ("auto", "f16") else 4}.get(
+                    args.kernel_precision, 4)
+                meta = cells * (4 + 8 + 4) + parts * (4 + 8 + 16)
+                sp = {"sponge_thickness": T, "sponge_cells": cells, "sponge_ms": t_sp, "sponge_transpose_ms": t_spv,
+                      "sponge_GBs": (2 * 19 * store * cells + meta) / (t_sp * 1e-3) / 1e9,
+                      "sponge_transpose_GBs": (2 * 19 * lam * cells + cells * (4 + 8 + 4)) / (t_spv * 1e-3) / 1e9}
         from implexity_amd.lbm import copy_bandwidth
 
         copy_gbs, copy_ms = copy_bandwidth(1 << 30, args.launches)  # 1 GiB read + 1 GiB written
         n = int(np.prod(shape))
         out = {"precision": args.kernel_precision, "les_constant": args.les, "wale_constant": args.wale,
-               "forward_ms": tf, "adjoint_ms": ta,
+               "coupling": args.coupling, **sp, "forward_ms": tf, "adjoint_ms": ta,
                "stream_only_ms": ts,
                "forward_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (tf * 1e-3) / 1e9,
                "adjoint_GBs": algorithmic_bytes("adjoint", args.kernel_precision) * n / (ta * 1e-3) / 1e9,

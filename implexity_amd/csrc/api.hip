@@ -895,8 +895,9 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
                 fail(LBM_ERR_CONTRACT, "the occupancy coupling supports FP64, FP32 and FP16 storage");
             if (!(cfg->psm_kappa >= 0.0 && cfg->psm_kappa < 1.0))
                 fail(LBM_ERR_CONTRACT, "saturation width kappa must lie in [0, 1)");
-            h->kind = h->kind == K_TRT ? int(K_TRT_PSM) : int(K_BGK_PSM);
             h->psm_law = cfg->coupling == LBM_COUPLING_PSM ? 0 : 1;
+            h->kind = h->kind == K_TRT ? int(h->psm_law ? K_TRT_PSMS : K_TRT_PSM)
+                                       : int(h->psm_law ? K_BGK_PSMS : K_BGK_PSM);
             h->psm_kappa = cfg->psm_kappa;
         }
         if (h->kind == K_CUMULANT && h->Q != 27)
@@ -975,7 +976,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
         const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES || h->kind == K_TRT_WALE ||
-                              h->kind == K_TRT_PSM;
+                              h->kind == K_TRT_PSM || h->kind == K_TRT_PSMS;
         if (trt_like && !(std::isfinite(h->magic) && h->magic > 0.0))
             fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
         h->om = trt_like ? odd_rate(h->omega, h->magic) : h->omega;
@@ -2012,6 +2013,26 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             auto launch = [&](uint64_t s) {
                 ThermalArgs ta = h->thermal_args(h->F.p, h->F.p, h->t + s);
                 hip_check(h->trun->adjoint(h->Q, ta, h->cfg()), "thermal launch");
+            };
+            launch(0);  // untimed
+            hip_check(hipEventRecord(h->ev0, h->stream), "event");
+            for (uint64_t s = 0; s < launches; s++) launch(s + 1);
+            hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        } else if (which == 5 || which == 6) {
+            // the sponge blend pass (5, in place on the live state, which it moves towards
+            // the far field) or its transpose (6, on a zero cotangent) alone
+            if (!h->nsc) fail(LBM_ERR_CONTRACT, "no sponge layer on this handle");
+            if (which == 6) {
+                ensure_adjoint_buffers(h);
+                hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
+                lam_init(h, 0.0, 0.0);
+            }
+            auto launch = [&](uint64_t s) {
+                const SpongeArgs sa = h->sponge_args(h->t + s);
+                if (which == 5)
+                    hip_check(h->run->sponge(h->Q, sa, h->F.p, nullptr, h->stream), "sponge launch");
+                else
+                    hip_check(h->run->sponge_vjp(h->Q, sa, h->L.p, h->lsc_ptr(), h->stream), "sponge transpose launch");
             };
             launch(0);  // untimed
             hip_check(hipEventRecord(h->ev0, h->stream), "event");

@@ -52,13 +52,15 @@ enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
 // K_BGK_WALE: TRT / BGK with the WALE closure (wale.h), constants [magic, C^2, floor].
 enum Kind : int {
     K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7,
-    K_TRT_WALE = 8, K_BGK_WALE = 9, K_TRT_PSM = 10, K_BGK_PSM = 11
+    K_TRT_WALE = 8, K_BGK_WALE = 9, K_TRT_PSM = 10, K_BGK_PSM = 11, K_TRT_PSMS = 12, K_BGK_PSMS = 13
 };
 constexpr bool is_wale(int kind) { return kind == K_TRT_WALE || kind == K_BGK_WALE; }
-// K_TRT_PSM / K_BGK_PSM: TRT / BGK with the occupancy coupling (psm.h); the design array
-// holds the occupancy d, constants [magic, kappa, law].
-constexpr bool is_psm(int kind) { return kind == K_TRT_PSM || kind == K_BGK_PSM; }
-constexpr int psm_base(int kind) { return kind == K_TRT_PSM ? K_TRT : K_BGK; }
+// K_TRT_PSM / K_BGK_PSM (Psm law) and K_TRT_PSMS / K_BGK_PSMS (PsmSuperposition): TRT /
+// BGK with the occupancy coupling (psm.h); the design array holds the occupancy d,
+// constants [magic, kappa, law].
+constexpr bool is_psm(int kind) { return kind >= K_TRT_PSM && kind <= K_BGK_PSMS; }
+constexpr bool psm_sup(int kind) { return kind == K_TRT_PSMS || kind == K_BGK_PSMS; }
+constexpr int psm_base(int kind) { return kind == K_TRT_PSM || kind == K_TRT_PSMS ? K_TRT : K_BGK; }
 enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
@@ -1002,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                     const C* M = static_cast<const C*>(A.sm);
                     const long long n = (long long)g.nx * g.ny * g.nz;
                     const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
-                    psm_blend<Q, C, S::shifted>(f, mac, alpha, ms, pk, C(A.omega), fs);
+                    psm_blend<Q, C, S::shifted, psm_sup(KIND)>(f, mac, alpha, ms, pk, C(A.omega), fs);
                 }
             } else {
                 collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt),
@@ -1127,16 +1129,20 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                 constexpr int BK = psm_base(KIND);
                 const C* pk = static_cast<const C*>(A.mrt);
                 if (alpha > C(0)) {
+                    constexpr bool SUP = psm_sup(KIND);
                     const long long n = (long long)g.nx * g.ny * g.nz;
                     const C* M = static_cast<const C*>(A.sm);
                     const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
-                    C fl[Q], fx[Q], rb, ub[3], db, mb[3];
-                    collide<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, fl, mac);
-                    psm_blend_vjp<Q, C, S::shifted>(f, fl, mac, alpha, ms, pk, C(A.omega), gb, fl, fx, rb, ub, db, mb);
-                    collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, fl, mac, fb, ab);
-                    psm_macro_chain<Q, C>(mac, rb, ub, fx);
+                    C gfl = C(0);  // <g, fluid collision> (Psm law)
+                    if constexpr (!SUP) {
+                        C fl[Q];
+                        collide<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, fl, mac);
 #pragma unroll
-                    for (int q = 0; q < Q; q++) fb[q] += fx[q];
+                        for (int q = 0; q < Q; q++) gfl += gb[q] * fl[q];
+                    }
+                    collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
+                    C db, mb[3];
+                    psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, alpha, ms, pk, C(A.omega), gb, gfl, fb, db, mb);
                     ab = db;
                     A.sm_bar[cell] += double(mb[0]);
                     A.sm_bar[n + cell] += double(mb[1]);

@@ -9,7 +9,8 @@
 // term is a deviation from w: s_eq - B w = w (B (rho - 1) + rho (B/d)(...)), and the
 // w terms of the bounce-back part cancel.
 // Constants of a handle with the coupling: pk[0] = magic, pk[1] = kappa, pk[2] = law
-// (0 Psm, 1 PsmSuperposition).  Included inside namespace lbm by kernels.h.
+// (0 Psm, 1 PsmSuperposition; the kernels take the law as a template argument).
+// Included inside namespace lbm by kernels.h.
 #pragma once
 
 template <class C>
@@ -56,117 +57,165 @@ LBM_DEV void psm_beta(C d, C t, C kappa, C& bod, C& bd, C& bt) {
     }
 }
 
+// x / d: a division in double precision (the reference's operation order), a product
+// with the reciprocal in single precision (the shifted storages).
+template <class C, bool SH>
+LBM_DEV C psm_over(C x, C d, C inv_d) {
+    if constexpr (SH)
+        return x * inv_d;
+    else
+        return x / d;
+}
+
 // The solid-equilibrium term of direction i: s_eq (f storage) or s_eq - B w (shifted).
 template <int Q, int i, class C, bool SH>
-LBM_DEV C psm_solid_eq(const Macro<Q, C>& m, C bb, C bod, C cm, C m2, C d) {
+LBM_DEV C psm_solid_eq(const Macro<Q, C>& m, C bb, C bod, C cm, C m2, C d, C inv_d) {
     constexpr double w = weight<Q>(i);
     if constexpr (SH)
-        return C(w) * (bb * m.drho + m.rho * (bod * cm * C(3.0) + bod * cm * cm * C(4.5) / d - bod * m2 * C(1.5) / d));
+        return C(w) * (bb * m.drho + m.rho * (bod * cm * C(3.0) + bod * cm * cm * C(4.5) * inv_d -
+                                              bod * m2 * C(1.5) * inv_d));
     else
         return m.rho * C(w) * (bb + bod * cm * C(3.0) + bod * cm * cm * C(4.5) / d - bod * m2 * C(1.5) / d);
 }
 
+// eq_k (f storage) or eq_k - w_k (shifted) of the cell's macroscopic state.
+template <int Q, int k, class C, bool SH>
+LBM_DEV C psm_eq(const Macro<Q, C>& m, C usq) {
+    const C cu = dotc<Q, k>(m.u);
+    if constexpr (SH)
+        return C(weight<Q>(k)) * (m.drho + m.rho * (cu * C(3.0) + cu * cu * C(4.5) - usq * C(1.5)));
+    else
+        return m.rho * C(weight<Q>(k)) * ((C(1) - usq * C(1.5)) + cu * C(3.0) + cu * cu * C(4.5));
+}
+
 // out holds the fluid collision of v on entry and the coupled post-collision values on
-// return (same representation as v); m is the alpha = 0 macroscopic state of v.
-template <int Q, class C, bool SH>
+// return (same representation as v); m is the alpha = 0 macroscopic state of v.  SUP
+// selects PsmSuperposition.  The equilibrium is formed per direction, not held.
+template <int Q, class C, bool SH, bool SUP>
 LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, C* out) {
     const C t = C(1) / omega - C(0.5);
     C bod, bd, bt;
     psm_beta<C>(d, t, pk[1], bod, bd, bt);
     const C bb = bod * d;
-    C eq[Q];
-    if constexpr (SH)
-        equilibrium_shifted<Q>(m, eq);
-    else
-        equilibrium<Q>(m.rho, m.u, eq);
     const C m2 = ms[0] * ms[0] + ms[1] * ms[1] + ms[2] * ms[2];
-    const bool sup = pk[2] != C(0);
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        constexpr int o = opp<Q>(i);
-        const C cm = dotc<Q, i>(ms);
-        const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d);
-        const C os = sup ? se - bb * eq[i] : bb * (v[o] - eq[o] - v[i]) + se;
-        out[i] = sup ? out[i] + os : v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
-    });
+    const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
+    const C inv_d = SH ? C(1) / d : C(0);
+    if constexpr (!SH) {
+        // the reference's equilibrium (lattice.rs), bit for bit in double precision
+        C eq[Q];
+        equilibrium<Q>(m.rho, m.u, eq);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr int o = opp<Q>(i);
+            const C cm = dotc<Q, i>(ms);
+            const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d, inv_d);
+            if constexpr (SUP) {
+                out[i] = out[i] + (se - bb * eq[i]);
+            } else {
+                const C os = bb * (v[o] - eq[o] - v[i]) + se;
+                out[i] = v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
+            }
+        });
+    } else {
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr int o = opp<Q>(i);
+            const C cm = dotc<Q, i>(ms);
+            const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d, inv_d);
+            if constexpr (SUP) {
+                out[i] = out[i] + (se - bb * psm_eq<Q, i, C, SH>(m, usq));
+            } else {
+                const C os = bb * (v[o] - psm_eq<Q, o, C, SH>(m, usq) - v[i]) + se;
+                out[i] = v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
+            }
+        });
+    }
 }
 
-// Transpose of psm_blend given the cotangent g of post: the cotangent of the fluid
-// collision's output (fluid_bar), the direct cotangent of v (fb, overwritten), the
-// cotangents of rho and u = j/rho + a/2 from the equilibrium and solid-equilibrium terms
-// (rho_bar, u_bar, overwritten), of d and of m (psm.rs:331-415).  fluid is the fluid
-// collision of v; fluid_bar may be the same array.
-template <int Q, class C, bool SH>
-LBM_DEV void psm_blend_vjp(const C* v, const C* fluid, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega,
-                           const C* g, C* fluid_bar, C* fb, C& rho_bar, C u_bar[3], C& d_bar, C m_bar[3]) {
+// One direction's share of the equilibrium transpose (cell.h equilibrium_vjp) for the
+// cotangent e of eq_k; usq_bar is folded into u_bar by psm_eq_vjp_end.
+template <int Q, int k, class C>
+LBM_DEV void psm_eq_vjp(const Macro<Q, C>& m, C usq, C e, C& rho_bar, C u_bar[3], C& usq_bar) {
+    const C cu = dotc<Q, k>(m.u);
+    const C gw = e * C(weight<Q>(k));
+    rho_bar += gw * ((C(1) - usq * C(1.5)) + cu * C(3.0) + cu * cu * C(4.5));
+    const C E = gw * m.rho;
+    const C cu_bar = E * (cu * C(9.0) + C(3.0));
+    if constexpr (cvec<Q>(k, 0) != 0) u_bar[0] += cu_bar * C(cvec<Q>(k, 0));
+    if constexpr (cvec<Q>(k, 1) != 0) u_bar[1] += cu_bar * C(cvec<Q>(k, 1));
+    if constexpr (cvec<Q>(k, 2) != 0) u_bar[2] += cu_bar * C(cvec<Q>(k, 2));
+    usq_bar -= E * C(1.5);
+}
+
+// Transpose of psm_blend (psm.rs:331-415) given the cotangent g of post.  On entry fb
+// holds the transpose of the fluid collision applied to g; gfl = <g, fluid> (Psm only).
+// Adds the blend's own terms to fb (the fluid part scaled by 1 - B for Psm, the
+// bounce-back and equilibrium terms, and rho, u = j/rho + a/2 carried to the
+// populations) and returns the cotangents of d and m.
+template <int Q, class C, bool SH, bool SUP>
+LBM_DEV void psm_blend_vjp(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, const C* g,
+                           C gfl, C* fb, C& d_bar, C m_bar[3]) {
     const C t = C(1) / omega - C(0.5);
     C bod, bd, bt;
     psm_beta<C>(d, t, pk[1], bod, bd, bt);
     const C bb = bod * d;
-    C eq[Q], eq_bar[Q];
-    if constexpr (SH)
-        equilibrium_shifted<Q>(m, eq);
-    else
-        equilibrium<Q>(m.rho, m.u, eq);
     const C m2 = ms[0] * ms[0] + ms[1] * ms[1] + ms[2] * ms[2];
-    const bool sup = pk[2] != C(0);
+    const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
     // rho w - eq of the bounce-back and superposition terms: w (rho - 1) - (eq - w) when shifted
     const C rw = SH ? m.drho : m.rho;
-    C b_bar = C(0), beta_bar = C(0), wsum = C(0);
-    rho_bar = C(0);
+    const C inv_d = SH ? C(1) / d : C(0);
+    C b_bar = C(0), beta_bar = C(0), wsum = C(0), rho_bar = C(0), usq_bar = C(0);
+    C u_bar[3] = {C(0), C(0), C(0)};
     d_bar = C(0);
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        u_bar[a] = C(0);
-        m_bar[a] = C(0);
-    }
+    for (int a = 0; a < 3; a++) m_bar[a] = C(0);
+    if constexpr (!SUP) {
+        // post = f + (1 - B)(fluid - f) + ...: the fluid transpose scales by 1 - B, f gets B g,
+        // and B gets <g, f - fluid>
+        C gf = C(0);
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        fb[q] = C(0);
-        eq_bar[q] = C(0);
+        for (int q = 0; q < Q; q++) {
+            fb[q] = fb[q] * (C(1) - bb) + g[q] * bb;
+            gf += g[q] * v[q];
+        }
+        b_bar = gf - gfl;
     }
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int o = opp<Q>(i);
         constexpr double w = weight<Q>(i);
         const C h = g[i];
-        const C fl = fluid[i];  // read before fluid_bar[i] is written: the two may alias
-        if (sup) {
-            fluid_bar[i] = g[i];
-            b_bar += h * (rw * C(w) - eq[i]);
-            eq_bar[i] -= h * bb;
+        if constexpr (SUP) {
+            b_bar += h * (rw * C(w) - psm_eq<Q, i, C, SH>(m, usq));
+            psm_eq_vjp<Q, i, C>(m, usq, -h * bb, rho_bar, u_bar, usq_bar);
         } else {
-            fluid_bar[i] = g[i] * (C(1) - bb);
-            fb[i] += g[i] * bb;
-            b_bar += g[i] * (v[i] - fl);
-            b_bar += h * (v[o] - eq[o] - v[i] + rw * C(w));
+            b_bar += h * (v[o] - psm_eq<Q, o, C, SH>(m, usq) - v[i] + rw * C(w));
             fb[o] += h * bb;
             fb[i] -= h * bb;
-            eq_bar[o] -= h * bb;
+            psm_eq_vjp<Q, o, C>(m, usq, -h * bb, rho_bar, u_bar, usq_bar);
         }
         const C cm = dotc<Q, i>(ms);
         const C hw = h * C(w);
-        rho_bar += hw * (bb + bod * cm * C(3.0) + bod * cm * cm * C(4.5) / d - bod * m2 * C(1.5) / d);
+        rho_bar += hw * (bb + bod * cm * C(3.0) + psm_over<C, SH>(bod * cm * cm * C(4.5), d, inv_d) -
+                         psm_over<C, SH>(bod * m2 * C(1.5), d, inv_d));
         const C wr = hw * m.rho;
-        beta_bar += wr * (cm * C(3.0) + cm * cm * C(4.5) / d - m2 * C(1.5) / d);
-        const C cm_bar = wr * bod * (cm * C(9.0) / d + C(3.0));
+        beta_bar += wr * (cm * C(3.0) + psm_over<C, SH>(cm * cm * C(4.5), d, inv_d) -
+                          psm_over<C, SH>(m2 * C(1.5), d, inv_d));
+        const C cm_bar = wr * bod * (psm_over<C, SH>(cm * C(9.0), d, inv_d) + C(3.0));
         if constexpr (cvec<Q>(i, 0) != 0) m_bar[0] += cm_bar * C(cvec<Q>(i, 0));
         if constexpr (cvec<Q>(i, 1) != 0) m_bar[1] += cm_bar * C(cvec<Q>(i, 1));
         if constexpr (cvec<Q>(i, 2) != 0) m_bar[2] += cm_bar * C(cvec<Q>(i, 2));
         wsum += wr;
-        d_bar += wr * bod * (m2 * C(1.5) - cm * cm * C(4.5)) / (d * d);
+        d_bar += psm_over<C, SH>(psm_over<C, SH>(wr * bod * (m2 * C(1.5) - cm * cm * C(4.5)), d, inv_d), d, inv_d);
     });
 #pragma unroll
-    for (int a = 0; a < 3; a++) m_bar[a] -= wsum * bod * ms[a] * C(3.0) / d;
-    equilibrium_vjp<Q>(m.rho, m.u, eq_bar, rho_bar, u_bar);
+    for (int a = 0; a < 3; a++) {
+        m_bar[a] -= psm_over<C, SH>(wsum * bod * ms[a] * C(3.0), d, inv_d);
+        u_bar[a] += usq_bar * m.u[a] * C(2.0);
+    }
     beta_bar += b_bar * d;
     d_bar += b_bar * bod + beta_bar * bd;
-}
-
-// fb[q] += rho_bar + c_q . (u_bar / rho) - u_bar . j / rho^2: the cotangents of rho and
-// u = j/rho + a/2 carried to the populations.
-template <int Q, class C>
-LBM_DEV void psm_macro_chain(const Macro<Q, C>& m, C rho_bar, const C u_bar[3], C* fb) {
+    // rho and u = j/rho + a/2 carried to the populations
     C jb[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -175,10 +224,10 @@ LBM_DEV void psm_macro_chain(const Macro<Q, C>& m, C rho_bar, const C u_bar[3], 
     }
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        C v = fb[q] + rho_bar;
-        if constexpr (cvec<Q>(q, 0) != 0) v += jb[0] * C(cvec<Q>(q, 0));
-        if constexpr (cvec<Q>(q, 1) != 0) v += jb[1] * C(cvec<Q>(q, 1));
-        if constexpr (cvec<Q>(q, 2) != 0) v += jb[2] * C(cvec<Q>(q, 2));
-        fb[q] = v;
+        C x = fb[q] + rho_bar;
+        if constexpr (cvec<Q>(q, 0) != 0) x += jb[0] * C(cvec<Q>(q, 0));
+        if constexpr (cvec<Q>(q, 1) != 0) x += jb[1] * C(cvec<Q>(q, 1));
+        if constexpr (cvec<Q>(q, 2) != 0) x += jb[2] * C(cvec<Q>(q, 2));
+        fb[q] = x;
     });
 }

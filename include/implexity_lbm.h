@@ -372,7 +372,8 @@ int lbm_get_thermal_lambda(lbm_handle* h, double* lambda, uint64_t len);
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
  * a copy, 3 forward step of the flow and the coupled temperature field, 4 thermal
- * adjoint step) on the handle's stream, measured with HIP events. */
+ * adjoint step, 5 sponge blend pass, 6 sponge transpose) on the handle's stream,
+ * measured with HIP events. */
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
 /* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
 int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);
