@@ -221,7 +221,8 @@ def main():
                     help="kernels mode: Smagorinsky constant of the timed operator (0: plain TRT)")
     ap.add_argument("--wale", type=float, default=0.0,
                     help="kernels mode: WALE constant of the timed operator (0: plain TRT)")
-    ap.add_argument("--coupling", default="brinkman", choices=["brinkman", "psm", "psm_superposition"],
+    ap.add_argument("--coupling", default="brinkman",
+                    choices=["brinkman", "psm", "psm_superposition", "brinkman_occupancy"],
                     help="kernels mode: fluid-solid coupling (PSM laws take the blob design's solid fraction as "
                          "occupancy)")
     ap.add_argument("--sponge", type=int, default=0,
@@ -261,7 +262,7 @@ def main():
         if args.coupling != "brinkman":
             from implexity_amd import design
 
-            cfg.coupling, cfg.psm_kappa = args.coupling, 0.1
+            cfg.coupling, cfg.psm_kappa, cfg.psm_c_alpha, cfg.psm_drag_shape = args.coupling, 0.1, 0.1, 0.1
             design_in = 1.0 - design.blob_design(shape, 64, 3, radius=(6.0, 20.0)).reshape(-1)  # solid fraction
         if args.sponge:
             cfg.sponges = [Sponge("xmax", args.sponge, 0.5, density=1.0, velocity=(0.03, 0.0, 0.0)),

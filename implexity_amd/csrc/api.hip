@@ -229,8 +229,8 @@ struct lbm_handle {
     // Occupancy coupling (K_TRT_PSM / K_BGK_PSM): the design array is the occupancy d;
     // solid momentum m (SoA, arithmetic precision), its FP64 cotangent, and a zero design
     // for the Brinkman-velocity outputs (fields).
-    int psm_law = -1;
-    double psm_kappa = 0.0;
+    int psm_law = -1;  // 0 Psm, 1 PsmSuperposition, 2 Brinkman with a moving solid
+    double psm_kappa = 0.0, psm_c_alpha = 0.0, psm_drag_shape = 0.0;
     DevBuf sm, sm_bar, alpha0;
     const ThermalRunner* trun = nullptr;
 
@@ -886,7 +886,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             h->les_c = cfg->les_constant;
             h->les_floor = cfg->les_norm_floor;
         }
-        if (cfg->coupling > LBM_COUPLING_PSM_SUPERPOSITION) fail(LBM_ERR_CONTRACT, "unknown coupling law");
+        if (cfg->coupling > LBM_COUPLING_BRINKMAN_OCCUPANCY) fail(LBM_ERR_CONTRACT, "unknown coupling law");
         if (cfg->coupling != LBM_COUPLING_BRINKMAN) {
             if (h->kind != K_TRT && h->kind != K_BGK)
                 fail(LBM_ERR_CONTRACT, "the occupancy coupling applies to the plain BGK and TRT operators");
@@ -895,9 +895,17 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
                 fail(LBM_ERR_CONTRACT, "the occupancy coupling supports FP64, FP32 and FP16 storage");
             if (!(cfg->psm_kappa >= 0.0 && cfg->psm_kappa < 1.0))
                 fail(LBM_ERR_CONTRACT, "saturation width kappa must lie in [0, 1)");
-            h->psm_law = cfg->coupling == LBM_COUPLING_PSM ? 0 : 1;
-            h->kind = h->kind == K_TRT ? int(h->psm_law ? K_TRT_PSMS : K_TRT_PSM)
-                                       : int(h->psm_law ? K_BGK_PSMS : K_BGK_PSM);
+            h->psm_law = cfg->coupling == LBM_COUPLING_PSM ? 0 : (cfg->coupling == LBM_COUPLING_PSM_SUPERPOSITION ? 1 : 2);
+            static const int kinds[3][2] = {{K_TRT_PSM, K_BGK_PSM}, {K_TRT_PSMS, K_BGK_PSMS}, {K_TRT_BRO, K_BGK_BRO}};
+            h->kind = kinds[h->psm_law][h->kind == K_TRT ? 0 : 1];
+            if (h->psm_law == 2) {
+                if (!(std::isfinite(cfg->psm_c_alpha) && cfg->psm_c_alpha >= 0.0))
+                    fail(LBM_ERR_CONTRACT, "drag coefficient c_alpha must be finite and non-negative");
+                if (!(std::isfinite(cfg->psm_drag_shape) && cfg->psm_drag_shape > 0.0))
+                    fail(LBM_ERR_CONTRACT, "drag shape must be finite and positive");
+                h->psm_c_alpha = cfg->psm_c_alpha;
+                h->psm_drag_shape = cfg->psm_drag_shape;
+            }
             h->psm_kappa = cfg->psm_kappa;
         }
         if (h->kind == K_CUMULANT && h->Q != 27)
@@ -976,7 +984,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
         const bool trt_like = h->kind == K_TRT || h->kind == K_MRT || h->kind == K_TRT_LES || h->kind == K_TRT_WALE ||
-                              h->kind == K_TRT_PSM || h->kind == K_TRT_PSMS;
+                              h->kind == K_TRT_PSM || h->kind == K_TRT_PSMS || h->kind == K_TRT_BRO;
         if (trt_like && !(std::isfinite(h->magic) && h->magic > 0.0))
             fail(LBM_ERR_CONTRACT, "TRT magic parameter must be finite and positive");
         h->om = trt_like ? odd_rate(h->omega, h->magic) : h->omega;
@@ -1164,7 +1172,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             is_wale(h->kind) || is_psm(h->kind)) {
             std::vector<double> c;
             if (is_psm(h->kind)) {
-                c = {h->magic, h->psm_kappa, double(h->psm_law)};
+                c = {h->magic, h->psm_kappa, double(h->psm_law), h->psm_c_alpha, h->psm_drag_shape};
             } else if (is_wale(h->kind)) {
                 c = {h->magic, h->wale_c2, h->wale_floor};
             } else if (h->kind == K_TRT_LES || h->kind == K_BGK_LES) {

@@ -52,15 +52,20 @@ enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
 // K_BGK_WALE: TRT / BGK with the WALE closure (wale.h), constants [magic, C^2, floor].
 enum Kind : int {
     K_BGK = 0, K_TRT = 1, K_MRT = 2, K_CUMULANT = 3, K_BGK_GUO = 4, K_STREAM_ONLY = 5, K_TRT_LES = 6, K_BGK_LES = 7,
-    K_TRT_WALE = 8, K_BGK_WALE = 9, K_TRT_PSM = 10, K_BGK_PSM = 11, K_TRT_PSMS = 12, K_BGK_PSMS = 13
+    K_TRT_WALE = 8, K_BGK_WALE = 9, K_TRT_PSM = 10, K_BGK_PSM = 11, K_TRT_PSMS = 12, K_BGK_PSMS = 13,
+    K_TRT_BRO = 14, K_BGK_BRO = 15
 };
 constexpr bool is_wale(int kind) { return kind == K_TRT_WALE || kind == K_BGK_WALE; }
-// K_TRT_PSM / K_BGK_PSM (Psm law) and K_TRT_PSMS / K_BGK_PSMS (PsmSuperposition): TRT /
-// BGK with the occupancy coupling (psm.h); the design array holds the occupancy d,
-// constants [magic, kappa, law].
-constexpr bool is_psm(int kind) { return kind >= K_TRT_PSM && kind <= K_BGK_PSMS; }
+// K_TRT_PSM / K_BGK_PSM (Psm law), K_TRT_PSMS / K_BGK_PSMS (PsmSuperposition) and
+// K_TRT_BRO / K_BGK_BRO (Brinkman law with a moving solid): TRT / BGK with the occupancy
+// coupling (psm.h); the design array holds the occupancy d, constants
+// [magic, kappa, law, c_alpha, drag shape].
+constexpr bool is_psm(int kind) { return kind >= K_TRT_PSM && kind <= K_BGK_BRO; }
 constexpr bool psm_sup(int kind) { return kind == K_TRT_PSMS || kind == K_BGK_PSMS; }
-constexpr int psm_base(int kind) { return kind == K_TRT_PSM || kind == K_TRT_PSMS ? K_TRT : K_BGK; }
+constexpr bool is_bro(int kind) { return kind == K_TRT_BRO || kind == K_BGK_BRO; }
+constexpr int psm_base(int kind) {
+    return kind == K_TRT_PSM || kind == K_TRT_PSMS || kind == K_TRT_BRO ? K_TRT : K_BGK;
+}
 enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
@@ -997,7 +1002,24 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                                        k, cell, G);
                 rate = wale_omega<C>(G, C(1) / C(A.omega), static_cast<const C*>(A.mrt));
             }
-            if constexpr (is_psm(KIND)) {  // the design is the occupancy d
+            if constexpr (is_bro(KIND)) {  // the design is the occupancy d
+                const C* pk = static_cast<const C*>(A.mrt);
+                if (alpha > C(0)) {
+                    const C* M = static_cast<const C*>(A.sm);
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                    C al, ga, part[4], z[3];
+                    psm_resistance<C>(alpha, pk[3], pk[4], pk[1], al, ga, part);
+                    brinkman_moving_macro<Q, C, S::shifted>(f, al, ga, a, ms, mac, z);
+                    const C om = psm_base(KIND) == K_TRT ? C(A.om) : rate;
+                    if constexpr (S::shifted)
+                        collide_pairs_shifted<Q>(f, mac, rate, om, fs);
+                    else
+                        collide_pairs<Q>(f, mac.rho, mac.u, mac.F, rate, om, fs);
+                } else {
+                    collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
+                }
+            } else if constexpr (is_psm(KIND)) {  // the design is the occupancy d
                 const C* pk = static_cast<const C*>(A.mrt);
                 collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
                 if (alpha > C(0)) {
@@ -1125,6 +1147,31 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                 C* W = static_cast<C*>(A.wgb);
 #pragma unroll
                 for (int t = 0; t < 9; t++) W[t * n + cell] = Gb[t / 3][t % 3];
+            } else if constexpr (is_bro(KIND)) {  // the design is the occupancy d; ab := d_bar
+                constexpr int BK = psm_base(KIND);
+                const C* pk = static_cast<const C*>(A.mrt);
+                if (alpha > C(0)) {
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C* M = static_cast<const C*>(A.sm);
+                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                    C al, ga, part[4], z[3], cb_rho, cb_u[3], cb_F[3];
+                    psm_resistance<C>(alpha, pk[3], pk[4], pk[1], al, ga, part);
+                    brinkman_moving_macro<Q, C, S::shifted>(f, al, ga, a, ms, mac, z);
+                    const C om = BK == K_TRT ? C(A.om) : C(A.omega);
+                    if constexpr (std::is_same<C, float>::value)
+                        collide_pairs_vjp_fast<Q>(mac.rho, mac.u, mac.F, C(A.omega), om, gb, fb, cb_rho, cb_u, cb_F);
+                    else
+                        collide_pairs_vjp<Q>(mac.rho, mac.u, mac.F, C(A.omega), om, gb, fb, cb_rho, cb_u, cb_F);
+                    C abr, gbr, mb[3];
+                    brinkman_moving_chain<Q, C>(mac, al, ga, a, ms, z, cb_rho, cb_u, cb_F, fb, abr, gbr, mb);
+                    ab = abr * part[0] + gbr * part[2];
+                    A.sm_bar[cell] += double(mb[0]);
+                    A.sm_bar[n + cell] += double(mb[1]);
+                    A.sm_bar[2 * n + cell] += double(mb[2]);
+                } else {
+                    collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
+                    ab = C(0);
+                }
             } else if constexpr (is_psm(KIND)) {  // the design is the occupancy d; ab := d_bar
                 constexpr int BK = psm_base(KIND);
                 const C* pk = static_cast<const C*>(A.mrt);

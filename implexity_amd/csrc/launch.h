@@ -173,6 +173,8 @@ struct RunnerImpl final : Runner {
             if (kind == K_BGK_PSM && a.sm) LBM_LAUNCH((k_forward<P, Q, K_BGK_PSM, M>), c, a);
             if (kind == K_TRT_PSMS && a.sm) LBM_LAUNCH((k_forward<P, Q, K_TRT_PSMS, M>), c, a);
             if (kind == K_BGK_PSMS && a.sm) LBM_LAUNCH((k_forward<P, Q, K_BGK_PSMS, M>), c, a);
+            if (kind == K_TRT_BRO && a.sm) LBM_LAUNCH((k_forward<P, Q, K_TRT_BRO, M>), c, a);
+            if (kind == K_BGK_BRO && a.sm) LBM_LAUNCH((k_forward<P, Q, K_BGK_BRO, M>), c, a);
             if constexpr (Q == 19) {
                 if (kind == K_BGK_GUO) LBM_LAUNCH((k_forward<P, 19, K_BGK_GUO, M>), c, a);
             } else {
@@ -227,6 +229,8 @@ struct RunnerImpl final : Runner {
             if (kind == K_BGK_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSM, M>), c, a);
             if (kind == K_TRT_PSMS && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_PSMS, M>), c, a);
             if (kind == K_BGK_PSMS && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSMS, M>), c, a);
+            if (kind == K_TRT_BRO && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_BRO, M>), c, a);
+            if (kind == K_BGK_BRO && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_BRO, M>), c, a);
             if constexpr (LP != P_F16S) {
                 if (kind == K_TRT_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_WALE, M>), c, a);
                 if (kind == K_BGK_WALE && a.wu && a.wgb) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_WALE, M>), c, a);

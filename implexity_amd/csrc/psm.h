@@ -231,3 +231,95 @@ LBM_DEV void psm_blend_vjp(const C* v, const Macro<Q, C>& m, C d, const C ms[3],
         fb[q] = x;
     });
 }
+
+// ---------------------------------------------------------------------------------
+// Brinkman law with a moving solid (psm.rs:143-160, 213-226, 301-330, 478-492): the
+// occupancy sets alpha = c_alpha e / ((q + 1) - e) and gamma (= alpha / d), the
+// macroscopic step is u = (j/rho + a/2 + gamma m/2) / (1 + alpha/2) with the force
+// rho (a + z), z = gamma m - alpha u, and the fluid collision runs on them.  At m = 0 it
+// is the stationary Brinkman operator with alpha = resistance(d).
+// Constants: pk[3] = c_alpha, pk[4] = drag shape q.
+// ---------------------------------------------------------------------------------
+template <class C>
+LBM_DEV void psm_resistance(C d, C c_alpha, C q, C kappa, C& alpha, C& gamma, C part[4]) {
+    const C e = psm_saturate(d, kappa);
+    const C de = psm_saturate_derivative(d, kappa);
+    const C den = C(1) * (q + C(1)) - e;
+    alpha = c_alpha * e / den;
+    const C a_e = c_alpha * (q + C(1)) / (den * den);
+    const C a_c = e / den;
+    part[0] = a_e * de;
+    part[1] = a_c;
+    if (d <= C(1) - kappa) {
+        gamma = c_alpha / den;
+        part[2] = c_alpha / (den * den);
+        part[3] = C(1) / den;
+    } else {
+        gamma = alpha / d;
+        part[2] = a_e * de / d - alpha / (d * d);
+        part[3] = a_c / d;
+    }
+}
+
+template <int Q, class C, bool SH>
+LBM_DEV void brinkman_moving_macro(const C* v, C alpha, C gamma, const C a[3], const C ms[3], Macro<Q, C>& m, C z[3]) {
+    if constexpr (SH) {
+        C dr;
+        moments<Q>(v, dr, m.j);
+        m.drho = dr;
+        m.rho = C(1) + dr;
+    } else {
+        moments<Q>(v, m.rho, m.j);
+        m.drho = m.rho - C(1);
+    }
+    m.den = C(1) + alpha * C(0.5);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        m.sv[k] = qdiv(m.j[k], m.rho) + a[k] * C(0.5) + gamma * ms[k] * C(0.5);
+        m.u[k] = qdiv(m.sv[k], m.den);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        z[k] = gamma * ms[k] - alpha * m.u[k];
+        m.F[k] = m.rho * (a[k] + z[k]);
+    }
+}
+
+// Cotangents of (f, alpha, gamma, m) from those of the collision's (rho, u, F); fb holds
+// the collision's direct cotangent on entry and the total on return.
+template <int Q, class C>
+LBM_DEV void brinkman_moving_chain(const Macro<Q, C>& m, C alpha, C gamma, const C a[3], const C ms[3], const C z[3],
+                                   C cb_rho, const C cb_u[3], const C cb_F[3], C* fb, C& alpha_bar, C& gamma_bar,
+                                   C m_bar[3]) {
+    C rho_bar = cb_rho;
+    C u_bar[3] = {cb_u[0], cb_u[1], cb_u[2]};
+    alpha_bar = C(0);
+    gamma_bar = C(0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        rho_bar += cb_F[k] * (a[k] + z[k]);
+        const C zb = cb_F[k] * m.rho;
+        gamma_bar += zb * ms[k];
+        m_bar[k] = zb * gamma;
+        alpha_bar -= zb * m.u[k];
+        u_bar[k] -= zb * alpha;
+    }
+    C jb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const C s_bar = qdiv(u_bar[k], m.den);
+        alpha_bar -= qdiv(u_bar[k] * m.sv[k], m.den * m.den) * C(0.5);
+        jb[k] = qdiv(s_bar, m.rho);
+        rho_bar -= qdiv(s_bar * m.j[k], m.rho * m.rho);
+        gamma_bar += s_bar * ms[k] * C(0.5);
+        m_bar[k] += s_bar * gamma * C(0.5);
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        C x = fb[q] + rho_bar;
+        if constexpr (cvec<Q>(q, 0) != 0) x += jb[0] * C(cvec<Q>(q, 0));
+        if constexpr (cvec<Q>(q, 1) != 0) x += jb[1] * C(cvec<Q>(q, 1));
+        if constexpr (cvec<Q>(q, 2) != 0) x += jb[2] * C(cvec<Q>(q, 2));
+        fb[q] = x;
+    });
+}

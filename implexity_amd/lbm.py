@@ -32,7 +32,7 @@ PRECISION = {"f64": 0, "f32": 1, "f16s": 2, "bf16s": 3}
 ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
 COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
-COUPLING = {"brinkman": 0, "psm": 1, "psm_superposition": 2}
+COUPLING = {"brinkman": 0, "psm": 1, "psm_superposition": 2, "brinkman_occupancy": 3}
 OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3, "thermal": 4}
 MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
 
@@ -106,6 +106,7 @@ class _Config(ctypes.Structure):
         ("wale_constant", ctypes.c_double), ("wale_floor", ctypes.c_double),
         ("n_sponges", ctypes.c_uint32), ("reserved3", ctypes.c_uint32), ("sponges", ctypes.c_void_p),
         ("coupling", ctypes.c_uint32), ("reserved4", ctypes.c_uint32), ("psm_kappa", ctypes.c_double),
+        ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
     ]
 
 
@@ -298,10 +299,13 @@ class LatticeConfig:
     # Sponge layers blended after collision (moving/sponge.rs, field.rs:1461-1475)
     sponges: list = field(default_factory=list)
     # Fluid-solid coupling (moving/psm.rs:198-267): "brinkman" (the design is alpha), or
-    # "psm" / "psm_superposition" (the design is the occupancy d, with a solid momentum
-    # per cell and the saturation width psm_kappa)
+    # "psm" / "psm_superposition" / "brinkman_occupancy" (the design is the occupancy d,
+    # with a solid momentum per cell and the saturation width psm_kappa; the last with
+    # alpha = resistance(d; psm_c_alpha, psm_drag_shape))
     coupling: str = "brinkman"
     psm_kappa: float = 0.0
+    psm_c_alpha: float = 0.0
+    psm_drag_shape: float = 0.1
 
 
 @dataclass
@@ -410,6 +414,8 @@ class Lattice:
             raise ValueError(f"coupling must be one of {sorted(COUPLING)}")
         c.coupling = COUPLING[cfg.coupling]
         c.psm_kappa = float(cfg.psm_kappa)
+        c.psm_c_alpha = float(cfg.psm_c_alpha)
+        c.psm_drag_shape = float(cfg.psm_drag_shape)
         c.n_sponges = len(cfg.sponges)
         c.sponges = ctypes.cast(sponges, ctypes.c_void_p)
         solid = None

@@ -143,13 +143,17 @@ typedef struct {
  *   PSM_SUPERPOSITION:  fluid + s_eq - B eq
  * with s_eq = rho w (B + (B/d)(3 c.m + 4.5 (c.m)^2/d - 1.5 |m|^2/d)) and Noble &
  * Torczynski's B = d t / (1 - d + t), t = tau+ - 1/2, saturated smoothly above
- * 1 - psm_kappa (psm.rs:96-152).  The gradient's alpha_bar is then d_bar (0 at d = 0),
- * and lbm_solid_momentum_bar returns m_bar.  BGK and TRT only; the objective is the
- * pressure drop. */
+ * 1 - psm_kappa (psm.rs:96-152).  BRINKMAN_OCCUPANCY: the Brinkman branch of
+ * cell_update with a moving solid (psm.rs:143-160, 213-226): alpha = psm_c_alpha e /
+ * ((psm_drag_shape + 1) - e) of the saturated occupancy e, gamma = alpha / d,
+ * u = (j/rho + a/2 + gamma m/2) / (1 + alpha/2) and the force rho (a + gamma m - alpha u).
+ * The gradient's alpha_bar is then d_bar (0 at d = 0), and lbm_solid_momentum_bar
+ * returns m_bar.  BGK and TRT only; the objective is the pressure drop. */
 typedef enum {
     LBM_COUPLING_BRINKMAN = 0,
     LBM_COUPLING_PSM = 1,
-    LBM_COUPLING_PSM_SUPERPOSITION = 2
+    LBM_COUPLING_PSM_SUPERPOSITION = 2,
+    LBM_COUPLING_BRINKMAN_OCCUPANCY = 3
 } lbm_coupling;
 
 typedef enum {
@@ -213,7 +217,9 @@ typedef struct {
     const lbm_sponge* sponges;
     uint32_t coupling;            /* lbm_coupling */
     uint32_t reserved4;
-    double psm_kappa;             /* saturation width in [0, 1) (PSM couplings) */
+    double psm_kappa;             /* saturation width in [0, 1) (occupancy couplings) */
+    double psm_c_alpha;           /* BRINKMAN_OCCUPANCY: drag coefficient c_alpha >= 0 */
+    double psm_drag_shape;        /* BRINKMAN_OCCUPANCY: drag shape q > 0 */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

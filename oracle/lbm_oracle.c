@@ -165,11 +165,13 @@ typedef struct {
      * psm_law >= 0 the per-cell design array is the occupancy d, cells with d > 0 couple
      * to a solid of momentum occ_m (3 per cell, NULL: at rest); orc_interval_vjp writes
      * the momentum cotangent into occ_m_bar (3 per cell) when it is set. */
-    int32_t psm_law;               /* -1 none, 0 Psm, 1 PsmSuperposition */
+    int32_t psm_law;               /* -1 none, 0 Psm, 1 PsmSuperposition, 2 Brinkman */
     int32_t pad2_;
     double psm_kappa;              /* saturation width */
     const double *occ_m;
     double *occ_m_bar;
+    double psm_c_alpha;            /* Brinkman law: c_alpha (CellParameters) */
+    double psm_drag_shape;         /* Brinkman law: drag_shape q */
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -1191,10 +1193,111 @@ static void psm_beta(double d, double t, double kappa, double *b_over_d, double 
 }
 
 void orc_psm_beta(double d, double t, double kappa, double *out3) { psm_beta(d, t, kappa, out3, out3 + 1, out3 + 2); }
+
+/* resistance (psm.rs:143-160): alpha = c_alpha e / ((q + 1) - e), gamma (= alpha / d) and
+ * the partials [d alpha/d d, d alpha/d c_alpha, d gamma/d d, d gamma/d c_alpha]. */
+static void psm_resistance(double d, double c_alpha, double q, double kappa, double *alpha, double *gamma,
+                           double partials[4]) {
+    double e = psm_saturate(d, kappa);
+    double de = psm_saturate_derivative(d, kappa);
+    double den = 1.0 * (q + 1.0) - e;
+    double a = c_alpha * e / den;
+    double a_e = c_alpha * (q + 1.0) / (den * den);
+    double a_c = e / den;
+    *alpha = a;
+    partials[0] = a_e * de;
+    partials[1] = a_c;
+    if (d <= 1.0 - kappa) {
+        *gamma = c_alpha / den;
+        partials[2] = c_alpha / (den * den);
+        partials[3] = 1.0 / den;
+    } else {
+        *gamma = a / d;
+        partials[2] = a_e * de / d - a / (d * d);
+        partials[3] = a_c / d;
+    }
+}
+
+void orc_psm_resistance(double d, double c_alpha, double q, double kappa, double *out6) {
+    psm_resistance(d, c_alpha, q, kappa, out6, out6 + 1, out6 + 2);
+}
+
+/* The Brinkman branch of cell_update with a moving solid (psm.rs:213-226):
+ * u = (j/rho + a/2 + gamma m/2) / (1 + alpha/2), z = gamma m - alpha u, force rho (a + z),
+ * dp = rho z. */
+static void brinkman_moving_macro(const lattice_t *L, const double *f, double alpha, double gamma, const double a[3],
+                                  const double m[3], double *rho, double j[3], double *den, double sv[3], double u[3],
+                                  double F[3], double z[3]) {
+    moments(L, f, rho, j);
+    *den = 1.0 + alpha * 0.5;
+    for (int d = 0; d < 3; d++) {
+        sv[d] = j[d] / *rho + a[d] * 0.5 + gamma * m[d] * 0.5;
+        u[d] = sv[d] / *den;
+    }
+    for (int d = 0; d < 3; d++) {
+        z[d] = gamma * m[d] - alpha * u[d];
+        F[d] = *rho * (a[d] + z[d]);
+    }
+}
+
+static void brinkman_moving_cell(const orc_problem *p, const double *f, double d, const double m[3], double *post,
+                                 double dp[3]) {
+    const lattice_t *L = lat(p->Q);
+    double alpha, gamma, part[4], rho, j[3], den, sv[3], u[3], F[3], z[3];
+    psm_resistance(d, p->psm_c_alpha, p->psm_drag_shape, p->psm_kappa, &alpha, &gamma, part);
+    brinkman_moving_macro(L, f, alpha, gamma, p->accel, m, &rho, j, &den, sv, u, F, z);
+    double om = p->kind == K_TRT ? orc_odd_rate(p->omega, p->magic) : p->omega;
+    collide_pairs(L, f, rho, u, F, p->omega, om, post);
+    for (int a = 0; a < 3; a++) dp[a] = rho * z[a];
+}
+
+/* cell_update_vjp, Brinkman branch (psm.rs:301-330, 478-492) without LES and omega. */
+static void brinkman_moving_vjp(const orc_problem *p, const double *f, double d, const double m[3], const double *g,
+                                const double dp_bar[3], double *f_bar, double *d_bar, double m_bar[3]) {
+    const lattice_t *L = lat(p->Q);
+    int Q = p->Q;
+    double alpha, gamma, part[4], rho, j[3], den, sv[3], u[3], F[3], z[3];
+    psm_resistance(d, p->psm_c_alpha, p->psm_drag_shape, p->psm_kappa, &alpha, &gamma, part);
+    brinkman_moving_macro(L, f, alpha, gamma, p->accel, m, &rho, j, &den, sv, u, F, z);
+    double om = p->kind == K_TRT ? orc_odd_rate(p->omega, p->magic) : p->omega;
+    double cb_rho, cb_u[3], cb_F[3], ob, omb;
+    collide_pairs_vjp(L, f, rho, u, F, p->omega, om, g, f_bar, &cb_rho, cb_u, cb_F, &ob, &omb);
+    double rho_bar = cb_rho, u_bar[3] = {cb_u[0], cb_u[1], cb_u[2]};
+    double alpha_bar = 0.0, gamma_bar = 0.0;
+    for (int a = 0; a < 3; a++) {
+        m_bar[a] = 0.0;
+        rho_bar += cb_F[a] * (p->accel[a] + z[a]) + dp_bar[a] * z[a];
+        double zb = (cb_F[a] + dp_bar[a]) * rho;
+        gamma_bar += zb * m[a];
+        m_bar[a] += zb * gamma;
+        alpha_bar -= zb * u[a];
+        u_bar[a] -= zb * alpha;
+    }
+    double j_bar[3];
+    for (int a = 0; a < 3; a++) {
+        double s_bar = u_bar[a] / den;
+        alpha_bar -= u_bar[a] * sv[a] / (den * den) * 0.5;
+        j_bar[a] = s_bar / rho;
+        rho_bar -= s_bar * j[a] / (rho * rho);
+        gamma_bar += s_bar * m[a] * 0.5;
+        m_bar[a] += s_bar * gamma * 0.5;
+    }
+    *d_bar = alpha_bar * part[0] + gamma_bar * part[2];
+    for (int i = 0; i < Q; i++) {
+        double v = f_bar[i] + rho_bar;
+        for (int a = 0; a < 3; a++)
+            if (L->cf[i][a] != 0.0) v += j_bar[a] * L->cf[i][a];
+        f_bar[i] = v;
+    }
+}
 double orc_psm_saturate(double d, double kappa) { return psm_saturate(d, kappa); }
 
 /* One coupled cell (d > 0): post-collision populations and the momentum exchange dp. */
 static void psm_cell(const orc_problem *p, const double *f, double d, const double m[3], double *post, double dp[3]) {
+    if (p->psm_law == 2) {
+        brinkman_moving_cell(p, f, d, m, post, dp);
+        return;
+    }
     const lattice_t *L = lat(p->Q);
     int Q = p->Q;
     double rho, j[3], den, sv[3], u[3], F[3];
@@ -1226,6 +1329,10 @@ static void psm_cell(const orc_problem *p, const double *f, double d, const doub
  * rho and u = j/rho + a/2 separately (the reference sums both chains first). */
 static void psm_cell_vjp(const orc_problem *p, const double *f, double d, const double m[3], const double *g,
                          const double dp_bar[3], double *f_bar, double *d_bar_out, double m_bar[3]) {
+    if (p->psm_law == 2) {
+        brinkman_moving_vjp(p, f, d, m, g, dp_bar, f_bar, d_bar_out, m_bar);
+        return;
+    }
     const lattice_t *L = lat(p->Q);
     int Q = p->Q;
     double rho, j[3], den, sv[3], u[3], F[3];

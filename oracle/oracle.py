@@ -108,9 +108,10 @@ class _Problem(ctypes.Structure):
         ("sponge_target", ctypes.c_void_p),
         ("psm_law", ctypes.c_int32), ("pad2_", ctypes.c_int32), ("psm_kappa", ctypes.c_double),
         ("occ_m", ctypes.c_void_p), ("occ_m_bar", ctypes.c_void_p),
+        ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
     ]
 
-PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1}
+PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1, "brinkman_occupancy": 2}
 
 
 def _ptr(a):
@@ -144,6 +145,14 @@ def psm_beta(d: float, t: float, kappa: float):
     out = np.zeros(3)
     lib().orc_psm_beta(ctypes.c_double(d), ctypes.c_double(t), ctypes.c_double(kappa), _ptr(out))
     return tuple(out)
+
+
+def psm_resistance(d: float, c_alpha: float, q: float, kappa: float):
+    """resistance of psm.rs:143-160: (alpha, gamma, [dalpha/dd, dalpha/dc, dgamma/dd, dgamma/dc])."""
+    out = np.zeros(6)
+    lib().orc_psm_resistance(ctypes.c_double(d), ctypes.c_double(c_alpha), ctypes.c_double(q),
+                             ctypes.c_double(kappa), _ptr(out))
+    return out[0], out[1], out[2:]
 
 
 def psm_saturate(d: float, kappa: float) -> float:
@@ -320,9 +329,12 @@ class Problem:
     # Sponge layers (moving/sponge.rs) blended after collision
     sponges: list = field(default_factory=list)
     # Occupancy coupling (moving/psm.rs:198-267): "brinkman" (the design is alpha), or
-    # "psm" / "psm_superposition" (the design is the occupancy d; solid momentum m per cell)
+    # "psm" / "psm_superposition" / "brinkman_occupancy" (the design is the occupancy d;
+    # solid momentum m per cell; the last with alpha = resistance(d; c_alpha, drag_shape))
     coupling: str = "brinkman"
     psm_kappa: float = 0.0
+    psm_c_alpha: float = 0.0
+    psm_drag_shape: float = 0.1
     solid_momentum: np.ndarray | None = None  # (cells, 3)
 
 
@@ -368,6 +380,7 @@ class Oracle:
         self._target = sponge_targets(prob.sponges, 0.0)
         p.sponge_target = _ptr(self._target)
         p.psm_law, p.psm_kappa = PSM_LAW[prob.coupling], float(prob.psm_kappa)
+        p.psm_c_alpha, p.psm_drag_shape = float(prob.psm_c_alpha), float(prob.psm_drag_shape)
         self.psm = prob.coupling != "brinkman"
         self._m = None
         self.m_bar = np.zeros((self.n, 3)) if self.psm else None

@@ -11,11 +11,12 @@ from oracle import oracle as orc
 def make_pair(shape, periodic=(False, False, False), Q=19, collision="trt", precision="f64", omega=1.25,
               magic=3.0 / 16.0, accel=(0.0, 0.0, 0.0), solid=None, ports=(), bulk_rate=float("nan"),
               even_rate=float("nan"), budget=0, les=(0.0, 0.0), wale=(0.0, 1e-12), sponges=(),
-              coupling=("brinkman", 0.0), solid_momentum=None):
+              coupling=("brinkman", 0.0), solid_momentum=None, drag=(0.0, 0.1)):
     """Returns (LatticeConfig, oracle.Problem) describing one problem; les = (Smagorinsky
     constant, norm floor), wale = (WALE constant, denominator floor); sponges are
     oracle.Sponge layers (copied into the library's Sponge); coupling = (law, kappa) with the
-    oracle's solid momentum (cells x 3; the library's is set with set_solid_momentum)."""
+    oracle's solid momentum (cells x 3; the library's is set with set_solid_momentum) and
+    drag = (c_alpha, drag shape) of the occupancy Brinkman law."""
     gp = [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
     op = [orc.Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
     gs = [Sponge(s.face, s.thickness, s.strength, s.density, tuple(s.velocity),
@@ -24,12 +25,12 @@ def make_pair(shape, periodic=(False, False, False), Q=19, collision="trt", prec
                         omega=omega, magic=magic, accel=accel, solid=solid, ports=gp, bulk_rate=bulk_rate,
                         even_rate=even_rate, checkpoint_budget_bytes=budget, les_constant=les[0],
                         les_norm_floor=les[1], wale_constant=wale[0], wale_floor=wale[1], sponges=gs,
-                        coupling=coupling[0], psm_kappa=coupling[1])
+                        coupling=coupling[0], psm_kappa=coupling[1], psm_c_alpha=drag[0], psm_drag_shape=drag[1])
     prob = orc.Problem(shape=shape, periodic=periodic, Q=Q, kind=collision, omega=omega, magic=magic,
                        bulk_rate=bulk_rate, even_rate=even_rate, accel=accel, solid=solid, ports=op,
                        les_constant=les[0], les_floor=les[1], wale_constant=wale[0], wale_floor=wale[1],
                        sponges=list(sponges), coupling=coupling[0], psm_kappa=coupling[1],
-                       solid_momentum=solid_momentum)
+                       solid_momentum=solid_momentum, psm_c_alpha=drag[0], psm_drag_shape=drag[1])
     return cfg, prob
 
 

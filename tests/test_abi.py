@@ -99,6 +99,8 @@ def _create(**kw):
     (dict(coupling="psm", psm_kappa=1.0), "kappa"),
     (dict(coupling="psm", precision="bf16s"), "FP64, FP32 and FP16"),
     (dict(coupling="psm", thermal=True, thermal_kappa=0.02), "temperature field"),
+    (dict(coupling="brinkman_occupancy", psm_c_alpha=-1.0), "c_alpha"),
+    (dict(coupling="brinkman_occupancy", psm_c_alpha=0.1, psm_drag_shape=0.0), "drag shape"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are

@@ -1,6 +1,7 @@
-"""GPU parity of the occupancy coupling (moving/psm.rs: the Psm and PsmSuperposition
-branches of cell_update :228-266 and cell_update_vjp :331-510, saturate / beta :96-152)
-on the stationary step, with the occupancy d as the design.
+"""GPU parity of the occupancy coupling (moving/psm.rs: the Psm, PsmSuperposition and
+moving-solid Brinkman branches of cell_update :213-266 and cell_update_vjp :301-510,
+saturate / beta / resistance :96-160) on the stationary step, with the occupancy d as
+the design.
 
 Bars: FP64 forward states bit-identical to the oracle (D3Q19 and D3Q27, BGK and TRT,
 solids, ports, partial, saturating and full occupancy, a moving solid); the FP64 step
@@ -44,6 +45,9 @@ CASES = [
     ((6, 5, 7), (True, True, True), 19, "bgk", "psm", False, False),
     ((6, 5, 4), (True, False, True), 27, "trt", "psm_superposition", True, False),
     ((6, 4, 5), (True, True, False), 27, "bgk", "psm", False, False),
+    ((7, 6, 5), (True, False, False), 19, "trt", "brinkman_occupancy", True, False),
+    ((9, 6, 6), (False, False, False), 19, "bgk", "brinkman_occupancy", False, True),
+    ((6, 5, 4), (True, False, True), 27, "trt", "brinkman_occupancy", False, False),
 ]
 
 
@@ -55,7 +59,7 @@ def _case(shape, periodic, Q, coll, law, with_solid, with_ports, precision="f64"
     m = _momentum(n, d, seed + 1)
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision=precision, omega=1.6,
                           accel=(1e-4, 0.0, 0.0) if not with_ports else (0.0, 0.0, 0.0), solid=solid, ports=ports,
-                          coupling=(law, KAPPA), solid_momentum=m)
+                          coupling=(law, KAPPA), solid_momentum=m, drag=(0.08, 0.1))
     return cfg, prob, solid, d, m
 
 
@@ -99,7 +103,7 @@ def test_psm_step_vjp_f64(shape, periodic, Q, coll, law, with_solid, with_ports)
     assert np.all(got_d[d == 0] == 0.0)
 
 
-@pytest.mark.parametrize("law", ["psm", "psm_superposition"])
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
 def test_psm_gradient_f64_matches_oracle(law):
     shape, steps = (10, 6, 5), 24
     cfg, prob, _, d, m = _case(shape, (False, False, False), 19, "trt", law, False, True, seed=11)
@@ -122,7 +126,7 @@ def test_psm_gradient_f64_matches_oracle(law):
 
 
 @pytest.mark.parametrize("precision,tol", [("f32", 2e-5), ("f16s", 3e-3)])
-@pytest.mark.parametrize("law", ["psm", "psm_superposition"])
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
 def test_psm_forward_reduced_precision(precision, tol, law):
     shape = (8, 6, 8)
     cfg, prob, _, d, m = _case(shape, (True, False, False), 19, "trt", law, False, False, precision)

@@ -12,9 +12,13 @@ from tests.lbm_cases import random_alpha
 C19, W19, _ = orc.lattice(19)
 
 
+C_ALPHA, DRAG_SHAPE = 0.08, 0.1
+
+
 def _oracle(law, kind="trt", kappa=0.1, shape=(4, 4, 4), accel=(2e-4, -1e-4, 0.0), **kw):
     return orc.Oracle(orc.Problem(shape=shape, periodic=(True, True, True), kind=kind, omega=1.3, accel=accel,
-                                  coupling=law, psm_kappa=kappa, **kw))
+                                  coupling=law, psm_kappa=kappa, psm_c_alpha=C_ALPHA, psm_drag_shape=DRAG_SHAPE,
+                                  **kw))
 
 
 def _state(seed, rho=1.01, amp=2e-3):
@@ -99,7 +103,62 @@ def test_vanishing_occupancy_is_the_fluid_collision(law):
         assert np.abs(post - fluid).max() <= 10 * d * np.abs(f).max()
 
 
-@pytest.mark.parametrize("law", ["psm", "psm_superposition"])
+def test_resistance_closed_form_and_partials():
+    """alpha = c e / ((q + 1) - e) with e the saturated occupancy, gamma = alpha / d
+    (c / ((q + 1) - d) below the band); the partials against central differences."""
+    kappa, c, q = 0.1, C_ALPHA, DRAG_SHAPE
+    for d in (0.05, 0.5, 0.93, 0.97, 1.05, 1.4):
+        a, g, part = orc.psm_resistance(d, c, q, kappa)
+        e = orc.psm_saturate(d, kappa)
+        assert abs(a - c * e / ((q + 1) - e)) <= 1e-16
+        assert abs(g - a / d) <= 1e-15 * max(1.0, g)
+        h = 1e-6
+        ap, gp, _ = orc.psm_resistance(d + h, c, q, kappa)
+        am, gm, _ = orc.psm_resistance(d - h, c, q, kappa)
+        assert abs((ap - am) / (2 * h) - part[0]) <= 1e-7 and abs((gp - gm) / (2 * h) - part[2]) <= 1e-7
+        ap, gp, _ = orc.psm_resistance(d, c + h, q, kappa)
+        am, gm, _ = orc.psm_resistance(d, c - h, q, kappa)
+        assert abs((ap - am) / (2 * h) - part[1]) <= 1e-7 and abs((gp - gm) / (2 * h) - part[3]) <= 1e-7
+
+
+@pytest.mark.parametrize("kind", ["trt", "bgk"])
+@pytest.mark.parametrize("d", [0.3, 0.95, 1.4])
+def test_brinkman_law_at_rest_is_the_stationary_operator(kind, d):
+    """With the solid at rest the Brinkman law is the stationary Brinkman collision with
+    alpha = resistance(d), bit for bit; a moving solid adds rho gamma m to the force, and
+    the exchange dp = rho z is the change of momentum beyond j + rho a."""
+    o = _oracle("brinkman_occupancy", kind)
+    a = np.array(o.prob.accel)
+    f = _state(5)
+    alpha, gamma, _ = orc.psm_resistance(d, C_ALPHA, DRAG_SHAPE, 0.1)
+    post, dp = o.psm_cell(f, d, np.zeros(3))
+    assert np.array_equal(post, o.cell_collide(f, alpha))
+    m = d * np.array([0.01, -0.02, 0.005])
+    post, dp = o.psm_cell(f, d, m)
+    rho, j = f.sum(), f @ C19
+    assert abs(post.sum() - rho) <= 1e-15
+    assert np.abs(post @ C19 - (j + rho * a + dp)).max() <= 1e-16
+    u = (j / rho + a / 2 + gamma * m / 2) / (1 + alpha / 2)
+    assert np.abs(dp - rho * (gamma * m - alpha * u)).max() <= 1e-16
+
+
+def test_brinkman_law_strong_drag_moves_with_the_solid():
+    """As the drag grows the cell's forced velocity u = (j + F/2)/rho tends to the solid's,
+    m / d.  For BGK at omega = 1 the post-collision momentum is rho u + F/2 = 2 rho u - j,
+    so u is read back as (post momentum + j) / (2 rho)."""
+    d = 0.6
+    us = np.array([0.02, -0.01, 0.004])
+    f = _state(6)
+    rho = f.sum()
+    o = orc.Oracle(orc.Problem(shape=(4, 4, 4), periodic=(True, True, True), kind="bgk", omega=1.0,
+                               accel=(0.0, 0.0, 0.0), coupling="brinkman_occupancy", psm_kappa=0.1,
+                               psm_c_alpha=1e9, psm_drag_shape=DRAG_SHAPE))
+    post, _ = o.psm_cell(f, d, d * us)
+    u = (post @ C19 + f @ C19) / (2 * rho)
+    assert np.abs(u - us).max() <= 1e-8
+
+
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
 @pytest.mark.parametrize("kind", ["trt", "bgk"])
 @pytest.mark.parametrize("d", [0.4, 0.95, 1.3])
 def test_cell_vjp_matches_central_differences(law, kind, d):
@@ -135,7 +194,7 @@ def _occupancy(shape, seed):
     return d
 
 
-@pytest.mark.parametrize("law", ["psm", "psm_superposition"])
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
 def test_step_transpose_and_gradient(law):
     """<lambda, J v> of the coupled step against <J^T lambda, v>, and the gradient of a
     pressure-drop stage cost with respect to the occupancy and the solid momentum
@@ -146,7 +205,8 @@ def test_step_transpose_and_gradient(law):
     rng = np.random.default_rng(4)
     m = rng.standard_normal((n, 3)) * 0.01
     prob = orc.Problem(shape=shape, periodic=(False, True, True), omega=1.3, accel=(1e-4, 0.0, 0.0),
-                       coupling=law, psm_kappa=0.1, solid_momentum=m)
+                       coupling=law, psm_kappa=0.1, solid_momentum=m, psm_c_alpha=C_ALPHA,
+                       psm_drag_shape=DRAG_SHAPE)
     o = orc.Oracle(prob)
     d = _occupancy(shape, 5)
     f = o.run(d, 4)
@@ -169,7 +229,8 @@ def test_step_transpose_and_gradient(law):
 
     def with_m(mm):
         p2 = orc.Problem(shape=shape, periodic=(False, True, True), omega=1.3, accel=(1e-4, 0.0, 0.0),
-                         coupling=law, psm_kappa=0.1, solid_momentum=mm)
+                         coupling=law, psm_kappa=0.1, solid_momentum=mm, psm_c_alpha=C_ALPHA,
+                         psm_drag_shape=DRAG_SHAPE)
         return orc.objective_value(orc.Oracle(p2), d, steps, ob)
 
     x = int(np.nonzero(d > 0)[0][0])
