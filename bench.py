@@ -225,6 +225,9 @@ def main():
                     choices=["brinkman", "psm", "psm_superposition", "brinkman_occupancy"],
                     help="kernels mode: fluid-solid coupling (PSM laws take the blob design's solid fraction as "
                          "occupancy)")
+    ap.add_argument("--design-map", type=float, default=0.0,
+                    help="kernels mode: also time the design map and its transpose with this filter radius "
+                         "(cells, beta 8)")
     ap.add_argument("--sponge", type=int, default=0,
                     help="kernels mode: thickness of sponge layers on the outlet and on one side wall (0: none); "
                          "also times the sponge pass and its transpose")
@@ -288,6 +291,23 @@ def main():
                 sp = {"sponge_thickness": T, "sponge_cells": cells, "sponge_ms": t_sp, "sponge_transpose_ms": t_spv,
                       "sponge_GBs": (2 * 19 * store * cells + meta) / (t_sp * 1e-3) / 1e9,
                       "sponge_transpose_GBs": (2 * 19 * lam * cells + cells * (4 + 8 + 4)) / (t_spv * 1e-3) / 1e9}
+        if args.design_map:
+            from implexity_amd.lbm import DesignMap
+
+            raw = np.random.default_rng(0).random(int(np.prod(shape)))
+            with DesignMap(shape, (False, False, False), args.design_map, 8.0, 0.5) as dm:
+                dm.map(raw)
+                dm.vjp(raw, raw)
+                t0 = time.perf_counter()
+                for _ in range(args.launches):
+                    phi = dm.map(raw)
+                t1 = time.perf_counter()
+                for _ in range(args.launches):
+                    dm.vjp(raw, phi)
+                t2 = time.perf_counter()
+            sp["design_map_radius"] = args.design_map
+            sp["design_map_ms_with_copies"] = (t1 - t0) * 1e3 / args.launches
+            sp["design_map_vjp_ms_with_copies"] = (t2 - t1) * 1e3 / args.launches
         from implexity_amd.lbm import copy_bandwidth
 
         copy_gbs, copy_ms = copy_bandwidth(1 << 30, args.launches)  # 1 GiB read + 1 GiB written

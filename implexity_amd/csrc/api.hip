@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/implexity_lbm.h"
+#include "design_map.h"
 #include "launch.h"
 #include "revolve.h"
 
@@ -2133,3 +2134,154 @@ extern "C" int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, 
         if (avg_ms) *avg_ms = avg;
     });
 }
+
+// ---------------------------------------------------------------------------------
+// Design map (design.rs:95-280) on the device.
+// ---------------------------------------------------------------------------------
+struct lbm_design_map {
+    int nx = 0, ny = 0, nz = 0;
+    int per[3] = {0, 0, 0};
+    long long n = 0;
+    int noff = 0;
+    double beta = 0.0, eta = 0.5, pa = 0.0, pden = 1.0;
+    DevBuf off, w, flags, fixed, raw, phi, bar, scale;
+    hipStream_t stream = nullptr;
+    ~lbm_design_map() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    DesignMapArgs args() const {
+        DesignMapArgs a{};
+        a.nx = nx;
+        a.ny = ny;
+        a.nz = nz;
+        a.px = per[0];
+        a.py = per[1];
+        a.pz = per[2];
+        a.noff = noff;
+        a.off = off.as<int>();
+        a.w = w.as<double>();
+        a.flags = flags.as<uint8_t>();
+        a.fixed = fixed.as<double>();
+        a.beta = beta;
+        a.eta = eta;
+        a.pa = pa;
+        a.pden = pden;
+        return a;
+    }
+    LaunchCfg cfg() const { return LaunchCfg{dim3(unsigned((n + 255) / 256)), dim3(256), stream}; }
+};
+
+extern "C" {
+
+int lbm_design_map_create(const lbm_design_map_config* cfg, const uint8_t* solid, const uint8_t* region,
+                          const double* fixed, uint64_t n, lbm_design_map** out) {
+    return guarded([&] {
+        if (!cfg || !out) fail(LBM_ERR_CONTRACT, "null config or output");
+        *out = nullptr;
+        if (cfg->struct_size != sizeof(lbm_design_map_config))
+            fail(LBM_ERR_CONTRACT, "lbm_design_map_config struct_size mismatch");
+        auto m = std::make_unique<lbm_design_map>();
+        for (int d = 0; d < 3; d++) {
+            if (cfg->shape[d] < 1 || cfg->shape[d] > 65535) fail(LBM_ERR_CONTRACT, "design map shape out of range");
+            m->per[d] = cfg->periodic[d] ? 1 : 0;
+        }
+        m->nx = cfg->shape[0];
+        m->ny = cfg->shape[1];
+        m->nz = cfg->shape[2];
+        m->n = (long long)m->nx * m->ny * m->nz;
+        if (n != uint64_t(m->n)) fail(LBM_ERR_SHAPE, "design map masks need one entry per cell");
+        // TopologyMap::normalize (design.rs:51-80) with the radius in cells
+        if (!(cfg->radius >= 0.0 && cfg->radius <= 4.0)) fail(LBM_ERR_CONTRACT, "filter radius must lie in [0,4] cells");
+        if (!(cfg->beta >= 0.0 && cfg->beta <= 32.0) || !(cfg->eta > 0.0 && cfg->eta < 1.0))
+            fail(LBM_ERR_CONTRACT, "Require beta in [0,32] and eta in (0,1)");
+        m->beta = cfg->beta;
+        m->eta = cfg->eta;
+        if (m->beta > 0.0) {
+            m->pa = std::tanh(m->beta * m->eta);
+            m->pden = m->pa + std::tanh(m->beta * (1.0 - m->eta));
+        }
+        // the cone's offsets and weights in the reference's loop order (design.rs:117-133)
+        std::vector<int> off;
+        std::vector<double> w;
+        if (cfg->radius > 0.0) {
+            const long long reach = (long long)std::ceil(cfg->radius);
+            for (long long a = -reach; a <= reach; a++)
+                for (long long b = -reach; b <= reach; b++)
+                    for (long long c = -reach; c <= reach; c++) {
+                        const double norm = std::sqrt(double(a * a + b * b + c * c));
+                        const double weight = std::max(cfg->radius - norm, 0.0);
+                        if (weight != 0.0) {
+                            off.push_back(int(a));
+                            off.push_back(int(b));
+                            off.push_back(int(c));
+                            w.push_back(weight);
+                        }
+                    }
+        }
+        m->noff = int(w.size());
+        std::vector<uint8_t> fl(size_t(m->n), 0);
+        for (long long x = 0; x < m->n; x++) {
+            const bool sol = solid && solid[x];
+            const bool reg = region ? region[x] != 0 : true;
+            fl[size_t(x)] = uint8_t((sol ? DM_SOLID : 0) | (reg && !sol ? DM_EDIT : 0));
+            if (fixed && !std::isfinite(fixed[x])) fail(LBM_ERR_CONTRACT, "fixed design values must be finite");
+        }
+        hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream create");
+        if (m->noff) {
+            upload(m->off, off.data(), off.size() * sizeof(int), "design map offsets");
+            upload(m->w, w.data(), w.size() * sizeof(double), "design map weights");
+        }
+        upload(m->flags, fl.data(), fl.size(), "design map flags");
+        m->fixed.alloc(size_t(m->n) * sizeof(double), "fixed design");
+        if (fixed)
+            hip_check(hipMemcpy(m->fixed.p, fixed, m->fixed.bytes, hipMemcpyHostToDevice), "upload");
+        else
+            hip_check(hipMemset(m->fixed.p, 0, m->fixed.bytes), "memset");
+        m->raw.alloc(size_t(m->n) * sizeof(double), "raw design");
+        m->phi.alloc(size_t(m->n) * sizeof(double), "mapped design");
+        *out = m.release();
+    });
+}
+
+int lbm_design_map_destroy(lbm_design_map* m) {
+    delete m;
+    return LBM_OK;
+}
+
+int lbm_design_map_apply(lbm_design_map* m, const double* raw, double* phi, uint64_t n) {
+    return guarded([&] {
+        if (!m || !raw || !phi) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(m->n)) fail(LBM_ERR_SHAPE, "design needs one value per cell");
+        const size_t bytes = size_t(m->n) * sizeof(double);
+        hip_check(hipMemcpyAsync(m->raw.p, raw, bytes, hipMemcpyHostToDevice, m->stream), "copy");
+        const LaunchCfg c = m->cfg();
+        hipLaunchKernelGGL(k_design_map, c.grid, c.block, 0, c.stream, m->args(), m->raw.as<double>(),
+                           m->phi.as<double>(), (long long)m->n);
+        hip_check(hipGetLastError(), "design map launch");
+        hip_check(hipMemcpyAsync(phi, m->phi.p, bytes, hipMemcpyDeviceToHost, m->stream), "copy");
+        hip_check(hipStreamSynchronize(m->stream), "sync");
+    });
+}
+
+int lbm_design_map_vjp(lbm_design_map* m, const double* raw, const double* phi_bar, double* raw_bar, uint64_t n) {
+    return guarded([&] {
+        if (!m || !raw || !phi_bar || !raw_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(m->n)) fail(LBM_ERR_SHAPE, "design needs one value per cell");
+        const size_t bytes = size_t(m->n) * sizeof(double);
+        m->bar.alloc(bytes, "design cotangent");
+        m->scale.alloc(bytes, "design map scale");
+        hip_check(hipMemcpyAsync(m->raw.p, raw, bytes, hipMemcpyHostToDevice, m->stream), "copy");
+        hip_check(hipMemcpyAsync(m->bar.p, phi_bar, bytes, hipMemcpyHostToDevice, m->stream), "copy");
+        const LaunchCfg c = m->cfg();
+        hipLaunchKernelGGL(k_design_scale, c.grid, c.block, 0, c.stream, m->args(), m->raw.as<double>(),
+                           m->bar.as<double>(), m->scale.as<double>(), (long long)m->n);
+        hip_check(hipGetLastError(), "design scale launch");
+        hipLaunchKernelGGL(k_design_gather, c.grid, c.block, 0, c.stream, m->args(), m->scale.as<double>(),
+                           m->bar.as<double>(), (long long)m->n);
+        hip_check(hipGetLastError(), "design gather launch");
+        hip_check(hipMemcpyAsync(raw_bar, m->bar.p, bytes, hipMemcpyDeviceToHost, m->stream), "copy");
+        hip_check(hipStreamSynchronize(m->stream), "sync");
+    });
+}
+
+}  // extern "C"

@@ -155,6 +155,12 @@ _SIGS = {
     "lbm_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_design": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_set_solid_momentum": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_design_map_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                               ctypes.c_void_p], ctypes.c_int),
+    "lbm_design_map_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "lbm_design_map_apply": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_design_map_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+                           ctypes.c_int),
     "lbm_solid_momentum_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_reset_initial": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
@@ -597,6 +603,67 @@ class Lattice:
                 "sponge_transpose": 6}[which]
         _check(lib().lbm_time_kernel(self._h, code, launches, ctypes.byref(ms)))
         return ms.value
+
+
+class _DesignMapConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("shape", ctypes.c_int32 * 3), ("periodic", ctypes.c_uint8 * 3),
+                ("reserved", ctypes.c_uint8 * 5), ("radius", ctypes.c_double), ("beta", ctypes.c_double),
+                ("eta", ctypes.c_double)]
+
+
+class DesignMap:
+    """Design map raw -> phi on the device (DesignMap, design.rs:95-280): cone filter of
+    `radius` cells over the editable cells (design region, not solid), then the tanh
+    projection (beta, eta); solid cells map to 0 and the rest of the lattice to `fixed`."""
+
+    def __init__(self, shape, periodic=(False, False, False), radius=0.0, beta=0.0, eta=0.5, solid=None,
+                 region=None, fixed=None):
+        self.shape = tuple(int(s) for s in shape)
+        self.n = int(np.prod(self.shape))
+        c = _DesignMapConfig()
+        c.struct_size = ctypes.sizeof(_DesignMapConfig)
+        for d in range(3):
+            c.shape[d] = self.shape[d]
+            c.periodic[d] = 1 if periodic[d] else 0
+        c.radius, c.beta, c.eta = float(radius), float(beta), float(eta)
+        sol = None if solid is None else np.ascontiguousarray(np.asarray(solid, bool).reshape(-1), np.uint8)
+        reg = None if region is None else np.ascontiguousarray(np.asarray(region, bool).reshape(-1), np.uint8)
+        fix = None if fixed is None else np.ascontiguousarray(fixed, np.float64).reshape(-1)
+        for arr in (sol, reg, fix):
+            if arr is not None and arr.size != self.n:
+                raise ShapeError("design map masks need one entry per cell")
+        h = ctypes.c_void_p()
+        _check(lib().lbm_design_map_create(ctypes.byref(c), _p(sol), _p(reg), _p(fix), self.n, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lbm_design_map_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def map(self, raw):
+        r = np.ascontiguousarray(raw, np.float64).reshape(-1)
+        out = np.empty(self.n)
+        _check(lib().lbm_design_map_apply(self._h, _p(r), _p(out), r.size))
+        return out
+
+    def vjp(self, raw, phi_bar):
+        r = np.ascontiguousarray(raw, np.float64).reshape(-1)
+        pb = np.ascontiguousarray(phi_bar, np.float64).reshape(-1)
+        if pb.size != r.size:
+            raise ShapeError("phi_bar needs one value per cell")
+        out = np.empty(self.n)
+        _check(lib().lbm_design_map_vjp(self._h, _p(r), _p(pb), _p(out), r.size))
+        return out
 
 
 class BinomialSchedule:

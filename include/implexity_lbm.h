@@ -375,6 +375,33 @@ int lbm_get_temperature(lbm_handle* h, double* T, uint64_t n);
 int lbm_thermal_design_bar(lbm_handle* h, double* phi_bar, uint64_t n);
 int lbm_get_thermal_lambda(lbm_handle* h, double* lambda, uint64_t len);
 
+/* Design map raw -> phi (DesignMap, design.rs:95-280; TopologyMap with the radius in
+ * cells, design.rs:37-80): phi = 0 on solid cells, the fixed value outside the design
+ * region, and on editable cells (design region, not solid) the cone filter
+ * sum_o w_o raw[x - o] / sum_o w_o over editable sources with w_o = radius - |o| > 0,
+ * followed by the projection (tanh(beta eta) + tanh(beta (f - eta))) /
+ * (tanh(beta eta) + tanh(beta (1 - eta))) when beta > 0.  The device computes the map
+ * (DesignMap::map) and its transpose (DesignMap::vjp) as gathers. */
+typedef struct lbm_design_map lbm_design_map;
+
+typedef struct {
+    uint32_t struct_size;   /* sizeof(lbm_design_map_config) */
+    int32_t shape[3];
+    uint8_t periodic[3];
+    uint8_t reserved[5];
+    double radius;          /* filter radius in cells (filter_radius_m / spacing_m), in [0, 4]; 0: none */
+    double beta;            /* projection sharpness in [0, 32]; 0: none */
+    double eta;             /* projection threshold in (0, 1) */
+} lbm_design_map_config;
+
+/* solid, region: one byte per cell or NULL (no solid cells / every cell editable);
+ * fixed: the design outside the region, or NULL (0). */
+int lbm_design_map_create(const lbm_design_map_config* cfg, const uint8_t* solid, const uint8_t* region,
+                          const double* fixed, uint64_t n, lbm_design_map** out);
+int lbm_design_map_destroy(lbm_design_map* m);
+int lbm_design_map_apply(lbm_design_map* m, const double* raw, double* phi, uint64_t n);
+int lbm_design_map_vjp(lbm_design_map* m, const double* raw, const double* phi_bar, double* raw_bar, uint64_t n);
+
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
  * a copy, 3 forward step of the flow and the coupled temperature field, 4 thermal

@@ -112,3 +112,20 @@ def test_contract_errors_before_device(kw, msg):
 def test_solid_mask_length_is_checked():
     with pytest.raises(lbm.ShapeError):
         _create(solid=np.zeros(5, bool))
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(radius=4.5), "radius"),
+    (dict(radius=-1.0), "radius"),
+    (dict(beta=40.0), "beta"),
+    (dict(eta=1.0), "eta"),
+])
+def test_design_map_contract_errors_before_device(kw, msg):
+    """TopologyMap::normalize (design.rs:51-80) conditions, refused before any device call."""
+    with pytest.raises(lbm.ContractError, match=msg):
+        lbm.DesignMap((4, 4, 4), **kw)
+
+
+def test_design_map_mask_length_is_checked():
+    with pytest.raises(lbm.ShapeError):
+        lbm.DesignMap((4, 4, 4), solid=np.zeros(5, bool))
