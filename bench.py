@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--design-map", type=float, default=0.0,
                     help="kernels mode: also time the design map and its transpose with this filter radius "
                          "(cells, beta 8)")
+    ap.add_argument("--pushforward", type=int, default=0,
+                    help="kernels mode: also time the push-forward of this many body points (cubic, width 1) "
+                         "and its transpose")
     ap.add_argument("--sponge", type=int, default=0,
                     help="kernels mode: thickness of sponge layers on the outlet and on one side wall (0: none); "
                          "also times the sponge pass and its transpose")
@@ -308,6 +311,26 @@ def main():
             sp["design_map_radius"] = args.design_map
             sp["design_map_ms_with_copies"] = (t1 - t0) * 1e3 / args.launches
             sp["design_map_vjp_ms_with_copies"] = (t2 - t1) * 1e3 / args.launches
+        if args.pushforward:
+            from implexity_amd.lbm import pushforward, pushforward_vjp
+
+            rng = np.random.default_rng(0)
+            npt = args.pushforward
+            xi = 3.0 + (np.array(shape, float) - 7.0) * rng.random((npt, 3))
+            pa, pv = rng.random(npt) * 1e-3, rng.standard_normal((npt, 3)) * 0.01
+            n = int(np.prod(shape))
+            db, mb = rng.standard_normal(n), rng.standard_normal((n, 3))
+            pushforward(shape, xi, pa, pv)
+            t0 = time.perf_counter()
+            for _ in range(args.launches):
+                pushforward(shape, xi, pa, pv)
+            t1 = time.perf_counter()
+            for _ in range(args.launches):
+                pushforward_vjp(shape, xi, pa, pv, db, mb)
+            t2 = time.perf_counter()
+            sp["pushforward_points"] = npt
+            sp["pushforward_ms_with_copies"] = (t1 - t0) * 1e3 / args.launches
+            sp["pushforward_vjp_ms_with_copies"] = (t2 - t1) * 1e3 / args.launches
         from implexity_amd.lbm import copy_bandwidth
 
         copy_gbs, copy_ms = copy_bandwidth(1 << 30, args.launches)  # 1 GiB read + 1 GiB written

@@ -14,6 +14,7 @@
 
 #include "../../include/implexity_lbm.h"
 #include "design_map.h"
+#include "pushforward.h"
 #include "launch.h"
 #include "revolve.h"
 
@@ -2281,6 +2282,115 @@ int lbm_design_map_vjp(lbm_design_map* m, const double* raw, const double* phi_b
         hip_check(hipGetLastError(), "design gather launch");
         hip_check(hipMemcpyAsync(raw_bar, m->bar.p, bytes, hipMemcpyDeviceToHost, m->stream), "copy");
         hip_check(hipStreamSynchronize(m->stream), "sync");
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Push-forward of body points (moving/pushforward.rs) on the device.
+// ---------------------------------------------------------------------------------
+namespace {
+
+PushArgs push_args(const lbm_pushforward_config* cfg, const double* xi, const double* a, const double* v,
+                   uint64_t np, uint64_t ncells) {
+    if (!cfg) fail(LBM_ERR_CONTRACT, "null config");
+    if (cfg->struct_size != sizeof(lbm_pushforward_config))
+        fail(LBM_ERR_CONTRACT, "lbm_pushforward_config struct_size mismatch");
+    PushArgs p{};
+    int n[3];
+    for (int d = 0; d < 3; d++) {
+        if (cfg->shape[d] < 1 || cfg->shape[d] > 65535) fail(LBM_ERR_CONTRACT, "push-forward shape out of range");
+        n[d] = cfg->shape[d];
+    }
+    p.nx = n[0];
+    p.ny = n[1];
+    p.nz = n[2];
+    p.px = cfg->periodic[0] ? 1 : 0;
+    p.py = cfg->periodic[1] ? 1 : 0;
+    p.pz = cfg->periodic[2] ? 1 : 0;
+    if (ncells != uint64_t(n[0]) * n[1] * n[2]) fail(LBM_ERR_SHAPE, "occupancy needs one entry per cell");
+    if (cfg->width_cells < 1 || cfg->width_cells > uint32_t(PF_MAX_H))
+        fail(LBM_ERR_CONTRACT, "kernel half-width must be 1 or 2 cells");
+    if (cfg->kernel > 1) fail(LBM_ERR_CONTRACT, "unknown interpolation kernel");
+    p.h = int(cfg->width_cells);
+    p.peskin = int(cfg->kernel);
+    p.np = (long long)np;
+    if (np && (!xi || !a)) fail(LBM_ERR_CONTRACT, "null point arrays");
+    const int per[3] = {p.px, p.py, p.pz};
+    const double hf = double(p.h);
+    for (uint64_t q = 0; q < np; q++) {
+        if (!std::isfinite(a[q])) fail(LBM_ERR_CONTRACT, "point weights must be finite");
+        for (int e = 0; e < 3; e++) {
+            const double x = xi[3 * q + e];
+            if (!std::isfinite(x)) fail(LBM_ERR_CONTRACT, "push-forward point " + std::to_string(q) + " is not finite");
+            if (v && !std::isfinite(v[3 * q + e])) fail(LBM_ERR_CONTRACT, "point velocities must be finite");
+            // Stencils::new_with_kernel (pushforward.rs:209-236) without symmetry faces
+            if (!per[e] && (x - 2.0 * hf <= -1.0 || x + 2.0 * hf >= double(n[e])))
+                fail(LBM_ERR_CONTRACT, "moving LBM admission refused (pushforward_support): point " + std::to_string(q) +
+                                           " has kernel support outside the lattice on axis " + std::to_string(e));
+        }
+    }
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_pushforward(const lbm_pushforward_config* cfg, const double* xi, const double* a, const double* v,
+                    uint64_t npoints, double* d, double* m, uint64_t ncells) {
+    return guarded([&] {
+        if (!d) fail(LBM_ERR_CONTRACT, "null occupancy output");
+        PushArgs p = push_args(cfg, xi, a, v, npoints, ncells);
+        DevBuf bxi, ba, bv, bd, bm;
+        bd.alloc(size_t(ncells) * sizeof(double), "occupancy");
+        hip_check(hipMemset(bd.p, 0, bd.bytes), "memset");
+        if (m) {
+            bm.alloc(size_t(ncells) * 3 * sizeof(double), "solid momentum");
+            hip_check(hipMemset(bm.p, 0, bm.bytes), "memset");
+        }
+        if (npoints) {
+            upload(bxi, xi, size_t(npoints) * 3 * sizeof(double), "points");
+            upload(ba, a, size_t(npoints) * sizeof(double), "point weights");
+            if (v && m) upload(bv, v, size_t(npoints) * 3 * sizeof(double), "point velocities");
+            p.xi = bxi.as<double>();
+            p.a = ba.as<double>();
+            p.v = bv.as<double>();
+            hipLaunchKernelGGL(k_pushforward, dim3(unsigned((npoints + 127) / 128)), dim3(128), 0, 0, p,
+                               bd.as<double>(), m ? bm.as<double>() : nullptr);
+            hip_check(hipGetLastError(), "push-forward launch");
+        }
+        hip_check(hipMemcpy(d, bd.p, bd.bytes, hipMemcpyDeviceToHost), "copy");
+        if (m) hip_check(hipMemcpy(m, bm.p, bm.bytes, hipMemcpyDeviceToHost), "copy");
+    });
+}
+
+int lbm_pushforward_vjp(const lbm_pushforward_config* cfg, const double* xi, const double* a, const double* v,
+                        uint64_t npoints, const double* d_bar, const double* m_bar, uint64_t ncells, double* xi_bar,
+                        double* a_bar, double* v_bar) {
+    return guarded([&] {
+        if (npoints && (!xi_bar || !a_bar)) fail(LBM_ERR_CONTRACT, "null cotangent output");
+        PushArgs p = push_args(cfg, xi, a, v, npoints, ncells);
+        if (npoints == 0) return;
+        DevBuf bxi, ba, bv, bdb, bmb, oxi, oa, ov;
+        upload(bxi, xi, size_t(npoints) * 3 * sizeof(double), "points");
+        upload(ba, a, size_t(npoints) * sizeof(double), "point weights");
+        if (v) upload(bv, v, size_t(npoints) * 3 * sizeof(double), "point velocities");
+        if (d_bar) upload(bdb, d_bar, size_t(ncells) * sizeof(double), "occupancy cotangent");
+        if (m_bar) upload(bmb, m_bar, size_t(ncells) * 3 * sizeof(double), "momentum cotangent");
+        oxi.alloc(size_t(npoints) * 3 * sizeof(double), "point cotangents");
+        oa.alloc(size_t(npoints) * sizeof(double), "weight cotangents");
+        if (v_bar) ov.alloc(size_t(npoints) * 3 * sizeof(double), "velocity cotangents");
+        p.xi = bxi.as<double>();
+        p.a = ba.as<double>();
+        p.v = bv.as<double>();
+        hipLaunchKernelGGL(k_pushforward_vjp, dim3(unsigned((npoints + 127) / 128)), dim3(128), 0, 0, p,
+                           bdb.as<double>(), bmb.as<double>(), oxi.as<double>(), oa.as<double>(), ov.as<double>());
+        hip_check(hipGetLastError(), "push-forward transpose launch");
+        hip_check(hipMemcpy(xi_bar, oxi.p, oxi.bytes, hipMemcpyDeviceToHost), "copy");
+        hip_check(hipMemcpy(a_bar, oa.p, oa.bytes, hipMemcpyDeviceToHost), "copy");
+        if (v_bar) hip_check(hipMemcpy(v_bar, ov.p, ov.bytes, hipMemcpyDeviceToHost), "copy");
     });
 }
 

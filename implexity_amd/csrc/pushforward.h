@@ -1,0 +1,174 @@
+// Push-forward of body points into the lattice occupancy (moving/pushforward.rs):
+//   d[c] = sum_q a_q W(c - xi_q),  m[c] = sum_q a_q v_q W(c - xi_q),
+// W the tensor product of per-axis weights of the cubic B-spline m4 or Peskin's 4-point
+// kernel of half-width h (support 4h cells from floor(x) - 2h + 1; the closed-form cubic
+// for h = 1), positions in lattice coordinates (cell c at coordinate c).  One thread per
+// point scatters its 64 h^3 weights with atomic adds (the reference sums per slab in
+// point order; the order here is the hardware's, so d and m agree to rounding).  The
+// transpose (endpoint_vjp, :946-967) is a per-point gather and needs no atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lbm {
+
+constexpr int PF_MAX_H = 2;
+
+struct PushArgs {
+    int nx, ny, nz;
+    int px, py, pz;
+    int h;        // kernel half-width in cells (1 or 2)
+    int peskin;   // 0 cubic B-spline, 1 Peskin 4-point
+    long long np; // points
+    const double* xi;  // 3 per point
+    const double* a;   // weight per point
+    const double* v;   // 3 per point, or null
+};
+
+__device__ __forceinline__ double pf_m4(double s) {
+    const double a = s < 0.0 ? -s : s;
+    if (a < 1.0) return (4.0 - a * a * 6.0 + a * a * a * 3.0) / 6.0;
+    if (a < 2.0) {
+        const double t = 2.0 - a;
+        return t * t * t / 6.0;
+    }
+    return 0.0;
+}
+__device__ __forceinline__ double pf_m4_derivative(double s) {
+    const bool neg = s < 0.0;
+    const double a = neg ? -s : s;
+    double d = 0.0;
+    if (a < 1.0) {
+        d = -a * 2.0 + a * a * 1.5;
+    } else if (a < 2.0) {
+        const double t = 2.0 - a;
+        d = -(t * t) * 0.5;
+    }
+    return neg ? -d : d;
+}
+__device__ __forceinline__ double pf_peskin4(double s) {
+    const double r = s < 0.0 ? -s : s;
+    if (r < 1.0) return (3.0 - r * 2.0 + sqrt(1.0 + r * 4.0 - r * r * 4.0)) / 8.0;
+    if (r < 2.0) return (5.0 - r * 2.0 - sqrt(-7.0 + r * 12.0 - r * r * 4.0)) / 8.0;
+    return 0.0;
+}
+__device__ __forceinline__ double pf_peskin4_derivative(double s) {
+    const bool neg = s < 0.0;
+    const double r = neg ? -s : s;
+    double d = 0.0;
+    if (r < 1.0)
+        d = (-2.0 + (2.0 - r * 4.0) / sqrt(1.0 + r * 4.0 - r * r * 4.0)) / 8.0;
+    else if (r < 2.0)
+        d = (-2.0 - (6.0 - r * 4.0) / sqrt(-7.0 + r * 12.0 - r * r * 4.0)) / 8.0;
+    return neg ? -d : d;
+}
+
+// Weights (and d/dx) of one coordinate over its 4h cells; returns the first cell.
+__device__ __forceinline__ int pf_axis(const PushArgs& A, double x, double* w, double* dw) {
+    const double bf = floor(x);
+    const int b = int(bf);
+    const int h = A.h;
+    const int first = b - 2 * h + 1;
+    if (h == 1 && !A.peskin) {
+        const double phi = x - bf;
+        const double r = 1.0 - phi;
+        const double p2 = phi * phi, r2 = r * r;
+        w[0] = r2 * r / 6.0;
+        w[1] = (p2 * (phi * 3.0 - 6.0) + 4.0) / 6.0;
+        w[2] = (((-phi * 3.0 + 3.0) * phi + 3.0) * phi + 1.0) / 6.0;
+        w[3] = p2 * phi / 6.0;
+        dw[0] = -r2 * 0.5;
+        dw[1] = phi * (phi * 1.5 - 2.0);
+        dw[2] = (-phi * 1.5 + 1.0) * phi + 0.5;
+        dw[3] = p2 * 0.5;
+        return first;
+    }
+    const double hf = double(h);
+    for (int t = 0; t < 4 * h; t++) {
+        const double r = (double(first + t) - x) / hf;
+        w[t] = (A.peskin ? pf_peskin4(r) : pf_m4(r)) / hf;
+        dw[t] = -(A.peskin ? pf_peskin4_derivative(r) : pf_m4_derivative(r)) / (hf * hf);
+    }
+    return first;
+}
+
+__device__ __forceinline__ int pf_wrap(int c, int n) {
+    c %= n;
+    return c < 0 ? c + n : c;
+}
+
+__global__ __launch_bounds__(128) void k_pushforward(PushArgs A, double* d, double* m) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.np) return;
+    double w[3][4 * PF_MAX_H], dw[3][4 * PF_MAX_H];
+    int first[3];
+    for (int e = 0; e < 3; e++) first[e] = pf_axis(A, A.xi[3 * q + e], w[e], dw[e]);
+    const int s = 4 * A.h;
+    const double aq = A.a[q];
+    double vq[3] = {0.0, 0.0, 0.0};
+    if (A.v)
+        for (int e = 0; e < 3; e++) vq[e] = A.v[3 * q + e];
+    for (int tx = 0; tx < s; tx++) {
+        const int cx = pf_wrap(first[0] + tx, A.nx);
+        const double wx = aq * w[0][tx];
+        for (int ty = 0; ty < s; ty++) {
+            const int cy = pf_wrap(first[1] + ty, A.ny);
+            const double wxy = wx * w[1][ty];
+            for (int tz = 0; tz < s; tz++) {
+                const long long cell = ((long long)cx * A.ny + cy) * A.nz + pf_wrap(first[2] + tz, A.nz);
+                const double ww = wxy * w[2][tz];
+                atomicAdd(&d[cell], ww);
+                if (m) {
+                    atomicAdd(&m[3 * cell], ww * vq[0]);
+                    atomicAdd(&m[3 * cell + 1], ww * vq[1]);
+                    atomicAdd(&m[3 * cell + 2], ww * vq[2]);
+                }
+            }
+        }
+    }
+}
+
+// xi_bar = a (sum grad W d_bar + sum grad W (m_bar . v)), a_bar = sum W (d_bar + m_bar . v),
+// v_bar = a sum W m_bar (endpoint_vjp).
+__global__ __launch_bounds__(128) void k_pushforward_vjp(PushArgs A, const double* d_bar, const double* m_bar,
+                                                         double* xi_bar, double* a_bar, double* v_bar) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.np) return;
+    double w[3][4 * PF_MAX_H], dw[3][4 * PF_MAX_H];
+    int first[3];
+    for (int e = 0; e < 3; e++) first[e] = pf_axis(A, A.xi[3 * q + e], w[e], dw[e]);
+    const int s = 4 * A.h;
+    double vq[3] = {0.0, 0.0, 0.0};
+    if (A.v)
+        for (int e = 0; e < 3; e++) vq[e] = A.v[3 * q + e];
+    double ks = 0.0, kv[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0}, gv[3][3] = {};
+    for (int tx = 0; tx < s; tx++) {
+        const int cx = pf_wrap(first[0] + tx, A.nx);
+        for (int ty = 0; ty < s; ty++) {
+            const int cy = pf_wrap(first[1] + ty, A.ny);
+            for (int tz = 0; tz < s; tz++) {
+                const long long cell = ((long long)cx * A.ny + cy) * A.nz + pf_wrap(first[2] + tz, A.nz);
+                const double W = w[0][tx] * w[1][ty] * w[2][tz];
+                const double dW[3] = {dw[0][tx] * w[1][ty] * w[2][tz], w[0][tx] * dw[1][ty] * w[2][tz],
+                                      w[0][tx] * w[1][ty] * dw[2][tz]};
+                const double db = d_bar ? d_bar[cell] : 0.0;
+                double mb[3] = {0.0, 0.0, 0.0};
+                if (m_bar)
+                    for (int e = 0; e < 3; e++) mb[e] = m_bar[3 * cell + e];
+                ks += W * db;
+                for (int e = 0; e < 3; e++) {
+                    kv[e] += W * mb[e];
+                    gs[e] += dW[e] * db;
+                    for (int f = 0; f < 3; f++) gv[e][f] += dW[e] * mb[f];
+                }
+            }
+        }
+    }
+    const double aq = A.a[q];
+    a_bar[q] = ks + kv[0] * vq[0] + kv[1] * vq[1] + kv[2] * vq[2];
+    for (int e = 0; e < 3; e++) {
+        xi_bar[3 * q + e] = aq * (gs[e] + gv[e][0] * vq[0] + gv[e][1] * vq[1] + gv[e][2] * vq[2]);
+        if (v_bar) v_bar[3 * q + e] = aq * kv[e];
+    }
+}
+
+}  // namespace lbm

@@ -158,6 +158,11 @@ _SIGS = {
     "lbm_design_map_create": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                ctypes.c_void_p], ctypes.c_int),
     "lbm_design_map_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "lbm_pushforward": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_pushforward_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p], ctypes.c_int),
     "lbm_design_map_apply": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_design_map_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                            ctypes.c_int),
@@ -664,6 +669,59 @@ class DesignMap:
         out = np.empty(self.n)
         _check(lib().lbm_design_map_vjp(self._h, _p(r), _p(pb), _p(out), r.size))
         return out
+
+
+class _PushConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("shape", ctypes.c_int32 * 3), ("periodic", ctypes.c_uint8 * 3),
+                ("reserved", ctypes.c_uint8 * 5), ("width_cells", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
+
+
+PUSH_KERNEL = {"cubic": 0, "peskin4": 1}
+
+
+def _push_config(shape, periodic, width, kernel):
+    if kernel not in PUSH_KERNEL:
+        raise ContractError("unknown interpolation kernel")
+    c = _PushConfig()
+    c.struct_size = ctypes.sizeof(_PushConfig)
+    for d in range(3):
+        c.shape[d] = int(shape[d])
+        c.periodic[d] = 1 if periodic[d] else 0
+    c.width_cells, c.kernel = int(width), PUSH_KERNEL[kernel]
+    return c
+
+
+def pushforward(shape, xi, a, v=None, periodic=(False, False, False), width=1, kernel="cubic"):
+    """Occupancy d (cells) and solid momentum m (cells x 3) of body points xi (lattice
+    coordinates) with weights a and velocities v (moving/pushforward.rs scatter)."""
+    n = int(np.prod(shape))
+    x = np.ascontiguousarray(xi, np.float64).reshape(-1)
+    aa = np.ascontiguousarray(a, np.float64).reshape(-1)
+    vv = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+    if x.size != 3 * aa.size or (vv is not None and vv.size != x.size):
+        raise ShapeError("points need three coordinates, one weight and three velocity components each")
+    d = np.empty(n)
+    m = np.empty(3 * n)
+    c = _push_config(shape, periodic, width, kernel)
+    _check(lib().lbm_pushforward(ctypes.byref(c), _p(x), _p(aa), _p(vv), aa.size, _p(d), _p(m), n))
+    return d, m.reshape(n, 3)
+
+
+def pushforward_vjp(shape, xi, a, v, d_bar, m_bar, periodic=(False, False, False), width=1, kernel="cubic"):
+    """(xi_bar, a_bar, v_bar) of <d_bar, d> + <m_bar, m> (endpoint_vjp)."""
+    n = int(np.prod(shape))
+    x = np.ascontiguousarray(xi, np.float64).reshape(-1)
+    aa = np.ascontiguousarray(a, np.float64).reshape(-1)
+    vv = np.ascontiguousarray(v, np.float64).reshape(-1)
+    db = np.ascontiguousarray(d_bar, np.float64).reshape(-1)
+    mb = np.ascontiguousarray(m_bar, np.float64).reshape(-1)
+    if x.size != 3 * aa.size or vv.size != x.size or db.size != n or mb.size != 3 * n:
+        raise ShapeError("push-forward cotangent shapes")
+    xb, ab, vb = np.empty(x.size), np.empty(aa.size), np.empty(x.size)
+    c = _push_config(shape, periodic, width, kernel)
+    _check(lib().lbm_pushforward_vjp(ctypes.byref(c), _p(x), _p(aa), _p(vv), aa.size, _p(db), _p(mb), n, _p(xb),
+                                     _p(ab), _p(vb)))
+    return xb.reshape(-1, 3), ab, vb.reshape(-1, 3)
 
 
 class BinomialSchedule:

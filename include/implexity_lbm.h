@@ -402,6 +402,30 @@ int lbm_design_map_destroy(lbm_design_map* m);
 int lbm_design_map_apply(lbm_design_map* m, const double* raw, double* phi, uint64_t n);
 int lbm_design_map_vjp(lbm_design_map* m, const double* raw, const double* phi_bar, double* raw_bar, uint64_t n);
 
+/* Push-forward of body points into the occupancy (moving/pushforward.rs, axes without
+ * symmetry faces): d[c] = sum_q a_q W(c - xi_q) and m[c] = sum_q a_q v_q W(c - xi_q), W
+ * the product of per-axis weights of the cubic B-spline (kernel 0) or Peskin's 4-point
+ * kernel (1) of half-width width_cells (support 4 width_cells cells per axis), positions
+ * in lattice coordinates (cell c at coordinate c; Frame::to_lattice maps metres to them).
+ * d and m feed lbm_set_design / lbm_set_solid_momentum of an occupancy-coupled handle;
+ * m is [cell][3].  The transpose gives the cotangents of the points, their weights and
+ * their velocities (endpoint_vjp).  A point whose support leaves a closed axis is
+ * refused (pushforward_support). */
+typedef struct {
+    uint32_t struct_size;   /* sizeof(lbm_pushforward_config) */
+    int32_t shape[3];
+    uint8_t periodic[3];
+    uint8_t reserved[5];
+    uint32_t width_cells;   /* 1 or 2 */
+    uint32_t kernel;        /* 0 cubic B-spline (m4), 1 Peskin 4-point */
+} lbm_pushforward_config;
+
+int lbm_pushforward(const lbm_pushforward_config* cfg, const double* xi, const double* a, const double* v,
+                    uint64_t npoints, double* d, double* m, uint64_t ncells);
+int lbm_pushforward_vjp(const lbm_pushforward_config* cfg, const double* xi, const double* a, const double* v,
+                        uint64_t npoints, const double* d_bar, const double* m_bar, uint64_t ncells, double* xi_bar,
+                        double* a_bar, double* v_bar);
+
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
  * a copy, 3 forward step of the flow and the coupled temperature field, 4 thermal

@@ -129,3 +129,15 @@ def test_design_map_contract_errors_before_device(kw, msg):
 def test_design_map_mask_length_is_checked():
     with pytest.raises(lbm.ShapeError):
         lbm.DesignMap((4, 4, 4), solid=np.zeros(5, bool))
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(width=3), "half-width"),
+    (dict(xi=[[0.5, 4.0, 4.0]]), "pushforward_support"),
+    (dict(xi=[[4.0, float("nan"), 4.0]]), "not finite"),
+])
+def test_pushforward_contract_errors_before_device(kw, msg):
+    args = dict(xi=[[4.0, 4.0, 4.0]], width=1)
+    args.update(kw)
+    with pytest.raises(lbm.ContractError, match=msg):
+        lbm.pushforward((8, 8, 8), args["xi"], [1.0], width=args["width"])
