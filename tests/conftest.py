@@ -14,11 +14,11 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    # Without a GPU driver at all, GPU tests cannot run; on a GPU box they run and fail
-    # loudly if the HIP path is broken.
+    # Without a GPU driver at all, GPU tests cannot run; on a machine with an MI355X they
+    # run and fail loudly if the HIP path is broken.
     if os.path.exists("/dev/kfd"):
         return
-    skip = pytest.mark.skip(reason="no GPU driver (/dev/kfd) in this container")
+    skip = pytest.mark.skip(reason="no GPU driver (/dev/kfd) on this machine")
     for item in items:
         if "gpu" in item.keywords:
             item.add_marker(skip)
