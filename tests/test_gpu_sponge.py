@@ -174,3 +174,60 @@ def test_sponge_with_temperature_field_f64_bit_exact():
         lat.interval(9)
         f, g = lat.state(), lat.thermal_state()
     assert np.array_equal(f, f_want) and np.array_equal(g, g_want)
+
+
+def test_sponge_with_temperature_field_gradient_f64():
+    """The coupled gradient (flow + temperature field) with a travelling-wave sponge on the
+    flow, recomputed under Every(4) checkpointing, against the oracle."""
+    from oracle.thermal import CoupledObjective, Thermal, ThermalOracle
+
+    shape, steps = (9, 6, 5), 13
+    (cfg, prob), _ = _case(shape, (False, False, False), 19, "trt", False, True, "faces", (0.0, 0.0, 0.0))
+    cfg.thermal, cfg.thermal_kappa, cfg.thermal_ratio, cfg.thermal_source, cfg.thermal_inlet = True, 0.02, 10.0, 1e-3, 0.3
+    cfg.checkpoint_every = 4
+    t = ThermalOracle(orc.Oracle(prob), Thermal(kappa=0.02, ratio=10.0, source=1e-3, inlet=0.3))
+    alpha = random_alpha(shape, 15)
+    phi = np.random.default_rng(16).random(t.n)
+    ob = Objective(kind="thermal", t_begin=6, t_end=steps, weight=1.0 / 8, terminal_weight=0.5, axis=0, plane_a=1,
+                   plane_b=shape[0] - 2, vector_weights=(1.0, 0.1, 0, 0, 0, 0))
+    oob = CoupledObjective(window=(6, steps), weight=1.0 / 8, terminal_weight=0.5, w_T=1.0, w_dp=0.1, axis=0,
+                           plane_a=1, plane_b=shape[0] - 2)
+    J_want, ab_want, pb_want, _, _ = t.scan_adjoint(oob, alpha, phi, steps)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_thermal_design(phi)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+        pb = lat.thermal_design_bar()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    for got, want in ((ab, ab_want), (pb, pb_want)):
+        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+
+
+def test_sponge_with_occupancy_coupling_gradient_f64():
+    """Sponge layers next to the Psm occupancy coupling: the occupancy and solid-momentum
+    gradient against the oracle."""
+    shape, steps = (10, 6, 5), 16
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(21)
+    d = random_alpha(shape, 22) * 10.0
+    d[rng.random(n) < 0.4] = 0.0
+    m = rng.standard_normal((n, 3)) * 0.005 * d[:, None]
+    cfg, prob = make_pair(shape, (False, False, False), precision="f64", omega=1.6, ports=channel_ports(shape, 0.03),
+                          sponges=_sponges("faces"), coupling=("psm", 0.1), solid_momentum=m)
+    ob = Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=8, t_end=steps,
+                   weight=1.0 / 9)
+    oob = orc.Objective(kind="pressure_drop", window=(8, steps), weight=1.0 / 9, axis=0, plane_a=1,
+                        plane_b=shape[0] - 2)
+    o = orc.Oracle(prob)
+    mbar = np.zeros((n, 3))
+    J_want, db_want, _, _ = orc.scan_adjoint(o, d, steps, oob, m_bar=mbar)
+    with Lattice(cfg) as lat:
+        lat.set_design(d)
+        lat.set_solid_momentum(m)
+        lat.initial_populations()
+        J, db, _, _ = lat.gradient(steps, ob)
+        mb = lat.solid_momentum_bar()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(db - db_want).max() <= 1e-11 * np.abs(db_want).max()
+    assert np.abs(mb - mbar).max() <= 1e-11 * np.abs(mbar).max()
