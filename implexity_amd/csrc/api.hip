@@ -441,9 +441,18 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
     ObjArgs r{};
     r.kind = OBJ_NONE;
     if (!o || o->kind == LBM_OBJ_NONE) return r;
-    if (is_psm(h->kind) && o->kind != LBM_OBJ_PRESSURE_DROP)
-        fail(LBM_ERR_CONTRACT, "with the occupancy coupling the objective is the pressure drop (the other "
-                               "objectives read a Brinkman coefficient)");
+    if (is_psm(h->kind) && o->kind != LBM_OBJ_PRESSURE_DROP && o->kind != LBM_OBJ_SOLID_FORCE)
+        fail(LBM_ERR_CONTRACT, "with the occupancy coupling the objective is the pressure drop or the solid force "
+                               "(the other objectives read a Brinkman coefficient)");
+    if (o->kind == LBM_OBJ_SOLID_FORCE) {
+        if (!is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "the solid-force objective needs the occupancy coupling");
+        if (o->terminal_weight != 0.0)
+            fail(LBM_ERR_CONTRACT, "the solid-force objective is a stage cost (terminal_weight must be 0)");
+        for (int k = 0; k < 3; k++) {
+            if (!std::isfinite(o->vector_weights[k])) fail(LBM_ERR_CONTRACT, "solid-force weights must be finite");
+            r.vw[k] = o->vector_weights[k];
+        }
+    }
     if (o->kind == LBM_OBJ_VECTOR) vector_weights(h, o->vector_weights, o->scales, r);
     r.axis = int(o->axis);
     const int len = o->axis == 0 ? h->nx : (o->axis == 1 ? h->ny : h->nz);
@@ -476,7 +485,10 @@ ObjArgs objective_template(const lbm_handle* h, const lbm_objective* o) {
 
 void validate_objective(const lbm_handle* h, const lbm_objective* o, uint64_t n) {
     if (!o || o->kind == LBM_OBJ_NONE) return;
-    if (o->kind > LBM_OBJ_THERMAL) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    if (o->kind > LBM_OBJ_SOLID_FORCE) fail(LBM_ERR_CONTRACT, "unknown objective kind");
+    // the exchange of a state is formed by the step that collides it: the final state has none
+    if (o->kind == LBM_OBJ_SOLID_FORCE && o->weight != 0.0 && o->t_end >= n)
+        fail(LBM_ERR_CONTRACT, "the solid-force window must end before the final state (t_end < nsteps)");
     if (o->kind == LBM_OBJ_THERMAL) {
         if (!h->thermal) fail(LBM_ERR_CONTRACT, "the thermal objective needs the coupled temperature field");
         if (!std::isfinite(o->vector_weights[0]) || !std::isfinite(o->vector_weights[1]))
@@ -506,6 +518,8 @@ void ensure_aos(lbm_handle* h) { h->aos.alloc(size_t(h->ncell) * h->Q * sizeof(d
 // alpha_max for rho >= 1/2 and |u| <= 1; the FP16 exponent keeps 4x headroom above it.
 double seed_bound_flow(const lbm_handle* h, const ObjArgs& o) {
     if (o.kind == OBJ_PRESSURE_DROP) return std::fabs(o.weight) * std::max(o.inv_na, o.inv_nb) / 3.0;
+    // |d (w . dp) / d f_q| <= 4 |w|_1 for rho >= 1/2 and |u| <= 1 (dp = sum_i Omega_s,i c_i)
+    if (o.kind == OBJ_SOLID_FORCE) return 4.0 * std::fabs(o.weight) * (std::fabs(o.vw[0]) + std::fabs(o.vw[1]) + std::fabs(o.vw[2]));
     if (o.kind == OBJ_DISSIPATION) return 16.0 * std::fabs(o.weight) * h->alpha_max;
     if (o.kind == OBJ_VECTOR) {
         // phi = sum_d A_d u_d + B rho |u|^2: |d phi/d f_q| <= 80 |B| + 8 sum |A_d| for rho >= 1/2, |u| <= 1

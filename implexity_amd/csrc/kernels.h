@@ -66,7 +66,9 @@ constexpr bool is_bro(int kind) { return kind == K_TRT_BRO || kind == K_BGK_BRO;
 constexpr int psm_base(int kind) {
     return kind == K_TRT_PSM || kind == K_TRT_PSMS || kind == K_TRT_BRO ? K_TRT : K_BGK;
 }
-enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3 };
+// OBJ_SOLID_FORCE: w . (momentum the coupled cells exchange with the solid in the
+// collision of the state read), w = vw[0..2] (occupancy couplings only).
+enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, OBJ_VECTOR = 3, OBJ_SOLID_FORCE = 5 };
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
 
@@ -1016,6 +1018,10 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                         collide_pairs_shifted<Q>(f, mac, rate, om, fs);
                     else
                         collide_pairs<Q>(f, mac.rho, mac.u, mac.F, rate, om, fs);
+                    if (A.obj.kind == OBJ_SOLID_FORCE)  // dp = rho z
+                        jsum += A.obj.weight * (A.obj.vw[0] * double(mac.rho * z[0]) +
+                                                A.obj.vw[1] * double(mac.rho * z[1]) +
+                                                A.obj.vw[2] * double(mac.rho * z[2]));
                 } else {
                     collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
                 }
@@ -1026,15 +1032,19 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                     const C* M = static_cast<const C*>(A.sm);
                     const long long n = (long long)g.nx * g.ny * g.nz;
                     const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
-                    psm_blend<Q, C, S::shifted, psm_sup(KIND)>(f, mac, alpha, ms, pk, C(A.omega), fs);
+                    C dp[3];
+                    psm_blend<Q, C, S::shifted, psm_sup(KIND)>(f, mac, alpha, ms, pk, C(A.omega), fs, dp);
+                    if (A.obj.kind == OBJ_SOLID_FORCE)
+                        jsum += A.obj.weight * (A.obj.vw[0] * double(dp[0]) + A.obj.vw[1] * double(dp[1]) +
+                                                A.obj.vw[2] * double(dp[2]));
                 }
             } else {
                 collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt),
                                                 fs, mac);
             }
-            if (A.obj.kind != OBJ_NONE) {
+            if (A.obj.kind != OBJ_NONE) {  // added to the solid-force share the coupling formed
                 const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
-                jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
+                jsum += objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
             }
             if (A.diag) {
                 msum = double(mac.rho);
@@ -1163,7 +1173,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     else
                         collide_pairs_vjp<Q>(mac.rho, mac.u, mac.F, C(A.omega), om, gb, fb, cb_rho, cb_u, cb_F);
                     C abr, gbr, mb[3];
-                    brinkman_moving_chain<Q, C>(mac, al, ga, a, ms, z, cb_rho, cb_u, cb_F, fb, abr, gbr, mb);
+                    const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    brinkman_moving_chain<Q, C>(mac, al, ga, a, ms, z, dpb, cb_rho, cb_u, cb_F, fb, abr, gbr, mb);
                     ab = abr * part[0] + gbr * part[2];
                     A.sm_bar[cell] += double(mb[0]);
                     A.sm_bar[n + cell] += double(mb[1]);
@@ -1189,7 +1200,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     }
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     C db, mb[3];
-                    psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, alpha, ms, pk, C(A.omega), gb, gfl, fb, db, mb);
+                    const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, alpha, ms, pk, C(A.omega), gb, gfl, dpb, fb, db, mb);
                     ab = db;
                     A.sm_bar[cell] += double(mb[0]);
                     A.sm_bar[n + cell] += double(mb[1]);

@@ -89,10 +89,11 @@ LBM_DEV C psm_eq(const Macro<Q, C>& m, C usq) {
 }
 
 // out holds the fluid collision of v on entry and the coupled post-collision values on
-// return (same representation as v); m is the alpha = 0 macroscopic state of v.  SUP
-// selects PsmSuperposition.  The equilibrium is formed per direction, not held.
+// return (same representation as v); m is the alpha = 0 macroscopic state of v; dp gets
+// the momentum exchanged with the solid, sum_i Omega_s,i c_i.  SUP selects
+// PsmSuperposition.  The equilibrium is formed per direction, not held.
 template <int Q, class C, bool SH, bool SUP>
-LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, C* out) {
+LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, C* out, C dp[3]) {
     const C t = C(1) / omega - C(0.5);
     C bod, bd, bt;
     psm_beta<C>(d, t, pk[1], bod, bd, bt);
@@ -100,6 +101,13 @@ LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], con
     const C m2 = ms[0] * ms[0] + ms[1] * ms[1] + ms[2] * ms[2];
     const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
     const C inv_d = SH ? C(1) / d : C(0);
+    dp[0] = dp[1] = dp[2] = C(0);
+    auto exch = [&](auto I, C os) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (cvec<Q>(i, 0) != 0) dp[0] += os * C(cvec<Q>(i, 0));
+        if constexpr (cvec<Q>(i, 1) != 0) dp[1] += os * C(cvec<Q>(i, 1));
+        if constexpr (cvec<Q>(i, 2) != 0) dp[2] += os * C(cvec<Q>(i, 2));
+    };
     if constexpr (!SH) {
         // the reference's equilibrium (lattice.rs), bit for bit in double precision
         C eq[Q];
@@ -109,12 +117,9 @@ LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], con
             constexpr int o = opp<Q>(i);
             const C cm = dotc<Q, i>(ms);
             const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d, inv_d);
-            if constexpr (SUP) {
-                out[i] = out[i] + (se - bb * eq[i]);
-            } else {
-                const C os = bb * (v[o] - eq[o] - v[i]) + se;
-                out[i] = v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
-            }
+            const C os = SUP ? se - bb * eq[i] : bb * (v[o] - eq[o] - v[i]) + se;
+            out[i] = SUP ? out[i] + os : v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
+            exch(I, os);
         });
     } else {
         static_for<0, Q>([&](auto I) {
@@ -122,12 +127,10 @@ LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], con
             constexpr int o = opp<Q>(i);
             const C cm = dotc<Q, i>(ms);
             const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d, inv_d);
-            if constexpr (SUP) {
-                out[i] = out[i] + (se - bb * psm_eq<Q, i, C, SH>(m, usq));
-            } else {
-                const C os = bb * (v[o] - psm_eq<Q, o, C, SH>(m, usq) - v[i]) + se;
-                out[i] = v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
-            }
+            const C os = SUP ? se - bb * psm_eq<Q, i, C, SH>(m, usq)
+                             : bb * (v[o] - psm_eq<Q, o, C, SH>(m, usq) - v[i]) + se;
+            out[i] = SUP ? out[i] + os : v[i] + (C(1) - bb) * (out[i] - v[i]) + os;
+            exch(I, os);
         });
     }
 }
@@ -147,14 +150,15 @@ LBM_DEV void psm_eq_vjp(const Macro<Q, C>& m, C usq, C e, C& rho_bar, C u_bar[3]
     usq_bar -= E * C(1.5);
 }
 
-// Transpose of psm_blend (psm.rs:331-415) given the cotangent g of post.  On entry fb
-// holds the transpose of the fluid collision applied to g; gfl = <g, fluid> (Psm only).
+// Transpose of psm_blend (psm.rs:331-415) given the cotangent g of post and dpb of the
+// exchange dp.  On entry fb holds the transpose of the fluid collision applied to g;
+// gfl = <g, fluid> (Psm only).
 // Adds the blend's own terms to fb (the fluid part scaled by 1 - B for Psm, the
 // bounce-back and equilibrium terms, and rho, u = j/rho + a/2 carried to the
 // populations) and returns the cotangents of d and m.
 template <int Q, class C, bool SH, bool SUP>
 LBM_DEV void psm_blend_vjp(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, const C* g,
-                           C gfl, C* fb, C& d_bar, C m_bar[3]) {
+                           C gfl, const C dpb[3], C* fb, C& d_bar, C m_bar[3]) {
     const C t = C(1) / omega - C(0.5);
     C bod, bd, bt;
     psm_beta<C>(d, t, pk[1], bod, bd, bt);
@@ -184,7 +188,10 @@ LBM_DEV void psm_blend_vjp(const C* v, const Macro<Q, C>& m, C d, const C ms[3],
         constexpr int i = decltype(I)::value;
         constexpr int o = opp<Q>(i);
         constexpr double w = weight<Q>(i);
-        const C h = g[i];
+        C h = g[i];  // cotangent of Omega_s,i: through post and through the exchange dp
+        if constexpr (cvec<Q>(i, 0) != 0) h += dpb[0] * C(cvec<Q>(i, 0));
+        if constexpr (cvec<Q>(i, 1) != 0) h += dpb[1] * C(cvec<Q>(i, 1));
+        if constexpr (cvec<Q>(i, 2) != 0) h += dpb[2] * C(cvec<Q>(i, 2));
         if constexpr (SUP) {
             b_bar += h * (rw * C(w) - psm_eq<Q, i, C, SH>(m, usq));
             psm_eq_vjp<Q, i, C>(m, usq, -h * bb, rho_bar, u_bar, usq_bar);
@@ -285,20 +292,21 @@ LBM_DEV void brinkman_moving_macro(const C* v, C alpha, C gamma, const C a[3], c
     }
 }
 
-// Cotangents of (f, alpha, gamma, m) from those of the collision's (rho, u, F); fb holds
-// the collision's direct cotangent on entry and the total on return.
+// Cotangents of (f, alpha, gamma, m) from those of the collision's (rho, u, F) and of the
+// exchange dp (dpb); fb holds the collision's direct cotangent on entry and the total on
+// return.
 template <int Q, class C>
 LBM_DEV void brinkman_moving_chain(const Macro<Q, C>& m, C alpha, C gamma, const C a[3], const C ms[3], const C z[3],
-                                   C cb_rho, const C cb_u[3], const C cb_F[3], C* fb, C& alpha_bar, C& gamma_bar,
-                                   C m_bar[3]) {
+                                   const C dpb[3], C cb_rho, const C cb_u[3], const C cb_F[3], C* fb, C& alpha_bar,
+                                   C& gamma_bar, C m_bar[3]) {
     C rho_bar = cb_rho;
     C u_bar[3] = {cb_u[0], cb_u[1], cb_u[2]};
     alpha_bar = C(0);
     gamma_bar = C(0);
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        rho_bar += cb_F[k] * (a[k] + z[k]);
-        const C zb = cb_F[k] * m.rho;
+    for (int k = 0; k < 3; k++) {  // force rho (a + z) and exchange dp = rho z
+        rho_bar += cb_F[k] * (a[k] + z[k]) + dpb[k] * z[k];
+        const C zb = (cb_F[k] + dpb[k]) * m.rho;
         gamma_bar += zb * ms[k];
         m_bar[k] = zb * gamma;
         alpha_bar -= zb * m.u[k];
@@ -322,4 +330,10 @@ LBM_DEV void brinkman_moving_chain(const Macro<Q, C>& m, C alpha, C gamma, const
         if constexpr (cvec<Q>(q, 2) != 0) x += jb[2] * C(cvec<Q>(q, 2));
         fb[q] = x;
     });
+}
+
+// Cotangent of the exchange dp from a solid-force stage cost (0 for other objectives).
+template <class C>
+LBM_DEV C psm_dp_bar(const ObjArgs& ob, int k) {
+    return ob.kind == OBJ_SOLID_FORCE ? C(ob.weight * ob.vw[k]) : C(0);
 }

@@ -33,7 +33,7 @@ ADJOINT_STORAGE = {"auto": 0, "f64": 1, "f32": 2, "f16": 3}
 COLLISION = {"bgk": 0, "trt": 1, "mrt": 2, "cumulant": 3, "bgk_guo": 4, "regularized": 5}
 FACE = {"xmin": 0, "xmax": 1, "ymin": 2, "ymax": 3, "zmin": 4, "zmax": 5}
 COUPLING = {"brinkman": 0, "psm": 1, "psm_superposition": 2, "brinkman_occupancy": 3}
-OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3, "thermal": 4}
+OBJECTIVE = {"none": 0, "dissipation": 1, "pressure_drop": 2, "vector": 3, "thermal": 4, "solid_force": 5}
 MAGIC_DEFAULT = 3.0 / 16.0  # moving/collision.rs:36
 
 

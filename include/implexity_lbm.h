@@ -231,10 +231,14 @@ typedef enum {
     LBM_OBJ_VECTOR = 3,        /* w . vector(state), the terminal responses of solver.rs:874-894
                                   (the design-volume term w[5] is not state-dependent and is
                                   left out of J; lbm_vector reports it) */
-    LBM_OBJ_THERMAL = 4        /* coupled temperature field: w[0] * (mean T over the cells with
+    LBM_OBJ_THERMAL = 4,       /* coupled temperature field: w[0] * (mean T over the cells with
                                   thermal design phi > 1/2) + w[1] * (pressure drop between
                                   plane_a and plane_b); w = vector_weights (no counterpart in
                                   the reference) */
+    LBM_OBJ_SOLID_FORCE = 5    /* occupancy couplings: w . (momentum the coupled cells exchange
+                                  with the solid in the collision of the state, CellOutput::dp,
+                                  psm.rs:240-262, summed over the cells), w = vector_weights[0..2];
+                                  a stage cost with t_end < nsteps and no terminal weight */
 } lbm_objective_kind;
 
 /* Physical scales of the terminal responses (LbmProblem::vector_scales, solver.rs:862-871). */

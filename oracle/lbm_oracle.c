@@ -1413,6 +1413,40 @@ void orc_psm_cell_vjp(const orc_problem *p, const double *f, double d, const dou
 static const double PSM_REST[3] = {0.0, 0.0, 0.0};
 static inline const double *psm_m(const orc_problem *p, long x) { return p->occ_m ? p->occ_m + 3 * x : PSM_REST; }
 
+/* Stage cost on the momentum the coupled cells exchange with the solid in the collision
+ * of `state` (CellOutput::dp, psm.rs:240-262): phi = sum over coupled cells of w . dp. */
+double orc_obj_solid_force(const orc_problem *p, const double *alpha, const double *state, const double w[3]) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double acc = 0.0, post[MAXQ], dp[3];
+    for (long x = 0; x < n; x++) {
+        if (is_solid(p, x) || p->psm_law < 0 || !(alpha[x] > 0.0)) continue;
+        psm_cell(p, state + x * Q, alpha[x], psm_m(p, x), post, dp);
+        acc += w[0] * dp[0] + w[1] * dp[1] + w[2] * dp[2];
+    }
+    return acc;
+}
+
+/* weight * d phi / d (state, d, m): f_bar (AoS), d_bar and m_bar (3 per cell), overwritten. */
+void orc_obj_solid_force_grad(const orc_problem *p, const double *alpha, const double *state, const double w[3],
+                              double weight, double *f_bar, double *d_bar, double *m_bar) {
+    long n = (long)p->nx * p->ny * p->nz;
+    int Q = p->Q;
+    double g[MAXQ], fb[MAXQ], db, mb[3];
+    const double dpb[3] = {weight * w[0], weight * w[1], weight * w[2]};
+    for (int q = 0; q < Q; q++) g[q] = 0.0;
+    for (long x = 0; x < n; x++) {
+        for (int q = 0; q < Q; q++) f_bar[x * Q + q] = 0.0;
+        d_bar[x] = 0.0;
+        for (int a = 0; a < 3; a++) m_bar[3 * x + a] = 0.0;
+        if (is_solid(p, x) || p->psm_law < 0 || !(alpha[x] > 0.0)) continue;
+        psm_cell_vjp(p, state + x * Q, alpha[x], psm_m(p, x), g, dpb, fb, &db, mb);
+        for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
+        d_bar[x] = db;
+        for (int a = 0; a < 3; a++) m_bar[3 * x + a] = mb[a];
+    }
+}
+
 void orc_port_populations(int Q, const double *nb, int kind, const double value[3], double *out) {
     const lattice_t *L = lat(Q);
     double rho_n, j[3], u_n[3], eb[MAXQ], en[MAXQ];

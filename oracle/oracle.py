@@ -82,6 +82,7 @@ def lib():
         _lib.orc_obj_dissipation.restype = d
         _lib.orc_obj_pressure_drop.restype = d
         _lib.orc_psm_saturate.restype = d
+        _lib.orc_obj_solid_force.restype = d
         _lib.orc_psm_saturate.argtypes = [d, d]
         del pd
     return _lib
@@ -600,6 +601,21 @@ class Oracle:
                              _ptr(pb))
         return fb, ab, pb
 
+    def solid_force(self, alpha, state, w):
+        """w . (momentum exchanged with the solid by the collision of state)."""
+        return lib().orc_obj_solid_force(self._pp, _ptr(np.ascontiguousarray(alpha, np.float64)),
+                                         _ptr(np.ascontiguousarray(state, np.float64)),
+                                         _ptr(np.ascontiguousarray(w, np.float64)))
+
+    def solid_force_grad(self, alpha, state, w, weight=1.0):
+        """(f_bar, d_bar, m_bar) of weight * solid_force."""
+        fb, db, mb = np.zeros(self.n * self.Q), np.zeros(self.n), np.zeros(3 * self.n)
+        lib().orc_obj_solid_force_grad(self._pp, _ptr(np.ascontiguousarray(alpha, np.float64)),
+                                       _ptr(np.ascontiguousarray(state, np.float64)),
+                                       _ptr(np.ascontiguousarray(w, np.float64)), ctypes.c_double(weight), _ptr(fb),
+                                       _ptr(db), _ptr(mb))
+        return fb, db, mb.reshape(self.n, 3)
+
     def pressure_drop_grad(self, axis, plane_a, plane_b, weight=1.0):
         fb = np.zeros(self.n * self.Q)
         lib().orc_obj_pressure_drop_grad(self._pp, ctypes.c_int(axis), ctypes.c_int(plane_a), ctypes.c_int(plane_b),
@@ -614,7 +630,7 @@ class Oracle:
 # ----------------------------------------------------------------------------------------
 @dataclass
 class Objective:
-    kind: str  # "dissipation" | "pressure_drop" | "vector"
+    kind: str  # "dissipation" | "pressure_drop" | "vector" | "solid_force" (w = vector_weights[:3])
     window: tuple = (1, 1)  # first and last state index (inclusive) carrying stage cost
     weight: float = 1.0
     terminal_weight: float = 0.0
@@ -629,6 +645,8 @@ class Objective:
         return self.plane_a % n, self.plane_b % n
 
     def value(self, orc: Oracle, alpha, state):
+        if self.kind == "solid_force":
+            return orc.solid_force(alpha, state, self.vector_weights[:3])
         if self.kind == "dissipation":
             return orc.dissipation(alpha, state)
         if self.kind == "vector":
@@ -637,6 +655,10 @@ class Objective:
         return orc.pressure_drop(state, self.axis, a, b)
 
     def grad(self, orc: Oracle, alpha, state, weight):
+        """(f_bar, alpha_bar); the solid force also leaves its m_bar in self.last_m_bar."""
+        if self.kind == "solid_force":
+            fb, db, self.last_m_bar = orc.solid_force_grad(alpha, state, self.vector_weights[:3], weight)
+            return fb, db
         if self.kind == "dissipation":
             return orc.dissipation_grad(alpha, state, weight)
         if self.kind == "vector":
@@ -688,6 +710,8 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
                 g, ga = objective.grad(orc, alpha, x_next, w)
                 lam = lam + g
                 abar += ga
+                if m_bar is not None and objective.kind == "solid_force":
+                    m_bar += objective.last_m_bar
             fb, ab = orc.interval_vjp(alpha, states[t - start], lam, t=t)
             abar += ab
             if m_bar is not None:

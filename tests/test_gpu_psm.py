@@ -152,3 +152,52 @@ def test_psm_gradient_f16_cotangents():
     with _lattice(cfg, d, m) as lat:
         _, db, _, _ = lat.gradient(steps, ob)
     assert np.linalg.norm(db - db_want) / np.linalg.norm(db_want) <= 1e-2
+
+
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
+def test_solid_force_gradient_f64_matches_oracle(law):
+    """The stage cost w . (momentum exchanged with the solid) over a window of states: J
+    and the occupancy and solid-momentum gradients against the oracle."""
+    shape, steps = (10, 6, 5), 20
+    cfg, prob, _, d, m = _case(shape, (False, False, False), 19, "trt", law, False, True, seed=17)
+    w = (1.0, -0.4, 0.25)
+    ob = Objective(kind="solid_force", t_begin=6, t_end=steps - 1, weight=0.5, vector_weights=w + (0, 0, 0))
+    oob = orc.Objective(kind="solid_force", window=(6, steps - 1), weight=0.5, vector_weights=w + (0, 0, 0))
+    o = orc.Oracle(prob)
+    mbar = np.zeros((o.n, 3))
+    J_want, db_want, _, _ = orc.scan_adjoint(o, d, steps, oob, m_bar=mbar)
+    with _lattice(cfg, d, m) as lat:
+        J, db, _, _ = lat.gradient(steps, ob)
+        mb = lat.solid_momentum_bar()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(db - db_want).max() <= 1e-11 * np.abs(db_want).max()
+    assert np.abs(mb - mbar).max() <= 1e-11 * np.abs(mbar).max()
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(t_end=20), "before the final state"),
+    (dict(terminal_weight=1.0), "stage cost"),
+])
+def test_solid_force_contract(kw, msg):
+    from implexity_amd.lbm import ContractError
+
+    shape = (10, 6, 5)
+    cfg, _, _, d, m = _case(shape, (False, False, False), 19, "trt", "psm", False, True, seed=17)
+    args = dict(kind="solid_force", t_begin=6, t_end=19, weight=0.5, vector_weights=(1.0, 0, 0, 0, 0, 0))
+    args.update(kw)
+    with _lattice(cfg, d, m) as lat:
+        with pytest.raises(ContractError, match=msg):
+            lat.gradient(20, Objective(**args))
+
+
+def test_solid_force_needs_the_occupancy_coupling():
+    from implexity_amd.lbm import ContractError
+
+    shape = (10, 6, 5)
+    cfg, _ = make_pair(shape, (False, False, False), precision="f64", ports=channel_ports(shape, 0.03))
+    with Lattice(cfg) as lat:
+        lat.set_design(random_alpha(shape, 3))
+        lat.initial_populations()
+        with pytest.raises(ContractError, match="occupancy coupling"):
+            lat.gradient(20, Objective(kind="solid_force", t_begin=6, t_end=19, weight=0.5,
+                                       vector_weights=(1.0, 0, 0, 0, 0, 0)))

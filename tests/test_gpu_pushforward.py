@@ -74,10 +74,12 @@ def _sphere(center, radius, spacing):
     return pts + np.asarray(center), np.full(len(pts), spacing ** 3)
 
 
-@pytest.mark.parametrize("law", ["psm", "brinkman_occupancy"])
-def test_point_gradient_through_the_coupled_flow(law):
-    """points -> (d, m) -> occupancy-coupled channel -> pressure drop -> (d_bar, m_bar) ->
-    point cotangents, on the device against the same chain over the oracle."""
+@pytest.mark.parametrize("law,kind", [("psm", "pressure_drop"), ("brinkman_occupancy", "pressure_drop"),
+                                      ("psm", "solid_force"), ("brinkman_occupancy", "solid_force")])
+def test_point_gradient_through_the_coupled_flow(law, kind):
+    """points -> (d, m) -> occupancy-coupled channel -> pressure drop or force on the body ->
+    (d_bar, m_bar) -> point cotangents, on the device against the same chain over the
+    oracle."""
     shape, steps = (14, 10, 10), 16
     xi, a = _sphere((6.6, 4.7, 5.2), 2.2, 0.7)
     v = np.tile([0.01, 0.0, -0.004], (len(a), 1))
@@ -85,8 +87,9 @@ def test_point_gradient_through_the_coupled_flow(law):
     d, m = lbm.pushforward(shape, xi, a, v, **pf)
     cfg, prob = make_pair(shape, (False, False, False), precision="f64", ports=channel_ports(shape, 0.03),
                           coupling=(law, 0.1), solid_momentum=m, drag=(0.08, 0.1))
-    ob = Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=8, t_end=steps,
-                   weight=1.0 / 9)
+    t_end = steps if kind == "pressure_drop" else steps - 1
+    ob = Objective(kind=kind, axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=8, t_end=t_end, weight=1.0 / 9,
+                   vector_weights=(1.0, 0.0, 0.0, 0, 0, 0))
     with Lattice(cfg) as lat:
         lat.set_design(d)
         lat.set_solid_momentum(m)
@@ -98,8 +101,8 @@ def test_point_gradient_through_the_coupled_flow(law):
     po = PushForwardOracle(shape, (False, False, False), 1, "cubic")
     d_o, m_o = po.scatter(xi, a, v)
     o = orc.Oracle(prob)
-    oob = orc.Objective(kind="pressure_drop", window=(8, steps), weight=1.0 / 9, axis=0, plane_a=1,
-                        plane_b=shape[0] - 2)
+    oob = orc.Objective(kind=kind, window=(8, t_end), weight=1.0 / 9, axis=0, plane_a=1, plane_b=shape[0] - 2,
+                        vector_weights=(1.0, 0.0, 0.0, 0, 0, 0))
     mbar = np.zeros((o.n, 3))
     _, db_o, _, _ = orc.scan_adjoint(o, d_o, steps, oob, m_bar=mbar)
     xw, aw, vw = po.vjp(xi, a, v, db_o, mbar)

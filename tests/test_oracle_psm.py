@@ -240,3 +240,40 @@ def test_step_transpose_and_gradient(law):
         mm[x, a] -= 1e-5
         fd = (with_m(mp) - with_m(mm)) / 2e-5
         assert abs(fd - mbar[x, a]) <= 1e-5 * max(abs(fd), 1e-8), (a, fd, mbar[x, a])
+
+
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
+def test_solid_force_objective_gradient(law):
+    """The stage cost w . (momentum exchanged with the solid) over a window of states:
+    its value is the sum of the cells' dp, and the design and solid-momentum gradients
+    through scan_adjoint match central differences."""
+    shape, steps = (8, 5, 4), 10
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(9)
+    m = rng.standard_normal((n, 3)) * 0.01
+    kw = dict(shape=shape, periodic=(False, True, True), omega=1.3, accel=(1e-4, 0.0, 0.0), coupling=law,
+              psm_kappa=0.1, psm_c_alpha=C_ALPHA, psm_drag_shape=DRAG_SHAPE)
+    o = orc.Oracle(orc.Problem(solid_momentum=m, **kw))
+    d = _occupancy(shape, 10)
+    ob = orc.Objective(kind="solid_force", window=(4, steps - 1), weight=0.5, vector_weights=(1.0, -0.3, 0.2, 0, 0, 0))
+    f = o.run(d, 3)
+    want = 0.0
+    for x in np.nonzero(d > 0)[0]:
+        want += np.array([1.0, -0.3, 0.2]) @ o.psm_cell(f.reshape(-1, 19)[x], d[x], m[x])[1]
+    assert abs(o.solid_force(d, f, (1.0, -0.3, 0.2)) - want) <= 1e-15 * max(1.0, abs(want)) * 50
+    mbar = np.zeros((n, 3))
+    _, db, _, _ = orc.scan_adjoint(o, d, steps, ob, m_bar=mbar)
+    # cells with d well above zero: the coupling has terms in m^2 / d
+    for idx in np.nonzero(d > 0.05)[0][:3]:
+        fd, rel = orc.fd_vs_ad_probe(lambda x: orc.objective_value(o, x, steps, ob), d, int(idx), None, db[idx],
+                                     step=1e-6)
+        assert rel <= 1e-5, (idx, fd, db[idx])
+    x = int(np.nonzero(d > 0.05)[0][1])
+    for a in range(3):
+        mp, mm = m.copy(), m.copy()
+        mp[x, a] += 1e-5
+        mm[x, a] -= 1e-5
+        fp = orc.objective_value(orc.Oracle(orc.Problem(solid_momentum=mp, **kw)), d, steps, ob)
+        fm = orc.objective_value(orc.Oracle(orc.Problem(solid_momentum=mm, **kw)), d, steps, ob)
+        fd = (fp - fm) / 2e-5
+        assert abs(fd - mbar[x, a]) <= 1e-5 * max(abs(fd), 1e-8), (a, fd, mbar[x, a])
