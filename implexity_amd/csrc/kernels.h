@@ -935,24 +935,25 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
             const C oe = e.oe;
             const C oo = KIND == K_TRT_LES ? odd_rate_of(oe, mrt[0]) : oe;
             const C dom = KIND == K_TRT_LES ? odd_rate_derivative_of(oe, mrt[0]) : C(1);
+            // the rate cotangents first: the non-equilibrium part is not needed after them
+            C ob, omb;
+            pairs_omega_bar<Q, C>(e.n, m, g, ob, omb);
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, oe, oo, g, fb, cb_rho, cb_u, cb_F);
             else
                 collide_pairs_vjp<Q>(m.rho, m.u, m.F, oe, oo, g, fb, cb_rho, cb_u, cb_F);
-            C ob, omb;
-            pairs_omega_bar<Q, C>(e.n, m, g, ob, omb);
             smagorinsky_vjp<Q, C>(e, m, mrt, ob + omb * dom, fb, cb_rho, cb_u);
         } else if constexpr (is_wale(KIND)) {
             const C oo = KIND == K_TRT_WALE ? odd_rate_of(omega, mrt[0]) : omega;
             const C dom = KIND == K_TRT_WALE ? odd_rate_derivative_of(omega, mrt[0]) : C(1);
-            if constexpr (std::is_same<C, float>::value)
-                collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
-            else
-                collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
             C n[Q], ob, omb;
             nonequilibrium<Q, C, SH>(v, m, n);
             pairs_omega_bar<Q, C>(n, m, g, ob, omb);
             *rate_bar = ob + omb * dom;
+            if constexpr (std::is_same<C, float>::value)
+                collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
+            else
+                collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
         } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,
