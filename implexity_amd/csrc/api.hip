@@ -234,6 +234,15 @@ struct lbm_handle {
     int psm_law = -1;  // 0 Psm, 1 PsmSuperposition, 2 Brinkman with a moving solid
     double psm_kappa = 0.0, psm_c_alpha = 0.0, psm_drag_shape = 0.0;
     DevBuf sm, sm_bar, alpha0;
+    // Second occupancy endpoint (d1, m1) blended over steps te0..te1, and its cotangents.
+    DevBuf d1, sm1, d1_bar, sm1_bar;
+    uint64_t te0 = 0, te1 = 1;
+    // Weight of the second endpoint in the step that reads state t.
+    double theta(uint64_t t) const {
+        if (!d1.p || t <= te0) return 0.0;
+        if (t >= te1) return 1.0;
+        return double(t - te0) / double(te1 - te0);
+    }
     const ThermalRunner* trun = nullptr;
 
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
@@ -259,6 +268,9 @@ struct lbm_handle {
         s.step = uint32_t(step);
         s.wu = wu.p;
         s.sm = sm.p;
+        s.d1 = d1.p;
+        s.sm1 = sm1.p;
+        s.theta = theta(step);
         if (thermal) {  // the flow kernel advances the temperature field in the same pass
             s.tpop = tpop(static_cast<const void*>(pop));
             s.tpop_out = tpop(pop);
@@ -369,6 +381,10 @@ struct lbm_handle {
         a.odd = odd;
         a.sm = sm.p;
         a.sm_bar = sm_bar.as<double>();
+        a.d1 = d1.p;
+        a.sm1 = sm1.p;
+        a.d1_bar = d1_bar.as<double>();
+        a.sm1_bar = sm1_bar.as<double>();
         a.obj.kind = OBJ_NONE;
         a.part = part.as<double>();
         a.npart = npart;
@@ -577,6 +593,10 @@ void ensure_adjoint_buffers(lbm_handle* h) {
     }
     if (is_wale(h->kind)) h->wgb.alloc(size_t(h->ncell) * 9 * h->csize(), "WALE gradient cotangents");
     if (is_psm(h->kind)) h->sm_bar.alloc(size_t(h->ncell) * 3 * sizeof(double), "solid momentum cotangent");
+    if (h->d1.p) {
+        h->d1_bar.alloc(size_t(h->ncell) * sizeof(double), "second occupancy cotangent");
+        h->sm1_bar.alloc(size_t(h->ncell) * 3 * sizeof(double), "second solid momentum cotangent");
+    }
 }
 
 // Snapshot slots for a horizon: min(budget / state bytes, nsteps); the pool only grows.
@@ -1317,6 +1337,56 @@ int lbm_solid_momentum_bar(lbm_handle* h, double* out, uint64_t len) {
     });
 }
 
+int lbm_set_occupancy_end(lbm_handle* h, const double* d1, const double* m1, uint64_t n, uint64_t t0,
+                          uint64_t t1) {
+    return guarded([&] {
+        if (!h || !d1) fail(LBM_ERR_CONTRACT, "null argument");
+        if (!is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "an occupancy endpoint needs the occupancy coupling");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "occupancy needs one value per cell");
+        if (t1 <= t0) fail(LBM_ERR_CONTRACT, "the blend needs t0 < t1");
+        const long long nc = h->ncell;
+        std::vector<double> soa(size_t(4 * nc), 0.0);
+        for (long long x = 0; x < nc; x++) {
+            if (!std::isfinite(d1[x]) || d1[x] < 0.0) fail(LBM_ERR_CONTRACT, "occupancy must be finite and non-negative");
+            soa[size_t(x)] = d1[x];
+            for (int a = 0; a < 3; a++) {
+                const double v = m1 ? m1[3 * x + a] : 0.0;
+                if (!std::isfinite(v)) fail(LBM_ERR_CONTRACT, "solid momentum must be finite");
+                soa[size_t((1 + a) * nc + x)] = v;
+            }
+        }
+        h->d1.alloc(size_t(nc) * h->csize(), "second occupancy");
+        h->sm1.alloc(size_t(nc) * 3 * h->csize(), "second solid momentum");
+        if (h->prec == LBM_F64) {
+            hip_check(hipMemcpy(h->d1.p, soa.data(), size_t(nc) * 8, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(h->sm1.p, soa.data() + nc, size_t(nc) * 24, hipMemcpyHostToDevice), "upload");
+        } else {
+            std::vector<float> f(soa.begin(), soa.end());
+            hip_check(hipMemcpy(h->d1.p, f.data(), size_t(nc) * 4, hipMemcpyHostToDevice), "upload");
+            hip_check(hipMemcpy(h->sm1.p, f.data() + nc, size_t(nc) * 12, hipMemcpyHostToDevice), "upload");
+        }
+        h->te0 = t0;
+        h->te1 = t1;
+    });
+}
+
+int lbm_occupancy_end_bar(lbm_handle* h, double* d1_bar, double* m1_bar, uint64_t n) {
+    return guarded([&] {
+        if (!h || !d1_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (!h->d1.p) fail(LBM_ERR_CONTRACT, "no second occupancy endpoint on this handle");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "occupancy needs one value per cell");
+        if (!h->d1_bar.p) fail(LBM_ERR_INVALID, "no adjoint sweep has run on this handle");
+        const long long nc = h->ncell;
+        std::vector<double> mb(size_t(3 * nc));
+        hip_check(hipMemcpyAsync(d1_bar, h->d1_bar.p, size_t(nc) * 8, hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipMemcpyAsync(mb.data(), h->sm1_bar.p, mb.size() * 8, hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        if (m1_bar)
+            for (long long x = 0; x < nc; x++)
+                for (int a = 0; a < 3; a++) m1_bar[3 * x + a] = mb[size_t(a * nc + x)];
+    });
+}
+
 int lbm_reset_initial(lbm_handle* h) {
     return guarded([&] {
         if (!h) fail(LBM_ERR_CONTRACT, "null handle");
@@ -1749,7 +1819,12 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
                       "port vjp");
         hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
         if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
+        if (h->d1_bar.p) {
+            hip_check(hipMemsetAsync(h->d1_bar.p, 0, h->d1_bar.bytes, h->stream), "memset");
+            hip_check(hipMemsetAsync(h->sm1_bar.p, 0, h->sm1_bar.bytes, h->stream), "memset");
+        }
         AdjArgs a = h->adj_args(h->F.p, int(t & 1));
+        a.theta = h->theta(t);
         flow_adjoint(h, a);
         hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(t & 1), h->aos.as<double>(),
                                   h->lsc_ptr(), h->cfg()),
@@ -1885,6 +1960,10 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                         // the seed kernel writes alpha_bar; the momentum cotangent starts at zero
                         if (h->sm_bar.p)
                             hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
+                        if (h->d1_bar.p) {
+                            hip_check(hipMemsetAsync(h->d1_bar.p, 0, h->d1_bar.bytes, h->stream), "memset");
+                            hip_check(hipMemsetAsync(h->sm1_bar.p, 0, h->sm1_bar.bytes, h->stream), "memset");
+                        }
                         hip_check(h->run->objective(Q, h->has_mask, ea, seed, h->L.p, h->alpha_bar.as<double>(), 1,
                                                     h->cfg()),
                                   "seed launch");
@@ -1921,6 +2000,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     hip_check(h->trun->port_vjp(h->port_args(base + k), h->tlam(), h->stream), "thermal port vjp");
                 aa.pop = cur;
                 aa.odd = int((base + k - 1) & 1);
+                aa.theta = h->theta(base + k - 1);
                 aa.obj = stage_obj(tmpl, obj, k - 1, nsteps, false);
                 if (h->thermal) {  // lambda_g and the velocity cotangent the flow kernel adds
                     ThermalArgs ta = h->thermal_args(cur, nullptr, base + k - 1);
@@ -2032,6 +2112,10 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
                 hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
                 hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
                 if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
+                if (h->d1_bar.p) {
+                    hip_check(hipMemsetAsync(h->d1_bar.p, 0, h->d1_bar.bytes, h->stream), "memset");
+                    hip_check(hipMemsetAsync(h->sm1_bar.p, 0, h->sm1_bar.bytes, h->stream), "memset");
+                }
                 hip_check(hipMemsetAsync(h->phi_bar.p, 0, h->phi_bar.bytes, h->stream), "memset");
             }
             auto launch = [&](uint64_t s) {
@@ -2095,6 +2179,10 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             hip_check(hipMemsetAsync(h->L.p, 0, h->L.bytes, h->stream), "memset");
             hip_check(hipMemsetAsync(h->alpha_bar.p, 0, h->alpha_bar.bytes, h->stream), "memset");
             if (h->sm_bar.p) hip_check(hipMemsetAsync(h->sm_bar.p, 0, h->sm_bar.bytes, h->stream), "memset");
+            if (h->d1_bar.p) {
+                hip_check(hipMemsetAsync(h->d1_bar.p, 0, h->d1_bar.bytes, h->stream), "memset");
+                hip_check(hipMemsetAsync(h->sm1_bar.p, 0, h->sm1_bar.bytes, h->stream), "memset");
+            }
             lam_init(h, 0.0, 0.0);
             AdjArgs a = h->adj_args(h->F.p, 0);
             a.odd = int(h->t & 1);

@@ -121,6 +121,9 @@ struct StepArgs {
     float tenc, tdec;
     const void* wu;   // WALE: u (3 x cells) and rho (cells) of the state read, arithmetic precision
     const void* sm;   // occupancy coupling: solid momentum m (3 x cells), arithmetic precision
+    const void* d1;   // occupancy coupling: second endpoint d1 (cells) and m1 (3 x cells), or null
+    const void* sm1;
+    double theta;     // weight of the second endpoint in this step
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
     long long npart;  // capacity of part, in blocks
@@ -163,6 +166,12 @@ struct AdjArgs {
     // (3 x cells), accumulated over the reverse sweep.
     const void* sm;
     double* sm_bar;
+    // Second occupancy endpoint (as StepArgs) and its FP64 cotangents.
+    const void* d1;
+    const void* sm1;
+    double theta;
+    double* d1_bar;
+    double* sm1_bar;
 };
 
 // 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
@@ -1007,12 +1016,11 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             }
             if constexpr (is_bro(KIND)) {  // the design is the occupancy d
                 const C* pk = static_cast<const C*>(A.mrt);
-                if (alpha > C(0)) {
-                    const C* M = static_cast<const C*>(A.sm);
-                    const long long n = (long long)g.nx * g.ny * g.nz;
-                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                C dc, ms[3];
+                psm_input<C>(alpha, A.sm, A.d1, A.sm1, A.theta, (long long)g.nx * g.ny * g.nz, cell, dc, ms);
+                if (dc > C(0)) {
                     C al, ga, part[4], z[3];
-                    psm_resistance<C>(alpha, pk[3], pk[4], pk[1], al, ga, part);
+                    psm_resistance<C>(dc, pk[3], pk[4], pk[1], al, ga, part);
                     brinkman_moving_macro<Q, C, S::shifted>(f, al, ga, a, ms, mac, z);
                     const C om = psm_base(KIND) == K_TRT ? C(A.om) : rate;
                     if constexpr (S::shifted)
@@ -1029,12 +1037,11 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             } else if constexpr (is_psm(KIND)) {  // the design is the occupancy d
                 const C* pk = static_cast<const C*>(A.mrt);
                 collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
-                if (alpha > C(0)) {
-                    const C* M = static_cast<const C*>(A.sm);
-                    const long long n = (long long)g.nx * g.ny * g.nz;
-                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                C dc, ms[3];
+                psm_input<C>(alpha, A.sm, A.d1, A.sm1, A.theta, (long long)g.nx * g.ny * g.nz, cell, dc, ms);
+                if (dc > C(0)) {
                     C dp[3];
-                    psm_blend<Q, C, S::shifted, psm_sup(KIND)>(f, mac, alpha, ms, pk, C(A.omega), fs, dp);
+                    psm_blend<Q, C, S::shifted, psm_sup(KIND)>(f, mac, dc, ms, pk, C(A.omega), fs, dp);
                     if (A.obj.kind == OBJ_SOLID_FORCE)
                         jsum += A.obj.weight * (A.obj.vw[0] * double(dp[0]) + A.obj.vw[1] * double(dp[1]) +
                                                 A.obj.vw[2] * double(dp[2]));
@@ -1161,12 +1168,12 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
             } else if constexpr (is_bro(KIND)) {  // the design is the occupancy d; ab := d_bar
                 constexpr int BK = psm_base(KIND);
                 const C* pk = static_cast<const C*>(A.mrt);
-                if (alpha > C(0)) {
-                    const long long n = (long long)g.nx * g.ny * g.nz;
-                    const C* M = static_cast<const C*>(A.sm);
-                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
+                const long long n = (long long)g.nx * g.ny * g.nz;
+                C dc, ms[3];
+                psm_input<C>(alpha, A.sm, A.d1, A.sm1, A.theta, n, cell, dc, ms);
+                if (dc > C(0)) {
                     C al, ga, part[4], z[3], cb_rho, cb_u[3], cb_F[3];
-                    psm_resistance<C>(alpha, pk[3], pk[4], pk[1], al, ga, part);
+                    psm_resistance<C>(dc, pk[3], pk[4], pk[1], al, ga, part);
                     brinkman_moving_macro<Q, C, S::shifted>(f, al, ga, a, ms, mac, z);
                     const C om = BK == K_TRT ? C(A.om) : C(A.omega);
                     if constexpr (std::is_same<C, float>::value)
@@ -1176,10 +1183,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     C abr, gbr, mb[3];
                     const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
                     brinkman_moving_chain<Q, C>(mac, al, ga, a, ms, z, dpb, cb_rho, cb_u, cb_F, fb, abr, gbr, mb);
-                    ab = abr * part[0] + gbr * part[2];
-                    A.sm_bar[cell] += double(mb[0]);
-                    A.sm_bar[n + cell] += double(mb[1]);
-                    A.sm_bar[2 * n + cell] += double(mb[2]);
+                    ab = psm_accumulate<C>(A, n, cell, abr * part[0] + gbr * part[2], mb);
                 } else {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     ab = C(0);
@@ -1187,11 +1191,11 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
             } else if constexpr (is_psm(KIND)) {  // the design is the occupancy d; ab := d_bar
                 constexpr int BK = psm_base(KIND);
                 const C* pk = static_cast<const C*>(A.mrt);
-                if (alpha > C(0)) {
+                const long long n = (long long)g.nx * g.ny * g.nz;
+                C dc, ms[3];
+                psm_input<C>(alpha, A.sm, A.d1, A.sm1, A.theta, n, cell, dc, ms);
+                if (dc > C(0)) {
                     constexpr bool SUP = psm_sup(KIND);
-                    const long long n = (long long)g.nx * g.ny * g.nz;
-                    const C* M = static_cast<const C*>(A.sm);
-                    const C ms[3] = {M[cell], M[n + cell], M[2 * n + cell]};
                     C gfl = C(0);  // <g, fluid collision> (Psm law)
                     if constexpr (!SUP) {
                         C fl[Q];
@@ -1202,11 +1206,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     C db, mb[3];
                     const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
-                    psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, alpha, ms, pk, C(A.omega), gb, gfl, dpb, fb, db, mb);
-                    ab = db;
-                    A.sm_bar[cell] += double(mb[0]);
-                    A.sm_bar[n + cell] += double(mb[1]);
-                    A.sm_bar[2 * n + cell] += double(mb[2]);
+                    psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, dc, ms, pk, C(A.omega), gb, gfl, dpb, fb, db, mb);
+                    ab = psm_accumulate<C>(A, n, cell, db, mb);
                 } else {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     ab = C(0);

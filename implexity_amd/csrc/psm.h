@@ -337,3 +337,44 @@ template <class C>
 LBM_DEV C psm_dp_bar(const ObjArgs& ob, int k) {
     return ob.kind == OBJ_SOLID_FORCE ? C(ob.weight * ob.vw[k]) : C(0);
 }
+
+// The occupancy d and solid momentum m a step collides with: the design value d0 and
+// m0, blended with the second endpoint when one is set as d0 (1 - theta) + d1 theta
+// (Field::solid_at, field.rs:1266-1275).
+template <class C>
+LBM_DEV void psm_input(C d0, const void* M0, const void* D1, const void* M1, double theta, long long n,
+                       long long cell, C& d, C ms[3]) {
+    const C* M = static_cast<const C*>(M0);
+    if (D1) {
+        const C th = C(theta), keep = C(1.0 - theta);
+        const C* Db = static_cast<const C*>(D1);
+        const C* Mb = static_cast<const C*>(M1);
+        d = d0 * keep + Db[cell] * th;
+#pragma unroll
+        for (int e = 0; e < 3; e++) ms[e] = M[e * n + cell] * keep + Mb[e * n + cell] * th;
+    } else {
+        d = d0;
+#pragma unroll
+        for (int e = 0; e < 3; e++) ms[e] = M[e * n + cell];
+    }
+}
+
+// Adds the solid-momentum cotangent mb to the endpoints' buffers and returns the share
+// of the occupancy cotangent db that goes where alpha_bar goes (the first endpoint's);
+// the second endpoint's shares are theta db and theta mb.
+template <class C, class Args>
+LBM_DEV C psm_accumulate(const Args& A, long long n, long long cell, C db, const C mb[3]) {
+    if (A.d1) {
+        const C th = C(A.theta), keep = C(1.0 - A.theta);
+        A.d1_bar[cell] += double(db * th);
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            A.sm_bar[e * n + cell] += double(mb[e] * keep);
+            A.sm1_bar[e * n + cell] += double(mb[e] * th);
+        }
+        return db * keep;
+    }
+#pragma unroll
+    for (int e = 0; e < 3; e++) A.sm_bar[e * n + cell] += double(mb[e]);
+    return db;
+}

@@ -167,6 +167,9 @@ _SIGS = {
     "lbm_design_map_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                            ctypes.c_int),
     "lbm_solid_momentum_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_set_occupancy_end": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                               ctypes.c_uint64], ctypes.c_int),
+    "lbm_occupancy_end_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_reset_initial": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_get_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
@@ -473,6 +476,18 @@ class Lattice:
         out = np.zeros(self.n * 3)
         _check(lib().lbm_solid_momentum_bar(self._h, _p(out), out.size))
         return out.reshape(self.n, 3)
+
+    def set_occupancy_end(self, d1, m1, t0, t1):
+        """Second occupancy endpoint blended in over the steps t0..t1."""
+        d = np.ascontiguousarray(d1, np.float64).reshape(-1)
+        m = None if m1 is None else np.ascontiguousarray(m1, np.float64).reshape(-1)
+        _check(lib().lbm_set_occupancy_end(self._h, _p(d), _p(m), d.size, int(t0), int(t1)))
+
+    def occupancy_end_bar(self):
+        d = np.zeros(self.n)
+        m = np.zeros(self.n * 3)
+        _check(lib().lbm_occupancy_end_bar(self._h, _p(d), _p(m), self.n))
+        return d, m.reshape(self.n, 3)
 
     # --- coupled temperature field ------------------------------------------------------
     def set_thermal_design(self, phi):

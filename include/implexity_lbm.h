@@ -300,6 +300,15 @@ int lbm_reset_initial(lbm_handle* h);
 int lbm_set_solid_momentum(lbm_handle* h, const double* m, uint64_t len);
 int lbm_solid_momentum_bar(lbm_handle* h, double* out, uint64_t len);
 
+/* Second occupancy endpoint (Occupancy::{d, m}[1], a body moving within an interval):
+ * the step that reads state t collides with d = d0 (1 - theta) + d1 theta and m likewise
+ * (Field::solid_at, field.rs:1266-1275), theta = 0 up to t0, (t - t0) / (t1 - t0) between,
+ * 1 from t1; d0, m0 are the design and lbm_set_solid_momentum.  m1 is [cell][3] or NULL
+ * (0).  The gradient's alpha_bar and lbm_solid_momentum_bar then hold the first
+ * endpoint's cotangents and lbm_occupancy_end_bar the second's. */
+int lbm_set_occupancy_end(lbm_handle* h, const double* d1, const double* m1, uint64_t n, uint64_t t0, uint64_t t1);
+int lbm_occupancy_end_bar(lbm_handle* h, double* d1_bar, double* m1_bar, uint64_t n);
+
 /* State exchange in the reference layout: layout 0 = AoS (x*Q+q), 1 = SoA (q*N+x). */
 int lbm_set_state(lbm_handle* h, int layout, const double* f, uint64_t len);
 int lbm_get_state(lbm_handle* h, int layout, double* f, uint64_t len);

@@ -172,6 +172,14 @@ typedef struct {
     double *occ_m_bar;
     double psm_c_alpha;            /* Brinkman law: c_alpha (CellParameters) */
     double psm_drag_shape;         /* Brinkman law: drag_shape q */
+    /* Second occupancy endpoint (Occupancy::{d, m}[1]): with occ_d1 set, a step collides
+     * with d = d0 (1 - theta) + d1 theta and m likewise (field.rs:1266-1275); the
+     * transpose splits the cotangents between the endpoints (occ_d1_bar, occ_m1_bar). */
+    const double *occ_d1;
+    const double *occ_m1;
+    double theta;
+    double *occ_d1_bar;
+    double *occ_m1_bar;
 } orc_problem;
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
@@ -1410,8 +1418,16 @@ void orc_psm_cell_vjp(const orc_problem *p, const double *f, double d, const dou
     psm_cell_vjp(p, f, d, m, g, dp_bar, f_bar, d_bar, m_bar);
 }
 
-static const double PSM_REST[3] = {0.0, 0.0, 0.0};
-static inline const double *psm_m(const orc_problem *p, long x) { return p->occ_m ? p->occ_m + 3 * x : PSM_REST; }
+/* The occupancy and solid momentum a step collides with (Field::solid_at, field.rs:1266-1275). */
+static inline double psm_d(const orc_problem *p, const double *alpha, long x) {
+    return p->occ_d1 ? alpha[x] * (1.0 - p->theta) + p->occ_d1[x] * p->theta : alpha[x];
+}
+static inline void psm_mv(const orc_problem *p, long x, double m[3]) {
+    for (int a = 0; a < 3; a++) {
+        double m0 = p->occ_m ? p->occ_m[3 * x + a] : 0.0;
+        m[a] = p->occ_d1 ? m0 * (1.0 - p->theta) + (p->occ_m1 ? p->occ_m1[3 * x + a] : 0.0) * p->theta : m0;
+    }
+}
 
 /* Stage cost on the momentum the coupled cells exchange with the solid in the collision
  * of `state` (CellOutput::dp, psm.rs:240-262): phi = sum over coupled cells of w . dp. */
@@ -1420,8 +1436,10 @@ double orc_obj_solid_force(const orc_problem *p, const double *alpha, const doub
     int Q = p->Q;
     double acc = 0.0, post[MAXQ], dp[3];
     for (long x = 0; x < n; x++) {
-        if (is_solid(p, x) || p->psm_law < 0 || !(alpha[x] > 0.0)) continue;
-        psm_cell(p, state + x * Q, alpha[x], psm_m(p, x), post, dp);
+        double d = psm_d(p, alpha, x), m[3];
+        if (is_solid(p, x) || p->psm_law < 0 || !(d > 0.0)) continue;
+        psm_mv(p, x, m);
+        psm_cell(p, state + x * Q, d, m, post, dp);
         acc += w[0] * dp[0] + w[1] * dp[1] + w[2] * dp[2];
     }
     return acc;
@@ -1439,8 +1457,10 @@ void orc_obj_solid_force_grad(const orc_problem *p, const double *alpha, const d
         for (int q = 0; q < Q; q++) f_bar[x * Q + q] = 0.0;
         d_bar[x] = 0.0;
         for (int a = 0; a < 3; a++) m_bar[3 * x + a] = 0.0;
-        if (is_solid(p, x) || p->psm_law < 0 || !(alpha[x] > 0.0)) continue;
-        psm_cell_vjp(p, state + x * Q, alpha[x], psm_m(p, x), g, dpb, fb, &db, mb);
+        double d = psm_d(p, alpha, x), m[3];
+        if (is_solid(p, x) || p->psm_law < 0 || !(d > 0.0)) continue;
+        psm_mv(p, x, m);
+        psm_cell_vjp(p, state + x * Q, d, m, g, dpb, fb, &db, mb);
         for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
         d_bar[x] = db;
         for (int a = 0; a < 3; a++) m_bar[3 * x + a] = mb[a];
@@ -1725,8 +1745,10 @@ void orc_interval(const orc_problem *p, const uint32_t *source, const double *al
         }
         if (p->psm_law >= 0) {
             double dp[3];
-            if (alpha[x] > 0.0)
-                psm_cell(p, before + x * Q, alpha[x], psm_m(p, x), post + x * Q, dp);
+            double d = psm_d(p, alpha, x), m[3];
+            psm_mv(p, x, m);
+            if (d > 0.0)
+                psm_cell(p, before + x * Q, d, m, post + x * Q, dp);
             else
                 cell_collide_rate(p, before + x * Q, 0.0, -1.0, post + x * Q);
             continue;
@@ -1795,6 +1817,8 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
     }
     const double tau0 = 1.0 / p->omega;
     if (p->psm_law >= 0 && p->occ_m_bar) memset(p->occ_m_bar, 0, sizeof(double) * 3 * n);
+    if (p->psm_law >= 0 && p->occ_d1 && p->occ_d1_bar) memset(p->occ_d1_bar, 0, sizeof(double) * n);
+    if (p->psm_law >= 0 && p->occ_d1 && p->occ_m1_bar) memset(p->occ_m1_bar, 0, sizeof(double) * 3 * n);
     double g[MAXQ], fb[MAXQ];
 #pragma omp parallel for schedule(static) private(g, fb)
     for (long x = 0; x < n; x++) {
@@ -1816,13 +1840,24 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
         if (p->psm_law >= 0) {
             static const double no_dp[3] = {0.0, 0.0, 0.0};
             double mb[3] = {0.0, 0.0, 0.0};
-            if (alpha[x] > 0.0) {
-                psm_cell_vjp(p, before + x * Q, alpha[x], psm_m(p, x), g, no_dp, fb, &ab, mb);
+            double d = psm_d(p, alpha, x), m[3];
+            psm_mv(p, x, m);
+            if (d > 0.0) {
+                psm_cell_vjp(p, before + x * Q, d, m, g, no_dp, fb, &ab, mb);
             } else {
                 cell_vjp_rate(p, before + x * Q, 0.0, -1.0, g, fb, &ab, NULL);
                 ab = 0.0;
             }
             for (int q = 0; q < Q; q++) f_bar[x * Q + q] = fb[q];
+            if (p->occ_d1) {  /* d = d0 (1 - theta) + d1 theta */
+                alpha_bar[x] = ab * (1.0 - p->theta);
+                if (p->occ_d1_bar) p->occ_d1_bar[x] = ab * p->theta;
+                for (int a = 0; a < 3; a++) {
+                    if (p->occ_m_bar) p->occ_m_bar[3 * x + a] = mb[a] * (1.0 - p->theta);
+                    if (p->occ_m1_bar) p->occ_m1_bar[3 * x + a] = mb[a] * p->theta;
+                }
+                continue;
+            }
             alpha_bar[x] = ab;
             if (p->occ_m_bar)
                 for (int a = 0; a < 3; a++) p->occ_m_bar[3 * x + a] = mb[a];

@@ -110,6 +110,8 @@ class _Problem(ctypes.Structure):
         ("psm_law", ctypes.c_int32), ("pad2_", ctypes.c_int32), ("psm_kappa", ctypes.c_double),
         ("occ_m", ctypes.c_void_p), ("occ_m_bar", ctypes.c_void_p),
         ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
+        ("occ_d1", ctypes.c_void_p), ("occ_m1", ctypes.c_void_p), ("theta", ctypes.c_double),
+        ("occ_d1_bar", ctypes.c_void_p), ("occ_m1_bar", ctypes.c_void_p),
     ]
 
 PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1, "brinkman_occupancy": 2}
@@ -336,6 +338,9 @@ class Problem:
     psm_kappa: float = 0.0
     psm_c_alpha: float = 0.0
     psm_drag_shape: float = 0.1
+    # second occupancy endpoint (d1, m1 (cells x 3), t0, t1): the step reading state t
+    # collides with d = d0 (1 - theta) + d1 theta, theta = clamp((t - t0) / (t1 - t0), 0, 1)
+    occupancy_end: tuple | None = None
     solid_momentum: np.ndarray | None = None  # (cells, 3)
 
 
@@ -389,6 +394,15 @@ class Oracle:
             self._m = (np.zeros((self.n, 3)) if prob.solid_momentum is None
                        else np.ascontiguousarray(np.asarray(prob.solid_momentum, np.float64).reshape(self.n, 3)))
             p.occ_m, p.occ_m_bar = _ptr(self._m), _ptr(self.m_bar)
+        self.d1_bar = self.m1_bar = None
+        if self.psm and prob.occupancy_end is not None:
+            d1, m1, _, _ = prob.occupancy_end
+            self._d1 = np.ascontiguousarray(d1, np.float64).reshape(-1)
+            self._m1 = (np.zeros((self.n, 3)) if m1 is None
+                        else np.ascontiguousarray(np.asarray(m1, np.float64).reshape(self.n, 3)))
+            self.d1_bar, self.m1_bar = np.zeros(self.n), np.zeros((self.n, 3))
+            p.occ_d1, p.occ_m1 = _ptr(self._d1), _ptr(self._m1)
+            p.occ_d1_bar, p.occ_m1_bar = _ptr(self.d1_bar), _ptr(self.m1_bar)
         self._p = p
         self._pp = ctypes.byref(p)
         self.source = np.zeros(self.n * self.Q, np.uint32)
@@ -511,9 +525,17 @@ class Oracle:
         lib().orc_initial(self._pp, _ptr(f))
         return f
 
+    def theta(self, t):
+        """Interpolation weight of the second occupancy endpoint for the step reading state t."""
+        if self.prob.occupancy_end is None:
+            return 0.0
+        _, _, t0, t1 = self.prob.occupancy_end
+        return 0.0 if t <= t0 else (1.0 if t >= t1 else (t - t0) / (t1 - t0))
+
     def set_time(self, t):
-        """Sponge targets of the step that reads state t."""
+        """Sponge targets and occupancy weight of the step that reads state t."""
         self._target[:] = sponge_targets(self.prob.sponges, float(t))
+        self._p.theta = float(self.theta(t))
 
     def interval(self, alpha, before, want_post=False, t=0):
         self.set_time(t)
@@ -672,10 +694,11 @@ class Objective:
         return self.weight if lo <= s <= hi else 0.0
 
 
-def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0=None, m_bar=None):
+def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0=None, m_bar=None, end_bar=None):
     """scan_adjoint with Checkpointing::Every(k) / All (scan.rs:140-246) over the oracle
     step.  Returns (J, alpha_bar, lambda_0, final_state); with occupancy coupling the
-    solid-momentum cotangent is summed into m_bar (cells x 3) when given."""
+    solid-momentum cotangent is summed into m_bar (cells x 3) when given, and with a second
+    occupancy endpoint its cotangents into end_bar = (d1_bar, m1_bar)."""
     alpha = np.asarray(alpha, np.float64)
     seg = steps if every is None else every
     x = orc.initial() if f0 is None else np.array(f0, np.float64)
@@ -685,6 +708,7 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
         x = orc.interval(alpha, x, t=t)
         w = objective.stage_weight(t + 1)
         if w:
+            orc.set_time(t + 1)
             J += w * objective.value(orc, alpha, x)
         if (t + 1) % seg == 0 and t + 1 < steps:
             checkpoints.append((t + 1, x.copy()))
@@ -707,15 +731,24 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
             x_next = states[t + 1 - start] if t + 1 < end else after
             w = objective.stage_weight(t + 1)
             if w:
+                orc.set_time(t + 1)  # the objective on state t + 1 reads its occupancy
                 g, ga = objective.grad(orc, alpha, x_next, w)
                 lam = lam + g
-                abar += ga
-                if m_bar is not None and objective.kind == "solid_force":
-                    m_bar += objective.last_m_bar
+                th = orc.theta(t + 1) if orc.d1_bar is not None else 0.0
+                abar += ga * (1.0 - th) if objective.kind == "solid_force" and orc.d1_bar is not None else ga
+                if objective.kind == "solid_force":
+                    if m_bar is not None:
+                        m_bar += objective.last_m_bar * (1.0 - th)
+                    if end_bar is not None and orc.d1_bar is not None:
+                        end_bar[0][...] += ga * th
+                        end_bar[1][...] += objective.last_m_bar * th
             fb, ab = orc.interval_vjp(alpha, states[t - start], lam, t=t)
             abar += ab
             if m_bar is not None:
                 m_bar += orc.m_bar
+            if end_bar is not None and orc.d1_bar is not None:
+                end_bar[0][...] += orc.d1_bar
+                end_bar[1][...] += orc.m1_bar
             lam = fb
         after = states[0]
     return J, abar, lam, final
@@ -729,6 +762,7 @@ def objective_value(orc: Oracle, alpha, steps, objective: Objective, f0=None):
         x = orc.interval(alpha, x, t=t)
         w = objective.stage_weight(t + 1)
         if w:
+            orc.set_time(t + 1)
             J += w * objective.value(orc, alpha, x)
     if objective.terminal_weight:
         J += objective.terminal_weight * objective.value(orc, alpha, x)

@@ -201,3 +201,40 @@ def test_solid_force_needs_the_occupancy_coupling():
         with pytest.raises(ContractError, match="occupancy coupling"):
             lat.gradient(20, Objective(kind="solid_force", t_begin=6, t_end=19, weight=0.5,
                                        vector_weights=(1.0, 0, 0, 0, 0, 0)))
+
+
+@pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
+def test_occupancy_endpoints_f64_match_oracle(law):
+    """A body moving within the interval: the occupancy blended from (d0, m0) to (d1, m1)
+    over steps 4..14 (field.rs:1266-1275).  FP64 states bit-identical; the gradient's
+    cotangents of both endpoints within 1e-11 of the oracle."""
+    shape, steps = (10, 6, 5), 18
+    cfg, prob, _, d0, m0 = _case(shape, (False, False, False), 19, "trt", law, False, True, seed=23)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(24)
+    d1 = np.roll(d0.reshape(shape), 1, axis=0).reshape(-1)
+    m1 = np.roll(m0.reshape(shape + (3,)), 1, axis=0).reshape(n, 3) + rng.standard_normal((n, 3)) * 1e-3
+    prob.occupancy_end = (d1, m1, 4, 14)
+    o = orc.Oracle(prob)
+    want = o.run(d0, 12)
+    with _lattice(cfg, d0, m0) as lat:
+        lat.set_occupancy_end(d1, m1, 4, 14)
+        lat.interval(12)
+        got = lat.state()
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+    ob = Objective(kind="pressure_drop", axis=0, plane_a=1, plane_b=shape[0] - 2, t_begin=8, t_end=steps,
+                   weight=1.0 / 11)
+    oob = orc.Objective(kind="pressure_drop", window=(8, steps), weight=1.0 / 11, axis=0, plane_a=1,
+                        plane_b=shape[0] - 2)
+    o = orc.Oracle(prob)
+    mbar, end = np.zeros((n, 3)), (np.zeros(n), np.zeros((n, 3)))
+    J_want, db_want, _, _ = orc.scan_adjoint(o, d0, steps, oob, m_bar=mbar, end_bar=end)
+    with _lattice(cfg, d0, m0) as lat:
+        lat.set_occupancy_end(d1, m1, 4, 14)
+        J, db, _, _ = lat.gradient(steps, ob)
+        mb = lat.solid_momentum_bar()
+        d1b, m1b = lat.occupancy_end_bar()
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    for got, want in ((db, db_want), (mb, mbar), (d1b, end[0]), (m1b, end[1])):
+        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()

@@ -277,3 +277,45 @@ def test_solid_force_objective_gradient(law):
         fm = orc.objective_value(orc.Oracle(orc.Problem(solid_momentum=mm, **kw)), d, steps, ob)
         fd = (fp - fm) / 2e-5
         assert abs(fd - mbar[x, a]) <= 1e-5 * max(abs(fd), 1e-8), (a, fd, mbar[x, a])
+
+
+def test_occupancy_endpoints():
+    """A second endpoint (d1, m1) blended as d0 (1 - theta) + d1 theta over steps t0..t1
+    (field.rs:1266-1275): equal endpoints give the single-occupancy states and split its
+    gradient into parts that sum to it; distinct endpoints' gradients match central
+    differences."""
+    shape, steps = (8, 5, 4), 10
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(12)
+    m0 = rng.standard_normal((n, 3)) * 0.01
+    d0 = _occupancy(shape, 13)
+    kw = dict(shape=shape, periodic=(False, True, True), omega=1.3, accel=(1e-4, 0.0, 0.0), coupling="psm",
+              psm_kappa=0.1)
+    ob = orc.Objective(kind="pressure_drop", window=(4, steps), weight=1.0, axis=0, plane_a=1, plane_b=6)
+    single = orc.Oracle(orc.Problem(solid_momentum=m0, **kw))
+    same = orc.Oracle(orc.Problem(solid_momentum=m0, occupancy_end=(d0, m0, 2, 8), **kw))
+    assert np.abs(single.run(d0, steps) - same.run(d0, steps)).max() <= 1e-15
+    mb = np.zeros((n, 3))
+    _, db, _, _ = orc.scan_adjoint(single, d0, steps, ob, m_bar=mb)
+    mb0, end = np.zeros((n, 3)), (np.zeros(n), np.zeros((n, 3)))
+    _, db0, _, _ = orc.scan_adjoint(same, d0, steps, ob, m_bar=mb0, end_bar=end)
+    assert np.abs(db0 + end[0] - db).max() <= 1e-13 * np.abs(db).max()
+    assert np.abs(mb0 + end[1] - mb).max() <= 1e-13 * np.abs(mb).max()
+    assert np.abs(end[0]).max() > 0.1 * np.abs(db).max()
+
+    d1 = np.clip(d0 + 0.2 * rng.standard_normal(n) * (d0 > 0), 0.0, None)
+    m1 = m0 + rng.standard_normal((n, 3)) * 0.005
+    mov = orc.Oracle(orc.Problem(solid_momentum=m0, occupancy_end=(d1, m1, 2, 8), **kw))
+    end = (np.zeros(n), np.zeros((n, 3)))
+    _, db0, _, _ = orc.scan_adjoint(mov, d0, steps, ob, end_bar=end)
+
+    def J(dd1):
+        o = orc.Oracle(orc.Problem(solid_momentum=m0, occupancy_end=(dd1, m1, 2, 8), **kw))
+        return orc.objective_value(o, d0, steps, ob)
+
+    for idx in np.nonzero((d0 > 0.05) & (d1 > 0.05))[0][:3]:
+        fd, rel = orc.fd_vs_ad_probe(J, d1, int(idx), None, end[0][idx], step=1e-5)
+        assert rel <= 1e-5, (idx, fd, end[0][idx])
+        fd, rel = orc.fd_vs_ad_probe(lambda x: orc.objective_value(mov, x, steps, ob), d0, int(idx), None, db0[idx],
+                                     step=1e-5)
+        assert rel <= 1e-5, (idx, fd, db0[idx])
