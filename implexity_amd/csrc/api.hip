@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "../../include/implexity_lbm.h"
+#include <hipcub/hipcub.hpp>
+
 #include "design_map.h"
 #include "pushforward.h"
 #include "launch.h"
@@ -2427,6 +2429,8 @@ PushArgs push_args(const lbm_pushforward_config* cfg, const double* xi, const do
             const double x = xi[3 * q + e];
             if (!std::isfinite(x)) fail(LBM_ERR_CONTRACT, "push-forward point " + std::to_string(q) + " is not finite");
             if (v && !std::isfinite(v[3 * q + e])) fail(LBM_ERR_CONTRACT, "point velocities must be finite");
+            if (per[e] && std::fabs(x) >= 1.0e9)
+                fail(LBM_ERR_CONTRACT, "push-forward point " + std::to_string(q) + " lies too far from the lattice");
             // Stencils::new_with_kernel (pushforward.rs:209-236) without symmetry faces
             if (!per[e] && (x - 2.0 * hf <= -1.0 || x + 2.0 * hf >= double(n[e])))
                 fail(LBM_ERR_CONTRACT, "moving LBM admission refused (pushforward_support): point " + std::to_string(q) +
@@ -2453,15 +2457,46 @@ int lbm_pushforward(const lbm_pushforward_config* cfg, const double* xi, const d
             hip_check(hipMemset(bm.p, 0, bm.bytes), "memset");
         }
         if (npoints) {
+            if (npoints >= (uint64_t(1) << 31) || ncells >= (uint64_t(1) << 31))
+                fail(LBM_ERR_CONTRACT, "push-forward needs fewer than 2^31 points and cells");
             upload(bxi, xi, size_t(npoints) * 3 * sizeof(double), "points");
             upload(ba, a, size_t(npoints) * sizeof(double), "point weights");
             if (v && m) upload(bv, v, size_t(npoints) * 3 * sizeof(double), "point velocities");
             p.xi = bxi.as<double>();
             p.a = ba.as<double>();
             p.v = bv.as<double>();
-            hipLaunchKernelGGL(k_pushforward, dim3(unsigned((npoints + 127) / 128)), dim3(128), 0, 0, p,
-                               bd.as<double>(), m ? bm.as<double>() : nullptr);
-            hip_check(hipGetLastError(), "push-forward launch");
+            const int s = 4 * p.h;
+            DevBuf key, idx, skey, sidx, wts, vel, start, end, temp;
+            key.alloc(size_t(npoints) * sizeof(int), "point keys");
+            idx.alloc(size_t(npoints) * sizeof(int), "point order");
+            skey.alloc(size_t(npoints) * sizeof(int), "sorted keys");
+            sidx.alloc(size_t(npoints) * sizeof(int), "sorted order");
+            wts.alloc(size_t(npoints) * 3 * s * sizeof(double), "sorted point weights");
+            if (m) vel.alloc(size_t(npoints) * 3 * sizeof(double), "sorted point velocities");
+            start.alloc(size_t(ncells) * sizeof(int), "bin starts");
+            end.alloc(size_t(ncells) * sizeof(int), "bin ends");
+            hip_check(hipMemset(start.p, 0, start.bytes), "memset");
+            hip_check(hipMemset(end.p, 0, end.bytes), "memset");
+            const unsigned pblocks = unsigned((npoints + 255) / 256);
+            hipLaunchKernelGGL(k_pf_keys, dim3(pblocks), dim3(256), 0, 0, p, key.as<int>(), idx.as<int>());
+            hip_check(hipGetLastError(), "push-forward keys launch");
+            int bits = 1;
+            while ((uint64_t(1) << bits) < ncells) bits++;
+            size_t temp_bytes = 0;
+            hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, key.as<int>(), skey.as<int>(),
+                                                         idx.as<int>(), sidx.as<int>(), int(npoints), 0, bits),
+                      "radix sort size");
+            temp.alloc(std::max<size_t>(temp_bytes, 16), "radix sort scratch");
+            hip_check(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, key.as<int>(), skey.as<int>(),
+                                                         idx.as<int>(), sidx.as<int>(), int(npoints), 0, bits),
+                      "radix sort");
+            hipLaunchKernelGGL(k_pf_bins, dim3(pblocks), dim3(256), 0, 0, p, skey.as<int>(), sidx.as<int>(),
+                               start.as<int>(), end.as<int>(), wts.as<double>(), m ? vel.as<double>() : nullptr);
+            hip_check(hipGetLastError(), "push-forward bins launch");
+            hipLaunchKernelGGL(k_pf_gather, dim3(unsigned((ncells + 255) / 256)), dim3(256), 0, 0, p, start.as<int>(),
+                               end.as<int>(), wts.as<double>(), m ? vel.as<double>() : nullptr, bd.as<double>(),
+                               m ? bm.as<double>() : nullptr, (long long)ncells);
+            hip_check(hipGetLastError(), "push-forward gather launch");
         }
         hip_check(hipMemcpy(d, bd.p, bd.bytes, hipMemcpyDeviceToHost), "copy");
         if (m) hip_check(hipMemcpy(m, bm.p, bm.bytes, hipMemcpyDeviceToHost), "copy");

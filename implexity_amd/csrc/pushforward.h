@@ -2,10 +2,12 @@
 //   d[c] = sum_q a_q W(c - xi_q),  m[c] = sum_q a_q v_q W(c - xi_q),
 // W the tensor product of per-axis weights of the cubic B-spline m4 or Peskin's 4-point
 // kernel of half-width h (support 4h cells from floor(x) - 2h + 1; the closed-form cubic
-// for h = 1), positions in lattice coordinates (cell c at coordinate c).  One thread per
-// point scatters its 64 h^3 weights with atomic adds (the reference sums per slab in
-// point order; the order here is the hardware's, so d and m agree to rounding).  The
-// transpose (endpoint_vjp, :946-967) is a per-point gather and needs no atomics.
+// for h = 1), positions in lattice coordinates (cell c at coordinate c).  The scatter
+// runs as a gather: the points are sorted by the cell their stencil starts at (a stable
+// radix sort), and every cell sums the points of the 64 h^3 start cells whose stencils
+// cover it, in a fixed order, so d and m are the same bits run to run and agree with the
+// reference's point-order sums to rounding.  The transpose (endpoint_vjp, :946-967) is a
+// per-point gather.  Neither uses atomics.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -96,34 +98,85 @@ __device__ __forceinline__ int pf_wrap(int c, int n) {
     return c < 0 ? c + n : c;
 }
 
-__global__ __launch_bounds__(128) void k_pushforward(PushArgs A, double* d, double* m) {
+// Sort key of every point: the cell its stencil starts at, wrapped on periodic axes (on
+// the others the stencil lies inside the lattice).
+__global__ __launch_bounds__(256) void k_pf_keys(PushArgs A, int* key, int* idx) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= A.np) return;
-    double w[3][4 * PF_MAX_H], dw[3][4 * PF_MAX_H];
+    const int n[3] = {A.nx, A.ny, A.nz};
     int first[3];
-    for (int e = 0; e < 3; e++) first[e] = pf_axis(A, A.xi[3 * q + e], w[e], dw[e]);
+    for (int e = 0; e < 3; e++) first[e] = pf_wrap(int(floor(A.xi[3 * q + e])) - 2 * A.h + 1, n[e]);
+    key[q] = int(((long long)first[0] * A.ny + first[1]) * A.nz + first[2]);
+    idx[q] = int(q);
+}
+
+// Per sorted point p: the bounds [start, end) of its start cell's bin, and its weights in
+// sorted order, wts[p * 3s + axis * s + t] (s = 4h, the x axis times a_q), vel[3p + e].
+__global__ __launch_bounds__(256) void k_pf_bins(PushArgs A, const int* skey, const int* sidx, int* start, int* end,
+                                                 double* wts, double* vel) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.np) return;
+    const int k = skey[p];
+    if (p == 0 || skey[p - 1] != k) start[k] = int(p);
+    if (p == A.np - 1 || skey[p + 1] != k) end[k] = int(p + 1);
+    const long long q = sidx[p];
     const int s = 4 * A.h;
     const double aq = A.a[q];
-    double vq[3] = {0.0, 0.0, 0.0};
-    if (A.v)
-        for (int e = 0; e < 3; e++) vq[e] = A.v[3 * q + e];
+    for (int e = 0; e < 3; e++) {
+        double w[4 * PF_MAX_H], dw[4 * PF_MAX_H];
+        pf_axis(A, A.xi[3 * q + e], w, dw);
+        for (int t = 0; t < s; t++) wts[p * 3 * s + e * s + t] = e == 0 ? aq * w[t] : w[t];
+        if (vel) vel[3 * p + e] = A.v ? A.v[3 * q + e] : 0.0;
+    }
+}
+
+// d[c] (and m[c]): over the start cells c - t, t in [0, 4h)^3 (wrapped on periodic axes),
+// and their points in point order, a_q w_x w_y w_z (times v_q).
+__global__ __launch_bounds__(256) void k_pf_gather(PushArgs A, const int* start, const int* end, const double* wts,
+                                                   const double* vel, double* d, double* m, long long ncell) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int ck = int(c % A.nz), cj = int((c / A.nz) % A.ny), ci = int(c / ((long long)A.nz * A.ny));
+    const int s = 4 * A.h;
+    double acc = 0.0, am[3] = {0.0, 0.0, 0.0};
     for (int tx = 0; tx < s; tx++) {
-        const int cx = pf_wrap(first[0] + tx, A.nx);
-        const double wx = aq * w[0][tx];
+        int bx = ci - tx;
+        if (bx < 0) {
+            if (!A.px) break;
+            bx = pf_wrap(bx, A.nx);
+        }
         for (int ty = 0; ty < s; ty++) {
-            const int cy = pf_wrap(first[1] + ty, A.ny);
-            const double wxy = wx * w[1][ty];
+            int by = cj - ty;
+            if (by < 0) {
+                if (!A.py) break;
+                by = pf_wrap(by, A.ny);
+            }
             for (int tz = 0; tz < s; tz++) {
-                const long long cell = ((long long)cx * A.ny + cy) * A.nz + pf_wrap(first[2] + tz, A.nz);
-                const double ww = wxy * w[2][tz];
-                atomicAdd(&d[cell], ww);
-                if (m) {
-                    atomicAdd(&m[3 * cell], ww * vq[0]);
-                    atomicAdd(&m[3 * cell + 1], ww * vq[1]);
-                    atomicAdd(&m[3 * cell + 2], ww * vq[2]);
+                int bz = ck - tz;
+                if (bz < 0) {
+                    if (!A.pz) break;
+                    bz = pf_wrap(bz, A.nz);
+                }
+                const long long b = ((long long)bx * A.ny + by) * A.nz + bz;
+                const int p1 = end[b];
+                for (int p = start[b]; p < p1; p++) {
+                    const double* w = wts + (long long)p * 3 * s;
+                    const double ww = w[tx] * w[s + ty] * w[2 * s + tz];
+                    acc += ww;
+                    if (m) {
+                        am[0] += ww * vel[3 * (long long)p];
+                        am[1] += ww * vel[3 * (long long)p + 1];
+                        am[2] += ww * vel[3 * (long long)p + 2];
+                    }
                 }
             }
         }
+    }
+    d[c] = acc;
+    if (m) {
+        m[3 * c] = am[0];
+        m[3 * c + 1] = am[1];
+        m[3 * c + 2] = am[2];
     }
 }
 

@@ -135,9 +135,10 @@ def test_design_map_mask_length_is_checked():
     (dict(width=3), "half-width"),
     (dict(xi=[[0.5, 4.0, 4.0]]), "pushforward_support"),
     (dict(xi=[[4.0, float("nan"), 4.0]]), "not finite"),
+    (dict(xi=[[4.0, 4.0, -3.0e9]], periodic=(True, True, True)), "too far"),
 ])
 def test_pushforward_contract_errors_before_device(kw, msg):
-    args = dict(xi=[[4.0, 4.0, 4.0]], width=1)
+    args = dict(xi=[[4.0, 4.0, 4.0]], width=1, periodic=(False, False, False))
     args.update(kw)
     with pytest.raises(lbm.ContractError, match=msg):
-        lbm.pushforward((8, 8, 8), args["xi"], [1.0], width=args["width"])
+        lbm.pushforward((8, 8, 8), args["xi"], [1.0], periodic=args["periodic"], width=args["width"])
