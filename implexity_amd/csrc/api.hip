@@ -2417,6 +2417,23 @@ PushArgs push_args(const lbm_pushforward_config* cfg, const double* xi, const do
     if (cfg->width_cells < 1 || cfg->width_cells > uint32_t(PF_MAX_H))
         fail(LBM_ERR_CONTRACT, "kernel half-width must be 1 or 2 cells");
     if (cfg->kernel > 1) fail(LBM_ERR_CONTRACT, "unknown interpolation kernel");
+    // Frame::new and Frame::with_mirrors (pushforward.rs:113-165)
+    const int span = 4 * int(cfg->width_cells);
+    for (int e = 0; e < 3; e++) {
+        const bool pe = cfg->periodic[e] != 0;
+        if (cfg->mirrored[e] > 3) fail(LBM_ERR_CONTRACT, "mirror flags are bits 0 and 1");
+        if (pe && n[e] > 1 && n[e] < span)
+            fail(LBM_ERR_CONTRACT, "periodic axis " + std::to_string(e) + " (" + std::to_string(n[e]) +
+                                       " cells) is shorter than the push-forward kernel support (" +
+                                       std::to_string(span) + " cells)");
+        if (!pe && n[e] == 1)
+            fail(LBM_ERR_CONTRACT,
+                 "a one-cell lattice axis (" + std::to_string(e) + ") must be periodic (planar) for the push-forward");
+        if (cfg->mirrored[e] && (pe || n[e] < span))
+            fail(LBM_ERR_CONTRACT, "push-forward mirror face on axis " + std::to_string(e) +
+                                       " needs a non-periodic axis of at least " + std::to_string(span) + " cells");
+        p.mirror[e] = cfg->mirrored[e];
+    }
     p.h = int(cfg->width_cells);
     p.peskin = int(cfg->kernel);
     p.np = (long long)np;
@@ -2431,8 +2448,13 @@ PushArgs push_args(const lbm_pushforward_config* cfg, const double* xi, const do
             if (v && !std::isfinite(v[3 * q + e])) fail(LBM_ERR_CONTRACT, "point velocities must be finite");
             if (per[e] && std::fabs(x) >= 1.0e9)
                 fail(LBM_ERR_CONTRACT, "push-forward point " + std::to_string(q) + " lies too far from the lattice");
-            // Stencils::new_with_kernel (pushforward.rs:209-236) without symmetry faces
-            if (!per[e] && (x - 2.0 * hf <= -1.0 || x + 2.0 * hf >= double(n[e])))
+            // Stencils::new_with_kernel (pushforward.rs:209-236)
+            if (per[e]) continue;
+            const bool lo = p.mirror[e] & 1, hi = p.mirror[e] & 2;
+            if ((lo && x < -0.5) || (hi && x > double(n[e]) - 0.5))
+                fail(LBM_ERR_CONTRACT, "moving LBM admission refused (pushforward_support): point " + std::to_string(q) +
+                                           " on axis " + std::to_string(e) + " lies beyond a symmetry face");
+            if ((!lo && x - 2.0 * hf <= -1.0) || (!hi && x + 2.0 * hf >= double(n[e])))
                 fail(LBM_ERR_CONTRACT, "moving LBM admission refused (pushforward_support): point " + std::to_string(q) +
                                            " has kernel support outside the lattice on axis " + std::to_string(e));
         }

@@ -688,13 +688,14 @@ class DesignMap:
 
 class _PushConfig(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("shape", ctypes.c_int32 * 3), ("periodic", ctypes.c_uint8 * 3),
-                ("reserved", ctypes.c_uint8 * 5), ("width_cells", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
+                ("mirrored", ctypes.c_uint8 * 3), ("reserved", ctypes.c_uint8 * 2), ("width_cells", ctypes.c_uint32),
+                ("kernel", ctypes.c_uint32)]
 
 
 PUSH_KERNEL = {"cubic": 0, "peskin4": 1}
 
 
-def _push_config(shape, periodic, width, kernel):
+def _push_config(shape, periodic, width, kernel, mirrored=None):
     if kernel not in PUSH_KERNEL:
         raise ContractError("unknown interpolation kernel")
     c = _PushConfig()
@@ -702,13 +703,17 @@ def _push_config(shape, periodic, width, kernel):
     for d in range(3):
         c.shape[d] = int(shape[d])
         c.periodic[d] = 1 if periodic[d] else 0
+        if mirrored is not None:
+            lo, hi = mirrored[d]
+            c.mirrored[d] = (1 if lo else 0) | (2 if hi else 0)
     c.width_cells, c.kernel = int(width), PUSH_KERNEL[kernel]
     return c
 
 
-def pushforward(shape, xi, a, v=None, periodic=(False, False, False), width=1, kernel="cubic"):
+def pushforward(shape, xi, a, v=None, periodic=(False, False, False), width=1, kernel="cubic", mirrored=None):
     """Occupancy d (cells) and solid momentum m (cells x 3) of body points xi (lattice
-    coordinates) with weights a and velocities v (moving/pushforward.rs scatter)."""
+    coordinates) with weights a and velocities v (moving/pushforward.rs scatter);
+    mirrored[axis] = (low face, high face) marks symmetry faces (Frame::with_mirrors)."""
     n = int(np.prod(shape))
     x = np.ascontiguousarray(xi, np.float64).reshape(-1)
     aa = np.ascontiguousarray(a, np.float64).reshape(-1)
@@ -717,12 +722,13 @@ def pushforward(shape, xi, a, v=None, periodic=(False, False, False), width=1, k
         raise ShapeError("points need three coordinates, one weight and three velocity components each")
     d = np.empty(n)
     m = np.empty(3 * n)
-    c = _push_config(shape, periodic, width, kernel)
+    c = _push_config(shape, periodic, width, kernel, mirrored)
     _check(lib().lbm_pushforward(ctypes.byref(c), _p(x), _p(aa), _p(vv), aa.size, _p(d), _p(m), n))
     return d, m.reshape(n, 3)
 
 
-def pushforward_vjp(shape, xi, a, v, d_bar, m_bar, periodic=(False, False, False), width=1, kernel="cubic"):
+def pushforward_vjp(shape, xi, a, v, d_bar, m_bar, periodic=(False, False, False), width=1, kernel="cubic",
+                    mirrored=None):
     """(xi_bar, a_bar, v_bar) of <d_bar, d> + <m_bar, m> (endpoint_vjp)."""
     n = int(np.prod(shape))
     x = np.ascontiguousarray(xi, np.float64).reshape(-1)
@@ -733,7 +739,7 @@ def pushforward_vjp(shape, xi, a, v, d_bar, m_bar, periodic=(False, False, False
     if x.size != 3 * aa.size or vv.size != x.size or db.size != n or mb.size != 3 * n:
         raise ShapeError("push-forward cotangent shapes")
     xb, ab, vb = np.empty(x.size), np.empty(aa.size), np.empty(x.size)
-    c = _push_config(shape, periodic, width, kernel)
+    c = _push_config(shape, periodic, width, kernel, mirrored)
     _check(lib().lbm_pushforward_vjp(ctypes.byref(c), _p(x), _p(aa), _p(vv), aa.size, _p(db), _p(mb), n, _p(xb),
                                      _p(ab), _p(vb)))
     return xb.reshape(-1, 3), ab, vb.reshape(-1, 3)

@@ -415,20 +415,26 @@ int lbm_design_map_destroy(lbm_design_map* m);
 int lbm_design_map_apply(lbm_design_map* m, const double* raw, double* phi, uint64_t n);
 int lbm_design_map_vjp(lbm_design_map* m, const double* raw, const double* phi_bar, double* raw_bar, uint64_t n);
 
-/* Push-forward of body points into the occupancy (moving/pushforward.rs, axes without
- * symmetry faces): d[c] = sum_q a_q W(c - xi_q) and m[c] = sum_q a_q v_q W(c - xi_q), W
+/* Push-forward of body points into the occupancy (moving/pushforward.rs):
+ * d[c] = sum_q a_q W(c - xi_q) and m[c] = sum_q a_q v_q W(c - xi_q), W
  * the product of per-axis weights of the cubic B-spline (kernel 0) or Peskin's 4-point
  * kernel (1) of half-width width_cells (support 4 width_cells cells per axis), positions
  * in lattice coordinates (cell c at coordinate c; Frame::to_lattice maps metres to them).
  * d and m feed lbm_set_design / lbm_set_solid_momentum of an occupancy-coupled handle;
  * m is [cell][3].  The transpose gives the cotangents of the points, their weights and
- * their velocities (endpoint_vjp).  A point whose support leaves a closed axis is
+ * their velocities (endpoint_vjp).  Frame::new / with_mirrors (:113-165): a periodic
+ * axis is at least 4 width_cells long unless it has one cell (planar: weight 1 on it), a
+ * closed axis has more than one cell, and a symmetry face (mirrored bit 0 at the low end,
+ * bit 1 at the high end) sits on a closed axis of at least 4 width_cells; the support
+ * is folded back into the lattice at a face it crosses (fold_axis, :344-362).  A point
+ * whose support leaves a closed axis without a face, or that lies beyond a face, is
  * refused (pushforward_support). */
 typedef struct {
     uint32_t struct_size;   /* sizeof(lbm_pushforward_config) */
     int32_t shape[3];
     uint8_t periodic[3];
-    uint8_t reserved[5];
+    uint8_t mirrored[3];    /* per axis: bit 0 symmetry face at cell -1/2, bit 1 at n - 1/2 */
+    uint8_t reserved[2];
     uint32_t width_cells;   /* 1 or 2 */
     uint32_t kernel;        /* 0 cubic B-spline (m4), 1 Peskin 4-point */
 } lbm_pushforward_config;

@@ -1,19 +1,24 @@
 """CPU restatement of the push-forward of body points into the lattice occupancy --
 TEST INFRASTRUCTURE ONLY (the checker of the device push-forward).
 
-Restates crates/implexity-physics-lbm/src/moving/pushforward.rs for axes without
-symmetry faces: the kernels m4 / m4_derivative (:41-70) and peskin4 /
-peskin4_derivative (:88-100), the per-axis weights of Stencils::new_with_kernel
-(:205-318; the closed-form cubic branch for width 1, the kernel formula otherwise), the
-scatter of a_q and a_q v_q into d and m (:766-860, summed here in point order), and
-endpoint_vjp (:946-967) for the cotangents of the points, their weights and
-velocities.  Positions are lattice coordinates (cell c at coordinate c;
-Frame::to_lattice, :186-189, maps metres to them).
+Restates crates/implexity-physics-lbm/src/moving/pushforward.rs: the frame checks of
+Frame::new / with_mirrors (:113-165; a periodic axis is at least the kernel support
+long unless it is one cell, planar, with weight 1 on its cell; a closed axis is longer
+than one cell; a symmetry face sits on a closed axis of at least the support), the
+kernels m4 / m4_derivative (:41-70) and peskin4 / peskin4_derivative (:88-100), the
+per-axis weights of Stencils::new_with_kernel (:205-318; the closed-form cubic branch
+for width 1, the kernel formula otherwise, folded by fold_axis, :344-362, into the
+lattice at a symmetry face the support crosses), the scatter of a_q and a_q v_q into d
+and m (:766-860, summed here in point order), and endpoint_vjp (:946-967) for the
+cotangents of the points, their weights and velocities.  Positions are lattice
+coordinates (cell c at coordinate c; Frame::to_lattice, :186-189, maps metres to them).
 
 Pins (tests/test_oracle_pushforward.py): both kernels are partitions of unity and
 reproduce linear functions, so the total occupancy is sum a_q and its first moment
-sum a_q xi_q; the width-1 closed form equals m4; Peskin's even/odd sums are 1/2; the
-transposes against central differences.
+sum a_q xi_q; the width-1 closed form equals m4; Peskin's even/odd sums are 1/2; a
+symmetry face equals the unfolded scatter of the point and its mirror image restricted
+to the lattice (the kernels are even); a planar axis equals the sum over a periodic
+axis of the support's length; the transposes against central differences.
 """
 from __future__ import annotations
 
@@ -88,12 +93,39 @@ def axis_weights(x, h, kernel):
     return first, w, dw
 
 
+def fold(w, first, start, n):
+    """fold_axis: weights of cells first + t reflected into the lattice (c < 0 -> -1 - c,
+    c >= n -> 2n - 1 - c) and summed at offsets from start."""
+    out = np.zeros_like(w)
+    for t in range(len(w)):
+        c = first + t
+        m = -1 - c if c < 0 else (2 * n - 1 - c if c >= n else c)
+        out[m - start] += w[t]
+    return out
+
+
 class PushForwardOracle:
-    def __init__(self, shape, periodic=(False, False, False), width=1, kernel="cubic"):
+    def __init__(self, shape, periodic=(False, False, False), width=1, kernel="cubic",
+                 mirrored=((False, False), (False, False), (False, False))):
         self.shape = tuple(int(s) for s in shape)
         self.periodic = tuple(bool(p) for p in periodic)
         self.h, self.kernel = int(width), kernel
+        self.mirrored = tuple((bool(lo), bool(hi)) for lo, hi in mirrored)
         self.n = int(np.prod(self.shape))
+        span = 4 * self.h
+        for a in range(3):
+            if self.periodic[a] and 1 < self.shape[a] < span:
+                raise ValueError(f"periodic axis {a} is shorter than the push-forward kernel support")
+            if not self.periodic[a] and self.shape[a] == 1:
+                raise ValueError(f"a one-cell lattice axis ({a}) must be periodic (planar)")
+            if any(self.mirrored[a]) and (self.periodic[a] or self.shape[a] < span):
+                raise ValueError(f"push-forward mirror face on axis {a} needs a non-periodic axis")
+
+    def planar(self, a):
+        return self.periodic[a] and self.shape[a] == 1
+
+    def support(self, a):
+        return 1 if self.planar(a) else 4 * self.h
 
     def _check(self, xi):
         for q, x in enumerate(xi):
@@ -103,18 +135,34 @@ class PushForwardOracle:
                     raise ValueError(f"push-forward point {q} is not finite")
                 if self.periodic[a]:
                     continue
-                if v - 2.0 * self.h <= -1.0 or v + 2.0 * self.h >= self.shape[a]:
+                lo, hi = self.mirrored[a]
+                n = self.shape[a]
+                if (lo and v < -0.5) or (hi and v > n - 0.5):
+                    raise ValueError(f"point {q} lies beyond a symmetry face")
+                if (not lo and v - 2.0 * self.h <= -1.0) or (not hi and v + 2.0 * self.h >= n):
                     raise ValueError(f"point {q} has kernel support outside the lattice")
+
+    def _axis(self, x, a):
+        """(first cell, weights, d weights / d x) of coordinate x on axis a, folded at a
+        symmetry face the support crosses."""
+        if self.planar(a):
+            return 0, np.ones(1), np.zeros(1)
+        first, w, dw = axis_weights(x, self.h, self.kernel)
+        n, span = self.shape[a], 4 * self.h
+        lo, hi = self.mirrored[a]
+        start = 0 if (lo and first < 0) else (n - span if (hi and first + span > n) else None)
+        if start is not None:
+            w, dw, first = fold(w, first, start, n), fold(dw, first, start, n), start
+        return first, w, dw
 
     def _stencil(self, x):
         """cells (flat, per tensor entry), weights, and the three partial weight products."""
-        per = [axis_weights(float(x[a]), self.h, self.kernel) for a in range(3)]
-        n = 4 * self.h
+        per = [self._axis(float(x[a]), a) for a in range(3)]
         out = []
         nx, ny, nz = self.shape
-        for tx in range(n):
-            for ty in range(n):
-                for tz in range(n):
+        for tx in range(len(per[0][1])):
+            for ty in range(len(per[1][1])):
+                for tz in range(len(per[2][1])):
                     c = [per[0][0] + tx, per[1][0] + ty, per[2][0] + tz]
                     c = [c[a] % self.shape[a] for a in range(3)]
                     cell = (c[0] * ny + c[1]) * nz + c[2]

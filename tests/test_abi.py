@@ -142,3 +142,21 @@ def test_pushforward_contract_errors_before_device(kw, msg):
     args.update(kw)
     with pytest.raises(lbm.ContractError, match=msg):
         lbm.pushforward((8, 8, 8), args["xi"], [1.0], periodic=args["periodic"], width=args["width"])
+
+
+NO_FACES = ((False, False), (False, False), (False, False))
+
+
+@pytest.mark.parametrize("shape,periodic,mirrored,xi,msg", [
+    ((3, 8, 8), (True, False, False), NO_FACES, [4.0, 4.0, 4.0], "shorter than the push-forward kernel support"),
+    ((8, 1, 8), (False, False, False), NO_FACES, [4.0, 0.0, 4.0], "must be periodic"),
+    ((8, 8, 8), (True, False, False), ((True, False), (False, False), (False, False)), [4.0, 4.0, 4.0],
+     "mirror face"),
+    ((8, 8, 8), (False, False, False), ((True, False), (False, False), (False, False)), [-0.51, 4.0, 4.0],
+     "beyond a symmetry face"),
+    ((8, 8, 8), (False, False, False), ((True, False), (False, False), (False, False)), [6.2, 4.0, 4.0],
+     "outside the lattice"),
+])
+def test_pushforward_frame_errors_before_device(shape, periodic, mirrored, xi, msg):
+    with pytest.raises(lbm.ContractError, match=msg):
+        lbm.pushforward(shape, [xi], [1.0], periodic=periodic, width=1, mirrored=mirrored)

@@ -58,12 +58,12 @@ def test_pushforward_vjp_matches_oracle(shape, periodic, h, kernel):
         assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("shape,periodic,h,kernel", [((3, 9, 10), (True, False, True), 1, "cubic"),
-                                                     ((5, 12, 3), (True, True, True), 2, "peskin4")])
+@pytest.mark.parametrize("shape,periodic,h,kernel", [((1, 9, 10), (True, False, True), 1, "cubic"),
+                                                     ((8, 12, 1), (True, True, True), 2, "peskin4")])
 def test_pushforward_clustered_and_wrapping_is_deterministic(shape, periodic, h, kernel):
-    """Many points per start cell, periodic axes shorter than the stencil (a point's
-    stencil covers a cell more than once) and coordinates outside [0, n) on periodic
-    axes: the oracle's sums, and the same bits on a second call."""
+    """Many points per start cell, a planar (one-cell periodic) axis, a periodic axis as
+    long as the support, and coordinates outside [0, n) on periodic axes: the oracle's
+    sums, and the same bits on a second call."""
     rng = np.random.default_rng(8)
     n = 300
     centre = np.array([s / 2.0 for s in shape])
@@ -81,6 +81,48 @@ def test_pushforward_clustered_and_wrapping_is_deterministic(shape, periodic, h,
     assert np.abs(m1 - m_want).max() <= 1e-13 * np.abs(m_want).max()
     assert np.array_equal(d1.view(np.uint64), d2.view(np.uint64))
     assert np.array_equal(m1.view(np.uint64), m2.view(np.uint64))
+
+
+MIRROR_CASES = [
+    ((12, 10, 14), (False, False, False), ((True, True), (False, False), (True, False)), 1, "cubic"),
+    ((12, 10, 14), (False, True, False), ((False, True), (False, False), (True, True)), 2, "cubic"),
+    ((9, 12, 10), (False, False, True), ((True, False), (True, True), (False, False)), 1, "peskin4"),
+    ((16, 1, 12), (False, True, False), ((True, True), (False, False), (False, True)), 2, "peskin4"),
+]
+
+
+def _face_points(shape, periodic, mirrored, h, seed, n=60):
+    """Points crowding the symmetry faces, inside the admissible range elsewhere."""
+    rng = np.random.default_rng(seed)
+    xi = np.empty((n, 3))
+    for e in range(3):
+        lo, hi = mirrored[e]
+        if periodic[e]:
+            xi[:, e] = rng.uniform(-2.0 * shape[e], 3.0 * shape[e], n)
+            continue
+        a = -0.5 if lo else 2.0 * h - 1.0 + 1e-6
+        b = shape[e] - 0.5 if hi else shape[e] - 2.0 * h - 1e-6
+        near = np.where(rng.random(n) < 0.5, rng.uniform(a, a + 2.0 * h, n), rng.uniform(b - 2.0 * h, b, n))
+        xi[:, e] = np.clip(near, a, b)
+    return xi, rng.random(n), rng.standard_normal((n, 3))
+
+
+@pytest.mark.parametrize("shape,periodic,mirrored,h,kernel", MIRROR_CASES)
+def test_pushforward_symmetry_faces_match_oracle(shape, periodic, mirrored, h, kernel):
+    xi, a, v = _face_points(shape, periodic, mirrored, h, 10)
+    o = PushForwardOracle(shape, periodic, h, kernel, mirrored)
+    d_want, m_want = o.scatter(xi, a, v)
+    d, m = lbm.pushforward(shape, xi, a, v, periodic, h, kernel, mirrored)
+    assert np.abs(d - d_want).max() <= 1e-14 * max(1.0, np.abs(d_want).max())
+    assert np.abs(m - m_want).max() <= 1e-14 * max(1.0, np.abs(m_want).max())
+    assert abs(d.sum() - a.sum()) <= 1e-12 * a.sum()
+    nn = int(np.prod(shape))
+    rng = np.random.default_rng(11)
+    db, mb = rng.standard_normal(nn), rng.standard_normal((nn, 3))
+    xw, aw, vw = o.vjp(xi, a, v, db, mb)
+    xb, ab, vb = lbm.pushforward_vjp(shape, xi, a, v, db, mb, periodic, h, kernel, mirrored)
+    for got, want in ((xb, xw), (ab, aw), (vb, vw)):
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
 
 
 def test_pushforward_totals_at_128_cubed():
