@@ -1892,6 +1892,18 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
             evs.push_back(e);
             return e;
         };
+        // A span that starts where the previous one ended shares its event: each record
+        // leaves the stream idle for a few microseconds.  `mark` is an event at the stream's
+        // current position (null once other work was queued after it).
+        hipEvent_t mark = nullptr;
+        auto span_start = [&]() {
+            if (!mark) mark = event();
+            return mark;
+        };
+        auto span_end = [&]() {
+            mark = event();
+            return mark;
+        };
         double fwd_ms = 0.0, adj_ms = 0.0;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd_spans, adj_spans, fwd_k, adj_k;
         // Every KSAMPLE-th step kernel is bracketed by events to time the kernel alone.
@@ -1928,6 +1940,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                                              h->stream),
                               "snapshot copy");
                     copies++;
+                    mark = nullptr;
                 }
                 break;
             case ActKind::Restore:
@@ -1942,7 +1955,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                 if (idx + 1 < sch.actions.size() && sch.actions[idx + 1].kind == ActKind::Snapshot &&
                     sch.actions[idx + 1].a == act.b)
                     last_dst = slot_ptr(sch.actions[idx + 1].b);
-                const hipEvent_t e0 = event();
+                const hipEvent_t e0 = span_start();
                 for (uint64_t t = act.a; t < act.b; t++) {
                     const ObjArgs ob = forward_phase ? stage_obj(tmpl, obj, t, nsteps, false) : none;
                     if (t + 1 == nsteps) {
@@ -1987,13 +2000,13 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     fsteps++;
                     cursor = t + 1;
                 }
-                fwd_spans.emplace_back(e0, event());
+                fwd_spans.emplace_back(e0, span_end());
                 break;
             }
             case ActKind::Reverse: {
                 const uint64_t k = act.a;  // reverse step k-1 -> k; `cur` holds state k-1
                 if (cursor + 1 != k) fail(LBM_ERR_INVALID, "reverse of a step whose states are not held");
-                const hipEvent_t e0 = event();
+                const hipEvent_t e0 = span_start();
                 if (h->np)
                     hip_check(h->run->port_vjp(Q, h->has_mask, h->port_args(base + k), h->L.p, h->pbuf.p,
                                                h->lsc_ptr(), h->stream),
@@ -2021,7 +2034,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     hip_check(h->run->port_save(Q, h->has_mask, h->port_args(base + k - 1), cur, h->pbuf.p,
                                                 h->stream),
                               "port save");
-                adj_spans.emplace_back(e0, event());
+                adj_spans.emplace_back(e0, span_end());
                 break;
             }
             }
