@@ -14,6 +14,11 @@
   size bit-identical to the oracle; 5000 steps in FP64, FP32 and FP16-shifted storage,
   with the relative L2 velocity error and the error of the summed dissipation of the
   reduced precisions against FP64.
+* 256^3 D3Q19 TRT inlet/outlet channel of the benchmark (tau+ = 0.55, parabolic
+  u_x = 0.03 inlet, density-1 outlet, 64 blobs): two FP64 steps bit-identical to the
+  oracle, the FP64 gradient of a 3-step pressure-drop window against the oracle's
+  scan_adjoint, and the FP16-shifted gradient (FP16 cotangents) of a 1000-step horizon
+  against the FP64 one (relative L2 <= 1e-3).
 * 384 x 192 x 192 D3Q19 flow coupled to the D3Q7 temperature field, 6000 steps: the
   FP16-storage design gradient against the FP64 one, and FP64 central differences on 16
   seeded voxels over a 1000-step horizon.
@@ -222,6 +227,65 @@ def test_cumulant_256_storage_precisions():
           ", ".join(f"{p}: velocity L2 {e[0]:.3e}, dissipation {e[1]:.3e}" for p, e in out.items()))
     assert out["f32"][0] <= 1e-3 and out["f32"][1] <= 1e-3, out
     assert out["f16s"][0] <= 5e-2 and out["f16s"][1] <= 5e-2, out
+
+
+# --- 256^3 D3Q19 TRT channel of the benchmark -------------------------------------------------
+SHAPE3 = (256, 256, 256)
+
+
+def _config3(precision="f64"):
+    from dataclasses import replace
+    phi = design.blob_design(SHAPE3, 64, 3, radius=(6.0, 20.0))
+    alpha = design.resistance(phi, 0.1, 0.1).reshape(-1)
+    cfg, prob = make_pair(SHAPE3, (False, False, False), Q=19, collision="trt", precision=precision,
+                          omega=1.0 / 0.55, ports=channel_ports(SHAPE3, 0.03))
+    if precision == "f16s":
+        cfg = replace(cfg, adjoint_storage="f16")
+    return cfg, prob, alpha
+
+
+def test_channel_256_two_steps_f64_bit_exact():
+    cfg, prob, alpha = _config3()
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        lat.interval(2)
+        got = lat.state()
+    want = orc.Oracle(prob).run(alpha, 2)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+def test_channel_256_window_gradient_matches_oracle_f64():
+    steps = 3
+    cfg, prob, alpha = _config3()
+    ob = _dp_objective(1, steps, SHAPE3[0])
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.initial_populations()
+        J, ab, _, _ = lat.gradient(steps, ob)
+    oob = orc.Objective(kind="pressure_drop", window=(1, steps), weight=ob.weight, axis=0, plane_a=1,
+                        plane_b=SHAPE3[0] - 2)
+    J_want, ab_want, _, _ = orc.scan_adjoint(orc.Oracle(prob), alpha, steps, oob)
+    assert abs(J - J_want) <= 1e-12 * abs(J_want)
+    assert np.abs(ab - ab_want).max() <= 1e-11 * np.abs(ab_want).max()
+
+
+def test_channel_256_gradient_f16_vs_f64():
+    steps = 1000
+    ob = _dp_objective(steps - 199, steps, SHAPE3[0])
+    out = {}
+    for precision in ("f64", "f16s"):
+        cfg, _, alpha = _config3(precision)
+        with Lattice(cfg) as lat:
+            lat.set_design(alpha)
+            lat.initial_populations()
+            J, ab, _, st = lat.gradient(steps, ob)
+        assert st.bad_cells == 0
+        out[precision] = (J, ab)
+    err = _l2(out["f16s"][1], out["f64"][1])
+    jrel = abs(out["f16s"][0] - out["f64"][0]) / abs(out["f64"][0])
+    print(f"\n256^3 channel, 1000 steps: FP16 gradient rel L2 {err:.3e}, J rel {jrel:.3e}")
+    assert err <= 1e-3 and jrel <= 5e-3, (err, jrel)
 
 
 # --- 384 x 192 x 192 flow + temperature, 6000 steps -----------------------------------------
