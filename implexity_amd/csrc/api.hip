@@ -82,9 +82,59 @@ struct DevBuf {
         }
         bytes = n;
     }
+    // Keeps the allocation when it already holds n bytes (contents are not preserved
+    // when it grows).
+    void grow(size_t n, const char* what) {
+        if (p && bytes >= n) return;
+        alloc(n, what);
+    }
     template <class T>
     T* as() const {
         return static_cast<T*>(p);
+    }
+};
+
+// Two pinned host chunks through which large device-to-host copies are staged: chunk
+// k + 1 is copied over PCIe while chunk k is copied out of pinned memory on the host.
+struct HostStage {
+    static constexpr size_t CHUNK = size_t(8) << 20;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() {
+        for (int b = 0; b < 2; b++) {
+            if (buf[b]) (void)hipHostFree(buf[b]);
+            if (ev[b]) (void)hipEventDestroy(ev[b]);
+        }
+    }
+    void ensure() {
+        for (int b = 0; b < 2; b++) {
+            if (!buf[b]) hip_check(hipHostMalloc(&buf[b], CHUNK, hipHostMallocDefault), "pinned staging allocation");
+            if (!ev[b]) hip_check(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming), "event");
+        }
+    }
+    // Copies `bytes` from device memory to host memory; the stream's earlier work is
+    // ordered before the copy.  Returns after the data is in `dst`.
+    void download(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+        ensure();
+        const size_t n = (bytes + CHUNK - 1) / CHUNK;
+        auto drain = [&](size_t k) {
+            const int b = int(k & 1);
+            hip_check(hipEventSynchronize(ev[b]), "staging sync");
+            const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off);
+            std::memcpy(static_cast<char*>(dst) + off, buf[b], len);
+        };
+        for (size_t k = 0; k < n; k++) {
+            const int b = int(k & 1);
+            if (k >= 2) drain(k - 2);
+            const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off);
+            hip_check(hipMemcpyAsync(buf[b], static_cast<const char*>(src) + off, len, hipMemcpyDeviceToHost, stream),
+                      "staged copy");
+            hip_check(hipEventRecord(ev[b], stream), "event record");
+        }
+        for (size_t k = n >= 2 ? n - 2 : 0; k < n; k++) drain(k);
     }
 };
 
@@ -193,6 +243,7 @@ struct lbm_handle {
     double les_c = 0.0, les_floor = 0.0;  // Smagorinsky closure (K_TRT_LES / K_BGK_LES)
     double wale_c2 = 0.0, wale_floor = 0.0;  // WALE closure (K_TRT_WALE / K_BGK_WALE)
     uint64_t every = 0;  // Checkpointing::Every(k); 0 = binomial
+    double mach_limit = 0.0, drift_limit = 1e-6;  // admission (0: none)
     int nx = 0, ny = 0, nz = 0;
     int per[3] = {0, 0, 0};
     double omega = 1.0, om = 1.0, tau = 1.0, magic = 3.0 / 16.0, bulk = NAN, even = NAN;
@@ -205,6 +256,9 @@ struct lbm_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf F, A, L, alpha, alpha_bar, mask, mrt, pcell, pnb, pkind, pval, pbuf, scal, aos, pool, part, lsc, fold;
+    DevBuf ledger;     // mass ledger of the last sweep (grows to the longest horizon)
+    bool abar_ready = false;  // alpha_bar holds the result of a gradient or step VJP
+    HostStage stage;   // pinned staging for device-to-host copies
     long long npart = 0;
     double alpha_max = 0.0;  // max |alpha| over the design (bounds dissipation seeds)
     std::vector<uint8_t> solid;
@@ -288,7 +342,7 @@ struct lbm_handle {
         s.part = nullptr;
         s.npart = npart;
         s.stage = fold.as<double>();
-        s.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
+        s.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + FOLD_VALUES * FOLD_MAX_BLOCKS);
         s.jacc = scal.as<double>();
         s.macc = scal.as<double>() + 1;
         s.dstat = reinterpret_cast<unsigned*>(scal.as<double>() + 4);
@@ -360,7 +414,7 @@ struct lbm_handle {
         t.obj_w = 0.0;
         t.part = nullptr;
         t.stage = fold.as<double>();
-        t.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
+        t.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + FOLD_VALUES * FOLD_MAX_BLOCKS);
         t.jacc = scal.as<double>();
         return t;
     }
@@ -391,7 +445,7 @@ struct lbm_handle {
         a.part = part.as<double>();
         a.npart = npart;
         a.stage = fold.as<double>();
-        a.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + 4 * FOLD_MAX_BLOCKS);
+        a.ticket = reinterpret_cast<unsigned*>(fold.as<double>() + FOLD_VALUES * FOLD_MAX_BLOCKS);
         a.lsc = lsc_ptr();
         a.s_next = 0.0;
         return a;
@@ -622,33 +676,78 @@ uint64_t reserve_pool(lbm_handle* h, uint64_t nsteps) {
 // every step (one partial per port or sponge block), for the admission drift
 // |M_s - M_0 - sum_{r<s} X_r| / M_0 (solver.rs:707-793).
 struct MassLedger {
-    DevBuf buf;
+    DevBuf* buf = nullptr;
     uint64_t n = 0;
     long long nbp = 0;  // port blocks, then sponge blocks
     long long npb = 0;  // port blocks
+    size_t used = 0;    // bytes of the ledger this sweep uses
     void init(lbm_handle* h, uint64_t nsteps) {
         n = nsteps;
         npb = (h->np + PORT_BLOCK - 1) / PORT_BLOCK;
         nbp = npb + (h->nsc + SPONGE_BLOCK - 1) / SPONGE_BLOCK;
-        buf.alloc(((n + 2) + (n + 2) * size_t(nbp)) * sizeof(double), "mass ledger");
-        hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream), "memset");
+        used = ((n + 2) + (n + 2) * size_t(nbp)) * sizeof(double);
+        buf = &h->ledger;
+        buf->grow(used, "mass ledger");
+        hip_check(hipMemsetAsync(buf->p, 0, used, h->stream), "memset");
     }
-    double* mass(uint64_t s) const { return buf.as<double>() + s; }
-    double* exch(uint64_t s) const { return buf.as<double>() + (n + 2) + s * size_t(nbp); }
-    // Call after the stream has been synchronised.
-    double drift() const {
-        std::vector<double> v(buf.bytes / sizeof(double));
-        hip_check(hipMemcpy(v.data(), buf.p, buf.bytes, hipMemcpyDeviceToHost), "copy");
+    double* mass(uint64_t s) const { return buf->as<double>() + s; }
+    double* exch(uint64_t s) const { return buf->as<double>() + (n + 2) + s * size_t(nbp); }
+    // Max relative drift |M_s - M_0 - sum_{r<s} X_r| / M_0 over the states 0..upto.  Call
+    // after the stream has been synchronised.
+    double drift(uint64_t upto) const {
+        std::vector<double> v(used / sizeof(double));
+        hip_check(hipMemcpy(v.data(), buf->p, used, hipMemcpyDeviceToHost), "copy");
         const double* M = v.data();
         double ex = 0.0, worst = 0.0;
-        for (uint64_t s = 0; s <= n; s++) {
+        for (uint64_t s = 0; s <= std::min(upto, n); s++) {
             worst = std::max(worst, std::fabs(M[s] - M[0] - ex));
             const double* X = v.data() + (n + 2) + s * size_t(nbp);
             for (long long b = 0; b < nbp; b++) ex += X[b];
         }
         return worst / M[0];
     }
+    double drift() const { return drift(n); }
 };
+
+// Diagnostics of a sweep's states as the fold kernels left them in the scalar record:
+// [0] max |u|^2 (float bits), [1] bad cells, [2] order-preserving key of max -f.
+struct SweepDiag {
+    double max_mach = 0.0, min_population = NAN;
+    uint64_t bad_cells = 0;
+};
+
+SweepDiag read_diag(lbm_handle* h) {
+    unsigned ds[3];
+    hip_check(hipMemcpyAsync(ds, h->scal.as<double>() + 4, sizeof(ds), hipMemcpyDeviceToHost, h->stream), "copy");
+    hip_check(hipStreamSynchronize(h->stream), "sync");
+    SweepDiag d;
+    float umax;
+    std::memcpy(&umax, &ds[0], 4);
+    d.max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
+    d.bad_cells = ds[1];
+    if (ds[2] != 0) {  // inverse of float_key
+        const unsigned u = (ds[2] & 0x80000000u) ? (ds[2] & 0x7fffffffu) : ~ds[2];
+        float fneg;
+        std::memcpy(&fneg, &u, 4);
+        d.min_population = -double(fneg);
+    }
+    return d;
+}
+
+// LbmProblem::admit (solver.rs:707-793): refuse a trajectory with a non-finite or
+// non-positive fluid population, a Mach number above the limit or a relative mass drift
+// above the bound.  No-op when the handle has no Mach limit.
+void admit(const lbm_handle* h, const SweepDiag& d, double drift) {
+    if (h->mach_limit <= 0.0) return;
+    if (d.bad_cells != 0)
+        fail(LBM_ERR_CONVERGENCE, "LBM trajectory has nonpositive or nonfinite populations (" +
+                                      std::to_string(d.bad_cells) + " cell states)");
+    if (d.max_mach > h->mach_limit || !(drift <= h->drift_limit)) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "LBM trajectory rejected: Mach=%.17g, mass drift=%.17g", d.max_mach, drift);
+        fail(LBM_ERR_CONVERGENCE, msg);
+    }
+}
 
 // The flow's transposed step on the cotangent buffer (a.pop and a.odd name the state):
 // the sponge transpose first; with the WALE closure, the velocity pre-pass before the
@@ -717,6 +816,18 @@ void thermal_final(lbm_handle* h, void* buf, uint64_t step, double obj_w, double
     ta.obj_w = obj_w;
     if (obj_w != 0.0) ta.part = h->part.as<double>();
     hip_check(h->trun->seed(ta, seed_w, do_seed, h->cfg()), "thermal objective launch");
+}
+
+// Arguments of the objective kernel on the final state of a sweep, with its diagnostics
+// (Mach, populations) folded into the sweep's record.  The occupancy couplings measure
+// the fluid velocity u = j / rho + a / 2 (zero design), the velocity their collisions
+// start from.
+StepArgs final_state_args(lbm_handle* h, void* buf, uint64_t step) {
+    StepArgs ea = h->step_args(buf, step);
+    ea.part = h->part.as<double>();
+    ea.diag = 1;
+    if (is_psm(h->kind)) ea.alpha = h->alpha0.p;
+    return ea;
 }
 
 void step_once(lbm_handle* h, void* buf, uint64_t step, const ObjArgs& obj, const MassLedger* led,
@@ -946,6 +1057,14 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             }
             h->psm_kappa = cfg->psm_kappa;
         }
+        // Admission (solver.rs:434-437: the Mach limit is positive and at most 0.2).
+        if (!(cfg->mach_limit == 0.0 || (cfg->mach_limit > 0.0 && cfg->mach_limit <= 0.2)))
+            fail(LBM_ERR_CONTRACT, "LBM mach_limit must be 0 (no admission) or in (0, 0.2]; this is not a "
+                                   "compressible gas solver");
+        if (!(cfg->drift_limit >= 0.0) || !std::isfinite(cfg->drift_limit))
+            fail(LBM_ERR_CONTRACT, "drift_limit must be finite and non-negative");
+        h->mach_limit = cfg->mach_limit;
+        h->drift_limit = cfg->drift_limit > 0.0 ? cfg->drift_limit : 1e-6;
         if (h->kind == K_CUMULANT && h->Q != 27)
             fail(LBM_ERR_CONTRACT, "cumulant collision requires the D3Q27 velocity set (got D3Q19)");
         if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
@@ -1162,11 +1281,9 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->scal.alloc(64, "scalars");
         {
             const long long b = std::min(256, ((h->nz + 63) / 64) * 64);
-            const long long blocks_3d = ((h->nz + b - 1) / b) * (long long)h->ny * h->nx;
-            const long long blocks_v = ((long long)h->nx * h->ny * (h->nz / 2) + 127) / 128;
-            h->npart = std::max(blocks_3d, blocks_v);
-            h->part.alloc(size_t(4 * h->npart) * sizeof(double), "block partials");
-            h->fold.alloc((4 * FOLD_MAX_BLOCKS + 2) * sizeof(double), "fold stage");
+            h->npart = ((h->nz + b - 1) / b) * (long long)h->ny * h->nx;
+            h->part.alloc(size_t(PART_STRIDE) * size_t(h->npart) * sizeof(double), "block partials");
+            h->fold.alloc((FOLD_VALUES * FOLD_MAX_BLOCKS + 2) * sizeof(double), "fold stage");
             hip_check(hipMemsetAsync(h->fold.p, 0, h->fold.bytes, h->stream), "memset");
         }
         if (h->has_mask) {
@@ -1460,31 +1577,32 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
             h->t++;
         }
         hip_check(hipEventRecord(h->ev1, h->stream), "event");
-        // Final state: objective (stage at n + terminal) and its mass.
-        StepArgs ea = h->step_args(h->F.p, h->t);
-        ea.part = h->part.as<double>();
+        // Final state: objective (stage at n + terminal), its mass and diagnostics.
+        StepArgs ea = final_state_args(h, h->F.p, h->t);
         ea.obj = stage_obj(tmpl, obj, nsteps, nsteps, true);
         ea.macc = led.mass(nsteps);
         ObjArgs none{};
         hip_check(h->run->objective(h->Q, h->has_mask, ea, none, nullptr, nullptr, 0, h->cfg()), "objective launch");
         thermal_final(h, h->F.p, h->t, ea.obj.tw, 0.0, 0);
-        double sc[4];
-        unsigned ds[2];
-        hip_check(hipMemcpyAsync(sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream), "copy");
-        hip_check(hipMemcpyAsync(ds, h->scal.as<double>() + 4, sizeof(ds), hipMemcpyDeviceToHost, h->stream), "copy");
-        hip_check(hipStreamSynchronize(h->stream), "sync");
-        if (J) *J = sc[0];
+        double J0 = 0.0;
+        hip_check(hipMemcpyAsync(&J0, h->scal.p, sizeof(J0), hipMemcpyDeviceToHost, h->stream), "copy");
+        const SweepDiag d = read_diag(h);
+        const double drift = led.drift();
+        if (J) *J = J0;
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->forward_steps = nsteps;
             stats->forward_ms = elapsed(h);
             stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            float umax;
-            std::memcpy(&umax, &ds[0], 4);
-            stats->max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
-            stats->bad_cells = ds[1];
-            stats->mass_drift = led.drift();
+            stats->max_mach = d.max_mach;
+            stats->bad_cells = d.bad_cells;
+            stats->min_population = d.min_population;
+            stats->mass_drift = drift;
+            stats->forward_kernel_avg_ms = NAN;
+            stats->adjoint_kernel_avg_ms = NAN;
+            stats->cotangent_max = NAN;
         }
+        admit(h, d, drift);
     });
 }
 
@@ -1837,7 +1955,17 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
                                  h->stream),
                   "copy");
         hip_check(hipStreamSynchronize(h->stream), "sync");
+        h->abar_ready = true;
         check_lam_range(h);
+    });
+}
+
+int lbm_alpha_bar(lbm_handle* h, double* alpha_bar, uint64_t n) {
+    return guarded([&] {
+        if (!h || !alpha_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "alpha_bar needs one entry per cell");
+        if (!h->abar_ready) fail(LBM_ERR_INVALID, "no gradient has been computed on this handle");
+        h->stage.download(alpha_bar, h->alpha_bar.p, size_t(n) * sizeof(double), h->stream);
     });
 }
 
@@ -1906,11 +2034,13 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         };
         double fwd_ms = 0.0, adj_ms = 0.0;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd_spans, adj_spans, fwd_k, adj_k;
-        // Every KSAMPLE-th step kernel is bracketed by events to time the kernel alone.
-        constexpr uint64_t KSAMPLE = 64;
+        // Every ksample-th step kernel (forward and adjoint counted apart) is bracketed by
+        // events to time the kernel alone: every 64th, and at least four of each kind on
+        // short horizons (each record leaves the stream idle for a few microseconds).
+        const uint64_t ksample = std::max<uint64_t>(1, std::min<uint64_t>(64, nsteps / 4));
         auto kspan = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, uint64_t count)
             -> const std::pair<hipEvent_t, hipEvent_t>* {
-            if (count % KSAMPLE != KSAMPLE / 2) return nullptr;
+            if (count % ksample != ksample / 2) return nullptr;
             hipEvent_t a, b;
             hip_check(hipEventCreate(&a), "event");
             hip_check(hipEventCreate(&b), "event");
@@ -1929,9 +2059,26 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         // (out of place), and a Snapshot is taken by writing the last step of the preceding
         // advance straight into its slot -- no whole-state copies.
         void* cur = h->F.p;
+        SweepDiag diag;
+        double drift = 0.0;
         for (size_t idx = 0; idx < sch.actions.size(); idx++) {
             const Action& act = sch.actions[idx];
             const bool forward_phase = idx < sch.reverse_start;
+            if (idx == sch.reverse_start) {  // the first sweep is complete: its diagnostics
+                diag = read_diag(h);
+                drift = led.drift();
+                mark = nullptr;
+                try {
+                    admit(h, diag, drift);
+                } catch (const Fail&) {  // refused: leave the handle at state 0, as on success
+                    hip_check(hipMemcpyAsync(h->F.p, slot_ptr(0), h->state_bytes, hipMemcpyDeviceToDevice,
+                                             h->stream),
+                              "restore");
+                    hip_check(hipStreamSynchronize(h->stream), "sync");
+                    for (auto e : evs) (void)hipEventDestroy(e);
+                    throw;
+                }
+            }
             switch (act.kind) {
             case ActKind::Snapshot:
                 if (cursor != act.a) fail(LBM_ERR_INVALID, "snapshot of a state the cursor does not hold");
@@ -1964,8 +2111,8 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                                   kspan(fwd_k, fsteps));
                         fsteps++;
                         // Objective on the final state and the cotangent seed lambda_n.
-                        StepArgs ea = h->step_args(h->A.p, base + nsteps);
-                        ea.part = h->part.as<double>();
+                        StepArgs ea = final_state_args(h, h->A.p, base + nsteps);
+                        ea.diag = forward_phase ? 1 : 0;
                         ea.obj = forward_phase ? stage_obj(tmpl, obj, nsteps, nsteps, true) : none;
                         ea.macc = led.mass(nsteps);
                         const ObjArgs seed = stage_obj(tmpl, obj, nsteps, nsteps, true);
@@ -2041,21 +2188,15 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         }
         // F holds some intermediate state; restore state 0 from slot 0 (taken first).
         hip_check(hipMemcpyAsync(h->F.p, slot_ptr(0), h->state_bytes, hipMemcpyDeviceToDevice, h->stream), "restore");
-        double sc[4];
-        unsigned ds[2];
-        hip_check(hipMemcpyAsync(sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, h->stream), "copy");
-        hip_check(hipMemcpyAsync(ds, h->scal.as<double>() + 4, sizeof(ds), hipMemcpyDeviceToHost, h->stream), "copy");
-        if (alpha_bar)
-            hip_check(hipMemcpyAsync(alpha_bar, h->alpha_bar.p, size_t(h->ncell) * sizeof(double),
-                                     hipMemcpyDeviceToHost, h->stream),
-                      "copy");
+        double J0 = 0.0;
+        hip_check(hipMemcpyAsync(&J0, h->scal.p, sizeof(J0), hipMemcpyDeviceToHost, h->stream), "copy");
+        if (alpha_bar) h->stage.download(alpha_bar, h->alpha_bar.p, size_t(h->ncell) * sizeof(double), h->stream);
         if (lambda0) {
             ensure_aos(h);
             hip_check(h->run->get_lam(Q, h->has_mask, h->g, h->L.p, h->mask.as<uint32_t>(), int(base & 1),
                                       h->aos.as<double>(), h->lsc_ptr(), h->cfg()),
                       "get_lam");
-            hip_check(hipMemcpyAsync(lambda0, h->aos.p, lambda_len * sizeof(double), hipMemcpyDeviceToHost, h->stream),
-                      "copy");
+            h->stage.download(lambda0, h->aos.p, lambda_len * sizeof(double), h->stream);
         }
         hip_check(hipStreamSynchronize(h->stream), "sync");
         for (auto& sp : fwd_spans) {
@@ -2080,7 +2221,8 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         const double fwd_kavg = kavg(fwd_k), adj_kavg = kavg(adj_k);
         for (auto e : evs) (void)hipEventDestroy(e);
         h->t = base;
-        if (J) *J = sc[0];
+        h->abar_ready = true;
+        if (J) *J = J0;
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->forward_steps = fsteps;
@@ -2091,11 +2233,10 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
             stats->forward_ms = fwd_ms;
             stats->adjoint_ms = adj_ms;
             stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            float umax;
-            std::memcpy(&umax, &ds[0], 4);
-            stats->max_mach = std::sqrt(3.0) * std::sqrt(double(umax));
-            stats->bad_cells = ds[1];
-            stats->mass_drift = led.drift();
+            stats->max_mach = diag.max_mach;
+            stats->bad_cells = diag.bad_cells;
+            stats->min_population = diag.min_population;
+            stats->mass_drift = drift;
             stats->forward_kernel_avg_ms = fwd_kavg;
             stats->adjoint_kernel_avg_ms = adj_kavg;
             stats->kernel_samples = fwd_k.size() + adj_k.size();

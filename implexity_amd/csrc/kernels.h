@@ -125,14 +125,15 @@ struct StepArgs {
     const void* sm1;
     double theta;     // weight of the second endpoint in this step
     ObjArgs obj;      // objective of the state being read (kind 0: none)
-    double* part;     // per-block partials [J, mass, max |u|^2, bad cells], 4 per block
+    double* part;     // per-block partials [J, mass, max |u|^2, bad cells, max -f], PART_STRIDE per block
     long long npart;  // capacity of part, in blocks
-    double* stage;    // FOLD_BLOCKS x 4 fold stage and its ticket counter (k_fold)
+    double* stage;    // FOLD_MAX_BLOCKS x FOLD_VALUES fold stage and its ticket counter (k_fold)
     unsigned* ticket;
     int* lsc;         // scale record of FP16 cotangents (seed writes), else null
     double* jacc;     // objective accumulator      (written by k_fold)
     double* macc;     // fluid mass of the state    (written by k_fold)
-    unsigned* dstat;  // [0] max |u|^2 (float bits), [1] bad cells (non-finite or f <= 0)
+    unsigned* dstat;  // [0] max |u|^2 (float bits), [1] bad cells (non-finite or f <= 0),
+                      // [2] float_key of max -f over fluid populations (0: none seen)
 };
 
 struct AdjArgs {
@@ -611,14 +612,18 @@ LBM_DEV long long block_linear() {
     return (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * blockIdx.z);
 }
 
-// Per-block partials of the objective, the fluid mass, max |u|^2 and the count of bad
-// cells, written without atomics (a single address hit by every block serialises in
-// L2); k_fold combines them, in a fixed order, after the step.
-LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* part) {
-    __shared__ double red[4][16];
+// Per-block partials of the objective, the fluid mass, max |u|^2, the count of bad
+// cells and the largest negated population (-min f over the block's fluid cells),
+// PART_STRIDE doubles per block, written without atomics (a single address hit by every
+// block serialises in L2); k_fold combines them, in a fixed order, after the step.
+constexpr int PART_STRIDE = 8;
+
+LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, float fneg, double* part) {
+    __shared__ double red[5][16];
     jv = wave_sum(jv);
     mv = wave_sum(mv);
     umax = wave_max(umax);
+    fneg = wave_max(fneg);
     const double nbad = double(__popcll(__ballot(bad)));
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;  // blockDim.x is a multiple of 64
     const int lane = tid & 63, wid = tid >> 6;
@@ -627,23 +632,33 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, double* 
         red[1][wid] = mv;
         red[2][wid] = double(umax);
         red[3][wid] = nbad;
+        red[4][wid] = double(fneg);
     }
     __syncthreads();
     if (tid == 0) {
         const int nw = int(blockDim.x * blockDim.y + 63) >> 6;
-        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = -INFINITY;
         for (int w = 0; w < nw; w++) {
             a += red[0][w];
             b += red[1][w];
             c = fmax(c, red[2][w]);
             d += red[3][w];
+            e = fmax(e, red[4][w]);
         }
-        double* o = part + 4 * block_linear();
+        double* o = part + PART_STRIDE * block_linear();
         o[0] = a;
         o[1] = b;
         o[2] = c;
         o[3] = d;
+        o[4] = e;
     }
+}
+
+// Order-preserving key of a float as an unsigned integer (0 sorts below every float),
+// so that a running max of signed values can be held in a zero-initialised word.
+LBM_DEV unsigned float_key(float v) {
+    const unsigned u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
 
@@ -682,15 +697,17 @@ __global__ __launch_bounds__(64) void k_lam_update(int* lsc, double s_next) {
     }
 }
 
-// Folds the per-block partials of one launch (part[4 b + 0..3] = objective, mass,
-// max, bad-cell count) in two fixed-order stages: each fold block reduces 256 partials
-// per pass (one 32-byte load per thread) into stage[4 block ..]; the last fold block to
+// Folds the per-block partials of one launch (part[PART_STRIDE b + 0..4] = objective,
+// mass, max |u|^2, bad-cell count, max -f) in two fixed-order stages: each fold block
+// reduces 256 partials per pass into stage[FOLD_VALUES block ..]; the last fold block to
 // finish (a ticket counter, released with device-scope fences) loads the stage values
 // one per thread, reduces them in the same fixed tree order, writes the results and
 // resets the ticket.  Deterministic for a given launch shape.
 //   FOLD_STEP:   jacc += objective; *macc = mass (if set); dstat[0] = max(dstat[0],
-//                max |u|^2) and dstat[1] += bad (if set).
+//                max |u|^2), dstat[1] += bad and dstat[2] = max(dstat[2], float_key(max -f))
+//                (if set).
 constexpr int FOLD_MAX_BLOCKS = 256;
+constexpr int FOLD_VALUES = 5;
 enum FoldMode { FOLD_STEP = 0 };
 
 LBM_HD inline int fold_blocks(long long nb) {
@@ -698,29 +715,36 @@ LBM_HD inline int fold_blocks(long long nb) {
     return int(g < 1 ? 1 : (g > FOLD_MAX_BLOCKS ? FOLD_MAX_BLOCKS : g));
 }
 
-// Block-wide fixed-order reduction of (sum a, sum b, max c, sum d); result in thread 0.
-LBM_DEV void block_fold4(double& a, double& b, double& c, double& d) {
-    __shared__ double red[4][4];
+// Block-wide fixed-order reduction of (sum a, sum b, max c, sum d, max e); result in
+// thread 0.
+LBM_DEV void block_fold5(double& a, double& b, double& c, double& d, double& e) {
+    __shared__ double red[5][4];
     a = wave_sum(a);
     b = wave_sum(b);
     d = wave_sum(d);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) c = fmax(c, __shfl_down(c, off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+        c = fmax(c, __shfl_down(c, off, 64));
+        e = fmax(e, __shfl_down(e, off, 64));
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         red[0][wid] = a;
         red[1][wid] = b;
         red[2][wid] = c;
         red[3][wid] = d;
+        red[4][wid] = e;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         a = b = c = d = 0.0;
+        e = -INFINITY;
         for (int w = 0; w < int(blockDim.x >> 6); w++) {
             a += red[0][w];
             b += red[1][w];
             c = fmax(c, red[2][w]);
             d += red[3][w];
+            e = fmax(e, red[4][w]);
         }
     }
 }
@@ -729,18 +753,24 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, double* stage, unsigned* ticket,
                                              double* jacc, double* macc, unsigned* dstat) {
     __shared__ bool last;
-    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = -INFINITY;
     const long long step = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += step) {
-        const double4 v = *reinterpret_cast<const double4*>(part + 4 * i);
+        const double4 v = *reinterpret_cast<const double4*>(part + PART_STRIDE * i);
         a += v.x;
         b += v.y;
         c = fmax(c, v.z);
         d += v.w;
+        e = fmax(e, part[PART_STRIDE * i + 4]);
     }
-    block_fold4(a, b, c, d);
+    block_fold5(a, b, c, d, e);
     if (threadIdx.x == 0) {
-        *reinterpret_cast<double4*>(stage + 4 * blockIdx.x) = double4{a, b, c, d};
+        double* so = stage + FOLD_VALUES * blockIdx.x;
+        so[0] = a;
+        so[1] = b;
+        so[2] = c;
+        so[3] = d;
+        so[4] = e;
         __threadfence();
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
@@ -748,14 +778,16 @@ __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, 
     if (!last) return;
     __threadfence();
     a = b = c = d = 0.0;
+    e = -INFINITY;
     if (threadIdx.x < gridDim.x) {
-        const double* sp = stage + 4 * threadIdx.x;
+        const double* sp = stage + FOLD_VALUES * threadIdx.x;
         a = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         b = __hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         c = __hip_atomic_load(sp + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         d = __hip_atomic_load(sp + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        e = __hip_atomic_load(sp + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    block_fold4(a, b, c, d);
+    block_fold5(a, b, c, d, e);
     if (threadIdx.x != 0) return;
     if (MODE == FOLD_STEP) {
         if (jacc) *jacc += a;
@@ -764,6 +796,10 @@ __global__ __launch_bounds__(256) void k_fold(const double* part, long long nb, 
             const unsigned ub = __float_as_uint(float(c));
             if (ub > dstat[0]) dstat[0] = ub;
             dstat[1] += unsigned(d);
+            if (e > -INFINITY) {
+                const unsigned ke = float_key(float(e));
+                if (ke > dstat[2]) dstat[2] = ke;
+            }
         }
     }
     *ticket = 0u;
@@ -994,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     const bool active = k < g.nz && j < g.ny;
     double jsum = 0.0, msum = 0.0;
-    float umax = 0.0f;
+    float umax = 0.0f, fneg = -INFINITY;
     unsigned bad = 0;
     if (active) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
@@ -1061,6 +1097,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                 for (int q = 0; q < Q; q++) {
                     const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
                     bad |= !(fq > C(0)) || !isfinite(float(fq));
+                    fneg = fmaxf(fneg, -float(fq));
                 }
             }
             if constexpr (TH) {
@@ -1096,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             });
         }
     }
-    if (A.part) block_partials(jsum, msum, umax, bad != 0, A.part);
+    if (A.part) block_partials(jsum, msum, umax, bad != 0, fneg, A.part);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1350,266 +1387,6 @@ __global__ __launch_bounds__(256) void k_wale_transpose(Geom g, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------------
-// Vectorised steps: each thread updates V consecutive cells along z with V-wide loads
-// and stores per population array (the esoteric-twist +z offsets make some of them
-// one element off alignment; gfx950 serves unaligned dword vector accesses).  Links
-// blocked by a wall are patched per cell with scalar loads (reads) or stores
-// (adjoint writes).  Chunks that contain a solid cell take the per-cell path, since
-// a vector store would overwrite the bounce-back slots that solid cells lend their
-// neighbours, and so does the last chunk of a row (its +z slots are in the ghost
-// region).  Requires nz % V == 0 and a non-periodic z axis.
-// ---------------------------------------------------------------------------------
-template <class T, int V>
-struct VecOf {
-    typedef T type __attribute__((ext_vector_type(V), aligned(sizeof(T))));
-};
-
-struct Chunk {
-    int i, j, k0;
-    long long cell0;
-};
-
-template <int V>
-LBM_DEV bool chunk_of(const Geom& g, Chunk& c) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int per_row = g.nz / V;
-    const long long row = t / per_row;
-    if (row >= (long long)g.nx * g.ny) return false;
-    c.k0 = int(t - row * per_row) * V;
-    c.j = int(row % g.ny);
-    c.i = int(row / g.ny);
-    c.cell0 = row * g.nz + c.k0;
-    return true;
-}
-
-template <int Q, bool MASK, int V>
-LBM_DEV bool chunk_masks(const Geom& g, const uint32_t* mask, const Chunk& c, uint32_t* m) {
-    bool solid = false;
-    if constexpr (MASK) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            m[v] = mask[c.cell0 + v];
-            solid |= (m[v] & SOLID_BIT) != 0;
-        }
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; v++) m[v] = face_mask<Q>(g, c.i, c.j, c.k0 + v);
-    }
-    return solid;
-}
-
-// Vector read of f at the read locations of V cells (parity odd), patched per cell for
-// blocked links.
-template <int P, int Q, int V>
-LBM_DEV void load_chunk(const Geom& g, const typename Store<P>::T* pop, const Cursor& cu, const uint32_t* m,
-                        int odd, typename VecOf<typename Store<P>::T, V>::type* vin) {
-    using VT = typename VecOf<typename Store<P>::T, V>::type;
-    uint32_t any = 0;
-#pragma unroll
-    for (int v = 0; v < V; v++) any |= m[v];
-    static_for<0, Q>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        const long long pos = cu.at(corner_neg<Q>(q));
-        const int arr_n = odd ? q : opp<Q>(q);
-        vin[q] = *reinterpret_cast<const VT*>(pop + field_index<Q>(g, arr_n, pos));
-        if ((any >> q) & 1u) {
-            const int arr_b = odd ? opp<Q>(q) : q;
-#pragma unroll
-            for (int v = 0; v < V; v++)
-                if ((m[v] >> q) & 1u) vin[q][v] = pop[field_index<Q>(g, arr_b, pos + v)];
-        }
-    });
-}
-
-template <int P, int Q, int KIND, bool MASK, int V>
-__global__ __launch_bounds__(128) void k_forward_v(StepArgs A) {
-    using S = Store<P>;
-    using C = typename S::C;
-    using T = typename S::T;
-    using VT = typename VecOf<T, V>::type;
-    const Geom& g = A.g;
-    Chunk ch;
-    double jsum = 0.0, msum = 0.0;
-    float umax = 0.0f;
-    unsigned bad = 0;
-    if (chunk_of<V>(g, ch)) {
-        uint32_t m[V];
-        // the last chunk of a row on a non-periodic z axis reaches the ghost region
-        const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m) || (!g.pz && ch.k0 + V == g.nz);
-        T* pop = static_cast<T*>(A.pop);
-        const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
-        const C* alpha_p = static_cast<const C*>(A.alpha);
-        if (!has_solid) {
-            const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0);
-            VT vin[Q];
-            load_chunk<P, Q, V>(g, pop, cu, m, A.odd, vin);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                C f[Q], fs[Q];
-                static_for<0, Q>([&](auto I) {
-                    constexpr int q = decltype(I)::value;
-                    f[q] = S::template dec<Q, q>(vin[q][v]);
-                });
-                const C alpha = alpha_p[ch.cell0 + v];
-                Macro<Q, C> mac;
-                collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                                static_cast<const C*>(A.mrt), fs, mac);
-                if (A.obj.kind != OBJ_NONE) {
-                    const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
-                    jsum += objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
-                }
-                if (A.diag) {
-                    msum += double(mac.rho);
-                    umax = fmaxf(umax, float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]));
-#pragma unroll
-                    for (int q = 0; q < Q; q++) {
-                        const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
-                        bad |= !(fq > C(0)) || !isfinite(float(fq));
-                    }
-                }
-                static_for<0, Q>([&](auto I) {
-                    constexpr int q = decltype(I)::value;
-                    vin[q][v] = S::template enc<Q, q>(fs[q]);
-                });
-            }
-            T* out = static_cast<T*>(A.pop_out);
-            static_for<0, Q>([&](auto I) {
-                constexpr int q = decltype(I)::value;
-                *reinterpret_cast<VT*>(out + write_loc<Q, q>(g, cu, A.odd)) = vin[q];
-            });
-        } else {
-            for (int v = 0; v < V; v++) {
-                if (m[v] & SOLID_BIT) continue;
-                const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0 + v);
-                C f[Q], fs[Q];
-                load_cell<P, Q>(g, pop, cu, m[v], A.odd, f);
-                const C alpha = alpha_p[ch.cell0 + v];
-                Macro<Q, C> mac;
-                collide<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                                static_cast<const C*>(A.mrt), fs, mac);
-                if (A.obj.kind != OBJ_NONE) {
-                    const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
-                    jsum += objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
-                }
-                if (A.diag) {
-                    msum += double(mac.rho);
-                    umax = fmaxf(umax, float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]));
-                }
-                T* out = static_cast<T*>(A.pop_out);
-                static_for<0, Q>([&](auto I) {
-                    constexpr int q = decltype(I)::value;
-                    S::template st<Q, q>(out + write_loc<Q, q>(g, cu, A.odd), fs[q]);
-                });
-            }
-        }
-    }
-    if (A.part) block_partials(jsum, msum, umax, bad != 0, A.part);
-}
-
-template <int P, int LP, int Q, int KIND, bool MASK, int V>
-__global__ __launch_bounds__(128) void k_adjoint_v(AdjArgs A) {
-    static_assert(LP != P_F16S, "scaled FP16 cotangents take the per-cell adjoint kernel");
-    using S = Store<P>;
-    using L = LamStore<LP>;
-    using C = typename S::C;
-    using T = typename S::T;
-    using LT = typename L::T;
-    using VT = typename VecOf<T, V>::type;
-    using VL = typename VecOf<LT, V>::type;
-    const Geom& g = A.g;
-    Chunk ch;
-    if (!chunk_of<V>(g, ch)) return;
-    uint32_t m[V];
-    const bool has_solid = chunk_masks<Q, MASK, V>(g, A.mask, ch, m) || (!g.pz && ch.k0 + V == g.nz);
-    const T* pop = static_cast<const T*>(A.pop);
-    LT* lam = static_cast<LT*>(A.lam);
-    const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
-    const C* alpha_p = static_cast<const C*>(A.alpha);
-    if (!has_solid) {
-        const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0);
-        uint32_t any = 0;
-#pragma unroll
-        for (int v = 0; v < V; v++) any |= m[v];
-        VT vin[Q];
-        VL vl[Q];
-        load_chunk<P, Q, V>(g, pop, cu, m, A.odd, vin);
-        static_for<0, Q>([&](auto I) {
-            constexpr int q = decltype(I)::value;
-            vl[q] = *reinterpret_cast<const VL*>(lam + write_loc<Q, q>(g, cu, A.odd));
-        });
-        double abv[V];
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            C f[Q], gb[Q], fb[Q];
-            static_for<0, Q>([&](auto I) {
-                constexpr int q = decltype(I)::value;
-                f[q] = S::template dec<Q, q>(vin[q][v]);
-                gb[q] = C(vl[q][v]);
-            });
-            const C alpha = alpha_p[ch.cell0 + v];
-            Macro<Q, C> mac;
-            C ab;
-            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
-            if (A.obj.kind != OBJ_NONE) {
-                const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
-                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
-            }
-            abv[v] = double(ab);
-            static_for<0, Q>([&](auto I) {
-                constexpr int q = decltype(I)::value;
-                vl[q][v] = LT(fb[q]);
-            });
-        }
-        typedef double VD __attribute__((ext_vector_type(V), aligned(8)));
-        VD* abp = reinterpret_cast<VD*>(A.alpha_bar + ch.cell0);
-        VD acc = *abp;
-#pragma unroll
-        for (int v = 0; v < V; v++) acc[v] += abv[v];
-        *abp = acc;
-        // Cotangents go to the read locations: vector store at the unblocked ones,
-        // then the blocked links of each cell to their bounce-back slots.
-        static_for<0, Q>([&](auto I) {
-            constexpr int q = decltype(I)::value;
-            const long long pos = cu.at(corner_neg<Q>(q));
-            const int arr_n = A.odd ? q : opp<Q>(q);
-            *reinterpret_cast<VL*>(lam + field_index<Q>(g, arr_n, pos)) = vl[q];
-            if ((any >> q) & 1u) {
-                const int arr_b = A.odd ? opp<Q>(q) : q;
-#pragma unroll
-                for (int v = 0; v < V; v++)
-                    if ((m[v] >> q) & 1u) lam[field_index<Q>(g, arr_b, pos + v)] = vl[q][v];
-            }
-        });
-    } else {
-        for (int v = 0; v < V; v++) {
-            if (m[v] & SOLID_BIT) continue;
-            const Cursor cu = make_cursor(g, ch.i, ch.j, ch.k0 + v);
-            C f[Q], gb[Q], fb[Q];
-            load_cell<P, Q>(g, pop, cu, m[v], A.odd, f);
-            static_for<0, Q>([&](auto I) {
-                constexpr int q = decltype(I)::value;
-                gb[q] = L::template ld<C>(lam + write_loc<Q, q>(g, cu, A.odd));
-            });
-            const C alpha = alpha_p[ch.cell0 + v];
-            Macro<Q, C> mac;
-            C ab;
-            collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
-                                                static_cast<const C*>(A.mrt), gb, mac, fb, ab);
-            if (A.obj.kind != OBJ_NONE) {
-                const int coord = A.obj.axis == 0 ? ch.i : (A.obj.axis == 1 ? ch.j : ch.k0 + v);
-                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
-            }
-            A.alpha_bar[ch.cell0 + v] += double(ab);
-            static_for<0, Q>([&](auto I) {
-                constexpr int q = decltype(I)::value;
-                L::template st<C>(lam + read_loc<Q, q>(g, cu, m[v], A.odd), fb[q]);
-            });
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------
 // Objective on one state: accumulate its value (eval) and/or write the cotangent seed
 // lambda = d J / d state at every fluid cell and alpha_bar = d J / d alpha (seed).
 // ---------------------------------------------------------------------------------
@@ -1623,6 +1400,8 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     double jsum = 0.0, msum = 0.0;
+    float umax = 0.0f, fneg = -INFINITY;
+    bool bad = false;
     if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
@@ -1639,6 +1418,15 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
                 jsum = objective_cell<S::shifted, Q, C>(A.obj, f, mac, alpha, coord);
             }
             msum = double(mac.rho);
+            if (A.diag) {  // the admission quantities of this state (as the step kernel's)
+                umax = float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]);
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
+                    bad |= !(fq > C(0)) || !isfinite(float(fq));
+                    fneg = fmaxf(fneg, -float(fq));
+                }
+            }
             if (do_seed) {
                 C fb[Q];
                 C ab = C(0);
@@ -1660,7 +1448,7 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
             alpha_bar[cell] = 0.0;
         }
     }
-    if (A.part) block_partials(jsum, msum, 0.0f, false, A.part);
+    if (A.part) block_partials(jsum, msum, umax, bad, fneg, A.part);
 }
 
 // ---------------------------------------------------------------------------------

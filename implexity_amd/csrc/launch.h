@@ -5,8 +5,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <cstdlib>
-
 #include "kernels.h"
 #include "thermal.h"
 
@@ -77,47 +75,6 @@ const Runner* runner_bf16s();
 // and the streaming-only debug operator; D3Q27 with BGK, TRT, MRT and cumulant.
 template <int P, int LP, bool ALL_KINDS>
 struct RunnerImpl final : Runner {
-    // Cells per thread, chosen from 256^3 D3Q19 TRT measurements: one (the two-cell
-    // kernels must take the per-cell path for the last chunk of every row, whose +z slots
-    // are in the ghost region, which costs more than the wider accesses gain).
-    // LBM_FWD_CELLS / LBM_ADJ_CELLS = 1 | 2 override them.
-    static int width(const char* env, int dflt) {
-        const char* e = std::getenv(env);
-        const int v = e ? std::atoi(e) : dflt;
-        return v == 2 ? 2 : 1;
-    }
-    static int fwd_width() {
-        static const int w = width("LBM_FWD_CELLS", 1);
-        return w;
-    }
-    static int adj_width() {
-        static const int w = width("LBM_ADJ_CELLS", 1);
-        return w;
-    }
-    static LaunchCfg vec_cfg(const Geom& g, int V, hipStream_t s) {
-        const long long chunks = (long long)g.nx * g.ny * (g.nz / V);
-        return LaunchCfg{dim3(unsigned((chunks + 127) / 128)), dim3(128), s};
-    }
-    // Vector paths (V cells per thread along z) exist for the two-relaxation kinds and the
-    // streaming-only operator; MRT, cumulant and BGK-Guo take the per-cell kernels (their
-    // register footprint at V = 2 spills).
-    template <int Q, bool M, int V>
-    static hipError_t fwd_vec(int kind, const StepArgs& a, const LaunchCfg& c) {
-        switch (kind) {
-        case K_TRT: LBM_LAUNCH((k_forward_v<P, Q, K_TRT, M, V>), c, a);
-        case K_BGK: LBM_LAUNCH((k_forward_v<P, Q, K_BGK, M, V>), c, a);
-        case K_STREAM_ONLY: LBM_LAUNCH((k_forward_v<P, Q, K_STREAM_ONLY, M, V>), c, a);
-        default: return hipErrorInvalidValue;
-        }
-    }
-    template <int Q, bool M, int V>
-    static hipError_t adj_vec(int kind, const AdjArgs& a, const LaunchCfg& c) {
-        switch (kind) {
-        case K_TRT: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_TRT, M, V>), c, a);
-        case K_BGK: LBM_LAUNCH((k_adjoint_v<P, LP, Q, K_BGK, M, V>), c, a);
-        default: return hipErrorInvalidValue;
-        }
-    }
     // Flow + coupled temperature field in one pass (no solid mask, per-cell kernels,
     // storages the temperature field supports).
     template <int Q>
@@ -141,22 +98,9 @@ struct RunnerImpl final : Runner {
         return hipErrorInvalidValue;
     }
     template <int Q, bool M>
-    static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c0, LaunchCfg& used) {
-        if (a.tpop) {
-            used = c0;
-            return M ? hipErrorInvalidValue : fwd_th<Q>(kind, a, c0);
-        }
-        const int V = fwd_width();
-        if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
-            const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
-            if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
-            used = c;
-            const hipError_t e = fwd_vec<Q, M, 2>(kind, a, c);
-            if (e != hipErrorInvalidValue) return e;
-        }
-        const LaunchCfg& c = c0;
+    static hipError_t fwd_q(int kind, const StepArgs& a, const LaunchCfg& c) {
+        if (a.tpop) return M ? hipErrorInvalidValue : fwd_th<Q>(kind, a, c);
         if (a.part && blocks(c) > a.npart) return hipErrorInvalidValue;
-        used = c0;
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_forward<P, Q, K_TRT, M>), c, a);
         case K_BGK: LBM_LAUNCH((k_forward<P, Q, K_BGK, M>), c, a);
@@ -205,17 +149,8 @@ struct RunnerImpl final : Runner {
         return hipErrorInvalidValue;
     }
     template <int Q, bool M>
-    static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c0) {
-        if (a.ubar) return M ? hipErrorInvalidValue : adj_ub<Q>(kind, a, c0);
-        if constexpr (LP != P_F16S) {  // scaled FP16 cotangents: per-cell kernel only
-            const int V = adj_width();
-            if (V > 1 && a.g.nz % V == 0 && !a.g.pz) {
-                const LaunchCfg c = vec_cfg(a.g, V, c0.stream);
-                const hipError_t e = adj_vec<Q, M, 2>(kind, a, c);
-                if (e != hipErrorInvalidValue) return e;
-            }
-        }
-        const LaunchCfg& c = c0;
+    static hipError_t adj_q(int kind, const AdjArgs& a, const LaunchCfg& c) {
+        if (a.ubar) return M ? hipErrorInvalidValue : adj_ub<Q>(kind, a, c);
         switch (kind) {
         case K_TRT: LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT, M>), c, a);
         case K_BGK: LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK, M>), c, a);
@@ -287,15 +222,14 @@ struct RunnerImpl final : Runner {
     hipError_t forward(int Q, int kind, bool mask, const StepArgs& a, const LaunchCfg& c,
                        hipEvent_t after) const override {
         if (!ok_step(kind, mask, a)) return hipErrorInvalidValue;
-        LaunchCfg used = c;
         hipError_t e;
         if (Q == 19)
-            e = mask ? fwd_q<19, true>(kind, a, c, used) : fwd_q<19, false>(kind, a, c, used);
+            e = mask ? fwd_q<19, true>(kind, a, c) : fwd_q<19, false>(kind, a, c);
         else
-            e = mask ? fwd_q<27, true>(kind, a, c, used) : fwd_q<27, false>(kind, a, c, used);
+            e = mask ? fwd_q<27, true>(kind, a, c) : fwd_q<27, false>(kind, a, c);
         if (e == hipSuccess && after) e = hipEventRecord(after, c.stream);
         if (e != hipSuccess || !a.part) return e;
-        return reduce(a, used, a.diag != 0, a.diag != 0);
+        return reduce(a, c, a.diag != 0, a.diag != 0);
     }
     hipError_t adjoint(int Q, int kind, bool mask, const AdjArgs& a, const LaunchCfg& c,
                        hipEvent_t after) const override {
@@ -312,7 +246,7 @@ struct RunnerImpl final : Runner {
     }
     hipError_t objective(int Q, bool mask, const StepArgs& a, const ObjArgs& seed, void* lam, double* abar,
                          int do_seed, const LaunchCfg& c) const override {
-        if (!a.pop || !a.alpha || !a.part || blocks(c) > a.npart || (mask && !a.mask) ||
+        if (!a.pop || !a.alpha || !a.part || blocks(c) > a.npart || (mask && !a.mask) || (a.diag && !a.dstat) ||
             (do_seed && (!lam || !abar || (LP == P_F16S && !a.lsc))))
             return hipErrorInvalidValue;
         if (Q == 19) {
@@ -332,7 +266,7 @@ struct RunnerImpl final : Runner {
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        return reduce(a, c, true, false);
+        return reduce(a, c, true, a.diag != 0);
     }
     static bool ok_sponge(const SpongeArgs& a) {
         return a.n > 0 && a.ns > 0 && a.ns <= MAX_SPONGES && a.cell && a.sigma && a.off && a.part && a.weight &&

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_thermal_seed(ThermalArgs A, double seed
             });
         }
     }
-    if (A.part) block_partials(jsum, 0.0, 0.0f, false, A.part);
+    if (A.part) block_partials(jsum, 0.0, 0.0f, false, -INFINITY, A.part);
 }
 
 // Port cells after streaming (state of parity A.odd).

@@ -107,6 +107,7 @@ class _Config(ctypes.Structure):
         ("n_sponges", ctypes.c_uint32), ("reserved3", ctypes.c_uint32), ("sponges", ctypes.c_void_p),
         ("coupling", ctypes.c_uint32), ("reserved4", ctypes.c_uint32), ("psm_kappa", ctypes.c_double),
         ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
+        ("mach_limit", ctypes.c_double), ("drift_limit", ctypes.c_double),
     ]
 
 
@@ -139,6 +140,7 @@ class Stats(ctypes.Structure):
         ("bad_cells", ctypes.c_uint64), ("forward_kernel_avg_ms", ctypes.c_double),
         ("adjoint_kernel_avg_ms", ctypes.c_double), ("kernel_samples", ctypes.c_uint64),
         ("cotangent_max", ctypes.c_double), ("cotangent_exponent", ctypes.c_int64),
+        ("min_population", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -188,6 +190,7 @@ _SIGS = {
     "lbm_step_vjp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                      ctypes.c_int),
     "lbm_reserve": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
+    "lbm_alpha_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_get_fields": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_debug_link_table": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_port_cells": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
@@ -320,6 +323,12 @@ class LatticeConfig:
     psm_kappa: float = 0.0
     psm_c_alpha: float = 0.0
     psm_drag_shape: float = 0.1
+    # Admission of the first sweep's trajectory (LbmProblem::admit, solver.rs:707-793):
+    # 0 = none; otherwise history/gradient raise ConvergenceError on a non-finite or
+    # non-positive fluid population, Mach above mach_limit (<= 0.2) or a relative mass
+    # drift above drift_limit (0: 1e-6)
+    mach_limit: float = 0.0
+    drift_limit: float = 0.0
 
 
 @dataclass
@@ -430,6 +439,8 @@ class Lattice:
         c.psm_kappa = float(cfg.psm_kappa)
         c.psm_c_alpha = float(cfg.psm_c_alpha)
         c.psm_drag_shape = float(cfg.psm_drag_shape)
+        c.mach_limit = float(cfg.mach_limit)
+        c.drift_limit = float(cfg.drift_limit)
         c.n_sponges = len(cfg.sponges)
         c.sponges = ctypes.cast(sponges, ctypes.c_void_p)
         solid = None
@@ -555,21 +566,30 @@ class Lattice:
         _check(lib().lbm_reserve(self._h, steps, ctypes.byref(slots)))
         return slots.value
 
-    def gradient(self, steps, objective: Objective | None, want_lambda0=False, lambda_final=None):
+    def gradient(self, steps, objective: Objective | None, want_lambda0=False, lambda_final=None,
+                 alpha_bar_on_device=False):
         """Forward + discrete adjoint from the current state (binomial checkpointing).
         lambda_final (reference AoS, optional) is added to the final state's cotangent, as
-        history_adjoint's lambda_final (solver.rs:798-815).
+        history_adjoint's lambda_final (solver.rs:798-815).  With alpha_bar_on_device the
+        gradient stays in device memory (alpha_bar is returned as None; fetch it with
+        ``alpha_bar()``).
         Returns (J, alpha_bar, lambda0 or None, stats)."""
         J = ctypes.c_double(0.0)
         st = Stats()
-        ab = np.zeros(self.n)
+        ab = None if alpha_bar_on_device else np.empty(self.n)
         lam = np.zeros(self.n * self.Q) if want_lambda0 else None
         o = objective._c() if objective is not None else None
         lf = None if lambda_final is None else np.ascontiguousarray(lambda_final, np.float64).reshape(-1)
         _check(lib().lbm_gradient_seeded(self._h, steps, ctypes.byref(o) if o is not None else None, _p(lf),
-                                         0 if lf is None else lf.size, ctypes.byref(J), _p(ab), ab.size, _p(lam),
+                                         0 if lf is None else lf.size, ctypes.byref(J), _p(ab), self.n, _p(lam),
                                          0 if lam is None else lam.size, ctypes.byref(st)))
         return J.value, ab, lam, st
+
+    def alpha_bar(self):
+        """dJ/dalpha of the last gradient or step VJP, copied out of device memory."""
+        ab = np.empty(self.n)
+        _check(lib().lbm_alpha_bar(self._h, _p(ab), self.n))
+        return ab
 
     def vector(self, scales, phi=None):
         """Terminal responses of the current state (solver.rs:874-894)."""

@@ -17,6 +17,8 @@
  *                         checkpointing                                solver.rs:797-815,
  *                         crates/implexity-ad/src/scan.rs:140-390, revolve.rs:128-388
  *   lbm_gradient_seeded   history_adjoint(history, alpha, tau, lambda_final) solver.rs:798-815
+ *                         (both with the admission of LbmProblem::admit when mach_limit > 0,
+ *                         solver.rs:707-793)
  *   lbm_step_vjp          LbmProblem::interval_vjp                    solver.rs:641-674
  *   lbm_vector            LbmProblem::vector (terminal responses)      solver.rs:874-894
  *   lbm_vector_vjp        LbmProblem::vector_vjp                      solver.rs:896-947
@@ -46,7 +48,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 4
+#define LBM_ABI_VERSION 5
 
 /* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
  * crates/implexity-ad/src/error.rs). */
@@ -220,6 +222,17 @@ typedef struct {
     double psm_kappa;             /* saturation width in [0, 1) (occupancy couplings) */
     double psm_c_alpha;           /* BRINKMAN_OCCUPANCY: drag coefficient c_alpha >= 0 */
     double psm_drag_shape;        /* BRINKMAN_OCCUPANCY: drag shape q > 0 */
+    /* Admission of the trajectory of a sweep (LbmProblem::admit, solver.rs:707-793; the
+     * moving solver's check_state, moving/field.rs:1735-1775).  With mach_limit > 0,
+     * lbm_forward and lbm_gradient(_seeded) refuse the states of their first sweep
+     * (states 0..n) with LBM_ERR_CONVERGENCE when a fluid population is non-finite or
+     * <= 0, when the Mach number sqrt(3) max |u| exceeds mach_limit (at most 0.2,
+     * solver.rs:434-437), or when the relative fluid-mass drift, port and sponge exchange
+     * included, exceeds drift_limit (0: 1e-6, the reference's bound, solver.rs:765).  The
+     * gradient refuses before its reverse sweep.  mach_limit = 0: no refusal (lbm_stats
+     * still reports the quantities). */
+    double mach_limit;            /* 0, or in (0, 0.2] */
+    double drift_limit;           /* >= 0; 0: 1e-6 */
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)
@@ -279,6 +292,8 @@ typedef struct {
     uint64_t kernel_samples;      /* steps timed for the two averages (gradient only) */
     double cotangent_max;         /* FP16 cotangents: max |lambda| of the last adjoint step (else NaN) */
     int64_t cotangent_exponent;   /* FP16 cotangents: exponent e of the stored lambda * 2^e (else 0) */
+    double min_population;        /* min over the fluid populations f (not f - w) of the states of
+                                     the first sweep (admit, solver.rs:716-727); NaN if none */
 } lbm_stats;
 
 typedef struct lbm_handle lbm_handle;
@@ -319,10 +334,18 @@ int lbm_forward(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double
 
 /* Forward + discrete adjoint from the current state (taken as state 0) over nsteps with
  * binomial checkpointing inside the handle's snapshot budget.  alpha_bar (n cells,
- * FP64) receives dJ/dalpha; lambda0 (optional, AoS N*Q) receives dJ/d state0.  The
- * handle's state is left at state 0. */
+ * FP64, or NULL to leave it in device memory for lbm_alpha_bar) receives dJ/dalpha;
+ * lambda0 (optional, AoS N*Q) receives dJ/d state0.  The handle's state is left at
+ * state 0. */
 int lbm_gradient(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj, double* J, double* alpha_bar,
                  uint64_t n_alpha, double* lambda0, uint64_t lambda_len, lbm_stats* stats);
+
+/* dJ/dalpha of the last lbm_gradient / lbm_gradient_seeded / lbm_step_vjp call (n cells,
+ * FP64).  A gradient called with alpha_bar = NULL leaves it in device memory, so that a
+ * caller that does not need it on the host at once (or a timed loop) skips the copy; this
+ * copies it out afterwards through the handle's pinned staging buffer.  No counterpart in
+ * the reference, whose history_adjoint returns it as Vec<f64> (solver.rs:797-815). */
+int lbm_alpha_bar(lbm_handle* h, double* alpha_bar, uint64_t n);
 
 /* lbm_gradient with an additional cotangent lambda_final (AoS N*Q, in the reference
  * layout) added to the seed of the final state: the drop-in for

@@ -754,6 +754,52 @@ def scan_adjoint(orc: Oracle, alpha, steps, objective: Objective, every=None, f0
     return J, abar, lam, final
 
 
+def trajectory(orc: Oracle, alpha, steps, f0=None):
+    """history (solver.rs:676-687) with the port mass exchange of every step, as the
+    reference's History carries it (boundary_mass_changes_lattice): (states, exchanges)."""
+    states = [orc.initial() if f0 is None else np.array(f0, np.float64)]
+    changes = []
+    for t in range(steps):
+        after, ex, _ = orc.interval(alpha, states[-1], want_post=True, t=t)
+        states.append(after)
+        changes.append(float(np.sum(ex)))
+    return states, changes
+
+
+def admit(orc: Oracle, alpha, states, changes, mach_limit, drift_limit=1e-6):
+    """LbmProblem::admit (solver.rs:707-793) over a kept trajectory.  Returns
+    (refusal or None, max Mach, relative mass drift, min fluid population); the refusal is
+    the reference's message category: "populations" (a non-finite value anywhere, or a
+    fluid population <= 0, :716-728) or "rejected" (Mach above mach_limit or drift above
+    drift_limit, :762-771).  Mach is sqrt(3) times the largest Brinkman velocity |u| over
+    the fluid cells of every state (:730-745); the drift is max_t |M_t - (M_0 + sum of the
+    port exchanges before t)| / M_0 with M_t the fluid mass (:746-761)."""
+    Q = orc.Q
+    fluid = orc.solid == 0
+    minimum = np.inf
+    for st in states:
+        if not np.all(np.isfinite(st)):
+            return "populations", np.nan, np.nan, np.nan
+        minimum = min(minimum, float(st.reshape(-1, Q)[fluid].min()))
+    mach, mass = 0.0, []
+    for st in states:
+        rho, u = orc.macroscopic(alpha, st)
+        mach = max(mach, float(np.sqrt((u[fluid] ** 2).sum(axis=1)).max()))
+        mass.append(float(rho[fluid].sum()))
+    mach *= np.sqrt(3.0)
+    err, cum = 0.0, 0.0
+    for t, m in enumerate(mass):
+        if t > 0 and changes:
+            cum += changes[t - 1]
+        err = max(err, abs(m - (mass[0] + cum)))
+    drift = err / mass[0]
+    if minimum <= 0.0:
+        return "populations", mach, drift, minimum
+    if mach > mach_limit or drift > drift_limit:
+        return "rejected", mach, drift, minimum
+    return None, mach, drift, minimum
+
+
 def objective_value(orc: Oracle, alpha, steps, objective: Objective, f0=None):
     """Forward-only objective (for finite differences)."""
     x = orc.initial() if f0 is None else np.array(f0, np.float64)

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lbm.lib().lbm_abi_version() == 4
+    assert lbm.lib().lbm_abi_version() == 5
 
 
 @pytest.mark.parametrize("n,s", [(1, 1), (3, 1), (10, 2), (100, 10), (1000, 30), (8000, 156)])
@@ -78,6 +78,9 @@ def _create(**kw):
     (dict(thermal=True, thermal_kappa=0.02, precision="bf16s"), "FP64, FP32 and FP16"),
     (dict(thermal=True, thermal_kappa=0.02, solid=np.eye(4, dtype=bool)[None].repeat(4, 0)), "no solid cells"),
     (dict(les_constant=0.1, collision="mrt"), "Smagorinsky closure applies"),
+    (dict(mach_limit=0.3), "mach_limit"),
+    (dict(mach_limit=-0.1), "mach_limit"),
+    (dict(mach_limit=0.1, drift_limit=-1.0), "drift_limit"),
     (dict(les_constant=-0.1), "non-negative"),
     (dict(wale_constant=0.5, collision="mrt"), "WALE closure applies"),
     (dict(wale_constant=0.5, les_constant=0.1), "choose one closure"),

@@ -38,8 +38,9 @@ def test_link_table_bit_exact(shape, periodic, Q, with_solid):
     assert np.array_equal(got, want)
 
 
-# nz divisible by 4 with a non-periodic z axis takes the vectorised 4-cells-per-thread
-# kernels; other shapes take the one-cell-per-thread kernels.  Both are covered.
+# The step kernels run one thread per cell over z rows padded to a 64-element pitch; the
+# cases vary nz (4 to 12) and the periodic axes so that rows end at different offsets in
+# the pitch and the twisted ghost slots sit in the x/y layers or the separate z region.
 FWD_CASES = [
     # shape, periodic, Q, collision, with_solid, with_ports, accel
     ((8, 6, 5), (True, False, False), 19, "trt", False, False, (1e-4, 0, 0)),
