@@ -29,6 +29,7 @@ UNITS = {
     "kern_f16s.hip": [],
     "kern_f16s_l16.hip": [],
     "kern_bf16s.hip": [],
+    "kern_jvp.hip": ["-ffp-contract=off"],
     "api.hip": ["-ffp-contract=off"],
     "revolve.cpp": ["-x", "hip"],
 }

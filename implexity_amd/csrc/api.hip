@@ -17,6 +17,7 @@
 
 #include "design_map.h"
 #include "pushforward.h"
+#include "jvp.h"
 #include "launch.h"
 #include "revolve.h"
 
@@ -268,6 +269,7 @@ struct lbm_handle {
     size_t pool_slots = 0;
     int np = 0;
     std::vector<double> h_alpha;
+    std::vector<double> h_consts;  // collision constants (the `mrt` array) in FP64
     // Coupled D3Q7 temperature field: its populations follow the Q flow arrays in every
     // state buffer (flow_bytes in), its cotangents the flow cotangents (flow_lam_bytes in).
     bool thermal = false;
@@ -1358,6 +1360,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             } else {
                 c.push_back(std::isnan(h->bulk) ? 1.0 : h->bulk);
             }
+            h->h_consts = c;
             if (h->prec == LBM_F64) {
                 h->mrt.alloc(c.size() * sizeof(double), "collision constants");
                 hip_check(hipMemcpy(h->mrt.p, c.data(), h->mrt.bytes, hipMemcpyHostToDevice), "upload");
@@ -1957,6 +1960,67 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
         hip_check(hipStreamSynchronize(h->stream), "sync");
         h->abar_ready = true;
         check_lam_range(h);
+    });
+}
+
+int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, double* d_next, uint64_t len) {
+    return guarded([&] {
+        if (!h || !d_state || !d_next) fail(LBM_ERR_CONTRACT, "null argument");
+        const int Q = h->Q;
+        if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "tangent length must be cells*Q");
+        const bool kind_ok = h->kind == K_BGK || h->kind == K_TRT || h->kind == K_MRT || h->kind == K_BGK_GUO ||
+                             h->kind == K_CUMULANT;
+        if (!kind_ok || h->thermal || h->nsc)
+            fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
+                                   "Brinkman operators without closures, sponges or the temperature field");
+        // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
+        // takes 32-bit byte offsets within an opposite pair of arrays.
+        if (uint64_t(h->g.stride) * 2 * sizeof(Dual) >= (uint64_t(1) << 32))
+            fail(LBM_ERR_CONTRACT, "lattice too large for the tangent step (arrays of 16-byte values above 2 GiB)");
+        for (uint64_t x = 0; x < len; x++)
+            if (!std::isfinite(d_state[x])) fail(LBM_ERR_CONTRACT, "tangent must be finite");
+        const uint64_t t = h->t;
+        const JvpRunner* jr = jvp_runner();
+        const LaunchCfg lc = h->cfg();
+        DevBuf dual, tan, da, dc;
+        dual.alloc(size_t(h->g.stride) * Q * sizeof(Dual), "tangent state");
+        hip_check(hipMemsetAsync(dual.p, 0, dual.bytes, h->stream), "memset");
+        tan.alloc(len * sizeof(double), "tangent");
+        ensure_aos(h);
+        // values: the state as the step reads it (decoded to FP64); tangents: the caller's
+        hip_check(h->run->get_state(Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(t & 1),
+                                    h->aos.as<double>(), lc),
+                  "get_state");
+        hip_check(hipMemcpyAsync(tan.p, d_state, len * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy");
+        hip_check(jr->set(Q, h->has_mask, h->g, dual.p, h->mask.as<uint32_t>(), int(t & 1), h->aos.as<double>(),
+                          tan.as<double>(), lc.grid, lc.block, h->stream),
+                  "set dual state");
+        std::vector<Dual> hd(size_t(h->ncell));
+        for (long long x = 0; x < h->ncell; x++) hd[x] = Dual(h->h_alpha[x], d_alpha ? d_alpha[x] : 0.0);
+        da.alloc(hd.size() * sizeof(Dual), "tangent design");
+        hip_check(hipMemcpyAsync(da.p, hd.data(), da.bytes, hipMemcpyHostToDevice, h->stream), "copy");
+        std::vector<Dual> hc(h->h_consts.begin(), h->h_consts.end());
+        if (!hc.empty()) {
+            dc.alloc(hc.size() * sizeof(Dual), "tangent constants");
+            hip_check(hipMemcpyAsync(dc.p, hc.data(), dc.bytes, hipMemcpyHostToDevice, h->stream), "copy");
+        }
+        StepArgs s = h->step_args(dual.p, t);
+        s.alpha = da.p;
+        s.mrt = dc.p;
+        s.tpop = nullptr;
+        s.wu = s.sm = s.d1 = s.sm1 = nullptr;
+        hip_check(jr->step(Q, h->kind, h->has_mask, s, lc.grid, lc.block, h->stream), "tangent step launch");
+        hip_check(jr->ports(Q, h->has_mask, h->port_args(t + 1), dual.p, h->stream), "tangent port launch");
+        hip_check(jr->get(Q, h->has_mask, h->g, dual.p, h->mask.as<uint32_t>(), int((t + 1) & 1), tan.as<double>(),
+                          lc.grid, lc.block, h->stream),
+                  "get tangent");
+        hip_check(hipMemcpyAsync(d_next, tan.p, len * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
+        // solid cells keep their populations through a step (solver.rs:134): so do their tangents
+        if (!h->solid.empty())
+            for (long long x = 0; x < h->ncell; x++)
+                if (h->solid[x])
+                    for (int q = 0; q < Q; q++) d_next[size_t(x) * Q + q] = d_state[size_t(x) * Q + q];
     });
 }
 

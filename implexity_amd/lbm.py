@@ -191,6 +191,8 @@ _SIGS = {
                      ctypes.c_int),
     "lbm_reserve": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p], ctypes.c_int),
     "lbm_alpha_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_step_jvp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+                     ctypes.c_int),
     "lbm_get_fields": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_debug_link_table": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_port_cells": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
@@ -617,6 +619,17 @@ class Lattice:
         ab = np.empty(self.n)
         _check(lib().lbm_step_vjp(self._h, _p(lam), _p(out), _p(ab), lam.size))
         return out, ab
+
+    def interval_jvp(self, d_state, d_alpha=None):
+        """Tangent of one step at the current state: J_state d_state + J_alpha d_alpha
+        (ScanStep::jvp, scan.rs:46-80); the state is not advanced."""
+        dx = np.ascontiguousarray(d_state, np.float64).reshape(-1)
+        da = None if d_alpha is None else np.ascontiguousarray(d_alpha, np.float64).reshape(-1)
+        if da is not None and da.size != self.n:
+            raise ValueError("d_alpha needs one entry per cell")
+        out = np.empty_like(dx)
+        _check(lib().lbm_step_jvp(self._h, _p(dx), _p(da), _p(out), dx.size))
+        return out
 
     def fields(self):
         rho = np.empty(self.n)

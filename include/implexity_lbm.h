@@ -20,6 +20,7 @@
  *                         (both with the admission of LbmProblem::admit when mach_limit > 0,
  *                         solver.rs:707-793)
  *   lbm_step_vjp          LbmProblem::interval_vjp                    solver.rs:641-674
+ *   lbm_step_jvp          ScanStep::jvp of the step                   crates/implexity-ad/src/scan.rs:46-80
  *   lbm_vector            LbmProblem::vector (terminal responses)      solver.rs:874-894
  *   lbm_vector_vjp        LbmProblem::vector_vjp                      solver.rs:896-947
  *   lbm_get_state         the reference's AoS state vector (x*Q+q)    solver.rs:90,325
@@ -374,6 +375,14 @@ int lbm_reserve(lbm_handle* h, uint64_t nsteps, uint64_t* slots);
 /* VJP of one step at the current state: lambda_prev = J^T lambda_next (AoS N*Q), and
  * alpha_bar (N) = d(lambda_next . step)/d alpha. */
 int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, double* alpha_bar, uint64_t len);
+
+/* Tangent of one step at the current state (ScanStep::jvp, crates/implexity-ad/src/
+ * scan.rs:46-80, whose default makes one vjp call per state entry): d_next = J_state d_state
+ * + J_alpha d_alpha (AoS N*Q; d_alpha has one entry per cell, or NULL for 0), computed in
+ * FP64 forward mode through the step's own collision, streaming and port code.  The
+ * handle's state is not advanced.  Covers the BGK, TRT, MRT, regularized, cumulant and
+ * BGK-Guo Brinkman operators (no closures, sponges or temperature field). */
+int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, double* d_next, uint64_t len);
 
 int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n);
 

@@ -1,0 +1,109 @@
+"""GPU tangent of one step (lbm_step_jvp, the drop-in for ScanStep::jvp of
+crates/implexity-ad/src/scan.rs:46-80) through the C ABI, in FP64:
+
+* against central differences of the oracle's step (solver.rs:603-614 restated) in the
+  same direction (state and design together), to the accuracy of h = 1e-6;
+* the dot-product identity <lambda, J v> = <J^T lambda, v> with the device's own step
+  VJP (lbm_step_vjp) on the other side, to rounding -- two independent device codes,
+  forward mode through the step kernels and the hand-written transposed kernels.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from implexity_amd.lbm import ContractError, Lattice
+from oracle import oracle as orc
+from tests.lbm_cases import channel_ports, make_pair, random_alpha, random_solid
+
+pytestmark = pytest.mark.gpu
+
+JVP_CASES = [
+    # shape, periodic, Q, collision, with_solid, with_ports, (bulk_rate, even_rate)
+    ((7, 6, 5), (True, False, False), 19, "trt", True, False, (None, None)),
+    ((7, 6, 5), (False, False, False), 19, "trt", False, True, (None, None)),
+    ((7, 6, 5), (True, False, False), 19, "bgk", True, False, (None, None)),
+    ((7, 6, 5), (True, False, False), 19, "bgk_guo", True, False, (None, None)),
+    ((6, 5, 6), (True, True, False), 19, "mrt", True, False, (1.1, 0.7)),
+    ((6, 5, 6), (False, False, False), 19, "mrt", False, True, (None, None)),
+    ((6, 5, 6), (True, True, False), 19, "regularized", True, False, (None, None)),
+    ((6, 5, 4), (True, False, False), 27, "trt", True, False, (None, None)),
+    ((6, 5, 4), (True, False, False), 27, "mrt", False, False, (1.3, 0.8)),
+    ((6, 5, 4), (True, False, False), 27, "cumulant", True, False, (0.8, None)),
+    ((7, 5, 4), (False, False, False), 27, "cumulant", False, True, (None, None)),
+]
+
+
+def _setup(shape, periodic, Q, coll, with_solid, with_ports, rates, omega=1.4):
+    solid = random_solid(shape, 13) if with_solid else None
+    ports = channel_ports(shape, 0.03) if with_ports else ()
+    if with_ports and solid is not None:
+        solid = solid.copy()
+        solid[:2] = False
+        solid[-2:] = False
+    bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
+    even = rates[1] if rates[1] is not None else float("nan")
+    cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", omega=omega,
+                          accel=(0, 0, 0) if with_ports else (1e-4, -3e-5, 0), solid=solid, ports=ports,
+                          bulk_rate=bulk, even_rate=even)
+    return cfg, prob, solid
+
+
+def _directions(o, Q, solid, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(o.n * Q) * 1e-3
+    a = rng.standard_normal(o.n) * 1e-2
+    lam = rng.standard_normal(o.n * Q)
+    if solid is not None:
+        s = solid.reshape(-1)
+        v.reshape(-1, Q)[s] = 0.0
+        lam.reshape(-1, Q)[s] = 0.0
+    return v, a, lam
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,rates", JVP_CASES)
+def test_step_jvp_central_differences(shape, periodic, Q, coll, with_solid, with_ports, rates):
+    cfg, prob, solid = _setup(shape, periodic, Q, coll, with_solid, with_ports, rates)
+    o = orc.Oracle(prob)
+    alpha = random_alpha(shape, 3)
+    f = o.run(alpha, 4)
+    v, a, _ = _directions(o, Q, solid, 5)
+    h = 1e-6
+    fd = (o.interval(alpha + h * a, f + h * v) - o.interval(alpha - h * a, f - h * v)) / (2 * h)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        got = lat.interval_jvp(v, a)
+        after = lat.state()  # the state is not advanced
+    fluid = np.ones(o.n, bool) if solid is None else ~solid.reshape(-1)
+    g, w = got.reshape(-1, Q)[fluid], fd.reshape(-1, Q)[fluid]
+    assert np.abs(g - w).max() <= 1e-7 * np.abs(w).max(), np.abs(g - w).max() / np.abs(w).max()
+    assert np.abs(after.reshape(-1, Q)[fluid] - f.reshape(-1, Q)[fluid]).max() == 0.0
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,rates", JVP_CASES)
+def test_step_jvp_vjp_dot_product(shape, periodic, Q, coll, with_solid, with_ports, rates):
+    """<lambda, J_x v + J_alpha a> = <J_x^T lambda, v> + <J_alpha^T lambda, a>, both sides
+    on the device."""
+    cfg, prob, solid = _setup(shape, periodic, Q, coll, with_solid, with_ports, rates)
+    o = orc.Oracle(prob)
+    alpha = random_alpha(shape, 7)
+    f = o.run(alpha, 3)
+    v, a, lam = _directions(o, Q, solid, 9)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        jv = lat.interval_jvp(v, a)
+        lam_prev, abar = lat.interval_vjp(lam)
+    fluid = np.ones(o.n, bool) if solid is None else ~solid.reshape(-1)
+    lhs = float(np.dot(lam.reshape(-1, Q)[fluid].ravel(), jv.reshape(-1, Q)[fluid].ravel()))
+    rhs = float(np.dot(lam_prev.reshape(-1, Q)[fluid].ravel(), v.reshape(-1, Q)[fluid].ravel()) + abar @ a)
+    scale = float(np.abs(lam).sum() * np.abs(jv).max())
+    assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
+
+
+def test_step_jvp_refuses_closures():
+    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", les=(0.1, 0.0))
+    with Lattice(cfg) as lat:
+        with pytest.raises(ContractError, match="lbm_step_jvp covers"):
+            lat.interval_jvp(np.zeros(lat.n * 19))

@@ -64,7 +64,10 @@ FWD_CASES = [
 ]
 
 
-def _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, precision, steps, omega=1.25):
+def _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, precision, steps, omega=1.25,
+              rates=(None, None)):
+    """rates = (bulk_rate, even_rate); None takes the default (1 for the cumulant's bulk
+    rate, omega+ for MRT)."""
     solid = random_solid(shape, 5) if with_solid else None
     ports = channel_ports(shape, 0.03) if with_ports else ()
     if with_ports and solid is not None:
@@ -72,8 +75,10 @@ def _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, precision
         solid = solid.copy()
         solid[:2] = False
         solid[-2:] = False
+    bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
+    even = rates[1] if rates[1] is not None else float("nan")
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision=precision, omega=omega, accel=accel,
-                          solid=solid, ports=ports, bulk_rate=1.0 if coll == "cumulant" else float("nan"))
+                          solid=solid, ports=ports, bulk_rate=bulk, even_rate=even)
     alpha = random_alpha(shape, 7)
     o = orc.Oracle(prob)
     want = o.run(alpha, steps)
@@ -89,6 +94,54 @@ def _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, precision
 def test_forward_f64_bit_exact(shape, periodic, Q, coll, with_solid, with_ports, accel):
     got, want, _ = _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, "f64", 12)
     assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+# MRT with its own bulk and even-moment rates and the cumulant with a bulk rate other than
+# 1 (moving/collision.rs:349-375, :680-681): every rate group and the trace relaxation
+# take their own value, so a swapped group or rate differs from the oracle.
+RATE_CASES = [
+    # shape, periodic, Q, collision, with_solid, with_ports, accel, (bulk_rate, even_rate), omega
+    ((7, 5, 6), (True, True, False), 19, "mrt", True, False, (1e-4, 0, 0), (1.1, 0.7), 1.6),
+    ((7, 6, 5), (False, False, False), 19, "mrt", False, True, (0, 0, 0), (0.6, 1.45), 1.25),
+    ((6, 5, 4), (True, False, False), 27, "mrt", True, False, (1e-4, 0, 0), (1.3, 0.8), 1.6),
+    ((6, 5, 5), (True, False, False), 27, "cumulant", True, False, (1e-4, 0, 0), (0.8, None), 1.6),
+    ((7, 5, 4), (False, False, False), 27, "cumulant", False, True, (0, 0, 0), (1.4, None), 1.25),
+]
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,accel,rates,omega", RATE_CASES)
+def test_forward_f64_bit_exact_nondefault_rates(shape, periodic, Q, coll, with_solid, with_ports, accel, rates,
+                                                omega):
+    got, want, _ = _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, "f64", 12, omega=omega,
+                             rates=rates)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    # the rates change the trajectory (else the case would pin nothing beyond the defaults)
+    dflt, _, _ = _run_both(shape, periodic, Q, coll, with_solid, with_ports, accel, "f64", 12, omega=omega)
+    assert np.abs(dflt - want).max() > 1e-9
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,with_solid,with_ports,accel,rates,omega", RATE_CASES)
+def test_step_vjp_f64_nondefault_rates(shape, periodic, Q, coll, with_solid, with_ports, accel, rates, omega):
+    solid = random_solid(shape, 13) if with_solid else None
+    ports = channel_ports(shape, 0.03) if with_ports else ()
+    cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", omega=omega, accel=accel,
+                          solid=solid, ports=ports, bulk_rate=rates[0],
+                          even_rate=rates[1] if rates[1] is not None else float("nan"))
+    alpha = random_alpha(shape, 17)
+    o = orc.Oracle(prob)
+    f = o.run(alpha, 5)
+    lam = np.random.default_rng(0).standard_normal(o.n * Q)
+    if solid is not None:
+        lam.reshape(-1, Q)[solid.reshape(-1)] = 0.0
+    want_f, want_a = o.interval_vjp(alpha, f, lam)
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(f)
+        got_f, got_a = lat.interval_vjp(lam)
+    fluid = np.ones(o.n, bool) if solid is None else ~solid.reshape(-1)
+    gf, wf = got_f.reshape(-1, Q)[fluid], want_f.reshape(-1, Q)[fluid]
+    assert np.abs(gf - wf).max() <= 1e-12 * np.abs(wf).max()
+    assert np.abs(got_a - want_a).max() <= 1e-12 * np.abs(want_a).max()
 
 
 @pytest.mark.parametrize("precision,tol", [("f32", 2e-5), ("f16s", 2e-3), ("bf16s", 1.5e-2)])
@@ -189,16 +242,21 @@ GRAD_CASES = [
     ((8, 6, 5), (False, False, False), 19, "trt", "pressure_drop", True, 3),
     ((8, 6, 5), (True, False, False), 19, "mrt", "dissipation", False, 4),
     ((6, 5, 4), (True, False, False), 27, "cumulant", "dissipation", False, 5),
+    ((8, 6, 5), (True, False, False), 19, "mrt", "dissipation", False, 0, (1.1, 0.7)),
+    ((6, 5, 4), (True, False, False), 27, "cumulant", "dissipation", False, 5, (0.8, None)),
 ]
 
 
-@pytest.mark.parametrize("shape,periodic,Q,coll,obj,with_ports,slots", GRAD_CASES)
-def test_gradient_f64_matches_oracle(shape, periodic, Q, coll, obj, with_ports, slots):
+@pytest.mark.parametrize("shape,periodic,Q,coll,obj,with_ports,slots,rates",
+                         [c if len(c) == 8 else c + ((None, None),) for c in GRAD_CASES])
+def test_gradient_f64_matches_oracle(shape, periodic, Q, coll, obj, with_ports, slots, rates):
     steps = 24
     ports = channel_ports(shape, 0.03) if with_ports else ()
     accel = (0, 0, 0) if with_ports else (2e-4, 0, 0)
+    bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
+    even = rates[1] if rates[1] is not None else float("nan")
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", accel=accel, ports=ports,
-                          bulk_rate=1.0 if coll == "cumulant" else float("nan"))
+                          bulk_rate=bulk, even_rate=even)
     o = orc.Oracle(prob)
     alpha = random_alpha(shape, 23)
     ob = _objective(obj, steps, shape)
@@ -429,3 +487,33 @@ def test_gradient_vector_objective_f16_cotangents():
             lat.initial_populations()
             res[adj] = lat.gradient(steps, ob)[1]
     assert np.linalg.norm(res["f16"] - res["f32"]) / np.linalg.norm(res["f32"]) <= 2e-3
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("Q", [19, 27])
+def test_state_exchange_soa_layout(precision, Q):
+    """lbm_set_state / lbm_get_state in the moving solver's SoA layout q*N + x
+    (field.rs:961) hold the same state as the AoS layout x*Q + q (solver.rs:90): uploaded in
+    one and read in either, bit for bit in FP64 and at FP32 rounding otherwise."""
+    shape = (6, 5, 7)
+    solid = random_solid(shape, 19)
+    cfg, prob = make_pair(shape, (True, False, False), Q=Q, precision=precision, solid=solid, accel=(1e-4, 0, 0))
+    alpha = random_alpha(shape, 21)
+    o = orc.Oracle(prob)
+    f = o.run(alpha, 4)  # AoS
+    soa = np.ascontiguousarray(f.reshape(-1, Q).T).reshape(-1)
+    fluid = ~solid.reshape(-1)
+    tol = 0.0 if precision == "f64" else 1e-7
+    with Lattice(cfg) as lat:
+        lat.set_design(alpha)
+        lat.set_state(soa, layout="soa")
+        aos_back = lat.state("aos").reshape(-1, Q)
+        soa_back = lat.state("soa").reshape(Q, -1)
+        assert np.array_equal(soa_back.T, aos_back)
+        assert np.abs(aos_back[fluid] - f.reshape(-1, Q)[fluid]).max() <= tol
+        # one step from the SoA upload equals one step from the AoS upload
+        lat.interval(1)
+        a = lat.state()
+        lat.set_state(f)
+        lat.interval(1)
+        assert np.array_equal(lat.state(), a)
