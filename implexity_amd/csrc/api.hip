@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -243,6 +244,7 @@ struct lbm_handle {
     bool regularized = false;
     double les_c = 0.0, les_floor = 0.0;  // Smagorinsky closure (K_TRT_LES / K_BGK_LES)
     double wale_c2 = 0.0, wale_floor = 0.0;  // WALE closure (K_TRT_WALE / K_BGK_WALE)
+    int device = 0;      // HIP device current at lbm_create (the handle's memory and stream)
     uint64_t every = 0;  // Checkpointing::Every(k); 0 = binomial
     double mach_limit = 0.0, drift_limit = 1e-6;  // admission (0: none)
     int nx = 0, ny = 0, nz = 0;
@@ -991,6 +993,83 @@ float elapsed(lbm_handle* h) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------
+// Workspace of lbm_vector_noalloc, for callers that cannot allocate device memory: the
+// per-block partial sums of the response kernel and of the design volume live in this
+// file-scope device array instead of a per-call allocation.  Every GPU the library runs
+// on holds its own copy until the process ends; calls on one GPU share it (and keep the
+// last call's contents), so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 19;  // 4 MiB
+constexpr int NOALLOC_PHI_BLOCKS = 256;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on device dev (which must be current), looked up once per
+// device; host code uses only this address (the array's name is a host-side stand-in).
+double* scratch_address(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) fail(LBM_ERR_DEVICE, "device index out of range for the workspace");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(LBM_ERR_DEVICE, std::string("workspace address lookup failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Block-wide sum of one value per thread (256 threads), result in thread 0, fixed order.
+__device__ double block_sum256(double v) {
+    __shared__ double red[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return r;
+}
+
+// part[b] = sum over x = b*256 + t (+ k * gridDim * 256) of phi[x], fixed order.
+__global__ __launch_bounds__(256) void k_noalloc_phi(const double* phi, long long n, double* part) {
+    double acc = 0.0;
+    const long long step = (long long)gridDim.x * 256;
+    for (long long x = (long long)blockIdx.x * 256 + threadIdx.x; x < n; x += step) acc += phi[x];
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// One block: folds the response kernel's partials sums[5 b + c] (b < nb) and the volume
+// partials (nphi of them) in a fixed order and writes the six responses of
+// LbmProblem::vector (solver.rs:874-894).
+__global__ __launch_bounds__(256) void k_noalloc_vector(const double* sums, long long nb, const double* phi_part,
+                                                        int nphi, double inv_fluid, double cell_volume,
+                                                        double* out) {
+    double acc[6];
+    for (int c = 0; c < 5; c++) {
+        double a = 0.0;
+        for (long long b = threadIdx.x; b < nb; b += 256) a += sums[5 * b + c];
+        acc[c] = block_sum256(a);
+    }
+    acc[5] = block_sum256(int(threadIdx.x) < nphi ? phi_part[threadIdx.x] : 0.0);
+    if (threadIdx.x == 0) {
+        out[0] = acc[0] * inv_fluid;
+        out[1] = acc[1] * inv_fluid;
+        out[2] = acc[2] * inv_fluid;
+        out[3] = 0.5 * acc[3] * cell_volume;
+        out[4] = acc[4] * cell_volume;
+        out[5] = acc[5] * cell_volume;
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -1261,6 +1340,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->nsc = int(sf.cell.size());
 
         // Device allocations.
+        hip_check(hipGetDevice(&h->device), "device query");
         hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream create");
         hip_check(hipEventCreate(&h->ev0), "event create");
         hip_check(hipEventCreate(&h->ev1), "event create");
@@ -1647,6 +1727,41 @@ int lbm_vector(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_sc
         out[3] = 0.5 * acc[3] * scales->cell_volume;
         out[4] = acc[4] * scales->cell_volume;
         out[5] = vol * scales->cell_volume;
+    });
+}
+
+int lbm_vector_noalloc(lbm_handle* h, const double* phi_dev, uint64_t n, const lbm_vector_scales* scales,
+                       double* out_dev) {
+    return guarded([&] {
+        if (!h || !out_dev) fail(LBM_ERR_CONTRACT, "null argument");
+        if (is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "terminal responses read a Brinkman coefficient; not with the occupancy coupling");
+        check_scales(scales);
+        if (phi_dev && n != uint64_t(h->ncell)) fail(LBM_ERR_SHAPE, "phi needs one entry per cell");
+        int dev = -1;
+        hip_check(hipGetDevice(&dev), "device query");
+        if (dev != h->device) fail(LBM_ERR_CONTRACT, "call lbm_vector_noalloc with the handle's device current");
+        const LaunchCfg c = h->cfg();
+        const long long nb = (long long)c.grid.x * c.grid.y * c.grid.z;
+        const size_t need = size_t(nb) * 5 + NOALLOC_PHI_BLOCKS;
+        if (need > NOALLOC_SCRATCH_DOUBLES)
+            fail(LBM_ERR_MEMORY, "lbm_vector_noalloc needs " + std::to_string(need) + " workspace values for this "
+                                 "lattice; its fixed workspace holds " + std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                                 " (use lbm_vector)");
+        double* ws = scratch_address(dev);
+        hip_check(h->run->vector(h->Q, h->has_mask, h->g, h->F.p, h->mask.as<uint32_t>(), int(h->t & 1), h->alpha.p,
+                                 h->a, scales->velocity, scales->density, 1.0 / scales->step_s, ws, c),
+                  "vector launch");
+        int nphi = 0;
+        if (phi_dev) {
+            nphi = NOALLOC_PHI_BLOCKS;
+            hipLaunchKernelGGL(k_noalloc_phi, dim3(NOALLOC_PHI_BLOCKS), dim3(256), 0, h->stream, phi_dev,
+                               (long long)n, ws + size_t(nb) * 5);
+            hip_check(hipGetLastError(), "volume launch");
+        }
+        hipLaunchKernelGGL(k_noalloc_vector, dim3(1), dim3(256), 0, h->stream, ws, nb, ws + size_t(nb) * 5, nphi,
+                           1.0 / double(fluid_cells(h)), scales->cell_volume, out_dev);
+        hip_check(hipGetLastError(), "fold launch");
+        hip_check(hipStreamSynchronize(h->stream), "sync");
     });
 }
 

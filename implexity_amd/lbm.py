@@ -193,6 +193,8 @@ _SIGS = {
     "lbm_alpha_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_step_jvp": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
                      ctypes.c_int),
+    "lbm_vector_noalloc": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p],
+                           ctypes.c_int),
     "lbm_get_fields": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_debug_link_table": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_port_cells": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
@@ -600,6 +602,13 @@ class Lattice:
         sc = _scales(scales)
         _check(lib().lbm_vector(self._h, _p(ph), 0 if ph is None else ph.size, ctypes.byref(sc), _p(out)))
         return out
+
+    def vector_noalloc(self, scales, out_dev, phi_dev=None):
+        """lbm_vector into device memory without allocating: out_dev (6 doubles) and phi_dev
+        (cells doubles, or None) are device addresses (ints) on the handle's device."""
+        sc = _scales(scales)
+        _check(lib().lbm_vector_noalloc(self._h, None if phi_dev is None else ctypes.c_void_p(phi_dev),
+                                        0 if phi_dev is None else self.n, ctypes.byref(sc), ctypes.c_void_p(out_dev)))
 
     def vector_vjp(self, scales, w, phi=None):
         """(f_bar, alpha_bar, phi_bar) of w . vector at the current state (solver.rs:896-947)."""

@@ -362,6 +362,18 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
  * (n = cells) may be NULL (volume 0).  Sums are FP64. */
 int lbm_vector(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, double* out);
 
+/* lbm_vector without any device allocation, for callers that cannot allocate (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  phi_dev (n = cells doubles, or NULL) and out_dev (6
+ * doubles) are device pointers on the handle's device, which must be current; the
+ * responses are those of lbm_vector (sums folded on the device in a fixed order instead
+ * of on the host).  The per-block partial sums live in a fixed 4 MiB device workspace
+ * of the library, so lattices needing more than (2^19 - 256) / 5 step blocks are
+ * refused with LBM_ERR_MEMORY (a 256^3 lattice needs 65536).  Calls on one GPU share the
+ * workspace: overlapping calls on one GPU race.  Returns after the stream has completed. */
+int lbm_vector_noalloc(lbm_handle* h, const double* phi_dev, uint64_t n, const lbm_vector_scales* scales,
+                       double* out_dev);
+
 /* VJP of w . vector at the current state (LbmProblem::vector_vjp, solver.rs:896-947):
  * f_bar (AoS N*Q), alpha_bar (N), phi_bar (N, may be NULL). */
 int lbm_vector_vjp(lbm_handle* h, const double* phi, uint64_t n, const lbm_vector_scales* scales, const double* w,
