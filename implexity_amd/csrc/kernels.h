@@ -413,9 +413,10 @@ struct LamStore<P_F16S> {
 // Addressing.
 // ---------------------------------------------------------------------------------
 // Storage indices are 32-bit: the host refuses lattices whose population arrays exceed
-// 4 GiB, so (index * element size) is a 32-bit byte offset from a wave-uniform array
-// base, which the compiler turns into SGPR-base + VGPR-offset global accesses with no
-// per-access 64-bit address arithmetic.
+// 2 GiB, so (index * element size), plus the array stride of the opposite array of a
+// pair, is a 32-bit byte offset from a wave-uniform array base, which the compiler turns
+// into SGPR-base + VGPR-offset global accesses with no per-access 64-bit address
+// arithmetic.
 // Rows are whole: a storage row holds the nz cells of one (i, j) and starts on a
 // 128-byte line (SZ is a multiple of 64 elements).  The +z ghost slot of a row (the
 // esoteric-twist slot beyond the last cell of a non-periodic z axis) lives in a
