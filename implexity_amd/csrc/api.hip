@@ -1151,8 +1151,9 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         if (h->kind == K_BGK_GUO && h->Q != 19) fail(LBM_ERR_CONTRACT, "BGK-Guo is the D3Q19 stationary operator");
         if (cfg->precision > LBM_BF16_SHIFTED) fail(LBM_ERR_CONTRACT, "unknown precision");
         h->prec = int(cfg->precision);
-        if (h->prec == LBM_BF16_SHIFTED && h->kind != K_TRT && h->kind != K_BGK)
-            fail(LBM_ERR_CONTRACT, "BF16 storage supports BGK and TRT only");
+        if (h->prec == LBM_BF16_SHIFTED && (is_wale(h->kind) || h->kind == K_TRT_LES || h->kind == K_BGK_LES))
+            fail(LBM_ERR_CONTRACT, "BF16 storage supports the operators without closures (FP64, FP32 or FP16 for "
+                                   "the Smagorinsky and WALE closures)");
         switch (cfg->adjoint_storage) {
         case LBM_ADJ_AUTO: h->lprec = h->prec == LBM_F64 ? LBM_F64 : (h->prec == LBM_F16_SHIFTED ? LBM_F16_SHIFTED : LBM_F32); break;
         case LBM_ADJ_F64:

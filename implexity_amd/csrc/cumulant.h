@@ -168,6 +168,97 @@ __device__ __forceinline__ void collide_cumulant(const C* f, C rho, const C u[3]
     });
 }
 
+// The same collision on deviations d = f - w (single-precision shifted storage).  The
+// chimera transform is linear in f for a fixed u, and the D3Q27 weights are the product
+// w1(cx) w1(cy) w1(cz) with w1(0) = 2/3, w1(+-1) = 1/6, whose central moments about u are
+// per axis (1, -u, 1/3 + u^2).  So the transform of d gives kappa(f) - kappa(w); the
+// post-collision moments are formed as deviations Delta = post - kappa(w) in which no two
+// O(1) quantities are subtracted (rho = 1 + drho, M2 = I/3 + E, the Wick products
+// expanded about I/3); and the inverse transform of Delta is d' = f' - w.  In exact
+// arithmetic this equals collide_cumulant; in FP32 it keeps the deviations' significant
+// bits instead of rounding at ulp(w).
+template <class C>
+__device__ __forceinline__ C mw(int e, C u) {  // central moment e of w1 about u
+    return e == 0 ? C(1) : (e == 1 ? -u : C(1.0 / 3.0) + u * u);
+}
+
+template <class C>
+__device__ __forceinline__ void collide_cumulant_shifted(const C* d, C drho, C rho, const C u[3], const C F[3],
+                                                         C omega, C bulk, C* out) {
+    C cube[27];
+    static_for<0, 27>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        cube[cube_of<q>()] = d[q];
+    });
+    chimera_forward(cube, u);  // kappa(d) = kappa(f) - kappa(w)
+    const C u0 = u[0], u1 = u[1], u2 = u[2];
+    const C q0 = u0 * u0, q1 = u1 * u1, q2 = u2 * u2;
+    // second order of f as deviations from I/3: kappa_aa(f) - 1/3 = u_a^2 + kappa_aa(d)
+    const C dxx = q0 + cube[18], dyy = q1 + cube[6], dzz = q2 + cube[2];
+    const C one_minus = C(1) - omega;
+    const C sxy = (dxx - dyy) * one_minus;
+    const C sxz = (dxx - dzz) * one_minus;
+    C dtr = dxx + dyy + dzz;        // trace - 1
+    dtr = dtr + (drho - dtr) * bulk;  // relaxed towards rho
+    const C ex = (dtr + sxy + sxz) * C(1.0 / 3.0);  // post M2 - I/3
+    const C ey = (dtr - sxy * C(2.0) + sxz) * C(1.0 / 3.0);
+    const C ez = (dtr + sxy - sxz * C(2.0)) * C(1.0 / 3.0);
+    const C pxy = (u0 * u1 + cube[12]) * one_minus;
+    const C pxz = (u0 * u2 + cube[10]) * one_minus;
+    const C pyz = (u1 * u2 + cube[4]) * one_minus;
+    const C irho = C(1) / rho;
+    const C r1 = -drho * irho;                          // 1/rho - 1
+    const C r2 = -drho * (C(2) + drho) * irho * irho;  // 1/rho^2 - 1
+    C post[27];
+    // odd orders 3 and 5 of the post state vanish: Delta = -kappa(w)
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        const int a = k / 9, b = (k / 3) % 3, c = k % 3;
+        post[k] = -(mw(a, u0) * mw(b, u1) * mw(c, u2));
+    }
+    post[0] = drho;
+    post[9] = F[0] * C(0.5) + u0;
+    post[3] = F[1] * C(0.5) + u1;
+    post[1] = F[2] * C(0.5) + u2;
+    post[18] = ex - q0;
+    post[6] = ey - q1;
+    post[2] = ez - q2;
+    post[12] = pxy - u0 * u1;
+    post[10] = pxz - u0 * u2;
+    post[4] = pyz - u1 * u2;
+    const C third = C(1.0 / 3.0), ninth = C(1.0 / 9.0);
+    // order 4, (a,a,b,b): W = M_aa M_bb + 2 M_ab^2 = 1/9 + A; kappa(w) = 1/9 + B
+    auto pair4 = [&](C ea, C eb, C pab, C qa, C qb) {
+        const C A = (ea + eb) * third + ea * eb + C(2) * pab * pab;
+        const C B = (qa + qb) * third + qa * qb;
+        return (A - B) + r1 * (ninth + A);
+    };
+    post[24] = pair4(ex, ey, pxy, q0, q1);  // (2,2,0)
+    post[20] = pair4(ex, ez, pxz, q0, q2);  // (2,0,2)
+    post[8] = pair4(ey, ez, pyz, q1, q2);   // (0,2,2)
+    // order 4, (a,a,b,c): W = M_aa M_bc + 2 M_ab M_ac; kappa(w) = (1/3 + u_a^2) u_b u_c
+    auto cross4 = [&](C ea, C pbc, C pab, C pac, C qa, C ub, C uc) {
+        const C W = (third + ea) * pbc + C(2) * pab * pac;
+        return (W - (third + qa) * ub * uc) + r1 * W;
+    };
+    post[22] = cross4(ex, pyz, pxy, pxz, q0, u1, u2);  // (2,1,1)
+    post[16] = cross4(ey, pxz, pxy, pyz, q1, u0, u2);  // (1,2,1)
+    post[14] = cross4(ez, pxy, pxz, pyz, q2, u0, u1);  // (1,1,2)
+    // order 6: W = Mxx Myy Mzz + 2 (Mxx Myz^2 + Myy Mxz^2 + Mzz Mxy^2) + 8 Mxy Mxz Myz
+    {
+        const C A6 = (ex + ey + ez) * ninth + (ex * ey + ex * ez + ey * ez) * third + ex * ey * ez +
+                     C(2) * ((third + ex) * pyz * pyz + (third + ey) * pxz * pxz + (third + ez) * pxy * pxy) +
+                     C(8) * pxy * pxz * pyz;
+        const C B6 = (q0 + q1 + q2) * ninth + (q0 * q1 + q0 * q2 + q1 * q2) * third + q0 * q1 * q2;
+        post[26] = (A6 - B6) + r2 * (C(1.0 / 27.0) + A6);
+    }
+    chimera_inverse(post, u);  // d' = f' - w
+    static_for<0, 27>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        out[q] = post[cube_of<q>()];
+    });
+}
+
 // 1D inverse-transform kernel T(c, e; u): raw value at velocity c in {-1,0,1} (index
 // c+1) produced by central moment e in {0,1,2}, and its derivative in u.
 template <class C>

@@ -958,10 +958,14 @@ LBM_DEV void collide(const C* v, C alpha, const C a[3], C omega, C om, C tau, co
             else
                 collide_mrt<Q>(v, m.rho, m.u, m.F, mrt, out);
         } else {
-            C f[Q], o[Q];
-            to_full<SH, Q>(v, f);
-            collide_cumulant(f, m.rho, m.u, m.F, omega, mrt[0], o);
-            from_full<SH, Q>(o, out);
+            if constexpr (SH) {
+                collide_cumulant_shifted(v, m.drho, m.rho, m.u, m.F, omega, mrt[0], out);
+            } else {
+                C f[Q], o[Q];
+                to_full<SH, Q>(v, f);
+                collide_cumulant(f, m.rho, m.u, m.F, omega, mrt[0], o);
+                from_full<SH, Q>(o, out);
+            }
         }
     }
 }

@@ -82,7 +82,7 @@ typedef enum {
     LBM_F64 = 0,
     LBM_F32 = 1,
     LBM_F16_SHIFTED = 2, /* FP16 of (f_q - w_q) times an exact power of two */
-    LBM_BF16_SHIFTED = 3 /* BF16 of (f_q - w_q); BGK and TRT only            */
+    LBM_BF16_SHIFTED = 3 /* BF16 of (f_q - w_q); no LES closures            */
 } lbm_precision;
 
 typedef enum { LBM_FACE_XMIN = 0, LBM_FACE_XMAX, LBM_FACE_YMIN, LBM_FACE_YMAX, LBM_FACE_ZMIN, LBM_FACE_ZMAX } lbm_face;

@@ -66,7 +66,7 @@ def _create(**kw):
     (dict(collision="cumulant"), "D3Q27"),
     (dict(lattice=27, collision="bgk_guo"), "D3Q19"),
     (dict(magic=-1.0), "magic"),
-    (dict(precision="bf16s", collision="mrt"), "BF16"),
+    (dict(precision="bf16s", les_constant=0.1), "BF16"),
     (dict(precision="f32", adjoint_storage="f16"), "FP16 cotangents"),
     (dict(precision="f64", adjoint_storage="f32"), "FP64 storage"),
     (dict(periodic=(True, False, False), ports=[lbm.Port("xmin", "pressure", (1.0,))]), "periodic"),

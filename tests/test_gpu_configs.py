@@ -219,14 +219,16 @@ def test_cumulant_256_storage_precisions():
     J64, _, u64, _, st = _run(cfg, alpha, STEPS2, ob)
     assert st.bad_cells == 0 and abs(st.mass_drift) < 1e-11
     out = {}
-    for precision in ("f32", "f16s"):
+    for precision in ("f32", "f16s", "bf16s"):
         cfg, _, _ = _config2(precision)
         J, _, u, _, st = _run(cfg, alpha, STEPS2, ob)
         out[precision] = (_l2(u, u64), abs(J - J64) / abs(J64))
     print("\n256^3 D3Q27 cumulant, 5000 steps: " +
           ", ".join(f"{p}: velocity L2 {e[0]:.3e}, dissipation {e[1]:.3e}" for p, e in out.items()))
-    assert out["f32"][0] <= 1e-3 and out["f32"][1] <= 1e-3, out
+    # FP32 arithmetic on deviations (the shifted chimera transform): ~2e-6 measured
+    assert out["f32"][0] <= 2e-5 and out["f32"][1] <= 2e-5, out
     assert out["f16s"][0] <= 5e-2 and out["f16s"][1] <= 5e-2, out
+    assert out["bf16s"][0] <= 5e-1 and out["bf16s"][1] <= 5e-1, out
 
 
 # --- 256^3 D3Q19 TRT channel of the benchmark -------------------------------------------------

@@ -147,14 +147,6 @@ def test_step_vjp_f64_nondefault_rates(shape, periodic, Q, coll, with_solid, wit
 @pytest.mark.parametrize("precision,tol", [("f32", 2e-5), ("f16s", 2e-3), ("bf16s", 1.5e-2)])
 @pytest.mark.parametrize("Q,coll", [(19, "trt"), (27, "trt"), (19, "mrt"), (19, "regularized"), (27, "cumulant")])
 def test_forward_reduced_precision(precision, tol, Q, coll):
-    if precision == "bf16s" and coll not in ("trt", "bgk"):
-        pytest.skip("BF16 storage supports BGK and TRT only")
-    if precision == "f32" and coll == "cumulant":
-        # The cumulant collision works on full populations f = w + d in FP32 (the
-        # chimera transform and the Wick closure are not linear in d), so each step
-        # rounds at ulp(w) ~ 3e-8 against deviations of ~1e-3: 20 steps leave a few
-        # 1e-4 of the deviation scale. TRT/MRT stay in shifted form and meet 2e-5.
-        tol = 1e-3
     shape = (8, 6, 8)
     got, want, o = _run_both(shape, (True, False, False), Q, coll, False, False, (1e-4, 0, 0), precision, 20)
     _, w, _ = orc.lattice(Q)
