@@ -1216,9 +1216,16 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->ny = cfg->shape[1];
         h->nz = cfg->shape[2];
         h->ncell = (long long)h->nx * h->ny * h->nz;
-        if (!std::isfinite(cfg->omega) || cfg->omega <= 0.0) fail(LBM_ERR_CONTRACT, "omega must be finite and positive");
-        h->omega = cfg->omega;
-        h->tau = 1.0 / cfg->omega;
+        if (cfg->tau != 0.0) {
+            if (!std::isfinite(cfg->tau) || cfg->tau <= 0.0) fail(LBM_ERR_CONTRACT, "tau must be finite and positive");
+            h->tau = cfg->tau;
+            h->omega = 1.0 / cfg->tau;
+        } else {
+            if (!std::isfinite(cfg->omega) || cfg->omega <= 0.0)
+                fail(LBM_ERR_CONTRACT, "omega must be finite and positive");
+            h->omega = cfg->omega;
+            h->tau = 1.0 / cfg->omega;
+        }
         if (!(h->tau >= 0.51 && h->tau <= 2.0))
             fail(LBM_ERR_CONTRACT, "Choose spacing/time step such that 0.51 <= relaxation time <= 2");
         h->magic = cfg->magic;
@@ -1278,56 +1285,99 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         h->lam_bytes = h->flow_lam_bytes + size_t(h->g.stride) * TQn * h->tlam_elem;
         h->run = runner_for(h->prec, h->lprec);
 
-        // Ports (moving/boundary.rs:318-380).
+        // Ports (moving/boundary.rs:318-380; ports.rs:107-262 for the stationary rules).
         std::vector<int> pkind;
         std::vector<double> pval;
+        std::vector<uint8_t> neighbour_row;  // 1: the row reads its inward neighbour
         std::vector<int> is_port(h->ncell, 0);
         for (uint32_t p = 0; p < cfg->n_ports; p++) {
             const lbm_port& port = cfg->ports[p];
             if (port.face > LBM_FACE_ZMAX) fail(LBM_ERR_CONTRACT, "unknown port face");
-            if (port.kind > LBM_PORT_PRESSURE) fail(LBM_ERR_CONTRACT, "Port kind must be velocity or pressure");
+            if (port.kind > LBM_PORT_PRESSURE_MOMENT)
+                fail(LBM_ERR_CONTRACT, "Port kind must be velocity or pressure (NEE, neighbour or moment rule)");
+            const bool stationary = port.kind >= LBM_PORT_PRESSURE_NEIGHBOR;
+            const bool moment = port.kind >= LBM_PORT_VELOCITY_MOMENT;
+            const bool pressure = port.kind == LBM_PORT_PRESSURE || port.kind == LBM_PORT_PRESSURE_NEIGHBOR ||
+                                  port.kind == LBM_PORT_PRESSURE_MOMENT;
+            if (stationary && h->Q != 19) fail(LBM_ERR_CONTRACT, "the stationary port rules (ports.rs) are D3Q19");
+            if (stationary && cfg->thermal)
+                fail(LBM_ERR_CONTRACT, "the temperature field takes the NEE velocity and pressure ports");
             const int axis = int(port.face) / 2;
             const int inward = (port.face % 2 == 0) ? 1 : -1;
             const int len = axis == 0 ? h->nx : (axis == 1 ? h->ny : h->nz);
             if (h->per[axis]) fail(LBM_ERR_CONTRACT, "A periodic face cannot be a port");
             if (len < 3) fail(LBM_ERR_CONTRACT, "port needs at least three cells along its axis");
             const int layer = inward > 0 ? 0 : len - 1;
+            if (port.mask && port.mask_len != uint64_t(h->ncell))
+                fail(LBM_ERR_SHAPE, "port mask needs one entry per cell");
+            if (!port.mask && port.mask_len) fail(LBM_ERR_CONTRACT, "port mask_len without a mask");
             std::vector<long long> cells;
             for (long long x = 0; x < h->ncell; x++) {
                 const int k = int(x % h->nz), j = int((x / h->nz) % h->ny), i = int(x / ((long long)h->nz * h->ny));
                 const int c = axis == 0 ? i : (axis == 1 ? j : k);
-                if (c == layer && (h->solid.empty() || !h->solid[x])) cells.push_back(x);
+                const bool wall = !h->solid.empty() && h->solid[x];
+                if (port.mask && port.mask[x]) {
+                    if (c != layer) fail(LBM_ERR_CONTRACT, "Port mask must lie on its face");
+                    if (wall) fail(LBM_ERR_CONTRACT, "Port cells must be fluid");
+                }
+                if (c == layer && !wall && (!port.mask || port.mask[x])) cells.push_back(x);
             }
             if (cells.empty()) fail(LBM_ERR_CONTRACT, "port has no cells");
+            if (port.profile && pressure) fail(LBM_ERR_CONTRACT, "a profile scales a velocity port's velocity");
             if (port.profile && port.profile_len != cells.size())
                 fail(LBM_ERR_SHAPE, "port profile needs one factor per port cell");
+            for (int d = 0; d < 3; d++)
+                if (!std::isfinite(port.value[d])) fail(LBM_ERR_CONTRACT, "port values must be finite");
+            if (pressure && !(port.value[0] > 0.0)) fail(LBM_ERR_CONTRACT, "pressure port density must be positive");
+            const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
+            const int row_kind = !stationary ? int(port.kind)
+                                             : ST_PORT_BASE + 6 * int(port.kind - LBM_PORT_PRESSURE_NEIGHBOR) +
+                                                   int(port.face);
             for (size_t r = 0; r < cells.size(); r++) {
                 const long long x = cells[r];
                 if (is_port[x]) fail(LBM_ERR_CONTRACT, "port cells overlap");
                 const long long step = axis == 0 ? (long long)h->ny * h->nz : (axis == 1 ? h->nz : 1);
-                const long long nb = x + inward * step;
-                if (!h->solid.empty() && h->solid[nb])
+                // the moment rules read the port cell itself (Port::sources, ports.rs:74-82)
+                const long long nb = moment ? x : x + inward * step;
+                if (!moment && !h->solid.empty() && h->solid[nb])
                     fail(LBM_ERR_CONTRACT, "inward neighbour of a port cell is a wall");
                 const double prof = port.profile ? port.profile[r] : 1.0;
                 if (!std::isfinite(prof)) fail(LBM_ERR_CONTRACT, "port profile must be finite");
                 is_port[x] = 1;
                 h->h_pcell.push_back(int(x));
                 h->h_pnb.push_back(int(nb));
-                pkind.push_back(int(port.kind));
-                if (port.kind == LBM_PORT_VELOCITY) {
+                neighbour_row.push_back(moment ? 0 : 1);
+                pkind.push_back(row_kind);
+                switch (port.kind) {
+                case LBM_PORT_VELOCITY:
                     for (int d = 0; d < 3; d++) pval.push_back(port.value[d] * prof);
-                } else {
-                    if (!(port.value[0] > 0.0)) fail(LBM_ERR_CONTRACT, "pressure port density must be positive");
+                    break;
+                case LBM_PORT_PRESSURE:
                     pval.push_back(port.value[0]);
                     pval.push_back(0.0);
                     pval.push_back(0.0);
+                    break;
+                case LBM_PORT_PRESSURE_NEIGHBOR:
+                    for (int d = 0; d < 3; d++) pval.push_back(port.value[d]);
+                    break;
+                case LBM_PORT_VELOCITY_MOMENT:  // v0 = u - a/2 (ports.rs:352)
+                    for (int d = 0; d < 3; d++) pval.push_back(port.value[d] * prof - 0.5 * h->a[d]);
+                    break;
+                default:
+                    pval.push_back(port.value[0]);
+                    pval.push_back(port.value[1] - 0.5 * h->a[t0]);
+                    pval.push_back(port.value[2] - 0.5 * h->a[t1]);
+                    break;
                 }
             }
         }
-        for (int nb : h->h_pnb)
-            if (is_port[nb]) fail(LBM_ERR_CONTRACT, "the inward neighbour of a port cell is itself a port cell");
+        for (size_t r = 0; r < h->h_pnb.size(); r++)
+            if (neighbour_row[r] && is_port[h->h_pnb[r]])
+                fail(LBM_ERR_CONTRACT, "the inward neighbour of a port cell is itself a port cell");
         {  // the port transposes add into the neighbours' cotangents, one port cell each
-            std::vector<int> nbs(h->h_pnb);
+            std::vector<int> nbs;
+            for (size_t r = 0; r < h->h_pnb.size(); r++)
+                if (neighbour_row[r]) nbs.push_back(h->h_pnb[r]);
             std::sort(nbs.begin(), nbs.end());
             if (std::adjacent_find(nbs.begin(), nbs.end()) != nbs.end())
                 fail(LBM_ERR_CONTRACT, "two port cells share an inward neighbour (an axis with ports on both faces "

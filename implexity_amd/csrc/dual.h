@@ -1,0 +1,54 @@
+// Dual numbers for forward-mode derivatives on the device: the tangent step (jvp.h)
+// and the cell Jacobians of the stationary port rules (kernels.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cell.h"
+
+namespace lbm {
+
+// a + eps b with eps^2 = 0: every operation propagates the derivative b of its value a.
+struct Dual {
+    double v, d;
+    __host__ __device__ constexpr Dual() : v(0.0), d(0.0) {}
+    __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
+    __host__ __device__ constexpr Dual(double x, double t) : v(x), d(t) {}
+    __host__ __device__ explicit constexpr operator double() const { return v; }
+    __host__ __device__ explicit constexpr operator float() const { return float(v); }
+    LBM_DEV Dual& operator+=(const Dual& b) {
+        v += b.v;
+        d += b.d;
+        return *this;
+    }
+    LBM_DEV Dual& operator-=(const Dual& b) {
+        v -= b.v;
+        d -= b.d;
+        return *this;
+    }
+    LBM_DEV Dual& operator*=(const Dual& b) {
+        const double bv = b.v, bd = b.d;
+        d = d * bv + v * bd;
+        v *= bv;
+        return *this;
+    }
+    LBM_DEV Dual& operator/=(const Dual& b) {
+        const double bv = b.v, bd = b.d;
+        v /= bv;
+        d = (d - v * bd) / bv;
+        return *this;
+    }
+};
+
+LBM_DEV Dual operator+(Dual a, const Dual& b) { return a += b; }
+LBM_DEV Dual operator-(Dual a, const Dual& b) { return a -= b; }
+LBM_DEV Dual operator*(Dual a, const Dual& b) { return a *= b; }
+LBM_DEV Dual operator/(Dual a, const Dual& b) { return a /= b; }
+LBM_DEV Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+LBM_DEV bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+LBM_DEV bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+LBM_DEV bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+LBM_DEV bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+LBM_DEV bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+LBM_DEV bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+
+}  // namespace lbm

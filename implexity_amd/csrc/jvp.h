@@ -14,50 +14,6 @@ namespace lbm {
 // Storage/arithmetic tag of the dual field (after the Prec values of kernels.h).
 constexpr int P_JVP = 4;
 
-// a + eps b with eps^2 = 0: every operation propagates the derivative b of its value a.
-struct Dual {
-    double v, d;
-    __host__ __device__ constexpr Dual() : v(0.0), d(0.0) {}
-    __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
-    __host__ __device__ constexpr Dual(double x, double t) : v(x), d(t) {}
-    __host__ __device__ explicit constexpr operator double() const { return v; }
-    __host__ __device__ explicit constexpr operator float() const { return float(v); }
-    LBM_DEV Dual& operator+=(const Dual& b) {
-        v += b.v;
-        d += b.d;
-        return *this;
-    }
-    LBM_DEV Dual& operator-=(const Dual& b) {
-        v -= b.v;
-        d -= b.d;
-        return *this;
-    }
-    LBM_DEV Dual& operator*=(const Dual& b) {
-        const double bv = b.v, bd = b.d;
-        d = d * bv + v * bd;
-        v *= bv;
-        return *this;
-    }
-    LBM_DEV Dual& operator/=(const Dual& b) {
-        const double bv = b.v, bd = b.d;
-        v /= bv;
-        d = (d - v * bd) / bv;
-        return *this;
-    }
-};
-
-LBM_DEV Dual operator+(Dual a, const Dual& b) { return a += b; }
-LBM_DEV Dual operator-(Dual a, const Dual& b) { return a -= b; }
-LBM_DEV Dual operator*(Dual a, const Dual& b) { return a *= b; }
-LBM_DEV Dual operator/(Dual a, const Dual& b) { return a /= b; }
-LBM_DEV Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
-LBM_DEV bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
-LBM_DEV bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
-LBM_DEV bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
-LBM_DEV bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
-LBM_DEV bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
-LBM_DEV bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
-
 // The dual field holds full populations f (no shift), 16 bytes per value.
 template <>
 struct Store<P_JVP> {

@@ -38,6 +38,7 @@
 
 #include "cell.h"
 #include "cumulant.h"
+#include "dual.h"
 #include "les.h"
 
 #ifndef LBM_FLOW_SR
@@ -1467,14 +1468,184 @@ struct PortArgs {
     int np;
     const int* cell;      // port cell (reference index)
     const int* nb;        // inward neighbour
-    const int* kind;      // 0 velocity, 1 pressure
+    const int* kind;      // 0 velocity, 1 pressure (NEE), ST_PORT_BASE + 6 rule + face
     const double* value;  // 3 per port cell
     const uint32_t* mask;
     int odd;              // parity of the state the ports act on
 };
 
+// ---------------------------------------------------------------------------------
+// Stationary D3Q19 port rules (ports.rs), on a row kind ST_PORT_BASE + 6 rule + face
+// (face 0..5 = xmin, xmax, ymin, ymax, zmin, zmax; sign +1 on the min faces):
+//   rule 0  reconstruct_neighbor (:484-497) from the inward neighbour nb:
+//           f = f^eq(rho_b, t) + nb - f^eq(rho_n, u_n), t = u_n along the normal and
+//           the authored tangential velocity (value[1], value[2]) across it;
+//   rule 1  reconstruct (:343-406, "legacy_moment") of a velocity port from the cell's
+//           own populations, value = v0 = u - a/2;
+//   rule 2  the same for a pressure port, value = (rho_b, tangential v0).
+// The moment rule replaces the unknown populations (c_axis = sign) by the bounce-back
+// proposal f_opp + 6 w rho (c.v) corrected through the right inverse of (1, c_t0, c_t1)
+// over the unknowns, which for D3Q19 is (1/5, c_t0 / 2, c_t1 / 2).  Sums run in
+// ascending q, as the reference's partitions list them, and the equilibrium is ports.rs's
+// rho w (1 + 3 cu + 4.5 cu^2 - 1.5 u^2), so the FP64 result is the reference's.
+// ---------------------------------------------------------------------------------
+constexpr int ST_PORT_BASE = 8;
+
+template <class C>
+LBM_DEV C st_dot_c(int q, const C u[3]) {
+    C acc = C(0);
+    bool first = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int c = cvec<19>(q, d);
+        if (c == 0) continue;
+        const C term = c > 0 ? u[d] : -u[d];
+        acc = first ? term : acc + term;
+        first = false;
+    }
+    return acc;
+}
+
+template <class C>
+LBM_DEV void st_moments(const C* f, C& rho, C m[3]) {
+    C r = f[0];
+#pragma unroll
+    for (int q = 1; q < 19; q++) r += f[q];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        C acc = C(0);
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            if (cvec<19>(q, d) == 1) acc += f[q];
+            if (cvec<19>(q, d) == -1) acc -= f[q];
+        }
+        m[d] = acc;
+    }
+    rho = r;
+}
+
+template <class C>
+LBM_DEV void st_equilibrium(C rho, const C u[3], C* eq) {
+    const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const C cu = st_dot_c(q, u);
+        eq[q] = rho * C(weight<19>(q)) * (C(1) + cu * C(3.0) + cu * cu * C(4.5) - usq * C(1.5));
+    }
+}
+
+template <class C>
+LBM_DEV void st_port(const C* f, int kind, const double* value, C* out) {
+    const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
+    const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
+    const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
+    if (rule == 0) {
+        C rho, m[3], vel[3], target[3], eb[19], el[19];
+        st_moments(f, rho, m);
+#pragma unroll
+        for (int d = 0; d < 3; d++) vel[d] = qdiv(m[d], rho);
+#pragma unroll
+        for (int d = 0; d < 3; d++) target[d] = d == axis ? vel[d] : C(d == t0 ? value[1] : value[2]);
+        st_equilibrium(C(value[0]), target, eb);
+        st_equilibrium(rho, vel, el);
+#pragma unroll
+        for (int q = 0; q < 19; q++) out[q] = eb[q] + f[q] - el[q];
+        return;
+    }
+    // compatibility: zero set (the rest population first) plus twice the outgoing set
+    C compat = f[0], out_sum = C(0);
+    bool first = true;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c == 0) compat += f[q];
+        if (c == -sign) {
+            out_sum = first ? f[q] : out_sum + f[q];
+            first = false;
+        }
+    }
+    compat += out_sum * C(2.0);
+    const double sgn = double(sign);
+    C rho, v[3];
+    if (rule == 1) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = C(value[d]);
+        rho = compat / C(1.0 - sgn * value[axis]);
+    } else {
+        rho = C(value[0]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[d] = C(d == t0 ? value[1] : value[2]);
+        v[axis] = (C(1) - compat / rho) * C(sgn);
+    }
+    C known[19], km[3], kmass, prop[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        const bool unknown = c == sign;
+        known[q] = unknown ? C(0) : f[q];
+        prop[q] = unknown ? f[opp<19>(q)] + rho * C(6.0 * weight<19>(q)) * st_dot_c(q, v) : C(0);
+    }
+    st_moments(known, kmass, km);
+    const C target0 = rho - kmass;
+    const C target1 = rho * v[t0] - km[t0], target2 = rho * v[t1] - km[t1];
+    C pa0 = C(0), pa1 = C(0), pa2 = C(0);
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c != sign) continue;
+        pa0 += prop[q];
+        pa1 += prop[q] * C(double(cvec<19>(q, t0)));
+        pa2 += prop[q] * C(double(cvec<19>(q, t1)));
+    }
+    const C r0 = target0 - pa0, r1 = target1 - pa1, r2 = target2 - pa2;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c != sign) {
+            out[q] = f[q];
+            continue;
+        }
+        const C corr = r0 * C(0.2) + r1 * C(0.5 * cvec<19>(q, t0)) + r2 * C(0.5 * cvec<19>(q, t1));
+        out[q] = prop[q] + corr;
+    }
+}
+
+// Transpose of a stationary rule, sum_r J[r][k] g[r] added to acc[k] in ascending r
+// (apply_ports_vjp, ports.rs:582-621), with J the cell Jacobian formed column by column
+// on dual numbers (cell_jacobian, :532-557): at the source for the neighbour rule, at
+// the lattice weights for the affine moment rules (CompiledPort::new, :514-520).
+template <class C>
+LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g, C* acc) {
+    const bool affine = (kind - ST_PORT_BASE) / 6 != 0;
+#pragma unroll 1
+    for (int k = 0; k < 19; k++) {
+        Dual x[19], y[19];
+#pragma unroll
+        for (int q = 0; q < 19; q++) x[q] = Dual(affine ? weight<19>(q) : double(src[q]), q == k ? 1.0 : 0.0);
+        st_port(x, kind, value, y);
+        double a = 0.0;
+        static_for<0, 19>([&](auto K) {
+            constexpr int kk = decltype(K)::value;
+            if (k == kk) a = double(acc[kk]);
+        });
+#pragma unroll
+        for (int r = 0; r < 19; r++)
+            if (double(g[r]) != 0.0) a += y[r].d * double(g[r]);
+        static_for<0, 19>([&](auto K) {
+            constexpr int kk = decltype(K)::value;
+            if (k == kk) acc[kk] = C(a);
+        });
+    }
+}
+
 template <int Q, class C>
 LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out) {
+    if constexpr (Q == 19) {
+        if (kind >= ST_PORT_BASE) {
+            st_port(nb, kind, value, out);
+            return;
+        }
+    }
     C rho_n, j[3], u_n[3];
     moments<Q>(nb, rho_n, j);
 #pragma unroll
@@ -1619,7 +1790,23 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
     });
 #pragma unroll
     for (int q = 0; q < Q; q++) fnb[q] = buf[(long long)q * A.np + r];
-    port_populations_vjp<Q, C>(fnb, A.kind[r], A.value + 3 * r, gx, nbb);
+    const int kind = A.kind[r];
+    if constexpr (Q == 19) {
+        if (kind >= ST_PORT_BASE) {
+            // the moment rules read the port cell itself: its cotangent is J^T g alone
+            const bool self = A.nb[r] == A.cell[r];
+#pragma unroll
+            for (int q = 0; q < Q; q++) nbb[q] = self ? C(0) : gn[q];
+            st_port_vjp(fnb, kind, A.value + 3 * r, gx, nbb);
+            static_for<0, Q>([&](auto I) {
+                constexpr int q = decltype(I)::value;
+                if (!self) L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0), sc);
+                L::template st<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), nbb[q], sc);
+            });
+            return;
+        }
+    }
+    port_populations_vjp<Q, C>(fnb, kind, A.value + 3 * r, gx, nbb);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
         L::template st<C>(lam + read_loc<Q, q>(A.g, cn, mn, A.odd), gn[q] + nbb[q], sc);

@@ -76,6 +76,7 @@ class _Port(ctypes.Structure):
     _fields_ = [
         ("face", ctypes.c_uint32), ("kind", ctypes.c_uint32), ("value", ctypes.c_double * 3),
         ("profile", ctypes.c_void_p), ("profile_len", ctypes.c_uint64),
+        ("mask", ctypes.c_void_p), ("mask_len", ctypes.c_uint64),
     ]
 
 
@@ -107,7 +108,7 @@ class _Config(ctypes.Structure):
         ("n_sponges", ctypes.c_uint32), ("reserved3", ctypes.c_uint32), ("sponges", ctypes.c_void_p),
         ("coupling", ctypes.c_uint32), ("reserved4", ctypes.c_uint32), ("psm_kappa", ctypes.c_double),
         ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
-        ("mach_limit", ctypes.c_double), ("drift_limit", ctypes.c_double),
+        ("mach_limit", ctypes.c_double), ("drift_limit", ctypes.c_double), ("tau", ctypes.c_double),
     ]
 
 
@@ -244,14 +245,27 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
+# lbm_port_kind: the moving solver's NEE rule (moving/boundary.rs:445-506), and the
+# stationary solver's reconstruct_neighbor (ports.rs:484-497) and moment reconstruction
+# ("legacy_moment", ports.rs:343-406), D3Q19.
+PORT_KIND = {"velocity": 0, "pressure": 1, "pressure_neighbor": 2, "velocity_moment": 3, "pressure_moment": 4}
+
+
 @dataclass
 class Port:
-    """Inlet/outlet face (moving/boundary.rs:126-143 PortSpec, NEE rule :445-506)."""
+    """Inlet/outlet on a face (moving/boundary.rs:126-143 PortSpec; ports.rs:56-71 Port).
+
+    value: velocity (3,) in lattice units for the velocity kinds; (rho,) for "pressure";
+    (rho, u_t0, u_t1) for the stationary pressure kinds, the authored tangential velocity
+    in ascending axis order.  profile: one factor per port cell (velocity kinds).  mask:
+    lattice-shaped booleans selecting the port's cells on the face (default: the whole
+    face)."""
 
     face: str
-    kind: str  # "velocity" | "pressure"
-    value: tuple  # velocity (3,) lattice units, or (rho,)
+    kind: str  # a key of PORT_KIND
+    value: tuple
     profile: np.ndarray | None = None
+    mask: np.ndarray | None = None
 
 
 @dataclass
@@ -333,6 +347,8 @@ class LatticeConfig:
     # drift above drift_limit (0: 1e-6)
     mach_limit: float = 0.0
     drift_limit: float = 0.0
+    # the relaxation time itself (0: 1/omega); > 0 replaces omega by 1/tau
+    tau: float = 0.0
 
 
 @dataclass
@@ -392,7 +408,9 @@ class Lattice:
         ports = (_Port * max(1, len(cfg.ports)))()
         for i, port in enumerate(cfg.ports):
             ports[i].face = FACE[port.face]
-            ports[i].kind = 0 if port.kind == "velocity" else 1
+            if port.kind not in PORT_KIND:
+                raise ContractError(f"unknown port kind {port.kind!r}")
+            ports[i].kind = PORT_KIND[port.kind]
             vals = list(port.value) + [0.0] * (3 - len(port.value))
             for d in range(3):
                 ports[i].value[d] = float(vals[d])
@@ -401,6 +419,11 @@ class Lattice:
                 self._profiles.append(prof)
                 ports[i].profile = prof.ctypes.data
                 ports[i].profile_len = prof.size
+            if port.mask is not None:
+                pm = np.ascontiguousarray(np.asarray(port.mask, bool).reshape(-1), np.uint8)
+                self._profiles.append(pm)
+                ports[i].mask = pm.ctypes.data
+                ports[i].mask_len = pm.size
         c.n_ports = len(cfg.ports)
         c.ports = ctypes.cast(ports, ctypes.c_void_p)
         c.checkpoint_budget_bytes = int(cfg.checkpoint_budget_bytes)
@@ -445,6 +468,7 @@ class Lattice:
         c.psm_drag_shape = float(cfg.psm_drag_shape)
         c.mach_limit = float(cfg.mach_limit)
         c.drift_limit = float(cfg.drift_limit)
+        c.tau = float(cfg.tau)
         c.n_sponges = len(cfg.sponges)
         c.sponges = ctypes.cast(sponges, ctypes.c_void_p)
         solid = None

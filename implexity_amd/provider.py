@@ -11,8 +11,12 @@ library doing the lattice work:
         phi_bar += (alpha_bar + alpha_bar_final) * resistance'(phi)
         gradient = DesignMap::vjp(raw, phi_bar)
 
-The problem normalisation (JSON), the admission report and the optimiser stay with the
-caller.
+``LbmSensitivities.from_problem`` builds the chain from the reference's JSON problem
+through LbmProblem::normalize (implexity_amd.problem): the lattice (D3Q19 BGK-Guo, the
+problem's ports), the design map, the step count, the response scales and the resistance,
+with admission (LbmProblem::admit, solver.rs:707-793) at the problem's Mach limit, so a
+trajectory the reference refuses raises ConvergenceError here too.  The optimiser stays
+with the caller.
 """
 from __future__ import annotations
 
@@ -20,6 +24,7 @@ import numpy as np
 
 from . import design
 from .lbm import DesignMap, Lattice, LatticeConfig, LbmError
+from .problem import LbmProblem, normalize
 
 RESPONSES = ("mean_velocity_x_m_s", "mean_velocity_y_m_s", "mean_velocity_z_m_s", "kinetic_energy_J",
              "porous_dissipation_W", "fluid_volume_m3")
@@ -38,8 +43,26 @@ class LbmSensitivities:
         self.alpha_max, self.q = float(alpha_max), float(q)
         self.lattice = Lattice(cfg)
 
+    @classmethod
+    def from_problem(cls, problem, precision="f64", **config):
+        """From a JSON problem dict (normalised here) or a normalised LbmProblem, as
+        LatticeBoltzmannProvider::parts does (provider.rs:249-262); `config` overrides
+        fields of the lattice configuration (adjoint storage, checkpointing)."""
+        prob = problem if isinstance(problem, LbmProblem) else normalize(problem)
+        dm = prob.design_map()
+        try:
+            out = cls(prob.lattice_config(precision, **config), prob.step_count, dm, prob.vector_scales(),
+                      alpha_max=prob.alpha_max, q=prob.drag_shape)
+        except Exception:
+            dm.close()
+            raise
+        out.problem, out._owns_map = prob, True
+        return out
+
     def close(self):
         self.lattice.close()
+        if getattr(self, "_owns_map", False):
+            self.map.close()
 
     def __enter__(self):
         return self

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 5
+#define LBM_ABI_VERSION 6
 
 /* Status codes, one per reference error category (crates/implexity-core/src/error.rs:34-60,
  * crates/implexity-ad/src/error.rs). */
@@ -87,19 +87,42 @@ typedef enum {
 
 typedef enum { LBM_FACE_XMIN = 0, LBM_FACE_XMAX, LBM_FACE_YMIN, LBM_FACE_YMAX, LBM_FACE_ZMIN, LBM_FACE_ZMAX } lbm_face;
 
-typedef enum { LBM_PORT_VELOCITY = 0, LBM_PORT_PRESSURE = 1 } lbm_port_kind;
+typedef enum {
+    LBM_PORT_VELOCITY = 0,          /* NEE velocity port (moving/boundary.rs:445-506)          */
+    LBM_PORT_PRESSURE = 1,          /* NEE pressure port, u_b = u_nb (boundary.rs:445-506)     */
+    LBM_PORT_PRESSURE_NEIGHBOR = 2, /* ports.rs reconstruct_neighbor (:484-497), D3Q19         */
+    LBM_PORT_VELOCITY_MOMENT = 3,   /* ports.rs reconstruct, "legacy_moment" (:343-406), D3Q19 */
+    LBM_PORT_PRESSURE_MOMENT = 4    /* the same with a given density                          */
+} lbm_port_kind;
 
-/* One inlet/outlet face, non-equilibrium-extrapolation rule of moving/boundary.rs:
- * 445-506 applied after streaming.  Port cells are the non-solid cells of the face in
- * ascending cell order (boundary.rs:331-333); each takes NEE populations from its
- * inward neighbour.  Velocity ports: rho_b = rho_nb, u_b = value * profile[r].
- * Pressure ports: rho_b = value[0], u_b = u_nb (zero-gradient outlet). */
+/* One inlet/outlet on a face, applied after streaming.  Port cells are the non-solid
+ * cells of the face (of `mask` when given) in ascending cell order (boundary.rs:331-333,
+ * ports.rs:200).
+ *   VELOCITY, PRESSURE: the moving solver's NEE rule from the inward neighbour.
+ *     Velocity: rho_b = rho_nb, u_b = value * profile[r].  Pressure: rho_b = value[0],
+ *     u_b = u_nb (zero-gradient outlet).
+ *   The stationary solver's rules (ports.rs, D3Q19 only, no temperature field):
+ *   PRESSURE_NEIGHBOR (ports.rs:484-497): from the inward neighbour,
+ *     f = f^eq(value[0], t) + f_nb - f^eq(rho_nb, u_nb), t = u_nb along the normal and
+ *     (value[1], value[2]) across it (the tangential axes in ascending order).
+ *   VELOCITY_MOMENT (ports.rs:343-406): from the port cell's own populations, the
+ *     unknown populations from bounce-back of the non-equilibrium part corrected to
+ *     density rho = (zero + 2 outgoing sums) / (1 - sign v0_n) and momentum rho v0,
+ *     v0 = value * profile[r] - a/2 with a the lattice acceleration.
+ *   PRESSURE_MOMENT: the same with rho = value[0] given, the normal velocity from it and
+ *     tangential v0 = (value[1], value[2]) - a/2.
+ * A neighbour rule's inward neighbour must be fluid, not a port cell, and not shared with
+ * another port cell.  The reference's authoring checks (ports.rs:107-262: no corners on
+ * closed faces, protected fluid, Mach and density limits) are the caller's. */
 typedef struct {
     uint32_t face;          /* lbm_face */
     uint32_t kind;          /* lbm_port_kind */
-    double value[3];        /* velocity (lattice) or density in value[0] */
-    const double* profile;  /* one factor per port cell, or NULL for 1 */
+    double value[3];        /* see lbm_port_kind */
+    const double* profile;  /* one factor per port cell (velocity kinds), or NULL for 1 */
     uint64_t profile_len;
+    const uint8_t* mask;    /* cells x (lattice order) nonzero on this port's cells, all on
+                               the face and fluid; NULL: every non-solid cell of the face */
+    uint64_t mask_len;      /* cells, or 0 with mask NULL */
 } lbm_port;
 
 /* Storage of the adjoint cotangent field (lambda).  AUTO: FP64 for FP64 storage, FP16
@@ -234,6 +257,10 @@ typedef struct {
      * still reports the quantities). */
     double mach_limit;            /* 0, or in (0, 0.2] */
     double drift_limit;           /* >= 0; 0: 1e-6 */
+    /* The relaxation time itself (solver.rs:487, tau = 1/2 + 3 nu dt / dx^2), for callers
+     * that hold tau rather than omega: > 0 replaces omega by 1/tau and BGK-Guo divides by
+     * this tau as given; 0: tau = 1/omega. */
+    double tau;
 } lbm_config;
 
 /* Objective J = sum over states s in [t_begin, t_end] of weight * phi(s)

@@ -180,7 +180,11 @@ typedef struct {
     double theta;
     double *occ_d1_bar;
     double *occ_m1_bar;
+    double tau; /* relaxation time tau+ as given (solver.rs:487); 0: 1/omega */
 } orc_problem;
+
+/* tau+ of the problem: as given, else 1/omega (solver.rs:487 states tau, the moving solver omega). */
+static double relax_tau(const orc_problem *p) { return p->tau > 0.0 ? p->tau : 1.0 / p->omega; }
 
 static inline int is_solid(const orc_problem *p, long x) { return p->solid ? p->solid[x] != 0 : 0; }
 
@@ -1064,7 +1068,7 @@ static void brinkman_macro(const lattice_t *L, const double *f, double alpha, co
 static void cell_collide_rate(const orc_problem *p, const double *f, double alpha, double rate, double *out) {
     const lattice_t *L = lat(p->Q);
     if (p->kind == K_BGK_GUO) {
-        collide_bgk_guo(f, alpha, p->accel, 1.0 / p->omega, out);
+        collide_bgk_guo(f, alpha, p->accel, relax_tau(p), out);
         return;
     }
     double rho, j[3], den, sv[3], u[3], F[3];
@@ -1098,7 +1102,7 @@ static void cell_vjp_rate(const orc_problem *p, const double *f, double alpha, d
                           double *f_bar, double *alpha_bar_out, double *rate_bar) {
     const lattice_t *L = lat(p->Q);
     if (p->kind == K_BGK_GUO) {
-        collide_bgk_guo_vjp(f, alpha, p->accel, 1.0 / p->omega, g, f_bar, alpha_bar_out);
+        collide_bgk_guo_vjp(f, alpha, p->accel, relax_tau(p), g, f_bar, alpha_bar_out);
         return;
     }
     int Q = p->Q;
@@ -1312,7 +1316,7 @@ static void psm_cell(const orc_problem *p, const double *f, double d, const doub
     brinkman_macro(L, f, 0.0, p->accel, &rho, j, &den, sv, u, F); /* u = j/rho + a/2, F = rho a */
     double fluid[MAXQ], eq[MAXQ];
     cell_collide_rate(p, f, 0.0, -1.0, fluid);
-    double t = 1.0 / p->omega - 0.5;
+    double t = relax_tau(p) - 0.5;
     double b_over_d, beta_d, beta_t;
     psm_beta(d, t, p->psm_kappa, &b_over_d, &beta_d, &beta_t);
     double bb = b_over_d * d;
@@ -1347,7 +1351,7 @@ static void psm_cell_vjp(const orc_problem *p, const double *f, double d, const 
     brinkman_macro(L, f, 0.0, p->accel, &rho, j, &den, sv, u, F);
     double fluid[MAXQ], eq[MAXQ], eq_bar[MAXQ], fluid_bar[MAXQ];
     cell_collide_rate(p, f, 0.0, -1.0, fluid);
-    double t = 1.0 / p->omega - 0.5;
+    double t = relax_tau(p) - 0.5;
     double b_over_d, beta_d, beta_t;
     psm_beta(d, t, p->psm_kappa, &b_over_d, &beta_d, &beta_t);
     double bb = b_over_d * d;
@@ -1514,6 +1518,220 @@ void orc_port_populations_vjp(int Q, const double *nb, int kind, const double va
 }
 
 /* ------------------------------------------------------------------------- */
+/* Stationary D3Q19 ports (ports.rs): the neighbour non-equilibrium pressure     */
+/* rule reconstruct_neighbor (:484-497) and the moment reconstruction           */
+/* `reconstruct` of "legacy_moment" ports (:317-406).  Each rule is written once */
+/* on dual numbers: the value part is the FP64 rule in the reference's operation */
+/* order, and the tangent of input direction k is column k of the cell Jacobian, */
+/* the way CompiledPort::cell_jacobian (:532-557) forms it with Dual<19>.        */
+/*                                                                               */
+/* A port row's kind is ST_PORT_BASE + 6 * rule + face (face 0..5 = xmin, xmax,  */
+/* ymin, ymax, zmin, zmax), rule 0 = neighbour pressure, 1 = moment velocity,    */
+/* 2 = moment pressure; its three values are                                     */
+/*   rule 0: rho_b and the two tangential velocities (ascending axes),          */
+/*   rule 1: v0 = u - a/2 (ports.rs:352), three components,                       */
+/*   rule 2: rho_b and the two tangential components of v0.                      */
+/* ------------------------------------------------------------------------- */
+#define ST_PORT_BASE 8
+
+static dual st_dot_c_d(int q, const dual u[3]) {
+    /* ports.rs:285-297: the first nonzero term starts the sum */
+    dual acc = dv(0.0);
+    int first = 1;
+    for (int d = 0; d < 3; d++) {
+        if (C19[q][d] == 0) continue;
+        dual term = C19[q][d] == 1 ? u[d] : dmuls(u[d], -1.0);
+        acc = first ? term : dadd(acc, term);
+        first = 0;
+    }
+    return acc;
+}
+
+static void st_moments_d(const dual *f, dual *rho, dual m[3]) {
+    /* ports.rs:299-316 */
+    dual r = f[0];
+    for (int q = 1; q < 19; q++) r = dadd(r, f[q]);
+    for (int d = 0; d < 3; d++) {
+        dual acc = dv(0.0);
+        for (int q = 0; q < 19; q++) {
+            if (C19[q][d] == 1) acc = dadd(acc, f[q]);
+            else if (C19[q][d] == -1) acc = dsub(acc, f[q]);
+        }
+        m[d] = acc;
+    }
+    *rho = r;
+}
+
+static void st_equilibrium_d(dual rho, const dual u[3], dual *eq) {
+    /* ports.rs:276-282: rho W (1 + 3 cu + 4.5 cu^2 - 1.5 u^2) */
+    const dual usq = dadd(dadd(dmul(u[0], u[0]), dmul(u[1], u[1])), dmul(u[2], u[2]));
+    for (int q = 0; q < 19; q++) {
+        const dual cu = st_dot_c_d(q, u);
+        dual b = dadd(dadd(dv(1.0), dmuls(cu, 3.0)), dmuls(dmul(cu, cu), 4.5));
+        b = dsub(b, dmuls(usq, 1.5));
+        eq[q] = dmul(dmuls(rho, L19.w[q]), b);
+    }
+}
+
+/* right_inverse (ports.rs:325-341) of the unknown set: rows = (1, c_t0, c_t1) over
+ * the unknowns, m = rows rows^T, ri[k][j] = sum_i rows[i][k] inv(m)[i][j].  For D3Q19
+ * m is diagonal (the unknowns of a face are symmetric in each tangential axis), so the
+ * LU solve of linalg3::inv returns inv = diag(1 / m_ii), which is what is formed here;
+ * a non-diagonal m is refused. */
+static int st_right_inverse(const int *unknown, int nu, const int tangent[2], double ri[][3]) {
+    double rows[3][9], m[3][3], inv[3][3];
+    for (int k = 0; k < nu; k++) {
+        rows[0][k] = 1.0;
+        rows[1][k] = (double)C19[unknown[k]][tangent[0]];
+        rows[2][k] = (double)C19[unknown[k]][tangent[1]];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < nu; k++) acc += rows[i][k] * rows[j][k];
+            m[i][j] = acc;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            if (i != j && m[i][j] != 0.0) return -1;
+            inv[i][j] = i == j ? 1.0 / m[i][i] : 0.0;
+        }
+    for (int k = 0; k < nu; k++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0.0;
+            for (int i = 0; i < 3; i++) acc += rows[i][k] * inv[i][j];
+            ri[k][j] = acc;
+        }
+    return 0;
+}
+
+/* reconstruct (ports.rs:343-406) of the cell's own populations f on face `face`;
+ * density_lattice given (pressure, rule 2) or not (velocity, rule 1). */
+static void st_reconstruct_d(const dual *f, int face, const double v0in[3], int pressure, double density,
+                             dual *out) {
+    const int axis = face / 2, sign = face % 2 == 0 ? 1 : -1;
+    int unknown[9], outgoing[9], zero[9], nu = 0, no = 0, nz = 0;
+    for (int q = 0; q < 19; q++) {
+        if (C19[q][axis] == sign) unknown[nu++] = q;
+        else if (C19[q][axis] == -sign) outgoing[no++] = q;
+        else zero[nz++] = q;
+    }
+    int tangent[2], nt = 0;
+    for (int a = 0; a < 3; a++)
+        if (a != axis) tangent[nt++] = a;
+    dual compat = f[zero[0]];
+    for (int k = 1; k < nz; k++) compat = dadd(compat, f[zero[k]]);
+    dual out_sum = f[outgoing[0]];
+    for (int k = 1; k < no; k++) out_sum = dadd(out_sum, f[outgoing[k]]);
+    compat = dadd(compat, dmuls(out_sum, 2.0));
+    const double sgn = (double)sign;
+    dual rho, v[3];
+    for (int d = 0; d < 3; d++) v[d] = dv(v0in[d]);
+    if (!pressure) {
+        rho = ddivs(compat, 1.0 - sgn * v0in[axis]);
+    } else {
+        rho = dv(density);
+        v[axis] = dmuls(dsub(dv(1.0), ddiv(compat, rho)), sgn);
+    }
+    dual proposal[9];
+    for (int k = 0; k < nu; k++) {
+        const int q = unknown[k];
+        proposal[k] = dadd(f[L19.opp[q]], dmul(dmuls(rho, 6.0 * L19.w[q]), st_dot_c_d(q, v)));
+    }
+    dual known[19], km[3], kmass;
+    for (int q = 0; q < 19; q++) known[q] = f[q];
+    for (int k = 0; k < nu; k++) known[unknown[k]] = dv(0.0);
+    st_moments_d(known, &kmass, km);
+    const dual target[3] = {dsub(rho, kmass), dsub(dmul(rho, v[tangent[0]]), km[tangent[0]]),
+                            dsub(dmul(rho, v[tangent[1]]), km[tangent[1]])};
+    dual pa[3] = {dv(0.0), dv(0.0), dv(0.0)};
+    for (int k = 0; k < nu; k++) {
+        const int q = unknown[k];
+        pa[0] = dadd(pa[0], proposal[k]);
+        pa[1] = dadd(pa[1], dmuls(proposal[k], (double)C19[q][tangent[0]]));
+        pa[2] = dadd(pa[2], dmuls(proposal[k], (double)C19[q][tangent[1]]));
+    }
+    const dual residual[3] = {dsub(target[0], pa[0]), dsub(target[1], pa[1]), dsub(target[2], pa[2])};
+    double ri[9][3];
+    st_right_inverse(unknown, nu, tangent, ri);
+    for (int q = 0; q < 19; q++) out[q] = f[q];
+    for (int k = 0; k < nu; k++) {
+        dual corr = dadd(dadd(dmuls(residual[0], ri[k][0]), dmuls(residual[1], ri[k][1])),
+                         dmuls(residual[2], ri[k][2]));
+        out[unknown[k]] = dadd(proposal[k], corr);
+    }
+}
+
+/* reconstruct_neighbor (ports.rs:484-497) from the inward neighbour's populations. */
+static void st_neighbor_d(const dual *nb, int face, double density, const double tang[2], dual *out) {
+    const int axis = face / 2;
+    dual rho, m[3], vel[3], target[3], eb[19], el[19];
+    st_moments_d(nb, &rho, m);
+    for (int d = 0; d < 3; d++) vel[d] = ddiv(m[d], rho);
+    for (int d = 0, t = 0; d < 3; d++) target[d] = d == axis ? vel[d] : dv(tang[t++]);
+    st_equilibrium_d(dv(density), target, eb);
+    st_equilibrium_d(rho, vel, el);
+    for (int q = 0; q < 19; q++) out[q] = dsub(dadd(eb[q], nb[q]), el[q]);
+}
+
+static void st_port_d(const dual *src, int kind, const double value[3], dual *out) {
+    const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
+    const int axis = face / 2;
+    if (rule == 0) {
+        st_neighbor_d(src, face, value[0], value + 1, out);
+        return;
+    }
+    double v0[3];
+    if (rule == 1) {
+        for (int d = 0; d < 3; d++) v0[d] = value[d];
+    } else {
+        for (int d = 0, t = 1; d < 3; d++) v0[d] = d == axis ? 0.0 : value[t++];
+    }
+    st_reconstruct_d(src, face, v0, rule == 2, value[0], out);
+}
+
+/* The stationary rule of row kind `kind` on its source cell (the neighbour for rule 0,
+ * the port cell itself for the moment rules). */
+void orc_stationary_port(const double *src, int kind, const double value[3], double *out) {
+    init_lattices();
+    dual x[19], y[19];
+    for (int q = 0; q < 19; q++) x[q] = dv(src[q]);
+    st_port_d(x, kind, value, y);
+    for (int q = 0; q < 19; q++) out[q] = y[q].v;
+}
+
+/* Cell Jacobian jac[r * 19 + k] = d out_r / d src_k (cell_jacobian, ports.rs:532-557).
+ * The moment rules are affine, so their Jacobian is taken once at the lattice weights,
+ * as CompiledPort::new does (:514-520); the neighbour rule at the source itself. */
+void orc_stationary_port_jacobian(const double *src, int kind, const double value[3], double *jac) {
+    init_lattices();
+    const int rule = (kind - ST_PORT_BASE) / 6;
+    dual x[19], y[19];
+    for (int k = 0; k < 19; k++) {
+        for (int q = 0; q < 19; q++) {
+            x[q].v = rule == 0 ? src[q] : L19.w[q];
+            x[q].d = q == k ? 1.0 : 0.0;
+        }
+        st_port_d(x, kind, value, y);
+        for (int r = 0; r < 19; r++) jac[r * 19 + k] = y[r].d;
+    }
+}
+
+int orc_stationary_right_inverse(int face, double *ri_out) {
+    const int axis = face / 2, sign = face % 2 == 0 ? 1 : -1;
+    int unknown[9], nu = 0, tangent[2], nt = 0;
+    for (int q = 0; q < 19; q++)
+        if (C19[q][axis] == sign) unknown[nu++] = q;
+    for (int a = 0; a < 3; a++)
+        if (a != axis) tangent[nt++] = a;
+    double ri[9][3];
+    if (st_right_inverse(unknown, nu, tangent, ri) != 0) return -1;
+    for (int k = 0; k < nu; k++)
+        for (int j = 0; j < 3; j++) ri_out[k * 3 + j] = ri[k][j];
+    return nu;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Problem level: initial_populations (solver.rs:593-601), interval             */
 /* (solver.rs:604-614: collide -> stream -> ports), interval_vjp (:642-674).    */
 /* ------------------------------------------------------------------------- */
@@ -1640,7 +1858,7 @@ static void wale_omega_vjp(const orc_problem *p, const double g[3][3], double ta
 static void wale_rates(const orc_problem *p, const double *state, double *rate, double *u, double *rho) {
     long n = (long)p->nx * p->ny * p->nz;
     wale_velocity(p, state, u, rho);
-    const double tau0 = 1.0 / p->omega;
+    const double tau0 = relax_tau(p);
 #pragma omp parallel for schedule(static)
     for (long x = 0; x < n; x++) {
         if (is_solid(p, x)) {
@@ -1693,7 +1911,10 @@ static void apply_ports(const orc_problem *p, double *after, double *exchange) {
     double vals[MAXQ];
     for (int r = 0; r < p->n_port_cells; r++) {
         long x = p->port_cell[r], nb = p->port_nb[r];
-        orc_port_populations(Q, after + nb * Q, p->port_kind[r], p->port_value + 3 * r, vals);
+        if (p->port_kind[r] >= ST_PORT_BASE)
+            orc_stationary_port(after + nb * Q, p->port_kind[r], p->port_value + 3 * r, vals);
+        else
+            orc_port_populations(Q, after + nb * Q, p->port_kind[r], p->port_value + 3 * r, vals);
         double delta = 0.0;
         for (int q = 0; q < Q; q++) {
             delta += vals[q] - after[x * Q + q];
@@ -1789,6 +2010,15 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
         int nz = 0;
         for (int q = 0; q < Q; q++) nz |= g[q] != 0.0;
         if (!nz) continue;
+        if (p->port_kind[r] >= ST_PORT_BASE) {  /* apply_ports_vjp, ports.rs:582-621 */
+            double jac[19 * 19];
+            orc_stationary_port_jacobian(after_pre_port + nb * Q, p->port_kind[r], p->port_value + 3 * r, jac);
+            for (int i = 0; i < Q; i++) {
+                if (g[i] == 0.0) continue;
+                for (int k = 0; k < Q; k++) work[nb * Q + k] += jac[i * Q + k] * g[i];
+            }
+            continue;
+        }
         orc_port_populations_vjp(Q, after_pre_port + nb * Q, p->port_kind[r], p->port_value + 3 * r, g, nb_bar);
         for (int k = 0; k < Q; k++) work[nb * Q + k] += nb_bar[k];
     }
@@ -1815,7 +2045,7 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
         gbar = (double *)calloc((size_t)9 * n, sizeof(double));
         wale_rates(p, before, rate, wu, wr);
     }
-    const double tau0 = 1.0 / p->omega;
+    const double tau0 = relax_tau(p);
     if (p->psm_law >= 0 && p->occ_m_bar) memset(p->occ_m_bar, 0, sizeof(double) * 3 * n);
     if (p->psm_law >= 0 && p->occ_d1 && p->occ_d1_bar) memset(p->occ_d1_bar, 0, sizeof(double) * n);
     if (p->psm_law >= 0 && p->occ_d1 && p->occ_m1_bar) memset(p->occ_m1_bar, 0, sizeof(double) * 3 * n);

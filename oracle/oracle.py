@@ -111,7 +111,7 @@ class _Problem(ctypes.Structure):
         ("occ_m", ctypes.c_void_p), ("occ_m_bar", ctypes.c_void_p),
         ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
         ("occ_d1", ctypes.c_void_p), ("occ_m1", ctypes.c_void_p), ("theta", ctypes.c_double),
-        ("occ_d1_bar", ctypes.c_void_p), ("occ_m1_bar", ctypes.c_void_p),
+        ("occ_d1_bar", ctypes.c_void_p), ("occ_m1_bar", ctypes.c_void_p), ("tau", ctypes.c_double),
     ]
 
 PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1, "brinkman_occupancy": 2}
@@ -162,12 +162,22 @@ def psm_saturate(d: float, kappa: float) -> float:
     return lib().orc_psm_saturate(ctypes.c_double(d), ctypes.c_double(kappa))
 
 
+# Port kinds: the moving solver's NEE rule (boundary.rs:445-506) and the stationary
+# solver's rules (ports.rs): "pressure_neighbor" = reconstruct_neighbor (:484-497),
+# "velocity_moment" / "pressure_moment" = reconstruct (:343-406, "legacy_moment").
+PORT_KINDS = ("velocity", "pressure", "pressure_neighbor", "velocity_moment", "pressure_moment")
+ST_PORT_BASE = 8  # row kind of a stationary rule: ST_PORT_BASE + 6 * rule + face
+
+
 @dataclass
 class Port:
     face: str
-    kind: str  # "velocity" | "pressure"
-    value: tuple  # velocity (3,) lattice units, or (rho,)
-    profile: np.ndarray | None = None  # one factor per port cell, face order
+    kind: str  # one of PORT_KINDS
+    # velocity kinds: velocity (3,) lattice units; "pressure": (rho,); the stationary
+    # pressure kinds: (rho, u_t0, u_t1), the authored tangential velocity in ascending axes
+    value: tuple
+    profile: np.ndarray | None = None  # one factor per port cell, ascending cell order
+    mask: np.ndarray | None = None  # lattice-shaped boolean subset of the face, or all of it
 
 
 @dataclass
@@ -317,6 +327,7 @@ class Problem:
     Q: int = 19
     kind: str = "trt"
     omega: float = 1.0
+    tau: float = 0.0  # BGK-Guo relaxation time as given; 0: 1/omega
     magic: float = 3.0 / 16.0
     bulk_rate: float = float("nan")
     even_rate: float = float("nan")
@@ -367,6 +378,7 @@ class Oracle:
             p.periodic[a] = int(bool(prob.periodic[a]))
             p.accel[a] = float(prob.accel[a])
         p.omega, p.magic = float(prob.omega), float(prob.magic)
+        p.tau = float(prob.tau)
         p.bulk_rate, p.even_rate = float(prob.bulk_rate), float(prob.even_rate)
         p.n_port_cells = len(self.port_cell)
         p.port_cell = _ptr(self.port_cell)
@@ -419,9 +431,12 @@ class Oracle:
 
     def port_cells(self):
         """Port cells in the order of moving/boundary.rs:318-380: ports in list order,
-        cells of a face in ascending cell index among non-wall cells; the inward
-        neighbour is one cell along the face normal; the value is the port value times
-        the cell's profile factor (field.rs:1242-1248)."""
+        cells of a face in ascending cell index among non-wall cells (those of the
+        port's mask when it has one, ports.rs:200); the inward neighbour is one cell
+        along the face normal, or the cell itself for the moment rules, whose source is
+        the port cell (Port::sources, ports.rs:74-82); the value is the port value times
+        the cell's profile factor (field.rs:1242-1248) and, for the moment rules, less
+        half the lattice acceleration (v0, ports.rs:352)."""
         nx, ny, nz = self.prob.shape
         shape = (nx, ny, nz)
         solid = (np.zeros(self.n, bool) if self.prob.solid is None
@@ -434,14 +449,20 @@ class Oracle:
             layer = 0 if inward > 0 else shape[axis] - 1
             idx = np.arange(self.n)
             co = np.stack(self.coords(idx))
-            face = idx[(co[axis] == layer) & ~solid]
+            on_face = (co[axis] == layer) & ~solid
+            if port.mask is not None:
+                on_face &= np.asarray(port.mask, bool).reshape(-1)
+            face = idx[on_face]
             prof = port.profile
             if prof is not None and len(prof) != len(face):
                 raise ValueError("profile needs one factor per port cell")
+            fi = 2 * axis + (0 if inward > 0 else 1)
+            moment = port.kind in ("velocity_moment", "pressure_moment")
+            acc = [float(a) for a in self.prob.accel]
             for r, x in enumerate(face):
                 c = list(self.coords(int(x)))
                 c[axis] += inward
-                nb = self.index(*c)
+                nb = int(x) if moment else self.index(*c)
                 if solid[nb]:
                     raise ValueError("inward neighbour of a port cell is a wall")
                 factor = 1.0 if prof is None else float(prof[r])
@@ -450,9 +471,22 @@ class Oracle:
                 if port.kind == "velocity":
                     kinds.append(0)
                     values.append([float(v) * factor for v in port.value])
-                else:
+                elif port.kind == "pressure":
                     kinds.append(1)
                     values.append([float(port.value[0]), 0.0, 0.0])
+                elif port.kind == "pressure_neighbor":
+                    kinds.append(ST_PORT_BASE + fi)
+                    values.append([float(v) for v in port.value])
+                elif port.kind == "velocity_moment":
+                    kinds.append(ST_PORT_BASE + 6 + fi)
+                    values.append([float(port.value[d]) * factor - 0.5 * acc[d] for d in range(3)])
+                elif port.kind == "pressure_moment":
+                    kinds.append(ST_PORT_BASE + 12 + fi)
+                    tang = [d for d in range(3) if d != axis]
+                    values.append([float(port.value[0])] + [float(port.value[1 + t]) - 0.5 * acc[d]
+                                                            for t, d in enumerate(tang)])
+                else:
+                    raise ValueError(f"unknown port kind {port.kind!r}")
         return cells, nbs, kinds, values
 
     # --- cell level ---------------------------------------------------------------------
@@ -476,6 +510,33 @@ class Oracle:
         out = np.zeros(self.Q)
         lib().orc_port_populations(ctypes.c_int(self.Q), _ptr(nb), ctypes.c_int(kind), _ptr(v), _ptr(out))
         return out
+
+    @staticmethod
+    def stationary_port(src, kind, value):
+        """ports.rs rule of row kind `kind` on the source cell's 19 populations."""
+        src = np.ascontiguousarray(src, np.float64)
+        v = np.ascontiguousarray(np.asarray(value, np.float64).reshape(3))
+        out = np.zeros(19)
+        lib().orc_stationary_port(_ptr(src), ctypes.c_int(kind), _ptr(v), _ptr(out))
+        return out
+
+    @staticmethod
+    def stationary_port_jacobian(src, kind, value):
+        """cell_jacobian (ports.rs:532-557): (19, 19), [r, k] = d out_r / d src_k."""
+        src = np.ascontiguousarray(src, np.float64)
+        v = np.ascontiguousarray(np.asarray(value, np.float64).reshape(3))
+        jac = np.zeros(19 * 19)
+        lib().orc_stationary_port_jacobian(_ptr(src), ctypes.c_int(kind), _ptr(v), _ptr(jac))
+        return jac.reshape(19, 19)
+
+    @staticmethod
+    def stationary_right_inverse(face):
+        """right_inverse (ports.rs:325-341) of a face's unknown set: (n_unknown, 3)."""
+        ri = np.zeros(27)
+        nu = lib().orc_stationary_right_inverse(ctypes.c_int(face), _ptr(ri))
+        if nu < 0:
+            raise ValueError("right inverse of a non-diagonal moment matrix")
+        return ri[:3 * nu].reshape(nu, 3)
 
     def port_populations_vjp(self, nb, kind, value, g):
         nb = np.ascontiguousarray(nb, np.float64)

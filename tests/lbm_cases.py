@@ -17,8 +17,8 @@ def make_pair(shape, periodic=(False, False, False), Q=19, collision="trt", prec
     oracle.Sponge layers (copied into the library's Sponge); coupling = (law, kappa) with the
     oracle's solid momentum (cells x 3; the library's is set with set_solid_momentum) and
     drag = (c_alpha, drag shape) of the occupancy Brinkman law."""
-    gp = [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
-    op = [orc.Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile")) for p in ports]
+    gp = [Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile"), p.get("mask")) for p in ports]
+    op = [orc.Port(p["face"], p["kind"], tuple(p["value"]), p.get("profile"), p.get("mask")) for p in ports]
     gs = [Sponge(s.face, s.thickness, s.strength, s.density, tuple(s.velocity),
                  None if s.wave is None else Wave(**vars(s.wave))) for s in sponges]
     cfg = LatticeConfig(shape=shape, periodic=periodic, lattice=Q, collision=collision, precision=precision,
@@ -53,3 +53,22 @@ def channel_ports(shape, u_in=0.02, rho_out=1.0):
         {"face": "xmin", "kind": "velocity", "value": (u_in, 0.0, 0.0), "profile": prof},
         {"face": "xmax", "kind": "pressure", "value": (rho_out,)},
     ]
+
+
+def stationary_ports(shape, u_in=0.02, rho_out=0.998, tangential=(0.0, 0.0), inlet="velocity_moment",
+                     outlet="pressure_neighbor", inset=(1, 0)):
+    """Stationary-solver ports on the x faces (ports.rs): an inlet of kind `inlet` and an
+    outlet of kind `outlet`, each on the face cells at least inset[a] cells from the y and
+    z edges (the reference refuses port cells on the edges of closed faces)."""
+    nx, ny, nz = shape
+    mask_in = np.zeros(shape, bool)
+    mask_out = np.zeros(shape, bool)
+    iy, iz = inset
+    mask_in[0, iy:ny - iy, iz:nz - iz] = True
+    mask_out[-1, iy:ny - iy, iz:nz - iz] = True
+
+    def port(face, kind, mask):
+        if kind.startswith("velocity"):
+            return {"face": face, "kind": kind, "value": (u_in, 0.0, 0.0), "mask": mask}
+        return {"face": face, "kind": kind, "value": (rho_out,) + tuple(tangential), "mask": mask}
+    return [port("xmin", inlet, mask_in), port("xmax", outlet, mask_out)]

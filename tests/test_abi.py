@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lbm.lib().lbm_abi_version() == 5
+    assert lbm.lib().lbm_abi_version() == 6
 
 
 @pytest.mark.parametrize("n,s", [(1, 1), (3, 1), (10, 2), (100, 10), (1000, 30), (8000, 156)])
@@ -104,6 +104,15 @@ def _create(**kw):
     (dict(coupling="psm", thermal=True, thermal_kappa=0.02), "temperature field"),
     (dict(coupling="brinkman_occupancy", psm_c_alpha=-1.0), "c_alpha"),
     (dict(coupling="brinkman_occupancy", psm_c_alpha=0.1, psm_drag_shape=0.0), "drag shape"),
+    (dict(lattice=27, ports=[lbm.Port("xmin", "velocity_moment", (0.01, 0, 0))]), "D3Q19"),
+    (dict(ports=[lbm.Port("xmin", "velocity_moment", (0.01, 0, 0), mask=np.eye(4, dtype=bool)[None].repeat(4, 0))]),
+     "lie on its face"),
+    (dict(ports=[lbm.Port("xmin", "pressure_moment", (0.0, 0, 0))]), "density must be positive"),
+    (dict(ports=[lbm.Port("xmin", "pressure_neighbor", (1.0, 0, 0), profile=np.ones(16))]), "profile"),
+    (dict(ports=[lbm.Port("xmin", "velocity_moment", (float("nan"), 0, 0))]), "finite"),
+    (dict(ports=[lbm.Port("xmin", "velocity_moment", (0.01, 0, 0), mask=np.zeros((4, 4, 4), bool))]), "no cells"),
+    (dict(thermal=True, thermal_kappa=0.02, ports=[lbm.Port("xmin", "velocity_moment", (0.01, 0, 0))]),
+     "temperature field"),
 ])
 def test_contract_errors_before_device(kw, msg):
     """CaeError::Contract conditions (solver.rs:414-550, collision.rs:244-292) are
