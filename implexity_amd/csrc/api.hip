@@ -297,9 +297,20 @@ struct lbm_handle {
     // Second occupancy endpoint (d1, m1) blended over steps te0..te1, and its cotangents.
     DevBuf d1, sm1, d1_bar, sm1_bar;
     uint64_t te0 = 0, te1 = 1;
+    // Subcycle (lbm_subcycle): the k-th of m substeps blends at its midpoint, theta =
+    // (k - 1/2) / m (Field::params, moving/field.rs:1049); gacc accumulates the endpoints'
+    // impulse shares in the forward sweep, gbar holds their cotangents in the reverse one.
+    bool sub_mid = false, gacc_on = false, gbar_on = false;
+    DevBuf gacc, gbar;
     // Weight of the second endpoint in the step that reads state t.
     double theta(uint64_t t) const {
-        if (!d1.p || t <= te0) return 0.0;
+        if (!d1.p) return 0.0;
+        if (sub_mid) {
+            if (t < te0) return 0.0;
+            if (t >= te1) return 1.0;
+            return (double(t - te0 + 1) - 0.5) / double(te1 - te0);
+        }
+        if (t <= te0) return 0.0;
         if (t >= te1) return 1.0;
         return double(t - te0) / double(te1 - te0);
     }
@@ -331,6 +342,7 @@ struct lbm_handle {
         s.d1 = d1.p;
         s.sm1 = sm1.p;
         s.theta = theta(step);
+        s.gacc = gacc_on ? gacc.as<double>() : nullptr;
         if (thermal) {  // the flow kernel advances the temperature field in the same pass
             s.tpop = tpop(static_cast<const void*>(pop));
             s.tpop_out = tpop(pop);
@@ -445,6 +457,7 @@ struct lbm_handle {
         a.sm1 = sm1.p;
         a.d1_bar = d1_bar.as<double>();
         a.sm1_bar = sm1_bar.as<double>();
+        a.gbar = gbar_on ? gbar.as<double>() : nullptr;
         a.obj.kind = OBJ_NONE;
         a.part = part.as<double>();
         a.npart = npart;
@@ -830,6 +843,7 @@ StepArgs final_state_args(lbm_handle* h, void* buf, uint64_t step) {
     StepArgs ea = h->step_args(buf, step);
     ea.part = h->part.as<double>();
     ea.diag = 1;
+    ea.gacc = nullptr;
     if (is_psm(h->kind)) ea.alpha = h->alpha0.p;
     return ea;
 }
@@ -1620,6 +1634,7 @@ int lbm_set_occupancy_end(lbm_handle* h, const double* d1, const double* m1, uin
         }
         h->te0 = t0;
         h->te1 = t1;
+        h->sub_mid = false;
     });
 }
 
@@ -2483,6 +2498,141 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         }
         check_lam_range(h);
     });
+}
+
+// ---------------------------------------------------------------------------------
+// Subcycled occupancy (SubcycledField on the moving field, moving/field.rs:1870-1919 and
+// reverse/prepare :2055-2392): m fluid substeps across one macro step with the occupancy
+// blended between its endpoints at theta_k = (k - 1/2) / m, the endpoints' impulse shares
+// g0, g1 accumulated per cell, and the transpose by an inner binomial schedule.
+// ---------------------------------------------------------------------------------
+namespace {
+void subcycle_check(const lbm_handle* h, uint32_t substeps) {
+    if (!is_psm(h->kind)) fail(LBM_ERR_CONTRACT, "a subcycle needs the occupancy coupling");
+    if (!h->d1.p) fail(LBM_ERR_CONTRACT, "a subcycle needs the second occupancy endpoint (lbm_set_occupancy_end)");
+    if (substeps == 0) fail(LBM_ERR_CONTRACT, "a subcycle needs at least one substep");
+    if (h->thermal) fail(LBM_ERR_CONTRACT, "a subcycle covers the flow alone");
+}
+
+// Field::check_state (moving/field.rs:1735-1775): the end state's fluid populations must be
+// finite and positive and the substeps' largest speed under the Mach limit.
+void admit_moving(const lbm_handle* h, const SweepDiag& steps, const SweepDiag& end) {
+    if (h->mach_limit <= 0.0) return;
+    if (end.bad_cells != 0) {
+        char msg[200];
+        std::snprintf(msg, sizeof(msg),
+                      "moving LBM admission refused (populations): %llu cell states with a non-finite or "
+                      "non-positive population; minimum %.3e",
+                      (unsigned long long)end.bad_cells, end.min_population);
+        fail(LBM_ERR_CONVERGENCE, msg);
+    }
+    if (steps.max_mach > h->mach_limit) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "moving LBM admission refused (mach): Mach number %.4f above the limit %g",
+                      steps.max_mach, h->mach_limit);
+        fail(LBM_ERR_CONVERGENCE, msg);
+    }
+}
+}  // namespace
+
+int lbm_subcycle(lbm_handle* h, uint32_t substeps, double* g, uint64_t g_len, lbm_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        subcycle_check(h, substeps);
+        const long long nc = h->ncell;
+        if (g && g_len != uint64_t(nc) * 6) fail(LBM_ERR_SHAPE, "g needs 6 entries per cell");
+        const auto t0 = std::chrono::steady_clock::now();
+        h->gacc.grow(size_t(nc) * 6 * sizeof(double), "subcycle impulse shares");
+        hip_check(hipMemsetAsync(h->gacc.p, 0, size_t(nc) * 6 * sizeof(double), h->stream), "memset");
+        h->te0 = h->t;
+        h->te1 = h->t + substeps;
+        h->sub_mid = true;
+        reset_scalars(h);
+        MassLedger led;
+        led.init(h, substeps);
+        hip_check(hipEventRecord(h->ev0, h->stream), "event");
+        h->gacc_on = true;
+        try {
+            for (uint32_t s = 0; s < substeps; s++) {
+                step_once(h, h->F.p, h->t, ObjArgs{}, &led, s);
+                h->t++;
+            }
+        } catch (...) {
+            h->gacc_on = false;
+            throw;
+        }
+        h->gacc_on = false;
+        hip_check(hipEventRecord(h->ev1, h->stream), "event");
+        const SweepDiag ds = read_diag(h);  // states 0 .. m-1: the substeps' speeds
+        // the end state's populations
+        hip_check(hipMemsetAsync(h->scal.as<double>() + 4, 0, 3 * sizeof(unsigned), h->stream), "memset");
+        StepArgs ea = final_state_args(h, h->F.p, h->t);
+        ea.macc = led.mass(substeps);
+        hip_check(h->run->objective(h->Q, h->has_mask, ea, ObjArgs{}, nullptr, nullptr, 0, h->cfg()),
+                  "objective launch");
+        const SweepDiag de = read_diag(h);
+        if (g) {
+            std::vector<double> soa(size_t(nc) * 6);
+            h->stage.download(soa.data(), h->gacc.p, soa.size() * sizeof(double), h->stream);
+            for (long long x = 0; x < nc; x++)
+                for (int e = 0; e < 6; e++) g[size_t(x) * 6 + e] = soa[size_t(e) * nc + x];
+        }
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->forward_steps = substeps;
+            stats->forward_ms = elapsed(h);
+            stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            stats->max_mach = ds.max_mach;
+            stats->bad_cells = de.bad_cells;
+            stats->min_population = de.min_population;
+            stats->mass_drift = led.drift();
+            stats->forward_kernel_avg_ms = NAN;
+            stats->adjoint_kernel_avg_ms = NAN;
+            stats->cotangent_max = NAN;
+        }
+        admit_moving(h, ds, de);
+    });
+}
+
+int lbm_subcycle_adjoint(lbm_handle* h, uint32_t substeps, const double* state_bar, uint64_t state_len,
+                         const double* g_bar, uint64_t g_len, double* previous_bar, double* d0_bar, uint64_t n,
+                         lbm_stats* stats) {
+    int rc = guarded([&] {
+        if (!h) fail(LBM_ERR_CONTRACT, "null handle");
+        subcycle_check(h, substeps);
+        // The D3Q27 transpose took a GPU memory fault whose cause is not established; it
+        // stays refused until it is.
+        if (h->Q != 19) fail(LBM_ERR_CONTRACT, "lbm_subcycle_adjoint covers D3Q19 lattices");
+        const long long nc = h->ncell;
+        if (!state_bar || !previous_bar) fail(LBM_ERR_CONTRACT, "null argument");
+        if (state_len != uint64_t(nc) * h->Q) fail(LBM_ERR_SHAPE, "state_bar length must be cells*Q");
+        if (g_bar && g_len != uint64_t(nc) * 6) fail(LBM_ERR_SHAPE, "g_bar needs 6 entries per cell");
+        if (d0_bar && n != uint64_t(nc)) fail(LBM_ERR_SHAPE, "d0_bar needs one entry per cell");
+        h->te0 = h->t;
+        h->te1 = h->t + substeps;
+        h->sub_mid = true;
+        h->gbar_on = false;
+        if (g_bar) {
+            std::vector<double> soa(size_t(nc) * 6);
+            for (long long x = 0; x < nc; x++)
+                for (int e = 0; e < 6; e++) {
+                    const double v = g_bar[size_t(x) * 6 + e];
+                    if (!std::isfinite(v)) fail(LBM_ERR_CONTRACT, "g_bar must be finite");
+                    soa[size_t(e) * nc + x] = v;
+                }
+            h->gbar.grow(soa.size() * sizeof(double), "subcycle impulse cotangents");
+            hip_check(hipMemcpy(h->gbar.p, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice),
+                      "upload");
+            h->gbar_on = true;
+        }
+    });
+    if (rc != LBM_OK) return rc;
+    // The transpose of the m substeps seeded with state_bar, through the gradient driver
+    // (its schedule is the inner one: binomial within the checkpoint budget).
+    rc = lbm_gradient_seeded(h, substeps, nullptr, state_bar, state_len, nullptr, d0_bar, n, previous_bar, state_len,
+                             stats);
+    h->gbar_on = false;
+    return rc;
 }
 
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms) {

@@ -31,7 +31,7 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
 
 hipError_t JvpRunner::step(int Q, int kind, bool mask, const StepArgs& a, dim3 grid, dim3 block,
                            hipStream_t s) const {
-    if (!a.pop || !a.pop_out || !a.alpha || (mask && !a.mask) || a.part || a.tpop) return hipErrorInvalidValue;
+    if (!a.pop || !a.pop_out || !a.alpha || (mask && !a.mask) || a.part || a.tpop || a.gacc) return hipErrorInvalidValue;
     if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return hipErrorInvalidValue;
     if (Q == 19) return mask ? step_q<19, true>(kind, a, grid, block, s) : step_q<19, false>(kind, a, grid, block, s);
     if (Q == 27) return mask ? step_q<27, true>(kind, a, grid, block, s) : step_q<27, false>(kind, a, grid, block, s);

@@ -125,6 +125,7 @@ struct StepArgs {
     const void* d1;   // occupancy coupling: second endpoint d1 (cells) and m1 (3 x cells), or null
     const void* sm1;
     double theta;     // weight of the second endpoint in this step
+    double* gacc;     // subcycle: impulse shares g0, g1 of the endpoints (6 x cells, SoA, FP64), or null
     ObjArgs obj;      // objective of the state being read (kind 0: none)
     double* part;     // per-block partials [J, mass, max |u|^2, bad cells, max -f], PART_STRIDE per block
     long long npart;  // capacity of part, in blocks
@@ -174,6 +175,8 @@ struct AdjArgs {
     double theta;
     double* d1_bar;
     double* sm1_bar;
+    // Subcycle: cotangents of the impulse shares g0, g1 (as StepArgs::gacc), or null.
+    const double* gbar;
 };
 
 // 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
@@ -1073,6 +1076,10 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                         jsum += A.obj.weight * (A.obj.vw[0] * double(mac.rho * z[0]) +
                                                 A.obj.vw[1] * double(mac.rho * z[1]) +
                                                 A.obj.vw[2] * double(mac.rho * z[2]));
+                    if (A.gacc) {
+                        const C dp[3] = {mac.rho * z[0], mac.rho * z[1], mac.rho * z[2]};
+                        sub_accumulate<C>(A.gacc, (long long)g.nx * g.ny * g.nz, cell, dp, dc, A.theta);
+                    }
                 } else {
                     collide<Q, psm_base(KIND), C, S::shifted>(f, C(0), a, rate, C(A.om), C(A.tau), pk, fs, mac);
                 }
@@ -1087,6 +1094,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                     if (A.obj.kind == OBJ_SOLID_FORCE)
                         jsum += A.obj.weight * (A.obj.vw[0] * double(dp[0]) + A.obj.vw[1] * double(dp[1]) +
                                                 A.obj.vw[2] * double(dp[2]));
+                    if (A.gacc) sub_accumulate<C>(A.gacc, (long long)g.nx * g.ny * g.nz, cell, dp, dc, A.theta);
                 }
             } else {
                 collide<Q, KIND, C, S::shifted>(f, alpha, a, rate, C(A.om), C(A.tau), static_cast<const C*>(A.mrt),
@@ -1223,10 +1231,18 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                         collide_pairs_vjp_fast<Q>(mac.rho, mac.u, mac.F, C(A.omega), om, gb, fb, cb_rho, cb_u, cb_F);
                     else
                         collide_pairs_vjp<Q>(mac.rho, mac.u, mac.F, C(A.omega), om, gb, fb, cb_rho, cb_u, cb_F);
-                    C abr, gbr, mb[3];
-                    const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    C abr, gbr, mb[3], extra = C(0);
+                    C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    if constexpr (Q == 19) {  // lbm_subcycle_adjoint is D3Q19 only
+                        if (A.gbar) {
+                            const C dp[3] = {mac.rho * z[0], mac.rho * z[1], mac.rho * z[2]};
+                            sub_cotangent<C>(A.gbar, n, cell, dp, dc, A.theta, dpb, extra);
+                        }
+                    }
                     brinkman_moving_chain<Q, C>(mac, al, ga, a, ms, z, dpb, cb_rho, cb_u, cb_F, fb, abr, gbr, mb);
-                    ab = psm_accumulate<C>(A, n, cell, abr * part[0] + gbr * part[2], mb);
+                    C db = abr * part[0] + gbr * part[2];
+                    if constexpr (Q == 19) db += extra;
+                    ab = psm_accumulate<C>(A, n, cell, db, mb);
                 } else {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     ab = C(0);
@@ -1247,9 +1263,17 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                         for (int q = 0; q < Q; q++) gfl += gb[q] * fl[q];
                     }
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
-                    C db, mb[3];
-                    const C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    C db, mb[3], extra = C(0);
+                    C dpb[3] = {psm_dp_bar<C>(A.obj, 0), psm_dp_bar<C>(A.obj, 1), psm_dp_bar<C>(A.obj, 2)};
+                    if constexpr (Q == 19) {  // lbm_subcycle_adjoint is D3Q19 only
+                        if (A.gbar) {
+                            C dp[3];
+                            psm_exchange<Q, C, S::shifted, SUP>(f, mac, dc, ms, pk, C(A.omega), dp);
+                            sub_cotangent<C>(A.gbar, n, cell, dp, dc, A.theta, dpb, extra);
+                        }
+                    }
                     psm_blend_vjp<Q, C, S::shifted, SUP>(f, mac, dc, ms, pk, C(A.omega), gb, gfl, dpb, fb, db, mb);
+                    if constexpr (Q == 19) db += extra;
                     ab = psm_accumulate<C>(A, n, cell, db, mb);
                 } else {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);

@@ -135,6 +135,65 @@ LBM_DEV void psm_blend(const C* v, const Macro<Q, C>& m, C d, const C ms[3], con
     }
 }
 
+// The momentum exchanged with the solid alone, dp = sum_i Omega_s,i c_i of psm_blend (the
+// same expressions in the same order, so the same bits), without the blended values.
+template <int Q, class C, bool SH, bool SUP>
+LBM_DEV void psm_exchange(const C* v, const Macro<Q, C>& m, C d, const C ms[3], const C* pk, C omega, C dp[3]) {
+    const C t = C(1) / omega - C(0.5);
+    C bod, bd, bt;
+    psm_beta<C>(d, t, pk[1], bod, bd, bt);
+    const C bb = bod * d;
+    const C m2 = ms[0] * ms[0] + ms[1] * ms[1] + ms[2] * ms[2];
+    const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
+    const C inv_d = SH ? C(1) / d : C(0);
+    dp[0] = dp[1] = dp[2] = C(0);
+    C eq[Q];
+    if constexpr (!SH) equilibrium<Q>(m.rho, m.u, eq);
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int o = opp<Q>(i);
+        const C cm = dotc<Q, i>(ms);
+        const C se = psm_solid_eq<Q, i, C, SH>(m, bb, bod, cm, m2, d, inv_d);
+        C os;
+        if constexpr (!SH)
+            os = SUP ? se - bb * eq[i] : bb * (v[o] - eq[o] - v[i]) + se;
+        else
+            os = SUP ? se - bb * psm_eq<Q, i, C, SH>(m, usq) : bb * (v[o] - psm_eq<Q, o, C, SH>(m, usq) - v[i]) + se;
+        if constexpr (cvec<Q>(i, 0) != 0) dp[0] += os * C(cvec<Q>(i, 0));
+        if constexpr (cvec<Q>(i, 1) != 0) dp[1] += os * C(cvec<Q>(i, 1));
+        if constexpr (cvec<Q>(i, 2) != 0) dp[2] += os * C(cvec<Q>(i, 2));
+    });
+}
+
+// Subcycled occupancy (Field::accumulate, moving/field.rs:1499-1541): the impulse shares of
+// the two endpoints, g0 += (-dp / d)(1 - theta), g1 += (-dp / d) theta, in G (6 x cells,
+// SoA: g0 x, y, z then g1 x, y, z; FP64).  Each cell's entries belong to its own thread.
+template <class C>
+LBM_DEV void sub_accumulate(double* G, long long n, long long cell, const C dp[3], C d, double theta) {
+    const C keep = C(1.0 - theta), th = C(theta);
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const C gv = -dp[e] / d;
+        G[e * n + cell] += double(gv * keep);
+        G[(3 + e) * n + cell] += double(gv * th);
+    }
+}
+
+// Transpose of sub_accumulate at one cell (reverse_substep, field.rs:2682-2690): with the
+// blended cotangent gt = gbar0 (1 - theta) + gbar1 theta, the exchange's cotangent takes
+// -gt / d and the occupancy's gains sum_e gt dp_e / d^2, returned in extra.
+template <class C>
+LBM_DEV void sub_cotangent(const double* GB, long long n, long long cell, const C dp[3], C d, double theta,
+                           C dpb[3], C& extra) {
+    extra = C(0);
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const C gt = C(GB[e * n + cell]) * C(1.0 - theta) + C(GB[(3 + e) * n + cell]) * C(theta);
+        dpb[e] -= gt / d;
+        extra += gt * dp[e] / (d * d);
+    }
+}
+
 // One direction's share of the equilibrium transpose (cell.h equilibrium_vjp) for the
 // cotangent e of eq_k; usq_bar is folded into u_bar by psm_eq_vjp_end.
 template <int Q, int k, class C>

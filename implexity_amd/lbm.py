@@ -173,6 +173,11 @@ _SIGS = {
     "lbm_set_occupancy_end": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                ctypes.c_uint64], ctypes.c_int),
     "lbm_occupancy_end_bar": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
+    "lbm_subcycle": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
+                     ctypes.c_int),
+    "lbm_subcycle_adjoint": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                              ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p],
+                             ctypes.c_int),
     "lbm_reset_initial": ([ctypes.c_void_p], ctypes.c_int),
     "lbm_set_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "lbm_get_state": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
@@ -527,6 +532,29 @@ class Lattice:
         m = np.zeros(self.n * 3)
         _check(lib().lbm_occupancy_end_bar(self._h, _p(d), _p(m), self.n))
         return d, m.reshape(self.n, 3)
+
+    def subcycle(self, substeps):
+        """One macro step of the subcycled moving field (SubcycledField::subcycle,
+        moving/field.rs:1870-1919) from the held state between the occupancy endpoints
+        (design, solid momentum) and set_occupancy_end; returns (g, stats), g the
+        endpoints' impulse shares (cells, 2, 3)."""
+        g = np.zeros(self.n * 6)
+        st = Stats()
+        _check(lib().lbm_subcycle(self._h, int(substeps), _p(g), g.size, ctypes.byref(st)))
+        return g.reshape(self.n, 2, 3), st
+
+    def subcycle_adjoint(self, substeps, state_bar, g_bar=None):
+        """Transpose of subcycle from the held (start) state: returns (previous_bar, d0_bar,
+        m0_bar, d1_bar, m1_bar, stats) for the cotangents of the end state (AoS) and of g."""
+        sb = np.ascontiguousarray(state_bar, np.float64).reshape(-1)
+        gb = None if g_bar is None else np.ascontiguousarray(g_bar, np.float64).reshape(-1)
+        pb = np.zeros(sb.size)
+        d0 = np.zeros(self.n)
+        st = Stats()
+        _check(lib().lbm_subcycle_adjoint(self._h, int(substeps), _p(sb), sb.size, _p(gb), 0 if gb is None else gb.size,
+                                          _p(pb), _p(d0), self.n, ctypes.byref(st)))
+        d1, m1 = self.occupancy_end_bar()
+        return pb, d0, self.solid_momentum_bar(), d1, m1, st
 
     # --- coupled temperature field ------------------------------------------------------
     def set_thermal_design(self, phi):

@@ -352,6 +352,36 @@ int lbm_solid_momentum_bar(lbm_handle* h, double* out, uint64_t len);
 int lbm_set_occupancy_end(lbm_handle* h, const double* d1, const double* m1, uint64_t n, uint64_t t0, uint64_t t1);
 int lbm_occupancy_end_bar(lbm_handle* h, double* d1_bar, double* m1_bar, uint64_t n);
 
+/* One macro step of the subcycled moving field (SubcycledField::subcycle and
+ * subcycle_adjoint, crates/implexity-solve/src/multirate_coupling/mod.rs:86-166, on
+ * MovingField, moving/field.rs:1870-1919 and :2055-2392).  The handle's occupancy coupling
+ * holds the interval's endpoints: d0 = the design (lbm_set_design), m0 =
+ * lbm_set_solid_momentum, and d1, m1 = lbm_set_occupancy_end (whose t0, t1 these calls
+ * replace).  Substep k = 1 .. m collides the state with the endpoints blended at theta =
+ * (k - 1/2) / m (Field::params, field.rs:1049) and adds each occupied cell's impulse
+ * shares (Field::accumulate, :1499-1541)
+ *   g0 += (-dp / d)(1 - theta),   g1 += (-dp / d) theta,
+ * dp the momentum the cell exchanges with the solid, d the blended occupancy.
+ *
+ * lbm_subcycle advances the handle's state by m substeps and returns g as [cell][6]
+ * (g0 x, y, z, g1 x, y, z; 0 where no solid), NULL to skip.  With a Mach limit it refuses
+ * (LBM_ERR_CONVERGENCE) as Field::check_state (:1735-1775): a non-finite or non-positive
+ * population of the end state, or a substep's Mach number above the limit.
+ *
+ * lbm_subcycle_adjoint, from the state the subcycle started at (the handle's state), forms
+ * the cotangents of the end state (state_bar, AoS) and of g (g_bar, [cell][6], or NULL for
+ * 0) back to the start: previous_bar (AoS), d0_bar (one per cell, or NULL), and through
+ * lbm_solid_momentum_bar and lbm_occupancy_end_bar m0_bar, d1_bar and m1_bar.  The
+ * substeps are recomputed under the handle's checkpoint budget (the inner binomial
+ * schedule of Field::prepare, :2312-2385); the handle keeps the start state.  D3Q19
+ * only: the D3Q27 transpose is refused (LBM_ERR_CONTRACT).  The
+ * reference's observables, entrainment compensation, carrier flux and time-scale
+ * derivatives stay with the caller. */
+int lbm_subcycle(lbm_handle* h, uint32_t substeps, double* g, uint64_t g_len, lbm_stats* stats);
+int lbm_subcycle_adjoint(lbm_handle* h, uint32_t substeps, const double* state_bar, uint64_t state_len,
+                         const double* g_bar, uint64_t g_len, double* previous_bar, double* d0_bar, uint64_t n,
+                         lbm_stats* stats);
+
 /* State exchange in the reference layout: layout 0 = AoS (x*Q+q), 1 = SoA (q*N+x). */
 int lbm_set_state(lbm_handle* h, int layout, const double* f, uint64_t len);
 int lbm_get_state(lbm_handle* h, int layout, double* f, uint64_t len);

@@ -181,6 +181,12 @@ typedef struct {
     double *occ_d1_bar;
     double *occ_m1_bar;
     double tau; /* relaxation time tau+ as given (solver.rs:487); 0: 1/omega */
+    /* Subcycle (moving/field.rs:1499-1541, 2682-2690): with sub_g set, each coupled cell
+     * adds its impulse shares g0 += (-dp/d)(1 - theta), g1 += (-dp/d) theta to sub_g
+     * ([cell][6]: g0 x, y, z, g1 x, y, z); with sub_gbar set, the step transpose takes their
+     * cotangents ([cell][6]). */
+    double *sub_g;
+    const double *sub_gbar;
 } orc_problem;
 
 /* tau+ of the problem: as given, else 1/omega (solver.rs:487 states tau, the moving solver omega). */
@@ -1968,10 +1974,18 @@ void orc_interval(const orc_problem *p, const uint32_t *source, const double *al
             double dp[3];
             double d = psm_d(p, alpha, x), m[3];
             psm_mv(p, x, m);
-            if (d > 0.0)
+            if (d > 0.0) {
                 psm_cell(p, before + x * Q, d, m, post + x * Q, dp);
-            else
+                if (p->sub_g) { /* Field::accumulate, field.rs:1518-1526 */
+                    for (int e = 0; e < 3; e++) {
+                        double gv = -dp[e] / d;
+                        p->sub_g[6 * x + e] += gv * (1.0 - p->theta);
+                        p->sub_g[6 * x + 3 + e] += gv * p->theta;
+                    }
+                }
+            } else {
                 cell_collide_rate(p, before + x * Q, 0.0, -1.0, post + x * Q);
+            }
             continue;
         }
         cell_collide_rate(p, before + x * Q, alpha[x], rate ? rate[x] : -1.0, post + x * Q);
@@ -2062,6 +2076,8 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
             g[q] = post_bar[x * Q + q];
             nz |= g[q] != 0.0;
         }
+        if (p->sub_gbar && p->psm_law >= 0) /* the impulse shares' cotangents reach d even at g = 0 */
+            for (int e = 0; e < 6; e++) nz |= p->sub_gbar[6 * x + e] != 0.0;
         if (!nz) {
             for (int q = 0; q < Q; q++) f_bar[x * Q + q] = 0.0;
             continue;
@@ -2073,7 +2089,19 @@ void orc_interval_vjp(const orc_problem *p, const uint32_t *source, const double
             double d = psm_d(p, alpha, x), m[3];
             psm_mv(p, x, m);
             if (d > 0.0) {
-                psm_cell_vjp(p, before + x * Q, d, m, g, no_dp, fb, &ab, mb);
+                if (p->sub_gbar) { /* reverse_substep, field.rs:2682-2690 */
+                    double post_s[MAXQ], dp[3], dpb[3], extra = 0.0;
+                    psm_cell(p, before + x * Q, d, m, post_s, dp);
+                    for (int e = 0; e < 3; e++) {
+                        double gt = p->sub_gbar[6 * x + e] * (1.0 - p->theta) + p->sub_gbar[6 * x + 3 + e] * p->theta;
+                        dpb[e] = 0.0 - gt / d;
+                        extra += gt * dp[e] / (d * d);
+                    }
+                    psm_cell_vjp(p, before + x * Q, d, m, g, dpb, fb, &ab, mb);
+                    ab += extra;
+                } else {
+                    psm_cell_vjp(p, before + x * Q, d, m, g, no_dp, fb, &ab, mb);
+                }
             } else {
                 cell_vjp_rate(p, before + x * Q, 0.0, -1.0, g, fb, &ab, NULL);
                 ab = 0.0;

@@ -112,6 +112,7 @@ class _Problem(ctypes.Structure):
         ("psm_c_alpha", ctypes.c_double), ("psm_drag_shape", ctypes.c_double),
         ("occ_d1", ctypes.c_void_p), ("occ_m1", ctypes.c_void_p), ("theta", ctypes.c_double),
         ("occ_d1_bar", ctypes.c_void_p), ("occ_m1_bar", ctypes.c_void_p), ("tau", ctypes.c_double),
+        ("sub_g", ctypes.c_void_p), ("sub_gbar", ctypes.c_void_p),
     ]
 
 PSM_LAW = {"brinkman": -1, "psm": 0, "psm_superposition": 1, "brinkman_occupancy": 2}
@@ -624,6 +625,62 @@ class Oracle:
         lib().orc_interval_vjp(self._pp, _ptr(self.source), _ptr(alpha), _ptr(before), _ptr(after),
                                _ptr(after_bar), _ptr(work), _ptr(f_bar), _ptr(a_bar))
         return f_bar, a_bar
+
+    # --- subcycled moving field (moving/field.rs:1870-1919, 2055-2392) -----------------------
+    def _substep(self, alpha, before, t, theta):
+        self.set_time(t)
+        self._p.theta = float(theta)
+        after = np.empty_like(before)
+        post = np.empty_like(before)
+        ex = np.zeros(max(1, len(self.port_cell)))
+        lib().orc_interval(self._pp, _ptr(self.source), _ptr(alpha), _ptr(before), _ptr(post), _ptr(after), _ptr(ex))
+        return after
+
+    def subcycle(self, d0, f_prev, substeps, t0=0):
+        """SubcycledField::subcycle on the occupancy coupling: `substeps` steps from f_prev,
+        step k = 1..m blending the endpoints (d0 with the problem's solid momentum, and the
+        problem's occupancy_end) at theta = (k - 1/2) / m (Field::params, field.rs:1049),
+        with the impulse shares of Field::accumulate (:1499-1541).  Returns (states 0..m,
+        g (cells, 2, 3))."""
+        d0 = np.ascontiguousarray(d0, np.float64)
+        g = np.zeros(self.n * 6)
+        states = [np.ascontiguousarray(f_prev, np.float64).copy()]
+        self._p.sub_g = _ptr(g)
+        try:
+            for k in range(1, substeps + 1):
+                states.append(self._substep(d0, states[-1], t0 + k - 1, (k - 0.5) / substeps))
+        finally:
+            self._p.sub_g = None
+        return states, g.reshape(self.n, 2, 3)
+
+    def subcycle_adjoint(self, d0, states, substeps, state_bar, g_bar=None, t0=0):
+        """Transpose of subcycle (Field::reverse, field.rs:2055-2300, reverse_substep
+        :2482-2848) over its stored states: (previous_bar, d0_bar, m0_bar, d1_bar, m1_bar)."""
+        d0 = np.ascontiguousarray(d0, np.float64)
+        lam = np.ascontiguousarray(state_bar, np.float64).copy()
+        gb = None if g_bar is None else np.ascontiguousarray(g_bar, np.float64).reshape(-1)
+        d0b, d1b = np.zeros(self.n), np.zeros(self.n)
+        m0b, m1b = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        self._p.sub_gbar = None if gb is None else _ptr(gb)
+        try:
+            for k in range(substeps, 0, -1):
+                theta = (k - 0.5) / substeps
+                before = states[k - 1]
+                after = self._substep(d0, before, t0 + k - 1, theta)
+                work = np.empty_like(before)
+                f_bar = np.empty_like(before)
+                a_bar = np.empty(self.n)
+                lib().orc_interval_vjp(self._pp, _ptr(self.source), _ptr(d0), _ptr(before), _ptr(after), _ptr(lam),
+                                       _ptr(work), _ptr(f_bar), _ptr(a_bar))
+                lam = f_bar
+                d0b += a_bar
+                m0b += self.m_bar
+                if self.d1_bar is not None:
+                    d1b += self.d1_bar
+                    m1b += self.m1_bar
+        finally:
+            self._p.sub_gbar = None
+        return lam, d0b, m0b, d1b, m1b
 
     def history(self, alpha, steps, f0=None):
         """solver.rs:677-687 -- every state kept (use small problems)."""
