@@ -266,6 +266,7 @@ struct lbm_handle {
     double alpha_max = 0.0;  // max |alpha| over the design (bounds dissipation seeds)
     std::vector<uint8_t> solid;
     std::vector<int> h_pcell, h_pnb;
+    bool st_ports = false;  // some port rows carry a stationary rule (ports.rs)
     uint64_t t = 0;
     uint64_t budget = 0;
     size_t pool_slots = 0;
@@ -477,6 +478,7 @@ struct lbm_handle {
         p.value = pval.as<double>();
         p.mask = mask.as<uint32_t>();
         p.odd = int(step & 1);
+        p.st = st_ports ? 1 : 0;
         return p;
     }
 };
@@ -1310,6 +1312,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             if (port.kind > LBM_PORT_PRESSURE_MOMENT)
                 fail(LBM_ERR_CONTRACT, "Port kind must be velocity or pressure (NEE, neighbour or moment rule)");
             const bool stationary = port.kind >= LBM_PORT_PRESSURE_NEIGHBOR;
+            h->st_ports = h->st_ports || stationary;
             const bool moment = port.kind >= LBM_PORT_VELOCITY_MOMENT;
             const bool pressure = port.kind == LBM_PORT_PRESSURE || port.kind == LBM_PORT_PRESSURE_NEIGHBOR ||
                                   port.kind == LBM_PORT_PRESSURE_MOMENT;

@@ -42,7 +42,12 @@ hipError_t JvpRunner::ports(int Q, bool mask, const PortArgs& a, void* pop, hipS
     if (a.np <= 0) return hipSuccess;
     if (!a.cell || !a.nb || !a.kind || !a.value || (mask && !a.mask)) return hipErrorInvalidValue;
     const dim3 grid((a.np + PORT_BLOCK - 1) / PORT_BLOCK), block(PORT_BLOCK);
-    if (Q == 19) {
+    if (Q == 19 && a.st) {
+        if (mask)
+            hipLaunchKernelGGL((k_port_apply<P_JVP, 19, true, true>), grid, block, 0, s, a, pop, nullptr);
+        else
+            hipLaunchKernelGGL((k_port_apply<P_JVP, 19, false, true>), grid, block, 0, s, a, pop, nullptr);
+    } else if (Q == 19) {
         if (mask)
             hipLaunchKernelGGL((k_port_apply<P_JVP, 19, true>), grid, block, 0, s, a, pop, nullptr);
         else

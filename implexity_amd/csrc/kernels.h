@@ -1496,6 +1496,7 @@ struct PortArgs {
     const double* value;  // 3 per port cell
     const uint32_t* mask;
     int odd;              // parity of the state the ports act on
+    int st;               // 1: some rows carry a stationary rule (the kernels with them compiled in)
 };
 
 // ---------------------------------------------------------------------------------
@@ -1662,9 +1663,9 @@ LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g
     }
 }
 
-template <int Q, class C>
+template <int Q, class C, bool ST = false>
 LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out) {
-    if constexpr (Q == 19) {
+    if constexpr (Q == 19 && ST) {
         if (kind >= ST_PORT_BASE) {
             st_port(nb, kind, value, out);
             return;
@@ -1744,7 +1745,9 @@ LBM_DEV uint32_t mask_of(const PortArgs& A, long long cell, Cursor& cu) {
 
 // f(port cell) := NEE(f(neighbour)); exch_part[block] gets the block's mass change
 // sum_q (new - old).
-template <int P, int Q, bool MASK>
+// ST: the stationary rules compiled in (rows of kind >= ST_PORT_BASE); the NEE-only
+// variant keeps the register budget of the plain rule.
+template <int P, int Q, bool MASK, bool ST = false>
 __global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, double* exch_part) {
     using S = Store<P>;
     using C = typename S::C;
@@ -1759,7 +1762,7 @@ __global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, doub
         load_cell<P, Q>(A.g, pop, cn, mn, A.odd, vnb);
         load_cell<P, Q>(A.g, pop, cx, mx, A.odd, old);
         to_full<S::shifted, Q>(vnb, fnb);
-        port_populations<Q, C>(fnb, A.kind[r], A.value + 3 * r, fout);
+        port_populations<Q, C, ST>(fnb, A.kind[r], A.value + 3 * r, fout);
         from_full<S::shifted, Q>(fout, out);
         store_cell_read<P, Q>(A.g, pop, cx, mx, A.odd, out);
         // Mass exchange of the stored (encoded) values, summed in order of q.
@@ -1795,7 +1798,7 @@ __global__ __launch_bounds__(128) void k_port_save(PortArgs A, const void* pop_,
 
 // Adjoint of the port pass on the cotangent of a state of parity A.odd:
 // lam(nb) += J_NEE(f_nb)^T lam(port); lam(port) := 0   (solver.rs:642-656).
-template <int P, int LP, int Q, bool MASK>
+template <int P, int LP, int Q, bool MASK, bool ST = false>
 __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const typename Store<P>::C* buf, const int* lsc) {
     using L = LamStore<LP>;
     using C = typename Store<P>::C;
@@ -1815,7 +1818,7 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
 #pragma unroll
     for (int q = 0; q < Q; q++) fnb[q] = buf[(long long)q * A.np + r];
     const int kind = A.kind[r];
-    if constexpr (Q == 19) {
+    if constexpr (Q == 19 && ST) {
         if (kind >= ST_PORT_BASE) {
             // the moment rules read the port cell itself: its cotangent is J^T g alone
             const bool self = A.nb[r] == A.cell[r];

@@ -291,6 +291,10 @@ struct RunnerImpl final : Runner {
         if (!ok_ports(mask, a) || !pop) return hipErrorInvalidValue;
         const LaunchCfg c = port_cfg(a, s);
         if (Q == 19) {
+            if (a.st) {
+                if (mask) LBM_LAUNCH((k_port_apply<P, 19, true, true>), c, a, pop, exch);
+                LBM_LAUNCH((k_port_apply<P, 19, false, true>), c, a, pop, exch);
+            }
             if (mask) LBM_LAUNCH((k_port_apply<P, 19, true>), c, a, pop, exch);
             LBM_LAUNCH((k_port_apply<P, 19, false>), c, a, pop, exch);
         }
@@ -317,6 +321,10 @@ struct RunnerImpl final : Runner {
         const LaunchCfg c = port_cfg(a, s);
         const C* b = static_cast<const C*>(buf);
         if (Q == 19) {
+            if (a.st) {
+                if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 19, true, true>), c, a, lam, b, lsc);
+                LBM_LAUNCH((k_port_vjp<P, LP, 19, false, true>), c, a, lam, b, lsc);
+            }
             if (mask) LBM_LAUNCH((k_port_vjp<P, LP, 19, true>), c, a, lam, b, lsc);
             LBM_LAUNCH((k_port_vjp<P, LP, 19, false>), c, a, lam, b, lsc);
         }

@@ -315,8 +315,9 @@ typedef struct {
     double max_mach;             /* sqrt(3) max |u| over the first sweep (admit, solver.rs:742-751) */
     double mass_drift;           /* max relative drift of fluid mass incl. port exchange */
     uint64_t bad_cells;          /* cells with a non-finite or non-positive population */
-    double forward_kernel_avg_ms; /* forward step kernel alone, HIP events on every 64th step */
-    double adjoint_kernel_avg_ms; /* adjoint step kernel alone, HIP events on every 64th step */
+    double forward_kernel_avg_ms; /* forward step kernel alone, HIP events on every 64th step
+                                     (every n/4-th, so at least four, on horizons below 256) */
+    double adjoint_kernel_avg_ms; /* adjoint step kernel alone, sampled the same way */
     uint64_t kernel_samples;      /* steps timed for the two averages (gradient only) */
     double cotangent_max;         /* FP16 cotangents: max |lambda| of the last adjoint step (else NaN) */
     int64_t cotangent_exponent;   /* FP16 cotangents: exponent e of the stored lambda * 2^e (else 0) */
