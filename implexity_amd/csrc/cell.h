@@ -278,6 +278,102 @@ LBM_DEV void collide_pairs_vjp_fast(const float rho, const float u[3], const flo
     rho_bar = rb;
 }
 
+// Transposed TRT / BGK collision and Brinkman map of one cell in the shifted single
+// precision form: collide_pairs_vjp (collision.rs:492-570) followed by brinkman_chain
+// below (psm.rs:317-330, 455-469), evaluated through the moments of the cotangent.
+// The equilibrium and source terms of direction q are w_q P(c_q), P of degree two in c_q
+// with coefficients in (rho, u, F):
+//   w_q [omega A - 3 kp uf + 4.5 omega rho (c.u)^2 + 9 kp (c.u)(c.F) + 3 om rho c.u + 3 km c.F],
+//   A = drho - 1.5 rho |u|^2,
+// so their pairing with g needs only M0 = sum w g, M1 = sum w c g and M2 = sum w c c^T g,
+// and the cotangents of rho, u and F follow from these ten numbers.  The Brinkman chain
+// maps them to cotangents R of the density and J of the momentum, and one pass forms
+//   fb_q = g_q - omega g+_q - om g-_q + R + c_q . J
+// (g+-: even/odd parts over the pair q, opp(q)).  R0, J0 are density and momentum
+// cotangents the caller adds (objective terms).  Returns the cotangent of alpha.
+template <int Q>
+LBM_DEV float trt_adjoint_shifted(const Macro<Q, float>& m, float alpha, const float a[3], float omega, float om,
+                                  const float* g, float R0, const float J0[3], float* fb) {
+    // moments of g; M2 as xx, yy, zz, xy, xz, yz
+    float M0 = float(weight<Q>(0)) * g[0];
+    float M1[3] = {0.0f, 0.0f, 0.0f}, M2[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int o = opp<Q>(i);
+        if constexpr (o > i) {
+            constexpr float w = float(weight<Q>(i));
+            constexpr int c0 = cvec<Q>(i, 0), c1 = cvec<Q>(i, 1), c2 = cvec<Q>(i, 2);
+            const float pa = g[i] + g[o], pb = g[i] - g[o];
+            M0 += w * pa;
+            if constexpr (c0 != 0) M1[0] += float(c0) * w * pb;
+            if constexpr (c1 != 0) M1[1] += float(c1) * w * pb;
+            if constexpr (c2 != 0) M1[2] += float(c2) * w * pb;
+            if constexpr (c0 != 0) M2[0] += w * pa;
+            if constexpr (c1 != 0) M2[1] += w * pa;
+            if constexpr (c2 != 0) M2[2] += w * pa;
+            if constexpr (c0 * c1 != 0) M2[3] += float(c0 * c1) * w * pa;
+            if constexpr (c0 * c2 != 0) M2[4] += float(c0 * c2) * w * pa;
+            if constexpr (c1 * c2 != 0) M2[5] += float(c1 * c2) * w * pa;
+        }
+    });
+    const float* u = m.u;
+    const float* F = m.F;
+    const float Mu[3] = {M2[0] * u[0] + M2[3] * u[1] + M2[4] * u[2], M2[3] * u[0] + M2[1] * u[1] + M2[5] * u[2],
+                         M2[4] * u[0] + M2[5] * u[1] + M2[2] * u[2]};
+    const float MF[3] = {M2[0] * F[0] + M2[3] * F[1] + M2[4] * F[2], M2[3] * F[0] + M2[1] * F[1] + M2[5] * F[2],
+                         M2[4] * F[0] + M2[5] * F[1] + M2[2] * F[2]};
+    const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const float uMu = u[0] * Mu[0] + u[1] * Mu[1] + u[2] * Mu[2];
+    const float uM1 = u[0] * M1[0] + u[1] * M1[1] + u[2] * M1[2];
+    const float rho = m.rho;
+    const float kp = 1.0f - omega * 0.5f, km = 1.0f - om * 0.5f;
+    // cotangents of rho (through drho and rho), u and F of the collision
+    float R = omega * M0 * (1.0f - 1.5f * usq) + 4.5f * omega * uMu + 3.0f * om * uM1;
+    const float s1 = -3.0f * omega * rho * M0, s2 = -3.0f * kp * M0;
+    const float s3 = 9.0f * omega * rho, s4 = 9.0f * kp, s5 = 3.0f * om * rho, s6 = 3.0f * km;
+    float ub[3], Fb[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ub[d] = s1 * u[d] + s2 * F[d] + s3 * Mu[d] + s4 * MF[d] + s5 * M1[d];
+        Fb[d] = s2 * u[d] + s4 * Mu[d] + s6 * M1[d];
+    }
+    // F = rho (a - alpha u)
+    float ab = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        R += Fb[d] * (a[d] - alpha * u[d]);
+        ab -= rho * Fb[d] * u[d];
+        ub[d] -= alpha * rho * Fb[d];
+    }
+    // u = (j / rho + a / 2) / den
+    const float irho = __builtin_amdgcn_rcpf(rho);
+    const float iden = __builtin_amdgcn_rcpf(m.den);
+    const float ir = irho * iden;
+    float J[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ab -= 0.5f * ub[d] * u[d] * iden;
+        J[d] = ub[d] * ir;
+        R -= J[d] * (m.j[d] * irho);
+    }
+    R += R0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) J[d] += J0[d];
+    // per-direction pass: fb_i = k1 g_i + k2 g_o + R + c_i.J, fb_o = k1 g_o + k2 g_i + R - c_i.J
+    const float k1 = 1.0f - 0.5f * (omega + om), k2 = 0.5f * (om - omega);
+    fb[0] = (1.0f - omega) * g[0] + R;
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int o = opp<Q>(i);
+        if constexpr (o > i) {
+            const float cJ = dotc<Q, i>(J);
+            fb[i] = k1 * g[i] + (k2 * g[o] + (R + cJ));
+            fb[o] = k1 * g[o] + (k2 * g[i] + (R - cJ));
+        }
+    });
+    return ab;
+}
+
 // Chain the collision cotangents through the Brinkman macroscopic map back to f and
 // alpha: psm.rs:317-330 and :455-469 with m = 0, dp_bar = 0.
 template <int Q, class C>

@@ -470,26 +470,26 @@ LBM_DEV T& elem(T* base, uint32_t idx) {
     return *reinterpret_cast<T*>(reinterpret_cast<B*>(base) + size_t(idx * uint32_t(sizeof(T))));
 }
 
-// Blocked-link mask of a cell from the domain faces alone (no solid cells).
+// Directions q whose velocity component along `axis` has sign `sgn`, as a bit mask.
+template <int Q>
+constexpr uint32_t dir_bits(int axis, int sgn) {
+    uint32_t b = 0;
+    for (int q = 1; q < Q; q++)
+        if (cvec<Q>(q, axis) * sgn > 0) b |= 1u << q;
+    return b;
+}
+
+// Blocked-link mask of a cell from the domain faces alone (no solid cells): on the low
+// face of an axis every link with a positive component there is blocked (its upstream
+// neighbour is outside), on the high face every link with a negative one.  The x and y
+// faces are uniform over a block's z row, so only the z terms are per lane.
 template <int Q>
 LBM_DEV uint32_t face_mask(const Geom& g, int i, int j, int k) {
     const bool lx = !g.px && i == 0, hx = !g.px && i == g.nx - 1;
     const bool ly = !g.py && j == 0, hy = !g.py && j == g.ny - 1;
     const bool lz = !g.pz && k == 0, hz = !g.pz && k == g.nz - 1;
-    uint32_t m = 0;
-    static_for<1, Q>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        constexpr int cx = cvec<Q>(q, 0), cy = cvec<Q>(q, 1), cz = cvec<Q>(q, 2);
-        bool b = false;
-        if constexpr (cx > 0) b = b || lx;
-        if constexpr (cx < 0) b = b || hx;
-        if constexpr (cy > 0) b = b || ly;
-        if constexpr (cy < 0) b = b || hy;
-        if constexpr (cz > 0) b = b || lz;
-        if constexpr (cz < 0) b = b || hz;
-        m |= b ? (1u << q) : 0u;
-    });
-    return m;
+    return (lx ? dir_bits<Q>(0, 1) : 0u) | (hx ? dir_bits<Q>(0, -1) : 0u) | (ly ? dir_bits<Q>(1, 1) : 0u) |
+           (hy ? dir_bits<Q>(1, -1) : 0u) | (lz ? dir_bits<Q>(2, 1) : 0u) | (hz ? dir_bits<Q>(2, -1) : 0u);
 }
 
 template <int Q, bool MASK>
@@ -517,10 +517,19 @@ LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
 // Element at the read location of f_q (see read_loc): the slot at x + a_opp(q) in the
 // wave-uniform array of an open link, or in the opposite array for a blocked link
 // (halfway bounce-back).  The array choice is a select between two uniform bases.
-template <int Q, int q, class T>
+// OPAQUE: pass the 32-bit byte offset through an empty asm statement first.  A kernel
+// that reads a field at these locations and writes another one there later (the adjoint
+// reads f and writes lambda at the read locations) would otherwise reuse the read's
+// offset across basic blocks as a 64-bit value and address the store with 64-bit VGPR
+// pairs; the barrier keeps the 32-bit offset, and the store takes the uniform base from
+// SGPRs with the offset in one VGPR.
+template <int Q, int q, class T, bool OPAQUE = false>
 LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
     if constexpr (q == 0) {
-        return elem(arr_base<Q>(pop, g, 0), cu.at(corner_neg<Q>(q)));
+        uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T));
+        if constexpr (OPAQUE) asm("" : "+v"(off));
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        return *reinterpret_cast<T*>(reinterpret_cast<B*>(arr_base<Q>(pop, g, 0)) + size_t(off));
     } else {
         // Array read: (odd xor blocked) ? q : opp(q); the pair's higher index sits one
         // array after the lower one.
@@ -530,8 +539,9 @@ LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd)
         const bool hi = pick_q == q_is_hi;
         using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
         B* base = reinterpret_cast<B*>(arr_base<Q>(pop, g, lo));
-        const uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T)) +
-                             (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
+        uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T)) +
+                       (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
+        if constexpr (OPAQUE) asm("" : "+v"(off));
         return *reinterpret_cast<T*>(base + size_t(off));
     }
 }
@@ -541,7 +551,7 @@ LBM_DEV T load_read(const T* pop, const Geom& g, const Cursor& cu, uint32_t m, i
 }
 template <int Q, int q, class T>
 LBM_DEV void store_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd, T v) {
-    at_read<Q, q>(pop, g, cu, m, odd) = v;
+    at_read<Q, q, T, true>(pop, g, cu, m, odd) = v;
 }
 // Element at the write location of f*_q (see write_loc).
 template <int Q, int q, class T>
@@ -841,32 +851,29 @@ LBM_DEV double objective_cell(const ObjArgs& ob, const C* f, const Macro<Q, C>& 
     return 0.0;
 }
 
-// Adds weight * d phi / d f and d phi / d alpha to fb / ab.
+// Gradient of weight * phi(state) as cotangents of the density and momentum of the
+// cell: d(weight phi)/d f_q = R + c_q . J (every objective depends on f through rho and
+// j only), and d(weight phi)/d alpha added to ab.
 template <int Q, class C>
-LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, int coord_axis, C* fb, C& ab) {
+LBM_DEV void objective_grad_rj(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, int coord_axis, C& R, C J[3],
+                               C& ab) {
+    R = C(0);
+    J[0] = J[1] = J[2] = C(0);
     if (ob.kind == OBJ_DISSIPATION) {
         const C w = C(ob.weight);
         const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
         C rho_bar = w * alpha * usq;
         const C usq_bar = w * m.rho * alpha;
         C al_bar = w * m.rho * usq;
-        C jb[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const C ub = usq_bar * C(2.0) * m.u[d];
             al_bar -= qdiv(ub * m.sv[d], m.den * m.den) * C(0.5);
             const C sb = qdiv(ub, m.den);
-            jb[d] = qdiv(sb, m.rho);
+            J[d] = qdiv(sb, m.rho);
             rho_bar -= qdiv(sb * m.j[d], m.rho * m.rho);
         }
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            C v = rho_bar;
-            if constexpr (cvec<Q>(i, 0) != 0) v += jb[0] * C(cvec<Q>(i, 0));
-            if constexpr (cvec<Q>(i, 1) != 0) v += jb[1] * C(cvec<Q>(i, 1));
-            if constexpr (cvec<Q>(i, 2) != 0) v += jb[2] * C(cvec<Q>(i, 2));
-            fb[i] += v;
-        });
+        R = rho_bar;
         ab += al_bar;
     } else if (ob.kind == OBJ_VECTOR) {
         // phi = sum_d A_d u_d + B rho |u|^2 with A_d = w vw_d vel and
@@ -879,32 +886,43 @@ LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, in
         const C usq = m.u[0] * m.u[0] + m.u[1] * m.u[1] + m.u[2] * m.u[2];
         C rho_bar = B * usq;
         C al_bar = bd * m.rho * usq;
-        C jb[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const C ub = w * C(ob.vw[d]) * V + C(2) * B * m.rho * m.u[d];
             al_bar -= qdiv(ub * m.sv[d], m.den * m.den) * C(0.5);
             const C sb = qdiv(ub, m.den);
-            jb[d] = qdiv(sb, m.rho);
+            J[d] = qdiv(sb, m.rho);
             rho_bar -= qdiv(sb * m.j[d], m.rho * m.rho);
         }
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            C v = rho_bar;
-            if constexpr (cvec<Q>(i, 0) != 0) v += jb[0] * C(cvec<Q>(i, 0));
-            if constexpr (cvec<Q>(i, 1) != 0) v += jb[1] * C(cvec<Q>(i, 1));
-            if constexpr (cvec<Q>(i, 2) != 0) v += jb[2] * C(cvec<Q>(i, 2));
-            fb[i] += v;
-        });
+        R = rho_bar;
         ab += al_bar;
     } else if (ob.kind == OBJ_PRESSURE_DROP) {
         const double s = coord_axis == ob.plane_a ? ob.inv_na : (coord_axis == ob.plane_b ? -ob.inv_nb : 0.0);
-        if (s != 0.0) {
-            const C gv = C(ob.weight * s / 3.0);
-#pragma unroll
-            for (int q = 0; q < Q; q++) fb[q] += gv;
-        }
+        R = C(ob.weight * s / 3.0);
     }
+}
+
+// Adds weight * d phi / d f and d phi / d alpha to fb / ab.
+template <int Q, class C>
+LBM_DEV void objective_grad(const ObjArgs& ob, const Macro<Q, C>& m, C alpha, int coord_axis, C* fb, C& ab) {
+    if (ob.kind != OBJ_DISSIPATION && ob.kind != OBJ_VECTOR && ob.kind != OBJ_PRESSURE_DROP) return;
+    C R, J[3];
+    objective_grad_rj<Q, C>(ob, m, alpha, coord_axis, R, J, ab);
+    if (ob.kind == OBJ_PRESSURE_DROP) {
+        if (R != C(0)) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) fb[q] += R;
+        }
+        return;
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        C v = R;
+        if constexpr (cvec<Q>(i, 0) != 0) v += J[0] * C(cvec<Q>(i, 0));
+        if constexpr (cvec<Q>(i, 1) != 0) v += J[1] * C(cvec<Q>(i, 1));
+        if constexpr (cvec<Q>(i, 2) != 0) v += J[2] * C(cvec<Q>(i, 2));
+        fb[i] += v;
+    });
 }
 
 // ---------------------------------------------------------------------------------
@@ -1182,6 +1200,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     using S = Store<P>;
     using L = LamStore<LP>;
     using C = typename S::C;
+    // TRT / BGK in single precision: the moment-form transpose (trt_adjoint_shifted)
+    constexpr bool FUSED = S::shifted && (KIND == K_TRT || KIND == K_BGK);
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
@@ -1279,19 +1299,47 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     collide_vjp<Q, BK, C, S::shifted>(f, C(0), a, C(A.omega), C(A.om), C(A.tau), pk, gb, mac, fb, ab);
                     ab = C(0);
                 }
+            } else if constexpr (FUSED) {
+                // objective and temperature-coupling terms enter as density and momentum
+                // cotangents, added in the transpose's single per-direction pass
+                macro_of<Q, C, true>(f, alpha, a, mac);
+                C R0 = C(0), J0[3] = {C(0), C(0), C(0)};
+                ab = C(0);
+                if (A.obj.kind != OBJ_NONE) {
+                    const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+                    objective_grad_rj<Q, C>(A.obj, mac, alpha, coord, R0, J0, ab);
+                }
+                if constexpr (UB) {
+                    const C* U = static_cast<const C*>(A.ubar);
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C ub[3] = {U[cell], U[n + cell], U[2 * n + cell]};
+                    C au = C(0);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const C s = ub[d] / mac.den / mac.rho;
+                        R0 -= s * (mac.j[d] / mac.rho);
+                        J0[d] += s;
+                        au += ub[d] * mac.u[d];
+                    }
+                    ab += C(-0.5) * au / mac.den;
+                }
+                ab += trt_adjoint_shifted<Q>(mac, alpha, a, C(A.omega), KIND == K_BGK ? C(A.omega) : C(A.om), gb, R0,
+                                             J0, fb);
             } else {
                 collide_vjp<Q, KIND, C, S::shifted>(f, alpha, a, C(A.omega), C(A.om), C(A.tau),
                                                     static_cast<const C*>(A.mrt), gb, mac, fb, ab);
             }
-            if (A.obj.kind != OBJ_NONE) {
-                const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
-                objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
-            }
-            if constexpr (UB) {
-                const C* U = static_cast<const C*>(A.ubar);
-                const long long n = (long long)g.nx * g.ny * g.nz;
-                const C ub[3] = {U[cell], U[n + cell], U[2 * n + cell]};
-                brinkman_u_vjp<Q, C>(mac, ub, fb, ab);
+            if constexpr (!FUSED) {
+                if (A.obj.kind != OBJ_NONE) {
+                    const int coord = A.obj.axis == 0 ? i : (A.obj.axis == 1 ? j : k);
+                    objective_grad<Q, C>(A.obj, mac, alpha, coord, fb, ab);
+                }
+                if constexpr (UB) {
+                    const C* U = static_cast<const C*>(A.ubar);
+                    const long long n = (long long)g.nx * g.ny * g.nz;
+                    const C ub[3] = {U[cell], U[n + cell], U[2 * n + cell]};
+                    brinkman_u_vjp<Q, C>(mac, ub, fb, ab);
+                }
             }
             A.alpha_bar[cell] += double(ab);
             static_for<0, Q>([&](auto I) {
