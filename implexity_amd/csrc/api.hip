@@ -259,6 +259,7 @@ struct lbm_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf F, A, L, alpha, alpha_bar, mask, mrt, pcell, pnb, pkind, pval, pbuf, scal, aos, pool, part, lsc, fold;
+    DevBuf W;          // working state of a gradient sweep (F keeps state 0 throughout)
     DevBuf ledger;     // mass ledger of the last sweep (grows to the longest horizon)
     bool abar_ready = false;  // alpha_bar holds the result of a gradient or step VJP
     HostStage stage;   // pinned staging for device-to-host copies
@@ -320,6 +321,8 @@ struct lbm_handle {
     size_t csize() const { return prec == LBM_F64 ? 8 : 4; }
     // Step grid: one block of b cells (a multiple of 64) along a z row, one row per grid
     // y, one x plane per grid z.  (Blocks of two y rows measured no faster at 256^3.)
+    // k_forward and k_adjoint take the row from (blockIdx.z, blockIdx.y) and rely on
+    // blockDim.y = 1.
     LaunchCfg cfg() const {
         const int b = std::min(256, ((nz + 63) / 64) * 64);
         return LaunchCfg{dim3((nz + b - 1) / b, ny, nx), dim3(b), stream};
@@ -661,6 +664,10 @@ void ensure_adjoint_buffers(lbm_handle* h) {
     }
     h->L.alloc(h->lam_bytes, "cotangent state");
     h->A.alloc(h->state_bytes, "ahead state");
+    if (h->W.bytes != h->state_bytes) {
+        h->W.alloc(h->state_bytes, "working state");
+        hip_check(hipMemsetAsync(h->W.p, 0, h->W.bytes, h->stream), "memset");
+    }
     h->alpha_bar.alloc(size_t(h->ncell) * sizeof(double), "alpha_bar");
     if (h->thermal) {
         h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
@@ -2254,7 +2261,12 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         if (sch.peak > slots)
             fail(LBM_ERR_MEMORY, "checkpoint budget holds " + std::to_string(slots) + " snapshots; checkpoint_every " +
                                      std::to_string(h->every) + " needs " + std::to_string(sch.peak));
-        auto slot_ptr = [&](uint64_t s) { return static_cast<char*>(h->pool.p) + size_t(s) * h->state_bytes; };
+        // State 0 stays in F for the whole sweep: its snapshot is F itself (`alias` is the
+        // slot that holds it), every step writes the working buffer W, a snapshot slot or
+        // the ahead buffer, and F is never written -- no copy in and no restore out.
+        long long alias = -1;
+        auto pool_ptr = [&](uint64_t s) { return static_cast<void*>(static_cast<char*>(h->pool.p) + size_t(s) * h->state_bytes); };
+        auto slot_ptr = [&](uint64_t s) { return (long long)s == alias ? h->F.p : pool_ptr(s); };
 
         reset_scalars(h);
         if (h->thermal) hip_check(hipMemsetAsync(h->phi_bar.p, 0, h->phi_bar.bytes, h->stream), "memset");
@@ -2309,27 +2321,21 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         void* cur = h->F.p;
         SweepDiag diag;
         double drift = 0.0;
+        // The first sweep's diagnostics are read after the reverse sweep (the states they
+        // cover are the first sweep's; recomputed steps record none), so the reverse sweep
+        // is queued behind the forward one without a host round trip; a refused
+        // trajectory fails the call all the same, with F still at state 0.
         for (size_t idx = 0; idx < sch.actions.size(); idx++) {
             const Action& act = sch.actions[idx];
             const bool forward_phase = idx < sch.reverse_start;
-            if (idx == sch.reverse_start) {  // the first sweep is complete: its diagnostics
-                diag = read_diag(h);
-                drift = led.drift();
-                mark = nullptr;
-                try {
-                    admit(h, diag, drift);
-                } catch (const Fail&) {  // refused: leave the handle at state 0, as on success
-                    hip_check(hipMemcpyAsync(h->F.p, slot_ptr(0), h->state_bytes, hipMemcpyDeviceToDevice,
-                                             h->stream),
-                              "restore");
-                    hip_check(hipStreamSynchronize(h->stream), "sync");
-                    for (auto e : evs) (void)hipEventDestroy(e);
-                    throw;
-                }
-            }
             switch (act.kind) {
             case ActKind::Snapshot:
                 if (cursor != act.a) fail(LBM_ERR_INVALID, "snapshot of a state the cursor does not hold");
+                if (act.a == 0 && cur == h->F.p && alias < 0) {  // state 0 is F
+                    alias = (long long)act.b;
+                    break;
+                }
+                if ((long long)act.b == alias) alias = -1;  // the slot now takes another state
                 if (cur != slot_ptr(act.b)) {
                     hip_check(hipMemcpyAsync(slot_ptr(act.b), cur, h->state_bytes, hipMemcpyDeviceToDevice,
                                              h->stream),
@@ -2345,11 +2351,12 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
             case ActKind::Release: break;
             case ActKind::Advance: {
                 if (cursor != act.a) fail(LBM_ERR_INVALID, "advance from a position the cursor does not hold");
-                // Where the advance ends: into the slot of an immediately following snapshot.
-                void* last_dst = h->F.p;
+                // Where the advance ends: into the slot of an immediately following snapshot
+                // (of a state after 0, so the slot's own storage).
+                void* last_dst = h->W.p;
                 if (idx + 1 < sch.actions.size() && sch.actions[idx + 1].kind == ActKind::Snapshot &&
                     sch.actions[idx + 1].a == act.b)
-                    last_dst = slot_ptr(sch.actions[idx + 1].b);
+                    last_dst = pool_ptr(sch.actions[idx + 1].b);
                 const hipEvent_t e0 = span_start();
                 for (uint64_t t = act.a; t < act.b; t++) {
                     const ObjArgs ob = forward_phase ? stage_obj(tmpl, obj, t, nsteps, false) : none;
@@ -2389,7 +2396,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                                       "port save");
                         break;
                     }
-                    void* dst = (t + 1 == act.b) ? last_dst : h->F.p;
+                    void* dst = (t + 1 == act.b) ? last_dst : h->W.p;
                     step_into(h, cur, dst, base + t, ob, forward_phase ? &led : nullptr, t, kspan(fwd_k, fsteps));
                     cur = dst;
                     fsteps++;
@@ -2434,8 +2441,14 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
             }
             }
         }
-        // F holds some intermediate state; restore state 0 from slot 0 (taken first).
-        hip_check(hipMemcpyAsync(h->F.p, slot_ptr(0), h->state_bytes, hipMemcpyDeviceToDevice, h->stream), "restore");
+        diag = read_diag(h);
+        drift = led.drift();
+        try {
+            admit(h, diag, drift);
+        } catch (const Fail&) {  // refused: the handle is at state 0 (F), as on success
+            for (auto e : evs) (void)hipEventDestroy(e);
+            throw;
+        }
         double J0 = 0.0;
         hip_check(hipMemcpyAsync(&J0, h->scal.p, sizeof(J0), hipMemcpyDeviceToHost, h->stream), "copy");
         if (alpha_bar) h->stage.download(alpha_bar, h->alpha_bar.p, size_t(h->ncell) * sizeof(double), h->stream);

@@ -1204,7 +1204,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
     constexpr bool FUSED = S::shifted && (KIND == K_TRT || KIND == K_BGK);
     const Geom& g = A.g;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y, i = blockIdx.z;  // one z row per block, as k_forward
     float lmax = 0.0f;  // max |lambda| written (FP16 cotangents)
     if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
@@ -1475,8 +1475,10 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
     using L = LamStore<LP>;
     using C = typename S::C;
     const Geom& g = A.g;
+    // step launches: a block is a run of blockDim.x cells of one z row (blockDim.y = 1),
+    // so the row (i, j) is uniform over the block
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y, i = blockIdx.z;
     double jsum = 0.0, msum = 0.0;
     float umax = 0.0f, fneg = -INFINITY;
     bool bad = false;
