@@ -254,6 +254,7 @@ struct lbm_handle {
     bool has_mask = false;
     Geom g{};
     long long ncell = 0;
+    long long arr_elems = 0;  // elements of one population array's share of a field (row-blocked)
     size_t state_bytes = 0, lam_bytes = 0;
     const Runner* run = nullptr;
     hipStream_t stream = nullptr;
@@ -1277,35 +1278,35 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             if (!any) h->solid.clear();
         }
 
-        // Storage geometry: one extra storage layer (the esoteric-twist ghost) on each
-        // non-periodic x and y axis; rows of SZ = nz rounded up to 64 elements, so that a
-        // row of any population array is a whole number of 128-byte lines; the +z ghost
-        // slots of a non-periodic z axis in their own region after the cells.  With the
-        // ghost inside the row (pitch nz + 1) neighbouring rows shared a line that two
-        // blocks fetched: 15-22 % more HBM reads than the algorithmic bytes at 256^3.
+        // Storage geometry (kernels.h, Addressing): row-blocked fields, one extra storage
+        // layer (the esoteric-twist ghost) on each non-periodic x and y axis, rows of SZ = nz
+        // rounded up to 64 elements (every array row a whole number of 128-byte lines), and
+        // on a non-periodic z axis GY ghost rows per plane holding the rows' +z ghost slots.
+        // (With the ghost inside the row, pitch nz + 1, neighbouring rows shared a line that
+        // two blocks fetched: 15-22 % more HBM reads than the algorithmic bytes at 256^3.)
         h->g.nx = h->nx;
         h->g.ny = h->ny;
         h->g.nz = h->nz;
         h->g.px = h->per[0];
         h->g.py = h->per[1];
         h->g.pz = h->per[2];
-        const int SX = h->nx + (h->per[0] ? 0 : 1);
+        h->g.SX = h->nx + (h->per[0] ? 0 : 1);
         h->g.SY = h->ny + (h->per[1] ? 0 : 1);
         h->g.SZ = ((h->nz + 63) / 64) * 64;
-        h->g.sxy = (long long)h->g.SY * h->g.SZ;
-        h->g.zg = (long long)SX * h->g.sxy;
-        const long long cells_and_ghosts = h->g.zg + (h->per[2] ? 0 : (long long)SX * h->g.SY);
-        h->g.stride = ((cells_and_ghosts + 63) / 64) * 64;
-        // Kernels address an opposite pair of population arrays with 32-bit byte offsets.
-        if (uint64_t(h->g.stride) * std::max({elem_bytes(h->prec), elem_bytes(h->lprec),
-                                              h->thermal ? h->tlam_elem : size_t(0)}) >=
+        h->g.GY = h->per[2] ? 0 : (h->g.SY + h->g.SZ - 1) / h->g.SZ;
+        // elements of one array's share of a field: a field of Qn arrays holds Qn times this
+        h->arr_elems = (long long)h->g.SX * (h->g.SY + h->g.GY) * h->g.SZ;
+        // Kernels address a cell's populations as a row block (64-bit) plus a 32-bit byte
+        // offset within its plane.
+        if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(h->Q) *
+                std::max({elem_bytes(h->prec), elem_bytes(h->lprec), h->thermal ? h->tlam_elem : size_t(0)}) >=
             (uint64_t(1) << 31))
-            fail(LBM_ERR_CONTRACT, "lattice too large: one population array would exceed 2 GiB");
-        h->flow_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->prec);
-        h->flow_lam_bytes = size_t(h->g.stride) * h->Q * elem_bytes(h->lprec);
+            fail(LBM_ERR_CONTRACT, "lattice too large: one x plane of the populations would exceed 2 GiB");
+        h->flow_bytes = size_t(h->arr_elems) * h->Q * elem_bytes(h->prec);
+        h->flow_lam_bytes = size_t(h->arr_elems) * h->Q * elem_bytes(h->lprec);
         const int TQn = h->thermal ? TQ : 0;
-        h->state_bytes = size_t(h->g.stride) * (h->Q + TQn) * elem_bytes(h->prec);
-        h->lam_bytes = h->flow_lam_bytes + size_t(h->g.stride) * TQn * h->tlam_elem;
+        h->state_bytes = size_t(h->arr_elems) * (h->Q + TQn) * elem_bytes(h->prec);
+        h->lam_bytes = h->flow_lam_bytes + size_t(h->arr_elems) * TQn * h->tlam_elem;
         h->run = runner_for(h->prec, h->lprec);
 
         // Ports (moving/boundary.rs:318-380; ports.rs:107-262 for the stationary rules).
@@ -2010,7 +2011,7 @@ int lbm_debug_link_table(lbm_handle* h, uint32_t* source, uint64_t len) {
         // source index (solver.rs:131-136).
         const Runner* r64 = runner_f64();
         DevBuf buf;
-        buf.alloc(size_t(h->g.stride) * Q * sizeof(double), "link-table state");
+        buf.alloc(size_t(h->arr_elems) * Q * sizeof(double), "link-table state");
         hip_check(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream), "memset");
         std::vector<double> ids(size_t(h->ncell) * Q);
         for (size_t i = 0; i < ids.size(); i++) ids[i] = double(i);
@@ -2165,16 +2166,16 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
             fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
                                    "Brinkman operators without closures, sponges or the temperature field");
         // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
-        // takes 32-bit byte offsets within an opposite pair of arrays.
-        if (uint64_t(h->g.stride) * 2 * sizeof(Dual) >= (uint64_t(1) << 32))
-            fail(LBM_ERR_CONTRACT, "lattice too large for the tangent step (arrays of 16-byte values above 2 GiB)");
+        // takes 32-bit byte offsets within an x plane of the field.
+        if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(Q) * sizeof(Dual) >= (uint64_t(1) << 31))
+            fail(LBM_ERR_CONTRACT, "lattice too large for the tangent step (an x plane of 16-byte values above 2 GiB)");
         for (uint64_t x = 0; x < len; x++)
             if (!std::isfinite(d_state[x])) fail(LBM_ERR_CONTRACT, "tangent must be finite");
         const uint64_t t = h->t;
         const JvpRunner* jr = jvp_runner();
         const LaunchCfg lc = h->cfg();
         DevBuf dual, tan, da, dc;
-        dual.alloc(size_t(h->g.stride) * Q * sizeof(Dual), "tangent state");
+        dual.alloc(size_t(h->arr_elems) * Q * sizeof(Dual), "tangent state");
         hip_check(hipMemsetAsync(dual.p, 0, dual.bytes, h->stream), "memset");
         tan.alloc(len * sizeof(double), "tangent");
         ensure_aos(h);

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_set_dual(Geom g, Dual* pop, const uint3
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
         store_read<Q, q>(pop, g, cu, m, odd, Dual(val[cell * Q + q], tan[cell * Q + q]));
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_get_tangent(Geom g, const Dual* pop, co
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
         tan[cell * Q + q] = load_read<Q, q>(pop, g, cu, m, odd).d;

@@ -73,13 +73,12 @@ enum ObjKind : int { OBJ_NONE = 0, OBJ_DISSIPATION = 1, OBJ_PRESSURE_DROP = 2, O
 
 constexpr uint32_t SOLID_BIT = 0x80000000u;
 
+// Storage geometry (the row-blocked layout is described with the addressing below).
 struct Geom {
     int nx, ny, nz;
-    int SY, SZ;
+    int SX, SY, SZ;  // storage planes, rows per plane (ghost layers included) and row length
+    int GY;          // +z ghost rows at the end of each plane (0 for a periodic z axis)
     int px, py, pz;
-    long long sxy;     // SY*SZ
-    long long stride;  // elements per population array
-    long long zg;      // start of the +z ghost slots (non-periodic z), one per (i, j) storage row
 };
 
 // Objective evaluated on one state: weight * phi(state), phi per cell as
@@ -416,58 +415,74 @@ struct LamStore<P_F16S> {
 // ---------------------------------------------------------------------------------
 // Addressing.
 // ---------------------------------------------------------------------------------
-// Storage indices are 32-bit: the host refuses lattices whose population arrays exceed
-// 2 GiB, so (index * element size), plus the array stride of the opposite array of a
-// pair, is a 32-bit byte offset from a wave-uniform array base, which the compiler turns
-// into SGPR-base + VGPR-offset global accesses with no per-access 64-bit address
-// arithmetic.
-// Rows are whole: a storage row holds the nz cells of one (i, j) and starts on a
-// 128-byte line (SZ is a multiple of 64 elements).  The +z ghost slot of a row (the
-// esoteric-twist slot beyond the last cell of a non-periodic z axis) lives in a
-// separate region at zg + i*SY + j, so that a row of one array is fetched as whole lines
-// with no line shared with the next row.
+// Row-blocked layout of a field of Qn population arrays.  A storage row (i, j) holds the
+// Qn arrays of its z row one after the other, SZ elements each (SZ: nz rounded up to 64,
+// so every array row starts on a 128-byte line):
+//   element (arr, i, j, k) = i * PP + j * RP + apos(arr) * SZ + k,
+//   RP = Qn * SZ (row pitch),  PP = (SY + GY) * RP (plane pitch).
+// A non-periodic x or y axis carries one extra storage plane / row (the esoteric-twist
+// ghost layer).  On a non-periodic z axis the +z ghost slot of row (i, j) sits in GY =
+// ceil(SY / SZ) ghost rows at the end of plane i: element
+//   i * PP + (SY + j / SZ) * RP + apos(arr) * SZ + j % SZ,
+// so a row of one array is fetched as whole lines, and the ghost slots of 64 rows share
+// a line.  Compared with one array after the other (structure of arrays), the 2 x Qn
+// streams of a step then fall in a few kB per row instead of Qn regions a field apart,
+// which measured 6-7 % more bandwidth for the twisted access pattern at 256^3.
+// An address is a row index (64-bit, uniform over a block of a step launch, where the row
+// is (blockIdx.z, blockIdx.y)) plus a 32-bit lane offset within the plane: the global
+// accesses take the row part in SGPRs and the lane part in one VGPR.  The host keeps a
+// plane below 4 GiB at the widest element.
+template <int Qn>
 struct Cursor {
-    uint32_t base;          // storage index of the cell
-    int32_t dx, dy, dz;     // storage offsets of the +x, +y, +z neighbours (wrapped)
-    uint32_t gbase;         // ghost slot of this row (used when zghost)
-    int32_t gdx, gdy;       // ghost-region offsets of the +x, +y rows (wrapped)
-    bool zghost;            // the +z neighbour is the ghost slot (k = nz-1, z not periodic)
-    LBM_DEV uint32_t at(int corner) const {
-        const int32_t xy = ((corner & 1) ? dx : 0) + ((corner & 2) ? dy : 0);
-        if (!(corner & 4)) return base + uint32_t(xy);
-        if (!zghost) return base + uint32_t(xy + dz);
-        return gbase + uint32_t(((corner & 1) ? gdx : 0) + ((corner & 2) ? gdy : 0));
+    struct Loc {
+        long long row;  // element index of the row block
+        uint32_t lane;  // element offset from it (array 0)
+    };
+    long long R[4];  // row blocks of the cell's row and its +x, +y, +x+y neighbours (wrapped)
+    uint32_t k, kz;  // the cell's z position and that of its +z neighbour (wrapped)
+    uint32_t G[4];   // +z ghost slot of each of the four rows, relative to its row block
+    bool zghost;     // the +z neighbour is the ghost slot (k = nz-1, z not periodic)
+    LBM_DEV Loc at(int corner) const {
+        const long long r = R[corner & 3];
+        if (!(corner & 4)) return Loc{r, k};
+        return Loc{r, zghost ? G[corner & 3] : kz};
     }
 };
 
-LBM_DEV Cursor make_cursor(const Geom& g, int i, int j, int k) {
-    Cursor c;
-    c.base = uint32_t(i) * uint32_t(g.sxy) + uint32_t(j) * uint32_t(g.SZ) + uint32_t(k);
-    const bool wx = g.px && i + 1 == g.nx, wy = g.py && j + 1 == g.ny;
-    c.dx = wx ? -int32_t(g.nx - 1) * int32_t(g.sxy) : int32_t(g.sxy);
-    c.dy = wy ? -int32_t(g.ny - 1) * int32_t(g.SZ) : int32_t(g.SZ);
-    c.dz = (g.pz && k + 1 == g.nz) ? -int32_t(g.nz - 1) : 1;
+template <int Qn>
+LBM_HD inline long long row_pitch(const Geom& g) {
+    return (long long)Qn * g.SZ;
+}
+template <int Qn>
+LBM_HD inline long long plane_pitch(const Geom& g) {
+    return (long long)(g.SY + g.GY) * row_pitch<Qn>(g);
+}
+
+template <int Qn>
+LBM_DEV Cursor<Qn> make_cursor(const Geom& g, int i, int j, int k) {
+    Cursor<Qn> c;
+    const long long RP = row_pitch<Qn>(g), PP = plane_pitch<Qn>(g);
+    const int i1 = (g.px && i + 1 == g.nx) ? 0 : i + 1;
+    const int j1 = (g.py && j + 1 == g.ny) ? 0 : j + 1;
+    c.R[0] = i * PP + j * RP;
+    c.R[1] = i1 * PP + j * RP;
+    c.R[2] = i * PP + j1 * RP;
+    c.R[3] = i1 * PP + j1 * RP;
+    c.k = uint32_t(k);
+    c.kz = (g.pz && k + 1 == g.nz) ? 0u : uint32_t(k + 1);
     c.zghost = !g.pz && k + 1 == g.nz;
-    c.gbase = uint32_t(g.zg) + uint32_t(i) * uint32_t(g.SY) + uint32_t(j);
-    c.gdx = wx ? -int32_t(g.nx - 1) * int32_t(g.SY) : int32_t(g.SY);
-    c.gdy = wy ? -int32_t(g.ny - 1) : 1;
+    const uint32_t sz = uint32_t(g.SZ), sy = uint32_t(g.SY), rp = uint32_t(RP);
+    const uint32_t g0 = (sy + uint32_t(j) / sz - uint32_t(j)) * rp + uint32_t(j) % sz;
+    const uint32_t g1 = (sy + uint32_t(j1) / sz - uint32_t(j1)) * rp + uint32_t(j1) % sz;
+    c.G[0] = c.G[1] = g0;
+    c.G[2] = c.G[3] = g1;
     return c;
 }
 
-// Population array `arr` of a field (uniform) and one element of it by storage index.
-template <int Q, class T>
-LBM_DEV T* arr_base(T* pop, const Geom& g, int arr) {
-    return pop + size_t(apos<Q>(arr)) * size_t(g.stride);
-}
-// Element index of (array arr, storage index idx) from the field base.
+// Element index of (array arr, location) from the field base.
 template <int Q>
-LBM_DEV long long field_index(const Geom& g, int arr, long long idx) {
-    return (long long)apos<Q>(arr) * g.stride + idx;
-}
-template <class T>
-LBM_DEV T& elem(T* base, uint32_t idx) {
-    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-    return *reinterpret_cast<T*>(reinterpret_cast<B*>(base) + size_t(idx * uint32_t(sizeof(T))));
+LBM_DEV long long field_index(const Geom& g, int arr, const typename Cursor<Q>::Loc& L) {
+    return L.row + (long long)apos<Q>(arr) * g.SZ + L.lane;
 }
 
 // Directions q whose velocity component along `axis` has sign `sgn`, as a bit mask.
@@ -500,80 +515,79 @@ LBM_DEV uint32_t cell_mask(const Geom& g, const uint32_t* mask, long long cell, 
         return face_mask<Q>(g, i, j, k);
 }
 
-// Read location (array, storage index) of f_q at a cell for a state of parity odd.
+// Read location (element index from the field base) of f_q at a cell for a state of
+// parity odd.
 template <int Q, int q>
-LBM_DEV long long read_loc(const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+LBM_DEV long long read_loc(const Geom& g, const Cursor<Q>& cu, uint32_t m, int odd) {
     const bool bl = (m >> q) & 1u;
     const int arr = ((odd != 0) != bl) ? q : opp<Q>(q);
     return field_index<Q>(g, arr, cu.at(corner_neg<Q>(q)));
 }
 // Write location of f*_q when stepping a state of parity odd.
 template <int Q, int q>
-LBM_DEV long long write_loc(const Geom& g, const Cursor& cu, int odd) {
+LBM_DEV long long write_loc(const Geom& g, const Cursor<Q>& cu, int odd) {
     const int arr = odd ? opp<Q>(q) : q;
     return field_index<Q>(g, arr, cu.at(corner_pos<Q>(q)));
 }
 
-// Element at the read location of f_q (see read_loc): the slot at x + a_opp(q) in the
-// wave-uniform array of an open link, or in the opposite array for a blocked link
-// (halfway bounce-back).  The array choice is a select between two uniform bases.
+// Element `lane` (+ one array when hi) of the array pair starting at array `lo` of the
+// row block at element `row`: a uniform base (row and pair) plus a 32-bit byte offset.
 // OPAQUE: pass the 32-bit byte offset through an empty asm statement first.  A kernel
 // that reads a field at these locations and writes another one there later (the adjoint
 // reads f and writes lambda at the read locations) would otherwise reuse the read's
 // offset across basic blocks as a 64-bit value and address the store with 64-bit VGPR
 // pairs; the barrier keeps the 32-bit offset, and the store takes the uniform base from
 // SGPRs with the offset in one VGPR.
+template <int Q, class T, bool OPAQUE = false>
+LBM_DEV T& pair_elem(T* pop, const Geom& g, long long row, int lo, uint32_t lane, bool hi) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    B* base = reinterpret_cast<B*>(pop + (row + (long long)apos<Q>(lo) * g.SZ));
+    uint32_t off = (lane + (hi ? uint32_t(g.SZ) : 0u)) * uint32_t(sizeof(T));
+    if constexpr (OPAQUE) asm("" : "+v"(off));
+    return *reinterpret_cast<T*>(base + size_t(off));
+}
+
+// Element at the read location of f_q (see read_loc): the slot at x + a_opp(q) in the
+// wave-uniform array of an open link, or in the opposite array for a blocked link
+// (halfway bounce-back).  The array choice is a select of the lane offset: the opposite
+// arrays of a pair are adjacent in the row block.
 template <int Q, int q, class T, bool OPAQUE = false>
-LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+LBM_DEV T& at_read(T* pop, const Geom& g, const Cursor<Q>& cu, uint32_t m, int odd) {
+    const auto L = cu.at(corner_neg<Q>(q));
     if constexpr (q == 0) {
-        uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T));
-        if constexpr (OPAQUE) asm("" : "+v"(off));
-        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-        return *reinterpret_cast<T*>(reinterpret_cast<B*>(arr_base<Q>(pop, g, 0)) + size_t(off));
+        return pair_elem<Q, T, OPAQUE>(pop, g, L.row, 0, L.lane, false);
     } else {
         // Array read: (odd xor blocked) ? q : opp(q); the pair's higher index sits one
         // array after the lower one.
         constexpr int lo = q < opp<Q>(q) ? q : opp<Q>(q);
         constexpr bool q_is_hi = q != lo;
         const bool pick_q = (odd != 0) != (((m >> q) & 1u) != 0);
-        const bool hi = pick_q == q_is_hi;
-        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-        B* base = reinterpret_cast<B*>(arr_base<Q>(pop, g, lo));
-        uint32_t off = cu.at(corner_neg<Q>(q)) * uint32_t(sizeof(T)) +
-                       (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
-        if constexpr (OPAQUE) asm("" : "+v"(off));
-        return *reinterpret_cast<T*>(base + size_t(off));
+        return pair_elem<Q, T, OPAQUE>(pop, g, L.row, lo, L.lane, pick_q == q_is_hi);
     }
 }
 template <int Q, int q, class T>
-LBM_DEV T load_read(const T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd) {
+LBM_DEV T load_read(const T* pop, const Geom& g, const Cursor<Q>& cu, uint32_t m, int odd) {
     return at_read<Q, q>(pop, g, cu, m, odd);
 }
 template <int Q, int q, class T>
-LBM_DEV void store_read(T* pop, const Geom& g, const Cursor& cu, uint32_t m, int odd, T v) {
+LBM_DEV void store_read(T* pop, const Geom& g, const Cursor<Q>& cu, uint32_t m, int odd, T v) {
     at_read<Q, q, T, true>(pop, g, cu, m, odd) = v;
 }
-// Element at the write location of f*_q (see write_loc).
+// Element at the write location of f*_q (see write_loc): array odd ? opp(q) : q.
 template <int Q, int q, class T>
-LBM_DEV T& at_write(T* pop, const Geom& g, const Cursor& cu, int odd) {
+LBM_DEV T& at_write(T* pop, const Geom& g, const Cursor<Q>& cu, int odd) {
+    const auto L = cu.at(corner_pos<Q>(q));
     if constexpr (q == 0) {
-        return elem(arr_base<Q>(pop, g, 0), cu.at(corner_pos<Q>(q)));
+        return pair_elem<Q, T>(pop, g, L.row, 0, L.lane, false);
     } else {
-        // Array written: odd ? opp(q) : q, addressed from the pair's base like at_read so
-        // that reads and writes of a field share one base per pair.
         constexpr int lo = q < opp<Q>(q) ? q : opp<Q>(q);
         constexpr bool q_is_hi = q != lo;
-        const bool hi = (odd == 0) == q_is_hi;
-        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-        B* base = reinterpret_cast<B*>(arr_base<Q>(pop, g, lo));
-        const uint32_t off = cu.at(corner_pos<Q>(q)) * uint32_t(sizeof(T)) +
-                             (hi ? uint32_t(g.stride) * uint32_t(sizeof(T)) : 0u);
-        return *reinterpret_cast<T*>(base + size_t(off));
+        return pair_elem<Q, T>(pop, g, L.row, lo, L.lane, (odd == 0) == q_is_hi);
     }
 }
 
 template <int P, int Q>
-LBM_DEV void load_cell(const Geom& g, const typename Store<P>::T* pop, const Cursor& cu, uint32_t m, int odd,
+LBM_DEV void load_cell(const Geom& g, const typename Store<P>::T* pop, const Cursor<Q>& cu, uint32_t m, int odd,
                        typename Store<P>::C* f) {
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
@@ -582,7 +596,7 @@ LBM_DEV void load_cell(const Geom& g, const typename Store<P>::T* pop, const Cur
 }
 
 template <int P, int Q>
-LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cursor& cu, uint32_t m, int odd,
+LBM_DEV void store_cell_read(const Geom& g, typename Store<P>::T* pop, const Cursor<Q>& cu, uint32_t m, int odd,
                              const typename Store<P>::C* f) {
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
@@ -1053,8 +1067,10 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
     using S = Store<P>;
     using C = typename S::C;
     const Geom& g = A.g;
+    // step launches: a block is a run of blockDim.x cells of one z row (blockDim.y = 1),
+    // so the row (i, j) is uniform over the block
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y, i = blockIdx.z;
     const bool active = k < g.nz && j < g.ny;
     double jsum = 0.0, msum = 0.0;
     float umax = 0.0f, fneg = -INFINITY;
@@ -1063,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
-            const Cursor cu = make_cursor(g, i, j, k);
+            const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
             typename S::T* pop = static_cast<typename S::T*>(A.pop);
             C f[Q], fs[Q];
             load_cell<P, Q>(g, pop, cu, m, A.odd, f);
@@ -1135,8 +1151,9 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             if constexpr (TH) {
                 using TS = TStore<P>;
                 const uint32_t mt = face_mask<TQ>(g, i, j, k);
+                const Cursor<TQ> ct = make_cursor<TQ>(g, i, j, k);  // the temperature field's rows
                 C gv[TQ], go[TQ], T;
-                load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), cu, mt, A.odd, A.tdec, gv);
+                load_tcell<P>(g, static_cast<const typename TS::T*>(A.tpop), ct, mt, A.odd, A.tdec, gv);
                 const C phi = static_cast<const C*>(A.phi)[cell];
                 thermal_collide<C>(gv, mac.u, thermal_omega<C>(A.kappa, A.ratio_m1, phi), C(A.source) * phi, go, T);
                 if (A.tw != 0.0 && phi > C(0.5)) jsum += A.tw * double(T);
@@ -1144,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                 const uint32_t key = (uint32_t(cell) * uint32_t(TQ)) ^ (A.step * 0x9e3779b9u);
                 static_for<0, TQ>([&](auto I) {
                     constexpr int q = decltype(I)::value;
-                    at_write<TQ, q>(to, g, cu, A.odd) = tstore_step<P>(go[q], A.tenc, key + uint32_t(q));
+                    at_write<TQ, q>(to, g, ct, A.odd) = tstore_step<P>(go[q], A.tenc, key + uint32_t(q));
                 });
             }
             typename S::T* out = static_cast<typename S::T*>(A.pop_out);
@@ -1211,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
             const LScale sc = lam_scale<LP>(A.lsc, true);
-            const Cursor cu = make_cursor(g, i, j, k);
+            const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
             const typename S::T* pop = static_cast<const typename S::T*>(A.pop);
             typename L::T* lam = static_cast<typename L::T*>(A.lam);
             C f[Q], gb[Q], fb[Q];
@@ -1381,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_wale_u(Geom g, const void* pop_, const 
         U[3 * n + cell] = C(1);
         return;
     }
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     C f[Q], rho, jv[3];
     load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
     moments<Q>(f, rho, jv);
@@ -1452,7 +1469,7 @@ __global__ __launch_bounds__(256) void k_wale_transpose(Geom g, const uint32_t* 
     if (ub[0] == C(0) && ub[1] == C(0) && ub[2] == C(0)) return;
     const C rho = U[3 * n + y];
     const C u[3] = {U[y], U[n + y], U[2 * n + y]};
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     typename L::T* lam = static_cast<typename L::T*>(lam_);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
@@ -1475,10 +1492,8 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
     using L = LamStore<LP>;
     using C = typename S::C;
     const Geom& g = A.g;
-    // step launches: a block is a run of blockDim.x cells of one z row (blockDim.y = 1),
-    // so the row (i, j) is uniform over the block
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
+    const int j = blockIdx.y, i = blockIdx.z;  // one z row per block, as k_forward
     double jsum = 0.0, msum = 0.0;
     float umax = 0.0f, fneg = -INFINITY;
     bool bad = false;
@@ -1486,7 +1501,7 @@ __global__ __launch_bounds__(256) void k_objective(StepArgs A, ObjArgs seed, voi
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, A.mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
-            const Cursor cu = make_cursor(g, i, j, k);
+            const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
             C f[Q];
             load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, m, A.odd, f);
             const C alpha = static_cast<const C*>(A.alpha)[cell];
@@ -1786,10 +1801,10 @@ LBM_DEV void decode_cell(const Geom& g, long long cell, int& i, int& j, int& k) 
 }
 
 template <int Q, bool MASK>
-LBM_DEV uint32_t mask_of(const PortArgs& A, long long cell, Cursor& cu) {
+LBM_DEV uint32_t mask_of(const PortArgs& A, long long cell, Cursor<Q>& cu) {
     int i, j, k;
     decode_cell(A.g, cell, i, j, k);
-    cu = make_cursor(A.g, i, j, k);
+    cu = make_cursor<Q>(A.g, i, j, k);
     return cell_mask<Q, MASK>(A.g, A.mask, cell, i, j, k);
 }
 
@@ -1805,7 +1820,7 @@ __global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, doub
     double delta = 0.0;
     if (r < A.np) {
         typename S::T* pop = static_cast<typename S::T*>(pop_);
-        Cursor cx, cn;
+        Cursor<Q> cx, cn;
         const uint32_t mx = mask_of<Q, MASK>(A, A.cell[r], cx);
         const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
         C vnb[Q], old[Q], fnb[Q], fout[Q], out[Q];
@@ -1837,7 +1852,7 @@ __global__ __launch_bounds__(128) void k_port_save(PortArgs A, const void* pop_,
     using C = typename S::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= A.np) return;
-    Cursor cn;
+    Cursor<Q> cn;
     const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
     C vnb[Q], fnb[Q];
     load_cell<P, Q>(A.g, static_cast<const typename S::T*>(pop_), cn, mn, A.odd, vnb);
@@ -1856,7 +1871,7 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
     if (r >= A.np) return;
     const LScale sc = lam_scale<LP>(lsc, false);
     typename L::T* lam = static_cast<typename L::T*>(lam_);
-    Cursor cx, cn;
+    Cursor<Q> cx, cn;
     const uint32_t mx = mask_of<Q, MASK>(A, A.cell[r], cx);
     const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
     C gx[Q], gn[Q], fnb[Q], nbb[Q];
@@ -1963,7 +1978,7 @@ __global__ __launch_bounds__(SPONGE_BLOCK) void k_sponge(SpongeArgs A, void* pop
         typename S::T* pop = static_cast<typename S::T*>(pop_);
         int i, j, k;
         decode_cell(A.g, A.cell[r], i, j, k);
-        const Cursor cu = make_cursor(A.g, i, j, k);
+        const Cursor<Q> cu = make_cursor<Q>(A.g, i, j, k);
         C tgt[Q];
         sponge_target<Q, C, S::shifted>(A, r, tgt);
         // in the shifted storages d' = d (1 - sigma) + sum_l s_l (f^eq_l - w), as sigma = sum_l s_l
@@ -1998,7 +2013,7 @@ __global__ __launch_bounds__(SPONGE_BLOCK) void k_sponge_vjp(SpongeArgs A, void*
     typename L::T* lam = static_cast<typename L::T*>(lam_);
     int i, j, k;
     decode_cell(A.g, A.cell[r], i, j, k);
-    const Cursor cu = make_cursor(A.g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(A.g, i, j, k);
     const C keep = C(1.0 - A.sigma[r]);
     static_for<0, Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
@@ -2024,7 +2039,7 @@ __global__ __launch_bounds__(256) void k_get_state(Geom g, const void* pop_, con
         for (int q = 0; q < Q; q++) aos[cell * Q + q] = weight<Q>(q);
         return;
     }
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     C f[Q];
     load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
 #pragma unroll
@@ -2041,7 +2056,7 @@ __global__ __launch_bounds__(256) void k_set_state(Geom g, void* pop_, const uin
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     C f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = S::shifted ? C(aos[cell * Q + q] - weight<Q>(q)) : C(aos[cell * Q + q]);
@@ -2061,7 +2076,7 @@ __global__ __launch_bounds__(256) void k_get_lam(Geom g, const void* lam_, const
         for (int q = 0; q < Q; q++) aos[cell * Q + q] = 0.0;
         return;
     }
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     const typename L::T* lam = static_cast<const typename L::T*>(lam_);
     const LScale sc = lam_scale<LP>(lsc, false);
     static_for<0, Q>([&](auto I) {
@@ -2080,7 +2095,7 @@ __global__ __launch_bounds__(256) void k_set_lam(Geom g, void* lam_, const uint3
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
     const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
     if (m & SOLID_BIT) return;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     typename L::T* lam = static_cast<typename L::T*>(lam_);
     const LScale sc = lam_scale<LP>(lsc, false);
     static_for<0, Q>([&](auto I) {
@@ -2108,7 +2123,7 @@ __global__ __launch_bounds__(256) void k_vector(Geom g, const void* pop_, const 
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
         const uint32_t m = cell_mask<Q, MASK>(g, mask, cell, i, j, k);
         if (!(m & SOLID_BIT)) {
-            const Cursor cu = make_cursor(g, i, j, k);
+            const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
             C f[Q];
             load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
             const C alpha = static_cast<const C*>(alpha_)[cell];
@@ -2163,7 +2178,7 @@ __global__ __launch_bounds__(256) void k_vector_vjp(Geom g, const void* pop_, co
         alpha_bar[cell] = 0.0;
         return;
     }
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
     C f[Q], fb[Q];
     load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
     const C alpha = static_cast<const C*>(alpha_)[cell];
@@ -2194,7 +2209,7 @@ __global__ __launch_bounds__(256) void k_fields(Geom g, const void* pop_, const 
 #pragma unroll
         for (int q = 0; q < Q; q++) f[q] = S::shifted ? C(0) : C(weight<Q>(q));
     } else {
-        const Cursor cu = make_cursor(g, i, j, k);
+        const Cursor<Q> cu = make_cursor<Q>(g, i, j, k);
         load_cell<P, Q>(g, static_cast<const typename S::T*>(pop_), cu, m, odd, f);
     }
     const C a[3] = {C(a0), C(a1), C(a2)};

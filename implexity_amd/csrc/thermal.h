@@ -60,10 +60,11 @@ __global__ __launch_bounds__(256) void k_thermal_adj(ThermalArgs A) {
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<Q> cf = make_cursor<Q>(g, i, j, k);
+    const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
     const uint32_t m = face_mask<Q>(g, i, j, k);
     C f[Q];
-    load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cu, m, A.odd, f);
+    load_cell<P, Q>(g, static_cast<const typename S::T*>(A.pop), cf, m, A.odd, f);
     const C alpha = static_cast<const C*>(A.alpha)[cell];
     const C a[3] = {C(A.a[0]), C(A.a[1]), C(A.a[2])};
     Macro<Q, C> mac;
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(256) void k_thermal_seed(ThermalArgs A, double seed
     double jsum = 0.0;
     if (k < g.nz && j < g.ny) {
         const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-        const Cursor cu = make_cursor(g, i, j, k);
+        const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
         const uint32_t mt = face_mask<TQ>(g, i, j, k);
         const C phi = static_cast<const C*>(A.phi)[cell];
         const bool solid = phi > C(0.5);
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(128) void k_thermal_port(PortArgs A, void* tpop_, d
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= A.np) return;
     typename TS::T* tp = static_cast<typename TS::T*>(tpop_);
-    Cursor cx, cn;
+    Cursor<TQ> cx, cn;
     const uint32_t mx = mask_of<TQ, false>(A, A.cell[r], cx);
     const uint32_t mn = mask_of<TQ, false>(A, A.nb[r], cn);
     C gn[TQ], out[TQ];
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(128) void k_thermal_port_vjp(PortArgs A, void* tlam
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= A.np) return;
     typename L::T* tl = static_cast<typename L::T*>(tlam_);
-    Cursor cx, cn;
+    Cursor<TQ> cx, cn;
     const uint32_t mx = mask_of<TQ, false>(A, A.cell[r], cx);
     const uint32_t mn = mask_of<TQ, false>(A, A.nb[r], cn);
     C gx[TQ], gn[TQ];
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void k_get_tstate(Geom g, const void* tpop, in
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
     typename TS::C gv[TQ];
     load_tcell<P>(g, static_cast<const typename TS::T*>(tpop), cu, face_mask<TQ>(g, i, j, k), odd, tdec, gv);
 #pragma unroll
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(256) void k_set_tstate(Geom g, void* tpop, int odd,
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
     const uint32_t mt = face_mask<TQ>(g, i, j, k);
     typename TS::T* tp = static_cast<typename TS::T*>(tpop);
     static_for<0, TQ>([&](auto I) {
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void k_get_tlam(Geom g, const void* tlam, int 
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
     const uint32_t mt = face_mask<TQ>(g, i, j, k);
     const typename L::T* tl = static_cast<const typename L::T*>(tlam);
     static_for<0, TQ>([&](auto I) {
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_set_tlam(Geom g, void* tlam, int odd, c
     const int j = blockIdx.y * blockDim.y + threadIdx.y, i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
     const long long cell = ((long long)i * g.ny + j) * g.nz + k;
-    const Cursor cu = make_cursor(g, i, j, k);
+    const Cursor<TQ> cu = make_cursor<TQ>(g, i, j, k);
     const uint32_t mt = face_mask<TQ>(g, i, j, k);
     typename L::T* tl = static_cast<typename L::T*>(tlam);
     static_for<0, TQ>([&](auto I) {

@@ -98,7 +98,7 @@ LBM_DEV void thermal_collide_vjp(const C* gv, const C u[3], C om, const C* gb, C
 }
 
 template <int P>
-LBM_DEV void load_tcell(const Geom& g, const typename TStore<P>::T* tp, const Cursor& cu, uint32_t mt, int odd,
+LBM_DEV void load_tcell(const Geom& g, const typename TStore<P>::T* tp, const Cursor<TQ>& cu, uint32_t mt, int odd,
                         float tdec, typename TStore<P>::C* gv) {
     static_for<0, TQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
