@@ -617,24 +617,53 @@ LBM_DEV T wave_sum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
 }
-LBM_DEV float wave_max(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
-    return v;
-}
 
-// Max over the 64 lanes of non-negative floats (compared as their bit patterns) with
-// DPP row shifts and row broadcasts, six VALU operations; lanes shifted in from outside
-// a row read 0.  The result is read from lane 63.
-LBM_DEV float wave_max_nonneg(float v) {
-    unsigned x = __float_as_uint(v);
+// Wave reductions with DPP row shifts and row broadcasts (an inclusive scan whose lane 63
+// holds the result): row_shr 1, 2, 4, 8 within rows of 16 lanes, then row_bcast 15 and
+// 31 across rows; a lane whose source is outside its row, and every lane of a row the
+// broadcast leaves out, reads 0, the neutral value of both reductions below.
+// Max of unsigned values: six VALU operations.
+LBM_DEV unsigned wave_max_u32(unsigned x) {
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, false)));  // row_shr:1
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, false)));  // row_shr:2
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x114, 0xf, 0xf, false)));  // row_shr:4
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x118, 0xf, 0xf, false)));  // row_shr:8
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x142, 0xa, 0xf, false)));  // row_bcast:15
     x = max(x, unsigned(__builtin_amdgcn_update_dpp(0, int(x), 0x143, 0xc, 0xf, false)));  // row_bcast:31
-    return __uint_as_float(unsigned(__builtin_amdgcn_readlane(int(x), 63)));
+    return unsigned(__builtin_amdgcn_readlane(int(x), 63));
+}
+// Max over the 64 lanes of non-negative floats (compared as their bit patterns).
+LBM_DEV float wave_max_nonneg(float v) { return __uint_as_float(wave_max_u32(__float_as_uint(v))); }
+
+// Order-preserving key of a float as an unsigned integer (0 sorts below every float),
+// so that a running max of signed values can be held in a zero-initialised word.
+LBM_DEV unsigned float_key(float v) {
+    const unsigned u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+LBM_DEV float key_float(unsigned x) { return __uint_as_float((x & 0x80000000u) ? (x & 0x7fffffffu) : ~x); }
+
+// A DPP move of a double (its two halves); 0 where the source is outside the row or the
+// row is masked off.
+template <int CTRL, int ROWS>
+LBM_DEV double dpp_f64(double v) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, ROWS, 0xf, false));
+    const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, ROWS, 0xf, false));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+// Sum over the 64 lanes in double precision, in a fixed order (the scan above).
+LBM_DEV double wave_sum_f64(double v) {
+    v += dpp_f64<0x111, 0xf>(v);
+    v += dpp_f64<0x112, 0xf>(v);
+    v += dpp_f64<0x114, 0xf>(v);
+    v += dpp_f64<0x118, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), 63));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), 63));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
 LBM_DEV long long block_linear() {
@@ -649,10 +678,10 @@ constexpr int PART_STRIDE = 8;
 
 LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, float fneg, double* part) {
     __shared__ double red[5][16];
-    jv = wave_sum(jv);
-    mv = wave_sum(mv);
-    umax = wave_max(umax);
-    fneg = wave_max(fneg);
+    jv = wave_sum_f64(jv);
+    mv = wave_sum_f64(mv);
+    umax = wave_max_nonneg(umax);
+    fneg = key_float(wave_max_u32(float_key(fneg)));
     const double nbad = double(__popcll(__ballot(bad)));
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;  // blockDim.x is a multiple of 64
     const int lane = tid & 63, wid = tid >> 6;
@@ -683,12 +712,6 @@ LBM_DEV void block_partials(double jv, double mv, float umax, bool bad, float fn
     }
 }
 
-// Order-preserving key of a float as an unsigned integer (0 sorts below every float),
-// so that a running max of signed values can be held in a zero-initialised word.
-LBM_DEV unsigned float_key(float v) {
-    const unsigned u = __float_as_uint(v);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
 
 
 // Block max of |lambda| (>= 0) raised into one of the LAM_SLOTS slots of the scale
@@ -1141,11 +1164,27 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
             if (A.diag) {
                 msum = double(mac.rho);
                 umax = float(mac.u[0] * mac.u[0] + mac.u[1] * mac.u[1] + mac.u[2] * mac.u[2]);
+                if constexpr (std::is_same<C, float>::value) {
+                    // a population is bad when it is not positive or not finite: its bits,
+                    // less one, as unsigned, reach 0x7f7fffff exactly then (zero, sign bit,
+                    // infinity, NaN); one unsigned max and one min per population
+                    float mn = INFINITY;
+                    unsigned key = 0u;
 #pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
-                    bad |= !(fq > C(0)) || !isfinite(float(fq));
-                    fneg = fmaxf(fneg, -float(fq));
+                    for (int q = 0; q < Q; q++) {
+                        const float fq = S::shifted ? f[q] + float(weight<Q>(q)) : f[q];
+                        mn = fminf(mn, fq);
+                        key = max(key, __float_as_uint(fq) - 1u);
+                    }
+                    bad = key >= 0x7f7fffffu;
+                    fneg = -mn;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const C fq = S::shifted ? f[q] + C(weight<Q>(q)) : f[q];
+                        bad |= !(fq > C(0)) || !isfinite(float(fq));
+                        fneg = fmaxf(fneg, -float(fq));
+                    }
                 }
             }
             if constexpr (TH) {
