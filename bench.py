@@ -45,10 +45,10 @@ VARIANTS = {
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the passes and the scaling are recorded in
 # profiles/pmc_traffic_d3q19_trt_256cubed.txt.
 PMC_TRAFFIC_BYTES_PER_LU = {
-    ("forward", "f16s"): 81.6,
-    ("adjoint", "f16s"): 136.4,
-    ("forward", "f32"): 158.6,
-    ("adjoint", "f32"): 251.6,
+    ("forward", "f16s"): 81.7,
+    ("adjoint", "f16s"): 136.5,
+    ("forward", "f32"): 159.0,
+    ("adjoint", "f32"): 253.0,
 }
 
 
