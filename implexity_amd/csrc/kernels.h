@@ -1064,6 +1064,11 @@ LBM_DEV void collide_vjp(const C* v, C alpha, const C a[3], C omega, C om, C tau
             else
                 collide_pairs_vjp<Q>(m.rho, m.u, m.F, omega, oo, g, fb, cb_rho, cb_u, cb_F);
         } else if constexpr (KIND == K_TRT || KIND == K_BGK) {
+            if constexpr (SH && std::is_same<C, float>::value) {  // the moment-form transpose, chain included
+                const C zero3[3] = {C(0), C(0), C(0)};
+                ab = trt_adjoint_shifted<Q>(m, alpha, a, omega, KIND == K_BGK ? omega : om, g, C(0), zero3, fb);
+                return;
+            }
             if constexpr (std::is_same<C, float>::value)
                 collide_pairs_vjp_fast<Q>(m.rho, m.u, m.F, omega, KIND == K_BGK ? omega : om, g, fb, cb_rho, cb_u,
                                           cb_F);
