@@ -168,6 +168,83 @@ int lat_c(int Q, int q, int d) { return Q == 19 ? cvec<19>(q, d) : cvec<27>(q, d
 double lat_w(int Q, int q) { return Q == 19 ? weight<19>(q) : weight<27>(q); }
 int lat_opp(int Q, int q) { return q == 0 ? 0 : Q - q; }
 
+// Exact rationals for the known-stress operator of the regularized ports: num/den in
+// lowest terms, den > 0; every operation refuses a result beyond 62 bits.
+struct Ratio {
+    long long num = 0, den = 1;
+    static Ratio of(__int128 n, __int128 d) {
+        if (d < 0) n = -n, d = -d;
+        __int128 a = n < 0 ? -n : n, b = d;
+        while (b != 0) {
+            const __int128 t = a % b;
+            a = b;
+            b = t;
+        }
+        if (a > 1) n /= a, d /= a;
+        const __int128 lim = __int128(1) << 62;
+        if (d == 0 || n >= lim || -n >= lim || d >= lim) fail(LBM_ERR_INVALID, "stress operator out of range");
+        return Ratio{(long long)n, (long long)d};
+    }
+    Ratio operator+(const Ratio& o) const { return of(__int128(num) * o.den + __int128(o.num) * den, __int128(den) * o.den); }
+    Ratio operator*(const Ratio& o) const { return of(__int128(num) * o.num, __int128(den) * o.den); }
+};
+
+// known_stress_operator (ports.rs:409-440) of D3Q19 face `face` as op[r * 14 + k]: the
+// pseudo-inverse of the known rows (c_axis != sign, ascending q) of the stress basis
+// 4.5 w_q (Q00, Q11, Q22, 2 Q01, 2 Q02, 2 Q12) weighted by 1/sqrt(w), times 1/sqrt(w)
+// per column.  The known rows B have rank 6, so that is (B^T W^-1 B)^-1 B^T W^-1, a
+// matrix of rationals: it is formed exactly and each entry rounded once to the nearest
+// double (the reference's SVD route agrees to a few units in the last place).
+void known_stress_operator(int face, double* op) {
+    const int axis = face / 2, sign = face % 2 == 0 ? 1 : -1;
+    static const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    std::vector<int> known;
+    for (int q = 0; q < 19; q++)
+        if (cvec<19>(q, axis) != sign) known.push_back(q);
+    if (known.size() != 14) fail(LBM_ERR_INVALID, "stress fit needs 14 known directions");
+    Ratio b[14][6], winv[14];
+    for (int k = 0; k < 14; k++) {
+        const int q = known[k], order = vorder<19>(q);
+        winv[k] = Ratio{order == 0 ? 3 : (order == 1 ? 18 : 36), 1};
+        const Ratio scale = order == 0 ? Ratio{3, 2} : (order == 1 ? Ratio{1, 4} : Ratio{1, 8});  // 4.5 w_q
+        for (int r = 0; r < 6; r++) {
+            const int ca = cvec<19>(q, ia[r]), cb = cvec<19>(q, ib[r]);
+            const Ratio e = r < 3 ? Ratio::of(3 * ca * ca - 1, 3) : Ratio{2 * ca * cb, 1};
+            b[k][r] = scale * e;
+        }
+    }
+    Ratio m[6][12];  // [N | I] -> [I | N^-1], N = B^T W^-1 B
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 12; c++) {
+            Ratio acc{c == r + 6 ? 1 : 0, 1};
+            if (c < 6)
+                for (int k = 0; k < 14; k++) acc = acc + b[k][r] * b[k][c] * winv[k];
+            m[r][c] = acc;
+        }
+    for (int c = 0; c < 6; c++) {
+        int piv = c;
+        while (piv < 6 && m[piv][c].num == 0) piv++;
+        if (piv == 6) fail(LBM_ERR_INVALID, "Regularized port stress fit is rank deficient or ill conditioned");
+        std::swap(m[c], m[piv]);
+        const Ratio inv = Ratio::of(m[c][c].den, m[c][c].num);
+        for (int j = 0; j < 12; j++) m[c][j] = m[c][j] * inv;
+        for (int r = 0; r < 6; r++) {
+            if (r == c || m[r][c].num == 0) continue;
+            const Ratio f{-m[r][c].num, m[r][c].den};
+            for (int j = 0; j < 12; j++) m[r][j] = m[r][j] + f * m[c][j];
+        }
+    }
+    for (int r = 0; r < 6; r++)
+        for (int k = 0; k < 14; k++) {
+            Ratio acc{0, 1};
+            for (int c = 0; c < 6; c++) acc = acc + m[r][6 + c] * b[k][c];
+            acc = acc * winv[k];
+            if (acc.num >= (1LL << 53) || -acc.num >= (1LL << 53) || acc.den >= (1LL << 53))
+                fail(LBM_ERR_INVALID, "stress operator entry beyond exact doubles");
+            op[r * 14 + k] = double(acc.num) / double(acc.den);  // one rounding: both exact
+        }
+}
+
 // Weighted Gram-Schmidt moment basis (moving/collision.rs:87-153): monomials up to
 // degree 6 in the order mass, momentum, bulk, shear, then degrees 3..6; group codes
 // 0 mass, 1 momentum, 2 bulk, 3 shear, 4 odd, 5 even-high.
@@ -260,6 +337,9 @@ struct lbm_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf F, A, L, alpha, alpha_bar, mask, mrt, pcell, pnb, pkind, pval, pbuf, scal, aos, pool, part, lsc, fold;
+    DevBuf stress;            // regularized ports: 6 faces x (6 x 14) known-stress operators
+    long long pre_off = 0;    // regularized ports: byte offset of the pre-port populations
+                              // in every state buffer (after the flow populations), else 0
     DevBuf W;          // working state of a gradient sweep (F keeps state 0 throughout)
     DevBuf ledger;     // mass ledger of the last sweep (grows to the longest horizon)
     bool abar_ready = false;  // alpha_bar holds the result of a gradient or step VJP
@@ -483,6 +563,8 @@ struct lbm_handle {
         p.mask = mask.as<uint32_t>();
         p.odd = int(step & 1);
         p.st = st_ports ? 1 : 0;
+        p.stress = stress.as<double>();
+        p.pre_off = pre_off;
         return p;
     }
 };
@@ -1314,17 +1396,25 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
         std::vector<double> pval;
         std::vector<uint8_t> neighbour_row;  // 1: the row reads its inward neighbour
         std::vector<int> is_port(h->ncell, 0);
+        bool has_regularized = false;
+        std::vector<double> stress_ops;  // the regularized rules' operators, 6 faces x 84
         for (uint32_t p = 0; p < cfg->n_ports; p++) {
             const lbm_port& port = cfg->ports[p];
             if (port.face > LBM_FACE_ZMAX) fail(LBM_ERR_CONTRACT, "unknown port face");
-            if (port.kind > LBM_PORT_PRESSURE_MOMENT)
-                fail(LBM_ERR_CONTRACT, "Port kind must be velocity or pressure (NEE, neighbour or moment rule)");
+            if (port.kind > LBM_PORT_PRESSURE_REGULARIZED)
+                fail(LBM_ERR_CONTRACT,
+                     "Port kind must be velocity or pressure (NEE, neighbour, moment or regularized rule)");
             const bool stationary = port.kind >= LBM_PORT_PRESSURE_NEIGHBOR;
             h->st_ports = h->st_ports || stationary;
-            const bool moment = port.kind >= LBM_PORT_VELOCITY_MOMENT;
+            const bool moment = port.kind >= LBM_PORT_VELOCITY_MOMENT;  // the cell is its own source
+            const bool regularized = port.kind >= LBM_PORT_VELOCITY_REGULARIZED;
             const bool pressure = port.kind == LBM_PORT_PRESSURE || port.kind == LBM_PORT_PRESSURE_NEIGHBOR ||
-                                  port.kind == LBM_PORT_PRESSURE_MOMENT;
+                                  port.kind == LBM_PORT_PRESSURE_MOMENT || port.kind == LBM_PORT_PRESSURE_REGULARIZED;
             if (stationary && h->Q != 19) fail(LBM_ERR_CONTRACT, "the stationary port rules (ports.rs) are D3Q19");
+            if (regularized && (h->a[0] != 0.0 || h->a[1] != 0.0 || h->a[2] != 0.0))
+                fail(LBM_ERR_CONTRACT, "regularized_known_stress ports require zero acceleration; forcing-consistent "
+                                       "stress reconstruction is not implemented");
+            has_regularized = has_regularized || regularized;
             if (stationary && cfg->thermal)
                 fail(LBM_ERR_CONTRACT, "the temperature field takes the NEE velocity and pressure ports");
             const int axis = int(port.face) / 2;
@@ -1386,6 +1476,7 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
                     for (int d = 0; d < 3; d++) pval.push_back(port.value[d]);
                     break;
                 case LBM_PORT_VELOCITY_MOMENT:  // v0 = u - a/2 (ports.rs:352)
+                case LBM_PORT_VELOCITY_REGULARIZED:
                     for (int d = 0; d < 3; d++) pval.push_back(port.value[d] * prof - 0.5 * h->a[d]);
                     break;
                 default:
@@ -1409,6 +1500,16 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
                                        "needs at least four cells)");
         }
         h->np = int(h->h_pcell.size());
+        if (has_regularized) {
+            // the transposed regularized rule reads the port cells' pre-port populations,
+            // which the port pass overwrites: every state buffer keeps them after its flow
+            // populations (the stationary rules take no temperature field)
+            std::vector<double> ops(6 * 84);
+            for (int f = 0; f < 6; f++) known_stress_operator(f, ops.data() + f * 84);
+            stress_ops = ops;
+            h->pre_off = (long long)((h->flow_bytes + 255) / 256 * 256);
+            h->state_bytes = size_t(h->pre_off) + size_t(h->np) * 19 * elem_bytes(h->prec);
+        }
 
         // Sponge layers (moving/sponge.rs:112-181).
         const SpongeField sf = build_sponges(h.get(), cfg);
@@ -1536,6 +1637,11 @@ int lbm_create(const lbm_config* cfg, const uint8_t* solid_mask, uint64_t mask_l
             hip_check(hipMemcpy(h->pnb.p, h->h_pnb.data(), h->pnb.bytes, hipMemcpyHostToDevice), "upload");
             hip_check(hipMemcpy(h->pkind.p, pkind.data(), h->pkind.bytes, hipMemcpyHostToDevice), "upload");
             hip_check(hipMemcpy(h->pval.p, pval.data(), h->pval.bytes, hipMemcpyHostToDevice), "upload");
+            if (!stress_ops.empty()) {
+                h->stress.alloc(stress_ops.size() * sizeof(double), "stress operators");
+                hip_check(hipMemcpy(h->stress.p, stress_ops.data(), h->stress.bytes, hipMemcpyHostToDevice),
+                          "upload");
+            }
         }
         h->budget = cfg->checkpoint_budget_bytes;
         h->every = cfg->checkpoint_every;
@@ -2202,7 +2308,11 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         s.tpop = nullptr;
         s.wu = s.sm = s.d1 = s.sm1 = nullptr;
         hip_check(jr->step(Q, h->kind, h->has_mask, s, lc.grid, lc.block, h->stream), "tangent step launch");
-        hip_check(jr->ports(Q, h->has_mask, h->port_args(t + 1), dual.p, h->stream), "tangent port launch");
+        {
+            PortArgs pa = h->port_args(t + 1);
+            pa.pre_off = 0;  // the tangent state has no pre-port region
+            hip_check(jr->ports(Q, h->has_mask, pa, dual.p, h->stream), "tangent port launch");
+        }
         hip_check(jr->get(Q, h->has_mask, h->g, dual.p, h->mask.as<uint32_t>(), int((t + 1) & 1), tan.as<double>(),
                           lc.grid, lc.block, h->stream),
                   "get tangent");

@@ -1606,6 +1606,10 @@ struct PortArgs {
     const uint32_t* mask;
     int odd;              // parity of the state the ports act on
     int st;               // 1: some rows carry a stationary rule (the kernels with them compiled in)
+    const double* stress; // regularized rules: 6 faces x (6 x 14) known-stress operators, else null
+    long long pre_off;    // regularized rules: byte offset from a state buffer to the pre-port
+                          // populations of its regularized rows (pre[q * np + r], storage type);
+                          // 0: not kept (buffers without the region)
 };
 
 // ---------------------------------------------------------------------------------
@@ -1616,7 +1620,16 @@ struct PortArgs {
 //           the authored tangential velocity (value[1], value[2]) across it;
 //   rule 1  reconstruct (:343-406, "legacy_moment") of a velocity port from the cell's
 //           own populations, value = v0 = u - a/2;
-//   rule 2  the same for a pressure port, value = (rho_b, tangential v0).
+//   rule 2  the same for a pressure port, value = (rho_b, tangential v0);
+//   rules 3, 4  reconstruct_regularized (:455-482, "regularized_known_stress") of a
+//           velocity / pressure port: the moment rule 1 / 2 gives the density and
+//           momentum, and the populations are rebuilt as eq(rho, u) + basis . s with the
+//           six stresses s fitted to the non-equilibrium parts of the 14 known
+//           populations (c_axis != sign) by the operator of known_stress_operator
+//           (:409-440); basis[q] = 4.5 w_q (Q00, Q11, Q22, 2 Q01, 2 Q02, 2 Q12),
+//           Q_ab = c_a c_b - delta_ab / 3.  The rule reads the port cell's own pre-port
+//           populations and is not affine, so its transpose needs them: the forward pass
+//           keeps them beside the state (PortArgs::pre_off).
 // The moment rule replaces the unknown populations (c_axis = sign) by the bounce-back
 // proposal f_opp + 6 w rho (c.v) corrected through the right inverse of (1, c_t0, c_t1)
 // over the unknowns, which for D3Q19 is (1/5, c_t0 / 2, c_t1 / 2).  Sums run in
@@ -1668,8 +1681,20 @@ LBM_DEV void st_equilibrium(C rho, const C u[3], C* eq) {
     }
 }
 
+// Entry r of basis[q] in the reference's floating-point order (ports.rs:413-417);
+// r = 0..5 <-> (00, 11, 22, 01, 02, 12).
+LBM_HD constexpr double st_basis(int q, int r) {
+    const int a = (r == 0 || r == 3 || r == 4) ? 0 : ((r == 1 || r == 5) ? 1 : 2);
+    const int b = r == 0 ? 0 : ((r == 1 || r == 3) ? 1 : 2);
+    const double qq = double(cvec<19>(q, a)) * double(cvec<19>(q, b)) - (a == b ? 1.0 / 3.0 : 0.0);
+    return 4.5 * weight<19>(q) * (r < 3 ? qq : 2.0 * qq);
+}
+
 template <class C>
-LBM_DEV void st_port(const C* f, int kind, const double* value, C* out) {
+LBM_DEV void st_port_moment(const C* f, int rule, int face, const double* value, C* out);
+
+template <class C>
+LBM_DEV void st_port(const C* f, int kind, const double* value, C* out, const double* stress) {
     const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
     const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
     const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
@@ -1686,6 +1711,47 @@ LBM_DEV void st_port(const C* f, int kind, const double* value, C* out) {
         for (int q = 0; q < 19; q++) out[q] = eb[q] + f[q] - el[q];
         return;
     }
+    if (rule <= 2) {
+        st_port_moment(f, rule, face, value, out);
+        return;
+    }
+    // the regularized rules: target = the moment rule, then eq + basis . (op . neq_known)
+    C target[19];
+    st_port_moment(f, rule - 2, face, value, target);
+    C rho, m[3], vel[3], eq[19];
+    st_moments(target, rho, m);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vel[d] = qdiv(m[d], rho);
+    st_equilibrium(rho, vel, eq);
+    const double* op = stress + face * 84;
+    C s[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        C acc = C(0);
+        int k = 0;
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+            if (c == sign) continue;
+            acc += (f[q] - eq[q]) * C(op[r * 14 + k]);
+            k++;
+        }
+        s[r] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        C acc = C(0);
+#pragma unroll
+        for (int r = 0; r < 6; r++) acc += s[r] * C(st_basis(q, r));
+        out[q] = eq[q] + acc;
+    }
+}
+
+// The moment reconstruction (rule 1: velocity, 2: pressure) of the cell's own populations.
+template <class C>
+LBM_DEV void st_port_moment(const C* f, int rule, int face, const double* value, C* out) {
+    const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
+    const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
     // compatibility: zero set (the rest population first) plus twice the outgoing set
     C compat = f[0], out_sum = C(0);
     bool first = true;
@@ -1746,17 +1812,19 @@ LBM_DEV void st_port(const C* f, int kind, const double* value, C* out) {
 
 // Transpose of a stationary rule, sum_r J[r][k] g[r] added to acc[k] in ascending r
 // (apply_ports_vjp, ports.rs:582-621), with J the cell Jacobian formed column by column
-// on dual numbers (cell_jacobian, :532-557): at the source for the neighbour rule, at
-// the lattice weights for the affine moment rules (CompiledPort::new, :514-520).
+// on dual numbers (cell_jacobian, :532-557): at the source for the neighbour and
+// regularized rules, at the lattice weights for the affine moment rules
+// (CompiledPort::new, :514-520).
 template <class C>
-LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g, C* acc) {
-    const bool affine = (kind - ST_PORT_BASE) / 6 != 0;
+LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g, C* acc, const double* stress) {
+    const int rule = (kind - ST_PORT_BASE) / 6;
+    const bool affine = rule == 1 || rule == 2;
 #pragma unroll 1
     for (int k = 0; k < 19; k++) {
         Dual x[19], y[19];
 #pragma unroll
         for (int q = 0; q < 19; q++) x[q] = Dual(affine ? weight<19>(q) : double(src[q]), q == k ? 1.0 : 0.0);
-        st_port(x, kind, value, y);
+        st_port(x, kind, value, y, stress);
         double a = 0.0;
         static_for<0, 19>([&](auto K) {
             constexpr int kk = decltype(K)::value;
@@ -1773,10 +1841,10 @@ LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g
 }
 
 template <int Q, class C, bool ST = false>
-LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out) {
+LBM_DEV void port_populations(const C* nb, int kind, const double* value, C* out, const double* stress) {
     if constexpr (Q == 19 && ST) {
         if (kind >= ST_PORT_BASE) {
-            st_port(nb, kind, value, out);
+            st_port(nb, kind, value, out, stress);
             return;
         }
     }
@@ -1871,9 +1939,19 @@ __global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, doub
         load_cell<P, Q>(A.g, pop, cn, mn, A.odd, vnb);
         load_cell<P, Q>(A.g, pop, cx, mx, A.odd, old);
         to_full<S::shifted, Q>(vnb, fnb);
-        port_populations<Q, C, ST>(fnb, A.kind[r], A.value + 3 * r, fout);
+        const int kind = A.kind[r];
+        port_populations<Q, C, ST>(fnb, kind, A.value + 3 * r, fout, A.stress);
         from_full<S::shifted, Q>(fout, out);
         store_cell_read<P, Q>(A.g, pop, cx, mx, A.odd, out);
+        if constexpr (Q == 19 && ST) {  // the regularized rows keep their pre-port populations
+            if (A.pre_off && kind >= ST_PORT_BASE + 18) {
+                typename S::T* pre = reinterpret_cast<typename S::T*>(reinterpret_cast<char*>(pop_) + A.pre_off);
+                static_for<0, Q>([&](auto I) {
+                    constexpr int q = decltype(I)::value;
+                    pre[(long long)q * A.np + r] = S::template enc<Q, q>(old[q]);
+                });
+            }
+        }
         // Mass exchange of the stored (encoded) values, summed in order of q.
         static_for<0, Q>([&](auto I) {
             constexpr int q = decltype(I)::value;
@@ -1889,17 +1967,27 @@ __global__ __launch_bounds__(128) void k_port_apply(PortArgs A, void* pop_, doub
     }
 }
 
-// Save the populations of every port neighbour (state of parity A.odd) to buf[q*np+r].
+// Save the populations of every port neighbour (state of parity A.odd) to buf[q*np+r];
+// for a regularized row, the port cell's pre-port populations kept beside the state.
 template <int P, int Q, bool MASK>
 __global__ __launch_bounds__(128) void k_port_save(PortArgs A, const void* pop_, typename Store<P>::C* buf) {
     using S = Store<P>;
     using C = typename S::C;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= A.np) return;
-    Cursor<Q> cn;
-    const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
     C vnb[Q], fnb[Q];
-    load_cell<P, Q>(A.g, static_cast<const typename S::T*>(pop_), cn, mn, A.odd, vnb);
+    if (Q == 19 && A.pre_off && A.kind[r] >= ST_PORT_BASE + 18) {
+        const typename S::T* pre =
+            reinterpret_cast<const typename S::T*>(static_cast<const char*>(pop_) + A.pre_off);
+        static_for<0, Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            vnb[q] = S::template dec<Q, q>(pre[(long long)q * A.np + r]);
+        });
+    } else {
+        Cursor<Q> cn;
+        const uint32_t mn = mask_of<Q, MASK>(A, A.nb[r], cn);
+        load_cell<P, Q>(A.g, static_cast<const typename S::T*>(pop_), cn, mn, A.odd, vnb);
+    }
     to_full<S::shifted, Q>(vnb, fnb);
 #pragma unroll
     for (int q = 0; q < Q; q++) buf[(long long)q * A.np + r] = fnb[q];
@@ -1933,7 +2021,7 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
             const bool self = A.nb[r] == A.cell[r];
 #pragma unroll
             for (int q = 0; q < Q; q++) nbb[q] = self ? C(0) : gn[q];
-            st_port_vjp(fnb, kind, A.value + 3 * r, gx, nbb);
+            st_port_vjp(fnb, kind, A.value + 3 * r, gx, nbb, A.stress);
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 if (!self) L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0), sc);

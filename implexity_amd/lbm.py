@@ -251,9 +251,11 @@ def _p(a):
 
 
 # lbm_port_kind: the moving solver's NEE rule (moving/boundary.rs:445-506), and the
-# stationary solver's reconstruct_neighbor (ports.rs:484-497) and moment reconstruction
-# ("legacy_moment", ports.rs:343-406), D3Q19.
-PORT_KIND = {"velocity": 0, "pressure": 1, "pressure_neighbor": 2, "velocity_moment": 3, "pressure_moment": 4}
+# stationary solver's reconstruct_neighbor (ports.rs:484-497), moment reconstruction
+# ("legacy_moment", ports.rs:343-406) and regularized reconstruction
+# ("regularized_known_stress", ports.rs:409-482), D3Q19.
+PORT_KIND = {"velocity": 0, "pressure": 1, "pressure_neighbor": 2, "velocity_moment": 3, "pressure_moment": 4,
+             "velocity_regularized": 5, "pressure_regularized": 6}
 
 
 @dataclass

@@ -17,10 +17,10 @@ D3Q19 BGK-Guo with the relaxation time tau handed over as tau (not as its recipr
 so the library divides by the reference's tau bit for bit; admission (LbmProblem::admit)
 uses the problem's Mach limit.  Ports map onto the library's stationary port kinds with
 their masks: "legacy_moment" to the moment reconstruction (ports.rs:343-406),
-"neighbor_non_equilibrium" to the neighbour rule (:484-497).  Ports with
-"regularized_known_stress" (:408-482), the implicit-geometry source and field
-registration (geometry::masks_from_document, check_registration) are refused with a
-contract error: they stay with the caller's host pipeline.
+"regularized_known_stress" to the regularized reconstruction (:409-482),
+"neighbor_non_equilibrium" to the neighbour rule (:484-497).  The implicit-geometry
+source and field registration (geometry::masks_from_document, check_registration) are
+refused with a contract error: they stay with the caller's host pipeline.
 """
 from __future__ import annotations
 
@@ -246,16 +246,14 @@ def _ports(value, shape, periodic, solid, design_region, fixed, accel, step_s, s
             if not (0.9 <= rho <= 1.1):
                 raise _err("Pressure port density must remain within 10% of reference")
         used |= mask
-        if rec == "regularized_known_stress":
-            raise _err(f"port {pid!r}: regularized_known_stress is not on this path; use legacy_moment or "
-                       "neighbor_non_equilibrium")
         tang = [u[a] for a in range(3) if a != axis]
+        rule = "regularized" if rec == "regularized_known_stress" else "moment"
         if rec == "neighbor_non_equilibrium":
             out.append(Port(face, "pressure_neighbor", (rho, tang[0], tang[1]), mask=mask))
         elif pressure:
-            out.append(Port(face, "pressure_moment", (rho, tang[0], tang[1]), mask=mask))
+            out.append(Port(face, f"pressure_{rule}", (rho, tang[0], tang[1]), mask=mask))
         else:
-            out.append(Port(face, "velocity_moment", tuple(u), mask=mask))
+            out.append(Port(face, f"velocity_{rule}", tuple(u), mask=mask))
         sources.append(np.roll(mask, 1 if low else -1, axis=axis) if rec == "neighbor_non_equilibrium" else None)
     for src in sources:  # ports.rs:264-270
         if src is not None and np.any(src & used):

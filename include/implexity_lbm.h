@@ -92,7 +92,9 @@ typedef enum {
     LBM_PORT_PRESSURE = 1,          /* NEE pressure port, u_b = u_nb (boundary.rs:445-506)     */
     LBM_PORT_PRESSURE_NEIGHBOR = 2, /* ports.rs reconstruct_neighbor (:484-497), D3Q19         */
     LBM_PORT_VELOCITY_MOMENT = 3,   /* ports.rs reconstruct, "legacy_moment" (:343-406), D3Q19 */
-    LBM_PORT_PRESSURE_MOMENT = 4    /* the same with a given density                          */
+    LBM_PORT_PRESSURE_MOMENT = 4,   /* the same with a given density                          */
+    LBM_PORT_VELOCITY_REGULARIZED = 5, /* ports.rs reconstruct_regularized (:455-482), D3Q19  */
+    LBM_PORT_PRESSURE_REGULARIZED = 6  /* the same with a given density                       */
 } lbm_port_kind;
 
 /* One inlet/outlet on a face, applied after streaming.  Port cells are the non-solid
@@ -111,6 +113,11 @@ typedef enum {
  *     v0 = value * profile[r] - a/2 with a the lattice acceleration.
  *   PRESSURE_MOMENT: the same with rho = value[0] given, the normal velocity from it and
  *     tangential v0 = (value[1], value[2]) - a/2.
+ *   VELOCITY_REGULARIZED, PRESSURE_REGULARIZED ("regularized_known_stress",
+ *     ports.rs:409-482): the density and momentum of the MOMENT rule, the populations
+ *     rebuilt as f^eq + the second-order stress basis times the six stresses fitted (least
+ *     squares weighted by 1/w) to the non-equilibrium parts of the 14 known populations.
+ *     Refused with a nonzero acceleration (ports.rs:154-158).
  * A neighbour rule's inward neighbour must be fluid, not a port cell, and not shared with
  * another port cell.  The reference's authoring checks (ports.rs:107-262: no corners on
  * closed faces, protected fluid, Mach and density limits) are the caller's. */
@@ -555,7 +562,8 @@ int lbm_pushforward_vjp(const lbm_pushforward_config* cfg, const double* xi, con
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
 /* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
 int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);
-/* Bytes of one population buffer on the device (storage padding included). */
+/* Bytes of one population buffer on the device (storage padding included, and with
+ * regularized ports the pre-port populations of their cells). */
 uint64_t lbm_state_bytes(const lbm_handle* h);
 
 #ifdef __cplusplus

@@ -1533,10 +1533,14 @@ void orc_port_populations_vjp(int Q, const double *nb, int kind, const double va
 /*                                                                               */
 /* A port row's kind is ST_PORT_BASE + 6 * rule + face (face 0..5 = xmin, xmax,  */
 /* ymin, ymax, zmin, zmax), rule 0 = neighbour pressure, 1 = moment velocity,    */
-/* 2 = moment pressure; its three values are                                     */
+/* 2 = moment pressure, 3 and 4 = the regularized rules of velocity and pressure */
+/* ports (reconstruct_regularized, :455-482, "regularized_known_stress"); its    */
+/* three values are                                                              */
 /*   rule 0: rho_b and the two tangential velocities (ascending axes),          */
-/*   rule 1: v0 = u - a/2 (ports.rs:352), three components,                       */
-/*   rule 2: rho_b and the two tangential components of v0.                      */
+/*   rules 1, 3: v0 = u - a/2 (ports.rs:352), three components,                   */
+/*   rules 2, 4: rho_b and the two tangential components of v0.                  */
+/* (The regularized rules are refused with an acceleration, ports.rs:154-158, so */
+/* their v0 is u.)                                                               */
 /* ------------------------------------------------------------------------- */
 #define ST_PORT_BASE 8
 
@@ -1680,11 +1684,213 @@ static void st_neighbor_d(const dual *nb, int face, double density, const double
     for (int q = 0; q < 19; q++) out[q] = dsub(dadd(eb[q], nb[q]), el[q]);
 }
 
+/* ------------------------------------------------------------------------- */
+/* The known-stress fit of the regularized rules (known_stress_operator,          */
+/* ports.rs:409-440).  basis[q] = 4.5 w_q (Q00, Q11, Q22, 2 Q01, 2 Q02, 2 Q12),   */
+/* Q_ab = c_a c_b - delta_ab / 3, in the reference's floating-point order; the    */
+/* known set is every q with c_axis != sign (14 directions, ascending); the       */
+/* operator is the pseudo-inverse of the known rows of the basis weighted by       */
+/* 1 / sqrt(w), times 1 / sqrt(w) on its columns.  With B the 14 x 6 known rows    */
+/* (rank 6) that is the weighted least-squares left inverse                        */
+/*     op = (B^T W^-1 B)^-1 B^T W^-1,                                             */
+/* a matrix of rationals (the square roots cancel).  It is formed here in exact    */
+/* rational arithmetic and each entry rounded once to the nearest double; the      */
+/* reference's SVD route (faer) lands within a few units in the last place of it.  */
+/* ------------------------------------------------------------------------- */
+typedef struct { long long n, d; } st_rat;  /* d > 0, lowest terms */
+
+static int st_rat_norm(__int128 n, __int128 d, st_rat *out) {
+    if (d == 0) return -1;
+    if (d < 0) { n = -n; d = -d; }
+    __int128 a = n < 0 ? -n : n, b = d;
+    while (b) { __int128 t = a % b; a = b; b = t; }
+    if (a > 1) { n /= a; d /= a; }
+    const __int128 lim = (__int128)1 << 62;
+    if (n >= lim || n <= -lim || d >= lim) return -1;
+    out->n = (long long)n;
+    out->d = (long long)d;
+    return 0;
+}
+
+static int st_rat_add(st_rat a, st_rat b, st_rat *out) {
+    return st_rat_norm((__int128)a.n * b.d + (__int128)b.n * a.d, (__int128)a.d * b.d, out);
+}
+
+static int st_rat_mul(st_rat a, st_rat b, st_rat *out) {
+    return st_rat_norm((__int128)a.n * b.n, (__int128)a.d * b.d, out);
+}
+
+/* 4.5 w_q e as an exact rational: 4.5 w_q = 3/2, 1/4, 1/8 for |c|^2 = 0, 1, 2, and
+ * e = c_a^2 - 1/3 on the diagonal entries, 2 c_a c_b off it. */
+static st_rat st_basis_rat(int q, int r) {
+    static const int A[6] = {0, 1, 2, 0, 0, 1}, B[6] = {0, 1, 2, 1, 2, 2};
+    const int a = A[r], b = B[r];
+    int n2 = 0;
+    for (int d = 0; d < 3; d++) n2 += C19[q][d] * C19[q][d];
+    const long long sd = n2 == 0 ? 2 : (n2 == 1 ? 4 : 8), sn = n2 == 0 ? 3 : 1;
+    st_rat e, out = {0, 0};
+    if (a == b) {
+        e.n = 3LL * C19[q][a] * C19[q][a] - 1;
+        e.d = 3;
+    } else {
+        e.n = 2LL * C19[q][a] * C19[q][b];
+        e.d = 1;
+    }
+    st_rat scale = {sn, sd};
+    if (st_rat_mul(scale, e, &out) != 0) out.d = 0;
+    return out;
+}
+
+/* basis[q][r] in the reference's floating-point order (ports.rs:413-417). */
+static double st_basis(int q, int r) {
+    static const int A[6] = {0, 1, 2, 0, 0, 1}, B[6] = {0, 1, 2, 1, 2, 2};
+    const int a = A[r], b = B[r];
+    const double qq = (double)C19[q][a] * (double)C19[q][b] - (a == b ? 1.0 / 3.0 : 0.0);
+    const double e = r < 3 ? qq : 2.0 * qq;
+    return 4.5 * L19.w[q] * e;
+}
+
+static int st_stress_known(int face, int known[19]) {
+    const int axis = face / 2, sign = face % 2 == 0 ? 1 : -1;
+    int nk = 0;
+    for (int q = 0; q < 19; q++)
+        if (C19[q][axis] != sign) known[nk++] = q;
+    return nk;
+}
+
+/* op[r * 14 + k], r over the six stress components, k over the known set. */
+static int st_stress_operator_exact(int face, double *op) {
+    int known[19];
+    const int nk = st_stress_known(face, known);
+    if (nk != 14) return -1;
+    st_rat bk[14][6], inv_w[14], n[6][12];
+    for (int k = 0; k < nk; k++) {
+        const int q = known[k];
+        int n2 = 0;
+        for (int d = 0; d < 3; d++) n2 += C19[q][d] * C19[q][d];
+        inv_w[k].n = n2 == 0 ? 3 : (n2 == 1 ? 18 : 36);
+        inv_w[k].d = 1;
+        for (int r = 0; r < 6; r++) {
+            bk[k][r] = st_basis_rat(q, r);
+            if (bk[k][r].d == 0) return -1;
+        }
+    }
+    /* [N | I], N = B^T W^-1 B, reduced to [I | N^-1] by Gauss-Jordan with the first
+     * nonzero pivot of each column (exact arithmetic needs no pivot search). */
+    for (int r = 0; r < 6; r++)
+        for (int s = 0; s < 12; s++) {
+            st_rat acc = {s == r + 6 ? 1 : 0, 1};
+            if (s < 6) {
+                acc.n = 0;
+                for (int k = 0; k < nk; k++) {
+                    st_rat t;
+                    if (st_rat_mul(bk[k][r], bk[k][s], &t) || st_rat_mul(t, inv_w[k], &t) ||
+                        st_rat_add(acc, t, &acc))
+                        return -1;
+                }
+            }
+            n[r][s] = acc;
+        }
+    for (int c = 0; c < 6; c++) {
+        int piv = -1;
+        for (int r = c; r < 6 && piv < 0; r++)
+            if (n[r][c].n != 0) piv = r;
+        if (piv < 0) return -1; /* rank below 6 */
+        for (int s = 0; s < 12; s++) {
+            st_rat t = n[c][s];
+            n[c][s] = n[piv][s];
+            n[piv][s] = t;
+        }
+        const st_rat p = n[c][c], pinv = {p.n < 0 ? -p.d : p.d, p.n < 0 ? -p.n : p.n};
+        for (int s = 0; s < 12; s++)
+            if (st_rat_mul(n[c][s], pinv, &n[c][s])) return -1;
+        for (int r = 0; r < 6; r++) {
+            if (r == c || n[r][c].n == 0) continue;
+            const st_rat f = {-n[r][c].n, n[r][c].d};
+            for (int s = 0; s < 12; s++) {
+                st_rat t;
+                if (st_rat_mul(f, n[c][s], &t) || st_rat_add(n[r][s], t, &n[r][s])) return -1;
+            }
+        }
+    }
+    for (int r = 0; r < 6; r++)
+        for (int k = 0; k < nk; k++) {
+            st_rat acc = {0, 1};
+            for (int s = 0; s < 6; s++) {
+                st_rat t;
+                if (st_rat_mul(n[r][6 + s], bk[k][s], &t) || st_rat_add(acc, t, &acc)) return -1;
+            }
+            if (st_rat_mul(acc, inv_w[k], &acc)) return -1;
+            op[r * 14 + k] = (double)acc.n / (double)acc.d; /* both exact: |n|, d < 2^53 */
+            if (acc.n >= (1LL << 53) || acc.n <= -(1LL << 53) || acc.d >= (1LL << 53)) return -1;
+        }
+    return 0;
+}
+
+static double g_stress_op[6][84];
+static int g_stress_ready[6];
+
+static const double *st_stress_operator(int face) {
+    if (!g_stress_ready[face]) {
+        if (st_stress_operator_exact(face, g_stress_op[face]) != 0) return NULL;
+        g_stress_ready[face] = 1;
+    }
+    return g_stress_op[face];
+}
+
+int orc_stationary_stress_operator(int face, double *op, int *known) {
+    init_lattices();
+    if (face < 0 || face > 5) return -1;
+    const double *src = st_stress_operator(face);
+    if (!src) return -1;
+    for (int i = 0; i < 84; i++) op[i] = src[i];
+    return st_stress_known(face, known);
+}
+
+double orc_stationary_stress_basis(int q, int r) {
+    init_lattices();
+    return st_basis(q, r);
+}
+
+/* reconstruct_regularized (ports.rs:455-482): the moment reconstruction `target` of
+ * the cell's own populations f, its equilibrium, the six stresses fitted to the known
+ * populations' non-equilibrium parts, and eq + basis . stress. */
+static void st_regularized_d(const dual *f, int face, const dual *target, dual *out) {
+    int known[19];
+    const int nk = st_stress_known(face, known);
+    const double *op = st_stress_operator(face);
+    dual rho, m[3], vel[3], eq[19], stress[6];
+    if (!op) { /* not reached for D3Q19: every face's fit has rank 6 */
+        for (int q = 0; q < 19; q++) out[q] = dv(NAN);
+        return;
+    }
+    st_moments_d(target, &rho, m);
+    for (int d = 0; d < 3; d++) vel[d] = ddiv(m[d], rho);
+    st_equilibrium_d(rho, vel, eq);
+    for (int r = 0; r < 6; r++) {
+        dual acc = dv(0.0);
+        for (int k = 0; k < nk; k++)
+            acc = dadd(acc, dmuls(dsub(f[known[k]], eq[known[k]]), op[r * 14 + k]));
+        stress[r] = acc;
+    }
+    for (int q = 0; q < 19; q++) {
+        dual acc = dv(0.0);
+        for (int r = 0; r < 6; r++) acc = dadd(acc, dmuls(stress[r], st_basis(q, r)));
+        out[q] = dadd(eq[q], acc);
+    }
+}
+
 static void st_port_d(const dual *src, int kind, const double value[3], dual *out) {
     const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
     const int axis = face / 2;
     if (rule == 0) {
         st_neighbor_d(src, face, value[0], value + 1, out);
+        return;
+    }
+    if (rule >= 3) {
+        dual target[19];
+        st_port_d(src, kind - 12, value, target);
+        st_regularized_d(src, face, target, out);
         return;
     }
     double v0[3];
@@ -1707,15 +1913,17 @@ void orc_stationary_port(const double *src, int kind, const double value[3], dou
 }
 
 /* Cell Jacobian jac[r * 19 + k] = d out_r / d src_k (cell_jacobian, ports.rs:532-557).
- * The moment rules are affine, so their Jacobian is taken once at the lattice weights,
- * as CompiledPort::new does (:514-520); the neighbour rule at the source itself. */
+ * The "legacy_moment" rules (1, 2) are affine, so their Jacobian is taken once at the
+ * lattice weights, as CompiledPort::new does (:514-520); the neighbour and regularized
+ * rules at the source itself. */
 void orc_stationary_port_jacobian(const double *src, int kind, const double value[3], double *jac) {
     init_lattices();
     const int rule = (kind - ST_PORT_BASE) / 6;
+    const int at_src = rule == 0 || rule >= 3;
     dual x[19], y[19];
     for (int k = 0; k < 19; k++) {
         for (int q = 0; q < 19; q++) {
-            x[q].v = rule == 0 ? src[q] : L19.w[q];
+            x[q].v = at_src ? src[q] : L19.w[q];
             x[q].d = q == k ? 1.0 : 0.0;
         }
         st_port_d(x, kind, value, y);

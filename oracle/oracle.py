@@ -84,6 +84,7 @@ def lib():
         _lib.orc_psm_saturate.restype = d
         _lib.orc_obj_solid_force.restype = d
         _lib.orc_psm_saturate.argtypes = [d, d]
+        _lib.orc_stationary_stress_basis.restype = d
         del pd
     return _lib
 
@@ -165,9 +166,15 @@ def psm_saturate(d: float, kappa: float) -> float:
 
 # Port kinds: the moving solver's NEE rule (boundary.rs:445-506) and the stationary
 # solver's rules (ports.rs): "pressure_neighbor" = reconstruct_neighbor (:484-497),
-# "velocity_moment" / "pressure_moment" = reconstruct (:343-406, "legacy_moment").
-PORT_KINDS = ("velocity", "pressure", "pressure_neighbor", "velocity_moment", "pressure_moment")
+# "velocity_moment" / "pressure_moment" = reconstruct (:343-406, "legacy_moment"),
+# "velocity_regularized" / "pressure_regularized" = reconstruct_regularized (:455-482,
+# "regularized_known_stress").
+PORT_KINDS = ("velocity", "pressure", "pressure_neighbor", "velocity_moment", "pressure_moment",
+              "velocity_regularized", "pressure_regularized")
+_SELF_SOURCE = ("velocity_moment", "pressure_moment", "velocity_regularized", "pressure_regularized")
 ST_PORT_BASE = 8  # row kind of a stationary rule: ST_PORT_BASE + 6 * rule + face
+_REGULARIZED_ACCEL = ("regularized_known_stress ports require zero acceleration; forcing-consistent stress "
+                      "reconstruction is not implemented")  # ports.rs:154-158
 
 
 @dataclass
@@ -458,7 +465,7 @@ class Oracle:
             if prof is not None and len(prof) != len(face):
                 raise ValueError("profile needs one factor per port cell")
             fi = 2 * axis + (0 if inward > 0 else 1)
-            moment = port.kind in ("velocity_moment", "pressure_moment")
+            moment = port.kind in _SELF_SOURCE
             acc = [float(a) for a in self.prob.accel]
             for r, x in enumerate(face):
                 c = list(self.coords(int(x)))
@@ -478,11 +485,15 @@ class Oracle:
                 elif port.kind == "pressure_neighbor":
                     kinds.append(ST_PORT_BASE + fi)
                     values.append([float(v) for v in port.value])
-                elif port.kind == "velocity_moment":
-                    kinds.append(ST_PORT_BASE + 6 + fi)
+                elif port.kind in ("velocity_moment", "velocity_regularized"):
+                    if port.kind == "velocity_regularized" and any(a != 0.0 for a in acc):
+                        raise ValueError(_REGULARIZED_ACCEL)
+                    kinds.append(ST_PORT_BASE + (6 if port.kind == "velocity_moment" else 18) + fi)
                     values.append([float(port.value[d]) * factor - 0.5 * acc[d] for d in range(3)])
-                elif port.kind == "pressure_moment":
-                    kinds.append(ST_PORT_BASE + 12 + fi)
+                elif port.kind in ("pressure_moment", "pressure_regularized"):
+                    if port.kind == "pressure_regularized" and any(a != 0.0 for a in acc):
+                        raise ValueError(_REGULARIZED_ACCEL)
+                    kinds.append(ST_PORT_BASE + (12 if port.kind == "pressure_moment" else 24) + fi)
                     tang = [d for d in range(3) if d != axis]
                     values.append([float(port.value[0])] + [float(port.value[1 + t]) - 0.5 * acc[d]
                                                             for t, d in enumerate(tang)])
@@ -529,6 +540,25 @@ class Oracle:
         jac = np.zeros(19 * 19)
         lib().orc_stationary_port_jacobian(_ptr(src), ctypes.c_int(kind), _ptr(v), _ptr(jac))
         return jac.reshape(19, 19)
+
+    @staticmethod
+    def stationary_stress_operator(face):
+        """known_stress_operator (ports.rs:409-440) of a face: (known q (14,), operator
+        (6, 14)), the operator the exact rational weighted least-squares left inverse
+        rounded once per entry."""
+        op = np.zeros(84)
+        known = np.zeros(19, np.int32)
+        nk = lib().orc_stationary_stress_operator(ctypes.c_int(face), _ptr(op),
+                                                  known.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        if nk != 14:
+            raise ValueError("stress fit of a face without rank 6")
+        return known[:nk].copy(), op.reshape(6, 14)
+
+    @staticmethod
+    def stationary_stress_basis():
+        """basis (ports.rs:413-417): (19, 6), 4.5 w_q (Q00, Q11, Q22, 2 Q01, 2 Q02, 2 Q12)."""
+        return np.array([[lib().orc_stationary_stress_basis(ctypes.c_int(q), ctypes.c_int(r)) for r in range(6)]
+                         for q in range(19)])
 
     @staticmethod
     def stationary_right_inverse(face):

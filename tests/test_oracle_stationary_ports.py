@@ -16,8 +16,25 @@ The pins are properties the rules are built to have, not their formulas:
 * the Jacobian equals central differences, and for the (affine) moment rules is the
   same at every point;
 * the step with these ports passes central differences of its VJP.
+
+The regularized rules of "regularized_known_stress" ports (reconstruct_regularized,
+:455-482, with the stress fit of known_stress_operator, :409-440) are pinned by
+* the operator: numpy's SVD pseudo-inverse of the weighted known rows (the reference's
+  route, with its rank-6 and condition-number checks) within 1e-14, and bit for bit the
+  correctly rounded value of the exact rational operator, which is identified by the two
+  identities that define a weighted least-squares left inverse (op B = I and
+  B^T W^-1 (I - B op) = 0) holding exactly in rationals;
+* a regularized state eq(rho, u) + basis . s with the port's own boundary values is a
+  fixed point, and the rule is idempotent on any cell;
+* the incoming (unknown) populations do not reach the result;
+* the result has the density and momentum of the moment reconstruction (the stress
+  basis carries neither);
+* central differences of the Jacobian (taken at the source, not affine) and of the step
+  VJP.
 """
 from __future__ import annotations
+
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -149,12 +166,13 @@ def test_neighbor_rule_differs_from_moving_pressure_rule():
     assert abs(js[1]) < 1e-16 and abs(jm[1]) > 1e-3
 
 
-@pytest.mark.parametrize("kind_off,face", [(0, 1), (0, 2), (6, 0), (6, 5), (12, 3), (12, 4)])
+@pytest.mark.parametrize("kind_off,face", [(0, 1), (0, 2), (6, 0), (6, 5), (12, 3), (12, 4), (18, 1), (18, 2),
+                                            (24, 0), (24, 5)])
 def test_jacobian_matches_central_differences(kind_off, face):
     rng = np.random.default_rng(kind_off + face)
     f = random_cell(rng)
     kind = BASE + kind_off + face
-    value = (1.004, 0.003, -0.002) if kind_off != 6 else (0.01, -0.02, 0.015)
+    value = (0.01, -0.02, 0.015) if kind_off in (6, 18) else (1.004, 0.003, -0.002)
     jac = O.stationary_port_jacobian(f, kind, value)
     h = 1e-6
     fd = np.empty((19, 19))
@@ -163,8 +181,10 @@ def test_jacobian_matches_central_differences(kind_off, face):
         e[k] = h
         fd[:, k] = (O.stationary_port(f + e, kind, value) - O.stationary_port(f - e, kind, value)) / (2 * h)
     assert np.abs(jac - fd).max() < 1e-8
-    if kind_off:  # affine: the same Jacobian everywhere
+    if kind_off in (6, 12):  # affine: the same Jacobian everywhere
         assert np.array_equal(jac, O.stationary_port_jacobian(f * 1.3 + 0.01, kind, value))
+    else:
+        assert not np.array_equal(jac, O.stationary_port_jacobian(f * 1.3 + 0.01, kind, value))
 
 
 def _problem(ports, accel=(0.0, 0.0, 0.0), shape=(6, 5, 4)):
@@ -207,3 +227,156 @@ def test_moment_port_cells_and_values():
     assert cells == expect and nbs == expect
     assert kinds == [BASE + 12 + 1] * 2
     assert values[0] == [1.01, 0.004 - 0.5 * 2e-4, -0.003 + 0.5 * 6e-4]
+
+
+# --- regularized_known_stress ---------------------------------------------------------
+
+def stress_basis():
+    """(19, 6): 4.5 w (Q00, Q11, Q22, 2 Q01, 2 Q02, 2 Q12), Q = c c - I/3."""
+    out = np.empty((19, 6))
+    for q in range(19):
+        Qm = np.outer(C[q], C[q]) - np.eye(3) / 3
+        out[q] = 4.5 * W[q] * np.array([Qm[0, 0], Qm[1, 1], Qm[2, 2], 2 * Qm[0, 1], 2 * Qm[0, 2], 2 * Qm[1, 2]])
+    return out
+
+
+def frac_basis():
+    w = [Fraction(1, 3) if not C[q].any() else (Fraction(1, 18) if np.abs(C[q]).sum() == 1 else Fraction(1, 36))
+         for q in range(19)]
+    out = []
+    for q in range(19):
+        c = [int(v) for v in C[q]]
+        Qm = [[Fraction(c[a] * c[b]) - (Fraction(1, 3) if a == b else 0) for b in range(3)] for a in range(3)]
+        out.append([Fraction(9, 2) * w[q] * e for e in (Qm[0][0], Qm[1][1], Qm[2][2], 2 * Qm[0][1], 2 * Qm[0][2],
+                                                         2 * Qm[1][2])])
+    return out, w
+
+
+@pytest.mark.parametrize("face", range(6))
+def test_stress_operator_matches_the_svd_pseudo_inverse(face):
+    axis, sign = face_axis_sign(face)
+    known = [q for q in range(19) if C[q, axis] != sign]
+    got_known, op = O.stationary_stress_operator(face)
+    assert list(got_known) == known
+    B = stress_basis()
+    assert np.array_equal(O.stationary_stress_basis(), B)
+    weighted = B[known] / np.sqrt(W[known])[:, None]
+    assert np.linalg.matrix_rank(weighted) == 6
+    assert np.linalg.cond(weighted) <= 1e12
+    ref = np.linalg.pinv(weighted) / np.sqrt(W[known])[None, :]
+    assert np.abs(op - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("face", range(6))
+def test_stress_operator_is_the_correctly_rounded_exact_operator(face):
+    """The exact operator has small denominators; recover it from the doubles and check
+    that it is the weighted least-squares left inverse exactly, then that every double is
+    its nearest (float(Fraction) rounds correctly)."""
+    known, op = O.stationary_stress_operator(face)
+    fb, fw = frac_basis()
+    B = [fb[q] for q in known]
+    winv = [1 / fw[q] for q in known]
+    opf = [[Fraction(float(v)).limit_denominator(100000) for v in row] for row in op]
+    for r in range(6):
+        for s_ in range(6):  # op B = I
+            assert sum(opf[r][k] * B[k][s_] for k in range(14)) == (1 if r == s_ else 0)
+    # B^T W^-1 (I - B op) = 0
+    resid = [[(1 if i == j else 0) - sum(B[i][s_] * opf[s_][j] for s_ in range(6)) for j in range(14)]
+             for i in range(14)]
+    for a in range(6):
+        for j in range(14):
+            assert sum(B[i][a] * winv[i] * resid[i][j] for i in range(14)) == 0
+    assert all(float(opf[r][k]) == op[r, k] for r in range(6) for k in range(14))
+
+
+def _regularized(face, pressure, v):
+    axis, _ = face_axis_sign(face)
+    tang = [a for a in range(3) if a != axis]
+    if pressure:
+        return BASE + 24 + face, lambda rho: (rho, v[tang[0]], v[tang[1]])
+    return BASE + 18 + face, lambda rho: tuple(v)
+
+
+@pytest.mark.parametrize("face", range(6))
+@pytest.mark.parametrize("pressure", [False, True])
+def test_regularized_fixed_point_and_idempotence(face, pressure):
+    rng = np.random.default_rng(60 + face + 6 * pressure)
+    rho, u = 1.012, np.array([0.023, -0.017, 0.011])
+    s = rng.standard_normal(6) * 1e-3
+    f = equilibrium(rho, u) + stress_basis() @ s
+    kind, value = _regularized(face, pressure, u)
+    out = O.stationary_port(f, kind, value(rho))
+    assert np.abs(out - f).max() < 2e-16
+    g = random_cell(rng)
+    once = O.stationary_port(g, kind, value(0.997))
+    twice = O.stationary_port(once, kind, value(0.997))
+    assert np.abs(twice - once).max() < 2e-16
+    assert np.abs(once - g).max() > 1e-4
+
+
+@pytest.mark.parametrize("face", range(6))
+@pytest.mark.parametrize("pressure", [False, True])
+def test_regularized_ignores_incoming_populations(face, pressure):
+    rng = np.random.default_rng(80 + face + 6 * pressure)
+    axis, sign = face_axis_sign(face)
+    f = random_cell(rng)
+    kind, value = _regularized(face, pressure, np.array([0.02, 0.01, -0.015]))
+    out = O.stationary_port(f, kind, value(1.006))
+    g = f.copy()
+    unknown = C[:, axis] == sign
+    g[unknown] += rng.standard_normal(unknown.sum()) * 1e-2
+    assert np.array_equal(O.stationary_port(g, kind, value(1.006)), out)
+    g = f.copy()
+    g[~unknown] += rng.standard_normal((~unknown).sum()) * 1e-6
+    assert not np.array_equal(O.stationary_port(g, kind, value(1.006)), out)
+
+
+@pytest.mark.parametrize("face", range(6))
+@pytest.mark.parametrize("pressure", [False, True])
+def test_regularized_moments_are_the_moment_rules(face, pressure):
+    rng = np.random.default_rng(100 + face + 6 * pressure)
+    f = random_cell(rng, neq=5e-3)
+    v = np.array([0.015, -0.02, 0.01])
+    kind, value = _regularized(face, pressure, v)
+    out = O.stationary_port(f, kind, value(1.004))
+    target = O.stationary_port(f, kind - 12, value(1.004))
+    r, j = moments(out)
+    rt, jt = moments(target)
+    assert abs(r - rt) < 2e-15 and np.abs(j - jt).max() < 2e-15
+    assert np.abs(out - target).max() > 1e-4  # the stress is refitted, not copied
+
+
+def test_regularized_port_rows_and_acceleration():
+    shape = (6, 5, 4)
+    mask = np.zeros(shape, bool)
+    mask[0, 1:4, 1:3] = True
+    ports = [orc.Port("xmin", "velocity_regularized", (0.02, 0.001, 0.0), mask=mask),
+             orc.Port("xmax", "pressure_regularized", (0.998, 0.003, 0.0))]
+    o = orc.Oracle(_problem(ports))
+    cells, nbs, kinds, values = o.port_cells()
+    assert cells == nbs
+    assert kinds[:6] == [BASE + 18] * 6 and kinds[6:] == [BASE + 24 + 1] * (len(kinds) - 6)
+    assert values[0] == [0.02, 0.001, 0.0] and values[6] == [0.998, 0.003, 0.0]
+    with pytest.raises(ValueError, match="zero acceleration"):
+        orc.Oracle(_problem(ports, accel=(0.0, 1e-5, 0.0))).port_cells()
+
+
+def test_step_vjp_central_differences_with_regularized_ports():
+    shape = (6, 5, 4)
+    mask = np.zeros(shape, bool)
+    mask[0, 1:-1, :] = True
+    ports = [orc.Port("xmin", "velocity_regularized", (0.02, 0.0, 0.004), mask=mask),
+             orc.Port("xmax", "pressure_regularized", (0.998, 0.002, 0.0))]
+    o = orc.Oracle(_problem(ports))
+    rng = np.random.default_rng(17)
+    alpha = rng.uniform(0, 0.05, o.n)
+    f = o.run(alpha, 3)
+    lam = rng.standard_normal(o.n * 19)
+    v = rng.standard_normal(o.n * 19) * 1e-3
+    fb, _ = o.interval_vjp(alpha, f, lam)
+    h = 1e-6
+    near = np.zeros(shape, bool)
+    near[:2], near[-2:] = True, True
+    for vv in (v, v * np.repeat(near.reshape(-1), 19)):
+        fd = (o.interval(alpha, f + h * vv) - o.interval(alpha, f - h * vv)) / (2 * h)
+        assert abs(lam @ fd - fb @ vv) <= 1e-8 * np.abs(lam).sum() * np.abs(fd).max()

@@ -149,6 +149,16 @@ def test_ports_map_to_stationary_kinds():
     assert outlet.mask[:, -1].all() and outlet.mask.sum() == SHAPE[0] * SHAPE[2]
     p2 = problem.normalize(_channel(recon_out="legacy_moment"))
     assert p2.ports[1].kind == "pressure_moment"
+    p3 = problem.normalize(_channel(recon_in="regularized_known_stress", recon_out="regularized_known_stress"))
+    assert [q.kind for q in p3.ports] == ["velocity_regularized", "pressure_regularized"]
+    assert p3.ports[0].value == inlet.value and p3.ports[1].value == (rho, 0.0, 0.0)
+
+
+def test_regularized_ports_refused_with_acceleration():
+    p = _channel(recon_in="regularized_known_stress", recon_out="legacy_moment")
+    p["acceleration_m_s2"] = [0.0, 0.0, 1e-3]
+    with pytest.raises(ContractError, match="regularized_known_stress ports require zero acceleration"):
+        problem.normalize(p)
 
 
 PORT_MISMATCH = [
@@ -165,7 +175,6 @@ PORT_MISMATCH = [
     (lambda p: p["ports"][1].update(velocity_m_s=[0.0, 1e-5, 0.0]), "normal component must be zero"),
     (lambda p: p["ports"][1].update(gauge_pressure_Pa=1.0), "within 10% of reference"),
     (lambda p: p["ports"][1].update(gauge_pressure_Pa="1"), "Finite scalar gauge pressure"),
-    (lambda p: p["ports"][0].update(reconstruction="regularized_known_stress"), "not on this path"),
 ]
 
 
