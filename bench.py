@@ -52,14 +52,35 @@ PMC_TRAFFIC_BYTES_PER_LU = {
 }
 
 
-def algorithmic_bytes(which, variant, Q=19):
+ABAR_RUN = 32  # reverse steps whose alpha_bar terms one FP32 partial gathers (FP32 arithmetic)
+
+
+def abar_bytes(variant, steps=None):
+    """alpha_bar bytes per lattice update of an adjoint step.  FP64: the FP64 read and
+    write (16 B).  FP32 arithmetic: runs of ABAR_RUN reverse steps gather it in an FP32
+    partial -- the first step of a run writes it (4 B), the middle ones read and write it
+    (8 B), the last adds it and its own term to the FP64 alpha_bar (4 + 8 + 8 B); a run of
+    one step updates alpha_bar directly (16 B).  steps None: a middle step (isolated
+    launches); else the average over a sweep of that many reverse steps."""
+    if variant == "f64":
+        return 16.0
+    if steps is None:
+        return 8.0
+
+    def run(n):
+        return 16 if n == 1 else 4 + 8 * (n - 2) + 20
+    full, rest = divmod(steps, ABAR_RUN)
+    return (full * run(ABAR_RUN) + (run(rest) if rest else 0)) / steps
+
+
+def algorithmic_bytes(which, variant, Q=19, steps=None):
     """Bytes per lattice update: the forward step reads and writes every population once
     and reads alpha; the adjoint step reads the state, reads and writes every cotangent,
-    reads alpha and updates the FP64 alpha_bar (16 B)."""
+    reads alpha and updates alpha_bar (abar_bytes)."""
     _, _, s, l, a = VARIANTS[variant]
     if which == "forward":
         return Q * s * 2 + a
-    return Q * s + Q * l * 2 + a + 16
+    return Q * s + Q * l * 2 + a + abar_bytes(variant, steps)
 
 
 def log(msg):
@@ -395,7 +416,7 @@ def main():
             t = res[prec]["run_fwd_ms" if which == "forward" else "run_adj_ms"]
         else:
             t = res[prec]["t_fwd_ms" if which == "forward" else "t_adj_ms"]
-        gbs = algorithmic_bytes(which, prec) * n / (t * 1e-3) / 1e9
+        gbs = algorithmic_bytes(which, prec, steps=args.steps) * n / (t * 1e-3) / 1e9
         return t, gbs
 
     # dominant kernel of the timed run: forward (incl. recomputation) or adjoint
@@ -427,8 +448,9 @@ def main():
                                f"{args.budget_gb:g} GB", "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": gbs_dom, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs_dom / HBM_PEAK_GBS, "traffic": traffic, "kernel": f"{dom} ({main_prec})",
-                     "algorithmic_bytes_per_launch": algorithmic_bytes(dom, main_prec) * n,
-                     "algorithmic_bytes_per_lu": algorithmic_bytes(dom, main_prec), "avg_kernel_ms": t_dom,
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(dom, main_prec, steps=args.steps) * n,
+                     "algorithmic_bytes_per_lu": algorithmic_bytes(dom, main_prec, steps=args.steps),
+                     "avg_kernel_ms": t_dom,
                      "avg_kernel_ms_source": "HIP events on the library stream around every min(64, K/4)-th step kernel of the timed gradient",
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-kernel scaled"},
     }

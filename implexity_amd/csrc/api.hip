@@ -168,6 +168,11 @@ int lat_c(int Q, int q, int d) { return Q == 19 ? cvec<19>(q, d) : cvec<27>(q, d
 double lat_w(int Q, int q) { return Q == 19 ? weight<19>(q) : weight<27>(q); }
 int lat_opp(int Q, int q) { return q == 0 ? 0 : Q - q; }
 
+// Reverse steps whose alpha_bar terms one FP32 partial gathers before the fold into
+// FP64: the partial's rounding stays near 32 ulp of FP32 per run, far below the
+// reduced-precision paths' own error, and the fold costs one FP64 read and write a run.
+constexpr uint64_t ABAR_RUN = 32;
+
 // Exact rationals for the known-stress operator of the regularized ports: num/den in
 // lowest terms, den > 0; every operation refuses a result beyond 62 bits.
 struct Ratio {
@@ -338,6 +343,7 @@ struct lbm_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf F, A, L, alpha, alpha_bar, mask, mrt, pcell, pnb, pkind, pval, pbuf, scal, aos, pool, part, lsc, fold;
     DevBuf stress;            // regularized ports: 6 faces x (6 x 14) known-stress operators
+    DevBuf abar32;            // FP32 partial of alpha_bar over a run of reverse steps
     long long pre_off = 0;    // regularized ports: byte offset of the pre-port populations
                               // in every state buffer (after the flow populations), else 0
     DevBuf W;          // working state of a gradient sweep (F keeps state 0 throughout)
@@ -741,6 +747,9 @@ void check_lam_range(lbm_handle* h) {
 }
 
 void ensure_adjoint_buffers(lbm_handle* h) {
+    if (h->Q == 27 && h->prec == LBM_F64 && (h->kind == K_TRT_PSM || h->kind == K_BGK_PSM))
+        fail(LBM_ERR_CONTRACT, "the FP64 D3Q27 Psm adjoint is switched off (a GPU memory fault with no established "
+                               "cause); use D3Q19, FP32 or FP16 storage, or the PsmSuperposition or Brinkman law");
     if (h->lam16() && !h->lsc.p) {
         h->lsc.alloc(LAM_REC_INTS * sizeof(int), "cotangent scale");
         hip_check(hipMemsetAsync(h->lsc.p, 0, h->lsc.bytes, h->stream), "memset");
@@ -752,6 +761,9 @@ void ensure_adjoint_buffers(lbm_handle* h) {
         hip_check(hipMemsetAsync(h->W.p, 0, h->W.bytes, h->stream), "memset");
     }
     h->alpha_bar.alloc(size_t(h->ncell) * sizeof(double), "alpha_bar");
+    // FP32 arithmetic: the gradient driver gathers alpha_bar over runs of reverse steps in
+    // an FP32 partial (8 bytes a cell a step instead of the FP64 read and write's 16)
+    if (h->prec != LBM_F64) h->abar32.alloc(size_t(h->ncell) * sizeof(float), "alpha_bar partial");
     if (h->thermal) {
         h->phi_bar.alloc(size_t(h->ncell) * sizeof(double), "phi_bar");
         h->ubar.alloc(size_t(h->ncell) * 3 * h->csize(), "velocity cotangent");
@@ -2424,6 +2436,7 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
         uint64_t cursor = 0;  // state index held in F
         ObjArgs none{};
         AdjArgs aa = h->adj_args(h->F.p, 0);
+        uint64_t abar_run = 0;  // reverse steps gathered in the FP32 alpha_bar partial
 
         // The cursor state lives in `cur` (the working buffer F or a snapshot slot).  A
         // Restore only moves `cur` to the slot; the next step reads the slot and writes F
@@ -2537,6 +2550,12 @@ int lbm_gradient_seeded(lbm_handle* h, uint64_t nsteps, const lbm_objective* obj
                     aa.ubar = h->ubar.p;
                 }
                 aa.s_next = k >= 2 ? seed_bound(h, stage_obj(tmpl, obj, k - 2, nsteps, false)) : 0.0;
+                if (h->abar32.p) {  // runs of ABAR_RUN steps, the last one folded at k = 1
+                    const bool fold = k == 1 || abar_run + 1 == ABAR_RUN;
+                    aa.abar32 = (abar_run == 0 && fold) ? nullptr : h->abar32.as<float>();
+                    aa.abar_mode = abar_run == 0 ? ABAR_FIRST : (fold ? ABAR_FOLD : ABAR_MID);
+                    abar_run = fold ? 0 : abar_run + 1;
+                }
                 {
                     const auto* ks = kspan(adj_k, rsteps);
                     if (ks) hip_check(hipEventRecord(ks->first, h->stream), "event record");
@@ -2849,7 +2868,12 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             lam_init(h, 0.0, 0.0);
             AdjArgs a = h->adj_args(h->F.p, 0);
             a.odd = int(h->t & 1);
+            // FP32 arithmetic: the timed launches are the middle of a gradient's run of
+            // reverse steps (alpha_bar gathered in the FP32 partial)
+            a.abar32 = h->abar32.as<float>();
+            a.abar_mode = ABAR_FIRST;
             flow_adjoint(h, a);  // untimed
+            a.abar_mode = ABAR_MID;
             hip_check(hipEventRecord(h->ev0, h->stream), "event");
             for (uint64_t s = 0; s < launches; s++) {
                 a.odd = int((h->t + s) & 1);

@@ -176,7 +176,15 @@ struct AdjArgs {
     double* sm1_bar;
     // Subcycle: cotangents of the impulse shares g0, g1 (as StepArgs::gacc), or null.
     const double* gbar;
+    // Runs of reverse steps may gather alpha_bar in an FP32 partial per cell (abar32) and
+    // fold it into the FP64 alpha_bar once per run: ABAR_FIRST writes the partial,
+    // ABAR_MID adds to it, ABAR_FOLD adds partial + this step's term to alpha_bar.
+    // abar32 null: every step adds to alpha_bar directly.
+    float* abar32;
+    int abar_mode;
 };
+
+enum { ABAR_FIRST = 0, ABAR_MID = 1, ABAR_FOLD = 2 };
 
 // 32-bit integer hash (xorshift-multiply, constants of the "lowbias32" finaliser).
 LBM_DEV uint32_t mix32(uint32_t x) {
@@ -1402,7 +1410,14 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
                     brinkman_u_vjp<Q, C>(mac, ub, fb, ab);
                 }
             }
-            A.alpha_bar[cell] += double(ab);
+            if (!A.abar32)
+                A.alpha_bar[cell] += double(ab);
+            else if (A.abar_mode == ABAR_FIRST)
+                A.abar32[cell] = float(ab);
+            else if (A.abar_mode == ABAR_MID)
+                A.abar32[cell] += float(ab);
+            else
+                A.alpha_bar[cell] += double(A.abar32[cell]) + double(ab);
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 store_read<Q, q>(lam, g, cu, m, A.odd, L::template enc<C>(fb[q], sc));

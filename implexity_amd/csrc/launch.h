@@ -160,8 +160,12 @@ struct RunnerImpl final : Runner {
             if (kind == K_MRT) LBM_LAUNCH((k_adjoint<P, LP, Q, K_MRT, M>), c, a);
             if (kind == K_TRT_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_LES, M>), c, a);
             if (kind == K_BGK_LES) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_LES, M>), c, a);
-            if (kind == K_TRT_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_PSM, M>), c, a);
-            if (kind == K_BGK_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSM, M>), c, a);
+            // The FP64 D3Q27 Psm adjoints are switched off (not compiled): two of their runs
+            // took a GPU memory fault whose cause is not established; the host refuses them.
+            if constexpr (!(P == P_F64 && Q == 27)) {
+                if (kind == K_TRT_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_PSM, M>), c, a);
+                if (kind == K_BGK_PSM && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSM, M>), c, a);
+            }
             if (kind == K_TRT_PSMS && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_PSMS, M>), c, a);
             if (kind == K_BGK_PSMS && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_BGK_PSMS, M>), c, a);
             if (kind == K_TRT_BRO && a.sm && a.sm_bar) LBM_LAUNCH((k_adjoint<P, LP, Q, K_TRT_BRO, M>), c, a);

@@ -14,7 +14,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from implexity_amd.lbm import Lattice, Objective
+from implexity_amd.lbm import ContractError, Lattice, Objective
 from oracle import oracle as orc
 from tests.lbm_cases import channel_ports, make_pair, random_alpha, random_solid
 
@@ -51,6 +51,13 @@ CASES = [
 ]
 
 
+# The FP64 D3Q27 Psm adjoints are switched off in the library (two of their runs took a
+# GPU memory fault with no established cause); the library refuses them.
+Q27_PSM_REASON = "FP64 D3Q27 Psm adjoint switched off: GPU memory fault with no established cause"
+VJP_CASES = [pytest.param(*c, marks=pytest.mark.skip(reason=Q27_PSM_REASON)) if c[2] == 27 and c[4] == "psm" else c
+             for c in CASES]
+
+
 def _case(shape, periodic, Q, coll, law, with_solid, with_ports, precision="f64", seed=7):
     n = int(np.prod(shape))
     solid = random_solid(shape, 5) if with_solid else None
@@ -81,7 +88,7 @@ def test_psm_forward_f64_bit_exact(shape, periodic, Q, coll, law, with_solid, wi
     assert np.array_equal(got, want), np.abs(got - want).max()
 
 
-@pytest.mark.parametrize("shape,periodic,Q,coll,law,with_solid,with_ports", CASES)
+@pytest.mark.parametrize("shape,periodic,Q,coll,law,with_solid,with_ports", VJP_CASES)
 def test_psm_step_vjp_f64(shape, periodic, Q, coll, law, with_solid, with_ports):
     cfg, prob, solid, d, m = _case(shape, periodic, Q, coll, law, with_solid, with_ports)
     o = orc.Oracle(prob)
@@ -101,6 +108,17 @@ def test_psm_step_vjp_f64(shape, periodic, Q, coll, law, with_solid, with_ports)
     assert np.abs(got_d - want_d).max() <= 1e-12 * np.abs(want_d).max()
     assert np.abs(got_m - want_m).max() <= 1e-12 * np.abs(want_m).max()
     assert np.all(got_d[d == 0] == 0.0)
+
+
+def test_psm_adjoint_refuses_fp64_d3q27():
+    """The switched-off FP64 D3Q27 Psm adjoint is refused on the host before any launch."""
+    cfg, prob, solid, d, m = _case((6, 4, 5), (True, True, False), 27, "bgk", "psm", False, False)
+    o = orc.Oracle(prob)
+    f = o.run(d, 2)
+    with _lattice(cfg, d, m) as lat:
+        lat.set_state(f)
+        with pytest.raises(ContractError, match="FP64 D3Q27 Psm adjoint is switched off"):
+            lat.interval_vjp(np.zeros(o.n * 27))
 
 
 @pytest.mark.parametrize("law", ["psm", "psm_superposition", "brinkman_occupancy"])
