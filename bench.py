@@ -46,9 +46,9 @@ VARIANTS = {
 # profiles/pmc_traffic_d3q19_trt_256cubed.txt.
 PMC_TRAFFIC_BYTES_PER_LU = {
     ("forward", "f16s"): 81.7,
-    ("adjoint", "f16s"): 136.5,
+    ("adjoint", "f16s"): 127.8,
     ("forward", "f32"): 159.0,
-    ("adjoint", "f32"): 253.0,
+    ("adjoint", "f32"): 243.2,
 }
 
 
