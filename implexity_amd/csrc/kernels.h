@@ -438,8 +438,8 @@ struct LamStore<P_F16S> {
 // which measured 6-7 % more bandwidth for the twisted access pattern at 256^3.
 // An address is a row index (64-bit, uniform over a block of a step launch, where the row
 // is (blockIdx.z, blockIdx.y)) plus a 32-bit lane offset within the plane: the global
-// accesses take the row part in SGPRs and the lane part in one VGPR.  The host keeps a
-// plane below 4 GiB at the widest element.
+// accesses take the row part in SGPRs and the lane part in one VGPR (an unsigned 32-bit
+// byte offset).  The host refuses a lattice whose x plane of populations reaches 2 GiB.
 template <int Qn>
 struct Cursor {
     struct Loc {
