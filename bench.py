@@ -328,6 +328,7 @@ def main():
             tf = lat.time_kernel("forward", args.launches)
             ta = lat.time_kernel("adjoint", args.launches)
             ts = lat.time_kernel("stream", args.launches)
+            tfd = lat.time_kernel("forward_diagnostics", args.launches)
             if args.sponge:
                 t_sp = lat.time_kernel("sponge", args.launches)
                 t_spv = lat.time_kernel("sponge_transpose", args.launches)
@@ -385,7 +386,7 @@ def main():
         n = int(np.prod(shape))
         out = {"precision": args.kernel_precision, "les_constant": args.les, "wale_constant": args.wale,
                "coupling": args.coupling, **sp, "forward_ms": tf, "adjoint_ms": ta,
-               "stream_only_ms": ts,
+               "stream_only_ms": ts, "forward_diagnostics_ms": tfd,
                "forward_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (tf * 1e-3) / 1e9,
                "adjoint_GBs": algorithmic_bytes("adjoint", args.kernel_precision) * n / (ta * 1e-3) / 1e9,
                "stream_only_GBs": algorithmic_bytes("forward", args.kernel_precision) * n / (ts * 1e-3) / 1e9,

@@ -2279,10 +2279,11 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         const int Q = h->Q;
         if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "tangent length must be cells*Q");
         const bool kind_ok = h->kind == K_BGK || h->kind == K_TRT || h->kind == K_MRT || h->kind == K_BGK_GUO ||
-                             h->kind == K_CUMULANT;
+                             h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES;
         if (!kind_ok || h->thermal || h->nsc)
             fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
-                                   "Brinkman operators without closures, sponges or the temperature field");
+                                   "Brinkman operators and the Smagorinsky closure, without the WALE closure, "
+                                   "occupancy couplings, sponges or the temperature field");
         // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
         // takes 32-bit byte offsets within an x plane of the field.
         if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(Q) * sizeof(Dual) >= (uint64_t(1) << 31))
@@ -2785,6 +2786,7 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
     return guarded([&] {
         if (!h || !avg_ms) fail(LBM_ERR_CONTRACT, "null argument");
         if (launches == 0) fail(LBM_ERR_CONTRACT, "launches must be positive");
+        if (which < 0 || which > 7) fail(LBM_ERR_CONTRACT, "unknown kernel selector");
         ObjArgs none{};
         if (which == 4) {
             // thermal adjoint kernel alone, in place (the state is not meaningful afterwards)
@@ -2828,7 +2830,7 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             hip_check(hipEventRecord(h->ev0, h->stream), "event");
             for (uint64_t s = 0; s < launches; s++) launch(s + 1);
             hip_check(hipEventRecord(h->ev1, h->stream), "event");
-        } else if (which == 0 || which == 2 || which == 3) {
+        } else if (which == 0 || which == 2 || which == 3 || which == 7) {
             // forward steps in place on the live state (no ports: the kernel alone); which
             // 2 runs the same addressing with the collision replaced by a copy (the
             // streaming-only memory ceiling of the layout); which 3 the flow kernel that
@@ -2838,6 +2840,10 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
             auto args = [&](uint64_t t) {
                 StepArgs sa = h->step_args(h->F.p, t);
                 if (which != 3) sa.tpop = nullptr;
+                if (which == 7) {  // a first-sweep step: admission diagnostics, partials folded
+                    sa.diag = 1;
+                    sa.part = h->part.as<double>();
+                }
                 return sa;
             };
             if (is_wale(h->kind))  // velocities the WALE kernel reads (the pre-pass itself is not timed)

@@ -14,6 +14,13 @@ struct Dual {
     __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
     __host__ __device__ constexpr Dual(double x, double t) : v(x), d(t) {}
     __host__ __device__ explicit constexpr operator double() const { return v; }
+    // d sqrt(x) = dx / (2 sqrt(x)), the value the FP64 square root (the Smagorinsky
+    // closure).  A hidden friend: found for Dual arguments only, so it does not hide
+    // ::sqrt for the other arithmetic types.
+    friend LBM_DEV Dual sqrt(const Dual& a) {
+        const double r = ::sqrt(a.v);
+        return Dual(r, a.d / (2.0 * r));
+    }
     __host__ __device__ explicit constexpr operator float() const { return float(v); }
     LBM_DEV Dual& operator+=(const Dual& b) {
         v += b.v;

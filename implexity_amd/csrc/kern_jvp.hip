@@ -11,6 +11,8 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
     case K_BGK: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK, M>), grid, block, 0, s, a); break;
     case K_TRT: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT, M>), grid, block, 0, s, a); break;
     case K_MRT: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_MRT, M>), grid, block, 0, s, a); break;
+    case K_TRT_LES: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_LES, M>), grid, block, 0, s, a); break;
+    case K_BGK_LES: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_LES, M>), grid, block, 0, s, a); break;
     default:
         if constexpr (Q == 19) {
             if (kind == K_BGK_GUO) {
@@ -32,7 +34,8 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
 hipError_t JvpRunner::step(int Q, int kind, bool mask, const StepArgs& a, dim3 grid, dim3 block,
                            hipStream_t s) const {
     if (!a.pop || !a.pop_out || !a.alpha || (mask && !a.mask) || a.part || a.tpop || a.gacc) return hipErrorInvalidValue;
-    if ((kind == K_MRT || kind == K_CUMULANT) && !a.mrt) return hipErrorInvalidValue;
+    if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt)
+        return hipErrorInvalidValue;
     if (Q == 19) return mask ? step_q<19, true>(kind, a, grid, block, s) : step_q<19, false>(kind, a, grid, block, s);
     if (Q == 27) return mask ? step_q<27, true>(kind, a, grid, block, s) : step_q<27, false>(kind, a, grid, block, s);
     return hipErrorInvalidValue;

@@ -41,10 +41,6 @@
 #include "dual.h"
 #include "les.h"
 
-#ifndef LBM_FLOW_SR
-#define LBM_FLOW_SR 0
-#endif
-
 namespace lbm {
 
 enum Prec : int { P_F64 = 0, P_F32 = 1, P_F16S = 2, P_BF16S = 3 };
@@ -1217,17 +1213,6 @@ __global__ __launch_bounds__(256) void k_forward(StepArgs A) {
                 });
             }
             typename S::T* out = static_cast<typename S::T*>(A.pop_out);
-#if LBM_FLOW_SR
-            if constexpr (P == P_F16S) {
-                const uint32_t hsh = mix32(uint32_t(cell) ^ (A.step * 0x9e3779b9u));
-                static_for<0, Q>([&](auto I) {
-                    constexpr int q = decltype(I)::value;
-                    constexpr float sc = float(1u << fp16_exp<Q>(q));
-                    const uint32_t r = (hsh * (0x9e3779b1u + 0x3c6ef372u * uint32_t(q))) >> 19;
-                    at_write<Q, q>(out, g, cu, A.odd) = f16_stochastic(fs[q] * sc, r);
-                });
-            } else
-#endif
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 at_write<Q, q>(out, g, cu, A.odd) = S::template enc<Q, q>(fs[q]);

@@ -716,7 +716,7 @@ class Lattice:
     def time_kernel(self, which="forward", launches=10):
         ms = ctypes.c_double(0.0)
         code = {"forward": 0, "adjoint": 1, "stream": 2, "coupled_forward": 3, "thermal_adjoint": 4, "sponge": 5,
-                "sponge_transpose": 6}[which]
+                "sponge_transpose": 6, "forward_diagnostics": 7}[which]
         _check(lib().lbm_time_kernel(self._h, code, launches, ctypes.byref(ms)))
         return ms.value
 

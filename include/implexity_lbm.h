@@ -458,7 +458,8 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
  * + J_alpha d_alpha (AoS N*Q; d_alpha has one entry per cell, or NULL for 0), computed in
  * FP64 forward mode through the step's own collision, streaming and port code.  The
  * handle's state is not advanced.  Covers the BGK, TRT, MRT, regularized, cumulant and
- * BGK-Guo Brinkman operators (no closures, sponges or temperature field). */
+ * BGK-Guo Brinkman operators and the Smagorinsky closure on BGK/TRT (not the WALE
+ * closure, the occupancy couplings, sponges or the temperature field). */
 int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, double* d_next, uint64_t len);
 
 int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n);
@@ -557,8 +558,9 @@ int lbm_pushforward_vjp(const lbm_pushforward_config* cfg, const double* xi, con
 /* Timing support: average device time of `launches` back-to-back launches of one kernel
  * (0 forward step, 1 adjoint step, 2 forward addressing with the collision replaced by
  * a copy, 3 forward step of the flow and the coupled temperature field, 4 thermal
- * adjoint step, 5 sponge blend pass, 6 sponge transpose) on the handle's stream,
- * measured with HIP events. */
+ * adjoint step, 5 sponge blend pass, 6 sponge transpose, 7 forward step as a gradient's
+ * first sweep runs it: admission diagnostics on and the per-block partials folded) on
+ * the handle's stream, measured with HIP events. */
 int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms);
 /* Device copy bandwidth (GB/s, read+write bytes) of a plain copy kernel. */
 int lbm_copy_bandwidth(uint64_t bytes, uint64_t iters, double* gbps, double* avg_ms);

@@ -6,6 +6,9 @@ crates/implexity-ad/src/scan.rs:46-80) through the C ABI, in FP64:
 * the dot-product identity <lambda, J v> = <J^T lambda, v> with the device's own step
   VJP (lbm_step_vjp) on the other side, to rounding -- two independent device codes,
   forward mode through the step kernels and the hand-written transposed kernels.
+
+The operators: BGK, TRT, MRT (default and non-default rates), regularized, BGK-Guo,
+cumulant, and the Smagorinsky closure (psm.rs:161-197) on BGK and TRT.
 """
 from __future__ import annotations
 
@@ -31,6 +34,10 @@ JVP_CASES = [
     ((6, 5, 4), (True, False, False), 27, "mrt", False, False, (1.3, 0.8)),
     ((6, 5, 4), (True, False, False), 27, "cumulant", True, False, (0.8, None)),
     ((7, 5, 4), (False, False, False), 27, "cumulant", False, True, (None, None)),
+    # the Smagorinsky closure (rates = (les constant, norm floor) for these)
+    ((7, 6, 5), (True, False, False), 19, "trt_les", True, False, (0.15, 1e-6)),
+    ((7, 6, 5), (False, False, False), 19, "bgk_les", False, True, (0.15, 1e-6)),
+    ((6, 5, 4), (True, False, False), 27, "trt_les", False, False, (0.15, 1e-6)),
 ]
 
 
@@ -41,11 +48,14 @@ def _setup(shape, periodic, Q, coll, with_solid, with_ports, rates, omega=1.4):
         solid = solid.copy()
         solid[:2] = False
         solid[-2:] = False
+    les = (0.0, 0.0)
+    if coll.endswith("_les"):  # the closure on BGK / TRT
+        coll, les, rates, omega = coll[:-4], rates, (None, None), 1.8
     bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
     even = rates[1] if rates[1] is not None else float("nan")
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", omega=omega,
                           accel=(0, 0, 0) if with_ports else (1e-4, -3e-5, 0), solid=solid, ports=ports,
-                          bulk_rate=bulk, even_rate=even)
+                          bulk_rate=bulk, even_rate=even, les=les)
     return cfg, prob, solid
 
 
@@ -103,7 +113,7 @@ def test_step_jvp_vjp_dot_product(shape, periodic, Q, coll, with_solid, with_por
 
 
 def test_step_jvp_refuses_closures():
-    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", les=(0.1, 0.0))
+    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", wale=(0.5, 1e-12))
     with Lattice(cfg) as lat:
         with pytest.raises(ContractError, match="lbm_step_jvp covers"):
             lat.interval_jvp(np.zeros(lat.n * 19))
