@@ -2280,10 +2280,10 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "tangent length must be cells*Q");
         const bool kind_ok = h->kind == K_BGK || h->kind == K_TRT || h->kind == K_MRT || h->kind == K_BGK_GUO ||
                              h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES;
-        if (!kind_ok || h->thermal || h->nsc)
+        if (!kind_ok || h->thermal)
             fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
-                                   "Brinkman operators and the Smagorinsky closure, without the WALE closure, "
-                                   "occupancy couplings, sponges or the temperature field");
+                                   "Brinkman operators, the Smagorinsky closure and sponge layers, without the "
+                                   "WALE closure, occupancy couplings or the temperature field");
         // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
         // takes 32-bit byte offsets within an x plane of the field.
         if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(Q) * sizeof(Dual) >= (uint64_t(1) << 31))
@@ -2321,6 +2321,8 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         s.tpop = nullptr;
         s.wu = s.sm = s.d1 = s.sm1 = nullptr;
         hip_check(jr->step(Q, h->kind, h->has_mask, s, lc.grid, lc.block, h->stream), "tangent step launch");
+        if (h->nsc)  // the blend of the sponge cells' post-collision values, before the ports
+            hip_check(jr->sponge(Q, h->sponge_args(t), dual.p, h->stream), "tangent sponge launch");
         {
             PortArgs pa = h->port_args(t + 1);
             pa.pre_off = 0;  // the tangent state has no pre-port region

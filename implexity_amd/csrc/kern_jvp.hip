@@ -41,6 +41,21 @@ hipError_t JvpRunner::step(int Q, int kind, bool mask, const StepArgs& a, dim3 g
     return hipErrorInvalidValue;
 }
 
+// The sponge blend on dual numbers: tangent (1 - sigma), the far field a constant.
+hipError_t JvpRunner::sponge(int Q, const SpongeArgs& a, void* pop, hipStream_t s) const {
+    if (a.n <= 0) return hipSuccess;
+    if (!pop || a.ns <= 0 || a.ns > MAX_SPONGES || !a.cell || !a.sigma || !a.off || !a.part || !a.weight || !a.phase)
+        return hipErrorInvalidValue;
+    const dim3 grid((a.n + SPONGE_BLOCK - 1) / SPONGE_BLOCK), block(SPONGE_BLOCK);
+    if (Q == 19)
+        hipLaunchKernelGGL((k_sponge<P_JVP, 19>), grid, block, 0, s, a, pop, nullptr);
+    else if (Q == 27)
+        hipLaunchKernelGGL((k_sponge<P_JVP, 27>), grid, block, 0, s, a, pop, nullptr);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t JvpRunner::ports(int Q, bool mask, const PortArgs& a, void* pop, hipStream_t s) const {
     if (a.np <= 0) return hipSuccess;
     if (!a.cell || !a.nb || !a.kind || !a.value || (mask && !a.mask)) return hipErrorInvalidValue;

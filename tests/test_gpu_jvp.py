@@ -8,7 +8,8 @@ crates/implexity-ad/src/scan.rs:46-80) through the C ABI, in FP64:
   forward mode through the step kernels and the hand-written transposed kernels.
 
 The operators: BGK, TRT, MRT (default and non-default rates), regularized, BGK-Guo,
-cumulant, and the Smagorinsky closure (psm.rs:161-197) on BGK and TRT.
+cumulant, and the Smagorinsky closure (psm.rs:161-197) on BGK and TRT; sponge layers
+(moving/sponge.rs) with a steady and a travelling-wave far field.
 """
 from __future__ import annotations
 
@@ -38,7 +39,22 @@ JVP_CASES = [
     ((7, 6, 5), (True, False, False), 19, "trt_les", True, False, (0.15, 1e-6)),
     ((7, 6, 5), (False, False, False), 19, "bgk_les", False, True, (0.15, 1e-6)),
     ((6, 5, 4), (True, False, False), 27, "trt_les", False, False, (0.15, 1e-6)),
+    # sponge layers (moving/sponge.rs): an affine blend after the collision
+    ((8, 6, 5), (False, True, True), 19, "trt_sponge", True, False, (None, None)),
+    ((7, 6, 6), (False, False, False), 19, "bgk_sponge", False, True, (None, None)),
+    ((6, 5, 6), (True, True, False), 27, "trt_sponge", False, False, (None, None)),
 ]
+
+_WAVE = orc.Wave((0.01, -0.004, 0.002), (0.6, 0.8, 0.0), 0.3, 0.02, phase=0.4, ramp=6.0, origin=(1.0, 2.0, 0.0))
+
+
+def _sponges(periodic, with_ports):
+    if with_ports:  # the ports hold the x faces
+        return [orc.Sponge("ymax", 2, 0.4, density=1.002, velocity=(0.01, 0.0, 0.0), wave=_WAVE)]
+    if not periodic[0]:  # x faces, one with a travelling wave
+        return [orc.Sponge("xmin", 2, 0.5, density=1.0, velocity=(0.02, 0.0, 0.0), wave=_WAVE),
+                orc.Sponge("xmax", 2, 0.4, density=1.002)]
+    return [orc.Sponge("zmax", 2, 0.35, density=1.01, velocity=(0.01, 0.0, 0.0), wave=_WAVE)]
 
 
 def _setup(shape, periodic, Q, coll, with_solid, with_ports, rates, omega=1.4):
@@ -48,14 +64,16 @@ def _setup(shape, periodic, Q, coll, with_solid, with_ports, rates, omega=1.4):
         solid = solid.copy()
         solid[:2] = False
         solid[-2:] = False
-    les = (0.0, 0.0)
+    les, sponges = (0.0, 0.0), ()
     if coll.endswith("_les"):  # the closure on BGK / TRT
         coll, les, rates, omega = coll[:-4], rates, (None, None), 1.8
+    if coll.endswith("_sponge"):
+        coll, sponges = coll[:-7], _sponges(periodic, with_ports)
     bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
     even = rates[1] if rates[1] is not None else float("nan")
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", omega=omega,
                           accel=(0, 0, 0) if with_ports else (1e-4, -3e-5, 0), solid=solid, ports=ports,
-                          bulk_rate=bulk, even_rate=even, les=les)
+                          bulk_rate=bulk, even_rate=even, les=les, sponges=sponges)
     return cfg, prob, solid
 
 
