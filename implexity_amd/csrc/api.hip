@@ -2279,11 +2279,12 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         const int Q = h->Q;
         if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "tangent length must be cells*Q");
         const bool kind_ok = h->kind == K_BGK || h->kind == K_TRT || h->kind == K_MRT || h->kind == K_BGK_GUO ||
-                             h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES;
+                             h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES ||
+                             is_wale(h->kind);
         if (!kind_ok || h->thermal)
             fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
-                                   "Brinkman operators, the Smagorinsky closure and sponge layers, without the "
-                                   "WALE closure, occupancy couplings or the temperature field");
+                                   "Brinkman operators, the Smagorinsky and WALE closures and sponge layers, "
+                                   "without the occupancy couplings or the temperature field");
         // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
         // takes 32-bit byte offsets within an x plane of the field.
         if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(Q) * sizeof(Dual) >= (uint64_t(1) << 31))
@@ -2320,6 +2321,14 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         s.mrt = dc.p;
         s.tpop = nullptr;
         s.wu = s.sm = s.d1 = s.sm1 = nullptr;
+        DevBuf du;
+        if (is_wale(h->kind)) {  // u and rho of every cell, with their tangents, before the in-place step
+            du.alloc(size_t(h->ncell) * 4 * sizeof(Dual), "tangent WALE velocities");
+            hip_check(jr->wale_u(Q, h->has_mask, h->g, dual.p, h->mask.as<uint32_t>(), int(t & 1), du.p, lc.grid,
+                                 lc.block, h->stream),
+                      "tangent WALE velocity launch");
+            s.wu = du.p;
+        }
         hip_check(jr->step(Q, h->kind, h->has_mask, s, lc.grid, lc.block, h->stream), "tangent step launch");
         if (h->nsc)  // the blend of the sponge cells' post-collision values, before the ports
             hip_check(jr->sponge(Q, h->sponge_args(t), dual.p, h->stream), "tangent sponge launch");

@@ -21,6 +21,11 @@ struct Dual {
         const double r = ::sqrt(a.v);
         return Dual(r, a.d / (2.0 * r));
     }
+    // a^p for a constant exponent p (the WALE closure's powers 1.25, 1.5, 2.5 of
+    // non-negative invariants): d(a^p) = p a^(p-1) da; the tangent of p is not used.
+    friend LBM_DEV Dual pow(const Dual& a, const Dual& p) {
+        return Dual(::pow(a.v, p.v), p.v * ::pow(a.v, p.v - 1.0) * a.d);
+    }
     __host__ __device__ explicit constexpr operator float() const { return float(v); }
     LBM_DEV Dual& operator+=(const Dual& b) {
         v += b.v;

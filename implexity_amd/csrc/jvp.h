@@ -79,6 +79,8 @@ struct JvpRunner {
     hipError_t step(int Q, int kind, bool mask, const StepArgs& a, dim3 grid, dim3 block, hipStream_t s) const;
     hipError_t ports(int Q, bool mask, const PortArgs& a, void* pop, hipStream_t s) const;
     hipError_t sponge(int Q, const SpongeArgs& a, void* pop, hipStream_t s) const;
+    hipError_t wale_u(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, void* U,
+                      dim3 grid, dim3 block, hipStream_t s) const;
     hipError_t set(int Q, bool mask, const Geom& g, void* pop, const uint32_t* m, int odd, const double* val,
                    const double* tan, dim3 grid, dim3 block, hipStream_t s) const;
     hipError_t get(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, double* tan,

@@ -13,6 +13,8 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
     case K_MRT: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_MRT, M>), grid, block, 0, s, a); break;
     case K_TRT_LES: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_LES, M>), grid, block, 0, s, a); break;
     case K_BGK_LES: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_LES, M>), grid, block, 0, s, a); break;
+    case K_TRT_WALE: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_WALE, M>), grid, block, 0, s, a); break;
+    case K_BGK_WALE: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_WALE, M>), grid, block, 0, s, a); break;
     default:
         if constexpr (Q == 19) {
             if (kind == K_BGK_GUO) {
@@ -34,11 +36,32 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
 hipError_t JvpRunner::step(int Q, int kind, bool mask, const StepArgs& a, dim3 grid, dim3 block,
                            hipStream_t s) const {
     if (!a.pop || !a.pop_out || !a.alpha || (mask && !a.mask) || a.part || a.tpop || a.gacc) return hipErrorInvalidValue;
-    if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES) && !a.mrt)
+    if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
         return hipErrorInvalidValue;
+    if (is_wale(kind) && !a.wu) return hipErrorInvalidValue;
     if (Q == 19) return mask ? step_q<19, true>(kind, a, grid, block, s) : step_q<19, false>(kind, a, grid, block, s);
     if (Q == 27) return mask ? step_q<27, true>(kind, a, grid, block, s) : step_q<27, false>(kind, a, grid, block, s);
     return hipErrorInvalidValue;
+}
+
+// The WALE pre-pass on the dual state: u and rho of every cell with their tangents.
+hipError_t JvpRunner::wale_u(int Q, bool mask, const Geom& g, const void* pop, const uint32_t* m, int odd, void* U,
+                             dim3 grid, dim3 block, hipStream_t s) const {
+    if (!pop || !U || (mask && !m)) return hipErrorInvalidValue;
+    if (Q == 19) {
+        if (mask)
+            hipLaunchKernelGGL((k_wale_u<P_JVP, 19, true>), grid, block, 0, s, g, pop, m, odd, U);
+        else
+            hipLaunchKernelGGL((k_wale_u<P_JVP, 19, false>), grid, block, 0, s, g, pop, m, odd, U);
+    } else if (Q == 27) {
+        if (mask)
+            hipLaunchKernelGGL((k_wale_u<P_JVP, 27, true>), grid, block, 0, s, g, pop, m, odd, U);
+        else
+            hipLaunchKernelGGL((k_wale_u<P_JVP, 27, false>), grid, block, 0, s, g, pop, m, odd, U);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // The sponge blend on dual numbers: tangent (1 - sigma), the far field a constant.

@@ -8,8 +8,9 @@ crates/implexity-ad/src/scan.rs:46-80) through the C ABI, in FP64:
   forward mode through the step kernels and the hand-written transposed kernels.
 
 The operators: BGK, TRT, MRT (default and non-default rates), regularized, BGK-Guo,
-cumulant, and the Smagorinsky closure (psm.rs:161-197) on BGK and TRT; sponge layers
-(moving/sponge.rs) with a steady and a travelling-wave far field.
+cumulant, the Smagorinsky (psm.rs:161-197) and WALE (moving/wale.rs) closures on BGK
+and TRT, and sponge layers (moving/sponge.rs) with a steady and a travelling-wave far
+field.
 """
 from __future__ import annotations
 
@@ -39,6 +40,10 @@ JVP_CASES = [
     ((7, 6, 5), (True, False, False), 19, "trt_les", True, False, (0.15, 1e-6)),
     ((7, 6, 5), (False, False, False), 19, "bgk_les", False, True, (0.15, 1e-6)),
     ((6, 5, 4), (True, False, False), 27, "trt_les", False, False, (0.15, 1e-6)),
+    # the WALE closure (moving/wale.rs; rates = (WALE constant, floor))
+    ((7, 6, 5), (True, False, False), 19, "trt_wale", True, False, (0.5, 1e-12)),
+    ((7, 6, 5), (False, False, False), 19, "bgk_wale", False, True, (0.5, 1e-12)),
+    ((6, 5, 4), (True, False, False), 27, "trt_wale", False, False, (0.5, 1e-12)),
     # sponge layers (moving/sponge.rs): an affine blend after the collision
     ((8, 6, 5), (False, True, True), 19, "trt_sponge", True, False, (None, None)),
     ((7, 6, 6), (False, False, False), 19, "bgk_sponge", False, True, (None, None)),
@@ -64,16 +69,18 @@ def _setup(shape, periodic, Q, coll, with_solid, with_ports, rates, omega=1.4):
         solid = solid.copy()
         solid[:2] = False
         solid[-2:] = False
-    les, sponges = (0.0, 0.0), ()
+    les, wale, sponges = (0.0, 0.0), (0.0, 1e-12), ()
     if coll.endswith("_les"):  # the closure on BGK / TRT
         coll, les, rates, omega = coll[:-4], rates, (None, None), 1.8
+    if coll.endswith("_wale"):
+        coll, wale, rates, omega = coll[:-5], rates, (None, None), 1.8
     if coll.endswith("_sponge"):
         coll, sponges = coll[:-7], _sponges(periodic, with_ports)
     bulk = rates[0] if rates[0] is not None else (1.0 if coll == "cumulant" else float("nan"))
     even = rates[1] if rates[1] is not None else float("nan")
     cfg, prob = make_pair(shape, periodic, Q=Q, collision=coll, precision="f64", omega=omega,
                           accel=(0, 0, 0) if with_ports else (1e-4, -3e-5, 0), solid=solid, ports=ports,
-                          bulk_rate=bulk, even_rate=even, les=les, sponges=sponges)
+                          bulk_rate=bulk, even_rate=even, les=les, wale=wale, sponges=sponges)
     return cfg, prob, solid
 
 
@@ -130,8 +137,8 @@ def test_step_jvp_vjp_dot_product(shape, periodic, Q, coll, with_solid, with_por
     assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
 
 
-def test_step_jvp_refuses_closures():
-    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", wale=(0.5, 1e-12))
+def test_step_jvp_refuses_occupancy_couplings():
+    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", coupling=("psm", 0.1))
     with Lattice(cfg) as lat:
         with pytest.raises(ContractError, match="lbm_step_jvp covers"):
             lat.interval_jvp(np.zeros(lat.n * 19))
