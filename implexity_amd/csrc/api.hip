@@ -2278,13 +2278,14 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         if (!h || !d_state || !d_next) fail(LBM_ERR_CONTRACT, "null argument");
         const int Q = h->Q;
         if (len != uint64_t(h->ncell) * Q) fail(LBM_ERR_SHAPE, "tangent length must be cells*Q");
+        const bool psm_law = h->kind == K_TRT_PSM || h->kind == K_BGK_PSM;
         const bool kind_ok = h->kind == K_BGK || h->kind == K_TRT || h->kind == K_MRT || h->kind == K_BGK_GUO ||
                              h->kind == K_CUMULANT || h->kind == K_TRT_LES || h->kind == K_BGK_LES ||
-                             is_wale(h->kind);
+                             is_wale(h->kind) || (is_psm(h->kind) && !(psm_law && h->Q == 27));
         if (!kind_ok || h->thermal)
             fail(LBM_ERR_CONTRACT, "lbm_step_jvp covers the BGK, TRT, MRT, regularized, cumulant and BGK-Guo "
-                                   "Brinkman operators, the Smagorinsky and WALE closures and sponge layers, "
-                                   "without the occupancy couplings or the temperature field");
+                                   "Brinkman operators, the Smagorinsky and WALE closures, sponge layers and the "
+                                   "occupancy couplings (the Psm law on D3Q19), without the temperature field");
         // The dual field (value, tangent) uses 16-byte elements; the twisted addressing
         // takes 32-bit byte offsets within an x plane of the field.
         if (uint64_t(h->g.SY + h->g.GY) * uint64_t(h->g.SZ) * uint64_t(Q) * sizeof(Dual) >= (uint64_t(1) << 31))
@@ -2321,7 +2322,33 @@ int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, do
         s.mrt = dc.p;
         s.tpop = nullptr;
         s.wu = s.sm = s.d1 = s.sm1 = nullptr;
-        DevBuf du;
+        DevBuf du, dsm, dd1, dsm1;
+        if (is_psm(h->kind)) {  // the solid momentum (and a second endpoint) as constants
+            hip_check(hipStreamSynchronize(h->stream), "sync");
+            auto to_dual = [&](const DevBuf& src, size_t count, DevBuf& dst, const char* what) {
+                std::vector<Dual> v(count);
+                if (h->prec == LBM_F64) {
+                    std::vector<double> x(count);
+                    hip_check(hipMemcpy(x.data(), src.p, count * sizeof(double), hipMemcpyDeviceToHost), "copy");
+                    for (size_t i = 0; i < count; i++) v[i] = Dual(x[i]);
+                } else {
+                    std::vector<float> x(count);
+                    hip_check(hipMemcpy(x.data(), src.p, count * sizeof(float), hipMemcpyDeviceToHost), "copy");
+                    for (size_t i = 0; i < count; i++) v[i] = Dual(double(x[i]));
+                }
+                dst.alloc(count * sizeof(Dual), what);
+                hip_check(hipMemcpy(dst.p, v.data(), dst.bytes, hipMemcpyHostToDevice), "upload");
+            };
+            const size_t n = size_t(h->ncell);
+            to_dual(h->sm, 3 * n, dsm, "tangent solid momentum");
+            s.sm = dsm.p;
+            if (h->d1.p) {
+                to_dual(h->d1, n, dd1, "tangent second occupancy");
+                to_dual(h->sm1, 3 * n, dsm1, "tangent second solid momentum");
+                s.d1 = dd1.p;
+                s.sm1 = dsm1.p;
+            }
+        }
         if (is_wale(h->kind)) {  // u and rho of every cell, with their tangents, before the in-place step
             du.alloc(size_t(h->ncell) * 4 * sizeof(Dual), "tangent WALE velocities");
             hip_check(jr->wale_u(Q, h->has_mask, h->g, dual.p, h->mask.as<uint32_t>(), int(t & 1), du.p, lc.grid,

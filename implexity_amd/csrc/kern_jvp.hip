@@ -15,8 +15,21 @@ hipError_t step_q(int kind, const StepArgs& a, dim3 grid, dim3 block, hipStream_
     case K_BGK_LES: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_LES, M>), grid, block, 0, s, a); break;
     case K_TRT_WALE: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_WALE, M>), grid, block, 0, s, a); break;
     case K_BGK_WALE: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_WALE, M>), grid, block, 0, s, a); break;
+    case K_TRT_PSMS: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_PSMS, M>), grid, block, 0, s, a); break;
+    case K_BGK_PSMS: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_PSMS, M>), grid, block, 0, s, a); break;
+    case K_TRT_BRO: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_TRT_BRO, M>), grid, block, 0, s, a); break;
+    case K_BGK_BRO: hipLaunchKernelGGL((k_forward<P_JVP, Q, K_BGK_BRO, M>), grid, block, 0, s, a); break;
     default:
         if constexpr (Q == 19) {
+            // the Psm law on dual numbers is D3Q19 only (on D3Q27 it spills to scratch heavily)
+            if (kind == K_TRT_PSM) {
+                hipLaunchKernelGGL((k_forward<P_JVP, 19, K_TRT_PSM, M>), grid, block, 0, s, a);
+                break;
+            }
+            if (kind == K_BGK_PSM) {
+                hipLaunchKernelGGL((k_forward<P_JVP, 19, K_BGK_PSM, M>), grid, block, 0, s, a);
+                break;
+            }
             if (kind == K_BGK_GUO) {
                 hipLaunchKernelGGL((k_forward<P_JVP, 19, K_BGK_GUO, M>), grid, block, 0, s, a);
                 break;
@@ -39,6 +52,7 @@ hipError_t JvpRunner::step(int Q, int kind, bool mask, const StepArgs& a, dim3 g
     if ((kind == K_MRT || kind == K_CUMULANT || kind == K_TRT_LES || kind == K_BGK_LES || is_wale(kind)) && !a.mrt)
         return hipErrorInvalidValue;
     if (is_wale(kind) && !a.wu) return hipErrorInvalidValue;
+    if (is_psm(kind) && (!a.sm || !a.mrt)) return hipErrorInvalidValue;
     if (Q == 19) return mask ? step_q<19, true>(kind, a, grid, block, s) : step_q<19, false>(kind, a, grid, block, s);
     if (Q == 27) return mask ? step_q<27, true>(kind, a, grid, block, s) : step_q<27, false>(kind, a, grid, block, s);
     return hipErrorInvalidValue;

@@ -458,8 +458,9 @@ int lbm_step_vjp(lbm_handle* h, const double* lambda_next, double* lambda_prev, 
  * + J_alpha d_alpha (AoS N*Q; d_alpha has one entry per cell, or NULL for 0), computed in
  * FP64 forward mode through the step's own collision, streaming and port code.  The
  * handle's state is not advanced.  Covers the BGK, TRT, MRT, regularized, cumulant and
- * BGK-Guo Brinkman operators, the Smagorinsky and WALE closures on BGK/TRT and sponge
- * layers (not the occupancy couplings or the temperature field). */
+ * BGK-Guo Brinkman operators, the Smagorinsky and WALE closures on BGK/TRT, sponge
+ * layers and the occupancy couplings (d_alpha is then the occupancy's tangent; the Psm
+ * law on D3Q19 only), not the temperature field. */
 int lbm_step_jvp(lbm_handle* h, const double* d_state, const double* d_alpha, double* d_next, uint64_t len);
 
 int lbm_get_fields(lbm_handle* h, double* rho, double* u, uint64_t n);

@@ -9,8 +9,9 @@ crates/implexity-ad/src/scan.rs:46-80) through the C ABI, in FP64:
 
 The operators: BGK, TRT, MRT (default and non-default rates), regularized, BGK-Guo,
 cumulant, the Smagorinsky (psm.rs:161-197) and WALE (moving/wale.rs) closures on BGK
-and TRT, and sponge layers (moving/sponge.rs) with a steady and a travelling-wave far
-field.
+and TRT, sponge layers (moving/sponge.rs) with a steady and a travelling-wave far field,
+and the occupancy couplings (the Psm law on D3Q19; PsmSuperposition and the moving
+Brinkman law on both lattices).
 """
 from __future__ import annotations
 
@@ -137,8 +138,45 @@ def test_step_jvp_vjp_dot_product(shape, periodic, Q, coll, with_solid, with_por
     assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs)
 
 
-def test_step_jvp_refuses_occupancy_couplings():
-    cfg, _ = make_pair((6, 5, 4), (True, False, False), precision="f64", coupling=("psm", 0.1))
+COUPLED = [
+    # shape, periodic, Q, collision, law, with solid, with ports (moving/psm.rs:213-266)
+    ((7, 6, 5), (True, False, False), 19, "trt", "psm", True, False),
+    ((9, 6, 6), (False, False, False), 19, "bgk", "psm_superposition", False, True),
+    ((7, 6, 5), (True, False, False), 19, "trt", "brinkman_occupancy", True, False),
+    ((6, 5, 4), (True, False, True), 27, "trt", "psm_superposition", True, False),
+    ((6, 5, 4), (True, False, True), 27, "bgk", "brinkman_occupancy", False, False),
+]
+
+
+@pytest.mark.parametrize("shape,periodic,Q,coll,law,with_solid,with_ports", COUPLED)
+def test_step_jvp_occupancy_couplings(shape, periodic, Q, coll, law, with_solid, with_ports):
+    """The occupancy d as the design and the solid momentum as a constant: central
+    differences of the oracle's step and the dot product with the device's step VJP
+    (the direction in d only where the coupling is on, d > 0)."""
+    from tests.test_gpu_psm import _case as psm_case
+    from tests.test_gpu_psm import _lattice as psm_lattice
+
+    cfg, prob, solid, d, m = psm_case(shape, periodic, Q, coll, law, with_solid, with_ports)
+    o = orc.Oracle(prob)
+    f = o.run(d, 4)
+    v, a, lam = _directions(o, Q, solid, 11)
+    a = a * (d > 0)
+    h = 1e-6
+    fd = (o.interval(d + h * a, f + h * v) - o.interval(d - h * a, f - h * v)) / (2 * h)
+    with psm_lattice(cfg, d, m) as lat:
+        lat.set_state(f)
+        jv = lat.interval_jvp(v, a)
+        lam_prev, dbar = lat.interval_vjp(lam)
+    fluid = np.ones(o.n, bool) if solid is None else ~solid.reshape(-1)
+    g, w = jv.reshape(-1, Q)[fluid], fd.reshape(-1, Q)[fluid]
+    assert np.abs(g - w).max() <= 1e-7 * np.abs(w).max(), np.abs(g - w).max() / np.abs(w).max()
+    lhs = float(np.dot(lam.reshape(-1, Q)[fluid].ravel(), g.ravel()))
+    rhs = float(np.dot(lam_prev.reshape(-1, Q)[fluid].ravel(), v.reshape(-1, Q)[fluid].ravel()) + dbar @ a)
+    assert abs(lhs - rhs) <= 1e-13 * float(np.abs(lam).sum() * np.abs(jv).max()), (lhs, rhs)
+
+
+def test_step_jvp_refuses_d3q27_psm_law():
+    cfg, _ = make_pair((6, 5, 4), (True, False, False), Q=27, precision="f64", coupling=("psm", 0.1))
     with Lattice(cfg) as lat:
         with pytest.raises(ContractError, match="lbm_step_jvp covers"):
-            lat.interval_jvp(np.zeros(lat.n * 19))
+            lat.interval_jvp(np.zeros(lat.n * 27))
