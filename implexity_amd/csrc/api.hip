@@ -166,7 +166,6 @@ size_t elem_bytes(int prec) { return prec == LBM_F64 ? 8 : (prec == LBM_F32 ? 4 
 // Lattice tables on the host (same order as lattice.h).
 int lat_c(int Q, int q, int d) { return Q == 19 ? cvec<19>(q, d) : cvec<27>(q, d); }
 double lat_w(int Q, int q) { return Q == 19 ? weight<19>(q) : weight<27>(q); }
-int lat_opp(int Q, int q) { return q == 0 ? 0 : Q - q; }
 
 // Reverse steps whose alpha_bar terms one FP32 partial gathers before the fold into
 // FP64: the partial's rounding stays near 32 ulp of FP32 per run, far below the
@@ -2825,7 +2824,6 @@ int lbm_time_kernel(lbm_handle* h, int which, uint64_t launches, double* avg_ms)
         if (!h || !avg_ms) fail(LBM_ERR_CONTRACT, "null argument");
         if (launches == 0) fail(LBM_ERR_CONTRACT, "launches must be positive");
         if (which < 0 || which > 7) fail(LBM_ERR_CONTRACT, "unknown kernel selector");
-        ObjArgs none{};
         if (which == 4) {
             // thermal adjoint kernel alone, in place (the state is not meaningful afterwards)
             if (!h->thermal) fail(LBM_ERR_CONTRACT, "no temperature field on this handle");

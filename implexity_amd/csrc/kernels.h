@@ -1697,7 +1697,7 @@ template <class C>
 LBM_DEV void st_port(const C* f, int kind, const double* value, C* out, const double* stress) {
     const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
     const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
-    const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
+    const int t0 = axis == 0 ? 1 : 0;  // the first tangential axis
     if (rule == 0) {
         C rho, m[3], vel[3], target[3], eb[19], el[19];
         st_moments(f, rho, m);
