@@ -1,5 +1,5 @@
-// Dual numbers for forward-mode derivatives on the device: the tangent step (jvp.h)
-// and the cell Jacobians of the stationary port rules (kernels.h).
+// Dual numbers for forward-mode derivatives on the device: the tangent step (jvp.h),
+// which runs the collision, closure, sponge and port code on them.
 #pragma once
 #include <hip/hip_runtime.h>
 

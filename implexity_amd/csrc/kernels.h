@@ -1810,34 +1810,184 @@ LBM_DEV void st_port_moment(const C* f, int rule, int face, const double* value,
     }
 }
 
-// Transpose of a stationary rule, sum_r J[r][k] g[r] added to acc[k] in ascending r
-// (apply_ports_vjp, ports.rs:582-621), with J the cell Jacobian formed column by column
-// on dual numbers (cell_jacobian, :532-557): at the source for the neighbour and
-// regularized rules, at the lattice weights for the affine moment rules
-// (CompiledPort::new, :514-520).
-template <class C>
-LBM_DEV void st_port_vjp(const C* src, int kind, const double* value, const C* g, C* acc, const double* stress) {
-    const int rule = (kind - ST_PORT_BASE) / 6;
-    const bool affine = rule == 1 || rule == 2;
-#pragma unroll 1
-    for (int k = 0; k < 19; k++) {
-        Dual x[19], y[19];
+// Analytic transposes of the stationary rules, in FP64, equal to the dual-number cell
+// Jacobian's J^T g (cell_jacobian, ports.rs:532-557) up to rounding.
+//
+// Moment rule (rules 1, 2; affine in f): known populations pass through; an unknown q
+// gets prop_q + r . (1/5, c_t0 / 2, c_t1 / 2) with prop_q = f_opp(q) + 6 w_q rho (c_q . v)
+// and r = (rho - kmass, rho v_t0 - km_t0, rho v_t1 - km_t1) - sum_unknown prop (1, c_t0,
+// c_t1); rho = compat / (1 - sign v0_n) for a velocity port, or given with
+// v_n = (1 - compat / rho) sign for a pressure port; compat = zero set + 2 outgoing set.
+// The transpose runs these steps backwards and adds to fb.
+LBM_DEV void st_moment_vjp(const double* f, int rule, int face, const double* value, const double* g, double* fb) {
+    const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
+    const int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;
+    const double sgn = double(sign);
+    // forward quantities the transpose needs: rho and v
+    double compat = 0.0;
 #pragma unroll
-        for (int q = 0; q < 19; q++) x[q] = Dual(affine ? weight<19>(q) : double(src[q]), q == k ? 1.0 : 0.0);
-        st_port(x, kind, value, y, stress);
-        double a = 0.0;
-        static_for<0, 19>([&](auto K) {
-            constexpr int kk = decltype(K)::value;
-            if (k == kk) a = double(acc[kk]);
-        });
-#pragma unroll
-        for (int r = 0; r < 19; r++)
-            if (double(g[r]) != 0.0) a += y[r].d * double(g[r]);
-        static_for<0, 19>([&](auto K) {
-            constexpr int kk = decltype(K)::value;
-            if (k == kk) acc[kk] = C(a);
-        });
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c == 0) compat += f[q];
+        if (c == -sign) compat += 2.0 * f[q];
     }
+    double rho, v[3];
+    if (rule == 1) {
+        for (int d = 0; d < 3; d++) v[d] = value[d];
+        rho = compat / (1.0 - sgn * value[axis]);
+    } else {
+        rho = value[0];
+        v[t0] = value[1];
+        v[t1] = value[2];
+        v[axis] = (1.0 - compat / rho) * sgn;
+    }
+    // out[q] = prop_q + 0.2 r0 + 0.5 c_t0 r1 + 0.5 c_t1 r2 (unknown), f_q (known)
+    double r0b = 0.0, r1b = 0.0, r2b = 0.0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c != sign) {
+            fb[q] += g[q];
+            continue;
+        }
+        r0b += 0.2 * g[q];
+        r1b += 0.5 * double(cvec<19>(q, t0)) * g[q];
+        r2b += 0.5 * double(cvec<19>(q, t1)) * g[q];
+    }
+    // r = target - pa: target_bar = r_bar, pa_bar = -r_bar; target0 = rho - kmass,
+    // target1/2 = rho v_t - km_t: the known populations' moments get -r_bar
+    double rhob = r0b + r1b * v[t0] + r2b * v[t1], vnb = 0.0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c == sign) {
+            // prop_bar = g_q + pa_bar . (1, c_t0, c_t1)
+            const double pb = g[q] - r0b - r1b * double(cvec<19>(q, t0)) - r2b * double(cvec<19>(q, t1));
+            fb[opp<19>(q)] += pb;
+            const double s6 = 6.0 * weight<19>(q);
+            double cv = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) cv += double(cvec<19>(q, d)) * v[d];
+            rhob += pb * s6 * cv;
+            vnb += pb * rho * s6 * double(cvec<19>(q, axis));
+        } else {
+            fb[q] -= r0b + r1b * double(cvec<19>(q, t0)) + r2b * double(cvec<19>(q, t1));
+        }
+    }
+    // compat's cotangent: through rho (velocity port) or v_n (pressure port)
+    const double cb = rule == 1 ? rhob / (1.0 - sgn * value[axis]) : -vnb * sgn / rho;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+        if (c == 0) fb[q] += cb;
+        if (c == -sign) fb[q] += 2.0 * cb;
+    }
+}
+
+// Transpose of eq_q = rho w_q (1 + 3 cu + 4.5 cu^2 - 1.5 u^2) (ports.rs:276-282) with the
+// populations' cotangent e: rho_bar and u_bar added to rb, ub.
+LBM_DEV void st_equilibrium_vjp(double rho, const double u[3], const double* e, double& rb, double ub[3]) {
+    const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const double cu = st_dot_c(q, u);
+        const double w = weight<19>(q);
+        rb += e[q] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+        const double sq = e[q] * rho * w;
+#pragma unroll
+        for (int d = 0; d < 3; d++) ub[d] += sq * ((3.0 + 9.0 * cu) * double(cvec<19>(q, d)) - 3.0 * u[d]);
+    }
+}
+
+// Transpose of (rho, m) = moments(f), u = m / rho, given rho_bar and u_bar: f_bar.
+LBM_DEV void st_velocity_vjp(double rho, const double u[3], double rb, const double ub[3], double* fb) {
+    double mb[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        mb[d] = ub[d] / rho;
+        rb -= ub[d] * u[d] / rho;
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++)
+        fb[q] += rb + mb[0] * double(cvec<19>(q, 0)) + mb[1] * double(cvec<19>(q, 1)) + mb[2] * double(cvec<19>(q, 2));
+}
+
+// J^T g of a stationary row's rule at its source populations src, added to acc.
+//   rule 0 (neighbour): out = eq(rho_b, t) + nb - eq(rho_n, u_n), t = (u_n along the
+//     normal, the authored tangential velocity across it);
+//   rules 1, 2: the moment rule above;
+//   rules 3, 4 (regularized): out = eq(rho, m / rho) + basis . s, s = op . (f_known -
+//     eq_known), (rho, m) the moments of target = the moment rule of f.
+template <class C>
+LBM_DEV void st_port_transpose(const C* src, int kind, const double* value, const C* gc, C* acc,
+                               const double* stress) {
+    const int rule = (kind - ST_PORT_BASE) / 6, face = (kind - ST_PORT_BASE) % 6;
+    const int axis = face >> 1, sign = (face & 1) ? -1 : 1;
+    double f[19], g[19], fb[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        f[q] = double(src[q]);
+        g[q] = double(gc[q]);
+        fb[q] = 0.0;
+    }
+    if (rule == 0) {
+        double rho, m[3], u[3], t[3];
+        st_moments(f, rho, m);
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[d] = m[d] / rho;
+        const int t0 = axis == 0 ? 1 : 0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) t[d] = d == axis ? u[d] : (d == t0 ? value[1] : value[2]);
+        double rbb = 0.0, tb[3] = {0.0, 0.0, 0.0}, rnb = 0.0, unb[3] = {0.0, 0.0, 0.0}, ng[19];
+        st_equilibrium_vjp(value[0], t, g, rbb, tb);  // rho_b is given: rbb unused
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            fb[q] = g[q];
+            ng[q] = -g[q];
+        }
+        st_equilibrium_vjp(rho, u, ng, rnb, unb);
+        unb[axis] += tb[axis];  // only the normal component of t follows the neighbour
+        st_velocity_vjp(rho, u, rnb, unb, fb);
+    } else if (rule <= 2) {
+        st_moment_vjp(f, rule, face, value, g, fb);
+    } else {
+        double target[19], rho, m[3], vel[3];
+        st_port_moment(f, rule - 2, face, value, target);
+        st_moments(target, rho, m);
+#pragma unroll
+        for (int d = 0; d < 3; d++) vel[d] = m[d] / rho;
+        const double* op = stress + face * 84;
+        double sb[6];
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < 19; q++) t += st_basis(q, r) * g[q];
+            sb[r] = t;
+        }
+        double eqb[19];
+        int k = 0;
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            eqb[q] = g[q];
+            const int c = cvec<19>(q, 0) * (axis == 0) + cvec<19>(q, 1) * (axis == 1) + cvec<19>(q, 2) * (axis == 2);
+            if (c == sign) continue;
+            double t = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; r++) t += op[r * 14 + k] * sb[r];
+            eqb[q] -= t;
+            fb[q] += t;
+            k++;
+        }
+        double rb = 0.0, vb[3] = {0.0, 0.0, 0.0}, tgb[19];
+        st_equilibrium_vjp(rho, vel, eqb, rb, vb);
+#pragma unroll
+        for (int q = 0; q < 19; q++) tgb[q] = 0.0;
+        st_velocity_vjp(rho, vel, rb, vb, tgb);
+        st_moment_vjp(f, rule - 2, face, value, tgb, fb);
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++) acc[q] = C(double(acc[q]) + fb[q]);
 }
 
 template <int Q, class C, bool ST = false>
@@ -2021,7 +2171,7 @@ __global__ __launch_bounds__(128) void k_port_vjp(PortArgs A, void* lam_, const 
             const bool self = A.nb[r] == A.cell[r];
 #pragma unroll
             for (int q = 0; q < Q; q++) nbb[q] = self ? C(0) : gn[q];
-            st_port_vjp(fnb, kind, A.value + 3 * r, gx, nbb, A.stress);
+            st_port_transpose(fnb, kind, A.value + 3 * r, gx, nbb, A.stress);
             static_for<0, Q>([&](auto I) {
                 constexpr int q = decltype(I)::value;
                 if (!self) L::template st<C>(lam + read_loc<Q, q>(A.g, cx, mx, A.odd), C(0), sc);

@@ -9,10 +9,11 @@ state buffers, so the reverse sweep reads them from recomputed states and snapsh
 
 Bars: the FP64 forward bit-exact (the stress operator is the correctly rounded exact
 operator on both sides), the step VJP within 1e-12 of the oracle's apply_ports_vjp
-(:582-621, Jacobians by dual numbers on both sides), the gradient within 1e-11, the
-tangent/transpose dot product within 1e-13, FP64 mass drift with the port exchange at
-rounding, reduced precision within the usual bounds of the largest population deviation
-(forward) and the FP32 gradient within 2e-5 relative L2.
+(:582-621; analytic transposes on the device, the dual-number cell Jacobian in the
+oracle), the gradient within 1e-11, the tangent/transpose dot product within 1e-13, FP64
+mass drift with the port exchange at rounding, reduced precision within the usual bounds
+of the largest population deviation (forward) and the FP32 gradient within 2e-5 relative
+L2.
 """
 from __future__ import annotations
 
@@ -139,7 +140,7 @@ def test_gradient_f32_matches_oracle():
 
 def test_step_jvp_dot_product_with_regularized_ports():
     """<lambda, J v> = <J^T lambda, v>: the tangent runs the rules on dual numbers, the
-    transpose through their dual-number Jacobians at the kept pre-port populations."""
+    transpose analytically at the kept pre-port populations."""
     cfg, prob, _ = _case("both_mrt")
     o = orc.Oracle(prob)
     alpha = random_alpha(prob.shape, 4)

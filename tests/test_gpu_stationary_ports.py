@@ -5,9 +5,9 @@ ports.rs) through the C ABI against the FP64 oracle: the moment reconstruction o
 
 Bars as for the other ports: the FP64 forward bit-exact (with the port cells and the
 sources bit-exact too), the step VJP within 1e-12 of the oracle's apply_ports_vjp
-(:582-621, Jacobians by dual numbers on both sides), the gradient within 1e-11, the
-mass drift with the port exchange at rounding, FP32 and FP16 storage within the usual
-bounds of the largest population deviation.
+(:582-621; analytic transposes on the device, the dual-number cell Jacobian in the
+oracle), the gradient within 1e-11, the mass drift with the port exchange at rounding,
+FP32 and FP16 storage within the usual bounds of the largest population deviation.
 """
 from __future__ import annotations
 
@@ -110,7 +110,7 @@ def test_gradient_f64_matches_oracle(case):
 
 def test_step_jvp_dot_product_with_stationary_ports():
     """<lambda, J v> = <J^T lambda, v>: the tangent runs the rules on dual numbers, the
-    transpose through their dual-number Jacobians."""
+    transpose is analytic."""
     shape, periodic, coll, inlet, outlet, accel, tangential, inset, with_solid = CASES[2]
     (cfg, prob), _ = _case(shape, periodic, coll, inlet, outlet, accel, tangential, inset, with_solid)
     o = orc.Oracle(prob)
