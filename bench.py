@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--cpu-window", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="gradient", choices=["gradient", "kernels", "thermofluid"])
+    ap.add_argument("--ports", default="nee", choices=["nee", "moment", "regularized"],
+                    help="the channel's ports: NEE (the benchmark), or the stationary moment / regularized rules")
     ap.add_argument("--kernel-precision", default="f32")
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--les", type=float, default=0.0,
@@ -295,7 +297,7 @@ def main():
     def alpha_cfg(precision):
         if "alpha" not in cache:
             cache["alpha"] = build_problem(shape, precision, budget)[1]
-        return _cfg_only(shape, precision, budget), cache["alpha"]
+        return _cfg_only(shape, precision, budget, args.ports), cache["alpha"]
 
     if args.mode == "thermofluid":
         thermofluid(args)
@@ -446,7 +448,9 @@ def main():
         "data": "synthetic (seeded 64-blob Brinkman design)",
         "config": {"workload": f"D3Q19 TRT channel {shape[0]}x{shape[1]}x{shape[2]}, tau+=0.55, u_in=0.03, "
                                f"forward+adjoint over {args.steps} steps, binomial checkpointing in "
-                               f"{args.budget_gb:g} GB", "precision": main_prec},
+                               f"{args.budget_gb:g} GB"
+                               + ("" if args.ports == "nee" else f", stationary {args.ports} ports"),
+                   "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": gbs_dom, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs_dom / HBM_PEAK_GBS, "traffic": traffic, "kernel": f"{dom} ({main_prec})",
                      "algorithmic_bytes_per_launch": algorithmic_bytes(dom, main_prec, steps=args.steps) * n,
@@ -516,14 +520,25 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def _cfg_only(shape, variant, budget):
+def _cfg_only(shape, variant, budget, port_rule="nee"):
+    """The channel: NEE ports (the benchmark's), or the stationary solver's moment or
+    regularized rules on the x faces without their y/z edges (ports.rs refuses port cells
+    on the edges of closed faces)."""
     from implexity_amd import design
     from implexity_amd.lbm import LatticeConfig, Port
 
     precision, adjoint_storage = VARIANTS[variant][:2]
     nx, ny, nz = shape
-    prof = design.parabolic_profile(ny, nz).reshape(-1)
-    ports = [Port("xmin", "velocity", (0.03, 0.0, 0.0), prof), Port("xmax", "pressure", (1.0,))]
+    if port_rule == "nee":
+        prof = design.parabolic_profile(ny, nz).reshape(-1)
+        ports = [Port("xmin", "velocity", (0.03, 0.0, 0.0), prof), Port("xmax", "pressure", (1.0,))]
+    else:
+        mi = np.zeros(shape, bool)
+        mo = np.zeros(shape, bool)
+        mi[0, 1:-1, 1:-1] = True
+        mo[-1, 1:-1, 1:-1] = True
+        ports = [Port("xmin", f"velocity_{port_rule}", (0.03, 0.0, 0.0), mask=mi),
+                 Port("xmax", f"pressure_{port_rule}", (1.0, 0.0, 0.0), mask=mo)]
     return LatticeConfig(shape=shape, periodic=(False, False, False), lattice=19, collision="trt",
                          precision=precision, omega=1.0 / 0.55, magic=3.0 / 16.0, ports=ports,
                          checkpoint_budget_bytes=budget, adjoint_storage=adjoint_storage)
