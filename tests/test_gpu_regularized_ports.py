@@ -12,8 +12,8 @@ operator on both sides), the step VJP within 1e-12 of the oracle's apply_ports_v
 (:582-621; analytic transposes on the device, the dual-number cell Jacobian in the
 oracle), the gradient within 1e-11, the tangent/transpose dot product within 1e-13, FP64
 mass drift with the port exchange at rounding, reduced precision within the usual bounds
-of the largest population deviation (forward) and the FP32 gradient within 2e-5 relative
-L2.
+of the largest population deviation (forward), the FP32 gradient within 2e-5 and the
+FP16-storage gradient within 2e-2 relative L2.
 """
 from __future__ import annotations
 
@@ -135,6 +135,17 @@ def test_gradient_f32_matches_oracle():
     err = np.linalg.norm(ab - ab_want) / np.linalg.norm(ab_want)
     assert err <= 2e-5, err
     assert abs(J - J_want) <= 2e-5 * abs(J_want)
+    assert np.isfinite(st.mass_drift)
+
+
+@pytest.mark.parametrize("name", ["moment_preg", "z_faces"])
+def test_gradient_f16s_matches_oracle(name):
+    """FP16 storage (the pre-port populations kept in FP16 too) with scaled FP16
+    cotangents, against the FP64 oracle."""
+    (J, ab, _, st), (J_want, ab_want, _) = _gradient(name, "f16s")
+    err = np.linalg.norm(ab - ab_want) / np.linalg.norm(ab_want)
+    assert err <= 2e-2, err
+    assert abs(J - J_want) <= 2e-2 * abs(J_want)
     assert np.isfinite(st.mass_drift)
 
 
